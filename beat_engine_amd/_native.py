@@ -1,0 +1,96 @@
+"""ctypes binding of libbeat_bem_hip.so (declarations in include/beat_bem.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbeat_bem_hip.so")
+
+BEAT_OK = 0
+BEAT_ERR_SINGULAR = -4
+PREC_FP64 = 0
+PREC_FP32_REFINED = 1
+PREC_FP32 = 2
+
+REFINE_STATUS = {0: "none", 1: "converged", 2: "stagnated", 3: "step_limit", 4: "rounding_limited", 5: "nonfinite"}
+
+EXPORTS = (
+    "beat_ctx_create", "beat_ctx_destroy", "beat_last_error", "beat_mesh_upload", "beat_mesh_destroy",
+    "beat_mesh_counts", "beat_bm_assemble", "beat_bm_solve", "beat_bm_solve_device", "beat_field_eval",
+    "beat_dense_lu_solve", "beat_refine_lu",
+)
+
+
+class RefineReport(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int),
+        ("status", ctypes.c_int),
+        ("working_status", ctypes.c_int),
+        ("history_len", ctypes.c_int),
+        ("rtol", ctypes.c_double),
+        ("working_residuals", ctypes.POINTER(ctypes.c_double)),
+        ("returned_residuals", ctypes.POINTER(ctypes.c_double)),
+        ("history", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [
+        ("assembly_s", ctypes.c_double),
+        ("factor_s", ctypes.c_double),
+        ("solve_s", ctypes.c_double),
+        ("regular_kernel_s", ctypes.c_double),
+        ("factorization_count", ctypes.c_int),
+        ("lu_panels", ctypes.c_int),
+        ("regular_pairs", ctypes.c_int64),
+        ("singular_pairs", ctypes.c_int64),
+        ("refine", RefineReport),
+    ]
+
+
+class SolveOptions(ctypes.Structure):
+    _fields_ = [("max_steps", ctypes.c_int), ("rtol", ctypes.c_double)]
+
+
+_lib = None
+
+
+class BeatError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"HIP library not built: {path} is missing (run __graft_entry__.build() or make -C beat_engine_amd/csrc)")
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    I64 = ctypes.c_int64
+    D = ctypes.c_double
+    L.beat_ctx_create.argtypes = [I, ctypes.POINTER(P)]
+    L.beat_ctx_destroy.argtypes = [P]
+    L.beat_ctx_destroy.restype = None
+    L.beat_last_error.argtypes = [P]
+    L.beat_last_error.restype = ctypes.c_char_p
+    L.beat_mesh_upload.argtypes = [P, P, I64, P, I64, I, I, ctypes.POINTER(P)]
+    L.beat_mesh_destroy.argtypes = [P]
+    L.beat_mesh_destroy.restype = None
+    L.beat_mesh_counts.argtypes = [P, P]
+    L.beat_bm_assemble.argtypes = [P, P, I, D, D, D, P, I64, P, P]
+    L.beat_bm_solve.argtypes = [P, P, I, D, D, D, P, I64, P, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveReport)]
+    L.beat_bm_solve_device.argtypes = [P, P, I, D, D, D, P, I64, P, ctypes.POINTER(SolveOptions),
+                                       ctypes.POINTER(SolveReport)]
+    L.beat_field_eval.argtypes = [P, P, I, D, P, P, I64, P, I64, P]
+    L.beat_dense_lu_solve.argtypes = [P, I, P, I64, P, I64, P]
+    L.beat_refine_lu.argtypes = [P, P, I64, P, I64, P, P, I, D, P, ctypes.POINTER(RefineReport)]
+    if path == LIB_PATH:
+        _lib = L
+    return L

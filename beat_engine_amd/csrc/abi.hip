@@ -1,0 +1,1002 @@
+// C ABI (include/beat_bem.h): contexts, meshes and the solve pipeline
+//   assembly (regular + singular + identity) -> LU -> triangular solves
+//   [-> complex128 iterative refinement]
+// for one frequency and every right-hand side, with a single factorization.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/beat_bem.h"
+#include "bem_common.h"
+#include "kernels.h"
+
+using namespace beat;
+
+namespace {
+
+constexpr int LU_NB = 64;
+constexpr int SOLVE_CHUNK = 8;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+hipError_t ensure(DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return hipSuccess;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    if (e != hipSuccess) return e;
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  hipError_t e = hipMalloc(&b.p, std::max<size_t>(bytes, 256));
+  if (e == hipSuccess) b.bytes = std::max<size_t>(bytes, 256);
+  return e;
+}
+
+template <typename T>
+T* as(DevBuf& b) {
+  return reinterpret_cast<T*>(b.p);
+}
+
+}  // namespace
+
+struct beat_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::mutex mu;
+  DevBuf a64, a32, b64, b32, q64, q32, x64, r64, n64, rn64, xn64, d32, work, chunk, norms, ipiv, lusmall, qin, pout,
+      fld_src, fld_out, fld_pts;
+  hipEvent_t ev[6] = {};
+};
+
+struct beat_mesh {
+  beat_ctx* ctx = nullptr;
+  HostMesh h;
+  double* rec64 = nullptr;
+  float* rec32 = nullptr;
+  double* verts = nullptr;
+  int32_t* fverts = nullptr;
+  uint8_t* floc = nullptr;
+  int32_t *tile_start = nullptr, *tile_count = nullptr, *tile_nv = nullptr, *tile_verts = nullptr;
+  SingularPair* spairs = nullptr;
+  int32_t *rule_off = nullptr, *rule_cnt = nullptr;
+  double* rule_data = nullptr;
+  double* rule3 = nullptr;  // regular rule: xi, eta, w per point
+  int32_t* face_perm = nullptr;
+  int32_t* vert_inv = nullptr;
+  int64_t regular_pairs = 0;
+};
+
+namespace {
+
+int fail(beat_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (expr);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      return fail(ctx, BEAT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+template <typename T>
+__global__ void gather_q_kernel(const double* qc, int64_t nf, int nrhs, const int32_t* perm, T* qi) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf * nrhs) return;
+  const int64_t f = i / nrhs;
+  const int d = (int)(i - f * nrhs);
+  const int64_t src = (int64_t)perm[f] + (int64_t)d * nf;
+  qi[2 * i] = (T)qc[2 * src];
+  qi[2 * i + 1] = (T)qc[2 * src + 1];
+}
+
+template <typename T>
+__global__ void scatter_p_kernel(const T* X, int64_t nv, int nrhs, const int32_t* inv, double* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv * nrhs) return;
+  const int64_t v = i % nv;
+  const int d = (int)(i / nv);
+  const int64_t s = (int64_t)inv[v] * nrhs + d;
+  out[2 * i] = (double)X[2 * s];
+  out[2 * i + 1] = (double)X[2 * s + 1];
+}
+
+// columns [c0, c0+nc) of an n x ld row-major complex matrix <-> n x nc buffer
+template <typename T>
+__global__ void extract_cols_kernel(const T* src, int64_t n, int ld, int c0, int nc, T* dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nc) return;
+  const int64_t r = i / nc;
+  const int c = (int)(i - r * nc);
+  dst[2 * i] = src[2 * (r * ld + c0 + c)];
+  dst[2 * i + 1] = src[2 * (r * ld + c0 + c) + 1];
+}
+template <typename T>
+__global__ void insert_cols_kernel(const T* src, int64_t n, int ld, int c0, int nc, T* dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nc) return;
+  const int64_t r = i / nc;
+  const int c = (int)(i - r * nc);
+  dst[2 * (r * ld + c0 + c)] = src[2 * i];
+  dst[2 * (r * ld + c0 + c) + 1] = src[2 * i + 1];
+}
+
+__global__ void upcast_kernel(const float* s, double* d, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = (double)s[i];
+}
+// delta = r (complex128 -> complex64), zero for columns whose mask is 0
+__global__ void masked_downcast_kernel(const double* r, float* d, int64_t n, int nrhs, const int* mask) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nrhs) return;
+  const int c = (int)(i % nrhs);
+  const bool on = mask[c] != 0;
+  d[2 * i] = on ? (float)r[2 * i] : 0.f;
+  d[2 * i + 1] = on ? (float)r[2 * i + 1] : 0.f;
+}
+__global__ void add_delta_kernel(const double* x, const float* d, double* out, int64_t n2) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = x[i] + (double)d[i];
+}
+// accepted columns: x[:, j] = next[:, j], r[:, j] = rnext[:, j]
+__global__ void accept_kernel(double* x, const double* nx, double* r, const double* nr, int64_t n, int nrhs,
+                              const int* mask) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nrhs) return;
+  const int c = (int)(i % nrhs);
+  if (!mask[c]) return;
+  x[2 * i] = nx[2 * i];
+  x[2 * i + 1] = nx[2 * i + 1];
+  r[2 * i] = nr[2 * i];
+  r[2 * i + 1] = nr[2 * i + 1];
+}
+// round a complex128 solution through complex64 and back
+__global__ void round32_kernel(const double* x, double* y, int64_t n2) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (double)(float)x[i];
+}
+// column-major caller matrix (complex128) -> row-major internal (T), with lda
+template <typename T>
+__global__ void transpose_in_kernel(const double* a, int64_t n, int64_t lda, T* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * n) return;
+  const int64_t r = i % n, c = i / n;
+  out[2 * (r * lda + c)] = (T)a[2 * i];
+  out[2 * (r * lda + c) + 1] = (T)a[2 * i + 1];
+}
+template <typename T>
+__global__ void transpose_in_f32_kernel(const float* a, int64_t n, int64_t lda, T* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * n) return;
+  const int64_t r = i % n, c = i / n;
+  out[2 * (r * lda + c)] = (T)a[2 * i];
+  out[2 * (r * lda + c) + 1] = (T)a[2 * i + 1];
+}
+// caller column-major n x nrhs (complex128 or complex64) -> row-major
+template <typename S, typename T>
+__global__ void cols_in_kernel(const S* b, int64_t n, int nrhs, T* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nrhs) return;
+  const int64_t r = i % n, c = i / n;
+  out[2 * (r * nrhs + c)] = (T)b[2 * i];
+  out[2 * (r * nrhs + c) + 1] = (T)b[2 * i + 1];
+}
+template <typename S, typename T>
+__global__ void cols_out_kernel(const S* x, int64_t n, int nrhs, T* out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nrhs) return;
+  const int64_t r = i % n, c = i / n;
+  out[2 * i] = (T)x[2 * (r * nrhs + c)];
+  out[2 * i + 1] = (T)x[2 * (r * nrhs + c) + 1];
+}
+
+inline unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
+inline unsigned grid_cap(int64_t n) { return (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n + 255) / 256), 16384); }
+
+int64_t lda_for(int64_t n) { return (n + 63) / 64 * 64; }
+
+template <typename T>
+int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
+  const int maxg = (int)((n + 63) / 64) + 1;
+  const size_t slot_rows = (size_t)2 * maxg * 64 * 2 * sizeof(double);
+  const size_t bytes = sizeof(int) * (n + 8) + sizeof(double) * 2 * maxg + sizeof(int) * 2 * maxg + slot_rows +
+                       2 * 64 * 2 * sizeof(double) + 4096;
+  HIPCHK(ensure(ctx->ipiv, bytes));
+  char* p = reinterpret_cast<char*>(ctx->ipiv.p);
+  auto take = [&](size_t b) {
+    char* r = p;
+    p += (b + 255) / 256 * 256;
+    return r;
+  };
+  w.ipiv = reinterpret_cast<int*>(take(sizeof(int) * (n + 8)));
+  w.bar = reinterpret_cast<int*>(take(sizeof(int) * 4));
+  w.info = w.bar + 1;
+  w.slot_val = reinterpret_cast<double*>(take(sizeof(double) * 2 * maxg));
+  w.slot_row = reinterpret_cast<int*>(take(sizeof(int) * 2 * maxg));
+  w.slot_rows = take(slot_rows);
+  w.slot_diag = take(2 * 64 * 2 * sizeof(double));
+  w.maxg = maxg;
+  (void)sizeof(T);
+  return BEAT_OK;
+}
+
+// Solve with LU factors for every column of the n x nrhs row-major B (in place).
+template <typename T>
+int solve_all(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, int nrhs) {
+  const int nc0 = std::min(nrhs, SOLVE_CHUNK);
+  HIPCHK(ensure(ctx->work, lu_solve_work_bytes(n, nc0, LU_NB, sizeof(T))));
+  if (nrhs <= SOLVE_CHUNK) {
+    HIPCHK(lu_solve<T>(LU, n, lda, ipiv, B, nrhs, as<T>(ctx->work), LU_NB, ctx->stream));
+    return BEAT_OK;
+  }
+  HIPCHK(ensure(ctx->chunk, sizeof(T) * 2 * n * SOLVE_CHUNK));
+  for (int c0 = 0; c0 < nrhs; c0 += SOLVE_CHUNK) {
+    const int nc = std::min(SOLVE_CHUNK, nrhs - c0);
+    hipLaunchKernelGGL((extract_cols_kernel<T>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream, (const T*)B, n,
+                       nrhs, c0, nc, as<T>(ctx->chunk));
+    HIPCHK(hipGetLastError());
+    HIPCHK(lu_solve<T>(LU, n, lda, ipiv, as<T>(ctx->chunk), nc, as<T>(ctx->work), LU_NB, ctx->stream));
+    hipLaunchKernelGGL((insert_cols_kernel<T>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream,
+                       (const T*)as<T>(ctx->chunk), n, nrhs, c0, nc, B);
+    HIPCHK(hipGetLastError());
+  }
+  return BEAT_OK;
+}
+
+// r = b - A x for all columns; A64 row-major.
+int residual_all(beat_ctx* ctx, const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r,
+                 int nrhs) {
+  if (nrhs <= SOLVE_CHUNK) {
+    HIPCHK(residual64(A, n, lda, x, b, r, nrhs, ctx->stream));
+    return BEAT_OK;
+  }
+  HIPCHK(ensure(ctx->chunk, sizeof(double) * 2 * n * SOLVE_CHUNK * 3));
+  double* cx = as<double>(ctx->chunk);
+  double* cb = cx + 2 * n * SOLVE_CHUNK;
+  double* cr = cb + 2 * n * SOLVE_CHUNK;
+  for (int c0 = 0; c0 < nrhs; c0 += SOLVE_CHUNK) {
+    const int nc = std::min(SOLVE_CHUNK, nrhs - c0);
+    hipLaunchKernelGGL((extract_cols_kernel<double>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream, x, n, nrhs,
+                       c0, nc, cx);
+    hipLaunchKernelGGL((extract_cols_kernel<double>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream, b, n, nrhs,
+                       c0, nc, cb);
+    HIPCHK(residual64(A, n, lda, cx, cb, cr, nc, ctx->stream));
+    hipLaunchKernelGGL((insert_cols_kernel<double>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream,
+                       (const double*)cr, n, nrhs, c0, nc, r);
+    HIPCHK(hipGetLastError());
+  }
+  return BEAT_OK;
+}
+
+int col_norms(beat_ctx* ctx, const double* X, int64_t n, int nrhs, std::vector<double>& out) {
+  HIPCHK(ensure(ctx->norms, sizeof(double) * (nrhs + 64 * nrhs)));
+  HIPCHK(column_norms64(X, n, nrhs, as<double>(ctx->norms), ctx->stream));
+  out.resize(nrhs);
+  HIPCHK(hipMemcpyAsync(out.data(), ctx->norms.p, sizeof(double) * nrhs, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BEAT_OK;
+}
+
+struct Refine {
+  int steps = 0, status = BEAT_REFINE_NONE, working = BEAT_REFINE_NONE, hist_len = 0;
+  std::vector<double> working_res, returned_res, history;
+};
+
+// beat_refine_cpu_lu's loop (BeatEngineCpuRefinement.jl:5-78) with device data:
+// LU32 factors of the working matrix, A64/b64 the residual operator and rhs.
+// x64 (n x nrhs) holds the initial complex64 solution widened; on return it holds
+// the accepted working solution. returned_round: score the complex64-rounded
+// solution (the reference's public type) or the complex128 working solution.
+int refine_loop(beat_ctx* ctx, const float* LU32, int64_t lda32, const int* ipiv, const double* A64, int64_t lda64,
+                const double* b64, double* x64, int64_t n, int nrhs, int max_steps, double rtol, bool returned_round,
+                Refine& out) {
+  HIPCHK(ensure(ctx->r64, sizeof(double) * 2 * n * nrhs));
+  HIPCHK(ensure(ctx->n64, sizeof(double) * 2 * n * nrhs));
+  HIPCHK(ensure(ctx->rn64, sizeof(double) * 2 * n * nrhs));
+  HIPCHK(ensure(ctx->d32, sizeof(float) * 2 * n * nrhs + sizeof(int) * 64 + 4096));
+  double* r = as<double>(ctx->r64);
+  double* nx = as<double>(ctx->n64);
+  double* nr = as<double>(ctx->rn64);
+  float* d = as<float>(ctx->d32);
+  std::vector<int> mask(nrhs);
+  HIPCHK(ensure(ctx->lusmall, sizeof(int) * nrhs + 256));
+  int* dmask = as<int>(ctx->lusmall);
+  std::vector<double> bn, rn;
+  if (col_norms(ctx, b64, n, nrhs, bn) != BEAT_OK) return BEAT_ERR_HIP;
+  auto errors_of = [&](const std::vector<double>& nrm) {
+    std::vector<double> e(nrhs);
+    for (int j = 0; j < nrhs; ++j)
+      e[j] = bn[j] == 0 ? (nrm[j] == 0 ? 0.0 : INFINITY) : nrm[j] / bn[j];
+    return e;
+  };
+  if (residual_all(ctx, A64, n, lda64, x64, b64, r, nrhs) != BEAT_OK) return BEAT_ERR_HIP;
+  if (col_norms(ctx, r, n, nrhs, rn) != BEAT_OK) return BEAT_ERR_HIP;
+  std::vector<double> errors = errors_of(rn);
+  out.history = errors;
+  out.hist_len = 1;
+  int steps = 0, status = BEAT_REFINE_STEP_LIMIT;
+  auto all_le = [&](const std::vector<double>& e) {
+    for (double v : e)
+      if (!(v <= rtol)) return false;
+    return true;
+  };
+  for (int step = 1; step <= max_steps; ++step) {
+    if (all_le(errors)) {
+      status = BEAT_REFINE_CONVERGED;
+      break;
+    }
+    for (int j = 0; j < nrhs; ++j) mask[j] = errors[j] <= rtol ? 0 : 1;
+    HIPCHK(hipMemcpyAsync(dmask, mask.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(masked_downcast_kernel, dim3(blocks_for(n * nrhs)), dim3(256), 0, ctx->stream, (const double*)r,
+                       d, n, nrhs, (const int*)dmask);
+    HIPCHK(hipGetLastError());
+    if (solve_all<float>(ctx, LU32, n, lda32, ipiv, d, nrhs) != BEAT_OK) return BEAT_ERR_HIP;
+    hipLaunchKernelGGL(add_delta_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, ctx->stream, (const double*)x64,
+                       (const float*)d, nx, 2 * n * nrhs);
+    HIPCHK(hipGetLastError());
+    if (residual_all(ctx, A64, n, lda64, nx, b64, nr, nrhs) != BEAT_OK) return BEAT_ERR_HIP;
+    std::vector<double> nrn;
+    if (col_norms(ctx, nr, n, nrhs, nrn) != BEAT_OK) return BEAT_ERR_HIP;
+    std::vector<double> ne = errors_of(nrn);
+    bool improved = false;
+    for (int j = 0; j < nrhs; ++j) {
+      mask[j] = (std::isfinite(ne[j]) && ne[j] < errors[j]) ? 1 : 0;
+      if (mask[j]) {
+        errors[j] = ne[j];
+        improved = true;
+      }
+    }
+    HIPCHK(hipMemcpyAsync(dmask, mask.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(accept_kernel, dim3(blocks_for(n * nrhs)), dim3(256), 0, ctx->stream, x64, (const double*)nx, r,
+                       (const double*)nr, n, nrhs, (const int*)dmask);
+    HIPCHK(hipGetLastError());
+    steps = step;
+    out.history.insert(out.history.end(), errors.begin(), errors.end());
+    out.hist_len += 1;
+    if (!improved) {
+      status = BEAT_REFINE_STAGNATED;
+      break;
+    }
+  }
+  if (all_le(errors)) status = BEAT_REFINE_CONVERGED;
+  std::vector<double> returned = errors;
+  if (returned_round) {
+    hipLaunchKernelGGL(round32_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, ctx->stream, (const double*)x64, nx,
+                       2 * n * nrhs);
+    HIPCHK(hipGetLastError());
+    if (residual_all(ctx, A64, n, lda64, nx, b64, nr, nrhs) != BEAT_OK) return BEAT_ERR_HIP;
+    std::vector<double> rr;
+    if (col_norms(ctx, nr, n, nrhs, rr) != BEAT_OK) return BEAT_ERR_HIP;
+    returned = errors_of(rr);
+  }
+  bool nonfinite_w = false, nonfinite_r = false;
+  for (double v : errors) nonfinite_w |= !std::isfinite(v);
+  for (double v : returned) nonfinite_r |= !std::isfinite(v);
+  const int working = nonfinite_w ? BEAT_REFINE_NONFINITE : status;
+  int st;
+  if (nonfinite_r) st = BEAT_REFINE_NONFINITE;
+  else if (all_le(returned)) st = BEAT_REFINE_CONVERGED;
+  else if (working == BEAT_REFINE_CONVERGED) st = BEAT_REFINE_ROUNDING_LIMITED;
+  else st = working;
+  out.steps = steps;
+  out.status = st;
+  out.working = working;
+  out.working_res = errors;
+  out.returned_res = returned;
+  return BEAT_OK;
+}
+
+void fill_report(beat_refine_report* rep, const Refine& R, double rtol, int nrhs) {
+  if (!rep) return;
+  rep->steps = R.steps;
+  rep->status = R.status;
+  rep->working_status = R.working;
+  rep->history_len = R.hist_len;
+  rep->rtol = rtol;
+  if (rep->working_residuals)
+    for (int j = 0; j < nrhs; ++j) rep->working_residuals[j] = R.working_res[j];
+  if (rep->returned_residuals)
+    for (int j = 0; j < nrhs; ++j) rep->returned_residuals[j] = R.returned_res[j];
+  if (rep->history)
+    for (size_t i = 0; i < R.history.size(); ++i) rep->history[i] = R.history[i];
+}
+
+template <typename T>
+T* mesh_rec(const beat_mesh* m);
+template <>
+double* mesh_rec<double>(const beat_mesh* m) {
+  return m->rec64;
+}
+template <>
+float* mesh_rec<float>(const beat_mesh* m) {
+  return m->rec32;
+}
+
+// Assemble A (row-major, lda) and B (n x nrhs row-major) in precision T from the
+// internal-order q64 (and q32 for T = float).
+template <typename T>
+int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, const double* q64, const T* qT,
+             int nrhs, T* A, int64_t lda, T* B) {
+  const int64_t n = m->h.nv;
+  HIPCHK(launch_fill<T>(A, 2 * n * lda, T(0), ctx->stream));
+  HIPCHK(launch_fill<T>(B, 2 * n * nrhs, T(0), ctx->stream));
+  RegularArgs<T> ra;
+  ra.rec = mesh_rec<T>(m);
+  ra.fverts = m->fverts;
+  ra.floc = m->floc;
+  ra.tile_start = m->tile_start;
+  ra.tile_count = m->tile_count;
+  ra.tile_nv = m->tile_nv;
+  ra.tile_verts = m->tile_verts;
+  ra.ntiles = (int)m->h.tile_start.size();
+  ra.A = A;
+  ra.lda = lda;
+  ra.B = B;
+  ra.q = qT;
+  ra.nrhs = nrhs;
+  ra.k = (T)k;
+  ra.krev = (T)(k / (2.0 * M_PI));
+  ra.eta_re = (T)er;
+  ra.eta_im = (T)ei;
+  for (int i = 0; i < 6; ++i) {
+    const bool on = i < m->h.nq;
+    const double xi = on ? m->h.rule_pts[2 * i] : 0, eta = on ? m->h.rule_pts[2 * i + 1] : 0;
+    ra.w[i] = on ? (T)m->h.rule_w[i] : T(0);
+    ra.phi[i][0] = (T)(1.0 - xi - eta);
+    ra.phi[i][1] = (T)xi;
+    ra.phi[i][2] = (T)eta;
+  }
+  HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+  HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, ctx->stream));
+  HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+  SingularArgs<T> sa;
+  sa.rec = m->rec64;
+  sa.verts = m->verts;
+  sa.fverts = m->fverts;
+  sa.pairs = m->spairs;
+  sa.npairs = (int64_t)m->h.spairs.size();
+  sa.rule_off = m->rule_off;
+  sa.rule_cnt = m->rule_cnt;
+  sa.rule_data = m->rule_data;
+  sa.A = A;
+  sa.lda = lda;
+  sa.B = B;
+  sa.q = q64;
+  sa.nrhs = nrhs;
+  sa.k = k;
+  sa.eta_re = er;
+  sa.eta_im = ei;
+  HIPCHK(launch_singular<T>(sa, ctx->stream));
+  IdentityArgs<T> ia;
+  ia.rec = m->rec64;
+  ia.fverts = m->fverts;
+  ia.nf = m->h.nf;
+  ia.A = A;
+  ia.lda = lda;
+  ia.B = B;
+  ia.q = q64;
+  ia.nrhs = nrhs;
+  ia.eta_re = er;
+  ia.eta_im = ei;
+  for (int p = 0; p < 3; ++p) {
+    ia.pd[p] = m->h.id_pd[p];
+    for (int b = 0; b < 3; ++b) ia.pp[p][b] = m->h.id_pp[p][b];
+  }
+  HIPCHK(launch_identity<T>(ia, ctx->stream));
+  return BEAT_OK;
+}
+
+int check_mesh(beat_ctx* ctx, const beat_mesh* m) {
+  if (!ctx) return BEAT_ERR_ARG;
+  if (!m || m->ctx != ctx) return fail(ctx, BEAT_ERR_ARG, "mesh does not belong to this context");
+  return BEAT_OK;
+}
+
+// q_dev: caller column-major complex128 (device); p_dev: caller column-major out.
+int solve_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei, const double* q_dev,
+               int nrhs, double* p_dev, const beat_solve_options* opts, beat_solve_report* rep) {
+  const int64_t n = m->h.nv, nf = m->h.nf;
+  const int64_t lda = lda_for(n);
+  hipStream_t s = ctx->stream;
+  int max_steps = opts ? opts->max_steps : 3;
+  double rtol = opts ? opts->rtol : 1e-10;
+  if (max_steps < 1 || max_steps > 8) return fail(ctx, BEAT_ERR_ARG, "max_steps must be in 1..8");
+  if (!(rtol > 0) || !std::isfinite(rtol)) return fail(ctx, BEAT_ERR_ARG, "rtol must be finite and positive");
+  HIPCHK(ensure(ctx->q64, sizeof(double) * 2 * nf * nrhs));
+  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, s, q_dev, nf, nrhs,
+                     (const int32_t*)m->face_perm, as<double>(ctx->q64));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev[0], s));
+  LuWork lw;
+  if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
+  int panels = 0;
+  Refine R;
+  if (precision == BEAT_PREC_FP64) {
+    HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
+    HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n * nrhs));
+    int rc = assemble<double>(ctx, m, k, er, ei, as<double>(ctx->q64), as<double>(ctx->q64), nrhs, as<double>(ctx->a64),
+                              lda, as<double>(ctx->b64));
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    HIPCHK(lu_factor<double>(as<double>(ctx->a64), n, lda, lw, LU_NB, s, &panels));
+    HIPCHK(hipEventRecord(ctx->ev[2], s));
+    rc = solve_all<double>(ctx, as<double>(ctx->a64), n, lda, lw.ipiv, as<double>(ctx->b64), nrhs);
+    if (rc) return rc;
+    hipLaunchKernelGGL((scatter_p_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
+                       (const double*)as<double>(ctx->b64), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
+    HIPCHK(hipGetLastError());
+  } else if (precision == BEAT_PREC_FP32) {
+    HIPCHK(ensure(ctx->q32, sizeof(float) * 2 * nf * nrhs));
+    HIPCHK(launch_downcast(as<double>(ctx->q64), as<float>(ctx->q32), 2 * nf * nrhs, s));
+    HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
+    HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * n * nrhs));
+    int rc = assemble<float>(ctx, m, k, er, ei, as<double>(ctx->q64), as<float>(ctx->q32), nrhs, as<float>(ctx->a32),
+                             lda, as<float>(ctx->b32));
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
+    HIPCHK(hipEventRecord(ctx->ev[2], s));
+    rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), nrhs);
+    if (rc) return rc;
+    hipLaunchKernelGGL((scatter_p_kernel<float>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
+                       (const float*)as<float>(ctx->b32), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
+    HIPCHK(hipGetLastError());
+  } else if (precision == BEAT_PREC_FP32_REFINED) {
+    HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
+    HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n * nrhs));
+    HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
+    HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * n * nrhs));
+    HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * n * nrhs));
+    int rc = assemble<double>(ctx, m, k, er, ei, as<double>(ctx->q64), as<double>(ctx->q64), nrhs, as<double>(ctx->a64),
+                              lda, as<double>(ctx->b64));
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    HIPCHK(launch_downcast(as<double>(ctx->a64), as<float>(ctx->a32), 2 * n * lda, s));
+    HIPCHK(launch_downcast(as<double>(ctx->b64), as<float>(ctx->b32), 2 * n * nrhs, s));
+    HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
+    HIPCHK(hipEventRecord(ctx->ev[2], s));
+    rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), nrhs);
+    if (rc) return rc;
+    hipLaunchKernelGGL(upcast_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, s, (const float*)as<float>(ctx->b32),
+                       as<double>(ctx->x64), 2 * n * nrhs);
+    HIPCHK(hipGetLastError());
+    rc = refine_loop(ctx, as<float>(ctx->a32), lda, lw.ipiv, as<double>(ctx->a64), lda, as<double>(ctx->b64),
+                     as<double>(ctx->x64), n, nrhs, max_steps, rtol, false, R);
+    if (rc) return rc;
+    hipLaunchKernelGGL((scatter_p_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
+                       (const double*)as<double>(ctx->x64), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
+    HIPCHK(hipGetLastError());
+  } else {
+    return fail(ctx, BEAT_ERR_ARG, "unknown precision");
+  }
+  HIPCHK(hipEventRecord(ctx->ev[3], s));
+  HIPCHK(hipEventSynchronize(ctx->ev[3]));
+  int info = 0;
+  HIPCHK(hipMemcpy(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost));
+  if (rep) {
+    float t01, t12, t23;
+    HIPCHK(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+    HIPCHK(hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
+    HIPCHK(hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
+    float t45;
+    HIPCHK(hipEventElapsedTime(&t45, ctx->ev[4], ctx->ev[5]));
+    rep->regular_kernel_s = t45 * 1e-3;
+    rep->assembly_s = t01 * 1e-3;
+    rep->factor_s = t12 * 1e-3;
+    rep->solve_s = t23 * 1e-3;
+    rep->factorization_count = 1;
+    rep->lu_panels = panels;
+    rep->regular_pairs = m->regular_pairs;
+    rep->singular_pairs = (int64_t)m->h.spairs.size();
+    if (precision == BEAT_PREC_FP32_REFINED) {
+      fill_report(&rep->refine, R, rtol, nrhs);
+    } else {
+      rep->refine.steps = 0;
+      rep->refine.status = BEAT_REFINE_NONE;
+      rep->refine.working_status = BEAT_REFINE_NONE;
+      rep->refine.history_len = 0;
+      rep->refine.rtol = rtol;
+    }
+  }
+  if (info != 0) return fail(ctx, BEAT_ERR_SINGULAR, "LU found an exactly zero pivot at column " + std::to_string(info));
+  return BEAT_OK;
+}
+
+template <typename T>
+hipError_t upload(T** dst, const std::vector<T>& v) {
+  if (v.empty()) {
+    *dst = nullptr;
+    return hipSuccess;
+  }
+  hipError_t e = hipMalloc((void**)dst, sizeof(T) * v.size());
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+extern "C" {
+
+int beat_ctx_create(int device, beat_ctx** out) {
+  if (!out) return BEAT_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return BEAT_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return BEAT_ERR_HIP;
+  beat_ctx* c = new beat_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return BEAT_ERR_HIP;
+  }
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete c;
+      return BEAT_ERR_HIP;
+    }
+  *out = c;
+  return BEAT_OK;
+}
+
+void beat_ctx_destroy(beat_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64,
+                    &c->rn64, &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
+                    &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts};
+  for (DevBuf* b : bufs)
+    if (b->p) (void)hipFree(b->p);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* beat_last_error(const beat_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int beat_mesh_upload(beat_ctx* ctx, const double* verts, int64_t nv, const int32_t* faces, int64_t nf,
+                     int quad_order, int singular_order, beat_mesh** out) {
+  if (!ctx || !out || !verts || !faces) return fail(ctx, BEAT_ERR_ARG, "null argument");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  *out = nullptr;
+  HIPCHK(hipSetDevice(ctx->device));
+  beat_mesh* m = new beat_mesh();
+  m->ctx = ctx;
+  std::string msg = build_host_mesh(verts, nv, faces, nf, quad_order, singular_order, m->h);
+  if (!msg.empty()) {
+    delete m;
+    return fail(ctx, BEAT_ERR_ARG, msg);
+  }
+  std::vector<float> rec32(m->h.rec.begin(), m->h.rec.end());
+  std::vector<double> rule3;
+  for (int q = 0; q < m->h.nq; ++q) rule3.insert(rule3.end(), {m->h.rule_pts[2 * q], m->h.rule_pts[2 * q + 1], m->h.rule_w[q]});
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = upload(&m->rec64, m->h.rec);
+  if (e == hipSuccess) e = upload(&m->rec32, rec32);
+  if (e == hipSuccess) e = upload(&m->verts, m->h.verts);
+  if (e == hipSuccess) e = upload(&m->fverts, m->h.fverts);
+  if (e == hipSuccess) e = upload(&m->floc, m->h.floc);
+  if (e == hipSuccess) e = upload(&m->tile_start, m->h.tile_start);
+  if (e == hipSuccess) e = upload(&m->tile_count, m->h.tile_count);
+  if (e == hipSuccess) e = upload(&m->tile_nv, m->h.tile_nv);
+  if (e == hipSuccess) e = upload(&m->tile_verts, m->h.tile_verts);
+  if (e == hipSuccess) e = upload(&m->spairs, m->h.spairs);
+  if (e == hipSuccess) e = upload(&m->rule_off, m->h.rule_off);
+  if (e == hipSuccess) e = upload(&m->rule_cnt, m->h.rule_cnt);
+  if (e == hipSuccess) e = upload(&m->rule_data, m->h.rule_data);
+  if (e == hipSuccess) e = upload(&m->rule3, rule3);
+  if (e == hipSuccess) e = upload(&m->face_perm, m->h.face_perm);
+  if (e == hipSuccess) e = upload(&m->vert_inv, m->h.vert_inv);
+  if (e != hipSuccess) {
+    beat_mesh_destroy(m);
+    return fail(ctx, BEAT_ERR_HIP, std::string("mesh upload: ") + hipGetErrorString(e));
+  }
+  m->regular_pairs = nf * nf - (int64_t)m->h.spairs.size();
+  *out = m;
+  return BEAT_OK;
+}
+
+void beat_mesh_destroy(beat_mesh* m) {
+  if (!m) return;
+  void* ptrs[] = {m->rec64, m->rec32, m->verts, m->fverts, m->floc, m->tile_start, m->tile_count, m->tile_nv,
+                  m->tile_verts, m->spairs, m->rule_off, m->rule_cnt, m->rule_data, m->rule3, m->face_perm,
+                  m->vert_inv};
+  if (m->ctx) (void)hipSetDevice(m->ctx->device);
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete m;
+}
+
+int beat_mesh_counts(const beat_mesh* m, int64_t* counts) {
+  if (!m || !counts) return BEAT_ERR_ARG;
+  counts[0] = m->regular_pairs;
+  counts[1] = m->h.n_coincident;
+  counts[2] = m->h.n_edge;
+  counts[3] = m->h.n_vertex;
+  counts[4] = (int64_t)m->h.tile_start.size();
+  counts[5] = m->h.nq;
+  return BEAT_OK;
+}
+
+int beat_bm_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei, const double* q,
+                     int64_t nrhs, double* a_out, double* b_out) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (!q || !a_out || !b_out || nrhs < 1) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (precision != BEAT_PREC_FP64 && precision != BEAT_PREC_FP32) return fail(ctx, BEAT_ERR_ARG, "precision");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  const int64_t n = m->h.nv, nf = m->h.nf, lda = lda_for(n);
+  hipStream_t s = ctx->stream;
+  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * nf * nrhs));
+  HIPCHK(hipMemcpyAsync(ctx->qin.p, q, sizeof(double) * 2 * nf * nrhs, hipMemcpyHostToDevice, s));
+  HIPCHK(ensure(ctx->q64, sizeof(double) * 2 * nf * nrhs));
+  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, s,
+                     (const double*)as<double>(ctx->qin), nf, (int)nrhs, (const int32_t*)m->face_perm,
+                     as<double>(ctx->q64));
+  HIPCHK(hipGetLastError());
+  std::vector<double> A((size_t)2 * n * lda), B((size_t)2 * n * nrhs);
+  if (precision == BEAT_PREC_FP64) {
+    HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
+    HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n * nrhs));
+    if (int rc = assemble<double>(ctx, m, k, er, ei, as<double>(ctx->q64), as<double>(ctx->q64), (int)nrhs,
+                                  as<double>(ctx->a64), lda, as<double>(ctx->b64)))
+      return rc;
+    HIPCHK(hipMemcpyAsync(A.data(), ctx->a64.p, sizeof(double) * A.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(B.data(), ctx->b64.p, sizeof(double) * B.size(), hipMemcpyDeviceToHost, s));
+  } else {
+    HIPCHK(ensure(ctx->q32, sizeof(float) * 2 * nf * nrhs));
+    HIPCHK(launch_downcast(as<double>(ctx->q64), as<float>(ctx->q32), 2 * nf * nrhs, s));
+    HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
+    HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * n * nrhs));
+    if (int rc = assemble<float>(ctx, m, k, er, ei, as<double>(ctx->q64), as<float>(ctx->q32), (int)nrhs,
+                                 as<float>(ctx->a32), lda, as<float>(ctx->b32)))
+      return rc;
+    std::vector<float> A32(A.size()), B32(B.size());
+    HIPCHK(hipMemcpyAsync(A32.data(), ctx->a32.p, sizeof(float) * A32.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(B32.data(), ctx->b32.p, sizeof(float) * B32.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::copy(A32.begin(), A32.end(), A.begin());
+    std::copy(B32.begin(), B32.end(), B.begin());
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  const auto& inv = m->h.vert_inv;
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t r = 0; r < n; ++r) {
+      const size_t src = 2 * ((size_t)inv[r] * lda + inv[c]);
+      a_out[2 * (r + c * n)] = A[src];
+      a_out[2 * (r + c * n) + 1] = A[src + 1];
+    }
+  for (int64_t d = 0; d < nrhs; ++d)
+    for (int64_t r = 0; r < n; ++r) {
+      const size_t src = 2 * ((size_t)inv[r] * nrhs + d);
+      b_out[2 * (r + d * n)] = B[src];
+      b_out[2 * (r + d * n) + 1] = B[src + 1];
+    }
+  return BEAT_OK;
+}
+
+int beat_bm_solve(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei, const double* q,
+                  int64_t nrhs, double* pressure_out, const beat_solve_options* opts, beat_solve_report* rep) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (!q || !pressure_out || nrhs < 1 || nrhs > 4096) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  const int64_t n = m->h.nv, nf = m->h.nf;
+  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * nf * nrhs));
+  HIPCHK(ensure(ctx->pout, sizeof(double) * 2 * n * nrhs));
+  HIPCHK(hipMemcpyAsync(ctx->qin.p, q, sizeof(double) * 2 * nf * nrhs, hipMemcpyHostToDevice, ctx->stream));
+  int rc = solve_impl(ctx, m, precision, k, er, ei, as<double>(ctx->qin), (int)nrhs, as<double>(ctx->pout), opts, rep);
+  if (rc != BEAT_OK && rc != BEAT_ERR_SINGULAR) return rc;
+  HIPCHK(hipMemcpyAsync(pressure_out, ctx->pout.p, sizeof(double) * 2 * n * nrhs, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return rc;
+}
+
+int beat_bm_solve_device(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei,
+                         const void* q_dev, int64_t nrhs, void* p_dev, const beat_solve_options* opts,
+                         beat_solve_report* rep) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (!q_dev || !p_dev || nrhs < 1 || nrhs > 4096) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  return solve_impl(ctx, m, precision, k, er, ei, (const double*)q_dev, (int)nrhs, (double*)p_dev, opts, rep);
+}
+
+int beat_field_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const double* pressure,
+                    const double* q, int64_t nrhs, const double* pts, int64_t npts, double* out) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (nrhs < 1 || npts < 0 || (npts > 0 && (!pressure || !q || !pts || !out))) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (precision != BEAT_PREC_FP64 && precision != BEAT_PREC_FP32) return fail(ctx, BEAT_ERR_ARG, "precision");
+  if (npts == 0) return BEAT_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t n = m->h.nv, nf = m->h.nf, ns = nf * m->h.nq;
+  // stage: pressure (n x nrhs, caller col-major), q (nf x nrhs) -> internal row-major
+  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * (nf + n) * nrhs));
+  double* qin = as<double>(ctx->qin);
+  double* pin = qin + 2 * nf * nrhs;
+  HIPCHK(hipMemcpyAsync(qin, q, sizeof(double) * 2 * nf * nrhs, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(pin, pressure, sizeof(double) * 2 * n * nrhs, hipMemcpyHostToDevice, s));
+  HIPCHK(ensure(ctx->q64, sizeof(double) * 2 * (nf + n) * nrhs));
+  double* qint = as<double>(ctx->q64);
+  double* pint = qint + 2 * nf * nrhs;
+  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, s, (const double*)qin, nf,
+                     (int)nrhs, (const int32_t*)m->face_perm, qint);
+  // pressure: internal row v takes caller row vert_perm[v]; gather with the vertex permutation
+  std::vector<int32_t> vp(m->h.vert_perm);
+  HIPCHK(ensure(ctx->lusmall, sizeof(int32_t) * n + 256));
+  HIPCHK(hipMemcpyAsync(ctx->lusmall.p, vp.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s, (const double*)pin, n,
+                     (int)nrhs, (const int32_t*)as<int32_t>(ctx->lusmall), pint);
+  HIPCHK(hipGetLastError());
+  const size_t es = precision == BEAT_PREC_FP64 ? sizeof(double) : sizeof(float);
+  HIPCHK(ensure(ctx->fld_src, es * (ns * 6 + 4 * ns * nrhs + 2 * (nf + n) * nrhs)));
+  HIPCHK(ensure(ctx->fld_pts, sizeof(double) * 3 * npts));
+  HIPCHK(ensure(ctx->fld_out, sizeof(double) * 2 * npts * nrhs));
+  HIPCHK(hipMemcpyAsync(ctx->fld_pts.p, pts, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, s));
+  std::vector<double> res((size_t)2 * npts * nrhs);
+  if (precision == BEAT_PREC_FP64) {
+    double* sp = as<double>(ctx->fld_src);
+    double* sn = sp + 3 * ns;
+    double* pw = sn + 3 * ns;
+    double* qw = pw + 2 * ns * nrhs;
+    HIPCHK(field_sources<double>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, pint, qint, (int)nrhs, sp, sn, pw, qw, s));
+    HIPCHK(field_eval<double>(sp, sn, pw, qw, ns, (int)nrhs, as<double>(ctx->fld_pts), npts, as<double>(ctx->fld_out),
+                              k, s));
+    HIPCHK(hipMemcpyAsync(res.data(), ctx->fld_out.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, s));
+  } else {
+    float* sp = as<float>(ctx->fld_src);
+    float* sn = sp + 3 * ns;
+    float* pw = sn + 3 * ns;
+    float* qw = pw + 2 * ns * nrhs;
+    float* p32 = qw + 2 * ns * nrhs;
+    float* q32 = p32 + 2 * n * nrhs;
+    HIPCHK(launch_downcast(pint, p32, 2 * n * nrhs, s));
+    HIPCHK(launch_downcast(qint, q32, 2 * nf * nrhs, s));
+    // points in float
+    float* pts32 = reinterpret_cast<float*>(as<double>(ctx->fld_pts) + 0);
+    std::vector<float> hp(3 * npts);
+    for (int64_t i = 0; i < 3 * npts; ++i) hp[i] = (float)pts[i];
+    HIPCHK(hipMemcpyAsync(pts32, hp.data(), sizeof(float) * hp.size(), hipMemcpyHostToDevice, s));
+    float* o32 = as<float>(ctx->fld_out);
+    HIPCHK(field_sources<float>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, p32, q32, (int)nrhs, sp, sn, pw, qw, s));
+    HIPCHK(field_eval<float>(sp, sn, pw, qw, ns, (int)nrhs, pts32, npts, o32, (float)k, s));
+    std::vector<float> r32(res.size());
+    HIPCHK(hipMemcpyAsync(r32.data(), o32, sizeof(float) * r32.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::copy(r32.begin(), r32.end(), res.begin());
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  for (int64_t d = 0; d < nrhs; ++d)
+    for (int64_t i = 0; i < npts; ++i) {
+      out[2 * (i + d * npts)] = res[2 * (i * nrhs + d)];
+      out[2 * (i + d * npts) + 1] = res[2 * (i * nrhs + d) + 1];
+    }
+  return BEAT_OK;
+}
+
+int beat_dense_lu_solve(beat_ctx* ctx, int precision, const double* a, int64_t n, const double* b, int64_t nrhs,
+                        double* x) {
+  if (!ctx) return BEAT_ERR_ARG;
+  if (!a || !b || !x || n < 1 || nrhs < 1) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (precision != BEAT_PREC_FP64 && precision != BEAT_PREC_FP32) return fail(ctx, BEAT_ERR_ARG, "precision");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t lda = lda_for(n);
+  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * (n * n + n * nrhs)));
+  double* ain = as<double>(ctx->qin);
+  double* bin = ain + 2 * n * n;
+  HIPCHK(hipMemcpyAsync(ain, a, sizeof(double) * 2 * n * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(bin, b, sizeof(double) * 2 * n * nrhs, hipMemcpyHostToDevice, s));
+  LuWork lw;
+  if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
+  HIPCHK(ensure(ctx->pout, sizeof(double) * 2 * n * nrhs));
+  int panels = 0;
+  if (precision == BEAT_PREC_FP64) {
+    HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
+    HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n * nrhs));
+    hipLaunchKernelGGL((transpose_in_kernel<double>), dim3(blocks_for(n * n)), dim3(256), 0, s, (const double*)ain, n,
+                       lda, as<double>(ctx->a64));
+    hipLaunchKernelGGL((cols_in_kernel<double, double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
+                       (const double*)bin, n, (int)nrhs, as<double>(ctx->b64));
+    HIPCHK(hipGetLastError());
+    HIPCHK(lu_factor<double>(as<double>(ctx->a64), n, lda, lw, LU_NB, s, &panels));
+    if (int rc = solve_all<double>(ctx, as<double>(ctx->a64), n, lda, lw.ipiv, as<double>(ctx->b64), (int)nrhs)) return rc;
+    hipLaunchKernelGGL((cols_out_kernel<double, double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
+                       (const double*)as<double>(ctx->b64), n, (int)nrhs, as<double>(ctx->pout));
+  } else {
+    HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
+    HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * n * nrhs));
+    hipLaunchKernelGGL((transpose_in_kernel<float>), dim3(blocks_for(n * n)), dim3(256), 0, s, (const double*)ain, n,
+                       lda, as<float>(ctx->a32));
+    hipLaunchKernelGGL((cols_in_kernel<double, float>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s, (const double*)bin,
+                       n, (int)nrhs, as<float>(ctx->b32));
+    HIPCHK(hipGetLastError());
+    HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
+    if (int rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), (int)nrhs)) return rc;
+    hipLaunchKernelGGL((cols_out_kernel<float, double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
+                       (const float*)as<float>(ctx->b32), n, (int)nrhs, as<double>(ctx->pout));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(x, ctx->pout.p, sizeof(double) * 2 * n * nrhs, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  int info = 0;
+  HIPCHK(hipMemcpy(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost));
+  if (info != 0) return fail(ctx, BEAT_ERR_SINGULAR, "LU found an exactly zero pivot at column " + std::to_string(info));
+  return BEAT_OK;
+}
+
+int beat_refine_lu(beat_ctx* ctx, const float* matrix, int64_t n, const float* rhs, int64_t nrhs,
+                   const double* residual_matrix, const double* residual_rhs, int max_steps, double rtol, float* x_out,
+                   beat_refine_report* rep) {
+  if (!ctx) return BEAT_ERR_ARG;
+  if (max_steps < 1 || max_steps > 3) return fail(ctx, BEAT_ERR_ARG, "max_steps must be in 1:3");
+  if (!(rtol > 0) || !std::isfinite(rtol)) return fail(ctx, BEAT_ERR_ARG, "rtol must be finite and positive");
+  if (!matrix || !rhs || !x_out || n < 1 || nrhs < 1) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t lda = lda_for(n);
+  // host staging: widen the complex64 inputs where no residual operator is given
+  std::vector<double> A64(2 * n * n), b64(2 * n * nrhs);
+  for (int64_t i = 0; i < 2 * n * n; ++i) A64[i] = residual_matrix ? residual_matrix[i] : (double)matrix[i];
+  for (int64_t i = 0; i < 2 * n * nrhs; ++i) b64[i] = residual_rhs ? residual_rhs[i] : (double)rhs[i];
+  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * (n * n + n * nrhs) + sizeof(float) * 2 * (n * n + n * nrhs)));
+  double* ain = as<double>(ctx->qin);
+  double* bin = ain + 2 * n * n;
+  float* mat32 = reinterpret_cast<float*>(bin + 2 * n * nrhs);
+  float* rhs32 = mat32 + 2 * n * n;
+  HIPCHK(hipMemcpyAsync(ain, A64.data(), sizeof(double) * A64.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(bin, b64.data(), sizeof(double) * b64.size(), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(mat32, matrix, sizeof(float) * 2 * n * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(rhs32, rhs, sizeof(float) * 2 * n * nrhs, hipMemcpyHostToDevice, s));
+  HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
+  HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
+  HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n * nrhs));
+  HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * n * nrhs));
+  HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * n * nrhs));
+  hipLaunchKernelGGL((transpose_in_kernel<double>), dim3(blocks_for(n * n)), dim3(256), 0, s, (const double*)ain, n, lda,
+                     as<double>(ctx->a64));
+  hipLaunchKernelGGL((transpose_in_f32_kernel<float>), dim3(blocks_for(n * n)), dim3(256), 0, s, (const float*)mat32, n,
+                     lda, as<float>(ctx->a32));
+  hipLaunchKernelGGL((cols_in_kernel<double, double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s, (const double*)bin,
+                     n, (int)nrhs, as<double>(ctx->b64));
+  hipLaunchKernelGGL((cols_in_kernel<float, float>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s, (const float*)rhs32, n,
+                     (int)nrhs, as<float>(ctx->b32));
+  HIPCHK(hipGetLastError());
+  LuWork lw;
+  if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
+  int panels = 0;
+  HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
+  if (int rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), (int)nrhs)) return rc;
+  hipLaunchKernelGGL(upcast_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, s, (const float*)as<float>(ctx->b32),
+                     as<double>(ctx->x64), 2 * n * nrhs);
+  HIPCHK(hipGetLastError());
+  Refine R;
+  if (int rc = refine_loop(ctx, as<float>(ctx->a32), lda, lw.ipiv, as<double>(ctx->a64), lda, as<double>(ctx->b64),
+                           as<double>(ctx->x64), n, (int)nrhs, max_steps, rtol, true, R))
+    return rc;
+  HIPCHK(ensure(ctx->pout, sizeof(float) * 2 * n * nrhs));
+  hipLaunchKernelGGL((cols_out_kernel<double, float>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
+                     (const double*)as<double>(ctx->x64), n, (int)nrhs, as<float>(ctx->pout));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(x_out, ctx->pout.p, sizeof(float) * 2 * n * nrhs, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  fill_report(rep, R, rtol, (int)nrhs);
+  return BEAT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,459 @@
+// Fused Burton-Miller assembly on gfx950.
+//
+//   A = 0.5 M_p1p1 - D + eta H          (N x N, P1 x P1)
+//   B = (-S - eta (K' + 0.5 M_p1dp0)) q (N x nrhs)
+//
+// per element pair and quadrature point pair, the algebra of the reference's
+// _beat_cpu_bm_regular_pair_blocks / _beat_cpu_bm_pair_blocks
+// (src/beat_engine/julia_local/src/BeatEngineCpuBurtonMiller.jl:62-167):
+//   lhs_s = -G' (r.n_y / r) w - eta k^2 (n_x.n_y) G w
+//   rhs_s = -G w - eta G' (-r.n_x / r) w,     G' = G (-1/r + i k)
+//   LHS_ab += phi_a(x) phi_b(y) lhs_s,  RHS_a += phi_a(x) rhs_s,
+//   LHS_ab += (curl_a . curl_b) eta sum(G w)
+// Regular pairs (no shared vertex) run in bm_regular_kernel: one workgroup per
+// (test tile, trial tile), each lane owns one test face, waves stride over the
+// trial faces; 3x3 blocks land in an LDS tile keyed by (test vertex slot, trial
+// vertex slot) and one global atomic per touched entry flushes it. Touching
+// pairs run in bm_singular_kernel in FP64 over the Duffy rules, one wavefront
+// per pair with a wavefront reduction.
+#include <hip/hip_runtime.h>
+
+#include "bem_common.h"
+#include "kernels.h"
+
+namespace beat {
+
+namespace {
+
+template <typename T>
+__device__ __forceinline__ void lds_add(T* p, T v) {
+  atomicAdd(p, v);
+}
+
+template <typename T>
+__device__ __forceinline__ void gadd(T* p, T v) {
+  unsafeAtomicAdd(p, v);
+}
+
+// sin and cos of k*r. FP32: the hardware sin/cos take revolutions; the phase is
+// reduced to [-0.5, 0.5] revolutions first. FP64: library sincos.
+__device__ __forceinline__ void phase_sincos(float r, float krev, float kk, float& s, float& c) {
+  (void)kk;
+  float t = r * krev;
+  t = t - rintf(t);
+  s = __builtin_amdgcn_sinf(t);
+  c = __builtin_amdgcn_cosf(t);
+}
+__device__ __forceinline__ void phase_sincos(double r, double krev, double kk, double& s, double& c) {
+  (void)krev;
+  sincos(kk * r, &s, &c);
+}
+
+__device__ __forceinline__ float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double inv_sqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-(x * y), y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-(x * y), y, 1.0);
+  y = fma(0.5 * y, e, y);
+  return y;
+}
+
+}  // namespace
+
+// Threads per workgroup: FP64 uses 4 waves so each wave may hold the full
+// unified register file; FP32 uses 8 waves.
+template <typename T>
+struct RegularBlock {
+  static constexpr int threads = sizeof(T) == 8 ? 256 : 512;
+};
+
+template <typename T, int NQ, bool PURE>
+__global__ __launch_bounds__(RegularBlock<T>::threads) void bm_regular_kernel(const RegularArgs<T> a) {
+  __shared__ T accA[TILE_V * TILE_V * 2];
+  __shared__ T accC[TILE_V * TILE_F * 2];
+  __shared__ T srec[TILE_F * FREC];
+  __shared__ T xs[NQ * 3 * 64];  // test points, [point*3 + c][lane]
+  __shared__ int sfv[TILE_F * 3];
+  __shared__ int sloc[TILE_F * 3];
+  __shared__ int grow[TILE_V], gcol[TILE_V];
+
+  const int64_t total = (int64_t)a.ntiles * a.ntiles;
+  const int64_t per = (total + 7) / 8;
+  const int64_t L = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (L >= total) return;
+  const int tt = (int)(L / a.ntiles), st = (int)(L % a.ntiles);
+  const int tid = threadIdx.x;
+  const int t0 = a.tile_start[tt], nT = a.tile_count[tt], nvT = a.tile_nv[tt];
+  const int s0 = a.tile_start[st], nS = a.tile_count[st], nvS = a.tile_nv[st];
+
+  for (int i = tid; i < TILE_V * TILE_V * 2; i += blockDim.x) accA[i] = T(0);
+  for (int i = tid; i < TILE_V * TILE_F * 2; i += blockDim.x) accC[i] = T(0);
+  for (int i = tid; i < nS * FREC; i += blockDim.x) srec[i] = a.rec[(int64_t)s0 * FREC + i];
+  for (int i = tid; i < nS * 3; i += blockDim.x) {
+    sfv[i] = a.fverts[(int64_t)s0 * 3 + i];
+    sloc[i] = a.floc[(int64_t)s0 * 3 + i];
+  }
+  if (tid < TILE_V) {
+    grow[tid] = a.tile_verts[(int64_t)tt * TILE_V + tid];
+    gcol[tid] = a.tile_verts[(int64_t)st * TILE_V + tid];
+  }
+  __syncthreads();
+
+  const int lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const bool have_test = lane < nT;
+  const int tf = t0 + (have_test ? lane : 0);
+  const T* tr = a.rec + (int64_t)tf * FREC;
+  if (wave == 0)
+    for (int i = 0; i < NQ * 3; ++i) xs[i * 64 + lane] = tr[i];
+  __syncthreads();
+  const T nT0 = tr[REC_N], nT1 = tr[REC_N + 1], nT2 = tr[REC_N + 2];
+  const T dT = tr[REC_D], areaT = tr[REC_AREA];
+  const int tv0 = a.fverts[(int64_t)tf * 3], tv1 = a.fverts[(int64_t)tf * 3 + 1], tv2 = a.fverts[(int64_t)tf * 3 + 2];
+  const int tl0 = a.floc[(int64_t)tf * 3], tl1 = a.floc[(int64_t)tf * 3 + 1], tl2 = a.floc[(int64_t)tf * 3 + 2];
+
+  const T k = a.k, krev = a.krev;
+  const T er = a.eta_re, ei = a.eta_im;
+
+  for (int sj = wave; sj < nS; sj += nwave) {
+    const int sv0 = sfv[3 * sj], sv1 = sfv[3 * sj + 1], sv2 = sfv[3 * sj + 2];
+    const bool adj = (tv0 == sv0) | (tv0 == sv1) | (tv0 == sv2) | (tv1 == sv0) | (tv1 == sv1) | (tv1 == sv2) |
+                     (tv2 == sv0) | (tv2 == sv1) | (tv2 == sv2);
+    if (!have_test || adj) continue;
+    const T* sr = srec + sj * FREC;
+    T y[NQ][3];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) y[j][c] = sr[3 * j + c];
+    const T nS0 = sr[REC_N], nS1 = sr[REC_N + 1], nS2 = sr[REC_N + 2];
+    const T dS = sr[REC_D];
+    const T cw = T(4) * areaT * sr[REC_AREA] * T(INV_FOUR_PI);
+    const T nprod = nT0 * nS0 + nT1 * nS1 + nT2 * nS2;
+    const T kk2n = k * k * nprod;
+    // (r.n_x) = y.n_x - d_x is a per-trial-point constant; (r.n_y) = d_y - x.n_y
+    // is a per-test-point constant (both faces are flat).
+    T PT[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) PT[j] = dT - (y[j][0] * nT0 + y[j][1] * nT1 + y[j][2] * nT2);
+    T Lr[3][3], Li[3][3], Rr[3], Ri[3], csr = 0, csi = 0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      Rr[p] = Ri[p] = 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) Lr[p][b] = Li[p][b] = 0;
+    }
+#pragma unroll 1
+    for (int i = 0; i < NQ; ++i) {
+      const T x0 = xs[(3 * i) * 64 + lane], x1 = xs[(3 * i + 1) * 64 + lane], x2 = xs[(3 * i + 2) * 64 + lane];
+      const T PS = dS - (x0 * nS0 + x1 * nS1 + x2 * nS2);
+      T phi_i[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) phi_i[p] = i == 0 ? T(a.phi[0][p]) : i == 1 ? T(a.phi[1][p]) : i == 2 ? T(a.phi[2][p])
+                                          : i == 3 ? T(a.phi[3][p]) : i == 4 ? T(a.phi[4][p]) : T(a.phi[5][p]);
+      const T wi = i == 0 ? a.w[0] : i == 1 ? a.w[1] : i == 2 ? a.w[2] : i == 3 ? a.w[3] : i == 4 ? a.w[4] : a.w[5];
+      const T cwi = cw * wi;
+      T Mr[3] = {0, 0, 0}, Mi[3] = {0, 0, 0}, Sr = 0, Si = 0;
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const T rx = y[j][0] - x0, ry = y[j][1] - x1, rz = y[j][2] - x2;
+        const T r2 = rx * rx + ry * ry + rz * rz;
+        const T ir = inv_sqrt(r2);
+        const T r = r2 * ir;
+        T s, c;
+        phase_sincos(r, krev, k, s, c);
+        const T g = ir * (cwi * T(a.w[j]));
+        const T Gr = c * g, Gi = s * g;
+        const T Dr = -ir * Gr - k * Gi, Di = -ir * Gi + k * Gr;  // G' w
+        const T t = PS * ir;      // (r.n_y)/r
+        const T u = PT[j] * ir;   // -(r.n_x)/r
+        T lr, li, hr, hi;
+        if (PURE) {
+          // eta = i ei
+          const T kap = ei * kk2n, eu = ei * u;
+          lr = -t * Dr + kap * Gi;
+          li = -t * Di - kap * Gr;
+          hr = -Gr + eu * Di;
+          hi = -Gi - eu * Dr;
+        } else {
+          const T knr = er * kk2n, kni = ei * kk2n;
+          lr = -t * Dr - (knr * Gr - kni * Gi);
+          li = -t * Di - (knr * Gi + kni * Gr);
+          const T eur = er * u, eui = ei * u;
+          hr = -Gr - (eur * Dr - eui * Di);
+          hi = -Gi - (eur * Di + eui * Dr);
+        }
+        csr += Gr;
+        csi += Gi;
+        Sr += hr;
+        Si += hi;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const T ph = T(a.phi[j][b]);
+          Mr[b] += ph * lr;
+          Mi[b] += ph * li;
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const T ph = phi_i[p];
+        Rr[p] += ph * Sr;
+        Ri[p] += ph * Si;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          Lr[p][b] += ph * Mr[b];
+          Li[p][b] += ph * Mi[b];
+        }
+      }
+    }
+    // hypersingular curl term: (curl_a . curl_b) * eta * sum(G w)
+    const T ecr = er * csr - ei * csi, eci = er * csi + ei * csr;
+    const int tl[3] = {tl0, tl1, tl2};
+    const int sl[3] = {sloc[3 * sj], sloc[3 * sj + 1], sloc[3 * sj + 2]};
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const T c0 = tr[REC_CURL + 3 * p], c1 = tr[REC_CURL + 3 * p + 1], c2 = tr[REC_CURL + 3 * p + 2];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const T cc = c0 * sr[REC_CURL + 3 * b] + c1 * sr[REC_CURL + 3 * b + 1] + c2 * sr[REC_CURL + 3 * b + 2];
+        T* dst = accA + 2 * (tl[p] * TILE_V + sl[b]);
+        lds_add(dst, Lr[p][b] + cc * ecr);
+        lds_add(dst + 1, Li[p][b] + cc * eci);
+      }
+      T* dc = accC + 2 * (tl[p] * TILE_F + sj);
+      lds_add(dc, Rr[p]);
+      lds_add(dc + 1, Ri[p]);
+    }
+  }
+  __syncthreads();
+
+  // flush the P1 x P1 block
+  for (int idx = tid; idx < nvT * nvS; idx += blockDim.x) {
+    const int r = idx / nvS, c = idx - r * nvS;
+    const T vr = accA[2 * (r * TILE_V + c)], vi = accA[2 * (r * TILE_V + c) + 1];
+    if (vr != T(0) || vi != T(0)) {
+      T* dst = a.A + 2 * ((int64_t)grow[r] * a.lda + gcol[c]);
+      gadd(dst, vr);
+      gadd(dst + 1, vi);
+    }
+  }
+  // flush the right-hand sides: B[row, d] += sum_s C[row, s] q[s, d]
+  for (int idx = tid; idx < nvT * a.nrhs; idx += blockDim.x) {
+    const int r = idx / a.nrhs, d = idx - r * a.nrhs;
+    T br = 0, bi = 0;
+    for (int s = 0; s < nS; ++s) {
+      const T cr = accC[2 * (r * TILE_F + s)], ci = accC[2 * (r * TILE_F + s) + 1];
+      const T* qq = a.q + 2 * ((int64_t)(s0 + s) * a.nrhs + d);
+      const T qr = qq[0], qi = qq[1];
+      br += cr * qr - ci * qi;
+      bi += cr * qi + ci * qr;
+    }
+    if (br != T(0) || bi != T(0)) {
+      T* dst = a.B + 2 * ((int64_t)grow[r] * a.nrhs + d);
+      gadd(dst, br);
+      gadd(dst + 1, bi);
+    }
+  }
+}
+
+// Touching pairs over the remapped Duffy rules, FP64 arithmetic; one
+// wavefront per pair, lanes stride over the rule points.
+template <typename TO>
+__global__ __launch_bounds__(256) void bm_singular_kernel(const SingularArgs<TO> a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t pidx = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (pidx >= a.npairs) return;
+  const SingularPair P = a.pairs[pidx];
+  const double* tr = a.rec + (int64_t)P.test * FREC;
+  const double* sr = a.rec + (int64_t)P.trial * FREC;
+  double tv[3][3], sv[3][3];
+  int tvi[3], svi[3];
+  for (int v = 0; v < 3; ++v) {
+    tvi[v] = a.fverts[(int64_t)P.test * 3 + v];
+    svi[v] = a.fverts[(int64_t)P.trial * 3 + v];
+    for (int c = 0; c < 3; ++c) {
+      tv[v][c] = a.verts[3 * (int64_t)tvi[v] + c];
+      sv[v][c] = a.verts[3 * (int64_t)svi[v] + c];
+    }
+  }
+  const double nT[3] = {tr[REC_N], tr[REC_N + 1], tr[REC_N + 2]};
+  const double nS[3] = {sr[REC_N], sr[REC_N + 1], sr[REC_N + 2]};
+  const double k = a.k, er = a.eta_re, ei = a.eta_im;
+  const double knr = er * (k * k * P.nprod), kni = ei * (k * k * P.nprod);
+  const int off = a.rule_off[P.rule], cnt = a.rule_cnt[P.rule];
+  double acc[26];
+#pragma unroll
+  for (int i = 0; i < 26; ++i) acc[i] = 0;
+  for (int q = lane; q < cnt; q += 64) {
+    const double* rd = a.rule_data + 5 * (int64_t)(off + q);
+    const double tx = rd[0], ty = rd[1], sx = rd[2], sy = rd[3];
+    const double ta = 1.0 - tx - ty, sa = 1.0 - sx - sy;
+    double x[3], y[3];
+    for (int c = 0; c < 3; ++c) {
+      x[c] = ta * tv[0][c] + tx * tv[1][c] + ty * tv[2][c];
+      y[c] = sa * sv[0][c] + sx * sv[1][c] + sy * sv[2][c];
+    }
+    const double rx = y[0] - x[0], ry = y[1] - x[1], rz = y[2] - x[2];
+    const double radius = sqrt(rx * rx + ry * ry + rz * rz);
+    if (radius == 0.0) continue;
+    const double ir = 1.0 / radius;
+    double s, c;
+    sincos(k * radius, &s, &c);
+    const double scale = INV_FOUR_PI * ir;
+    const double Gr = c * scale, Gi = s * scale;
+    const double gr = Gr * -ir - Gi * k, gi = Gr * k + Gi * -ir;  // G (-1/r + i k)
+    const double w = rd[4] * P.jac;
+    const double wgr = Gr * w, wgi = Gi * w;
+    const double tdot = (rx * nS[0] + ry * nS[1] + rz * nS[2]) * ir;
+    const double sdot = -(rx * nT[0] + ry * nT[1] + rz * nT[2]) * ir;
+    // lhs_s = -grad*tdot*w - kn*wg ; rhs_s = -wg - eta*grad*sdot*w
+    const double lr = -gr * tdot * w - (knr * wgr - kni * wgi);
+    const double li = -gi * tdot * w - (knr * wgi + kni * wgr);
+    const double egr = er * gr - ei * gi, egi = er * gi + ei * gr;
+    const double hr = -wgr - egr * sdot * w, hi = -wgi - egi * sdot * w;
+    const double pt[3] = {ta, tx, ty}, ps[3] = {sa, sx, sy};
+    for (int p = 0; p < 3; ++p) {
+      acc[18 + 2 * p] += pt[p] * hr;
+      acc[19 + 2 * p] += pt[p] * hi;
+      for (int b = 0; b < 3; ++b) {
+        const double bp = pt[p] * ps[b];
+        acc[2 * (3 * p + b)] += bp * lr;
+        acc[2 * (3 * p + b) + 1] += bp * li;
+      }
+    }
+    acc[24] += wgr;
+    acc[25] += wgi;
+  }
+#pragma unroll
+  for (int i = 0; i < 26; ++i) {
+    double v = acc[i];
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    acc[i] = v;
+  }
+  // eta * sum(G w) times the curl products
+  const double ecr = er * acc[24] - ei * acc[25], eci = er * acc[25] + ei * acc[24];
+  if (lane < 9) {
+    const int p = lane / 3, b = lane % 3;
+    const double cc = tr[REC_CURL + 3 * p] * sr[REC_CURL + 3 * b] + tr[REC_CURL + 3 * p + 1] * sr[REC_CURL + 3 * b + 1] +
+                      tr[REC_CURL + 3 * p + 2] * sr[REC_CURL + 3 * b + 2];
+    double vr = 0, vi = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      if (i == lane) {
+        vr = acc[2 * i] + cc * ecr;
+        vi = acc[2 * i + 1] + cc * eci;
+      }
+    TO* dst = a.A + 2 * ((int64_t)tvi[p] * a.lda + svi[b]);
+    gadd(dst, (TO)vr);
+    gadd(dst + 1, (TO)vi);
+  }
+  for (int e = lane; e < 3 * a.nrhs; e += 64) {
+    const int p = e / a.nrhs, d = e - p * a.nrhs;
+    double hr = 0, hi = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (i == p) {
+        hr = acc[18 + 2 * i];
+        hi = acc[19 + 2 * i];
+      }
+    const double* qq = a.q + 2 * ((int64_t)P.trial * a.nrhs + d);
+    const double vr = hr * qq[0] - hi * qq[1], vi = hr * qq[1] + hi * qq[0];
+    TO* dst = a.B + 2 * ((int64_t)tvi[p] * a.nrhs + d);
+    gadd(dst, (TO)vr);
+    gadd(dst + 1, (TO)vi);
+  }
+}
+
+// Identity blocks: A += 0.5 M_p1p1, B += -0.5 eta M_p1dp0 q (BeatEngineCpuBurtonMiller.jl:450-453).
+template <typename TO>
+__global__ void bm_identity_kernel(const IdentityArgs<TO> a) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= a.nf) return;
+  const double area = a.rec[f * FREC + REC_AREA];
+  int v[3];
+  for (int p = 0; p < 3; ++p) v[p] = a.fverts[3 * f + p];
+  for (int p = 0; p < 3; ++p) {
+    for (int b = 0; b < 3; ++b) gadd(a.A + 2 * ((int64_t)v[p] * a.lda + v[b]), (TO)(0.5 * (area * a.pp[p][b])));
+    const double m = area * a.pd[p];
+    // (-0.5 eta) * m * q
+    const double cr = -0.5 * a.eta_re * m, ci = -0.5 * a.eta_im * m;
+    for (int d = 0; d < a.nrhs; ++d) {
+      const double* qq = a.q + 2 * (f * a.nrhs + d);
+      gadd(a.B + 2 * ((int64_t)v[p] * a.nrhs + d), (TO)(cr * qq[0] - ci * qq[1]));
+      gadd(a.B + 2 * ((int64_t)v[p] * a.nrhs + d) + 1, (TO)(cr * qq[1] + ci * qq[0]));
+    }
+  }
+}
+
+template <typename T>
+__global__ void fill_kernel(T* p, int64_t n, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void downcast_kernel(const double* src, float* dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
+// ---- launchers -------------------------------------------------------------------
+
+template <typename T>
+hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure, hipStream_t s) {
+  const int64_t total = (int64_t)a.ntiles * a.ntiles;
+  const int64_t per = (total + 7) / 8;
+  dim3 grid((unsigned)(per * 8)), block(RegularBlock<T>::threads);
+  if (nq == 6) {
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 6, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 6, false>), grid, block, 0, s, a);
+  } else if (nq == 3) {
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 3, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 3, false>), grid, block, 0, s, a);
+  } else {
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 1, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 1, false>), grid, block, 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+template <typename TO>
+hipError_t launch_singular(const SingularArgs<TO>& a, hipStream_t s) {
+  if (a.npairs == 0) return hipSuccess;
+  dim3 grid((unsigned)((a.npairs + 3) / 4)), block(256);
+  hipLaunchKernelGGL((bm_singular_kernel<TO>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename TO>
+hipError_t launch_identity(const IdentityArgs<TO>& a, hipStream_t s) {
+  dim3 grid((unsigned)((a.nf + 255) / 256)), block(256);
+  hipLaunchKernelGGL((bm_identity_kernel<TO>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL((fill_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_downcast(const double* src, float* dst, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(downcast_kernel, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
+  return hipGetLastError();
+}
+
+template hipError_t launch_regular<float>(const RegularArgs<float>&, int, bool, hipStream_t);
+template hipError_t launch_regular<double>(const RegularArgs<double>&, int, bool, hipStream_t);
+template hipError_t launch_singular<float>(const SingularArgs<float>&, hipStream_t);
+template hipError_t launch_singular<double>(const SingularArgs<double>&, hipStream_t);
+template hipError_t launch_identity<float>(const IdentityArgs<float>&, hipStream_t);
+template hipError_t launch_identity<double>(const IdentityArgs<double>&, hipStream_t);
+template hipError_t launch_fill<float>(float*, int64_t, float, hipStream_t);
+template hipError_t launch_fill<double>(double*, int64_t, double, hipStream_t);
+template hipError_t launch_fill<int>(int*, int64_t, int, hipStream_t);
+
+}  // namespace beat

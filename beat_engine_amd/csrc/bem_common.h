@@ -1,0 +1,60 @@
+// Shared definitions for the gfx950 Burton-Miller library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace beat {
+
+// Face tiles: up to TILE_F faces that touch at most TILE_V distinct vertices.
+constexpr int TILE_F = 64;
+constexpr int TILE_V = 64;
+// Per-face geometry record (FREC scalars): regular quadrature points (q*3+c, up
+// to 6 points), unit normal, surface curls (a*3+c), area, plane offset v1.n.
+constexpr int FREC = 32;
+constexpr int REC_N = 18;
+constexpr int REC_CURL = 21;
+constexpr int REC_AREA = 30;
+constexpr int REC_D = 31;
+
+constexpr double INV_FOUR_PI = 0.079577471545947667884441881686257181;
+
+// Singular (touching) pair: internal test/trial face, rule, jacobian and normal product.
+struct SingularPair {
+  int32_t test, trial, rule, pad;
+  double jac, nprod;
+};
+
+// Host-side mesh after preprocessing; all arrays in internal (tile) order
+// except where noted.
+struct HostMesh {
+  int64_t nv = 0, nf = 0;
+  int quad_order = 4, singular_order = 4, nq = 6;
+  std::vector<double> rule_pts;  // nq x 2
+  std::vector<double> rule_w;    // nq
+  std::vector<int32_t> face_perm;  // internal face -> caller face
+  std::vector<int32_t> vert_perm;  // internal vertex -> caller vertex
+  std::vector<int32_t> vert_inv;   // caller vertex -> internal vertex
+  std::vector<int32_t> fverts;     // 3 per internal face: internal vertex ids
+  std::vector<uint8_t> floc;       // 3 per internal face: vertex slot in its tile
+  std::vector<double> rec;         // FREC per internal face
+  std::vector<double> verts;       // 3 per internal vertex
+  std::vector<int32_t> tile_start, tile_count, tile_nv;
+  std::vector<int32_t> tile_verts;  // TILE_V per tile
+  std::vector<SingularPair> spairs;
+  std::vector<int32_t> rule_off, rule_cnt, rule_kind;
+  std::vector<double> rule_data;  // 5 per rule point: test xi, eta, trial xi, eta, weight
+  int64_t n_coincident = 0, n_edge = 0, n_vertex = 0;
+  double id_pp[3][3];  // unit-area P1 x P1 mass block from the rule
+  double id_pd[3];     // unit-area P1 x DP0 block
+};
+
+std::string build_host_mesh(const double* verts, int64_t nv, const int32_t* faces, int64_t nf, int quad_order,
+                            int singular_order, HostMesh& out);
+
+// Reference rules (restated from BeatEngineCore.jl, see mesh_setup.cpp).
+int triangle_rule(int order, double* pts, double* w);
+
+}  // namespace beat

@@ -1,0 +1,114 @@
+// Kernel argument blocks and launchers shared by the HIP sources and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bem_common.h"
+
+namespace beat {
+
+template <typename T>
+struct RegularArgs {
+  const T* rec;            // FREC per face (internal order)
+  const int32_t* fverts;   // 3 per face
+  const uint8_t* floc;     // 3 per face
+  const int32_t* tile_start;
+  const int32_t* tile_count;
+  const int32_t* tile_nv;
+  const int32_t* tile_verts;  // TILE_V per tile
+  int ntiles;
+  T* A;                    // row-major complex, lda
+  int64_t lda;
+  T* B;                    // N x nrhs row-major complex
+  const T* q;              // F x nrhs row-major complex (internal face order)
+  int nrhs;
+  T k, krev;               // signed wavenumber, k / (2 pi)
+  T eta_re, eta_im;
+  T w[6];                  // rule weights
+  T phi[6][3];             // P1 basis at the rule points
+};
+
+template <typename TO>
+struct SingularArgs {
+  const double* rec;
+  const double* verts;     // 3 per internal vertex
+  const int32_t* fverts;
+  const SingularPair* pairs;
+  int64_t npairs;
+  const int32_t* rule_off;
+  const int32_t* rule_cnt;
+  const double* rule_data;
+  TO* A;
+  int64_t lda;
+  TO* B;
+  const double* q;         // F x nrhs row-major complex128
+  int nrhs;
+  double k, eta_re, eta_im;
+};
+
+template <typename TO>
+struct IdentityArgs {
+  const double* rec;
+  const int32_t* fverts;
+  int64_t nf;
+  TO* A;
+  int64_t lda;
+  TO* B;
+  const double* q;
+  int nrhs;
+  double eta_re, eta_im;
+  double pp[3][3];
+  double pd[3];
+};
+
+template <typename T>
+hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure_imag_eta, hipStream_t s);
+template <typename TO>
+hipError_t launch_singular(const SingularArgs<TO>& a, hipStream_t s);
+template <typename TO>
+hipError_t launch_identity(const IdentityArgs<TO>& a, hipStream_t s);
+template <typename T>
+hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s);
+hipError_t launch_downcast(const double* src, float* dst, int64_t n, hipStream_t s);
+
+// ---- dense LU (row-major complex, leading dimension lda) -------------------------
+struct LuWork {
+  int* ipiv;          // n, global row index chosen at each column
+  int* bar;           // grid barrier counter
+  double* slot_val;   // 2 x maxg
+  int* slot_row;      // 2 x maxg
+  void* slot_rows;    // 2 x maxg x nb complex T
+  void* slot_diag;    // 2 x nb complex T
+  int* info;          // first zero pivot (1-based), 0 if none
+  int maxg;
+};
+
+template <typename T>
+hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_t s, int* panels_out);
+
+// Forward/backward solves with the factors of lu_factor; B is n x nrhs row-major
+// complex T and is overwritten by the solution. work holds 2 n nrhs complex T.
+template <typename T>
+hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, int nrhs, T* work, int nb,
+                    hipStream_t s);
+
+// r = b - A x (complex128), A row-major n x n, x and b n x nrhs row-major.
+hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
+                      hipStream_t s);
+size_t lu_solve_work_bytes(int64_t n, int nrhs, int nb, size_t elem);
+// per-column 2-norms of an n x nrhs complex128 row-major matrix into out[nrhs]
+// (out holds nrhs norms followed by 64*nrhs partial sums)
+hipError_t column_norms64(const double* X, int64_t n, int nrhs, double* out, hipStream_t s);
+
+// ---- field -----------------------------------------------------------------------
+template <typename T>
+hipError_t field_eval(const T* src_pts, const T* src_nrm, const T* pw, const T* qw, int64_t nsrc, int nrhs,
+                      const T* pts, int64_t npts, T* out, T k, hipStream_t s);
+// rule: 3 doubles per regular rule point (xi, eta, weight); p is N x nrhs and q
+// F x nrhs (row-major, internal order); outputs are per source (F * nq).
+template <typename T>
+hipError_t field_sources(const double* rec, const int32_t* fverts, int64_t nf, int nq, const double* rule, const T* p,
+                         const T* q, int nrhs, T* src_pts, T* src_nrm, T* pw, T* qw, hipStream_t s);
+
+}  // namespace beat

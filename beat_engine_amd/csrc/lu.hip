@@ -1,0 +1,863 @@
+// Blocked right-looking complex LU with partial pivoting on gfx950, and the
+// multi-RHS triangular solves that use it.
+//
+// Storage: row-major interleaved complex, leading dimension lda. Per panel of nb
+// columns:
+//   panel_kernel     cooperative grid; each workgroup holds a block of panel rows
+//                    in LDS; per column a grid-wide FP64 |re|+|im| pivot search
+//                    (the LAPACK izamax measure), swap, scale and rank-1 update
+//   swap_trsm_kernel row swaps of the trailing columns and U12 = L11^-1 A12
+//   gemm kernels     A22 -= L21 U12 as one real GEMM on MFMA: complex operands
+//                    are expanded to 2x2 real blocks [[br, bi], [-bi, br]]
+//                    while staged (the 4-real-multiply complex product)
+// Columns left of the panel are never swapped; the forward solve applies each
+// panel's swaps to the right-hand sides just before that panel's block, which is
+// equivalent to LAPACK's getrs on fully swapped factors.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "bem_common.h"
+#include "kernels.h"
+
+namespace beat {
+
+namespace {
+
+constexpr int PNB = 64;   // max panel width
+constexpr int RPW = 64;   // panel rows per workgroup
+
+template <typename T>
+struct PanelArgs {
+  T* A;
+  int64_t n, lda, j0;
+  int nb;
+  int* ipiv;
+  int* bar;
+  int bar_base;
+  double* slot_val;
+  int* slot_row;
+  T* slot_rows;
+  T* slot_diag;
+  int maxg;
+  int* info;
+};
+
+__device__ __forceinline__ void st_relaxed(double* p, double v) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __double_as_longlong(v), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_relaxed(const double* p) {
+  return __longlong_as_double(__hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void st_relaxed(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_relaxed(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// complex element store/load through 64-bit atomics (float complex = one word)
+__device__ __forceinline__ void st_cplx(float* p, float re, float im) {
+  unsigned long long v = ((unsigned long long)__float_as_uint(im) << 32) | __float_as_uint(re);
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void ld_cplx(const float* p, float& re, float& im) {
+  unsigned long long v =
+      __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  re = __uint_as_float((unsigned)(v & 0xffffffffu));
+  im = __uint_as_float((unsigned)(v >> 32));
+}
+__device__ __forceinline__ void st_cplx(double* p, double re, double im) {
+  st_relaxed(p, re);
+  st_relaxed(p + 1, im);
+}
+__device__ __forceinline__ void ld_cplx(const double* p, double& re, double& im) {
+  re = ld_relaxed(p);
+  im = ld_relaxed(p + 1);
+}
+
+__device__ __forceinline__ void grid_sync(int* bar, int target) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
+  __shared__ T rows[RPW * PNB * 2];
+  __shared__ T piv[PNB * 2];
+  __shared__ double rv[256];
+  __shared__ int rr[256];
+  __shared__ T linv[RPW * 2];
+  __shared__ int s_p;
+  const int tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
+  const int64_t r0 = a.j0 + (int64_t)g * RPW;
+  const int nr = (int)min((int64_t)RPW, a.n - r0);
+  const int nb = a.nb;
+  for (int idx = tid; idx < nr * nb; idx += blockDim.x) {
+    const int r = idx / nb, c = idx - r * nb;
+    const T* src = a.A + 2 * ((r0 + r) * a.lda + a.j0 + c);
+    rows[2 * (r * PNB + c)] = src[0];
+    rows[2 * (r * PNB + c) + 1] = src[1];
+  }
+  __syncthreads();
+  for (int jj = 0; jj < nb; ++jj) {
+    const int64_t col = a.j0 + jj;
+    const int par = jj & 1;
+    // local pivot candidate: max |re|+|im| in FP64, first index on ties
+    double best = -1.0;
+    int brow = -1;
+    for (int r = tid; r < nr; r += blockDim.x) {
+      if (r0 + r < col) continue;
+      const double v = fabs((double)rows[2 * (r * PNB + jj)]) + fabs((double)rows[2 * (r * PNB + jj) + 1]);
+      if (v > best) {
+        best = v;
+        brow = (int)(r0 + r);
+      }
+    }
+    rv[tid] = best;
+    rr[tid] = brow;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+      if (tid < o) {
+        const double v2 = rv[tid + o];
+        const int r2 = rr[tid + o];
+        if (v2 > rv[tid] || (v2 == rv[tid] && r2 >= 0 && (rr[tid] < 0 || r2 < rr[tid]))) {
+          rv[tid] = v2;
+          rr[tid] = r2;
+        }
+      }
+      __syncthreads();
+    }
+    best = rv[0];
+    brow = rr[0];
+    // publish candidate value, row index and row content; the owner of the
+    // diagonal row publishes that row too
+    if (tid == 0) {
+      st_relaxed(a.slot_val + par * a.maxg + g, best);
+      st_relaxed(a.slot_row + par * a.maxg + g, brow);
+    }
+    if (brow >= 0 && tid < nb) {
+      const int lr = (int)(brow - r0);
+      st_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + g) * PNB + tid), rows[2 * (lr * PNB + tid)],
+              rows[2 * (lr * PNB + tid) + 1]);
+    }
+    const bool own_diag = col >= r0 && col < r0 + nr;
+    if (own_diag && tid < nb) {
+      const int lr = (int)(col - r0);
+      st_cplx(a.slot_diag + 2 * (par * PNB + tid), rows[2 * (lr * PNB + tid)], rows[2 * (lr * PNB + tid) + 1]);
+    }
+    grid_sync(a.bar, a.bar_base + (jj + 1) * G);
+    // choose the global pivot: largest value, smallest row on ties
+    double v = -1.0;
+    int r = -1, wg = -1;
+    for (int i = tid; i < G; i += blockDim.x) {
+      const double vi = ld_relaxed(a.slot_val + par * a.maxg + i);
+      const int ri = ld_relaxed(a.slot_row + par * a.maxg + i);
+      if (ri >= 0 && (vi > v || (vi == v && ri < r))) {
+        v = vi;
+        r = ri;
+        wg = i;
+      }
+    }
+    rv[tid] = v;
+    rr[tid] = r;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+      if (tid < o) {
+        const double v2 = rv[tid + o];
+        const int r2 = rr[tid + o];
+        if (r2 >= 0 && (v2 > rv[tid] || (v2 == rv[tid] && (rr[tid] < 0 || r2 < rr[tid])))) {
+          rv[tid] = v2;
+          rr[tid] = r2;
+        }
+      }
+      __syncthreads();
+    }
+    const int p = rr[0];
+    const int pwg = (int)((p - a.j0) / RPW);
+    (void)wg;
+    if (tid < nb) {
+      T re, im;
+      ld_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + pwg) * PNB + tid), re, im);
+      piv[2 * tid] = re;
+      piv[2 * tid + 1] = im;
+    }
+    if (g == 0 && tid == 0) a.ipiv[col] = p;
+    __syncthreads();
+    if (p != col) {
+      if (p >= r0 && p < r0 + nr && tid < nb) {
+        T re, im;
+        ld_cplx(a.slot_diag + 2 * (par * PNB + tid), re, im);
+        const int lr = (int)(p - r0);
+        rows[2 * (lr * PNB + tid)] = re;
+        rows[2 * (lr * PNB + tid) + 1] = im;
+      }
+      if (own_diag && tid < nb) {
+        const int lr = (int)(col - r0);
+        rows[2 * (lr * PNB + tid)] = piv[2 * tid];
+        rows[2 * (lr * PNB + tid) + 1] = piv[2 * tid + 1];
+      }
+    }
+    __syncthreads();
+    const T pr = piv[2 * jj], pi = piv[2 * jj + 1];
+    if (pr == T(0) && pi == T(0)) {
+      if (g == 0 && tid == 0 && *a.info == 0) *a.info = (int)col + 1;
+      continue;  // LAPACK leaves the column unscaled and goes on
+    }
+    // multipliers l = a / pivot for rows below the diagonal
+    const T den = pr * pr + pi * pi;
+    const T ipr = pr / den, ipi = -pi / den;
+    for (int r2 = tid; r2 < nr; r2 += blockDim.x) {
+      if (r0 + r2 <= col) continue;
+      const T ar = rows[2 * (r2 * PNB + jj)], ai = rows[2 * (r2 * PNB + jj) + 1];
+      const T lr = ar * ipr - ai * ipi, li = ar * ipi + ai * ipr;
+      rows[2 * (r2 * PNB + jj)] = lr;
+      rows[2 * (r2 * PNB + jj) + 1] = li;
+      linv[2 * r2] = lr;
+      linv[2 * r2 + 1] = li;
+    }
+    __syncthreads();
+    const int w = nb - jj - 1;
+    if (w > 0) {
+      for (int idx = tid; idx < nr * w; idx += blockDim.x) {
+        const int r2 = idx / w, c = jj + 1 + (idx - r2 * w);
+        if (r0 + r2 <= col) continue;
+        const T lr = linv[2 * r2], li = linv[2 * r2 + 1];
+        const T ur = piv[2 * c], ui = piv[2 * c + 1];
+        rows[2 * (r2 * PNB + c)] -= lr * ur - li * ui;
+        rows[2 * (r2 * PNB + c) + 1] -= lr * ui + li * ur;
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < nr * nb; idx += blockDim.x) {
+    const int r = idx / nb, c = idx - r * nb;
+    T* dst = a.A + 2 * ((r0 + r) * a.lda + a.j0 + c);
+    dst[0] = rows[2 * (r * PNB + c)];
+    dst[1] = rows[2 * (r * PNB + c) + 1];
+  }
+}
+
+// Row swaps of the trailing columns followed by U12 = L11^-1 A12; one thread per
+// column, 64 columns per workgroup.
+template <typename T>
+__global__ __launch_bounds__(64) void swap_trsm_kernel(T* A, int64_t n, int64_t lda, int64_t j0, int nb,
+                                                       const int* ipiv) {
+  __shared__ T L[PNB * PNB * 2];
+  __shared__ T X[PNB * 64 * 2];
+  const int tid = threadIdx.x;
+  const int64_t c0 = j0 + nb + (int64_t)blockIdx.x * 64;
+  const int64_t cc = c0 + tid;
+  const bool act = cc < n;
+  for (int idx = tid; idx < nb * nb; idx += 64) {
+    const int i = idx / nb, k = idx - i * nb;
+    L[2 * (i * PNB + k)] = A[2 * ((j0 + i) * lda + j0 + k)];
+    L[2 * (i * PNB + k) + 1] = A[2 * ((j0 + i) * lda + j0 + k) + 1];
+  }
+  if (act)
+    for (int i = 0; i < nb; ++i) {
+      X[2 * (i * 64 + tid)] = A[2 * ((j0 + i) * lda + cc)];
+      X[2 * (i * 64 + tid) + 1] = A[2 * ((j0 + i) * lda + cc) + 1];
+    }
+  __syncthreads();
+  if (!act) return;
+  for (int i = 0; i < nb; ++i) {
+    const int64_t p = ipiv[j0 + i];
+    if (p == j0 + i) continue;
+    const T xr = X[2 * (i * 64 + tid)], xi = X[2 * (i * 64 + tid) + 1];
+    if (p < j0 + nb) {
+      const int pi = (int)(p - j0);
+      X[2 * (i * 64 + tid)] = X[2 * (pi * 64 + tid)];
+      X[2 * (i * 64 + tid) + 1] = X[2 * (pi * 64 + tid) + 1];
+      X[2 * (pi * 64 + tid)] = xr;
+      X[2 * (pi * 64 + tid) + 1] = xi;
+    } else {
+      T* g = A + 2 * (p * lda + cc);
+      X[2 * (i * 64 + tid)] = g[0];
+      X[2 * (i * 64 + tid) + 1] = g[1];
+      g[0] = xr;
+      g[1] = xi;
+    }
+  }
+  for (int i = 1; i < nb; ++i) {
+    T sr = X[2 * (i * 64 + tid)], si = X[2 * (i * 64 + tid) + 1];
+    for (int k = 0; k < i; ++k) {
+      const T lr = L[2 * (i * PNB + k)], li = L[2 * (i * PNB + k) + 1];
+      const T xr = X[2 * (k * 64 + tid)], xi = X[2 * (k * 64 + tid) + 1];
+      sr -= lr * xr - li * xi;
+      si -= lr * xi + li * xr;
+    }
+    X[2 * (i * 64 + tid)] = sr;
+    X[2 * (i * 64 + tid) + 1] = si;
+  }
+  for (int i = 0; i < nb; ++i) {
+    A[2 * ((j0 + i) * lda + cc)] = X[2 * (i * 64 + tid)];
+    A[2 * ((j0 + i) * lda + cc) + 1] = X[2 * (i * 64 + tid) + 1];
+  }
+}
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// C[m x n] -= A[m x kc] B[kc x n], complex64, row-major (lda/ldb/ldc in complex
+// elements). Workgroup tile 128 rows x 64 complex columns, 4 waves of 64 x 64
+// real outputs, v_mfma_f32_32x32x2_f32 (one complex k per instruction).
+__global__ __launch_bounds__(256) void cgemm_sub_f32(const float* __restrict__ A, int64_t lda,
+                                                     const float* __restrict__ B, int64_t ldb, float* C, int64_t ldc,
+                                                     int64_t m, int64_t n, int kc) {
+  constexpr int BM = 128, BN = 64, KB = 16;
+  __shared__ float As[BM * KB * 2];
+  __shared__ float Bs[KB * BN * 2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t row0 = (int64_t)blockIdx.y * BM, col0 = (int64_t)blockIdx.x * BN;
+  f32x16 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+  const int h = lane >> 5, l32 = lane & 31;
+  for (int k0 = 0; k0 < kc; k0 += KB) {
+    for (int idx = tid; idx < BM * KB; idx += 256) {
+      const int r = idx / KB, kk = idx - r * KB;
+      const int64_t gr = row0 + r;
+      float re = 0.f, im = 0.f;
+      if (gr < m && k0 + kk < kc) {
+        re = A[2 * (gr * lda + k0 + kk)];
+        im = A[2 * (gr * lda + k0 + kk) + 1];
+      }
+      As[2 * (r * KB + kk)] = re;
+      As[2 * (r * KB + kk) + 1] = im;
+    }
+    for (int idx = tid; idx < KB * BN; idx += 256) {
+      const int kk = idx / BN, c = idx - kk * BN;
+      const int64_t gc = col0 + c;
+      float re = 0.f, im = 0.f;
+      if (gc < n && k0 + kk < kc) {
+        re = B[2 * ((int64_t)(k0 + kk) * ldb + gc)];
+        im = B[2 * ((int64_t)(k0 + kk) * ldb + gc) + 1];
+      }
+      Bs[2 * (kk * BN + c)] = re;
+      Bs[2 * (kk * BN + c) + 1] = im;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < KB; ++kk) {
+      float av[2], bv[2];
+      for (int mi = 0; mi < 2; ++mi) {
+        const int r = wm * 64 + mi * 32 + l32;
+        av[mi] = As[2 * (r * KB + kk) + h];
+      }
+      for (int ni = 0; ni < 2; ++ni) {
+        const int j = wn * 64 + ni * 32 + l32;  // real column in the tile
+        const int jc = j >> 1, part = j & 1;
+        const float br = Bs[2 * (kk * BN + jc)], bi = Bs[2 * (kk * BN + jc) + 1];
+        bv[ni] = h == 0 ? (part == 0 ? br : bi) : (part == 0 ? -bi : br);
+      }
+      for (int mi = 0; mi < 2; ++mi)
+        for (int ni = 0; ni < 2; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int mi = 0; mi < 2; ++mi)
+    for (int ni = 0; ni < 2; ++ni)
+      for (int v = 0; v < 16; ++v) {
+        const int r = wm * 64 + mi * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+        const int j = wn * 64 + ni * 32 + l32;
+        const int64_t gr = row0 + r, gcr = 2 * col0 + j;  // real column
+        if (gr < m && gcr < 2 * n) C[gr * 2 * ldc + gcr] -= acc[mi][ni][v];
+      }
+}
+
+// complex128 variant: tile 64 rows x 32 complex columns, 4 waves of 32 x 32 real
+// outputs, v_mfma_f64_16x16x4_f64 (two complex k per instruction).
+__global__ __launch_bounds__(256) void zgemm_sub_f64(const double* __restrict__ A, int64_t lda,
+                                                     const double* __restrict__ B, int64_t ldb, double* C,
+                                                     int64_t ldc, int64_t m, int64_t n, int kc) {
+  constexpr int BM = 64, BN = 32, KB = 16;
+  __shared__ double As[BM * KB * 2];
+  __shared__ double Bs[KB * BN * 2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int64_t row0 = (int64_t)blockIdx.y * BM, col0 = (int64_t)blockIdx.x * BN;
+  f64x4 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
+  const int kq = lane >> 4, l16 = lane & 15;
+  for (int k0 = 0; k0 < kc; k0 += KB) {
+    for (int idx = tid; idx < BM * KB; idx += 256) {
+      const int r = idx / KB, kk = idx - r * KB;
+      const int64_t gr = row0 + r;
+      double re = 0.0, im = 0.0;
+      if (gr < m && k0 + kk < kc) {
+        re = A[2 * (gr * lda + k0 + kk)];
+        im = A[2 * (gr * lda + k0 + kk) + 1];
+      }
+      As[2 * (r * KB + kk)] = re;
+      As[2 * (r * KB + kk) + 1] = im;
+    }
+    for (int idx = tid; idx < KB * BN; idx += 256) {
+      const int kk = idx / BN, c = idx - kk * BN;
+      const int64_t gc = col0 + c;
+      double re = 0.0, im = 0.0;
+      if (gc < n && k0 + kk < kc) {
+        re = B[2 * ((int64_t)(k0 + kk) * ldb + gc)];
+        im = B[2 * ((int64_t)(k0 + kk) * ldb + gc) + 1];
+      }
+      Bs[2 * (kk * BN + c)] = re;
+      Bs[2 * (kk * BN + c) + 1] = im;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int kk = 0; kk < KB; kk += 2) {
+      // real k index 2*kk + kq: complex k = kk + kq/2, part kq%2
+      const int kcx = kk + (kq >> 1), hp = kq & 1;
+      double av[2], bv[2];
+      for (int mi = 0; mi < 2; ++mi) {
+        const int r = wm * 32 + mi * 16 + l16;
+        av[mi] = As[2 * (r * KB + kcx) + hp];
+      }
+      for (int ni = 0; ni < 2; ++ni) {
+        const int j = wn * 32 + ni * 16 + l16;
+        const int jc = j >> 1, part = j & 1;
+        const double br = Bs[2 * (kcx * BN + jc)], bi = Bs[2 * (kcx * BN + jc) + 1];
+        bv[ni] = hp == 0 ? (part == 0 ? br : bi) : (part == 0 ? -bi : br);
+      }
+      for (int mi = 0; mi < 2; ++mi)
+        for (int ni = 0; ni < 2; ++ni)
+          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int mi = 0; mi < 2; ++mi)
+    for (int ni = 0; ni < 2; ++ni)
+      for (int v = 0; v < 4; ++v) {
+        const int r = wm * 32 + mi * 16 + 4 * kq + v;
+        const int j = wn * 32 + ni * 16 + l16;
+        const int64_t gr = row0 + r, gcr = 2 * col0 + j;
+        if (gr < m && gcr < 2 * n) C[gr * 2 * ldc + gcr] -= acc[mi][ni][v];
+      }
+}
+
+template <typename T>
+hipError_t gemm_sub(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
+                    int kc, hipStream_t s);
+
+template <>
+hipError_t gemm_sub<float>(const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
+                           int64_t m, int64_t n, int kc, hipStream_t s) {
+  if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
+  dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 127) / 128));
+  hipLaunchKernelGGL(cgemm_sub_f32, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
+  return hipGetLastError();
+}
+
+template <>
+hipError_t gemm_sub<double>(const double* A, int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
+                            int64_t m, int64_t n, int kc, hipStream_t s) {
+  if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
+  dim3 grid((unsigned)((n + 31) / 32), (unsigned)((m + 63) / 64));
+  hipLaunchKernelGGL(zgemm_sub_f64, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
+  return hipGetLastError();
+}
+
+// ---- solves ----------------------------------------------------------------------
+
+// Inverses of the unit-lower and upper diagonal blocks of every panel.
+template <typename T>
+__global__ __launch_bounds__(64) void diag_inverse_kernel(const T* LU, int64_t n, int64_t lda, int nbmax, T* Linv,
+                                                          T* Uinv) {
+  __shared__ T D[PNB * PNB * 2];
+  const int k = blockIdx.x, tid = threadIdx.x;
+  const int64_t j0 = (int64_t)k * nbmax;
+  const int nb = (int)min((int64_t)nbmax, n - j0);
+  for (int idx = tid; idx < nb * nb; idx += 64) {
+    const int i = idx / nb, c = idx - i * nb;
+    D[2 * (i * PNB + c)] = LU[2 * ((j0 + i) * lda + j0 + c)];
+    D[2 * (i * PNB + c) + 1] = LU[2 * ((j0 + i) * lda + j0 + c) + 1];
+  }
+  __syncthreads();
+  T* Li = Linv + 2 * (int64_t)k * PNB * PNB;
+  T* Ui = Uinv + 2 * (int64_t)k * PNB * PNB;
+  if (tid < nb) {
+    const int c = tid;
+    // unit lower: L X = I, column c
+    for (int i = 0; i < nb; ++i) {
+      T sr = (i == c) ? T(1) : T(0), si = 0;
+      for (int q = 0; q < i; ++q) {
+        const T lr = D[2 * (i * PNB + q)], li = D[2 * (i * PNB + q) + 1];
+        const T xr = Li[2 * (q * PNB + c)], xi = Li[2 * (q * PNB + c) + 1];
+        sr -= lr * xr - li * xi;
+        si -= lr * xi + li * xr;
+      }
+      Li[2 * (i * PNB + c)] = sr;
+      Li[2 * (i * PNB + c) + 1] = si;
+    }
+    // upper: U X = I, column c
+    for (int i = nb - 1; i >= 0; --i) {
+      T sr = (i == c) ? T(1) : T(0), si = 0;
+      for (int q = i + 1; q < nb; ++q) {
+        const T ur = D[2 * (i * PNB + q)], ui = D[2 * (i * PNB + q) + 1];
+        const T xr = Ui[2 * (q * PNB + c)], xi = Ui[2 * (q * PNB + c) + 1];
+        sr -= ur * xr - ui * xi;
+        si -= ur * xi + ui * xr;
+      }
+      const T dr = D[2 * (i * PNB + i)], di = D[2 * (i * PNB + i) + 1];
+      const T den = dr * dr + di * di;
+      Ui[2 * (i * PNB + c)] = (sr * dr + si * di) / den;
+      Ui[2 * (i * PNB + c) + 1] = (si * dr - sr * di) / den;
+    }
+  }
+}
+
+// For panel k: the rows whose content the panel's swaps move, with the row each
+// one takes its content from (top rows first, then displaced rows below).
+__global__ void swap_lists_kernel(const int* ipiv, int64_t n, int nbmax, int npanels, int* rows, int* src, int* cnt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npanels) return;
+  const int64_t j0 = (int64_t)k * nbmax;
+  const int nb = (int)min((int64_t)nbmax, n - j0);
+  int* R = rows + (int64_t)k * 2 * PNB;
+  int* S = src + (int64_t)k * 2 * PNB;
+  int m = 0;
+  for (int i = 0; i < nb; ++i) {
+    R[m] = (int)(j0 + i);
+    S[m] = (int)(j0 + i);
+    ++m;
+  }
+  for (int i = 0; i < nb; ++i) {
+    const int a = (int)(j0 + i), b = ipiv[j0 + i];
+    if (a == b) continue;
+    int ia = -1, ib = -1;
+    for (int t = 0; t < m; ++t) {
+      if (R[t] == a) ia = t;
+      if (R[t] == b) ib = t;
+    }
+    if (ib < 0) {
+      R[m] = b;
+      S[m] = b;
+      ib = m++;
+    }
+    const int tmp = S[ia];
+    S[ia] = S[ib];
+    S[ib] = tmp;
+  }
+  cnt[k] = m;
+}
+
+// Forward step of panel k: y = Linv_k (P_k b)_top; rows below: b - L21 y.
+// Reads bin, writes bout (rows >= j0); y is written to both buffers.
+template <typename T>
+__global__ __launch_bounds__(256) void fwd_step_kernel(const T* LU, int64_t n, int64_t lda, int64_t j0, int nb,
+                                                       const T* Linv, const int* lrows, const int* lsrc, int lcnt,
+                                                       const T* bin, T* bout, T* bkeep, int nrhs) {
+  __shared__ T top[PNB * 8 * 2];
+  __shared__ T y[PNB * 8 * 2];
+  __shared__ int srows[2 * PNB], ssrc[2 * PNB];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < lcnt; i += blockDim.x) {
+    srows[i] = lrows[i];
+    ssrc[i] = lsrc[i];
+  }
+  __syncthreads();
+  for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
+    const int i = idx / nrhs, d = idx - i * nrhs;
+    const int64_t sr = ssrc[i];  // top rows come first in the list
+    top[2 * (i * 8 + d)] = bin[2 * (sr * nrhs + d)];
+    top[2 * (i * 8 + d) + 1] = bin[2 * (sr * nrhs + d) + 1];
+  }
+  __syncthreads();
+  for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
+    const int i = idx / nrhs, d = idx - i * nrhs;
+    T sr = 0, si = 0;
+    for (int q = 0; q < nb; ++q) {
+      const T lr = Linv[2 * (i * PNB + q)], li = Linv[2 * (i * PNB + q) + 1];
+      const T br = top[2 * (q * 8 + d)], bi = top[2 * (q * 8 + d) + 1];
+      sr += lr * br - li * bi;
+      si += lr * bi + li * br;
+    }
+    y[2 * (i * 8 + d)] = sr;
+    y[2 * (i * 8 + d) + 1] = si;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0)
+    for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
+      const int i = idx / nrhs, d = idx - i * nrhs;
+      const int64_t r = j0 + i;
+      bout[2 * (r * nrhs + d)] = y[2 * (i * 8 + d)];
+      bout[2 * (r * nrhs + d) + 1] = y[2 * (i * 8 + d) + 1];
+      bkeep[2 * (r * nrhs + d)] = y[2 * (i * 8 + d)];
+      bkeep[2 * (r * nrhs + d) + 1] = y[2 * (i * 8 + d) + 1];
+    }
+  const int64_t rbase = j0 + nb + (int64_t)blockIdx.x * (256 / 8) * 4;
+  for (int idx = tid; idx < 128 * nrhs; idx += blockDim.x) {
+    const int lr = idx / nrhs, d = idx - lr * nrhs;
+    const int64_t r = rbase + lr;
+    if (r >= n) continue;
+    int64_t sr = r;
+    for (int t = nb; t < lcnt; ++t)
+      if (srows[t] == r) sr = ssrc[t];
+    T vr = bin[2 * (sr * nrhs + d)], vi = bin[2 * (sr * nrhs + d) + 1];
+    const T* L = LU + 2 * (r * lda + j0);
+    for (int q = 0; q < nb; ++q) {
+      const T lr2 = L[2 * q], li2 = L[2 * q + 1];
+      const T yr = y[2 * (q * 8 + d)], yi = y[2 * (q * 8 + d) + 1];
+      vr -= lr2 * yr - li2 * yi;
+      vi -= lr2 * yi + li2 * yr;
+    }
+    bout[2 * (r * nrhs + d)] = vr;
+    bout[2 * (r * nrhs + d) + 1] = vi;
+  }
+}
+
+// Backward step of panel k: x_k = Uinv_k b_k; rows above: b -= U x_k (in place).
+template <typename T>
+__global__ __launch_bounds__(256) void bwd_step_kernel(const T* LU, int64_t lda, int64_t j0, int nb, const T* Uinv,
+                                                       T* b, T* x, int nrhs) {
+  __shared__ T xs[PNB * 8 * 2];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
+    const int i = idx / nrhs, d = idx - i * nrhs;
+    T sr = 0, si = 0;
+    for (int q = 0; q < nb; ++q) {
+      const T ur = Uinv[2 * (i * PNB + q)], ui = Uinv[2 * (i * PNB + q) + 1];
+      const T br = b[2 * ((j0 + q) * nrhs + d)], bi = b[2 * ((j0 + q) * nrhs + d) + 1];
+      sr += ur * br - ui * bi;
+      si += ur * bi + ui * br;
+    }
+    xs[2 * (i * 8 + d)] = sr;
+    xs[2 * (i * 8 + d) + 1] = si;
+  }
+  __syncthreads();
+  if (blockIdx.x == 0)
+    for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
+      const int i = idx / nrhs, d = idx - i * nrhs;
+      x[2 * ((j0 + i) * nrhs + d)] = xs[2 * (i * 8 + d)];
+      x[2 * ((j0 + i) * nrhs + d) + 1] = xs[2 * (i * 8 + d) + 1];
+    }
+  const int64_t rbase = (int64_t)blockIdx.x * 128;
+  for (int idx = tid; idx < 128 * nrhs; idx += blockDim.x) {
+    const int lr = idx / nrhs, d = idx - lr * nrhs;
+    const int64_t r = rbase + lr;
+    if (r >= j0) continue;
+    T vr = b[2 * (r * nrhs + d)], vi = b[2 * (r * nrhs + d) + 1];
+    const T* U = LU + 2 * (r * lda + j0);
+    for (int q = 0; q < nb; ++q) {
+      const T ur = U[2 * q], ui = U[2 * q + 1];
+      const T xr = xs[2 * (q * 8 + d)], xi = xs[2 * (q * 8 + d) + 1];
+      vr -= ur * xr - ui * xi;
+      vi -= ur * xi + ui * xr;
+    }
+    b[2 * (r * nrhs + d)] = vr;
+    b[2 * (r * nrhs + d) + 1] = vi;
+  }
+}
+
+template <typename T>
+__global__ void copy_kernel(const T* src, T* dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+// r = b - A x, complex128; 32 rows x up to 8 right-hand sides per workgroup,
+// x staged through LDS in chunks of 64 rows.
+__global__ __launch_bounds__(256) void residual64_kernel(const double* A, int64_t n, int64_t lda, const double* x,
+                                                         const double* b, double* r, int nrhs) {
+  __shared__ double xs[64 * 8 * 2];
+  const int tid = threadIdx.x;
+  const int rl = tid >> 3, d = tid & 7;
+  const int64_t row = (int64_t)blockIdx.x * 32 + rl;
+  const bool act = row < n && d < nrhs;
+  double sr = 0, si = 0;
+  for (int64_t k0 = 0; k0 < n; k0 += 64) {
+    const int kc = (int)min((int64_t)64, n - k0);
+    __syncthreads();
+    for (int idx = tid; idx < kc * nrhs; idx += 256) {
+      const int kk = idx / nrhs, dd = idx - kk * nrhs;
+      xs[2 * (kk * 8 + dd)] = x[2 * ((k0 + kk) * nrhs + dd)];
+      xs[2 * (kk * 8 + dd) + 1] = x[2 * ((k0 + kk) * nrhs + dd) + 1];
+    }
+    __syncthreads();
+    if (act) {
+      const double* a = A + 2 * (row * lda + k0);
+      for (int kk = 0; kk < kc; ++kk) {
+        const double ar = a[2 * kk], ai = a[2 * kk + 1];
+        const double xr = xs[2 * (kk * 8 + d)], xi = xs[2 * (kk * 8 + d) + 1];
+        sr = fma(ar, xr, sr);
+        sr = fma(-ai, xi, sr);
+        si = fma(ar, xi, si);
+        si = fma(ai, xr, si);
+      }
+    }
+  }
+  if (act) {
+    r[2 * (row * nrhs + d)] = b[2 * (row * nrhs + d)] - sr;
+    r[2 * (row * nrhs + d) + 1] = b[2 * (row * nrhs + d) + 1] - si;
+  }
+}
+
+__global__ __launch_bounds__(256) void colnorm_partial_kernel(const double* X, int64_t n, int nrhs, double* part) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x;
+  for (int d = 0; d < nrhs; ++d) {
+    double s = 0;
+    for (int64_t r = (int64_t)blockIdx.x * 256 + tid; r < n; r += (int64_t)gridDim.x * 256) {
+      const double a = X[2 * (r * nrhs + d)], b = X[2 * (r * nrhs + d) + 1];
+      s += a * a + b * b;
+    }
+    red[tid] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (tid < o) red[tid] += red[tid + o];
+      __syncthreads();
+    }
+    if (tid == 0) part[(int64_t)blockIdx.x * nrhs + d] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void colnorm_final_kernel(const double* part, int nparts, int nrhs, double* out) {
+  const int d = threadIdx.x;
+  if (d >= nrhs) return;
+  double s = 0;
+  for (int p = 0; p < nparts; ++p) s += part[(int64_t)p * nrhs + d];
+  out[d] = sqrt(s);
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_t s, int* panels_out) {
+  if (nb > PNB) nb = PNB;
+  hipError_t e;
+  if ((e = launch_fill<int>(w.bar, 1, 0, s)) != hipSuccess) return e;
+  if ((e = launch_fill<int>(w.info, 1, 0, s)) != hipSuccess) return e;
+  int bar_base = 0, panels = 0;
+  for (int64_t j0 = 0; j0 < n; j0 += nb) {
+    const int b = (int)std::min<int64_t>(nb, n - j0);
+    const int G = (int)((n - j0 + RPW - 1) / RPW);
+    if (G > w.maxg) return hipErrorInvalidValue;
+    PanelArgs<T> pa;
+    pa.A = A;
+    pa.n = n;
+    pa.lda = lda;
+    pa.j0 = j0;
+    pa.nb = b;
+    pa.ipiv = w.ipiv;
+    pa.bar = w.bar;
+    pa.bar_base = bar_base;
+    pa.slot_val = w.slot_val;
+    pa.slot_row = w.slot_row;
+    pa.slot_rows = reinterpret_cast<T*>(w.slot_rows);
+    pa.slot_diag = reinterpret_cast<T*>(w.slot_diag);
+    pa.maxg = w.maxg;
+    pa.info = w.info;
+    void* args[] = {&pa};
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G), dim3(256), args, 0, s);
+    if (e != hipSuccess) return e;
+    bar_base += G * b;
+    const int64_t rest = n - j0 - b;
+    if (rest > 0) {
+      hipLaunchKernelGGL((swap_trsm_kernel<T>), dim3((unsigned)((rest + 63) / 64)), dim3(64), 0, s, A, n, lda, j0, b,
+                         (const int*)w.ipiv);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      e = gemm_sub<T>(A + 2 * ((j0 + b) * lda + j0), lda, A + 2 * (j0 * lda + j0 + b), lda,
+                      A + 2 * ((j0 + b) * lda + j0 + b), lda, rest, rest, b, s);
+      if (e != hipSuccess) return e;
+    }
+    ++panels;
+  }
+  if (panels_out) *panels_out = panels;
+  return hipSuccess;
+}
+
+// Solve workspace layout (complex T units): Linv/Uinv npanels*PNB*PNB each,
+// two n x nrhs buffers, swap lists in an int area appended at the end.
+template <typename T>
+hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, int nrhs, T* work, int nb,
+                    hipStream_t s) {
+  if (nb > PNB) nb = PNB;
+  if (nrhs > 8) return hipErrorInvalidValue;
+  const int np = (int)((n + nb - 1) / nb);
+  T* Linv = work;
+  T* Uinv = Linv + 2 * (int64_t)np * PNB * PNB;
+  T* b0 = Uinv + 2 * (int64_t)np * PNB * PNB;
+  T* b1 = b0 + 2 * n * nrhs;
+  int* lrows = reinterpret_cast<int*>(b1 + 2 * n * nrhs);
+  int* lsrc = lrows + (int64_t)np * 2 * PNB;
+  int* lcnt = lsrc + (int64_t)np * 2 * PNB;
+  hipError_t e;
+  hipLaunchKernelGGL((diag_inverse_kernel<T>), dim3(np), dim3(64), 0, s, LU, n, lda, nb, Linv, Uinv);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(swap_lists_kernel, dim3((np + 63) / 64), dim3(64), 0, s, ipiv, n, nb, np, lrows, lsrc, lcnt);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  std::vector<int> hcnt(np);
+  if ((e = hipMemcpyAsync(hcnt.data(), lcnt, sizeof(int) * np, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  hipLaunchKernelGGL((copy_kernel<T>), dim3(1024), dim3(256), 0, s, B, b0, 2 * n * nrhs);
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  T* bin = b0;
+  T* bout = b1;
+  T* keep = B;  // finished rows are written to B directly
+  for (int k = 0; k < np; ++k) {
+    const int64_t j0 = (int64_t)k * nb;
+    const int b = (int)std::min<int64_t>(nb, n - j0);
+    const int64_t below = n - j0 - b;
+    const unsigned G = (unsigned)std::max<int64_t>(1, (below + 127) / 128);
+    hipLaunchKernelGGL((fwd_step_kernel<T>), dim3(G), dim3(256), 0, s, LU, n, lda, j0, b,
+                       Linv + 2 * (int64_t)k * PNB * PNB, lrows + (int64_t)k * 2 * PNB, lsrc + (int64_t)k * 2 * PNB,
+                       hcnt[k], (const T*)bin, bout, keep, nrhs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    std::swap(bin, bout);
+  }
+  // forward result: rows finished at their panel live in B; B is now y.
+  // backward: b := y in a scratch buffer, x written to B
+  hipLaunchKernelGGL((copy_kernel<T>), dim3(1024), dim3(256), 0, s, (const T*)B, b0, 2 * n * nrhs);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  for (int k = np - 1; k >= 0; --k) {
+    const int64_t j0 = (int64_t)k * nb;
+    const int b = (int)std::min<int64_t>(nb, n - j0);
+    const unsigned G = (unsigned)std::max<int64_t>(1, (j0 + 127) / 128);
+    hipLaunchKernelGGL((bwd_step_kernel<T>), dim3(G), dim3(256), 0, s, LU, lda, j0, b,
+                       Uinv + 2 * (int64_t)k * PNB * PNB, b0, B, nrhs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+size_t lu_solve_work_bytes(int64_t n, int nrhs, int nb, size_t elem) {
+  const int np = (int)((n + nb - 1) / nb);
+  return elem * 2 * ((size_t)np * PNB * PNB * 2 + (size_t)n * nrhs * 2) + sizeof(int) * ((size_t)np * 4 * PNB + np) + 256;
+}
+
+hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
+                      hipStream_t s) {
+  if (nrhs > 8) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(residual64_kernel, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, s, A, n, lda, x, b, r, nrhs);
+  return hipGetLastError();
+}
+
+hipError_t column_norms64(const double* X, int64_t n, int nrhs, double* out, hipStream_t s) {
+  // out must hold nrhs + 64*nrhs doubles: norms first, then the partial sums
+  const int parts = 64;
+  double* part = out + nrhs;
+  hipLaunchKernelGGL(colnorm_partial_kernel, dim3(parts), dim3(256), 0, s, X, n, nrhs, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(colnorm_final_kernel, dim3(1), dim3(64), 0, s, (const double*)part, parts, nrhs, out);
+  return hipGetLastError();
+}
+
+template hipError_t lu_factor<float>(float*, int64_t, int64_t, LuWork&, int, hipStream_t, int*);
+template hipError_t lu_factor<double>(double*, int64_t, int64_t, LuWork&, int, hipStream_t, int*);
+template hipError_t lu_solve<float>(const float*, int64_t, int64_t, const int*, float*, int, float*, int, hipStream_t);
+template hipError_t lu_solve<double>(const double*, int64_t, int64_t, const int*, double*, int, double*, int,
+                                     hipStream_t);
+
+}  // namespace beat

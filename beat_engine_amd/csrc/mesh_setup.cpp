@@ -1,0 +1,408 @@
+// Per-mesh preprocessing on the host, done once per mesh and reused across
+// frequencies: face and vertex reordering along a Morton curve, face tiles for
+// the assembly kernel, per-face geometry records, the touching-pair list and
+// the Duffy rules for it, and the unit-area identity blocks.
+//
+// The quadrature constants and Duffy maps follow the reference's definitions
+// (src/beat_engine/julia_local/src/BeatEngineCore.jl): triangle_rule :660-692,
+// gauss_rule_1d :713-740, duffy_rule :742-802, remap_shared_* :804-826,
+// adjacency_info :598-622, build_singular_correction_cache :1049-1089,
+// surface_gradients/curls :856-873, l2_identity_element_matrix :905-923.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <set>
+
+#include "bem_common.h"
+
+namespace beat {
+
+namespace {
+
+struct P2 {
+  double x, y;
+};
+
+// 6-point degree-4 rule with the reference's truncated literals (weights x 0.5).
+void rule6(double* p, double* w) {
+  const double pts[12] = {0.4459484909159651, 0.4459484909159651, 0.0915762135097710, 0.0915762135097700,
+                          0.1081030181680700, 0.4459484909159651, 0.4459484909159651, 0.1081030181680700,
+                          0.8168475729804590, 0.0915762135097700, 0.0915762135097710, 0.8168475729804580};
+  const double ww[6] = {0.2233815896780110, 0.1099517436553220, 0.2233815896780110,
+                        0.2233815896780110, 0.1099517436553220, 0.1099517436553220};
+  for (int i = 0; i < 12; ++i) p[i] = pts[i];
+  for (int i = 0; i < 6; ++i) w[i] = 0.5 * ww[i];
+}
+
+void gauss01(int order, std::vector<double>& x, std::vector<double>& w) {
+  if (order == 1) {
+    x = {0.5};
+    w = {1.0};
+  } else if (order == 2) {
+    double a = 0.5 / std::sqrt(3.0);
+    x = {0.5 - a, 0.5 + a};
+    w = {0.5, 0.5};
+  } else if (order == 3) {
+    double a = std::sqrt(3.0 / 5.0) / 2.0;
+    x = {0.5 - a, 0.5, 0.5 + a};
+    w = {5.0 / 18.0, 4.0 / 9.0, 5.0 / 18.0};
+  } else {
+    double s = std::sqrt(6.0 / 5.0);
+    double x1 = std::sqrt(3.0 / 7.0 - 2.0 / 7.0 * s) / 2.0;
+    double x2 = std::sqrt(3.0 / 7.0 + 2.0 / 7.0 * s) / 2.0;
+    double w1 = (18.0 + std::sqrt(30.0)) / 72.0;
+    double w2 = (18.0 - std::sqrt(30.0)) / 72.0;
+    x = {0.5 - x2, 0.5 - x1, 0.5 + x1, 0.5 + x2};
+    w = {w2, w1, w1, w2};
+  }
+}
+
+struct Rule {
+  std::vector<P2> tp, sp;
+  std::vector<double> w;
+  void add(double tx, double ty, double sx, double sy, double wt) {
+    // points are stored as (x - y, y) of the sub-map coordinates
+    tp.push_back({tx - ty, ty});
+    sp.push_back({sx - sy, sy});
+    w.push_back(wt);
+  }
+};
+
+// Sauter-Schwab style sub-maps: kind 1 coincident (6), 2 edge (5), 3 vertex (2).
+Rule duffy(int order, int kind) {
+  std::vector<double> g, gw;
+  gauss01(order, g, gw);
+  const size_t n = g.size();
+  std::vector<P2> tens;
+  std::vector<double> tw;
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = 0; j < n; ++j) {
+      tens.push_back({g[j], g[i]});
+      tw.push_back(gw[i] * gw[j]);
+    }
+  Rule r;
+  for (size_t a = 0; a < tens.size(); ++a)
+    for (size_t b = 0; b < tens.size(); ++b) {
+      const double s = tens[a].x, e1 = tens[a].y, e2 = tens[b].x, e3 = tens[b].y;
+      const double e12 = e1 * e2, e123 = e12 * e3, base = tw[a] * tw[b];
+      if (kind == 1) {
+        const double wt = base * (s * s * s) * (e1 * e1) * e2;
+        r.add(s, s * (1.0 - e1 + e12), s * (1.0 - e123), s * (1.0 - e1), wt);
+        r.add(s * (1.0 - e123), s * (1.0 - e1), s, s * (1.0 - e1 + e12), wt);
+        r.add(s, s * (e1 - e12 + e123), s * (1.0 - e12), s * (e1 - e12), wt);
+        r.add(s * (1.0 - e12), s * (e1 - e12), s, s * (e1 - e12 + e123), wt);
+        r.add(s * (1.0 - e123), s * (e1 - e123), s, s * (e1 - e12), wt);
+        r.add(s, s * (e1 - e12), s * (1.0 - e123), s * (e1 - e123), wt);
+      } else if (kind == 2) {
+        const double wt = base * (s * s * s) * (e1 * e1);
+        r.add(s, s * e1 * e3, s * (1.0 - e12), s * e1 * (1.0 - e2), wt);
+        r.add(s, s * e1, s * (1.0 - e123), s * e1 * e2 * (1.0 - e3), wt * e2);
+        r.add(s * (1.0 - e12), s * e1 * (1.0 - e2), s, s * e123, wt * e2);
+        r.add(s * (1.0 - e123), s * e12 * (1.0 - e3), s, s * e1, wt * e2);
+        r.add(s * (1.0 - e123), s * e1 * (1.0 - e2 * e3), s, s * e12, wt * e2);
+      } else {
+        const double wt = base * (s * s * s) * e2;
+        r.add(s, s * e1, s * e2, s * e2 * e3, wt);
+        r.add(s * e2, s * e2 * e3, s, s * e1, wt);
+      }
+    }
+  return r;
+}
+
+P2 remap_vertex(P2 p, int v) {  // v: 1-based shared vertex
+  if (v == 1) return p;
+  if (v == 2) return {1.0 - p.x - p.y, p.y};
+  return {p.x, 1.0 - p.x - p.y};
+}
+
+P2 remap_edge(P2 p, int a, int b) {  // a, b: 1-based shared vertices
+  const P2 ref[3] = {{0, 0}, {1, 0}, {0, 1}};
+  const int c = 6 - a - b;
+  const P2 v0 = ref[a - 1], v1 = ref[b - 1], v2 = ref[c - 1];
+  return {v0.x + p.x * (v1.x - v0.x) + p.y * (v2.x - v0.x), v0.y + p.x * (v1.y - v0.y) + p.y * (v2.y - v0.y)};
+}
+
+inline uint32_t spread10(uint32_t v) {
+  v &= 0x3ff;
+  v = (v | (v << 16)) & 0x30000ff;
+  v = (v | (v << 8)) & 0x300f00f;
+  v = (v | (v << 4)) & 0x30c30c3;
+  v = (v | (v << 2)) & 0x9249249;
+  return v;
+}
+
+}  // namespace
+
+int triangle_rule(int order, double* pts, double* w) {
+  if (order <= 1) {
+    pts[0] = pts[1] = 1.0 / 3.0;
+    w[0] = 0.5;
+    return 1;
+  }
+  if (order == 4) {
+    rule6(pts, w);
+    return 6;
+  }
+  const double p[6] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+  for (int i = 0; i < 6; ++i) pts[i] = p[i];
+  for (int i = 0; i < 3; ++i) w[i] = 1.0 / 6.0;
+  return 3;
+}
+
+std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64_t nf, int quad_order,
+                            int singular_order, HostMesh& m) {
+  if (nv < 3 || nf < 1) return "mesh needs at least 3 vertices and 1 face";
+  if (nv > (int64_t)1 << 30 || nf > (int64_t)1 << 30) return "mesh too large";
+  if (singular_order < 1 || singular_order > 4) return "singular_order must be in 1..4";
+  for (int64_t e = 0; e < 3 * nf; ++e)
+    if (F[e] < 0 || F[e] >= nv) return "face vertex index out of range";
+  m.nv = nv;
+  m.nf = nf;
+  m.quad_order = quad_order;
+  m.singular_order = singular_order;
+  double rp[12], rw[6];
+  m.nq = triangle_rule(quad_order, rp, rw);
+  m.rule_pts.assign(rp, rp + 2 * m.nq);
+  m.rule_w.assign(rw, rw + m.nq);
+
+  // --- per caller face geometry
+  std::vector<double> cen(3 * nf), nrm(3 * nf), area(nf);
+  for (int64_t e = 0; e < nf; ++e) {
+    const double* a = V + 3 * F[3 * e];
+    const double* b = V + 3 * F[3 * e + 1];
+    const double* c = V + 3 * F[3 * e + 2];
+    double e1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double e2[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double cr[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    double n = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    if (!(n > 0)) return "degenerate face " + std::to_string(e);
+    area[e] = n / 2.0;
+    for (int k = 0; k < 3; ++k) {
+      nrm[3 * e + k] = cr[k] / n;
+      cen[3 * e + k] = (a[k] + b[k] + c[k]) / 3.0;
+    }
+  }
+
+  // --- Morton order of face centroids
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t e = 0; e < nf; ++e)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], cen[3 * e + k]);
+      hi[k] = std::max(hi[k], cen[3 * e + k]);
+    }
+  double span = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1e-300});
+  std::vector<uint64_t> key(nf);
+  for (int64_t e = 0; e < nf; ++e) {
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) q[k] = (uint32_t)std::min(1023.0, std::floor((cen[3 * e + k] - lo[k]) / span * 1024.0));
+    key[e] = ((uint64_t)(spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2)) << 32) | (uint64_t)e;
+  }
+  std::sort(key.begin(), key.end());
+  m.face_perm.resize(nf);
+  for (int64_t i = 0; i < nf; ++i) m.face_perm[i] = (int32_t)(key[i] & 0xffffffffu);
+
+  // --- vertex numbering: first appearance along the face order
+  m.vert_inv.assign(nv, -1);
+  m.vert_perm.clear();
+  for (int64_t i = 0; i < nf; ++i)
+    for (int k = 0; k < 3; ++k) {
+      int v = F[3 * m.face_perm[i] + k];
+      if (m.vert_inv[v] < 0) {
+        m.vert_inv[v] = (int32_t)m.vert_perm.size();
+        m.vert_perm.push_back(v);
+      }
+    }
+  for (int64_t v = 0; v < nv; ++v)  // vertices no face touches keep an identity row
+    if (m.vert_inv[v] < 0) {
+      m.vert_inv[v] = (int32_t)m.vert_perm.size();
+      m.vert_perm.push_back((int32_t)v);
+    }
+  m.verts.resize(3 * nv);
+  for (int64_t i = 0; i < nv; ++i)
+    for (int k = 0; k < 3; ++k) m.verts[3 * i + k] = V[3 * m.vert_perm[i] + k];
+  m.fverts.resize(3 * nf);
+  for (int64_t i = 0; i < nf; ++i)
+    for (int k = 0; k < 3; ++k) m.fverts[3 * i + k] = m.vert_inv[F[3 * m.face_perm[i] + k]];
+
+  // --- tiles
+  m.tile_start.clear();
+  m.tile_count.clear();
+  m.tile_nv.clear();
+  m.tile_verts.clear();
+  m.floc.assign(3 * nf, 0);
+  {
+    int64_t i = 0;
+    while (i < nf) {
+      int start = (int)i, count = 0;
+      std::vector<int32_t> tv;
+      while (i < nf && count < TILE_F) {
+        int add = 0;
+        for (int k = 0; k < 3; ++k) {
+          int v = m.fverts[3 * i + k];
+          bool have = std::find(tv.begin(), tv.end(), v) != tv.end();
+          bool dup = false;
+          for (int kk = 0; kk < k; ++kk) dup |= m.fverts[3 * i + kk] == v;
+          if (!have && !dup) ++add;
+        }
+        if ((int)tv.size() + add > TILE_V) break;
+        for (int k = 0; k < 3; ++k) {
+          int v = m.fverts[3 * i + k];
+          auto it = std::find(tv.begin(), tv.end(), v);
+          if (it == tv.end()) {
+            tv.push_back(v);
+            it = tv.end() - 1;
+          }
+          m.floc[3 * i + k] = (uint8_t)(it - tv.begin());
+        }
+        ++count;
+        ++i;
+      }
+      m.tile_start.push_back(start);
+      m.tile_count.push_back(count);
+      m.tile_nv.push_back((int)tv.size());
+      tv.resize(TILE_V, tv.empty() ? 0 : tv[0]);
+      m.tile_verts.insert(m.tile_verts.end(), tv.begin(), tv.end());
+    }
+  }
+
+  // --- face records (internal order)
+  m.rec.assign((size_t)FREC * nf, 0.0);
+  for (int64_t i = 0; i < nf; ++i) {
+    const int64_t e = m.face_perm[i];
+    const double* v[3] = {V + 3 * F[3 * e], V + 3 * F[3 * e + 1], V + 3 * F[3 * e + 2]};
+    double* r = &m.rec[(size_t)FREC * i];
+    for (int q = 0; q < m.nq; ++q) {
+      const double xi = m.rule_pts[2 * q], eta = m.rule_pts[2 * q + 1], a = 1.0 - xi - eta;
+      for (int k = 0; k < 3; ++k) r[3 * q + k] = a * v[0][k] + xi * v[1][k] + eta * v[2][k];
+    }
+    const double* n = &nrm[3 * e];
+    for (int k = 0; k < 3; ++k) r[REC_N + k] = n[k];
+    // surface curls n x grad(phi_a); grads through the inverse Gram matrix
+    double e1[3], e2[3];
+    for (int k = 0; k < 3; ++k) {
+      e1[k] = v[1][k] - v[0][k];
+      e2[k] = v[2][k] - v[0][k];
+    }
+    const double g11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+    const double g12 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+    const double g22 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+    const double det = g11 * g22 - g12 * g12;
+    const double i11 = g22 / det, i12 = -g12 / det, i22 = g11 / det;
+    double gr[3][3];
+    for (int k = 0; k < 3; ++k) {
+      const double l1 = e1[k] * i11 + e2[k] * i12, l2 = e1[k] * i12 + e2[k] * i22;
+      gr[0][k] = -l1 - l2;
+      gr[1][k] = l1;
+      gr[2][k] = l2;
+    }
+    for (int a = 0; a < 3; ++a) {
+      r[REC_CURL + 3 * a + 0] = n[1] * gr[a][2] - n[2] * gr[a][1];
+      r[REC_CURL + 3 * a + 1] = n[2] * gr[a][0] - n[0] * gr[a][2];
+      r[REC_CURL + 3 * a + 2] = n[0] * gr[a][1] - n[1] * gr[a][0];
+    }
+    r[REC_AREA] = area[e];
+    r[REC_D] = v[0][0] * n[0] + v[0][1] * n[1] + v[0][2] * n[2];
+  }
+
+  // --- touching pairs and their remapped Duffy rules (reference order per test face)
+  std::vector<int32_t> internal_of(nf);
+  for (int64_t i = 0; i < nf; ++i) internal_of[m.face_perm[i]] = (int32_t)i;
+  std::vector<std::vector<int32_t>> v2e(nv);
+  for (int64_t e = 0; e < nf; ++e)
+    for (int k = 0; k < 3; ++k) v2e[F[3 * e + k]].push_back((int32_t)e);
+  Rule base[3] = {duffy(singular_order, 1), duffy(singular_order, 2), duffy(singular_order, 3)};
+  std::map<std::array<int, 5>, int> rule_index;
+  m.rule_off.clear();
+  m.rule_cnt.clear();
+  m.rule_kind.clear();
+  m.rule_data.clear();
+  m.spairs.clear();
+  m.n_coincident = m.n_edge = m.n_vertex = 0;
+  for (int64_t t = 0; t < nf; ++t) {
+    std::vector<int32_t> cand;
+    for (int k = 0; k < 3; ++k)
+      for (int32_t s : v2e[F[3 * t + k]])
+        if (std::find(cand.begin(), cand.end(), s) == cand.end()) cand.push_back(s);
+    for (int32_t s : cand) {
+      int sa[9], sb[9], ns = 0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          if (F[3 * t + i] == F[3 * s + j]) {
+            sa[ns] = i + 1;
+            sb[ns] = j + 1;
+            ++ns;
+          }
+      if (ns == 0) continue;
+      std::array<int, 5> k5;
+      int kind;
+      if (ns >= 3) {
+        kind = 1;
+        k5 = {1, 0, 0, 0, 0};
+      } else if (ns == 2) {
+        if (sb[1] < sb[0]) {
+          std::swap(sa[0], sa[1]);
+          std::swap(sb[0], sb[1]);
+        }
+        kind = 2;
+        k5 = {2, sa[0], sa[1], sb[0], sb[1]};
+      } else {
+        kind = 3;
+        k5 = {3, sa[0], 0, sb[0], 0};
+      }
+      auto it = rule_index.find(k5);
+      int ri;
+      if (it == rule_index.end()) {
+        const Rule& b = base[kind - 1];
+        ri = (int)m.rule_off.size();
+        m.rule_off.push_back((int32_t)(m.rule_data.size() / 5));
+        m.rule_cnt.push_back((int32_t)b.w.size());
+        m.rule_kind.push_back(kind);
+        for (size_t q = 0; q < b.w.size(); ++q) {
+          P2 a = b.tp[q], c = b.sp[q];
+          if (kind == 2) {
+            a = remap_edge(a, k5[1], k5[2]);
+            c = remap_edge(c, k5[3], k5[4]);
+          } else if (kind == 3) {
+            a = remap_vertex(a, k5[1]);
+            c = remap_vertex(c, k5[3]);
+          }
+          m.rule_data.insert(m.rule_data.end(), {a.x, a.y, c.x, c.y, b.w[q]});
+        }
+        rule_index[k5] = ri;
+      } else {
+        ri = it->second;
+      }
+      SingularPair p;
+      p.test = internal_of[t];
+      p.trial = internal_of[s];
+      p.rule = ri;
+      p.pad = kind;
+      p.jac = (2.0 * area[t]) * (2.0 * area[s]);
+      p.nprod = nrm[3 * t] * nrm[3 * s] + nrm[3 * t + 1] * nrm[3 * s + 1] + nrm[3 * t + 2] * nrm[3 * s + 2];
+      m.spairs.push_back(p);
+      if (kind == 1) ++m.n_coincident;
+      else if (kind == 2) ++m.n_edge;
+      else ++m.n_vertex;
+    }
+  }
+
+  // --- unit-area identity blocks from the rule (jac = 2 A)
+  for (int a = 0; a < 3; ++a) {
+    m.id_pd[a] = 0;
+    for (int b = 0; b < 3; ++b) m.id_pp[a][b] = 0;
+  }
+  for (int q = 0; q < m.nq; ++q) {
+    const double xi = m.rule_pts[2 * q], eta = m.rule_pts[2 * q + 1];
+    const double ph[3] = {1.0 - xi - eta, xi, eta};
+    for (int a = 0; a < 3; ++a) {
+      m.id_pd[a] += ph[a] * m.rule_w[q] * 2.0;
+      for (int b = 0; b < 3; ++b) m.id_pp[a][b] += ph[a] * ph[b] * m.rule_w[q] * 2.0;
+    }
+  }
+  return "";
+}
+
+}  // namespace beat

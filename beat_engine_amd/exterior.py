@@ -1,0 +1,298 @@
+"""Host-side mirror of the reference's exterior Burton-Miller interface over the
+gfx950 library.
+
+  solve_exterior_direct_hip        <- solve_exterior_direct_cuda
+                                      (src/beat_engine/julia_local/BeatEngineCompiledDriver.jl:870-915)
+  assemble_burton_miller_neumann_system_hip
+                                   <- assemble_burton_miller_neumann_system_cpu
+                                      (src/BeatEngineCpuBurtonMiller.jl:276-469)
+  evaluate_galerkin_field_hip      <- evaluate_galerkin_field_cpu (src/BeatEngineCpuField.jl:1-49)
+  beat_refine_lu                   <- beat_refine_cpu_lu (src/BeatEngineCpuRefinement.jl:5-78)
+  dense_solve_diagnostics          <- beat_dense_solve_diagnostics (src/BeatEngineDenseSolve.jl:1160-1196)
+
+Arguments keep the reference's meaning: k is the wavenumber (normalised by
+outgoing_wavenumber), q_neumann is F x drives, pressures come back per drive.
+"""
+from __future__ import annotations
+
+import ctypes
+import weakref
+
+import numpy as np
+
+from . import _native
+from .mesh import BoundaryMesh
+from .phasor import NEGATIVE_TIME_PHASOR, burton_miller_coupling, outgoing_wavenumber
+
+PRECISIONS = {
+    "float64": _native.PREC_FP64,
+    "float32_refined": _native.PREC_FP32_REFINED,
+    "float32": _native.PREC_FP32,
+}
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _precision(p) -> int:
+    if isinstance(p, int):
+        return p
+    key = str(p).lower()
+    if key not in PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(PRECISIONS)}")
+    return PRECISIONS[key]
+
+
+class Engine:
+    """One device context: a stream and the workspace the solves reuse."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _native.load()
+        h = ctypes.c_void_p()
+        rc = self._lib.beat_ctx_create(int(device), ctypes.byref(h))
+        if rc != 0:
+            raise _native.BeatError(f"beat_ctx_create({device}) failed with {rc}")
+        self._h = h
+        self._meshes = {}
+        self._fin = weakref.finalize(self, self._lib.beat_ctx_destroy, h)
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._lib.beat_last_error(self._h).decode()
+            err = _native.BeatError(f"{what} failed ({rc}): {msg}")
+            err.code = rc
+            raise err
+
+    def close(self):
+        for m in list(self._meshes.values()):
+            m.close()
+        self._meshes.clear()
+        self._fin()
+
+    def upload(self, mesh: BoundaryMesh, quadrature_order: int = 4, singular_order: int = 4) -> "DeviceMesh":
+        key = (id(mesh), quadrature_order, singular_order)
+        dm = self._meshes.get(key)
+        if dm is None or dm._mesh_ref() is not mesh:
+            dm = DeviceMesh(self, mesh, quadrature_order, singular_order)
+            self._meshes[key] = dm
+        return dm
+
+    # -- dense linear algebra --------------------------------------------------------
+    def dense_lu_solve(self, A, B, precision="float64"):
+        A = np.asfortranarray(A, dtype=np.complex128)
+        vec = np.ndim(B) == 1
+        B = np.asfortranarray(np.reshape(B, (A.shape[0], -1)), dtype=np.complex128)
+        if A.shape[0] != A.shape[1] or B.shape[0] != A.shape[0]:
+            raise ValueError("dimension mismatch")
+        X = np.zeros_like(B)
+        rc = self._lib.beat_dense_lu_solve(self._h, _precision(precision), _ptr(A), A.shape[0], _ptr(B), B.shape[1],
+                                           _ptr(X))
+        self._check(rc, "beat_dense_lu_solve")
+        return X[:, 0] if vec else X
+
+    def refine_lu(self, matrix, rhs, residual_matrix=None, residual_rhs=None, max_steps=3, rtol=1e-10):
+        if not (1 <= max_steps <= 3):
+            raise ValueError("max_steps must be in 1:3")
+        if not (np.isfinite(rtol) and rtol > 0):
+            raise ValueError("rtol must be finite and positive")
+        M = np.asfortranarray(matrix, dtype=np.complex64)
+        n = M.shape[0]
+        if M.shape[1] != n:
+            raise ValueError("matrix must be square")
+        vec = np.ndim(rhs) == 1
+        b = np.asfortranarray(np.reshape(rhs, (n, -1)) if not vec else np.asarray(rhs)[:, None], dtype=np.complex64)
+        if b.shape[0] != n:
+            raise ValueError("right-hand sides differ")
+        Ar = None if residual_matrix is None else np.asfortranarray(residual_matrix, dtype=np.complex128)
+        if Ar is not None and Ar.shape != M.shape:
+            raise ValueError("residual matrix differs")
+        br = None if residual_rhs is None else np.asfortranarray(np.reshape(residual_rhs, b.shape), dtype=np.complex128)
+        nrhs = b.shape[1]
+        x = np.zeros_like(b)
+        work = np.zeros(nrhs)
+        ret = np.zeros(nrhs)
+        hist = np.zeros((max_steps + 1) * nrhs)
+        rep = _native.RefineReport()
+        rep.working_residuals = work.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rep.returned_residuals = ret.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rep.history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rc = self._lib.beat_refine_lu(self._h, _ptr(M), n, _ptr(b), nrhs, None if Ar is None else _ptr(Ar),
+                                      None if br is None else _ptr(br), int(max_steps), float(rtol), _ptr(x),
+                                      ctypes.byref(rep))
+        self._check(rc, "beat_refine_lu")
+        report = dict(steps=rep.steps, status=_native.REFINE_STATUS[rep.status],
+                      working_status=_native.REFINE_STATUS[rep.working_status], rtol=rep.rtol,
+                      working_relative_residuals=work.copy(), returned_relative_residuals=ret.copy(),
+                      history=[hist[i * nrhs:(i + 1) * nrhs].copy() for i in range(rep.history_len)])
+        return (x[:, 0] if vec else x), report
+
+
+class DeviceMesh:
+    """Mesh resident on the device with its tiles, touching pairs and Duffy rules."""
+
+    def __init__(self, engine: Engine, mesh: BoundaryMesh, quadrature_order=4, singular_order=4):
+        self.engine = engine
+        self._mesh_ref = weakref.ref(mesh)
+        self.num_vertices = mesh.num_vertices
+        self.num_faces = mesh.num_faces
+        self.quadrature_order = quadrature_order
+        self.singular_order = singular_order
+        h = ctypes.c_void_p()
+        rc = engine._lib.beat_mesh_upload(engine._h, _ptr(mesh.vertices), mesh.num_vertices, _ptr(mesh.faces),
+                                          mesh.num_faces, int(quadrature_order), int(singular_order), ctypes.byref(h))
+        engine._check(rc, "beat_mesh_upload")
+        self._h = h
+        self._fin = weakref.finalize(self, engine._lib.beat_mesh_destroy, h)
+
+    def close(self):
+        self._fin()
+
+    def counts(self) -> dict:
+        c = np.zeros(6, dtype=np.int64)
+        self.engine._check(self.engine._lib.beat_mesh_counts(self._h, _ptr(c)), "beat_mesh_counts")
+        return dict(regular_pairs=int(c[0]), coincident=int(c[1]), edge=int(c[2]), vertex=int(c[3]),
+                    tiles=int(c[4]), rule_points=int(c[5]))
+
+    def _q(self, q):
+        q = np.asarray(q)
+        if q.ndim == 1:
+            q = q[:, None]
+        if q.shape[0] != self.num_faces:
+            raise ValueError("Fused Burton-Miller assembly needs one Neumann row per DP0 dof.")
+        return np.asfortranarray(q, dtype=np.complex128)
+
+    def assemble(self, k, coupling, q, precision="float64"):
+        qq = self._q(q)
+        n = self.num_vertices
+        A = np.zeros((n, n), dtype=np.complex128, order="F")
+        B = np.zeros((n, qq.shape[1]), dtype=np.complex128, order="F")
+        rc = self.engine._lib.beat_bm_assemble(self.engine._h, self._h, _precision(precision), float(k),
+                                               float(coupling.real), float(coupling.imag), _ptr(qq), qq.shape[1],
+                                               _ptr(A), _ptr(B))
+        self.engine._check(rc, "beat_bm_assemble")
+        return A, B
+
+    def solve(self, k, coupling, q, precision="float32_refined", max_steps=3, rtol=1e-10):
+        qq = self._q(q)
+        n, nrhs = self.num_vertices, qq.shape[1]
+        P = np.zeros((n, nrhs), dtype=np.complex128, order="F")
+        opts = _native.SolveOptions(int(max_steps), float(rtol))
+        rep = _native.SolveReport()
+        work = np.zeros(nrhs)
+        hist = np.zeros((max_steps + 1) * nrhs)
+        rep.refine.working_residuals = work.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rep.refine.returned_residuals = work.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        rep.refine.history = hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        prec = _precision(precision)
+        rc = self.engine._lib.beat_bm_solve(self.engine._h, self._h, prec, float(k), float(coupling.real),
+                                            float(coupling.imag), _ptr(qq), nrhs, _ptr(P), ctypes.byref(opts),
+                                            ctypes.byref(rep))
+        self.engine._check(rc, "beat_bm_solve")
+        report = dict(assembly_s=rep.assembly_s, factor_s=rep.factor_s, solve_s=rep.solve_s,
+                      regular_kernel_s=rep.regular_kernel_s,
+                      factorization_count=rep.factorization_count, lu_panels=rep.lu_panels,
+                      regular_pairs=rep.regular_pairs, singular_pairs=rep.singular_pairs)
+        if prec == _native.PREC_FP32_REFINED:
+            report["refinement"] = dict(
+                steps=rep.refine.steps, status=_native.REFINE_STATUS[rep.refine.status],
+                working_status=_native.REFINE_STATUS[rep.refine.working_status], rtol=rep.refine.rtol,
+                working_relative_residuals=work.copy(),
+                history=[hist[i * nrhs:(i + 1) * nrhs].copy() for i in range(rep.refine.history_len)])
+        return P, report
+
+    def field(self, k, pressure, q, points, precision="float64"):
+        p = np.asarray(pressure)
+        if p.ndim == 1:
+            p = p[:, None]
+        p = np.asfortranarray(p, dtype=np.complex128)
+        qq = self._q(q)
+        pts = np.ascontiguousarray(np.reshape(points, (-1, 3)), dtype=np.float64)
+        out = np.zeros((len(pts), p.shape[1]), dtype=np.complex128, order="F")
+        rc = self.engine._lib.beat_field_eval(self.engine._h, self._h, _precision(precision), float(k), _ptr(p),
+                                              _ptr(qq), p.shape[1], _ptr(pts), len(pts), _ptr(out))
+        self.engine._check(rc, "beat_field_eval")
+        return out
+
+
+_default_engine = None
+
+
+def default_engine() -> Engine:
+    global _default_engine
+    if _default_engine is None:
+        _default_engine = Engine(0)
+    return _default_engine
+
+
+def _rule_order(rule) -> int:
+    if rule is None:
+        return 4
+    return int(getattr(rule, "order", rule))
+
+
+def assemble_burton_miller_neumann_system_hip(mesh: BoundaryMesh, q_neumann, k, rule=4, *, singular_order=4,
+                                              precision="float64", convention=NEGATIVE_TIME_PHASOR, engine=None):
+    """(matrix, rhs) of the fused system; k is normalised to the outgoing wavenumber."""
+    eng = engine or default_engine()
+    dm = eng.upload(mesh, _rule_order(rule), singular_order)
+    ks = outgoing_wavenumber(k, convention)
+    A, B = dm.assemble(ks, burton_miller_coupling(ks, convention), q_neumann, precision)
+    return dict(matrix=A, rhs=B, assembly_mode="hip_fused_burton_miller", on_gpu=True, **dm.counts())
+
+
+def solve_exterior_direct_hip(mesh: BoundaryMesh, p1_space, dp0_space, neumann_values, wavenumber, rule=4, *,
+                              precision="float32_refined", singular_order=4, refinement_steps=3,
+                              refinement_rtol=1e-10, convention=NEGATIVE_TIME_PHASOR, engine=None):
+    """Same contract as solve_exterior_direct_cuda: one fused assembly and one
+    factorization for every excitation; returns (pressures, assembly_s, solve_s,
+    rhs) with rhs = (mode, operator_bytes), plus the dense-solve diagnostics
+    as a fifth element."""
+    del p1_space, dp0_space  # P1 = vertices and DP0 = faces, as in the reference
+    eng = engine or default_engine()
+    dm = eng.upload(mesh, _rule_order(rule), singular_order)
+    q = np.stack([np.asarray(v, dtype=np.complex128) for v in neumann_values], axis=1)
+    ks = outgoing_wavenumber(wavenumber, convention)
+    P, rep = dm.solve(ks, burton_miller_coupling(ks, convention), q, precision, refinement_steps, refinement_rtol)
+    pressures = [P[:, j].copy() for j in range(P.shape[1])]
+    diag = dense_solve_diagnostics(rep, precision)
+    return pressures, rep["assembly_s"], rep["factor_s"] + rep["solve_s"], dict(mode="fused", operator_bytes=0), diag
+
+
+def evaluate_galerkin_field_hip(eval_points, mesh: BoundaryMesh, pressure, q_neumann, k, rule=4, *,
+                                precision="float64", convention=NEGATIVE_TIME_PHASOR, engine=None):
+    eng = engine or default_engine()
+    dm = eng.upload(mesh, _rule_order(rule), 4)
+    out = dm.field(outgoing_wavenumber(k, convention), pressure, q_neumann, eval_points, precision)
+    return out[:, 0] if np.ndim(pressure) == 1 else out
+
+
+def beat_refine_lu(matrix, rhs, residual_matrix=None, residual_rhs=None, max_steps=3, rtol=1e-10, engine=None):
+    return (engine or default_engine()).refine_lu(matrix, rhs, residual_matrix, residual_rhs, max_steps, rtol)
+
+
+def dense_solve_diagnostics(report: dict, precision="float32_refined") -> dict:
+    """Keys of beat_dense_solve_diagnostics plus the exterior diagnostics the
+    driver records (burton_miller_assembly, linear_solver, factorization_count)."""
+    d = {
+        "burton_miller_assembly": "direct_system",
+        "linear_solver": "hip_mfma_lu",
+        "factorization_count": report.get("factorization_count", 1),
+        "assembly_s": report.get("assembly_s"),
+        "factor_s": report.get("factor_s"),
+        "solve_s": report.get("solve_s"),
+        "precision": str(precision),
+    }
+    ref = report.get("refinement")
+    if ref is not None:
+        res = [float(v) if np.isfinite(v) else None for v in ref["working_relative_residuals"]]
+        d.update({
+            "dense_solve_refinement_steps": ref["steps"],
+            "dense_solve_refinement_status": ref["status"],
+            "dense_solve_refinement_working_status": ref["working_status"],
+            "dense_solve_refinement_tolerance": ref["rtol"],
+            "dense_solve_refinement_working_residuals": res,
+            "dense_solve_refinement_returned_residuals": res,
+            "dense_solve_refinement_operator": "float64",
+        })
+    return d

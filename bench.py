@@ -1,0 +1,238 @@
+"""Frequency-solve throughput of the exterior Burton-Miller path on one MI355X.
+
+Workload (one step = one freq-solve): level-5 icosphere, radius 0.1 m, vertices
+jittered radially by N(0, 0.02 h) (20480 triangles, 10242 P1 dofs), 8 complex
+N(0,1) normal-velocity drives, q = i rho omega v (rho 1.21, c 343), quadrature
+order 4 / singular order 4, frequencies cycling through 20 log-spaced values
+100 Hz - 5 kHz. Each step assembles the system in complex128, factors it once
+in complex64 (trailing update on FP32 MFMA) and refines all 8 right-hand sides
+with complex128 residuals to 1e-10 (the FP32-MFMA + refinement path). The plain
+complex128 GPU path is timed on the same inputs for the ratio, and the refined
+pressures are compared with it (relative L2 per drive).
+
+Prints one JSON line. --gpus above 1 is not supported: this build runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "freq-solves/s @N=20480,8 RHS: FP32-MFMA+IR vs FP64 GPU at ≤1e-8 rel error"
+FP64_VALU_PEAK_TFLOPS = 78.6
+FLOP_PER_QPAIR = 72  # fused Burton-Miller flops per quadrature-point pair
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--level", type=int, default=5)
+    ap.add_argument("--nrhs", type=int, default=8)
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-faces", type=int, default=192)
+    ap.add_argument("--cpu-lu-n", type=int, default=3072)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(mesh, nrhs, sample_faces, lu_n):
+    """Oracle (C++ port of the reference CPU path, OpenMP) on a bounded sample:
+    regular pairs of `sample_faces` test faces against all trial faces, scaled
+    to all test faces; the full singular pass; complex128 LAPACK getrf (scipy's
+    OpenBLAS) at n = lu_n scaled by n^3; solve of all drives at full size."""
+    import scipy.linalg as sl
+
+    import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    oracle.set_threads(cores)
+    w = 2 * math.pi * 1000.0
+    k = w / 343.0
+    eta = complex(0.0, 1.0 / k)
+    rng = np.random.default_rng(5)
+    q = 1j * 1.21 * w * (rng.normal(size=(mesh.num_faces, nrhs)) + 1j * rng.normal(size=(mesh.num_faces, nrhs)))
+    t = time.perf_counter()
+    oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q, test_range=(0, 0), singular=False, identity=False)
+    t_alloc = time.perf_counter() - t
+    t = time.perf_counter()
+    oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q, test_range=(0, sample_faces), singular=False,
+                       identity=False)
+    t_reg = max(time.perf_counter() - t - t_alloc, 1e-9)
+    t = time.perf_counter()
+    oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q, test_range=(0, 0), singular=True, identity=True)
+    t_sing = max(time.perf_counter() - t - t_alloc, 0.0)
+    asm = t_reg * mesh.num_faces / sample_faces + t_sing
+    n = mesh.num_vertices
+    A = rng.normal(size=(lu_n, lu_n)) + 1j * rng.normal(size=(lu_n, lu_n)) + 40 * np.eye(lu_n)
+    t = time.perf_counter()
+    lu = sl.lu_factor(A, check_finite=False, overwrite_a=True)
+    t_lu = (time.perf_counter() - t) * (n / lu_n) ** 3
+    B = rng.normal(size=(lu_n, nrhs)) + 0j
+    t = time.perf_counter()
+    sl.lu_solve(lu, B, check_finite=False)
+    t_sv = (time.perf_counter() - t) * (n / lu_n) ** 2
+    total = asm + t_lu + t_sv
+    return {
+        "value": 1.0 / total,
+        "unit": "freq-solves/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"fused FP64 assembly of {sample_faces} of {mesh.num_faces} test faces x all trial faces "
+                   f"({t_reg:.2f} s) scaled to all, full singular pass ({t_sing:.2f} s), complex128 getrf at "
+                   f"n={lu_n} scaled by (n/{lu_n})^3 ({t_lu:.2f} s), getrs x{nrhs} ({t_sv:.3f} s); "
+                   f"{cores} threads"),
+        "s_per_freq": {"assembly": asm, "lu": t_lu, "solve": t_sv},
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding of this "
+              "path is described in DESIGN.md and not built.", file=sys.stderr)
+        sys.exit(2)
+    import ctypes
+
+    import torch
+
+    from beat_engine_amd import _native
+    from beat_engine_amd.exterior import Engine
+    from beat_engine_amd.mesh import jittered_icosphere
+    from beat_engine_amd.phasor import burton_miller_coupling, neumann_scale
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("bench.py needs a GPU")
+    mesh = jittered_icosphere(args.level, 0.1, 0.02, seed=20480)
+    nrhs = args.nrhs
+    freqs = np.logspace(np.log10(100.0), np.log10(5000.0), 20)
+    rng = np.random.default_rng(20480)
+    v = rng.normal(size=(mesh.num_faces, nrhs)) + 1j * rng.normal(size=(mesh.num_faces, nrhs))
+    eng = Engine(0)
+    t = time.perf_counter()
+    dm = eng.upload(mesh, 4, 4)
+    log(f"mesh upload {time.perf_counter() - t:.2f} s: {dm.counts()}")
+    lib = _native.load()
+    dev = torch.device("cuda:0")
+    # q per frequency, column-major F x nrhs complex128 == row-major (nrhs, F)
+    qs = []
+    for f in freqs:
+        q = neumann_scale(1.21, 2 * math.pi * f) * v
+        qs.append(torch.from_numpy(np.ascontiguousarray(q.T)).to(dev))
+    p_mixed = torch.zeros((nrhs, mesh.num_vertices), dtype=torch.complex128, device=dev)
+    p_fp64 = torch.zeros_like(p_mixed)
+    torch.cuda.synchronize()
+
+    def step(i, prec, out):
+        f = freqs[i % len(freqs)]
+        k = 2 * math.pi * f / 343.0
+        eta = burton_miller_coupling(k)
+        opts = _native.SolveOptions(8, 1e-10)
+        rep = _native.SolveReport()
+        rc = lib.beat_bm_solve_device(eng._h, dm._h, prec, k, eta.real, eta.imag,
+                                      ctypes.c_void_p(qs[i % len(freqs)].data_ptr()), nrhs,
+                                      ctypes.c_void_p(out.data_ptr()), ctypes.byref(opts), ctypes.byref(rep))
+        eng._check(rc, "beat_bm_solve_device")
+        return rep
+
+    for i in range(args.warmup):
+        r = step(i, _native.PREC_FP32_REFINED, p_mixed)
+        log(f"warmup {i}: asm {r.assembly_s:.3f} lu {r.factor_s:.3f} solve {r.solve_s:.3f} s")
+    torch.cuda.synchronize()
+    reps = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        reps.append(step(i, _native.PREC_FP32_REFINED, p_mixed))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i, r in enumerate(reps):
+        log(f"step {i} f={freqs[i % 20]:.1f}: asm {r.assembly_s:.3f} (regular {r.regular_kernel_s:.3f}) "
+            f"lu {r.factor_s:.3f} solve+refine {r.solve_s:.3f} s, refine steps {r.refine.steps} "
+            f"status {_native.REFINE_STATUS[r.refine.status]}")
+    sec = (t1 - t0) / args.steps
+    last = (args.steps - 1) % len(freqs)
+
+    # plain complex128 GPU path on the same inputs
+    t = time.perf_counter()
+    reps64 = []
+    for j in range(args.fp64_steps):
+        reps64.append(step(last if j == 0 else j, _native.PREC_FP64, p_fp64))
+    torch.cuda.synchronize()
+    sec64 = (time.perf_counter() - t) / max(1, args.fp64_steps)
+    # refined vs FP64 at the same (last) frequency
+    step(last, _native.PREC_FP32_REFINED, p_mixed)
+    step(last, _native.PREC_FP64, p_fp64)
+    torch.cuda.synchronize()
+    a = p_mixed.cpu().numpy()
+    b = p_fp64.cpu().numpy()
+    err = float(max(np.linalg.norm(a[j] - b[j]) / np.linalg.norm(b[j]) for j in range(nrhs)))
+    log(f"fp64 path {sec64:.3f} s/freq; refined vs fp64 max rel L2 {err:.2e}")
+
+    reg_s = float(np.mean([r.regular_kernel_s for r in reps]))
+    counts = dm.counts()
+    qpairs = counts["regular_pairs"] * counts["rule_points"] ** 2
+    flops = qpairs * FLOP_PER_QPAIR
+    achieved = flops / reg_s / 1e12
+    out = {
+        "metric": METRIC,
+        "value": 1.0 / sec,
+        "unit": "freq-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 assembly + c64 LU + c128 refinement",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"level-{args.level} jittered icosphere r=0.1 m ({mesh.num_faces} triangles, "
+                         f"{mesh.num_vertices} P1), {nrhs} RHS, 20 log-spaced f 100-5000 Hz, q4/s4"),
+            "precision": "float32_refined",
+            "refinement_rtol": 1e-10,
+        },
+        "fp64_gpu_freq_solves_per_s": 1.0 / sec64,
+        "speedup_vs_fp64_gpu": sec64 / sec,
+        "rel_err_vs_fp64_gpu": err,
+        "stage_s": {
+            "assembly": float(np.mean([r.assembly_s for r in reps])),
+            "regular_kernel": reg_s,
+            "lu_c64": float(np.mean([r.factor_s for r in reps])),
+            "solve_refine": float(np.mean([r.solve_s for r in reps])),
+            "fp64_lu": float(np.mean([r.factor_s for r in reps64])),
+            "fp64_solve": float(np.mean([r.solve_s for r in reps64])),
+        },
+        "element_pair_interactions_per_s": counts["regular_pairs"] / reg_s,
+        "roofline": {
+            "kernel": "bm_regular_kernel<double,6>",
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP64_VALU_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_VALU_PEAK_TFLOPS,
+            "traffic": None,
+            "algorithmic_flop_per_launch": flops,
+        },
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline(mesh, nrhs, args.cpu_sample_faces, args.cpu_lu_n)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = out["value"] / cb["value"]
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+/* C ABI of the gfx950 exterior Burton-Miller library (libbeat_bem_hip.so).
+ *
+ * The reference's exterior solve reaches its accelerator backends from
+ * solve_exterior_request (src/beat_engine/julia_local/BeatEngineCompiledDriver.jl:1049-1635);
+ * its direct_system branch calls solve_exterior_direct_cuda (:870-915), which
+ * returns (pressures, assembly_s, solve_s, rhs). These entry points are what a
+ * Julia `ccall` sibling, solve_exterior_direct_hip, binds to with the same
+ * contract. Conventions:
+ *   - complex arrays are interleaved (re, im) doubles, column-major, in the
+ *     caller's numbering (P1 dof = vertex, DP0 dof = face), 0-based;
+ *   - the wavenumber is the signed outgoing k (outgoing_wavenumber) and the
+ *     coupling the signed eta (burton_miller_coupling), BeatEnginePhasor.jl:1-24;
+ *   - q is the Neumann data F x nrhs, already scaled (neumann_scale * v_n);
+ *   - every call returns BEAT_OK (0) or a negative code; beat_last_error gives
+ *     the message. Host pointers are only touched during the call. One call at
+ *     a time per context; contexts hold no thread-local state.
+ */
+#ifndef BEAT_BEM_H
+#define BEAT_BEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEAT_OK 0
+#define BEAT_ERR_ARG (-1)
+#define BEAT_ERR_HIP (-2)
+#define BEAT_ERR_ALLOC (-3)
+#define BEAT_ERR_SINGULAR (-4)
+
+/* Arithmetic of a solve. FP64: assembly, LU and solve in complex128.
+ * FP32_REFINED: complex128 assembly held for the residual, complex64 LU
+ * (trailing update on FP32 MFMA) and complex128 iterative refinement; the
+ * returned pressure is the complex128 working solution.
+ * FP32: complex64 assembly, LU and solve, as the reference's float32 option. */
+#define BEAT_PREC_FP64 0
+#define BEAT_PREC_FP32_REFINED 1
+#define BEAT_PREC_FP32 2
+
+/* Refinement status values (beat_refine_cpu_lu's symbols,
+ * src/beat_engine/julia_local/src/BeatEngineCpuRefinement.jl:5-78). */
+#define BEAT_REFINE_NONE 0
+#define BEAT_REFINE_CONVERGED 1
+#define BEAT_REFINE_STAGNATED 2
+#define BEAT_REFINE_STEP_LIMIT 3
+#define BEAT_REFINE_ROUNDING_LIMITED 4
+#define BEAT_REFINE_NONFINITE 5
+
+typedef struct beat_ctx beat_ctx;
+typedef struct beat_mesh beat_mesh;
+
+/* Refinement report. The arrays are caller-owned and may be NULL:
+ * working/returned residuals hold nrhs values, history (max_steps+1)*nrhs
+ * (row e = entry e of the reference's history). */
+typedef struct {
+  int steps;
+  int status;
+  int working_status;
+  int history_len;
+  double rtol;
+  double* working_residuals;
+  double* returned_residuals;
+  double* history;
+} beat_refine_report;
+
+typedef struct {
+  double assembly_s; /* zero fill, regular, singular and identity passes */
+  double factor_s;   /* LU (and the complex128 -> complex64 copy when refined) */
+  double solve_s;    /* triangular solves and refinement */
+  double regular_kernel_s; /* the regular-pair assembly kernel alone */
+  int factorization_count;
+  int lu_panels;
+  int64_t regular_pairs;
+  int64_t singular_pairs;
+  beat_refine_report refine; /* filled for BEAT_PREC_FP32_REFINED */
+} beat_solve_report;
+
+typedef struct {
+  int max_steps; /* refinement steps, 1..8 (reference default 3) */
+  double rtol;   /* relative residual target per column (reference 1e-10) */
+} beat_solve_options;
+
+/* Context on one device (stream, workspace). Replaces the implicit device
+ * state the reference's ROCm backend keeps in AMDGPU.jl. */
+int beat_ctx_create(int device, beat_ctx** out);
+void beat_ctx_destroy(beat_ctx* ctx);
+const char* beat_last_error(const beat_ctx* ctx);
+
+/* Mesh, P1/DP0 spaces, regular rule, touching-pair cache and Duffy rules, once
+ * per mesh. Replaces BoundaryMesh (BeatEngineCore.jl:344-364), build_p1_space /
+ * build_dp0_space (:445-446), triangle_rule (:660), build_singular_correction_cache
+ * (:1049) and build_rocm_regular_assembly_cache (BeatEngineRocmRegular.jl:134-205).
+ * verts: nv x 3 row-major; faces: nf x 3 row-major, 0-based, outward winding. */
+int beat_mesh_upload(beat_ctx* ctx, const double* verts, int64_t nv, const int32_t* faces, int64_t nf,
+                     int quad_order, int singular_order, beat_mesh** out);
+void beat_mesh_destroy(beat_mesh* mesh);
+/* counts[0..5] = regular pairs, coincident, edge, vertex pairs, tiles, regular-rule points */
+int beat_mesh_counts(const beat_mesh* mesh, int64_t* counts);
+
+/* The assembled system A (nv x nv) and B (nv x nrhs), complex128, without
+ * solving: the operator-level parity point. Replaces
+ * assemble_burton_miller_neumann_system_cpu (BeatEngineCpuBurtonMiller.jl:276-469)
+ * and assemble_burton_miller_neumann_system_columns_cuda
+ * (BeatEngineCudaBurtonMiller.jl:785). precision: BEAT_PREC_FP64 or BEAT_PREC_FP32. */
+int beat_bm_assemble(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, double eta_re, double eta_im,
+                     const double* q, int64_t nrhs, double* a_out, double* b_out);
+
+/* Assemble, factor once and solve every right-hand side: the body of
+ * solve_exterior_direct_cuda (BeatEngineCompiledDriver.jl:870-915) and
+ * solve_burton_miller_neumann_system_cpu_with_report (BeatEngineCpuBurtonMiller.jl:490-508).
+ * pressure_out: nv x nrhs complex128. opts and rep may be NULL. */
+int beat_bm_solve(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, double eta_re, double eta_im,
+                  const double* q, int64_t nrhs, double* pressure_out, const beat_solve_options* opts,
+                  beat_solve_report* rep);
+
+/* Same with device-resident q (complex128 nf x nrhs) and pressure (complex128
+ * nv x nrhs), both column-major device pointers on the context's device. */
+int beat_bm_solve_device(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, double eta_re,
+                         double eta_im, const void* q_dev, int64_t nrhs, void* pressure_dev,
+                         const beat_solve_options* opts, beat_solve_report* rep);
+
+/* Field at npts points (npts x 3 row-major) from pressure (nv x nrhs) and q
+ * (nf x nrhs): evaluate_galerkin_field_cpu (BeatEngineCpuField.jl:1-49) /
+ * evaluate_galerkin_field_rocm (BeatEngineRocmField.jl:150). out: npts x nrhs.
+ * precision: BEAT_PREC_FP64 or BEAT_PREC_FP32. */
+int beat_field_eval(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, const double* pressure,
+                    const double* q, int64_t nrhs, const double* pts, int64_t npts, double* out);
+
+/* Dense LU with partial pivoting and solve of nrhs columns, the library routine
+ * behind _beat_dense_lu_solve (BeatEngineDenseSolve.jl:1104-1110) and
+ * rocm_dense_lu! (BeatEngineRocmOperators.jl:132-159). a: n x n, b and x:
+ * n x nrhs, complex128 column-major. precision FP64 or FP32. */
+int beat_dense_lu_solve(beat_ctx* ctx, int precision, const double* a, int64_t n, const double* b, int64_t nrhs,
+                        double* x);
+
+/* beat_refine_cpu_lu (BeatEngineCpuRefinement.jl:5-78) on the GPU: complex64 LU of
+ * `matrix`, complex128 residuals with residual_matrix / residual_rhs (NULL: the
+ * complex64 inputs widened), per-column acceptance. matrix, rhs, x_out are
+ * complex64 (interleaved floats) column-major; max_steps in 1..3. */
+int beat_refine_lu(beat_ctx* ctx, const float* matrix, int64_t n, const float* rhs, int64_t nrhs,
+                   const double* residual_matrix, const double* residual_rhs, int max_steps, double rtol,
+                   float* x_out, beat_refine_report* rep);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BEAT_BEM_H */

@@ -1,0 +1,213 @@
+"""GPU parity of the HIP path against the CPU oracle, through the C ABI.
+
+Bars: assembled matrices relative Frobenius <= 1e-12 (FP64) / 1e-5 (FP32);
+FP64 pressure <= 1e-10; FP32-LU + FP64 refinement pressure <= 1e-8; field
+<= 1e-12 (FP64) / 1e-5 (FP32); the reference's golden sweep norms <= 1e-4 and
+the analytic sphere's 0.05 dB / 0.5 deg gates."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from beat_engine_amd import exterior
+from beat_engine_amd.mesh import fibonacci_sphere, icosphere, jittered_icosphere, load_gmsh22
+from beat_engine_amd.phasor import burton_miller_coupling, neumann_scale
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = exterior.Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("n,nrhs", [(3, 1), (64, 2), (65, 3), (200, 8), (333, 11)])
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-11), ("float32", 2e-3)])
+def test_dense_lu_solve(eng, n, nrhs, precision, tol):
+    say("dense_lu", n, nrhs, precision)
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    B = rng.normal(size=(n, nrhs)) + 1j * rng.normal(size=(n, nrhs))
+    X = eng.dense_lu_solve(A, B, precision)
+    Xr = np.linalg.solve(A, B)
+    assert rel(X, Xr) < tol * max(1.0, np.linalg.cond(A) / 1e3)
+    assert rel(A @ X, B) < (1e-12 if precision == "float64" else 1e-4)
+
+
+def test_dense_lu_needs_pivoting(eng):
+    # zero leading entry forces a row exchange at the first column
+    A = np.array([[0, 1, 2], [3, 4, 5], [6, 7, 9]], dtype=complex)
+    B = np.eye(3, dtype=complex)
+    X = eng.dense_lu_solve(A, B, "float64")
+    assert np.allclose(A @ X, B, atol=1e-13)
+
+
+def _case(mesh, f=700.0, nrhs=3, seed=2):
+    w = 2 * np.pi * f
+    k = w / 343.0
+    eta = burton_miller_coupling(k)
+    rng = np.random.default_rng(seed)
+    q = neumann_scale(1.21, w) * (rng.normal(size=(mesh.num_faces, nrhs)) + 1j * rng.normal(size=(mesh.num_faces, nrhs)))
+    return k, eta, q
+
+
+MESHES = {
+    "tetrahedra": lambda: load_gmsh22(os.path.join(GOLDEN, "two_tetrahedra.msh"), 0.05),
+    "ico2": lambda: icosphere(0.1, 2),
+    "ico3_jitter": lambda: jittered_icosphere(3, 0.1, 0.02, seed=7),
+}
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 1e-5)])
+def test_assembly_matches_oracle(eng, name, precision, tol):
+    say("assembly", name, precision)
+    mesh = MESHES[name]()
+    k, eta, q = _case(mesh)
+    dm = eng.upload(mesh, 4, 4)
+    A, B = dm.assemble(k, eta, q, precision)
+    Ar, Br = oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q)
+    assert rel(A, Ar) <= tol
+    assert rel(B, Br) <= tol
+
+
+def test_assembly_three_point_rule(eng):
+    """quadrature_order 2 / singular_order 2 (the golden sweep's settings)."""
+    mesh = load_gmsh22(os.path.join(GOLDEN, "sample.msh"), 0.001)
+    k, eta, q = _case(mesh, 2000.0, 2)
+    dm = eng.upload(mesh, 2, 2)
+    A, B = dm.assemble(k, eta, q, "float64")
+    Ar, Br = oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q, quad_order=2, singular_order=2)
+    assert rel(A, Ar) <= 1e-12
+    assert rel(B, Br) <= 1e-12
+
+
+def test_general_complex_coupling(eng):
+    mesh = icosphere(0.1, 2)
+    k, _, q = _case(mesh)
+    eta = 0.3 + 0.8j / k
+    A, B = eng.upload(mesh).assemble(k, eta, q, "float64")
+    Ar, Br = oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q)
+    assert rel(A, Ar) <= 1e-12 and rel(B, Br) <= 1e-12
+
+
+@pytest.fixture(scope="module")
+def l3_system():
+    mesh = jittered_icosphere(3, 0.1, 0.02, seed=11)
+    k, eta, q = _case(mesh, 1500.0, 4, seed=3)
+    Ar, Br = oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q)
+    pr = oracle.lu_solve(Ar, Br)
+    return mesh, k, eta, q, pr
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-10), ("float32_refined", 1e-8), ("float32", 1e-4)])
+def test_solve_matches_oracle(eng, l3_system, precision, tol):
+    say("solve", precision)
+    mesh, k, eta, q, pr = l3_system
+    P, rep = eng.upload(mesh).solve(k, eta, q, precision)
+    assert rel(P, pr) <= tol
+    assert rep["factorization_count"] == 1
+    if precision == "float32_refined":
+        assert rep["refinement"]["status"] == "converged"
+
+
+def test_solve_many_rhs(eng):
+    """more right-hand sides than one solve pass holds (chunked columns)."""
+    mesh = icosphere(0.1, 2)
+    k, eta, q = _case(mesh, 900.0, 19, seed=4)
+    Ar, Br = oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q)
+    pr = oracle.lu_solve(Ar, Br)
+    P, rep = eng.upload(mesh).solve(k, eta, q, "float32_refined")
+    assert rel(P, pr) <= 1e-8
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 1e-5)])
+def test_field_matches_oracle(eng, l3_system, precision, tol):
+    mesh, k, eta, q, pr = l3_system
+    pts = np.concatenate([fibonacci_sphere(70, 2.0), fibonacci_sphere(30, 0.25)])
+    F = eng.upload(mesh).field(k, pr, q, pts, precision)
+    Fr = oracle.field(mesh.vertices, mesh.faces, k, pr, q, pts)
+    assert rel(F, Fr) <= tol
+
+
+def test_refine_known_answers(eng):
+    """cpu_refinement_tests.jl:91-129 through the GPU refinement."""
+    I3 = np.eye(3, dtype=np.complex64)
+    exact = np.array([1 + 2j, 3 - 4j, 0], dtype=np.complex64)
+    x, r = eng.refine_lu(I3, exact)
+    assert np.array_equal(x, exact) and r["steps"] == 0 and r["status"] == "converged"
+    x, r = eng.refine_lu(I3, exact, residual_matrix=np.zeros((3, 3), complex))
+    assert np.array_equal(x, exact) and r["status"] == "stagnated" and r["steps"] == 1
+    assert [h.tolist() for h in r["history"]] == [[1.0], [1.0]]
+    x, r = eng.refine_lu(I3, np.zeros((3, 2), np.complex64))
+    assert not x.any() and r["steps"] == 0 and r["status"] == "converged"
+    I2 = np.eye(2, dtype=np.complex64)
+    res_op = np.array([[-1, 0], [0, 0.5]], dtype=complex)
+    x, r = eng.refine_lu(I2, I2, residual_matrix=res_op)
+    assert np.array_equal(x, np.array([[1, 0], [0, 1.875]], dtype=np.complex64))
+    assert r["status"] == r["working_status"] == "step_limit" and r["steps"] == 3
+    assert [h.tolist() for h in r["history"]] == [[2, .5], [2, .25], [2, .125], [2, .0625]]
+    tri = np.array([[1, 3], [0, 7]], dtype=np.complex64)
+    x, r = eng.refine_lu(tri, np.ones(2, np.complex64), max_steps=1)
+    assert r["steps"] == 1 and r["working_status"] == "converged" and r["status"] == "rounding_limited"
+
+
+def test_refine_matches_oracle_restatement(eng):
+    rng = np.random.default_rng(9)
+    n = 150
+    A = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) + 12 * np.eye(n)).astype(np.complex64)
+    b = (rng.normal(size=(n, 3)) + 1j * rng.normal(size=(n, 3))).astype(np.complex64)
+    x, r = eng.refine_lu(A, b)
+    xo, ro = oracle.refine_cpu_lu(A, b)
+    assert r["status"] == ro["status"] and r["steps"] == ro["steps"]
+    assert rel(x.astype(complex), xo.astype(complex)) < 1e-6
+
+
+def test_golden_sweep_norms(eng):
+    """results/baseline_sweep_cpu.json: sample.msh, q2/s2, throat tag 2, float32."""
+    mesh = load_gmsh22(os.path.join(GOLDEN, "sample.msh"), 0.001)
+    g = json.load(open(os.path.join(GOLDEN, "baseline_sweep_cpu.json")))
+    pts = fibonacci_sphere(74, 2.0)
+    dm = eng.upload(mesh, 2, 2)
+    for row in g["frequencies"]:
+        w = 2 * np.pi * row["frequency_hz"]
+        k = w / 343.0
+        q = np.zeros(mesh.num_faces, complex)
+        q[mesh.physical_tags == 2] = 1j * 1.21 * w
+        P, _ = dm.solve(k, burton_miller_coupling(k), q, "float32")
+        fl = dm.field(k, P, q, pts, "float32")
+        assert np.linalg.norm(P) == pytest.approx(row["pressure_norm"], rel=1e-4)
+        assert np.linalg.norm(fl) == pytest.approx(row["field_norm"], rel=1e-4)
+
+
+def test_analytic_pulsating_sphere(eng):
+    """analytical_exterior_tests.jl:128-155 on the GPU path (refined and FP64)."""
+    mesh = icosphere(0.1, 3)
+    f, a, rho, c = 1000.0, 0.1, 1.2041, 343.0
+    w = 2 * np.pi * f
+    k = w / c
+    q = np.full(mesh.num_faces, neumann_scale(rho, w))
+    pts = fibonacci_sphere(128, 3.0)
+    ka = 1j * k * a
+    ref = np.array([rho * c * (a / np.linalg.norm(x)) * ka / (ka - 1) * np.exp(1j * k * (np.linalg.norm(x) - a)) for x in pts])
+    dm = eng.upload(mesh)
+    for prec in ("float32_refined", "float64"):
+        P, _ = dm.solve(k, burton_miller_coupling(k), q, prec)
+        fl = dm.field(k, P, q, pts, "float64")[:, 0]
+        ratio = fl / ref
+        assert np.max(np.abs(20 * np.log10(np.abs(ratio)))) <= 0.05
+        assert np.max(np.abs(np.degrees(np.angle(ratio)))) <= 0.5
