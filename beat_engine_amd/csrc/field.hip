@@ -115,12 +115,13 @@ __global__ __launch_bounds__(FIELD_PTS) void field_kernel(const T* sp, const T* 
           ai[d] += dr * pim + di * pr - (Gr * qim + Gi * qr);
         }
       }
-      for (int d = 0; d < nd; ++d) {
-        unsafeAtomicAdd(out + 2 * (pi * nrhs + d0 + d), ar[d]);
-        unsafeAtomicAdd(out + 2 * (pi * nrhs + d0 + d) + 1, ai[d]);
-      }
     }
   }
+  if (pi < npts)
+    for (int d = 0; d < nd; ++d) {
+      unsafeAtomicAdd(out + 2 * (pi * nrhs + d0 + d), ar[d]);
+      unsafeAtomicAdd(out + 2 * (pi * nrhs + d0 + d) + 1, ai[d]);
+    }
 }
 
 }  // namespace

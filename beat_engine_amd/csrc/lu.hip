@@ -307,7 +307,9 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // C[m x n] -= A[m x kc] B[kc x n], complex64, row-major (lda/ldb/ldc in complex
 // elements). Workgroup tile 128 rows x 64 complex columns, 4 waves of 64 x 64
-// real outputs, v_mfma_f32_32x32x2_f32 (one complex k per instruction).
+// real outputs, v_mfma_f32_32x32x2_f32 (one complex k per instruction). Lane l
+// holds A[l%32][l/32] and B[l/32][l%32]; output item v of lane l is
+// D[8(v/4) + 4(l/32) + v%4][l%32].
 __global__ __launch_bounds__(256) void cgemm_sub_f32(const float* __restrict__ A, int64_t lda,
                                                      const float* __restrict__ B, int64_t ldb, float* C, int64_t ldc,
                                                      int64_t m, int64_t n, int kc) {
@@ -376,7 +378,8 @@ __global__ __launch_bounds__(256) void cgemm_sub_f32(const float* __restrict__ A
 }
 
 // complex128 variant: tile 64 rows x 32 complex columns, 4 waves of 32 x 32 real
-// outputs, v_mfma_f64_16x16x4_f64 (two complex k per instruction).
+// outputs, v_mfma_f64_16x16x4_f64 (two complex k per instruction). Lane l holds
+// A[l%16][l/16] and B[l/16][l%16]; output item v of lane l is D[l/16 + 4v][l%16].
 __global__ __launch_bounds__(256) void zgemm_sub_f64(const double* __restrict__ A, int64_t lda,
                                                      const double* __restrict__ B, int64_t ldb, double* C,
                                                      int64_t ldc, int64_t m, int64_t n, int kc) {
@@ -439,7 +442,7 @@ __global__ __launch_bounds__(256) void zgemm_sub_f64(const double* __restrict__ 
   for (int mi = 0; mi < 2; ++mi)
     for (int ni = 0; ni < 2; ++ni)
       for (int v = 0; v < 4; ++v) {
-        const int r = wm * 32 + mi * 16 + 4 * kq + v;
+        const int r = wm * 32 + mi * 16 + kq + 4 * v;  // D row = lane/16 + 4 v
         const int j = wn * 32 + ni * 16 + l16;
         const int64_t gr = row0 + r, gcr = 2 * col0 + j;
         if (gr < m && gcr < 2 * n) C[gr * 2 * ldc + gcr] -= acc[mi][ni][v];
