@@ -66,6 +66,8 @@ struct beat_mesh {
   int32_t* fverts = nullptr;
   uint8_t* floc = nullptr;
   int32_t *tile_start = nullptr, *tile_count = nullptr, *tile_nv = nullptr, *tile_verts = nullptr;
+  int32_t *stile_start = nullptr, *stile_count = nullptr, *stile_nv = nullptr, *stile_verts = nullptr;
+  uint8_t* flocs = nullptr;
   SingularPair* spairs = nullptr;
   int32_t *rule_off = nullptr, *rule_cnt = nullptr;
   double* rule_data = nullptr;
@@ -209,7 +211,7 @@ template <typename T>
 int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   const int maxg = (int)((n + 63) / 64) + 1;
   const size_t slot_rows = (size_t)2 * maxg * 64 * 2 * sizeof(double);
-  const size_t bytes = sizeof(int) * (n + 8) + sizeof(double) * 2 * maxg + sizeof(int) * 2 * maxg + slot_rows +
+  const size_t bytes = sizeof(int) * (n + 8) + sizeof(double) * 2 * maxg + sizeof(long long) * 2 * maxg + slot_rows +
                        2 * 64 * 2 * sizeof(double) + 4096;
   HIPCHK(ensure(ctx->ipiv, bytes));
   char* p = reinterpret_cast<char*>(ctx->ipiv.p);
@@ -222,7 +224,7 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   w.bar = reinterpret_cast<int*>(take(sizeof(int) * 4));
   w.info = w.bar + 1;
   w.slot_val = reinterpret_cast<double*>(take(sizeof(double) * 2 * maxg));
-  w.slot_row = reinterpret_cast<int*>(take(sizeof(int) * 2 * maxg));
+  w.slot_row = reinterpret_cast<int*>(take(sizeof(unsigned long long) * 2 * maxg));  // 64-bit slot tags
   w.slot_rows = take(slot_rows);
   w.slot_diag = take(2 * 64 * 2 * sizeof(double));
   w.maxg = maxg;
@@ -439,6 +441,12 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   ra.tile_nv = m->tile_nv;
   ra.tile_verts = m->tile_verts;
   ra.ntiles = (int)m->h.tile_start.size();
+  ra.flocs = m->flocs;
+  ra.stile_start = m->stile_start;
+  ra.stile_count = m->stile_count;
+  ra.stile_nv = m->stile_nv;
+  ra.stile_verts = m->stile_verts;
+  ra.nstiles = (int)m->h.stile_start.size();
   ra.A = A;
   ra.lda = lda;
   ra.B = B;
@@ -691,6 +699,11 @@ int beat_mesh_upload(beat_ctx* ctx, const double* verts, int64_t nv, const int32
   if (e == hipSuccess) e = upload(&m->tile_count, m->h.tile_count);
   if (e == hipSuccess) e = upload(&m->tile_nv, m->h.tile_nv);
   if (e == hipSuccess) e = upload(&m->tile_verts, m->h.tile_verts);
+  if (e == hipSuccess) e = upload(&m->stile_start, m->h.stile_start);
+  if (e == hipSuccess) e = upload(&m->stile_count, m->h.stile_count);
+  if (e == hipSuccess) e = upload(&m->stile_nv, m->h.stile_nv);
+  if (e == hipSuccess) e = upload(&m->stile_verts, m->h.stile_verts);
+  if (e == hipSuccess) e = upload(&m->flocs, m->h.flocs);
   if (e == hipSuccess) e = upload(&m->spairs, m->h.spairs);
   if (e == hipSuccess) e = upload(&m->rule_off, m->h.rule_off);
   if (e == hipSuccess) e = upload(&m->rule_cnt, m->h.rule_cnt);
@@ -710,7 +723,7 @@ int beat_mesh_upload(beat_ctx* ctx, const double* verts, int64_t nv, const int32
 void beat_mesh_destroy(beat_mesh* m) {
   if (!m) return;
   void* ptrs[] = {m->rec64, m->rec32, m->verts, m->fverts, m->floc, m->tile_start, m->tile_count, m->tile_nv,
-                  m->tile_verts, m->spairs, m->rule_off, m->rule_cnt, m->rule_data, m->rule3, m->face_perm,
+                  m->tile_verts, m->stile_start, m->stile_count, m->stile_nv, m->stile_verts, m->flocs, m->spairs, m->rule_off, m->rule_cnt, m->rule_data, m->rule3, m->face_perm,
                   m->vert_inv};
   if (m->ctx) (void)hipSetDevice(m->ctx->device);
   for (void* p : ptrs)

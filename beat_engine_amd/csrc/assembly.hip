@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "bem_common.h"
+#include "bem_math.h"
 #include "kernels.h"
 
 namespace beat {
@@ -46,7 +47,7 @@ __device__ __forceinline__ void phase_sincos(float r, float krev, float kk, floa
 }
 __device__ __forceinline__ void phase_sincos(double r, double krev, double kk, double& s, double& c) {
   (void)krev;
-  sincos(kk * r, &s, &c);
+  sincos_phase(kk * r, &s, &c);
 }
 
 __device__ __forceinline__ float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -61,66 +62,58 @@ __device__ __forceinline__ double inv_sqrt(double x) {
 
 }  // namespace
 
-// Threads per workgroup: FP64 uses 4 waves so each wave may hold the full
-// unified register file; FP32 uses 8 waves.
-template <typename T>
-struct RegularBlock {
-  static constexpr int threads = sizeof(T) == 8 ? 256 : 512;
-};
-
+// One workgroup per (test tile, trial tile): 4 waves, lane = test face, each wave
+// takes every 4th trial face. Trial-face data are read with wave-uniform
+// addresses (scalar loads); 3x3 blocks go to the LDS tile accA[test vertex slot]
+// [trial vertex slot] and the P1 x DP0 blocks to accC[test vertex slot][trial
+// face]; both are flushed once with global atomics. Two workgroups fit per CU.
 template <typename T, int NQ, bool PURE>
-__global__ __launch_bounds__(RegularBlock<T>::threads) void bm_regular_kernel(const RegularArgs<T> a) {
-  __shared__ T accA[TILE_V * TILE_V * 2];
-  __shared__ T accC[TILE_V * TILE_F * 2];
-  __shared__ T srec[TILE_F * FREC];
+__global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T> a) {
+  __shared__ T accA[TILE_V * TILE_VS * 2];
+  __shared__ T accC[TILE_V * TILE_FS * 2];
   __shared__ T xs[NQ * 3 * 64];  // test points, [point*3 + c][lane]
-  __shared__ int sfv[TILE_F * 3];
-  __shared__ int sloc[TILE_F * 3];
-  __shared__ int grow[TILE_V], gcol[TILE_V];
+  __shared__ int grow[TILE_V], gcol[TILE_VS];
 
-  const int64_t total = (int64_t)a.ntiles * a.ntiles;
+  const int64_t total = (int64_t)a.ntiles * a.nstiles;
   const int64_t per = (total + 7) / 8;
   const int64_t L = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
   if (L >= total) return;
-  const int tt = (int)(L / a.ntiles), st = (int)(L % a.ntiles);
+  const int tt = (int)(L / a.nstiles), st = (int)(L % a.nstiles);
   const int tid = threadIdx.x;
   const int t0 = a.tile_start[tt], nT = a.tile_count[tt], nvT = a.tile_nv[tt];
-  const int s0 = a.tile_start[st], nS = a.tile_count[st], nvS = a.tile_nv[st];
+  const int s0 = a.stile_start[st], nS = a.stile_count[st], nvS = a.stile_nv[st];
+  const T* __restrict__ rec = a.rec;
+  const int32_t* __restrict__ fverts = a.fverts;
+  const uint8_t* __restrict__ flocs = a.flocs;
 
-  for (int i = tid; i < TILE_V * TILE_V * 2; i += blockDim.x) accA[i] = T(0);
-  for (int i = tid; i < TILE_V * TILE_F * 2; i += blockDim.x) accC[i] = T(0);
-  for (int i = tid; i < nS * FREC; i += blockDim.x) srec[i] = a.rec[(int64_t)s0 * FREC + i];
-  for (int i = tid; i < nS * 3; i += blockDim.x) {
-    sfv[i] = a.fverts[(int64_t)s0 * 3 + i];
-    sloc[i] = a.floc[(int64_t)s0 * 3 + i];
-  }
-  if (tid < TILE_V) {
-    grow[tid] = a.tile_verts[(int64_t)tt * TILE_V + tid];
-    gcol[tid] = a.tile_verts[(int64_t)st * TILE_V + tid];
-  }
-  __syncthreads();
+  for (int i = tid; i < TILE_V * TILE_VS * 2; i += blockDim.x) accA[i] = T(0);
+  for (int i = tid; i < TILE_V * TILE_FS * 2; i += blockDim.x) accC[i] = T(0);
+  if (tid < TILE_V) grow[tid] = a.tile_verts[(int64_t)tt * TILE_V + tid];
+  if (tid < TILE_VS) gcol[tid] = a.stile_verts[(int64_t)st * TILE_VS + tid];
 
-  const int lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
+  const int lane = tid & 63, wave = tid >> 6;
   const bool have_test = lane < nT;
   const int tf = t0 + (have_test ? lane : 0);
-  const T* tr = a.rec + (int64_t)tf * FREC;
+  const T* tr = rec + (int64_t)tf * FREC;
   if (wave == 0)
     for (int i = 0; i < NQ * 3; ++i) xs[i * 64 + lane] = tr[i];
   __syncthreads();
   const T nT0 = tr[REC_N], nT1 = tr[REC_N + 1], nT2 = tr[REC_N + 2];
   const T dT = tr[REC_D], areaT = tr[REC_AREA];
-  const int tv0 = a.fverts[(int64_t)tf * 3], tv1 = a.fverts[(int64_t)tf * 3 + 1], tv2 = a.fverts[(int64_t)tf * 3 + 2];
+  const int tv0 = fverts[(int64_t)tf * 3], tv1 = fverts[(int64_t)tf * 3 + 1], tv2 = fverts[(int64_t)tf * 3 + 2];
   const int tl0 = a.floc[(int64_t)tf * 3], tl1 = a.floc[(int64_t)tf * 3 + 1], tl2 = a.floc[(int64_t)tf * 3 + 2];
 
   const T k = a.k, krev = a.krev;
   const T er = a.eta_re, ei = a.eta_im;
+  const int wu = __builtin_amdgcn_readfirstlane(wave);
 
-  for (int sj = wave; sj < nS; sj += nwave) {
-    const int sv0 = sfv[3 * sj], sv1 = sfv[3 * sj + 1], sv2 = sfv[3 * sj + 2];
+  for (int sj = wu; sj < nS; sj += 4) {
+    const int64_t sf = s0 + sj;
+    const int sv0 = fverts[sf * 3], sv1 = fverts[sf * 3 + 1], sv2 = fverts[sf * 3 + 2];
     const bool adj = (tv0 == sv0) | (tv0 == sv1) | (tv0 == sv2) | (tv1 == sv0) | (tv1 == sv1) | (tv1 == sv2) |
                      (tv2 == sv0) | (tv2 == sv1) | (tv2 == sv2);
     if (!have_test || adj) continue;
-    const T* sr = srec + sj * FREC;
+    const T* sr = rec + sf * FREC;
     T y[NQ][3];
 #pragma unroll
     for (int j = 0; j < NQ; ++j)
@@ -209,18 +202,18 @@ __global__ __launch_bounds__(RegularBlock<T>::threads) void bm_regular_kernel(co
     // hypersingular curl term: (curl_a . curl_b) * eta * sum(G w)
     const T ecr = er * csr - ei * csi, eci = er * csi + ei * csr;
     const int tl[3] = {tl0, tl1, tl2};
-    const int sl[3] = {sloc[3 * sj], sloc[3 * sj + 1], sloc[3 * sj + 2]};
+    const int sl[3] = {flocs[sf * 3], flocs[sf * 3 + 1], flocs[sf * 3 + 2]};
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
       const T c0 = tr[REC_CURL + 3 * p], c1 = tr[REC_CURL + 3 * p + 1], c2 = tr[REC_CURL + 3 * p + 2];
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
         const T cc = c0 * sr[REC_CURL + 3 * b] + c1 * sr[REC_CURL + 3 * b + 1] + c2 * sr[REC_CURL + 3 * b + 2];
-        T* dst = accA + 2 * (tl[p] * TILE_V + sl[b]);
+        T* dst = accA + 2 * (tl[p] * TILE_VS + sl[b]);
         lds_add(dst, Lr[p][b] + cc * ecr);
         lds_add(dst + 1, Li[p][b] + cc * eci);
       }
-      T* dc = accC + 2 * (tl[p] * TILE_F + sj);
+      T* dc = accC + 2 * (tl[p] * TILE_FS + sj);
       lds_add(dc, Rr[p]);
       lds_add(dc + 1, Ri[p]);
     }
@@ -230,7 +223,7 @@ __global__ __launch_bounds__(RegularBlock<T>::threads) void bm_regular_kernel(co
   // flush the P1 x P1 block
   for (int idx = tid; idx < nvT * nvS; idx += blockDim.x) {
     const int r = idx / nvS, c = idx - r * nvS;
-    const T vr = accA[2 * (r * TILE_V + c)], vi = accA[2 * (r * TILE_V + c) + 1];
+    const T vr = accA[2 * (r * TILE_VS + c)], vi = accA[2 * (r * TILE_VS + c) + 1];
     if (vr != T(0) || vi != T(0)) {
       T* dst = a.A + 2 * ((int64_t)grow[r] * a.lda + gcol[c]);
       gadd(dst, vr);
@@ -242,7 +235,7 @@ __global__ __launch_bounds__(RegularBlock<T>::threads) void bm_regular_kernel(co
     const int r = idx / a.nrhs, d = idx - r * a.nrhs;
     T br = 0, bi = 0;
     for (int s = 0; s < nS; ++s) {
-      const T cr = accC[2 * (r * TILE_F + s)], ci = accC[2 * (r * TILE_F + s) + 1];
+      const T cr = accC[2 * (r * TILE_FS + s)], ci = accC[2 * (r * TILE_FS + s) + 1];
       const T* qq = a.q + 2 * ((int64_t)(s0 + s) * a.nrhs + d);
       const T qr = qq[0], qi = qq[1];
       br += cr * qr - ci * qi;
@@ -298,7 +291,7 @@ __global__ __launch_bounds__(256) void bm_singular_kernel(const SingularArgs<TO>
     if (radius == 0.0) continue;
     const double ir = 1.0 / radius;
     double s, c;
-    sincos(k * radius, &s, &c);
+    sincos_phase(k * radius, &s, &c);
     const double scale = INV_FOUR_PI * ir;
     const double Gr = c * scale, Gi = s * scale;
     const double gr = Gr * -ir - Gi * k, gi = Gr * k + Gi * -ir;  // G (-1/r + i k)
@@ -400,9 +393,9 @@ __global__ void downcast_kernel(const double* src, float* dst, int64_t n) {
 
 template <typename T>
 hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure, hipStream_t s) {
-  const int64_t total = (int64_t)a.ntiles * a.ntiles;
+  const int64_t total = (int64_t)a.ntiles * a.nstiles;
   const int64_t per = (total + 7) / 8;
-  dim3 grid((unsigned)(per * 8)), block(RegularBlock<T>::threads);
+  dim3 grid((unsigned)(per * 8)), block(256);
   if (nq == 6) {
     if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 6, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((bm_regular_kernel<T, 6, false>), grid, block, 0, s, a);

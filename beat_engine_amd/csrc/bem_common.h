@@ -11,6 +11,10 @@ namespace beat {
 // Face tiles: up to TILE_F faces that touch at most TILE_V distinct vertices.
 constexpr int TILE_F = 64;
 constexpr int TILE_V = 64;
+// Trial tiles of the regular assembly: up to TILE_FS faces touching at most
+// TILE_VS distinct vertices (a second partition of the same face order).
+constexpr int TILE_FS = 32;
+constexpr int TILE_VS = 32;
 // Per-face geometry record (FREC scalars): regular quadrature points (q*3+c, up
 // to 6 points), unit normal, surface curls (a*3+c), area, plane offset v1.n.
 constexpr int FREC = 32;
@@ -43,6 +47,9 @@ struct HostMesh {
   std::vector<double> verts;       // 3 per internal vertex
   std::vector<int32_t> tile_start, tile_count, tile_nv;
   std::vector<int32_t> tile_verts;  // TILE_V per tile
+  std::vector<int32_t> stile_start, stile_count, stile_nv;
+  std::vector<int32_t> stile_verts;  // TILE_VS per trial tile
+  std::vector<uint8_t> flocs;        // 3 per internal face: vertex slot in its trial tile
   std::vector<SingularPair> spairs;
   std::vector<int32_t> rule_off, rule_cnt, rule_kind;
   std::vector<double> rule_data;  // 5 per rule point: test xi, eta, trial xi, eta, weight

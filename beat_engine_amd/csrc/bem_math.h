@@ -1,0 +1,63 @@
+// FP64 sine and cosine for the Green's function phase k r.
+//
+// Cody-Waite reduction by pi/2 with a three-part constant (the first two parts
+// hold 30 significant bits, so n * part is exact for |n| < 2^22), followed by
+// the Taylor polynomials of sin to degree 15 and cos to degree 16 on
+// |y| <= pi/4, where the first omitted terms are below 1e-16 relative. The
+// constants are pi/2 split from a 60-digit value of pi (Machin's formula) and
+// the reciprocal factorials 1/n!. Phases beyond 2^20 pi/2 go to the library
+// routine.
+#pragma once
+#include <cmath>
+
+#if defined(__HIPCC__)
+#define BEAT_HD __host__ __device__ __forceinline__
+#else
+#define BEAT_HD inline
+#endif
+
+namespace beat {
+
+BEAT_HD void sincos_reduced(double x, double* s, double* c) {
+  const double n = rint(x * 0.6366197723675814);  // 2/pi
+  double y = fma(-n, 1.5707963276654482, x);
+  y = fma(-n, -8.705515692000731e-10, y);
+  y = fma(-n, -3.50343439808993e-19, y);
+  const double y2 = y * y;
+  double ps = -7.647163731819816e-13;        // -1/15!
+  ps = fma(ps, y2, 1.6059043836821613e-10);  // 1/13!
+  ps = fma(ps, y2, -2.505210838544172e-08);  // -1/11!
+  ps = fma(ps, y2, 2.7557319223985893e-06);  // 1/9!
+  ps = fma(ps, y2, -0.0001984126984126984);  // -1/7!
+  ps = fma(ps, y2, 0.008333333333333333);    // 1/5!
+  ps = fma(ps, y2, -0.16666666666666666);    // -1/3!
+  const double sy = fma(ps * y2, y, y);
+  double pc = 4.779477332387385e-14;         // 1/16!
+  pc = fma(pc, y2, -1.1470745597729725e-11); // -1/14!
+  pc = fma(pc, y2, 2.08767569878681e-09);    // 1/12!
+  pc = fma(pc, y2, -2.755731922398589e-07);  // -1/10!
+  pc = fma(pc, y2, 2.48015873015873e-05);    // 1/8!
+  pc = fma(pc, y2, -0.001388888888888889);   // -1/6!
+  pc = fma(pc, y2, 0.041666666666666664);    // 1/4!
+  pc = fma(pc, y2, -0.5);
+  const double cy = fma(pc, y2, 1.0);
+  // x = n pi/2 + y
+  const int q = (int)(long long)n & 3;
+  double ss = (q & 1) ? cy : sy;
+  double cc = (q & 1) ? sy : cy;
+  if (q & 2) ss = -ss;
+  if ((q + 1) & 2) cc = -cc;
+  *s = ss;
+  *c = cc;
+}
+
+BEAT_HD void sincos_phase(double x, double* s, double* c) {
+  if (fabs(x) < 1647099.3291652855) {  // 2^20 pi/2
+    sincos_reduced(x, s, c);
+  } else {
+    *s = sin(x);
+    *c = cos(x);
+  }
+}
+
+}  // namespace beat

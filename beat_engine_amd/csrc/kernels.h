@@ -18,6 +18,12 @@ struct RegularArgs {
   const int32_t* tile_nv;
   const int32_t* tile_verts;  // TILE_V per tile
   int ntiles;
+  const uint8_t* flocs;       // 3 per face: vertex slot in its trial tile
+  const int32_t* stile_start;
+  const int32_t* stile_count;
+  const int32_t* stile_nv;
+  const int32_t* stile_verts;  // TILE_VS per trial tile
+  int nstiles;
   T* A;                    // row-major complex, lda
   int64_t lda;
   T* B;                    // N x nrhs row-major complex
@@ -77,7 +83,7 @@ struct LuWork {
   int* ipiv;          // n, global row index chosen at each column
   int* bar;           // grid barrier counter
   double* slot_val;   // 2 x maxg
-  int* slot_row;      // 2 x maxg
+  int* slot_row;      // 2 x maxg 64-bit tag words (column tag << 32 | row)
   void* slot_rows;    // 2 x maxg x nb complex T
   void* slot_diag;    // 2 x nb complex T
   int* info;          // first zero pivot (1-based), 0 if none

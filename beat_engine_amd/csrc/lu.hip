@@ -27,6 +27,7 @@ namespace {
 
 constexpr int PNB = 64;   // max panel width
 constexpr int RPW = 64;   // panel rows per workgroup
+constexpr int MAXBW = 256; // outer block width (columns factored before the wide trailing update)
 
 template <typename T>
 struct PanelArgs {
@@ -87,174 +88,179 @@ __device__ __forceinline__ void grid_sync(int* bar, int target) {
   __syncthreads();
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
-  __shared__ T rows[RPW * PNB * 2];
-  __shared__ T piv[PNB * 2];
-  __shared__ double rv[256];
-  __shared__ int rr[256];
-  __shared__ T linv[RPW * 2];
-  __shared__ int s_p;
-  const int tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
-  const int64_t r0 = a.j0 + (int64_t)g * RPW;
-  const int nr = (int)min((int64_t)RPW, a.n - r0);
-  const int nb = a.nb;
-  for (int idx = tid; idx < nr * nb; idx += blockDim.x) {
-    const int r = idx / nb, c = idx - r * nb;
-    const T* src = a.A + 2 * ((r0 + r) * a.lda + a.j0 + c);
-    rows[2 * (r * PNB + c)] = src[0];
-    rows[2 * (r * PNB + c) + 1] = src[1];
-  }
-  __syncthreads();
-  for (int jj = 0; jj < nb; ++jj) {
-    const int64_t col = a.j0 + jj;
-    const int par = jj & 1;
-    // local pivot candidate: max |re|+|im| in FP64, first index on ties
-    double best = -1.0;
-    int brow = -1;
-    for (int r = tid; r < nr; r += blockDim.x) {
-      if (r0 + r < col) continue;
-      const double v = fabs((double)rows[2 * (r * PNB + jj)]) + fabs((double)rows[2 * (r * PNB + jj) + 1]);
-      if (v > best) {
-        best = v;
-        brow = (int)(r0 + r);
-      }
+// Pivot candidate order: larger |re|+|im| first, then the smaller row index.
+__device__ __forceinline__ bool better(double v2, int r2, double v, int r) {
+  return r2 >= 0 && (r < 0 || v2 > v || (v2 == v && r2 < r));
+}
+__device__ __forceinline__ void wave_argmax(double& v, int& r) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double v2 = __shfl_xor(v, o, 64);
+    const int r2 = __shfl_xor(r, o, 64);
+    if (better(v2, r2, v, r)) {
+      v = v2;
+      r = r2;
     }
-    rv[tid] = best;
-    rr[tid] = brow;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-      if (tid < o) {
-        const double v2 = rv[tid + o];
-        const int r2 = rr[tid + o];
-        if (v2 > rv[tid] || (v2 == rv[tid] && r2 >= 0 && (rr[tid] < 0 || r2 < rr[tid]))) {
-          rv[tid] = v2;
-          rr[tid] = r2;
-        }
-      }
-      __syncthreads();
-    }
-    best = rv[0];
-    brow = rr[0];
-    // publish candidate value, row index and row content; the owner of the
-    // diagonal row publishes that row too
-    if (tid == 0) {
-      st_relaxed(a.slot_val + par * a.maxg + g, best);
-      st_relaxed(a.slot_row + par * a.maxg + g, brow);
-    }
-    if (brow >= 0 && tid < nb) {
-      const int lr = (int)(brow - r0);
-      st_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + g) * PNB + tid), rows[2 * (lr * PNB + tid)],
-              rows[2 * (lr * PNB + tid) + 1]);
-    }
-    const bool own_diag = col >= r0 && col < r0 + nr;
-    if (own_diag && tid < nb) {
-      const int lr = (int)(col - r0);
-      st_cplx(a.slot_diag + 2 * (par * PNB + tid), rows[2 * (lr * PNB + tid)], rows[2 * (lr * PNB + tid) + 1]);
-    }
-    grid_sync(a.bar, a.bar_base + (jj + 1) * G);
-    // choose the global pivot: largest value, smallest row on ties
-    double v = -1.0;
-    int r = -1, wg = -1;
-    for (int i = tid; i < G; i += blockDim.x) {
-      const double vi = ld_relaxed(a.slot_val + par * a.maxg + i);
-      const int ri = ld_relaxed(a.slot_row + par * a.maxg + i);
-      if (ri >= 0 && (vi > v || (vi == v && ri < r))) {
-        v = vi;
-        r = ri;
-        wg = i;
-      }
-    }
-    rv[tid] = v;
-    rr[tid] = r;
-    __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-      if (tid < o) {
-        const double v2 = rv[tid + o];
-        const int r2 = rr[tid + o];
-        if (r2 >= 0 && (v2 > rv[tid] || (v2 == rv[tid] && (rr[tid] < 0 || r2 < rr[tid])))) {
-          rv[tid] = v2;
-          rr[tid] = r2;
-        }
-      }
-      __syncthreads();
-    }
-    const int p = rr[0];
-    const int pwg = (int)((p - a.j0) / RPW);
-    (void)wg;
-    if (tid < nb) {
-      T re, im;
-      ld_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + pwg) * PNB + tid), re, im);
-      piv[2 * tid] = re;
-      piv[2 * tid + 1] = im;
-    }
-    if (g == 0 && tid == 0) a.ipiv[col] = p;
-    __syncthreads();
-    if (p != col) {
-      if (p >= r0 && p < r0 + nr && tid < nb) {
-        T re, im;
-        ld_cplx(a.slot_diag + 2 * (par * PNB + tid), re, im);
-        const int lr = (int)(p - r0);
-        rows[2 * (lr * PNB + tid)] = re;
-        rows[2 * (lr * PNB + tid) + 1] = im;
-      }
-      if (own_diag && tid < nb) {
-        const int lr = (int)(col - r0);
-        rows[2 * (lr * PNB + tid)] = piv[2 * tid];
-        rows[2 * (lr * PNB + tid) + 1] = piv[2 * tid + 1];
-      }
-    }
-    __syncthreads();
-    const T pr = piv[2 * jj], pi = piv[2 * jj + 1];
-    if (pr == T(0) && pi == T(0)) {
-      if (g == 0 && tid == 0 && *a.info == 0) *a.info = (int)col + 1;
-      continue;  // LAPACK leaves the column unscaled and goes on
-    }
-    // multipliers l = a / pivot for rows below the diagonal
-    const T den = pr * pr + pi * pi;
-    const T ipr = pr / den, ipi = -pi / den;
-    for (int r2 = tid; r2 < nr; r2 += blockDim.x) {
-      if (r0 + r2 <= col) continue;
-      const T ar = rows[2 * (r2 * PNB + jj)], ai = rows[2 * (r2 * PNB + jj) + 1];
-      const T lr = ar * ipr - ai * ipi, li = ar * ipi + ai * ipr;
-      rows[2 * (r2 * PNB + jj)] = lr;
-      rows[2 * (r2 * PNB + jj) + 1] = li;
-      linv[2 * r2] = lr;
-      linv[2 * r2 + 1] = li;
-    }
-    __syncthreads();
-    const int w = nb - jj - 1;
-    if (w > 0) {
-      for (int idx = tid; idx < nr * w; idx += blockDim.x) {
-        const int r2 = idx / w, c = jj + 1 + (idx - r2 * w);
-        if (r0 + r2 <= col) continue;
-        const T lr = linv[2 * r2], li = linv[2 * r2 + 1];
-        const T ur = piv[2 * c], ui = piv[2 * c + 1];
-        rows[2 * (r2 * PNB + c)] -= lr * ur - li * ui;
-        rows[2 * (r2 * PNB + c) + 1] -= lr * ui + li * ur;
-      }
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < nr * nb; idx += blockDim.x) {
-    const int r = idx / nb, c = idx - r * nb;
-    T* dst = a.A + 2 * ((r0 + r) * a.lda + a.j0 + c);
-    dst[0] = rows[2 * (r * PNB + c)];
-    dst[1] = rows[2 * (r * PNB + c) + 1];
   }
 }
 
-// Row swaps of the trailing columns followed by U12 = L11^-1 A12; one thread per
-// column, 64 columns per workgroup.
+__device__ __forceinline__ void st_u64(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_u64(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One wavefront per workgroup; lane r owns panel row r0 + r. The panel block is
+// kept column-major in LDS (col[c][r], rows padded to RPW + 1) so that the
+// column operations are bank-conflict free.
+//
+// Cross-workgroup exchange per column, without a shared counter: every
+// workgroup writes its candidate value and row content to its slot, waits for
+// those stores, then writes the slot's tag word (column tag << 32 | row). Readers
+// poll the tag words of all slots; a slot whose tag matches the current column
+// holds this column's data. Slots alternate between two buffers by column
+// parity; a workgroup can only overwrite a buffer after every workgroup has
+// published the following column, that is after all reads of the buffer ended.
+// Slot tags are cleared before each factorization, and the tag of a column is
+// its index + 1.
 template <typename T>
-__global__ __launch_bounds__(64) void swap_trsm_kernel(T* A, int64_t n, int64_t lda, int64_t j0, int nb,
-                                                       const int* ipiv) {
+__global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
+  constexpr int LDR = RPW + 1;
+  __shared__ T col[PNB * LDR * 2];
+  __shared__ T piv[PNB * 2];
+  const int lane = threadIdx.x, g = blockIdx.x, G = gridDim.x;
+  const int64_t r0 = a.j0 + (int64_t)g * RPW;
+  const int nr = (int)min((int64_t)RPW, a.n - r0);
+  const int nb = a.nb;
+  unsigned long long* tags = reinterpret_cast<unsigned long long*>(a.slot_row);
+  for (int r = 0; r < nr; ++r)
+    if (lane < nb) {
+      const T* src = a.A + 2 * ((r0 + r) * a.lda + a.j0 + lane);
+      col[2 * (lane * LDR + r)] = src[0];
+      col[2 * (lane * LDR + r) + 1] = src[1];
+    }
+  __syncthreads();
+  for (int jj = 0; jj < nb; ++jj) {
+    const int64_t cg = a.j0 + jj;
+    const unsigned tag = (unsigned)(cg + 1);
+    const int par = jj & 1;
+    double v = -1.0;
+    int row = -1;
+    if (lane < nr && r0 + lane >= cg) {
+      v = fabs((double)col[2 * (jj * LDR + lane)]) + fabs((double)col[2 * (jj * LDR + lane) + 1]);
+      row = (int)(r0 + lane);
+    }
+    wave_argmax(v, row);
+    // publish: value, candidate row content, diagonal row content, then the tag
+    if (lane == 0) st_relaxed(a.slot_val + par * a.maxg + g, v);
+    if (row >= 0 && lane < nb) {
+      const int lr = (int)(row - r0);
+      st_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + g) * PNB + lane), col[2 * (lane * LDR + lr)],
+              col[2 * (lane * LDR + lr) + 1]);
+    }
+    const bool own_diag = cg >= r0 && cg < r0 + nr;
+    if (own_diag && lane < nb) {
+      const int lr = (int)(cg - r0);
+      st_cplx(a.slot_diag + 2 * (par * PNB + lane), col[2 * (lane * LDR + lr)], col[2 * (lane * LDR + lr) + 1]);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (lane == 0)
+      st_u64(tags + par * a.maxg + g, ((unsigned long long)tag << 32) | (unsigned long long)(unsigned)row);
+    // gather every slot of this column (lanes poll disjoint slots)
+    v = -1.0;
+    row = -1;
+    for (int i = lane; i < G; i += 64) {
+      unsigned long long t;
+      while (((t = ld_u64(tags + par * a.maxg + i)) >> 32) != tag) __builtin_amdgcn_s_sleep(1);
+      const int ri = (int)(unsigned)(t & 0xffffffffu);
+      const double vi = ld_relaxed(a.slot_val + par * a.maxg + i);
+      if (better(vi, ri, v, row)) {
+        v = vi;
+        row = ri;
+      }
+    }
+    wave_argmax(v, row);
+    const int p = row;
+    const int pwg = (int)((p - a.j0) / RPW);
+    T pr = 0, pim = 0;
+    if (lane < nb) {
+      ld_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + pwg) * PNB + lane), pr, pim);
+      piv[2 * lane] = pr;
+      piv[2 * lane + 1] = pim;
+    }
+    if (g == 0 && lane == 0) a.ipiv[cg] = p;
+    if (p != cg && lane < nb) {
+      if (p >= r0 && p < r0 + nr) {
+        T dr, di;
+        ld_cplx(a.slot_diag + 2 * (par * PNB + lane), dr, di);
+        col[2 * (lane * LDR + (p - r0))] = dr;
+        col[2 * (lane * LDR + (p - r0)) + 1] = di;
+      }
+      if (own_diag) {
+        col[2 * (lane * LDR + (cg - r0))] = pr;
+        col[2 * (lane * LDR + (cg - r0)) + 1] = pim;
+      }
+    }
+    __syncthreads();
+    const T vr = piv[2 * jj], vi = piv[2 * jj + 1];
+    if (vr == T(0) && vi == T(0)) {
+      if (g == 0 && lane == 0 && *a.info == 0) *a.info = (int)cg + 1;
+      continue;  // LAPACK leaves the column unscaled and goes on
+    }
+    const T den = vr * vr + vi * vi;
+    const T ir = vr / den, ii = -vi / den;
+    if (lane < nr && r0 + lane > cg) {
+      const T ar = col[2 * (jj * LDR + lane)], ai = col[2 * (jj * LDR + lane) + 1];
+      const T lr = ar * ir - ai * ii, li = ar * ii + ai * ir;
+      col[2 * (jj * LDR + lane)] = lr;
+      col[2 * (jj * LDR + lane) + 1] = li;
+      for (int c = jj + 1; c < nb; ++c) {
+        const T ur = piv[2 * c], ui = piv[2 * c + 1];
+        col[2 * (c * LDR + lane)] -= lr * ur - li * ui;
+        col[2 * (c * LDR + lane) + 1] -= lr * ui + li * ur;
+      }
+    }
+    __syncthreads();
+  }
+  for (int r = 0; r < nr; ++r)
+    if (lane < nb) {
+      T* dst = a.A + 2 * ((r0 + r) * a.lda + a.j0 + lane);
+      dst[0] = col[2 * (lane * LDR + r)];
+      dst[1] = col[2 * (lane * LDR + r) + 1];
+    }
+}
+
+// Row exchanges ipiv[k] <-> k for k in [k0, k1), in order, on columns [c0, c1).
+template <typename T>
+__global__ void laswp_cols_kernel(T* A, int64_t lda, int64_t c0, int64_t c1, const int* ipiv, int64_t k0, int64_t k1) {
+  const int64_t c = c0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= c1) return;
+  for (int64_t k = k0; k < k1; ++k) {
+    const int64_t p = ipiv[k];
+    if (p == k) continue;
+    T* x = A + 2 * (k * lda + c);
+    T* y = A + 2 * (p * lda + c);
+    const T xr = x[0], xi = x[1];
+    x[0] = y[0];
+    x[1] = y[1];
+    y[0] = xr;
+    y[1] = xi;
+  }
+}
+
+// Optional row swaps of the panel's pivots on columns [cb, ce), then
+// U12 = L11^-1 A12 for the panel's nb rows; one thread per column, 64 columns
+// per workgroup.
+template <typename T>
+__global__ __launch_bounds__(64) void swap_trsm_kernel(T* A, int64_t cb, int64_t ce, int64_t lda, int64_t j0, int nb,
+                                                       const int* ipiv, int do_swaps) {
   __shared__ T L[PNB * PNB * 2];
   __shared__ T X[PNB * 64 * 2];
   const int tid = threadIdx.x;
-  const int64_t c0 = j0 + nb + (int64_t)blockIdx.x * 64;
-  const int64_t cc = c0 + tid;
-  const bool act = cc < n;
+  const int64_t cc = cb + (int64_t)blockIdx.x * 64 + tid;
+  const bool act = cc < ce;
   for (int idx = tid; idx < nb * nb; idx += 64) {
     const int i = idx / nb, k = idx - i * nb;
     L[2 * (i * PNB + k)] = A[2 * ((j0 + i) * lda + j0 + k)];
@@ -267,7 +273,7 @@ __global__ __launch_bounds__(64) void swap_trsm_kernel(T* A, int64_t n, int64_t 
     }
   __syncthreads();
   if (!act) return;
-  for (int i = 0; i < nb; ++i) {
+  for (int i = 0; do_swaps && i < nb; ++i) {
     const int64_t p = ipiv[j0 + i];
     if (p == j0 + i) continue;
     const T xr = X[2 * (i * 64 + tid)], xi = X[2 * (i * 64 + tid) + 1];
@@ -520,38 +526,44 @@ __global__ __launch_bounds__(64) void diag_inverse_kernel(const T* LU, int64_t n
   }
 }
 
-// For panel k: the rows whose content the panel's swaps move, with the row each
-// one takes its content from (top rows first, then displaced rows below).
-__global__ void swap_lists_kernel(const int* ipiv, int64_t n, int nbmax, int npanels, int* rows, int* src, int* cnt) {
+// Swap list of leaf k (rows whose content moves, and the row each takes it
+// from; the leaf's own rows first). The first leaf of each outer block carries
+// the swaps of the whole block (its L columns are stored in post-block row
+// order); the other leaves of the block carry none.
+__global__ void swap_lists_kernel(const int* ipiv, int64_t n, int nb, int bw, int nleaves, int* rows, int* src,
+                                  int* cnt) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= npanels) return;
-  const int64_t j0 = (int64_t)k * nbmax;
-  const int nb = (int)min((int64_t)nbmax, n - j0);
-  int* R = rows + (int64_t)k * 2 * PNB;
-  int* S = src + (int64_t)k * 2 * PNB;
+  if (k >= nleaves) return;
+  const int64_t j0 = (int64_t)k * nb;
+  int* R = rows + (int64_t)k * 2 * MAXBW;
+  int* S = src + (int64_t)k * 2 * MAXBW;
   int m = 0;
-  for (int i = 0; i < nb; ++i) {
+  const bool first = (j0 % bw) == 0;
+  const int span = (int)min((int64_t)(first ? bw : nb), n - j0);
+  for (int i = 0; i < span; ++i) {
     R[m] = (int)(j0 + i);
     S[m] = (int)(j0 + i);
     ++m;
   }
-  for (int i = 0; i < nb; ++i) {
-    const int a = (int)(j0 + i), b = ipiv[j0 + i];
-    if (a == b) continue;
-    int ia = -1, ib = -1;
-    for (int t = 0; t < m; ++t) {
-      if (R[t] == a) ia = t;
-      if (R[t] == b) ib = t;
+  if (first)
+    for (int i = 0; i < span; ++i) {
+      const int a = (int)(j0 + i), b = ipiv[j0 + i];
+      if (a == b) continue;
+      const int ia = i;
+      int ib = -1;
+      if (b < j0 + span) ib = (int)(b - j0);
+      else
+        for (int t = span; t < m; ++t)
+          if (R[t] == b) ib = t;
+      if (ib < 0) {
+        R[m] = b;
+        S[m] = b;
+        ib = m++;
+      }
+      const int tmp = S[ia];
+      S[ia] = S[ib];
+      S[ib] = tmp;
     }
-    if (ib < 0) {
-      R[m] = b;
-      S[m] = b;
-      ib = m++;
-    }
-    const int tmp = S[ia];
-    S[ia] = S[ib];
-    S[ib] = tmp;
-  }
   cnt[k] = m;
 }
 
@@ -563,7 +575,7 @@ __global__ __launch_bounds__(256) void fwd_step_kernel(const T* LU, int64_t n, i
                                                        const T* bin, T* bout, T* bkeep, int nrhs) {
   __shared__ T top[PNB * 8 * 2];
   __shared__ T y[PNB * 8 * 2];
-  __shared__ int srows[2 * PNB], ssrc[2 * PNB];
+  __shared__ int srows[2 * MAXBW], ssrc[2 * MAXBW];
   const int tid = threadIdx.x;
   for (int i = tid; i < lcnt; i += blockDim.x) {
     srows[i] = lrows[i];
@@ -737,52 +749,99 @@ __global__ void colnorm_final_kernel(const double* part, int nparts, int nrhs, d
 }  // namespace
 
 template <typename T>
+static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, LuWork& w, int& bar_base,
+                               hipStream_t s) {
+  const int G = (int)((n - j0 + RPW - 1) / RPW);
+  if (G > w.maxg) return hipErrorInvalidValue;
+  PanelArgs<T> pa;
+  pa.A = A;
+  pa.n = n;
+  pa.lda = lda;
+  pa.j0 = j0;
+  pa.nb = b;
+  pa.ipiv = w.ipiv;
+  pa.bar = w.bar;
+  pa.bar_base = bar_base;
+  pa.slot_val = w.slot_val;
+  pa.slot_row = w.slot_row;
+  pa.slot_rows = reinterpret_cast<T*>(w.slot_rows);
+  pa.slot_diag = reinterpret_cast<T*>(w.slot_diag);
+  pa.maxg = w.maxg;
+  pa.info = w.info;
+  void* args[] = {&pa};
+  hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G), dim3(64), args,
+                                            0, s);
+  bar_base += G * b;
+  return e;
+}
+
+template <typename T>
+static hipError_t launch_swap_trsm(T* A, int64_t cb, int64_t ce, int64_t lda, int64_t j0, int b, const int* ipiv,
+                                   int do_swaps, hipStream_t s) {
+  if (ce <= cb) return hipSuccess;
+  hipLaunchKernelGGL((swap_trsm_kernel<T>), dim3((unsigned)((ce - cb + 63) / 64)), dim3(64), 0, s, A, cb, ce, lda, j0,
+                     b, ipiv, do_swaps);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_laswp(T* A, int64_t lda, int64_t c0, int64_t c1, const int* ipiv, int64_t k0, int64_t k1,
+                               hipStream_t s) {
+  if (c1 <= c0 || k1 <= k0) return hipSuccess;
+  hipLaunchKernelGGL((laswp_cols_kernel<T>), dim3((unsigned)((c1 - c0 + 255) / 256)), dim3(256), 0, s, A, lda, c0, c1,
+                     ipiv, k0, k1);
+  return hipGetLastError();
+}
+
+// Outer blocks of LU_BW columns factored as LU_NB-wide leaves (panel, swaps of
+// the block's other columns, leaf TRSM and in-block GEMM); the block's swaps,
+// TRSM and one K = LU_BW GEMM then update the trailing matrix.
+template <typename T>
 hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_t s, int* panels_out) {
   if (nb > PNB) nb = PNB;
+  const int bw = MAXBW;
   hipError_t e;
-  if ((e = launch_fill<int>(w.bar, 1, 0, s)) != hipSuccess) return e;
   if ((e = launch_fill<int>(w.info, 1, 0, s)) != hipSuccess) return e;
+  if ((e = launch_fill<int>(w.slot_row, 4 * (int64_t)w.maxg, 0, s)) != hipSuccess) return e;
   int bar_base = 0, panels = 0;
-  for (int64_t j0 = 0; j0 < n; j0 += nb) {
-    const int b = (int)std::min<int64_t>(nb, n - j0);
-    const int G = (int)((n - j0 + RPW - 1) / RPW);
-    if (G > w.maxg) return hipErrorInvalidValue;
-    PanelArgs<T> pa;
-    pa.A = A;
-    pa.n = n;
-    pa.lda = lda;
-    pa.j0 = j0;
-    pa.nb = b;
-    pa.ipiv = w.ipiv;
-    pa.bar = w.bar;
-    pa.bar_base = bar_base;
-    pa.slot_val = w.slot_val;
-    pa.slot_row = w.slot_row;
-    pa.slot_rows = reinterpret_cast<T*>(w.slot_rows);
-    pa.slot_diag = reinterpret_cast<T*>(w.slot_diag);
-    pa.maxg = w.maxg;
-    pa.info = w.info;
-    void* args[] = {&pa};
-    e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G), dim3(256), args, 0, s);
-    if (e != hipSuccess) return e;
-    bar_base += G * b;
-    const int64_t rest = n - j0 - b;
-    if (rest > 0) {
-      hipLaunchKernelGGL((swap_trsm_kernel<T>), dim3((unsigned)((rest + 63) / 64)), dim3(64), 0, s, A, n, lda, j0, b,
-                         (const int*)w.ipiv);
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-      e = gemm_sub<T>(A + 2 * ((j0 + b) * lda + j0), lda, A + 2 * (j0 * lda + j0 + b), lda,
-                      A + 2 * ((j0 + b) * lda + j0 + b), lda, rest, rest, b, s);
-      if (e != hipSuccess) return e;
+  const T* cA = A;
+  for (int64_t J = 0; J < n; J += bw) {
+    const int64_t Wc = std::min<int64_t>(bw, n - J);
+    const int64_t Je = J + Wc;
+    for (int64_t j0 = J; j0 < Je; j0 += nb) {
+      const int b = (int)std::min<int64_t>(nb, Je - j0);
+      if ((e = launch_panel<T>(A, n, lda, j0, b, w, bar_base, s)) != hipSuccess) return e;
+      ++panels;
+      if ((e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, s)) != hipSuccess) return e;
+      if (j0 + b < Je) {
+        if ((e = launch_swap_trsm<T>(A, j0 + b, Je, lda, j0, b, w.ipiv, 1, s)) != hipSuccess) return e;
+        e = gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + j0 + b), lda,
+                        A + 2 * ((j0 + b) * lda + j0 + b), lda, n - j0 - b, Je - j0 - b, b, s);
+        if (e != hipSuccess) return e;
+      }
     }
-    ++panels;
+    const int64_t rest = n - Je;
+    if (rest <= 0) continue;
+    if ((e = launch_laswp<T>(A, lda, Je, n, w.ipiv, J, Je, s)) != hipSuccess) return e;
+    for (int64_t j0 = J; j0 < Je; j0 += nb) {
+      const int b = (int)std::min<int64_t>(nb, Je - j0);
+      if ((e = launch_swap_trsm<T>(A, Je, n, lda, j0, b, w.ipiv, 0, s)) != hipSuccess) return e;
+      if (j0 + b < Je) {
+        e = gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + Je), lda,
+                        A + 2 * ((j0 + b) * lda + Je), lda, Je - j0 - b, rest, b, s);
+        if (e != hipSuccess) return e;
+      }
+    }
+    e = gemm_sub<T>(cA + 2 * (Je * lda + J), lda, cA + 2 * (J * lda + Je), lda, A + 2 * (Je * lda + Je), lda, rest,
+                    rest, (int)Wc, s);
+    if (e != hipSuccess) return e;
   }
   if (panels_out) *panels_out = panels;
   return hipSuccess;
 }
 
-// Solve workspace layout (complex T units): Linv/Uinv npanels*PNB*PNB each,
-// two n x nrhs buffers, swap lists in an int area appended at the end.
+// Solve workspace layout (complex T units): Linv/Uinv per leaf (PNB x PNB each),
+// two n x nrhs buffers, then the per-leaf swap lists (ints).
 template <typename T>
 hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, int nrhs, T* work, int nb,
                     hipStream_t s) {
@@ -794,12 +853,13 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, 
   T* b0 = Uinv + 2 * (int64_t)np * PNB * PNB;
   T* b1 = b0 + 2 * n * nrhs;
   int* lrows = reinterpret_cast<int*>(b1 + 2 * n * nrhs);
-  int* lsrc = lrows + (int64_t)np * 2 * PNB;
-  int* lcnt = lsrc + (int64_t)np * 2 * PNB;
+  int* lsrc = lrows + (int64_t)np * 2 * MAXBW;
+  int* lcnt = lsrc + (int64_t)np * 2 * MAXBW;
   hipError_t e;
   hipLaunchKernelGGL((diag_inverse_kernel<T>), dim3(np), dim3(64), 0, s, LU, n, lda, nb, Linv, Uinv);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(swap_lists_kernel, dim3((np + 63) / 64), dim3(64), 0, s, ipiv, n, nb, np, lrows, lsrc, lcnt);
+  hipLaunchKernelGGL(swap_lists_kernel, dim3((np + 63) / 64), dim3(64), 0, s, ipiv, n, nb, MAXBW, np, lrows, lsrc,
+                     lcnt);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   std::vector<int> hcnt(np);
   if ((e = hipMemcpyAsync(hcnt.data(), lcnt, sizeof(int) * np, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
@@ -807,20 +867,17 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, 
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
   T* bin = b0;
   T* bout = b1;
-  T* keep = B;  // finished rows are written to B directly
   for (int k = 0; k < np; ++k) {
     const int64_t j0 = (int64_t)k * nb;
     const int b = (int)std::min<int64_t>(nb, n - j0);
     const int64_t below = n - j0 - b;
     const unsigned G = (unsigned)std::max<int64_t>(1, (below + 127) / 128);
     hipLaunchKernelGGL((fwd_step_kernel<T>), dim3(G), dim3(256), 0, s, LU, n, lda, j0, b,
-                       Linv + 2 * (int64_t)k * PNB * PNB, lrows + (int64_t)k * 2 * PNB, lsrc + (int64_t)k * 2 * PNB,
-                       hcnt[k], (const T*)bin, bout, keep, nrhs);
+                       Linv + 2 * (int64_t)k * PNB * PNB, lrows + (int64_t)k * 2 * MAXBW,
+                       lsrc + (int64_t)k * 2 * MAXBW, hcnt[k], (const T*)bin, bout, B, nrhs);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     std::swap(bin, bout);
   }
-  // forward result: rows finished at their panel live in B; B is now y.
-  // backward: b := y in a scratch buffer, x written to B
   hipLaunchKernelGGL((copy_kernel<T>), dim3(1024), dim3(256), 0, s, (const T*)B, b0, 2 * n * nrhs);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   for (int k = np - 1; k >= 0; --k) {
@@ -836,7 +893,8 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, 
 
 size_t lu_solve_work_bytes(int64_t n, int nrhs, int nb, size_t elem) {
   const int np = (int)((n + nb - 1) / nb);
-  return elem * 2 * ((size_t)np * PNB * PNB * 2 + (size_t)n * nrhs * 2) + sizeof(int) * ((size_t)np * 4 * PNB + np) + 256;
+  return elem * 2 * ((size_t)np * PNB * PNB * 2 + (size_t)n * nrhs * 2) + sizeof(int) * ((size_t)np * 4 * MAXBW + np) +
+         256;
 }
 
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,

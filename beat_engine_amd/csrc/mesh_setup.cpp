@@ -227,18 +227,20 @@ std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64
   for (int64_t i = 0; i < nf; ++i)
     for (int k = 0; k < 3; ++k) m.fverts[3 * i + k] = m.vert_inv[F[3 * m.face_perm[i] + k]];
 
-  // --- tiles
-  m.tile_start.clear();
-  m.tile_count.clear();
-  m.tile_nv.clear();
-  m.tile_verts.clear();
-  m.floc.assign(3 * nf, 0);
-  {
+  // --- tiles: test tiles (TILE_F faces / TILE_V vertices) and trial tiles
+  // (TILE_FS / TILE_VS), both greedy along the face order
+  auto tiles = [&](int maxf, int maxv, std::vector<int32_t>& st, std::vector<int32_t>& cnt,
+                   std::vector<int32_t>& tnv, std::vector<int32_t>& tverts, std::vector<uint8_t>& loc) {
+    st.clear();
+    cnt.clear();
+    tnv.clear();
+    tverts.clear();
+    loc.assign(3 * nf, 0);
     int64_t i = 0;
     while (i < nf) {
       int start = (int)i, count = 0;
       std::vector<int32_t> tv;
-      while (i < nf && count < TILE_F) {
+      while (i < nf && count < maxf) {
         int add = 0;
         for (int k = 0; k < 3; ++k) {
           int v = m.fverts[3 * i + k];
@@ -247,7 +249,7 @@ std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64
           for (int kk = 0; kk < k; ++kk) dup |= m.fverts[3 * i + kk] == v;
           if (!have && !dup) ++add;
         }
-        if ((int)tv.size() + add > TILE_V) break;
+        if ((int)tv.size() + add > maxv) break;
         for (int k = 0; k < 3; ++k) {
           int v = m.fverts[3 * i + k];
           auto it = std::find(tv.begin(), tv.end(), v);
@@ -255,18 +257,20 @@ std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64
             tv.push_back(v);
             it = tv.end() - 1;
           }
-          m.floc[3 * i + k] = (uint8_t)(it - tv.begin());
+          loc[3 * i + k] = (uint8_t)(it - tv.begin());
         }
         ++count;
         ++i;
       }
-      m.tile_start.push_back(start);
-      m.tile_count.push_back(count);
-      m.tile_nv.push_back((int)tv.size());
-      tv.resize(TILE_V, tv.empty() ? 0 : tv[0]);
-      m.tile_verts.insert(m.tile_verts.end(), tv.begin(), tv.end());
+      st.push_back(start);
+      cnt.push_back(count);
+      tnv.push_back((int)tv.size());
+      tv.resize(maxv, tv.empty() ? 0 : tv[0]);
+      tverts.insert(tverts.end(), tv.begin(), tv.end());
     }
-  }
+  };
+  tiles(TILE_F, TILE_V, m.tile_start, m.tile_count, m.tile_nv, m.tile_verts, m.floc);
+  tiles(TILE_FS, TILE_VS, m.stile_start, m.stile_count, m.stile_nv, m.stile_verts, m.flocs);
 
   // --- face records (internal order)
   m.rec.assign((size_t)FREC * nf, 0.0);

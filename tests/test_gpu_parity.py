@@ -35,7 +35,7 @@ def eng():
     e.close()
 
 
-@pytest.mark.parametrize("n,nrhs", [(3, 1), (64, 2), (65, 3), (200, 8), (333, 11)])
+@pytest.mark.parametrize("n,nrhs", [(3, 1), (64, 2), (65, 3), (200, 8), (333, 11), (600, 4), (777, 3)])
 @pytest.mark.parametrize("precision,tol", [("float64", 1e-11), ("float32", 2e-3)])
 def test_dense_lu_solve(eng, n, nrhs, precision, tol):
     say("dense_lu", n, nrhs, precision)
@@ -45,7 +45,7 @@ def test_dense_lu_solve(eng, n, nrhs, precision, tol):
     X = eng.dense_lu_solve(A, B, precision)
     Xr = np.linalg.solve(A, B)
     assert rel(X, Xr) < tol * max(1.0, np.linalg.cond(A) / 1e3)
-    assert rel(A @ X, B) < (1e-12 if precision == "float64" else 1e-3)
+    assert rel(A @ X, B) < (1e-11 if precision == "float64" else 1e-3)
 
 
 def test_dense_lu_needs_pivoting(eng):
