@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -50,6 +51,8 @@ T* as(DevBuf& b) {
 struct beat_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // look-ahead stream of the LU
+  hipEvent_t ev_fact = nullptr, ev_far = nullptr;
   std::string err;
   std::mutex mu;
   DevBuf a64, a32, b64, b32, q64, q32, x64, r64, n64, rn64, xn64, d32, work, chunk, norms, ipiv, lusmall, qin, pout,
@@ -211,8 +214,9 @@ template <typename T>
 int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   const int maxg = (int)((n + 63) / 64) + 1;
   const size_t slot_rows = (size_t)2 * maxg * 64 * 2 * sizeof(double);
+  const size_t linv_bytes = (size_t)8 * 64 * 64 * 2 * sizeof(double);
   const size_t bytes = sizeof(int) * (n + 8) + sizeof(double) * 2 * maxg + sizeof(long long) * 2 * maxg + slot_rows +
-                       2 * 64 * 2 * sizeof(double) + 4096;
+                       2 * 64 * 2 * sizeof(double) + linv_bytes + 4096;
   HIPCHK(ensure(ctx->ipiv, bytes));
   char* p = reinterpret_cast<char*>(ctx->ipiv.p);
   auto take = [&](size_t b) {
@@ -227,6 +231,10 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   w.slot_row = reinterpret_cast<int*>(take(sizeof(unsigned long long) * 2 * maxg));  // 64-bit slot tags
   w.slot_rows = take(slot_rows);
   w.slot_diag = take(2 * 64 * 2 * sizeof(double));
+  w.linv = take(linv_bytes);
+  w.side = ctx->side;
+  w.ev_fact = ctx->ev_fact;
+  w.ev_far = ctx->ev_far;
   w.maxg = maxg;
   (void)sizeof(T);
   return BEAT_OK;
@@ -652,6 +660,17 @@ int beat_ctx_create(int device, beat_ctx** out) {
       delete c;
       return BEAT_ERR_HIP;
     }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fact, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_far, hipEventDisableTiming) != hipSuccess) {
+    delete c;
+    return BEAT_ERR_HIP;
+  }
+  if (const char* v = std::getenv("BEAT_LU_LOOKAHEAD"))
+    if (v[0] == '0') {
+      (void)hipStreamDestroy(c->side);
+      c->side = nullptr;
+    }
   *out = c;
   return BEAT_OK;
 }
@@ -667,6 +686,9 @@ void beat_ctx_destroy(beat_ctx* c) {
     if (b->p) (void)hipFree(b->p);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->ev_fact) (void)hipEventDestroy(c->ev_fact);
+  if (c->ev_far) (void)hipEventDestroy(c->ev_far);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }

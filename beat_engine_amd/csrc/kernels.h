@@ -87,7 +87,10 @@ struct LuWork {
   void* slot_rows;    // 2 x maxg x nb complex T
   void* slot_diag;    // 2 x nb complex T
   int* info;          // first zero pivot (1-based), 0 if none
+  void* linv;         // 2 x 4 x 64 x 64 complex (double-sized): leaf inverses of two outer blocks
   int maxg;
+  hipStream_t side;   // look-ahead stream (nullptr: no look-ahead)
+  hipEvent_t ev_fact, ev_far;
 };
 
 template <typename T>

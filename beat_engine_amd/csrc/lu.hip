@@ -43,6 +43,7 @@ struct PanelArgs {
   T* slot_diag;
   int maxg;
   int* info;
+  T* linv;  // PNB x PNB row-major: inverse of the leaf's unit-lower diagonal block
 };
 
 __device__ __forceinline__ void st_relaxed(double* p, double v) {
@@ -230,6 +231,38 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
       dst[0] = col[2 * (lane * LDR + r)];
       dst[1] = col[2 * (lane * LDR + r) + 1];
     }
+  // The first workgroup holds the leaf's diagonal block (already stored back):
+  // invert its unit-lower part in place, columns right to left as in LAPACK's
+  // trti2: X(i,j) = -(L(i,j) + sum_{k=j+1}^{i-1} X(i,k) L(k,j)), where X and L
+  // share col[][] (col[k][i] is entry (i,k)). Lane i owns row i.
+  if (g == 0 && a.linv) {
+    const int i = lane;
+    for (int j = nb - 2; j >= 0; --j) {
+      T sr = 0, si = 0;
+      if (i > j && i < nb) {
+        sr = col[2 * (j * LDR + i)];
+        si = col[2 * (j * LDR + i) + 1];
+        for (int k = j + 1; k < i; ++k) {
+          const T xr = col[2 * (k * LDR + i)], xi = col[2 * (k * LDR + i) + 1];
+          const T lr = col[2 * (j * LDR + k)], li = col[2 * (j * LDR + k) + 1];
+          sr += xr * lr - xi * li;
+          si += xr * li + xi * lr;
+        }
+      }
+      __syncthreads();
+      if (i > j && i < nb) {
+        col[2 * (j * LDR + i)] = -sr;
+        col[2 * (j * LDR + i) + 1] = -si;
+      }
+      __syncthreads();
+    }
+    for (int r = 0; r < nb; ++r)
+      if (lane < nb) {
+        const bool lower = lane < r;
+        a.linv[2 * (r * PNB + lane)] = lower ? col[2 * (lane * LDR + r)] : (lane == r ? T(1) : T(0));
+        a.linv[2 * (r * PNB + lane) + 1] = lower ? col[2 * (lane * LDR + r) + 1] : T(0);
+      }
+  }
 }
 
 // Row exchanges ipiv[k] <-> k for k in [k0, k1), in order, on columns [c0, c1).
@@ -311,169 +344,208 @@ __global__ __launch_bounds__(64) void swap_trsm_kernel(T* A, int64_t cb, int64_t
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// C[m x n] -= A[m x kc] B[kc x n], complex64, row-major (lda/ldb/ldc in complex
-// elements). Workgroup tile 128 rows x 64 complex columns, 4 waves of 64 x 64
-// real outputs, v_mfma_f32_32x32x2_f32 (one complex k per instruction). Lane l
-// holds A[l%32][l/32] and B[l/32][l%32]; output item v of lane l is
-// D[8(v/4) + 4(l/32) + v%4][l%32].
-__global__ __launch_bounds__(256) void cgemm_sub_f32(const float* __restrict__ A, int64_t lda,
-                                                     const float* __restrict__ B, int64_t ldb, float* C, int64_t ldc,
-                                                     int64_t m, int64_t n, int kc) {
-  constexpr int BM = 128, BN = 64, KB = 16;
-  __shared__ float As[BM * KB * 2];
-  __shared__ float Bs[KB * BN * 2];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  const int64_t row0 = (int64_t)blockIdx.y * BM, col0 = (int64_t)blockIdx.x * BN;
-  f32x16 acc[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j)
-      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
-  const int h = lane >> 5, l32 = lane & 31;
-  for (int k0 = 0; k0 < kc; k0 += KB) {
-    for (int idx = tid; idx < BM * KB; idx += 256) {
-      const int r = idx / KB, kk = idx - r * KB;
-      const int64_t gr = row0 + r;
-      float re = 0.f, im = 0.f;
-      if (gr < m && k0 + kk < kc) {
-        re = A[2 * (gr * lda + k0 + kk)];
-        im = A[2 * (gr * lda + k0 + kk) + 1];
-      }
-      As[2 * (r * KB + kk)] = re;
-      As[2 * (r * KB + kk) + 1] = im;
-    }
-    for (int idx = tid; idx < KB * BN; idx += 256) {
-      const int kk = idx / BN, c = idx - kk * BN;
-      const int64_t gc = col0 + c;
-      float re = 0.f, im = 0.f;
-      if (gc < n && k0 + kk < kc) {
-        re = B[2 * ((int64_t)(k0 + kk) * ldb + gc)];
-        im = B[2 * ((int64_t)(k0 + kk) * ldb + gc) + 1];
-      }
-      Bs[2 * (kk * BN + c)] = re;
-      Bs[2 * (kk * BN + c) + 1] = im;
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < KB; ++kk) {
-      float av[2], bv[2];
-      for (int mi = 0; mi < 2; ++mi) {
-        const int r = wm * 64 + mi * 32 + l32;
-        av[mi] = As[2 * (r * KB + kk) + h];
-      }
-      for (int ni = 0; ni < 2; ++ni) {
-        const int j = wn * 64 + ni * 32 + l32;  // real column in the tile
-        const int jc = j >> 1, part = j & 1;
-        const float br = Bs[2 * (kk * BN + jc)], bi = Bs[2 * (kk * BN + jc) + 1];
-        bv[ni] = h == 0 ? (part == 0 ? br : bi) : (part == 0 ? -bi : br);
-      }
-      for (int mi = 0; mi < 2; ++mi)
-        for (int ni = 0; ni < 2; ++ni)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
-    }
-    __syncthreads();
-  }
-  for (int mi = 0; mi < 2; ++mi)
-    for (int ni = 0; ni < 2; ++ni)
-      for (int v = 0; v < 16; ++v) {
-        const int r = wm * 64 + mi * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-        const int j = wn * 64 + ni * 32 + l32;
-        const int64_t gr = row0 + r, gcr = 2 * col0 + j;  // real column
-        if (gr < m && gcr < 2 * n) C[gr * 2 * ldc + gcr] -= acc[mi][ni][v];
-      }
-}
+// C[m x n] (-)= A[m x kc] B[kc x n] for row-major interleaved complex matrices
+// (leading dimensions in complex elements), as one real GEMM on MFMA: the right
+// operand's complex entries are expanded to 2x2 real blocks [[br, bi], [-bi, br]]
+// when read from LDS. Workgroup tile 128 rows x 64 complex columns, 4 waves of
+// 64 x 64 real outputs; K staged through LDS in chunks of 16 complex with the
+// next chunk prefetched into registers while the current one is multiplied.
+//   complex64:  v_mfma_f32_32x32x2_f32, lane l holds A[l%32][l/32], B[l/32][l%32],
+//               output item v of lane l is D[8(v/4) + 4(l/32) + v%4][l%32]
+//   complex128: v_mfma_f64_16x16x4_f64, lane l holds A[l%16][l/16], B[l/16][l%16],
+//               output item v of lane l is D[l/16 + 4v][l%16]
+template <typename T>
+struct GemmCfg;
+template <>
+struct GemmCfg<float> {
+  static constexpr int VEC = 4;  // scalars per 16-byte vector
+};
+template <>
+struct GemmCfg<double> {
+  static constexpr int VEC = 2;
+};
 
-// complex128 variant: tile 64 rows x 32 complex columns, 4 waves of 32 x 32 real
-// outputs, v_mfma_f64_16x16x4_f64 (two complex k per instruction). Lane l holds
-// A[l%16][l/16] and B[l/16][l%16]; output item v of lane l is D[l/16 + 4v][l%16].
-__global__ __launch_bounds__(256) void zgemm_sub_f64(const double* __restrict__ A, int64_t lda,
-                                                     const double* __restrict__ B, int64_t ldb, double* C,
-                                                     int64_t ldc, int64_t m, int64_t n, int kc) {
-  constexpr int BM = 64, BN = 32, KB = 16;
-  __shared__ double As[BM * KB * 2];
-  __shared__ double Bs[KB * BN * 2];
+template <typename T, bool SET>
+__global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
+                                                    int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n, int kc) {
+  constexpr int BM = 128, BN = 64, KB = 16;
+  constexpr int VEC = GemmCfg<T>::VEC;
+  constexpr int AP = 2 * KB + VEC;  // padded row lengths in scalars
+  constexpr int BP = 2 * BN + VEC;
+  constexpr int AV = BM * 2 * KB / VEC / 256;  // vectors per thread
+  constexpr int BV = KB * 2 * BN / VEC / 256;
+  typedef T vec __attribute__((ext_vector_type(VEC)));
+  __shared__ __attribute__((aligned(16))) T As[BM * AP];
+  __shared__ __attribute__((aligned(16))) T Bs[KB * BP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int64_t row0 = (int64_t)blockIdx.y * BM, col0 = (int64_t)blockIdx.x * BN;
-  f64x4 acc[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j)
-      for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
-  const int kq = lane >> 4, l16 = lane & 15;
-  for (int k0 = 0; k0 < kc; k0 += KB) {
-    for (int idx = tid; idx < BM * KB; idx += 256) {
-      const int r = idx / KB, kk = idx - r * KB;
+  vec ra[AV], rb[BV];
+
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int it = 0; it < AV; ++it) {
+      const int idx = tid + 256 * it;
+      const int r = idx / (2 * KB / VEC), cv = idx % (2 * KB / VEC);
       const int64_t gr = row0 + r;
-      double re = 0.0, im = 0.0;
-      if (gr < m && k0 + kk < kc) {
-        re = A[2 * (gr * lda + k0 + kk)];
-        im = A[2 * (gr * lda + k0 + kk) + 1];
+      const int kcx = k0 + cv * VEC / 2;  // first complex column of this vector
+      const T* src = A + 2 * (gr * lda + kcx);
+      vec v;
+      if (gr < m && kcx + VEC / 2 <= kc && ((uintptr_t)src & 15) == 0) {
+        v = *reinterpret_cast<const vec*>(src);
+      } else {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) v[e] = (gr < m && kcx + e / 2 < kc) ? src[e] : T(0);
       }
-      As[2 * (r * KB + kk)] = re;
-      As[2 * (r * KB + kk) + 1] = im;
+      ra[it] = v;
     }
-    for (int idx = tid; idx < KB * BN; idx += 256) {
-      const int kk = idx / BN, c = idx - kk * BN;
-      const int64_t gc = col0 + c;
-      double re = 0.0, im = 0.0;
-      if (gc < n && k0 + kk < kc) {
-        re = B[2 * ((int64_t)(k0 + kk) * ldb + gc)];
-        im = B[2 * ((int64_t)(k0 + kk) * ldb + gc) + 1];
+#pragma unroll
+    for (int it = 0; it < BV; ++it) {
+      const int idx = tid + 256 * it;
+      const int r = idx / (2 * BN / VEC), cv = idx % (2 * BN / VEC);
+      const int gk = k0 + r;
+      const int64_t gc = col0 + cv * VEC / 2;
+      const T* src = B + 2 * ((int64_t)gk * ldb + gc);
+      vec v;
+      if (gk < kc && gc + VEC / 2 <= n && ((uintptr_t)src & 15) == 0) {
+        v = *reinterpret_cast<const vec*>(src);
+      } else {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) v[e] = (gk < kc && gc + e / 2 < n) ? src[e] : T(0);
       }
-      Bs[2 * (kk * BN + c)] = re;
-      Bs[2 * (kk * BN + c) + 1] = im;
+      rb[it] = v;
     }
-    __syncthreads();
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int it = 0; it < AV; ++it) {
+      const int idx = tid + 256 * it;
+      const int r = idx / (2 * KB / VEC), cv = idx % (2 * KB / VEC);
+      *reinterpret_cast<vec*>(&As[r * AP + cv * VEC]) = ra[it];
+    }
+#pragma unroll
+    for (int it = 0; it < BV; ++it) {
+      const int idx = tid + 256 * it;
+      const int r = idx / (2 * BN / VEC), cv = idx % (2 * BN / VEC);
+      *reinterpret_cast<vec*>(&Bs[r * BP + cv * VEC]) = rb[it];
+    }
+  };
+
+  if constexpr (sizeof(T) == 4) {
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+    const int h = lane >> 5, l32 = lane & 31;
+    load(0);
+    for (int k0 = 0; k0 < kc; k0 += KB) {
+      stash();
+      __syncthreads();
+      if (k0 + KB < kc) load(k0 + KB);
+#pragma unroll 4
+      for (int kk = 0; kk < KB; ++kk) {
+        float av[2], bv[2];
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi) av[mi] = As[(wm * 64 + mi * 32 + l32) * AP + 2 * kk + h];
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni) {
+          const int j = wn * 64 + ni * 32 + l32;
+          const int jc = j >> 1, part = j & 1;
+          const float br = Bs[kk * BP + 2 * jc], bi = Bs[kk * BP + 2 * jc + 1];
+          bv[ni] = h == 0 ? (part == 0 ? br : bi) : (part == 0 ? -bi : br);
+        }
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < 2; ++ni)
+            acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int r = wm * 64 + mi * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+          const int j = wn * 64 + ni * 32 + l32;
+          const int64_t gr = row0 + r, gcr = 2 * col0 + j;
+          if (gr < m && gcr < 2 * n) {
+            if (SET) C[gr * 2 * ldc + gcr] = acc[mi][ni][v];
+            else C[gr * 2 * ldc + gcr] -= acc[mi][ni][v];
+          }
+        }
+  } else {
+    f64x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[i][j][v] = 0.0;
+    const int kq = lane >> 4, l16 = lane & 15;
+    load(0);
+    for (int k0 = 0; k0 < kc; k0 += KB) {
+      stash();
+      __syncthreads();
+      if (k0 + KB < kc) load(k0 + KB);
 #pragma unroll 2
-    for (int kk = 0; kk < KB; kk += 2) {
-      // real k index 2*kk + kq: complex k = kk + kq/2, part kq%2
-      const int kcx = kk + (kq >> 1), hp = kq & 1;
-      double av[2], bv[2];
-      for (int mi = 0; mi < 2; ++mi) {
-        const int r = wm * 32 + mi * 16 + l16;
-        av[mi] = As[2 * (r * KB + kcx) + hp];
+      for (int kk = 0; kk < KB; kk += 2) {
+        const int kcx = kk + (kq >> 1), hp = kq & 1;
+        double av[4], bv[4];
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) av[mi] = As[(wm * 64 + mi * 16 + l16) * AP + 2 * kcx + hp];
+#pragma unroll
+        for (int ni = 0; ni < 4; ++ni) {
+          const int j = wn * 64 + ni * 16 + l16;
+          const int jc = j >> 1, part = j & 1;
+          const double br = Bs[kcx * BP + 2 * jc], bi = Bs[kcx * BP + 2 * jc + 1];
+          bv[ni] = hp == 0 ? (part == 0 ? br : bi) : (part == 0 ? -bi : br);
+        }
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+          for (int ni = 0; ni < 4; ++ni)
+            acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
       }
-      for (int ni = 0; ni < 2; ++ni) {
-        const int j = wn * 32 + ni * 16 + l16;
-        const int jc = j >> 1, part = j & 1;
-        const double br = Bs[2 * (kcx * BN + jc)], bi = Bs[2 * (kcx * BN + jc) + 1];
-        bv[ni] = hp == 0 ? (part == 0 ? br : bi) : (part == 0 ? -bi : br);
-      }
-      for (int mi = 0; mi < 2; ++mi)
-        for (int ni = 0; ni < 2; ++ni)
-          acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
+      __syncthreads();
     }
-    __syncthreads();
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+      for (int ni = 0; ni < 4; ++ni)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = wm * 64 + mi * 16 + kq + 4 * v;
+          const int j = wn * 64 + ni * 16 + l16;
+          const int64_t gr = row0 + r, gcr = 2 * col0 + j;
+          if (gr < m && gcr < 2 * n) {
+            if (SET) C[gr * 2 * ldc + gcr] = acc[mi][ni][v];
+            else C[gr * 2 * ldc + gcr] -= acc[mi][ni][v];
+          }
+        }
   }
-  for (int mi = 0; mi < 2; ++mi)
-    for (int ni = 0; ni < 2; ++ni)
-      for (int v = 0; v < 4; ++v) {
-        const int r = wm * 32 + mi * 16 + kq + 4 * v;  // D row = lane/16 + 4 v
-        const int j = wn * 32 + ni * 16 + l16;
-        const int64_t gr = row0 + r, gcr = 2 * col0 + j;
-        if (gr < m && gcr < 2 * n) C[gr * 2 * ldc + gcr] -= acc[mi][ni][v];
-      }
 }
 
 template <typename T>
 hipError_t gemm_sub(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
-                    int kc, hipStream_t s);
-
-template <>
-hipError_t gemm_sub<float>(const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
-                           int64_t m, int64_t n, int kc, hipStream_t s) {
+                    int kc, hipStream_t s) {
   if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
   dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 127) / 128));
-  hipLaunchKernelGGL(cgemm_sub_f32, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
+  hipLaunchKernelGGL((zgemm_kernel<T, false>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
   return hipGetLastError();
 }
 
-template <>
-hipError_t gemm_sub<double>(const double* A, int64_t lda, const double* B, int64_t ldb, double* C, int64_t ldc,
-                            int64_t m, int64_t n, int kc, hipStream_t s) {
+template <typename T>
+hipError_t gemm_set(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
+                    int kc, hipStream_t s) {
   if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
-  dim3 grid((unsigned)((n + 31) / 32), (unsigned)((m + 63) / 64));
-  hipLaunchKernelGGL(zgemm_sub_f64, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
+  dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 127) / 128));
+  hipLaunchKernelGGL((zgemm_kernel<T, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
   return hipGetLastError();
 }
 
@@ -564,27 +636,35 @@ __global__ void swap_lists_kernel(const int* ipiv, int64_t n, int nb, int bw, in
       S[ia] = S[ib];
       S[ib] = tmp;
     }
+  // rows below the leaf's own rows, sorted for binary search
+  const int own = (int)min((int64_t)nb, n - j0);
+  for (int t = own + 1; t < m; ++t) {
+    const int r = R[t], v = S[t];
+    int u = t - 1;
+    while (u >= own && R[u] > r) {
+      R[u + 1] = R[u];
+      S[u + 1] = S[u];
+      --u;
+    }
+    R[u + 1] = r;
+    S[u + 1] = v;
+  }
   cnt[k] = m;
 }
 
-// Forward step of panel k: y = Linv_k (P_k b)_top; rows below: b - L21 y.
-// Reads bin, writes bout (rows >= j0); y is written to both buffers.
+// Forward step of leaf k: y = Linv_k (P b)_top, then b_r - L_r y for the rows
+// below; one thread per row (256 rows per workgroup), y in LDS. Reads bin and
+// writes bout (rows >= j0); y is also written to bkeep.
 template <typename T>
 __global__ __launch_bounds__(256) void fwd_step_kernel(const T* LU, int64_t n, int64_t lda, int64_t j0, int nb,
                                                        const T* Linv, const int* lrows, const int* lsrc, int lcnt,
                                                        const T* bin, T* bout, T* bkeep, int nrhs) {
   __shared__ T top[PNB * 8 * 2];
   __shared__ T y[PNB * 8 * 2];
-  __shared__ int srows[2 * MAXBW], ssrc[2 * MAXBW];
   const int tid = threadIdx.x;
-  for (int i = tid; i < lcnt; i += blockDim.x) {
-    srows[i] = lrows[i];
-    ssrc[i] = lsrc[i];
-  }
-  __syncthreads();
   for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
     const int i = idx / nrhs, d = idx - i * nrhs;
-    const int64_t sr = ssrc[i];  // top rows come first in the list
+    const int64_t sr = lsrc[i];  // the leaf's own rows come first in the list
     top[2 * (i * 8 + d)] = bin[2 * (sr * nrhs + d)];
     top[2 * (i * 8 + d) + 1] = bin[2 * (sr * nrhs + d) + 1];
   }
@@ -611,28 +691,44 @@ __global__ __launch_bounds__(256) void fwd_step_kernel(const T* LU, int64_t n, i
       bkeep[2 * (r * nrhs + d)] = y[2 * (i * 8 + d)];
       bkeep[2 * (r * nrhs + d) + 1] = y[2 * (i * 8 + d) + 1];
     }
-  const int64_t rbase = j0 + nb + (int64_t)blockIdx.x * (256 / 8) * 4;
-  for (int idx = tid; idx < 128 * nrhs; idx += blockDim.x) {
-    const int lr = idx / nrhs, d = idx - lr * nrhs;
-    const int64_t r = rbase + lr;
-    if (r >= n) continue;
-    int64_t sr = r;
-    for (int t = nb; t < lcnt; ++t)
-      if (srows[t] == r) sr = ssrc[t];
-    T vr = bin[2 * (sr * nrhs + d)], vi = bin[2 * (sr * nrhs + d) + 1];
-    const T* L = LU + 2 * (r * lda + j0);
-    for (int q = 0; q < nb; ++q) {
-      const T lr2 = L[2 * q], li2 = L[2 * q + 1];
-      const T yr = y[2 * (q * 8 + d)], yi = y[2 * (q * 8 + d) + 1];
-      vr -= lr2 * yr - li2 * yi;
-      vi -= lr2 * yi + li2 * yr;
+  const int64_t r = j0 + nb + (int64_t)blockIdx.x * 256 + tid;
+  if (r >= n) return;
+  // source row of r after the leaf's swaps: binary search in the sorted extras
+  int64_t src = r;
+  int lo = nb, hi = lcnt - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int rm = lrows[mid];
+    if (rm == r) {
+      src = lsrc[mid];
+      break;
     }
-    bout[2 * (r * nrhs + d)] = vr;
-    bout[2 * (r * nrhs + d) + 1] = vi;
+    if (rm < r) lo = mid + 1;
+    else hi = mid - 1;
+  }
+  T ar[8], ai[8];
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    ar[d] = d < nrhs ? bin[2 * (src * nrhs + d)] : T(0);
+    ai[d] = d < nrhs ? bin[2 * (src * nrhs + d) + 1] : T(0);
+  }
+  const T* L = LU + 2 * (r * lda + j0);
+  for (int q = 0; q < nb; ++q) {
+    const T lr = L[2 * q], li = L[2 * q + 1];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const T yr = y[2 * (q * 8 + d)], yi = y[2 * (q * 8 + d) + 1];
+      ar[d] -= lr * yr - li * yi;
+      ai[d] -= lr * yi + li * yr;
+    }
+  }
+  for (int d = 0; d < nrhs; ++d) {
+    bout[2 * (r * nrhs + d)] = ar[d];
+    bout[2 * (r * nrhs + d) + 1] = ai[d];
   }
 }
 
-// Backward step of panel k: x_k = Uinv_k b_k; rows above: b -= U x_k (in place).
+// Backward step of leaf k: x_k = Uinv_k b_k; rows above: b_r - U_r x_k (in place).
 template <typename T>
 __global__ __launch_bounds__(256) void bwd_step_kernel(const T* LU, int64_t lda, int64_t j0, int nb, const T* Uinv,
                                                        T* b, T* x, int nrhs) {
@@ -657,21 +753,27 @@ __global__ __launch_bounds__(256) void bwd_step_kernel(const T* LU, int64_t lda,
       x[2 * ((j0 + i) * nrhs + d)] = xs[2 * (i * 8 + d)];
       x[2 * ((j0 + i) * nrhs + d) + 1] = xs[2 * (i * 8 + d) + 1];
     }
-  const int64_t rbase = (int64_t)blockIdx.x * 128;
-  for (int idx = tid; idx < 128 * nrhs; idx += blockDim.x) {
-    const int lr = idx / nrhs, d = idx - lr * nrhs;
-    const int64_t r = rbase + lr;
-    if (r >= j0) continue;
-    T vr = b[2 * (r * nrhs + d)], vi = b[2 * (r * nrhs + d) + 1];
-    const T* U = LU + 2 * (r * lda + j0);
-    for (int q = 0; q < nb; ++q) {
-      const T ur = U[2 * q], ui = U[2 * q + 1];
+  const int64_t r = (int64_t)blockIdx.x * 256 + tid;
+  if (r >= j0) return;
+  T ar[8], ai[8];
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    ar[d] = d < nrhs ? b[2 * (r * nrhs + d)] : T(0);
+    ai[d] = d < nrhs ? b[2 * (r * nrhs + d) + 1] : T(0);
+  }
+  const T* U = LU + 2 * (r * lda + j0);
+  for (int q = 0; q < nb; ++q) {
+    const T ur = U[2 * q], ui = U[2 * q + 1];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
       const T xr = xs[2 * (q * 8 + d)], xi = xs[2 * (q * 8 + d) + 1];
-      vr -= ur * xr - ui * xi;
-      vi -= ur * xi + ui * xr;
+      ar[d] -= ur * xr - ui * xi;
+      ai[d] -= ur * xi + ui * xr;
     }
-    b[2 * (r * nrhs + d)] = vr;
-    b[2 * (r * nrhs + d) + 1] = vi;
+  }
+  for (int d = 0; d < nrhs; ++d) {
+    b[2 * (r * nrhs + d)] = ar[d];
+    b[2 * (r * nrhs + d) + 1] = ai[d];
   }
 }
 
@@ -750,7 +852,7 @@ __global__ void colnorm_final_kernel(const double* part, int nparts, int nrhs, d
 
 template <typename T>
 static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, LuWork& w, int& bar_base,
-                               hipStream_t s) {
+                               T* linv, hipStream_t s) {
   const int G = (int)((n - j0 + RPW - 1) / RPW);
   if (G > w.maxg) return hipErrorInvalidValue;
   PanelArgs<T> pa;
@@ -768,6 +870,7 @@ static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, 
   pa.slot_diag = reinterpret_cast<T*>(w.slot_diag);
   pa.maxg = w.maxg;
   pa.info = w.info;
+  pa.linv = linv;
   void* args[] = {&pa};
   hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G), dim3(64), args,
                                             0, s);
@@ -796,6 +899,11 @@ static hipError_t launch_laswp(T* A, int64_t lda, int64_t c0, int64_t c1, const 
 // Outer blocks of LU_BW columns factored as LU_NB-wide leaves (panel, swaps of
 // the block's other columns, leaf TRSM and in-block GEMM); the block's swaps,
 // TRSM and one K = LU_BW GEMM then update the trailing matrix.
+// Look-ahead: with a side stream, block J's update of the columns beyond the
+// next block runs there while the main stream updates the next block's columns
+// and factors it; the main stream waits for that far update before it updates
+// the block after. Leaf inverses alternate between two buffers by block parity,
+// so a far update still reading block J's inverses never sees block J+1's.
 template <typename T>
 hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_t s, int* panels_out) {
   if (nb > PNB) nb = PNB;
@@ -805,37 +913,64 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   if ((e = launch_fill<int>(w.slot_row, 4 * (int64_t)w.maxg, 0, s)) != hipSuccess) return e;
   int bar_base = 0, panels = 0;
   const T* cA = A;
-  for (int64_t J = 0; J < n; J += bw) {
-    const int64_t Wc = std::min<int64_t>(bw, n - J);
-    const int64_t Je = J + Wc;
+  T* linv = reinterpret_cast<T*>(w.linv);
+  const bool look_ahead = w.side != nullptr;
+  auto leaf_inv = [&](int64_t j0, int64_t J) {
+    return linv + 2 * ((((J / bw) & 1) * (bw / nb)) + (j0 - J) / nb) * PNB * PNB;
+  };
+  // block [J, Je)'s update of columns [c0, c1): swaps, U12 by leaf inverses, GEMM
+  auto update = [&](int64_t J, int64_t Je, int64_t c0, int64_t c1, hipStream_t st) -> hipError_t {
+    if (c1 <= c0) return hipSuccess;
+    hipError_t r;
+    if ((r = launch_laswp<T>(A, lda, c0, c1, w.ipiv, J, Je, st)) != hipSuccess) return r;
     for (int64_t j0 = J; j0 < Je; j0 += nb) {
       const int b = (int)std::min<int64_t>(nb, Je - j0);
-      if ((e = launch_panel<T>(A, n, lda, j0, b, w, bar_base, s)) != hipSuccess) return e;
+      r = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + c0), lda, A + 2 * (j0 * lda + c0), lda, b, c1 - c0,
+                      b, st);
+      if (r != hipSuccess) return r;
+      if (j0 + b < Je) {
+        r = gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + c0), lda,
+                        A + 2 * ((j0 + b) * lda + c0), lda, Je - j0 - b, c1 - c0, b, st);
+        if (r != hipSuccess) return r;
+      }
+    }
+    return gemm_sub<T>(cA + 2 * (Je * lda + J), lda, cA + 2 * (J * lda + c0), lda, A + 2 * (Je * lda + c0), lda,
+                       n - Je, c1 - c0, (int)(Je - J), st);
+  };
+  for (int64_t J = 0; J < n; J += bw) {
+    const int64_t Je = std::min<int64_t>(n, J + bw);
+    for (int64_t j0 = J; j0 < Je; j0 += nb) {
+      const int b = (int)std::min<int64_t>(nb, Je - j0);
+      if ((e = launch_panel<T>(A, n, lda, j0, b, w, bar_base, leaf_inv(j0, J), s)) != hipSuccess) return e;
       ++panels;
       if ((e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, s)) != hipSuccess) return e;
       if (j0 + b < Je) {
-        if ((e = launch_swap_trsm<T>(A, j0 + b, Je, lda, j0, b, w.ipiv, 1, s)) != hipSuccess) return e;
+        // U12 = L11^-1 P A12 on the block's remaining columns, then the in-block update
+        if ((e = launch_laswp<T>(A, lda, j0 + b, Je, w.ipiv, j0, j0 + b, s)) != hipSuccess) return e;
+        e = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + j0 + b), lda, A + 2 * (j0 * lda + j0 + b), lda, b,
+                        Je - j0 - b, b, s);
+        if (e != hipSuccess) return e;
         e = gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + j0 + b), lda,
                         A + 2 * ((j0 + b) * lda + j0 + b), lda, n - j0 - b, Je - j0 - b, b, s);
         if (e != hipSuccess) return e;
       }
     }
-    const int64_t rest = n - Je;
-    if (rest <= 0) continue;
-    if ((e = launch_laswp<T>(A, lda, Je, n, w.ipiv, J, Je, s)) != hipSuccess) return e;
-    for (int64_t j0 = J; j0 < Je; j0 += nb) {
-      const int b = (int)std::min<int64_t>(nb, Je - j0);
-      if ((e = launch_swap_trsm<T>(A, Je, n, lda, j0, b, w.ipiv, 0, s)) != hipSuccess) return e;
-      if (j0 + b < Je) {
-        e = gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + Je), lda,
-                        A + 2 * ((j0 + b) * lda + Je), lda, Je - j0 - b, rest, b, s);
-        if (e != hipSuccess) return e;
+    if (Je >= n) continue;
+    const int64_t Kn = std::min<int64_t>(n, Je + bw);
+    if (look_ahead) {
+      if ((e = hipEventRecord(w.ev_fact, s)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;  // previous block's far update
+      if (Kn < n) {
+        if ((e = hipStreamWaitEvent(w.side, w.ev_fact, 0)) != hipSuccess) return e;
+        if ((e = update(J, Je, Kn, n, w.side)) != hipSuccess) return e;
+        if ((e = hipEventRecord(w.ev_far, w.side)) != hipSuccess) return e;
       }
+      if ((e = update(J, Je, Je, Kn, s)) != hipSuccess) return e;
+    } else {
+      if ((e = update(J, Je, Je, n, s)) != hipSuccess) return e;
     }
-    e = gemm_sub<T>(cA + 2 * (Je * lda + J), lda, cA + 2 * (J * lda + Je), lda, A + 2 * (Je * lda + Je), lda, rest,
-                    rest, (int)Wc, s);
-    if (e != hipSuccess) return e;
   }
+  if (look_ahead && (e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;
   if (panels_out) *panels_out = panels;
   return hipSuccess;
 }
@@ -871,7 +1006,7 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, 
     const int64_t j0 = (int64_t)k * nb;
     const int b = (int)std::min<int64_t>(nb, n - j0);
     const int64_t below = n - j0 - b;
-    const unsigned G = (unsigned)std::max<int64_t>(1, (below + 127) / 128);
+    const unsigned G = (unsigned)std::max<int64_t>(1, (below + 255) / 256);
     hipLaunchKernelGGL((fwd_step_kernel<T>), dim3(G), dim3(256), 0, s, LU, n, lda, j0, b,
                        Linv + 2 * (int64_t)k * PNB * PNB, lrows + (int64_t)k * 2 * MAXBW,
                        lsrc + (int64_t)k * 2 * MAXBW, hcnt[k], (const T*)bin, bout, B, nrhs);
@@ -883,7 +1018,7 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, 
   for (int k = np - 1; k >= 0; --k) {
     const int64_t j0 = (int64_t)k * nb;
     const int b = (int)std::min<int64_t>(nb, n - j0);
-    const unsigned G = (unsigned)std::max<int64_t>(1, (j0 + 127) / 128);
+    const unsigned G = (unsigned)std::max<int64_t>(1, (j0 + 255) / 256);
     hipLaunchKernelGGL((bwd_step_kernel<T>), dim3(G), dim3(256), 0, s, LU, lda, j0, b,
                        Uinv + 2 * (int64_t)k * PNB * PNB, b0, B, nrhs);
     if ((e = hipGetLastError()) != hipSuccess) return e;
