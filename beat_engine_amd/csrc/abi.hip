@@ -215,7 +215,7 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   const int maxg = (int)((n + 63) / 64) + 1;
   const size_t slot_rows = (size_t)2 * maxg * 64 * 2 * sizeof(double);
   const size_t linv_bytes = (size_t)8 * 64 * 64 * 2 * sizeof(double);
-  const size_t bytes = sizeof(int) * (n + 8) + sizeof(double) * 2 * maxg + sizeof(long long) * 2 * maxg + slot_rows +
+  const size_t bytes = sizeof(int) * (n + 8) + sizeof(double) * 2 * maxg + sizeof(long long) * 4 * maxg + slot_rows +
                        2 * 64 * 2 * sizeof(double) + linv_bytes + 4096;
   HIPCHK(ensure(ctx->ipiv, bytes));
   char* p = reinterpret_cast<char*>(ctx->ipiv.p);
@@ -228,7 +228,7 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   w.bar = reinterpret_cast<int*>(take(sizeof(int) * 4));
   w.info = w.bar + 1;
   w.slot_val = reinterpret_cast<double*>(take(sizeof(double) * 2 * maxg));
-  w.slot_row = reinterpret_cast<int*>(take(sizeof(unsigned long long) * 2 * maxg));  // 64-bit slot tags
+  w.slot_row = reinterpret_cast<int*>(take(sizeof(unsigned long long) * 4 * maxg));  // slot words
   w.slot_rows = take(slot_rows);
   w.slot_diag = take(2 * 64 * 2 * sizeof(double));
   w.linv = take(linv_bytes);

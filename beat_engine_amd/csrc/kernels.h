@@ -83,7 +83,7 @@ struct LuWork {
   int* ipiv;          // n, global row index chosen at each column
   int* bar;           // grid barrier counter
   double* slot_val;   // 2 x maxg
-  int* slot_row;      // 2 x maxg 64-bit tag words (column tag << 32 | row)
+  int* slot_row;      // 2 x maxg slots of two 64-bit words (tag << 32 | row, tagged value)
   void* slot_rows;    // 2 x maxg x nb complex T
   void* slot_diag;    // 2 x nb complex T
   int* info;          // first zero pivot (1-based), 0 if none

@@ -153,8 +153,9 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
       row = (int)(r0 + lane);
     }
     wave_argmax(v, row);
-    // publish: value, candidate row content, diagonal row content, then the tag
-    if (lane == 0) st_relaxed(a.slot_val + par * a.maxg + g, v);
+    // publish: candidate row content and diagonal row content, then the slot:
+    // (tag << 32 | row) and the candidate value with the tag in its low mantissa
+    // bits (a relative change below 2^-36), so one poll returns both
     if (row >= 0 && lane < nb) {
       const int lr = (int)(row - r0);
       st_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + g) * PNB + lane), col[2 * (lane * LDR + lr)],
@@ -167,16 +168,26 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (lane == 0)
-      st_u64(tags + par * a.maxg + g, ((unsigned long long)tag << 32) | (unsigned long long)(unsigned)row);
+    if (lane == 0) {
+      unsigned long long* slot = tags + 2 * (par * a.maxg + g);
+      const unsigned long long vb = ((unsigned long long)__double_as_longlong(v) & ~0xffffull) | (tag & 0xffffu);
+      st_u64(slot + 1, vb);
+      st_u64(slot, ((unsigned long long)tag << 32) | (unsigned long long)(unsigned)row);
+    }
     // gather every slot of this column (lanes poll disjoint slots)
     v = -1.0;
     row = -1;
     for (int i = lane; i < G; i += 64) {
-      unsigned long long t;
-      while (((t = ld_u64(tags + par * a.maxg + i)) >> 32) != tag) __builtin_amdgcn_s_sleep(1);
+      const unsigned long long* slot = tags + 2 * (par * a.maxg + i);
+      unsigned long long t, vb;
+      do {
+        t = ld_u64(slot);
+        vb = ld_u64(slot + 1);
+        if ((t >> 32) == tag && (vb & 0xffffu) == (tag & 0xffffu)) break;
+        __builtin_amdgcn_s_sleep(1);
+      } while (true);
       const int ri = (int)(unsigned)(t & 0xffffffffu);
-      const double vi = ld_relaxed(a.slot_val + par * a.maxg + i);
+      const double vi = __longlong_as_double((long long)(vb & ~0xffffull));
       if (better(vi, ri, v, row)) {
         v = vi;
         row = ri;
@@ -910,7 +921,7 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   const int bw = MAXBW;
   hipError_t e;
   if ((e = launch_fill<int>(w.info, 1, 0, s)) != hipSuccess) return e;
-  if ((e = launch_fill<int>(w.slot_row, 4 * (int64_t)w.maxg, 0, s)) != hipSuccess) return e;
+  if ((e = launch_fill<int>(w.slot_row, 8 * (int64_t)w.maxg, 0, s)) != hipSuccess) return e;
   int bar_base = 0, panels = 0;
   const T* cA = A;
   T* linv = reinterpret_cast<T*>(w.linv);
