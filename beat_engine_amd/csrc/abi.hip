@@ -215,8 +215,8 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   const int maxg = (int)((n + 63) / 64) + 1;
   const size_t slot_rows = (size_t)2 * maxg * 64 * 2 * sizeof(double);
   const size_t linv_bytes = (size_t)8 * 64 * 64 * 2 * sizeof(double);
-  const size_t bytes = sizeof(int) * (n + 8) + sizeof(double) * 2 * maxg + sizeof(long long) * 4 * maxg + slot_rows +
-                       2 * 64 * 2 * sizeof(double) + linv_bytes + 4096;
+  const size_t bytes = sizeof(int) * (n + 8) + sizeof(long long) * 4 * maxg + slot_rows + 2 * 64 * 2 * sizeof(double) +
+                       linv_bytes + 4096;
   HIPCHK(ensure(ctx->ipiv, bytes));
   char* p = reinterpret_cast<char*>(ctx->ipiv.p);
   auto take = [&](size_t b) {
@@ -225,10 +225,8 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
     return r;
   };
   w.ipiv = reinterpret_cast<int*>(take(sizeof(int) * (n + 8)));
-  w.bar = reinterpret_cast<int*>(take(sizeof(int) * 4));
-  w.info = w.bar + 1;
-  w.slot_val = reinterpret_cast<double*>(take(sizeof(double) * 2 * maxg));
-  w.slot_row = reinterpret_cast<int*>(take(sizeof(unsigned long long) * 4 * maxg));  // slot words
+  w.info = reinterpret_cast<int*>(take(sizeof(int) * 4));
+  w.slots = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 4 * maxg));
   w.slot_rows = take(slot_rows);
   w.slot_diag = take(2 * 64 * 2 * sizeof(double));
   w.linv = take(linv_bytes);

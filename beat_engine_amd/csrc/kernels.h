@@ -81,9 +81,7 @@ hipError_t launch_downcast(const double* src, float* dst, int64_t n, hipStream_t
 // ---- dense LU (row-major complex, leading dimension lda) -------------------------
 struct LuWork {
   int* ipiv;          // n, global row index chosen at each column
-  int* bar;           // grid barrier counter
-  double* slot_val;   // 2 x maxg
-  int* slot_row;      // 2 x maxg slots of two 64-bit words (tag << 32 | row, tagged value)
+  unsigned long long* slots;  // 2 x maxg slots of two words (tag << 32 | row, tagged value)
   void* slot_rows;    // 2 x maxg x nb complex T
   void* slot_diag;    // 2 x nb complex T
   int* info;          // first zero pivot (1-based), 0 if none

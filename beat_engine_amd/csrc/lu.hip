@@ -35,10 +35,7 @@ struct PanelArgs {
   int64_t n, lda, j0;
   int nb;
   int* ipiv;
-  int* bar;
-  int bar_base;
-  double* slot_val;
-  int* slot_row;
+  unsigned long long* slots;  // 2 x maxg slots of two words: (tag << 32 | row), tagged value
   T* slot_rows;
   T* slot_diag;
   int maxg;
@@ -46,70 +43,70 @@ struct PanelArgs {
   T* linv;  // PNB x PNB row-major: inverse of the leaf's unit-lower diagonal block
 };
 
-__device__ __forceinline__ void st_relaxed(double* p, double v) {
-  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), __double_as_longlong(v), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double ld_relaxed(const double* p) {
-  return __longlong_as_double(__hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT));
-}
-__device__ __forceinline__ void st_relaxed(int* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int ld_relaxed(const int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// complex element store/load through 64-bit atomics (float complex = one word)
-__device__ __forceinline__ void st_cplx(float* p, float re, float im) {
-  unsigned long long v = ((unsigned long long)__float_as_uint(im) << 32) | __float_as_uint(re);
-  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void ld_cplx(const float* p, float& re, float& im) {
-  unsigned long long v =
-      __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  re = __uint_as_float((unsigned)(v & 0xffffffffu));
-  im = __uint_as_float((unsigned)(v >> 32));
-}
-__device__ __forceinline__ void st_cplx(double* p, double re, double im) {
-  st_relaxed(p, re);
-  st_relaxed(p + 1, im);
-}
-__device__ __forceinline__ void ld_cplx(const double* p, double& re, double& im) {
-  re = ld_relaxed(p);
-  im = ld_relaxed(p + 1);
-}
-
-__device__ __forceinline__ void grid_sync(int* bar, int target) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(bar, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) __builtin_amdgcn_s_sleep(1);
-  }
-  __syncthreads();
-}
-
-// Pivot candidate order: larger |re|+|im| first, then the smaller row index.
-__device__ __forceinline__ bool better(double v2, int r2, double v, int r) {
-  return r2 >= 0 && (r < 0 || v2 > v || (v2 == v && r2 < r));
-}
-__device__ __forceinline__ void wave_argmax(double& v, int& r) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const double v2 = __shfl_xor(v, o, 64);
-    const int r2 = __shfl_xor(r, o, 64);
-    if (better(v2, r2, v, r)) {
-      v = v2;
-      r = r2;
-    }
-  }
-}
-
 __device__ __forceinline__ void st_u64(unsigned long long* p, unsigned long long v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ unsigned long long ld_u64(const unsigned long long* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// complex element store/load through 64-bit atomics (float complex = one word)
+__device__ __forceinline__ void st_cplx(float* p, float re, float im) {
+  st_u64(reinterpret_cast<unsigned long long*>(p), ((unsigned long long)__float_as_uint(im) << 32) | __float_as_uint(re));
+}
+__device__ __forceinline__ void ld_cplx(const float* p, float& re, float& im) {
+  const unsigned long long v = ld_u64(reinterpret_cast<const unsigned long long*>(p));
+  re = __uint_as_float((unsigned)(v & 0xffffffffu));
+  im = __uint_as_float((unsigned)(v >> 32));
+}
+__device__ __forceinline__ void st_cplx(double* p, double re, double im) {
+  st_u64(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(re));
+  st_u64(reinterpret_cast<unsigned long long*>(p) + 1, (unsigned long long)__double_as_longlong(im));
+}
+__device__ __forceinline__ void ld_cplx(const double* p, double& re, double& im) {
+  re = __longlong_as_double((long long)ld_u64(reinterpret_cast<const unsigned long long*>(p)));
+  im = __longlong_as_double((long long)ld_u64(reinterpret_cast<const unsigned long long*>(p) + 1));
+}
+
+// Wavefront reductions of 32-bit keys (full wavefront active): DPP exchanges
+// within each row of 16 lanes (quad_perm [1,0,3,2], quad_perm [2,3,0,1],
+// row_half_mirror, row_mirror), then the four row results by readlane. The
+// result is wavefront-uniform.
+__device__ __forceinline__ unsigned umax32(unsigned a, unsigned b) { return a > b ? a : b; }
+__device__ __forceinline__ unsigned umin32(unsigned a, unsigned b) { return a < b ? a : b; }
+template <bool MAX>
+__device__ __forceinline__ unsigned wave_reduce_u32(unsigned x) {
+  auto op = [](unsigned a, unsigned b) { return MAX ? umax32(a, b) : umin32(a, b); };
+  x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false));
+  x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false));
+  x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, false));
+  x = op(x, (unsigned)__builtin_amdgcn_mov_dpp((int)x, 0x140, 0xF, 0xF, false));
+  const unsigned a = (unsigned)__builtin_amdgcn_readlane((int)x, 0);
+  const unsigned b = (unsigned)__builtin_amdgcn_readlane((int)x, 16);
+  const unsigned c = (unsigned)__builtin_amdgcn_readlane((int)x, 32);
+  const unsigned d = (unsigned)__builtin_amdgcn_readlane((int)x, 48);
+  return op(op(a, b), op(c, d));
+}
+
+// Pivot choice over the wavefront: the largest non-negative value v among the
+// lanes with valid set, ties to the smallest row (LAPACK's izamax takes the
+// first maximum). Compares the high words, then the low words of the tied
+// lanes, then takes the smallest row among exact ties. Returns row -1 when no
+// lane is valid.
+__device__ __forceinline__ void wave_pick(double v, int row, bool valid, double& vbest, int& rbest) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  const unsigned h = valid ? (unsigned)(bits >> 32) + 1u : 0u;
+  const unsigned H = wave_reduce_u32<true>(h);
+  const unsigned lo = h == H ? (unsigned)bits : 0u;
+  const unsigned Lo = wave_reduce_u32<true>(lo);
+  const bool tie = h == H && (unsigned)bits == Lo && valid;
+  const unsigned R = wave_reduce_u32<false>(tie ? (unsigned)row : 0xffffffffu);
+  rbest = H == 0 ? -1 : (int)R;
+  vbest = __longlong_as_double((long long)(((unsigned long long)(H - 1u) << 32) | Lo));
+}
+
+// Candidate order inside one lane: larger value first, then the smaller row.
+__device__ __forceinline__ bool better(double v2, int r2, double v, int r) {
+  return r2 >= 0 && (r < 0 || v2 > v || (v2 == v && r2 < r));
 }
 
 // One wavefront per workgroup; lane r owns panel row r0 + r. The panel block is
@@ -117,45 +114,54 @@ __device__ __forceinline__ unsigned long long ld_u64(const unsigned long long* p
 // column operations are bank-conflict free.
 //
 // Cross-workgroup exchange per column, without a shared counter: every
-// workgroup writes its candidate value and row content to its slot, waits for
-// those stores, then writes the slot's tag word (column tag << 32 | row). Readers
-// poll the tag words of all slots; a slot whose tag matches the current column
-// holds this column's data. Slots alternate between two buffers by column
-// parity; a workgroup can only overwrite a buffer after every workgroup has
-// published the following column, that is after all reads of the buffer ended.
-// Slot tags are cleared before each factorization, and the tag of a column is
-// its index + 1.
+// workgroup writes its candidate row content (and the diagonal row, if it owns
+// it) to its slot area, waits for those stores, then writes the slot's two
+// words: (column tag << 32 | row) and the candidate value with the tag in its
+// low 16 mantissa bits (a relative change below 2^-36). Readers poll both words
+// of every slot until both carry the column's tag. Slots alternate between two
+// buffers by column parity; a workgroup can only overwrite a buffer after every
+// workgroup has published the following column, that is after all reads of the
+// buffer ended. Slot words are cleared before each factorization, and the tag of
+// a column is its index + 1.
 template <typename T>
 __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
   constexpr int LDR = RPW + 1;
+  constexpr int U = 8;  // unroll of the LDS-bound loops (independent reads in flight)
   __shared__ T col[PNB * LDR * 2];
   __shared__ T piv[PNB * 2];
   const int lane = threadIdx.x, g = blockIdx.x, G = gridDim.x;
   const int64_t r0 = a.j0 + (int64_t)g * RPW;
   const int nr = (int)min((int64_t)RPW, a.n - r0);
   const int nb = a.nb;
-  unsigned long long* tags = reinterpret_cast<unsigned long long*>(a.slot_row);
-  for (int r = 0; r < nr; ++r)
-    if (lane < nb) {
-      const T* src = a.A + 2 * ((r0 + r) * a.lda + a.j0 + lane);
-      col[2 * (lane * LDR + r)] = src[0];
-      col[2 * (lane * LDR + r) + 1] = src[1];
-    }
+  // stage the block: lane = column, 16 rows in flight
+  for (int rb = 0; rb < nr; rb += 16) {
+    T vr[16], vi[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (rb + u < nr && lane < nb) {
+        const T* src = a.A + 2 * ((r0 + rb + u) * a.lda + a.j0 + lane);
+        vr[u] = src[0];
+        vi[u] = src[1];
+      }
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (rb + u < nr && lane < nb) {
+        col[2 * (lane * LDR + rb + u)] = vr[u];
+        col[2 * (lane * LDR + rb + u) + 1] = vi[u];
+      }
+  }
   __syncthreads();
   for (int jj = 0; jj < nb; ++jj) {
     const int64_t cg = a.j0 + jj;
     const unsigned tag = (unsigned)(cg + 1);
     const int par = jj & 1;
-    double v = -1.0;
-    int row = -1;
-    if (lane < nr && r0 + lane >= cg) {
-      v = fabs((double)col[2 * (jj * LDR + lane)]) + fabs((double)col[2 * (jj * LDR + lane) + 1]);
-      row = (int)(r0 + lane);
-    }
-    wave_argmax(v, row);
-    // publish: candidate row content and diagonal row content, then the slot:
-    // (tag << 32 | row) and the candidate value with the tag in its low mantissa
-    // bits (a relative change below 2^-36), so one poll returns both
+    const bool mine = lane < nr && r0 + lane >= cg;
+    const double mag =
+        mine ? fabs((double)col[2 * (jj * LDR + lane)]) + fabs((double)col[2 * (jj * LDR + lane) + 1]) : 0.0;
+    double v;
+    int row;
+    wave_pick(mag, (int)(r0 + lane), mine, v, row);
+    // publish the candidate row content and the diagonal row content, then the slot
     if (row >= 0 && lane < nb) {
       const int lr = (int)(row - r0);
       st_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + g) * PNB + lane), col[2 * (lane * LDR + lr)],
@@ -169,32 +175,46 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (lane == 0) {
-      unsigned long long* slot = tags + 2 * (par * a.maxg + g);
+      unsigned long long* slot = a.slots + 2 * (par * a.maxg + g);
       const unsigned long long vb = ((unsigned long long)__double_as_longlong(v) & ~0xffffull) | (tag & 0xffffu);
       st_u64(slot + 1, vb);
       st_u64(slot, ((unsigned long long)tag << 32) | (unsigned long long)(unsigned)row);
     }
-    // gather every slot of this column (lanes poll disjoint slots)
-    v = -1.0;
-    row = -1;
-    for (int i = lane; i < G; i += 64) {
-      const unsigned long long* slot = tags + 2 * (par * a.maxg + i);
-      unsigned long long t, vb;
-      do {
-        t = ld_u64(slot);
-        vb = ld_u64(slot + 1);
-        if ((t >> 32) == tag && (vb & 0xffffu) == (tag & 0xffffu)) break;
-        __builtin_amdgcn_s_sleep(1);
-      } while (true);
-      const int ri = (int)(unsigned)(t & 0xffffffffu);
-      const double vi = __longlong_as_double((long long)(vb & ~0xffffull));
-      if (better(vi, ri, v, row)) {
-        v = vi;
-        row = ri;
+    // gather every slot of this column: lanes poll disjoint slots, four slots'
+    // loads in flight per lane before any wait
+    double bv = -1.0;
+    int brow = -1;
+    for (int base = lane; base < G; base += 4 * 64) {
+      unsigned long long t[4], vb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        t[u] = vb[u] = 0;
+        if (base + 64 * u < G) {
+          const unsigned long long* slot = a.slots + 2 * (par * a.maxg + base + 64 * u);
+          t[u] = ld_u64(slot);
+          vb[u] = ld_u64(slot + 1);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (base + 64 * u >= G) break;
+        const unsigned long long* slot = a.slots + 2 * (par * a.maxg + base + 64 * u);
+        while ((t[u] >> 32) != tag || (vb[u] & 0xffffu) != (tag & 0xffffu)) {
+          __builtin_amdgcn_s_sleep(1);
+          t[u] = ld_u64(slot);
+          vb[u] = ld_u64(slot + 1);
+        }
+        const int ri = (int)(unsigned)(t[u] & 0xffffffffu);
+        const double vi = __longlong_as_double((long long)(vb[u] & ~0xffffull));
+        if (better(vi, ri, bv, brow)) {
+          bv = vi;
+          brow = ri;
+        }
       }
     }
-    wave_argmax(v, row);
-    const int p = row;
+    double pv;
+    int p;
+    wave_pick(bv, brow, brow >= 0, pv, p);
     const int pwg = (int)((p - a.j0) / RPW);
     T pr = 0, pim = 0;
     if (lane < nb) {
@@ -228,7 +248,23 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
       const T lr = ar * ir - ai * ii, li = ar * ii + ai * ir;
       col[2 * (jj * LDR + lane)] = lr;
       col[2 * (jj * LDR + lane) + 1] = li;
-      for (int c = jj + 1; c < nb; ++c) {
+      int c = jj + 1;
+      for (; c + U <= nb; c += U) {
+        T xr[U], xi[U], ur[U], ui[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          xr[u] = col[2 * ((c + u) * LDR + lane)];
+          xi[u] = col[2 * ((c + u) * LDR + lane) + 1];
+          ur[u] = piv[2 * (c + u)];
+          ui[u] = piv[2 * (c + u) + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          col[2 * ((c + u) * LDR + lane)] = xr[u] - (lr * ur[u] - li * ui[u]);
+          col[2 * ((c + u) * LDR + lane) + 1] = xi[u] - (lr * ui[u] + li * ur[u]);
+        }
+      }
+      for (; c < nb; ++c) {
         const T ur = piv[2 * c], ui = piv[2 * c + 1];
         col[2 * (c * LDR + lane)] -= lr * ur - li * ui;
         col[2 * (c * LDR + lane) + 1] -= lr * ui + li * ur;
@@ -244,21 +280,42 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
     }
   // The first workgroup holds the leaf's diagonal block (already stored back):
   // invert its unit-lower part in place, columns right to left as in LAPACK's
-  // trti2: X(i,j) = -(L(i,j) + sum_{k=j+1}^{i-1} X(i,k) L(k,j)), where X and L
-  // share col[][] (col[k][i] is entry (i,k)). Lane i owns row i.
+  // trti2. With the strict upper part of col cleared and its diagonal set to 1,
+  // lane i forms X(i,j) = -sum_{k=j+1}^{nb-1} col[k][i] L(k,j) over a uniform k
+  // range: col[k][i] holds X(i,k) for j < k < i, 1 for k = i and 0 for k > i,
+  // and L(k,j) = col[j][k] is still intact while column j is being formed.
   if (g == 0 && a.linv) {
     const int i = lane;
+    __syncthreads();
+    if (i < nb)
+      for (int c = i; c < nb; ++c) {
+        col[2 * (c * LDR + i)] = c == i ? T(1) : T(0);
+        col[2 * (c * LDR + i) + 1] = T(0);
+      }
+    __syncthreads();
     for (int j = nb - 2; j >= 0; --j) {
       T sr = 0, si = 0;
-      if (i > j && i < nb) {
-        sr = col[2 * (j * LDR + i)];
-        si = col[2 * (j * LDR + i) + 1];
-        for (int k = j + 1; k < i; ++k) {
-          const T xr = col[2 * (k * LDR + i)], xi = col[2 * (k * LDR + i) + 1];
-          const T lr = col[2 * (j * LDR + k)], li = col[2 * (j * LDR + k) + 1];
-          sr += xr * lr - xi * li;
-          si += xr * li + xi * lr;
+      int k = j + 1;
+      for (; k + U <= nb; k += U) {
+        T xr[U], xi[U], lr[U], li[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          xr[u] = col[2 * ((k + u) * LDR + i)];
+          xi[u] = col[2 * ((k + u) * LDR + i) + 1];
+          lr[u] = col[2 * (j * LDR + k + u)];
+          li[u] = col[2 * (j * LDR + k + u) + 1];
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          sr += xr[u] * lr[u] - xi[u] * li[u];
+          si += xr[u] * li[u] + xi[u] * lr[u];
+        }
+      }
+      for (; k < nb; ++k) {
+        const T xr = col[2 * (k * LDR + i)], xi = col[2 * (k * LDR + i) + 1];
+        const T lr = col[2 * (j * LDR + k)], li = col[2 * (j * LDR + k) + 1];
+        sr += xr * lr - xi * li;
+        si += xr * li + xi * lr;
       }
       __syncthreads();
       if (i > j && i < nb) {
@@ -269,9 +326,8 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
     }
     for (int r = 0; r < nb; ++r)
       if (lane < nb) {
-        const bool lower = lane < r;
-        a.linv[2 * (r * PNB + lane)] = lower ? col[2 * (lane * LDR + r)] : (lane == r ? T(1) : T(0));
-        a.linv[2 * (r * PNB + lane) + 1] = lower ? col[2 * (lane * LDR + r) + 1] : T(0);
+        a.linv[2 * (r * PNB + lane)] = col[2 * (lane * LDR + r)];
+        a.linv[2 * (r * PNB + lane) + 1] = col[2 * (lane * LDR + r) + 1];
       }
   }
 }
@@ -291,64 +347,6 @@ __global__ void laswp_cols_kernel(T* A, int64_t lda, int64_t c0, int64_t c1, con
     x[1] = y[1];
     y[0] = xr;
     y[1] = xi;
-  }
-}
-
-// Optional row swaps of the panel's pivots on columns [cb, ce), then
-// U12 = L11^-1 A12 for the panel's nb rows; one thread per column, 64 columns
-// per workgroup.
-template <typename T>
-__global__ __launch_bounds__(64) void swap_trsm_kernel(T* A, int64_t cb, int64_t ce, int64_t lda, int64_t j0, int nb,
-                                                       const int* ipiv, int do_swaps) {
-  __shared__ T L[PNB * PNB * 2];
-  __shared__ T X[PNB * 64 * 2];
-  const int tid = threadIdx.x;
-  const int64_t cc = cb + (int64_t)blockIdx.x * 64 + tid;
-  const bool act = cc < ce;
-  for (int idx = tid; idx < nb * nb; idx += 64) {
-    const int i = idx / nb, k = idx - i * nb;
-    L[2 * (i * PNB + k)] = A[2 * ((j0 + i) * lda + j0 + k)];
-    L[2 * (i * PNB + k) + 1] = A[2 * ((j0 + i) * lda + j0 + k) + 1];
-  }
-  if (act)
-    for (int i = 0; i < nb; ++i) {
-      X[2 * (i * 64 + tid)] = A[2 * ((j0 + i) * lda + cc)];
-      X[2 * (i * 64 + tid) + 1] = A[2 * ((j0 + i) * lda + cc) + 1];
-    }
-  __syncthreads();
-  if (!act) return;
-  for (int i = 0; do_swaps && i < nb; ++i) {
-    const int64_t p = ipiv[j0 + i];
-    if (p == j0 + i) continue;
-    const T xr = X[2 * (i * 64 + tid)], xi = X[2 * (i * 64 + tid) + 1];
-    if (p < j0 + nb) {
-      const int pi = (int)(p - j0);
-      X[2 * (i * 64 + tid)] = X[2 * (pi * 64 + tid)];
-      X[2 * (i * 64 + tid) + 1] = X[2 * (pi * 64 + tid) + 1];
-      X[2 * (pi * 64 + tid)] = xr;
-      X[2 * (pi * 64 + tid) + 1] = xi;
-    } else {
-      T* g = A + 2 * (p * lda + cc);
-      X[2 * (i * 64 + tid)] = g[0];
-      X[2 * (i * 64 + tid) + 1] = g[1];
-      g[0] = xr;
-      g[1] = xi;
-    }
-  }
-  for (int i = 1; i < nb; ++i) {
-    T sr = X[2 * (i * 64 + tid)], si = X[2 * (i * 64 + tid) + 1];
-    for (int k = 0; k < i; ++k) {
-      const T lr = L[2 * (i * PNB + k)], li = L[2 * (i * PNB + k) + 1];
-      const T xr = X[2 * (k * 64 + tid)], xi = X[2 * (k * 64 + tid) + 1];
-      sr -= lr * xr - li * xi;
-      si -= lr * xi + li * xr;
-    }
-    X[2 * (i * 64 + tid)] = sr;
-    X[2 * (i * 64 + tid) + 1] = si;
-  }
-  for (int i = 0; i < nb; ++i) {
-    A[2 * ((j0 + i) * lda + cc)] = X[2 * (i * 64 + tid)];
-    A[2 * ((j0 + i) * lda + cc) + 1] = X[2 * (i * 64 + tid) + 1];
   }
 }
 
@@ -862,8 +860,7 @@ __global__ void colnorm_final_kernel(const double* part, int nparts, int nrhs, d
 }  // namespace
 
 template <typename T>
-static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, LuWork& w, int& bar_base,
-                               T* linv, hipStream_t s) {
+static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, LuWork& w, T* linv, hipStream_t s) {
   const int G = (int)((n - j0 + RPW - 1) / RPW);
   if (G > w.maxg) return hipErrorInvalidValue;
   PanelArgs<T> pa;
@@ -873,29 +870,14 @@ static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, 
   pa.j0 = j0;
   pa.nb = b;
   pa.ipiv = w.ipiv;
-  pa.bar = w.bar;
-  pa.bar_base = bar_base;
-  pa.slot_val = w.slot_val;
-  pa.slot_row = w.slot_row;
+  pa.slots = w.slots;
   pa.slot_rows = reinterpret_cast<T*>(w.slot_rows);
   pa.slot_diag = reinterpret_cast<T*>(w.slot_diag);
   pa.maxg = w.maxg;
   pa.info = w.info;
   pa.linv = linv;
   void* args[] = {&pa};
-  hipError_t e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G), dim3(64), args,
-                                            0, s);
-  bar_base += G * b;
-  return e;
-}
-
-template <typename T>
-static hipError_t launch_swap_trsm(T* A, int64_t cb, int64_t ce, int64_t lda, int64_t j0, int b, const int* ipiv,
-                                   int do_swaps, hipStream_t s) {
-  if (ce <= cb) return hipSuccess;
-  hipLaunchKernelGGL((swap_trsm_kernel<T>), dim3((unsigned)((ce - cb + 63) / 64)), dim3(64), 0, s, A, cb, ce, lda, j0,
-                     b, ipiv, do_swaps);
-  return hipGetLastError();
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G), dim3(64), args, 0, s);
 }
 
 template <typename T>
@@ -921,8 +903,8 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   const int bw = MAXBW;
   hipError_t e;
   if ((e = launch_fill<int>(w.info, 1, 0, s)) != hipSuccess) return e;
-  if ((e = launch_fill<int>(w.slot_row, 8 * (int64_t)w.maxg, 0, s)) != hipSuccess) return e;
-  int bar_base = 0, panels = 0;
+  if ((e = launch_fill<int>(reinterpret_cast<int*>(w.slots), 8 * (int64_t)w.maxg, 0, s)) != hipSuccess) return e;
+  int panels = 0;
   const T* cA = A;
   T* linv = reinterpret_cast<T*>(w.linv);
   const bool look_ahead = w.side != nullptr;
@@ -952,7 +934,7 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
     const int64_t Je = std::min<int64_t>(n, J + bw);
     for (int64_t j0 = J; j0 < Je; j0 += nb) {
       const int b = (int)std::min<int64_t>(nb, Je - j0);
-      if ((e = launch_panel<T>(A, n, lda, j0, b, w, bar_base, leaf_inv(j0, J), s)) != hipSuccess) return e;
+      if ((e = launch_panel<T>(A, n, lda, j0, b, w, leaf_inv(j0, J), s)) != hipSuccess) return e;
       ++panels;
       if ((e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, s)) != hipSuccess) return e;
       if (j0 + b < Je) {
