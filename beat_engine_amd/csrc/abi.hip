@@ -239,12 +239,14 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
 }
 
 // Solve with LU factors for every column of the n x nrhs row-major B (in place).
+// prepare: first solve after a factorization (builds the per-factorization solve
+// data in ctx->work; later solves with the same factors pass false).
 template <typename T>
-int solve_all(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, int nrhs) {
-  const int nc0 = std::min(nrhs, SOLVE_CHUNK);
-  HIPCHK(ensure(ctx->work, lu_solve_work_bytes(n, nc0, LU_NB, sizeof(T))));
+int solve_all(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, int nrhs, bool prepare) {
+  HIPCHK(ensure(ctx->work, lu_solve_work_bytes(n, sizeof(T))));
+  if (prepare) HIPCHK(lu_solve_prepare<T>(LU, n, lda, ipiv, as<T>(ctx->work), ctx->stream));
   if (nrhs <= SOLVE_CHUNK) {
-    HIPCHK(lu_solve<T>(LU, n, lda, ipiv, B, nrhs, as<T>(ctx->work), LU_NB, ctx->stream));
+    HIPCHK(lu_solve<T>(LU, n, lda, B, nrhs, as<T>(ctx->work), ctx->stream));
     return BEAT_OK;
   }
   HIPCHK(ensure(ctx->chunk, sizeof(T) * 2 * n * SOLVE_CHUNK));
@@ -253,7 +255,7 @@ int solve_all(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const int* ipi
     hipLaunchKernelGGL((extract_cols_kernel<T>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream, (const T*)B, n,
                        nrhs, c0, nc, as<T>(ctx->chunk));
     HIPCHK(hipGetLastError());
-    HIPCHK(lu_solve<T>(LU, n, lda, ipiv, as<T>(ctx->chunk), nc, as<T>(ctx->work), LU_NB, ctx->stream));
+    HIPCHK(lu_solve<T>(LU, n, lda, as<T>(ctx->chunk), nc, as<T>(ctx->work), ctx->stream));
     hipLaunchKernelGGL((insert_cols_kernel<T>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream,
                        (const T*)as<T>(ctx->chunk), n, nrhs, c0, nc, B);
     HIPCHK(hipGetLastError());
@@ -348,7 +350,7 @@ int refine_loop(beat_ctx* ctx, const float* LU32, int64_t lda32, const int* ipiv
     hipLaunchKernelGGL(masked_downcast_kernel, dim3(blocks_for(n * nrhs)), dim3(256), 0, ctx->stream, (const double*)r,
                        d, n, nrhs, (const int*)dmask);
     HIPCHK(hipGetLastError());
-    if (solve_all<float>(ctx, LU32, n, lda32, ipiv, d, nrhs) != BEAT_OK) return BEAT_ERR_HIP;
+    if (solve_all<float>(ctx, LU32, n, lda32, ipiv, d, nrhs, false) != BEAT_OK) return BEAT_ERR_HIP;
     hipLaunchKernelGGL(add_delta_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, ctx->stream, (const double*)x64,
                        (const float*)d, nx, 2 * n * nrhs);
     HIPCHK(hipGetLastError());
@@ -544,7 +546,7 @@ int solve_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, doubl
     HIPCHK(hipEventRecord(ctx->ev[1], s));
     HIPCHK(lu_factor<double>(as<double>(ctx->a64), n, lda, lw, LU_NB, s, &panels));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    rc = solve_all<double>(ctx, as<double>(ctx->a64), n, lda, lw.ipiv, as<double>(ctx->b64), nrhs);
+    rc = solve_all<double>(ctx, as<double>(ctx->a64), n, lda, lw.ipiv, as<double>(ctx->b64), nrhs, true);
     if (rc) return rc;
     hipLaunchKernelGGL((scatter_p_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
                        (const double*)as<double>(ctx->b64), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
@@ -560,7 +562,7 @@ int solve_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, doubl
     HIPCHK(hipEventRecord(ctx->ev[1], s));
     HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), nrhs);
+    rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), nrhs, true);
     if (rc) return rc;
     hipLaunchKernelGGL((scatter_p_kernel<float>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
                        (const float*)as<float>(ctx->b32), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
@@ -579,7 +581,7 @@ int solve_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, doubl
     HIPCHK(launch_downcast(as<double>(ctx->b64), as<float>(ctx->b32), 2 * n * nrhs, s));
     HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), nrhs);
+    rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), nrhs, true);
     if (rc) return rc;
     hipLaunchKernelGGL(upcast_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, s, (const float*)as<float>(ctx->b32),
                        as<double>(ctx->x64), 2 * n * nrhs);
@@ -784,7 +786,7 @@ int beat_bm_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k,
     HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n * nrhs));
     if (int rc = assemble<double>(ctx, m, k, er, ei, as<double>(ctx->q64), as<double>(ctx->q64), (int)nrhs,
                                   as<double>(ctx->a64), lda, as<double>(ctx->b64)))
-      return rc;
+    return rc;
     HIPCHK(hipMemcpyAsync(A.data(), ctx->a64.p, sizeof(double) * A.size(), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(B.data(), ctx->b64.p, sizeof(double) * B.size(), hipMemcpyDeviceToHost, s));
   } else {
@@ -794,7 +796,7 @@ int beat_bm_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k,
     HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * n * nrhs));
     if (int rc = assemble<float>(ctx, m, k, er, ei, as<double>(ctx->q64), as<float>(ctx->q32), (int)nrhs,
                                  as<float>(ctx->a32), lda, as<float>(ctx->b32)))
-      return rc;
+    return rc;
     std::vector<float> A32(A.size()), B32(B.size());
     HIPCHK(hipMemcpyAsync(A32.data(), ctx->a32.p, sizeof(float) * A32.size(), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(B32.data(), ctx->b32.p, sizeof(float) * B32.size(), hipMemcpyDeviceToHost, s));
@@ -947,7 +949,8 @@ int beat_dense_lu_solve(beat_ctx* ctx, int precision, const double* a, int64_t n
                        (const double*)bin, n, (int)nrhs, as<double>(ctx->b64));
     HIPCHK(hipGetLastError());
     HIPCHK(lu_factor<double>(as<double>(ctx->a64), n, lda, lw, LU_NB, s, &panels));
-    if (int rc = solve_all<double>(ctx, as<double>(ctx->a64), n, lda, lw.ipiv, as<double>(ctx->b64), (int)nrhs)) return rc;
+    if (int rc = solve_all<double>(ctx, as<double>(ctx->a64), n, lda, lw.ipiv, as<double>(ctx->b64), (int)nrhs, true))
+    return rc;
     hipLaunchKernelGGL((cols_out_kernel<double, double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
                        (const double*)as<double>(ctx->b64), n, (int)nrhs, as<double>(ctx->pout));
   } else {
@@ -959,7 +962,8 @@ int beat_dense_lu_solve(beat_ctx* ctx, int precision, const double* a, int64_t n
                        n, (int)nrhs, as<float>(ctx->b32));
     HIPCHK(hipGetLastError());
     HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
-    if (int rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), (int)nrhs)) return rc;
+    if (int rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), (int)nrhs, true))
+    return rc;
     hipLaunchKernelGGL((cols_out_kernel<float, double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
                        (const float*)as<float>(ctx->b32), n, (int)nrhs, as<double>(ctx->pout));
   }
@@ -1014,7 +1018,8 @@ int beat_refine_lu(beat_ctx* ctx, const float* matrix, int64_t n, const float* r
   if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
   int panels = 0;
   HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
-  if (int rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), (int)nrhs)) return rc;
+  if (int rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), (int)nrhs, true))
+    return rc;
   hipLaunchKernelGGL(upcast_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, s, (const float*)as<float>(ctx->b32),
                      as<double>(ctx->x64), 2 * n * nrhs);
   HIPCHK(hipGetLastError());

@@ -94,16 +94,20 @@ struct LuWork {
 template <typename T>
 hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_t s, int* panels_out);
 
-// Forward/backward solves with the factors of lu_factor; B is n x nrhs row-major
-// complex T and is overwritten by the solution. work holds 2 n nrhs complex T.
+// Forward/backward solves with the factors of lu_factor. lu_solve_prepare runs
+// once per factorization (diagonal-block inverses, row gather tables) into work
+// (lu_solve_work_bytes); lu_solve then overwrites B (n x nrhs row-major complex
+// T, nrhs <= 8) with the solution. Solves sharing one work area run in stream
+// order.
 template <typename T>
-hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, int nrhs, T* work, int nb,
-                    hipStream_t s);
+hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* work, hipStream_t s);
+template <typename T>
+hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, hipStream_t s);
+size_t lu_solve_work_bytes(int64_t n, size_t elem);
 
 // r = b - A x (complex128), A row-major n x n, x and b n x nrhs row-major.
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
                       hipStream_t s);
-size_t lu_solve_work_bytes(int64_t n, int nrhs, int nb, size_t elem);
 // per-column 2-norms of an n x nrhs complex128 row-major matrix into out[nrhs]
 // (out holds nrhs norms followed by 64*nrhs partial sums)
 hipError_t column_norms64(const double* X, int64_t n, int nrhs, double* out, hipStream_t s);

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "bem_common.h"
@@ -607,226 +608,258 @@ __global__ __launch_bounds__(64) void diag_inverse_kernel(const T* LU, int64_t n
   }
 }
 
-// Swap list of leaf k (rows whose content moves, and the row each takes it
-// from; the leaf's own rows first). The first leaf of each outer block carries
-// the swaps of the whole block (its L columns are stored in post-block row
-// order); the other leaves of the block carry none.
-__global__ void swap_lists_kernel(const int* ipiv, int64_t n, int nb, int bw, int nleaves, int* rows, int* src,
-                                  int* cnt) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nleaves) return;
-  const int64_t j0 = (int64_t)k * nb;
-  int* R = rows + (int64_t)k * 2 * MAXBW;
-  int* S = src + (int64_t)k * 2 * MAXBW;
-  int m = 0;
-  const bool first = (j0 % bw) == 0;
-  const int span = (int)min((int64_t)(first ? bw : nb), n - j0);
-  for (int i = 0; i < span; ++i) {
-    R[m] = (int)(j0 + i);
-    S[m] = (int)(j0 + i);
-    ++m;
-  }
-  if (first)
-    for (int i = 0; i < span; ++i) {
-      const int a = (int)(j0 + i), b = ipiv[j0 + i];
-      if (a == b) continue;
-      const int ia = i;
-      int ib = -1;
-      if (b < j0 + span) ib = (int)(b - j0);
-      else
-        for (int t = span; t < m; ++t)
-          if (R[t] == b) ib = t;
-      if (ib < 0) {
-        R[m] = b;
-        S[m] = b;
-        ib = m++;
-      }
-      const int tmp = S[ia];
-      S[ia] = S[ib];
-      S[ib] = tmp;
-    }
-  // rows below the leaf's own rows, sorted for binary search
-  const int own = (int)min((int64_t)nb, n - j0);
-  for (int t = own + 1; t < m; ++t) {
-    const int r = R[t], v = S[t];
-    int u = t - 1;
-    while (u >= own && R[u] > r) {
-      R[u + 1] = R[u];
-      S[u + 1] = S[u];
-      --u;
-    }
-    R[u + 1] = r;
-    S[u + 1] = v;
-  }
-  cnt[k] = m;
-}
-
-// Forward step of leaf k: y = Linv_k (P b)_top, then b_r - L_r y for the rows
-// below; one thread per row (256 rows per workgroup), y in LDS. Reads bin and
-// writes bout (rows >= j0); y is also written to bkeep.
-template <typename T>
-__global__ __launch_bounds__(256) void fwd_step_kernel(const T* LU, int64_t n, int64_t lda, int64_t j0, int nb,
-                                                       const T* Linv, const int* lrows, const int* lsrc, int lcnt,
-                                                       const T* bin, T* bout, T* bkeep, int nrhs) {
-  __shared__ T top[PNB * 8 * 2];
-  __shared__ T y[PNB * 8 * 2];
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
-    const int i = idx / nrhs, d = idx - i * nrhs;
-    const int64_t sr = lsrc[i];  // the leaf's own rows come first in the list
-    top[2 * (i * 8 + d)] = bin[2 * (sr * nrhs + d)];
-    top[2 * (i * 8 + d) + 1] = bin[2 * (sr * nrhs + d) + 1];
-  }
-  __syncthreads();
-  for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
-    const int i = idx / nrhs, d = idx - i * nrhs;
-    T sr = 0, si = 0;
-    for (int q = 0; q < nb; ++q) {
-      const T lr = Linv[2 * (i * PNB + q)], li = Linv[2 * (i * PNB + q) + 1];
-      const T br = top[2 * (q * 8 + d)], bi = top[2 * (q * 8 + d) + 1];
-      sr += lr * br - li * bi;
-      si += lr * bi + li * br;
-    }
-    y[2 * (i * 8 + d)] = sr;
-    y[2 * (i * 8 + d) + 1] = si;
-  }
-  __syncthreads();
-  if (blockIdx.x == 0)
-    for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
-      const int i = idx / nrhs, d = idx - i * nrhs;
-      const int64_t r = j0 + i;
-      bout[2 * (r * nrhs + d)] = y[2 * (i * 8 + d)];
-      bout[2 * (r * nrhs + d) + 1] = y[2 * (i * 8 + d) + 1];
-      bkeep[2 * (r * nrhs + d)] = y[2 * (i * 8 + d)];
-      bkeep[2 * (r * nrhs + d) + 1] = y[2 * (i * 8 + d) + 1];
-    }
-  const int64_t r = j0 + nb + (int64_t)blockIdx.x * 256 + tid;
+// Row gather tables of the forward solve. The L columns of outer block J are
+// stored in the row order after block J's own exchanges (later blocks never
+// swap them), while the solve runs in the final order. For final row r,
+// G[J][r] is the row it occupied after block J's exchanges, found by undoing the
+// exchanges of blocks J+1.. in reverse; perm[r] is its original row.
+__global__ void gather_table_kernel(const int* ipiv, int64_t n, int bw, int nJ, int* G, int* perm) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
-  // source row of r after the leaf's swaps: binary search in the sorted extras
-  int64_t src = r;
-  int lo = nb, hi = lcnt - 1;
-  while (lo <= hi) {
-    const int mid = (lo + hi) >> 1;
-    const int rm = lrows[mid];
-    if (rm == r) {
-      src = lsrc[mid];
-      break;
-    }
-    if (rm < r) lo = mid + 1;
-    else hi = mid - 1;
-  }
-  T ar[8], ai[8];
-#pragma unroll
-  for (int d = 0; d < 8; ++d) {
-    ar[d] = d < nrhs ? bin[2 * (src * nrhs + d)] : T(0);
-    ai[d] = d < nrhs ? bin[2 * (src * nrhs + d) + 1] : T(0);
-  }
-  const T* L = LU + 2 * (r * lda + j0);
-  for (int q = 0; q < nb; ++q) {
-    const T lr = L[2 * q], li = L[2 * q + 1];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      const T yr = y[2 * (q * 8 + d)], yi = y[2 * (q * 8 + d) + 1];
-      ar[d] -= lr * yr - li * yi;
-      ai[d] -= lr * yi + li * yr;
+  int q = (int)r;
+  for (int J = nJ - 1; J >= 0; --J) {
+    G[(int64_t)J * n + r] = q;
+    const int64_t k0 = (int64_t)J * bw, k1 = min(n, k0 + bw);
+    for (int64_t k = k1 - 1; k >= k0; --k) {
+      const int p = ipiv[k];
+      if (q == (int)k) q = p;
+      else if (q == p) q = (int)k;
     }
   }
-  for (int d = 0; d < nrhs; ++d) {
-    bout[2 * (r * nrhs + d)] = ar[d];
-    bout[2 * (r * nrhs + d) + 1] = ai[d];
-  }
+  perm[r] = q;
 }
 
-// Backward step of leaf k: x_k = Uinv_k b_k; rows above: b_r - U_r x_k (in place).
+// dst (n x 8, zero-padded) = rows perm[r] of src (n x nrhs)
 template <typename T>
-__global__ __launch_bounds__(256) void bwd_step_kernel(const T* LU, int64_t lda, int64_t j0, int nb, const T* Uinv,
-                                                       T* b, T* x, int nrhs) {
-  __shared__ T xs[PNB * 8 * 2];
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
-    const int i = idx / nrhs, d = idx - i * nrhs;
-    T sr = 0, si = 0;
-    for (int q = 0; q < nb; ++q) {
-      const T ur = Uinv[2 * (i * PNB + q)], ui = Uinv[2 * (i * PNB + q) + 1];
-      const T br = b[2 * ((j0 + q) * nrhs + d)], bi = b[2 * ((j0 + q) * nrhs + d) + 1];
-      sr += ur * br - ui * bi;
-      si += ur * bi + ui * br;
-    }
-    xs[2 * (i * 8 + d)] = sr;
-    xs[2 * (i * 8 + d) + 1] = si;
+__global__ void gather_rhs_kernel(const T* src, int64_t n, int nrhs, const int* perm, T* dst) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * 8) return;
+  const int64_t r = idx >> 3;
+  const int d = (int)(idx & 7);
+  T re = 0, im = 0;
+  if (d < nrhs) {
+    const int64_t s = perm[r];
+    re = src[2 * (s * nrhs + d)];
+    im = src[2 * (s * nrhs + d) + 1];
   }
-  __syncthreads();
-  if (blockIdx.x == 0)
-    for (int idx = tid; idx < nb * nrhs; idx += blockDim.x) {
-      const int i = idx / nrhs, d = idx - i * nrhs;
-      x[2 * ((j0 + i) * nrhs + d)] = xs[2 * (i * 8 + d)];
-      x[2 * ((j0 + i) * nrhs + d) + 1] = xs[2 * (i * 8 + d) + 1];
-    }
-  const int64_t r = (int64_t)blockIdx.x * 256 + tid;
-  if (r >= j0) return;
-  T ar[8], ai[8];
-#pragma unroll
-  for (int d = 0; d < 8; ++d) {
-    ar[d] = d < nrhs ? b[2 * (r * nrhs + d)] : T(0);
-    ai[d] = d < nrhs ? b[2 * (r * nrhs + d) + 1] : T(0);
-  }
-  const T* U = LU + 2 * (r * lda + j0);
-  for (int q = 0; q < nb; ++q) {
-    const T ur = U[2 * q], ui = U[2 * q + 1];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      const T xr = xs[2 * (q * 8 + d)], xi = xs[2 * (q * 8 + d) + 1];
-      ar[d] -= ur * xr - ui * xi;
-      ai[d] -= ur * xi + ui * xr;
-    }
-  }
-  for (int d = 0; d < nrhs; ++d) {
-    b[2 * (r * nrhs + d)] = ar[d];
-    b[2 * (r * nrhs + d) + 1] = ai[d];
-  }
+  dst[2 * idx] = re;
+  dst[2 * idx + 1] = im;
 }
+
+__device__ __forceinline__ void st_flag(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ int ld_flag(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 template <typename T>
-__global__ void copy_kernel(const T* src, T* dst, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
-}
+struct TriArgs {
+  const T* LU;
+  int64_t n, lda;
+  const T* dinv;      // per leaf PNB x PNB: Linv (forward) or Uinv (backward)
+  const int* gather;  // forward: outer-block row gather tables (nJ x n)
+  int bw;             // outer block width of the factorization
+  const T* src;       // n x 8: forward P b, backward y
+  T* pub;             // n x 8: block results, written with coherent stores for the other workgroups
+  int* flags;         // per leaf: epoch once the leaf's result is published
+  int epoch;
+  int np;
+  T* out;             // backward: caller's n x nrhs solution
+  int nrhs;
+};
 
-// r = b - A x, complex128; 32 rows x up to 8 right-hand sides per workgroup,
-// x staged through LDS in chunks of 64 rows.
-__global__ __launch_bounds__(256) void residual64_kernel(const double* A, int64_t n, int64_t lda, const double* x,
-                                                         const double* b, double* r, int nrhs) {
-  __shared__ double xs[64 * 8 * 2];
-  const int tid = threadIdx.x;
-  const int rl = tid >> 3, d = tid & 7;
-  const int64_t row = (int64_t)blockIdx.x * 32 + rl;
-  const bool act = row < n && d < nrhs;
-  double sr = 0, si = 0;
-  for (int64_t k0 = 0; k0 < n; k0 += 64) {
-    const int kc = (int)min((int64_t)64, n - k0);
+// Persistent block triangular solve with 8 right-hand sides (zero-padded): one
+// workgroup per leaf of PNB rows, all resident (cooperative launch). Forward
+// (L y = P b) workgroup i consumes the results of leaves 0..i-1 in order as
+// their flags appear, backward (U x = y) those of leaves np-1..i+1; each adds
+// -M_ik z_k to its rows, then forms its own result with the leaf's diagonal
+// inverse and publishes it. Thread t owns row t/4 and the column quarter t%4 of
+// every PNB-wide block (partial sums reduced over the four threads once per
+// leaf). The matrix segment of the next block is loaded before its flag is
+// awaited, so only the flag latency and the block product sit on the chain.
+template <typename T, bool FWD>
+__global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
+  constexpr int YS = PNB * 8 + 6;  // staged block result; 2-complex pad per 16-row group
+  __shared__ T ys[2 * YS];
+  __shared__ T bs[2 * PNB * 8];
+  const int t = threadIdx.x, rl = t >> 2, kq = t & 3;
+  const int i = FWD ? (int)blockIdx.x : a.np - 1 - (int)blockIdx.x;
+  const int64_t base = (int64_t)i * PNB;
+  const int nbi = (int)min((int64_t)PNB, a.n - base);
+  const int64_t r = base + rl;
+  const bool rowok = rl < nbi;
+  // this thread's quarter of its row of the diagonal-block inverse
+  T dr[16], di[16];
+  const T* D = a.dinv + 2 * ((int64_t)i * PNB * PNB + (int64_t)rl * PNB + 16 * kq);
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const bool ok = rowok && 16 * kq + u < nbi;
+    dr[u] = ok ? D[2 * u] : T(0);
+    di[u] = ok ? D[2 * u + 1] : T(0);
+  }
+  T accr[8], acci[8];
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    accr[d] = (kq == 0 && rowok) ? a.src[2 * (r * 8 + d)] : T(0);
+    acci[d] = (kq == 0 && rowok) ? a.src[2 * (r * 8 + d) + 1] : T(0);
+  }
+  const int nk = FWD ? i : a.np - 1 - i;
+  for (int s = 0; s < nk; ++s) {
+    const int k = FWD ? s : a.np - 1 - s;
+    const int64_t kb = (int64_t)k * PNB;
+    const int nbk = (int)min((int64_t)PNB, a.n - kb);
+    T mr[16], mi[16];
+    {
+      const int64_t srow = FWD ? (rowok ? (int64_t)a.gather[(kb / a.bw) * a.n + r] : 0) : r;
+      const T* M = a.LU + 2 * (srow * a.lda + kb + 16 * kq);
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const bool ok = rowok && 16 * kq + u < nbk;
+        mr[u] = ok ? M[2 * u] : T(0);
+        mi[u] = ok ? M[2 * u + 1] : T(0);
+      }
+    }
+    if (t == 0)
+      while (ld_flag(a.flags + k) != a.epoch) __builtin_amdgcn_s_sleep(1);
     __syncthreads();
-    for (int idx = tid; idx < kc * nrhs; idx += 256) {
-      const int kk = idx / nrhs, dd = idx - kk * nrhs;
-      xs[2 * (kk * 8 + dd)] = x[2 * ((k0 + kk) * nrhs + dd)];
-      xs[2 * (kk * 8 + dd) + 1] = x[2 * ((k0 + kk) * nrhs + dd) + 1];
+    for (int idx = t; idx < PNB * 8; idx += 256) {
+      const int kk = idx >> 3, d = idx & 7;
+      T zr = 0, zi = 0;
+      if (kk < nbk) ld_cplx(a.pub + 2 * ((kb + kk) * 8 + d), zr, zi);
+      ys[2 * (idx + (kk >> 4) * 2)] = zr;
+      ys[2 * (idx + (kk >> 4) * 2) + 1] = zi;
     }
     __syncthreads();
-    if (act) {
-      const double* a = A + 2 * (row * lda + k0);
-      for (int kk = 0; kk < kc; ++kk) {
-        const double ar = a[2 * kk], ai = a[2 * kk + 1];
-        const double xr = xs[2 * (kk * 8 + d)], xi = xs[2 * (kk * 8 + d) + 1];
-        sr = fma(ar, xr, sr);
-        sr = fma(-ai, xi, sr);
-        si = fma(ar, xi, si);
-        si = fma(ai, xr, si);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int o = (16 * kq + u) * 8 + kq * 2;
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        const T zr = ys[2 * (o + d)], zi = ys[2 * (o + d) + 1];
+        accr[d] -= mr[u] * zr - mi[u] * zi;
+        acci[d] -= mr[u] * zi + mi[u] * zr;
       }
     }
   }
-  if (act) {
-    r[2 * (row * nrhs + d)] = b[2 * (row * nrhs + d)] - sr;
-    r[2 * (row * nrhs + d) + 1] = b[2 * (row * nrhs + d) + 1] - si;
+  // full row sums over the four threads of the row, then the diagonal block
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    accr[d] += __shfl_xor(accr[d], 1, 64);
+    acci[d] += __shfl_xor(acci[d], 1, 64);
+    accr[d] += __shfl_xor(accr[d], 2, 64);
+    acci[d] += __shfl_xor(acci[d], 2, 64);
   }
+  if (kq == 0)
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      bs[2 * (rl * 8 + d)] = accr[d];
+      bs[2 * (rl * 8 + d) + 1] = acci[d];
+    }
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < 8; ++d) accr[d] = acci[d] = 0;
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int q = 16 * kq + u;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+      const T br = bs[2 * (q * 8 + d)], bi = bs[2 * (q * 8 + d) + 1];
+      accr[d] += dr[u] * br - di[u] * bi;
+      acci[d] += dr[u] * bi + di[u] * br;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    accr[d] += __shfl_xor(accr[d], 1, 64);
+    acci[d] += __shfl_xor(acci[d], 1, 64);
+    accr[d] += __shfl_xor(accr[d], 2, 64);
+    acci[d] += __shfl_xor(acci[d], 2, 64);
+  }
+  // publish: thread kq stores columns 2kq, 2kq+1 of its row
+  if (rowok)
+#pragma unroll
+    for (int d = 0; d < 8; ++d)
+      if ((d >> 1) == kq) {
+        st_cplx(a.pub + 2 * (r * 8 + d), accr[d], acci[d]);
+        if (!FWD && d < a.nrhs) {
+          a.out[2 * (r * a.nrhs + d)] = accr[d];
+          a.out[2 * (r * a.nrhs + d) + 1] = acci[d];
+        }
+      }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (t == 0) st_flag(a.flags + i, a.epoch);
+}
+
+// r = b - A x, complex128, 8 right-hand sides (zero-padded). Reading A once is
+// the cost, so lanes run along the row (coalesced 1 KB per wavefront load): each
+// wavefront accumulates 4 rows x 8 right-hand sides over k = lane (mod 64), with
+// x staged in LDS ([d][k], conflict-free) in chunks shared by the workgroup's 8
+// wavefronts (32 rows). The 64 partial sums per lane are then reduced across the
+// wavefront by recursive halving (63 exchanges), leaving sum number l in lane l.
+constexpr int RES_WAVES = 8, RES_ROWS = 4, RES_KC = 256;
+
+template <int N>
+__device__ __forceinline__ void halve_step(double* v, int lane) {
+  const bool hi = (lane & N) != 0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const double keep = hi ? v[N + j] : v[j];
+    const double send = hi ? v[j] : v[N + j];
+    v[j] = keep + __shfl_xor(send, N, 64);
+  }
+}
+
+__global__ __launch_bounds__(512) void residual64_kernel(const double* A, int64_t n, int64_t lda, const double* x,
+                                                         const double* b, double* r, int nrhs) {
+  __shared__ double xs[8 * RES_KC * 2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t row0 = ((int64_t)blockIdx.x * RES_WAVES + wave) * RES_ROWS;
+  double acc[RES_ROWS * 8 * 2];
+#pragma unroll
+  for (int j = 0; j < RES_ROWS * 8 * 2; ++j) acc[j] = 0;
+  for (int64_t k0 = 0; k0 < n; k0 += RES_KC) {
+    const int kc = (int)min((int64_t)RES_KC, n - k0);
+    __syncthreads();
+    for (int idx = tid; idx < RES_KC * 8; idx += 64 * RES_WAVES) {
+      const int kk = idx >> 3, d = idx & 7;
+      double xr = 0, xi = 0;
+      if (kk < kc && d < nrhs) {
+        xr = x[2 * ((k0 + kk) * nrhs + d)];
+        xi = x[2 * ((k0 + kk) * nrhs + d) + 1];
+      }
+      xs[2 * (d * RES_KC + kk)] = xr;
+      xs[2 * (d * RES_KC + kk) + 1] = xi;
+    }
+    __syncthreads();
+    for (int kk = lane; kk < kc; kk += 64) {
+      double xr[8], xi[8];
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        xr[d] = xs[2 * (d * RES_KC + kk)];
+        xi[d] = xs[2 * (d * RES_KC + kk) + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < RES_ROWS; ++q) {
+        const int64_t row = row0 + q;
+        if (row < n) {
+          const double ar = A[2 * (row * lda + k0 + kk)], ai = A[2 * (row * lda + k0 + kk) + 1];
+#pragma unroll
+          for (int d = 0; d < 8; ++d) {
+            acc[(q * 8 + d) * 2] = fma(ar, xr[d], fma(-ai, xi[d], acc[(q * 8 + d) * 2]));
+            acc[(q * 8 + d) * 2 + 1] = fma(ar, xi[d], fma(ai, xr[d], acc[(q * 8 + d) * 2 + 1]));
+          }
+        }
+      }
+    }
+  }
+  halve_step<32>(acc, lane);
+  halve_step<16>(acc, lane);
+  halve_step<8>(acc, lane);
+  halve_step<4>(acc, lane);
+  halve_step<2>(acc, lane);
+  halve_step<1>(acc, lane);
+  // lane l holds sum number l = (q * 8 + d) * 2 + (0: re, 1: im)
+  const int q = lane >> 4, d = (lane >> 1) & 7, c = lane & 1;
+  const int64_t row = row0 + q;
+  if (row < n && d < nrhs) r[2 * (row * nrhs + d) + c] = b[2 * (row * nrhs + d) + c] - acc[0];
 }
 
 __global__ __launch_bounds__(256) void colnorm_partial_kernel(const double* X, int64_t n, int nrhs, double* part) {
@@ -968,67 +1001,113 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   return hipSuccess;
 }
 
-// Solve workspace layout (complex T units): Linv/Uinv per leaf (PNB x PNB each),
-// two n x nrhs buffers, then the per-leaf swap lists (ints).
-template <typename T>
-hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, int nrhs, T* work, int nb,
-                    hipStream_t s) {
-  if (nb > PNB) nb = PNB;
-  if (nrhs > 8) return hipErrorInvalidValue;
-  const int np = (int)((n + nb - 1) / nb);
-  T* Linv = work;
-  T* Uinv = Linv + 2 * (int64_t)np * PNB * PNB;
-  T* b0 = Uinv + 2 * (int64_t)np * PNB * PNB;
-  T* b1 = b0 + 2 * n * nrhs;
-  int* lrows = reinterpret_cast<int*>(b1 + 2 * n * nrhs);
-  int* lsrc = lrows + (int64_t)np * 2 * MAXBW;
-  int* lcnt = lsrc + (int64_t)np * 2 * MAXBW;
-  hipError_t e;
-  hipLaunchKernelGGL((diag_inverse_kernel<T>), dim3(np), dim3(64), 0, s, LU, n, lda, nb, Linv, Uinv);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(swap_lists_kernel, dim3((np + 63) / 64), dim3(64), 0, s, ipiv, n, nb, MAXBW, np, lrows, lsrc,
-                     lcnt);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  std::vector<int> hcnt(np);
-  if ((e = hipMemcpyAsync(hcnt.data(), lcnt, sizeof(int) * np, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-  hipLaunchKernelGGL((copy_kernel<T>), dim3(1024), dim3(256), 0, s, B, b0, 2 * n * nrhs);
-  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-  T* bin = b0;
-  T* bout = b1;
-  for (int k = 0; k < np; ++k) {
-    const int64_t j0 = (int64_t)k * nb;
-    const int b = (int)std::min<int64_t>(nb, n - j0);
-    const int64_t below = n - j0 - b;
-    const unsigned G = (unsigned)std::max<int64_t>(1, (below + 255) / 256);
-    hipLaunchKernelGGL((fwd_step_kernel<T>), dim3(G), dim3(256), 0, s, LU, n, lda, j0, b,
-                       Linv + 2 * (int64_t)k * PNB * PNB, lrows + (int64_t)k * 2 * MAXBW,
-                       lsrc + (int64_t)k * 2 * MAXBW, hcnt[k], (const T*)bin, bout, B, nrhs);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    std::swap(bin, bout);
-  }
-  hipLaunchKernelGGL((copy_kernel<T>), dim3(1024), dim3(256), 0, s, (const T*)B, b0, 2 * n * nrhs);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  for (int k = np - 1; k >= 0; --k) {
-    const int64_t j0 = (int64_t)k * nb;
-    const int b = (int)std::min<int64_t>(nb, n - j0);
-    const unsigned G = (unsigned)std::max<int64_t>(1, (j0 + 255) / 256);
-    hipLaunchKernelGGL((bwd_step_kernel<T>), dim3(G), dim3(256), 0, s, LU, lda, j0, b,
-                       Uinv + 2 * (int64_t)k * PNB * PNB, b0, B, nrhs);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  return hipSuccess;
+// Solve workspace (complex T units, then ints): Linv and Uinv per leaf (PNB x PNB
+// each), three n x 8 blocks (gathered rhs, forward results, backward results),
+// the row gather tables (nJ x n), the total row permutation (n) and the forward
+// and backward flags (np each).
+struct SolveLayout {
+  int np, nJ;
+  size_t linv, uinv, b0, yf, xb, cend;  // complex offsets
+};
+static SolveLayout solve_layout(int64_t n) {
+  SolveLayout L;
+  L.np = (int)((n + PNB - 1) / PNB);
+  L.nJ = (int)((n + MAXBW - 1) / MAXBW);
+  L.linv = 0;
+  L.uinv = L.linv + (size_t)L.np * PNB * PNB;
+  L.b0 = L.uinv + (size_t)L.np * PNB * PNB;
+  L.yf = L.b0 + (size_t)n * 8;
+  L.xb = L.yf + (size_t)n * 8;
+  L.cend = L.xb + (size_t)n * 8;
+  return L;
 }
 
-size_t lu_solve_work_bytes(int64_t n, int nrhs, int nb, size_t elem) {
-  const int np = (int)((n + nb - 1) / nb);
-  return elem * 2 * ((size_t)np * PNB * PNB * 2 + (size_t)n * nrhs * 2) + sizeof(int) * ((size_t)np * 4 * MAXBW + np) +
-         256;
+size_t lu_solve_work_bytes(int64_t n, size_t elem) {
+  const SolveLayout L = solve_layout(n);
+  return elem * 2 * L.cend + 256 + sizeof(int) * ((size_t)L.nJ * n + n + 2 * (size_t)L.np) + 256;
+}
+
+template <typename T>
+static int* solve_ints(T* work, const SolveLayout& L) {
+  const uintptr_t p = reinterpret_cast<uintptr_t>(work + 2 * L.cend);
+  return reinterpret_cast<int*>((p + 255) & ~uintptr_t(255));
+}
+
+static int next_epoch() {
+  static std::atomic<int> counter{0};
+  int e = counter.fetch_add(1) + 1;
+  return (e & 0x7fffffff) == 0 ? 1 : (e & 0x7fffffff);
+}
+
+template <typename T>
+hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* work, hipStream_t s) {
+  const SolveLayout L = solve_layout(n);
+  int* G = solve_ints(work, L);
+  int* perm = G + (size_t)L.nJ * n;
+  int* flags = perm + n;
+  hipLaunchKernelGGL((diag_inverse_kernel<T>), dim3(L.np), dim3(64), 0, s, LU, n, lda, PNB, work + 2 * L.linv,
+                     work + 2 * L.uinv);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(gather_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ipiv, n, MAXBW, L.nJ, G,
+                     perm);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return launch_fill<int>(flags, 2 * (int64_t)L.np, 0, s);
+}
+
+template <typename T>
+hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, hipStream_t s) {
+  if (nrhs > 8 || nrhs < 1) return hipErrorInvalidValue;
+  const SolveLayout L = solve_layout(n);
+  int* G = solve_ints(work, L);
+  int* perm = G + (size_t)L.nJ * n;
+  int* flags = perm + n;
+  hipLaunchKernelGGL((gather_rhs_kernel<T>), dim3((unsigned)((n * 8 + 255) / 256)), dim3(256), 0, s, (const T*)B, n,
+                     nrhs, (const int*)perm, work + 2 * L.b0);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int epoch = next_epoch();
+  TriArgs<T> ta;
+  ta.LU = LU;
+  ta.n = n;
+  ta.lda = lda;
+  ta.bw = MAXBW;
+  ta.epoch = epoch;
+  ta.np = L.np;
+  ta.nrhs = nrhs;
+  ta.out = B;
+  // forward
+  ta.dinv = work + 2 * L.linv;
+  ta.gather = G;
+  ta.src = work + 2 * L.b0;
+  ta.pub = work + 2 * L.yf;
+  ta.flags = flags;
+  {
+    void* args[] = {&ta};
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&tri_solve_kernel<T, true>), dim3(L.np), dim3(256),
+                                   args, 0, s);
+    if (e != hipSuccess) return e;
+  }
+  // backward
+  ta.dinv = work + 2 * L.uinv;
+  ta.gather = nullptr;
+  ta.src = work + 2 * L.yf;
+  ta.pub = work + 2 * L.xb;
+  ta.flags = flags + L.np;
+  {
+    void* args[] = {&ta};
+    e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&tri_solve_kernel<T, false>), dim3(L.np), dim3(256),
+                                   args, 0, s);
+  }
+  return e;
 }
 
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
                       hipStream_t s) {
   if (nrhs > 8) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(residual64_kernel, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, s, A, n, lda, x, b, r, nrhs);
+  constexpr int rows = RES_WAVES * RES_ROWS;
+  hipLaunchKernelGGL(residual64_kernel, dim3((unsigned)((n + rows - 1) / rows)), dim3(64 * RES_WAVES), 0, s, A, n, lda,
+                     x, b, r, nrhs);
   return hipGetLastError();
 }
 
@@ -1045,8 +1124,9 @@ hipError_t column_norms64(const double* X, int64_t n, int nrhs, double* out, hip
 
 template hipError_t lu_factor<float>(float*, int64_t, int64_t, LuWork&, int, hipStream_t, int*);
 template hipError_t lu_factor<double>(double*, int64_t, int64_t, LuWork&, int, hipStream_t, int*);
-template hipError_t lu_solve<float>(const float*, int64_t, int64_t, const int*, float*, int, float*, int, hipStream_t);
-template hipError_t lu_solve<double>(const double*, int64_t, int64_t, const int*, double*, int, double*, int,
-                                     hipStream_t);
+template hipError_t lu_solve_prepare<float>(const float*, int64_t, int64_t, const int*, float*, hipStream_t);
+template hipError_t lu_solve_prepare<double>(const double*, int64_t, int64_t, const int*, double*, hipStream_t);
+template hipError_t lu_solve<float>(const float*, int64_t, int64_t, float*, int, float*, hipStream_t);
+template hipError_t lu_solve<double>(const double*, int64_t, int64_t, double*, int, double*, hipStream_t);
 
 }  // namespace beat
