@@ -14,6 +14,7 @@
 
 #include "../../include/beat_bem.h"
 #include "bem_common.h"
+#include "bem_math.h"
 #include "kernels.h"
 
 using namespace beat;
@@ -58,6 +59,7 @@ struct beat_ctx {
   DevBuf a64, a32, b64, b32, q64, q32, x64, r64, n64, rn64, xn64, d32, work, chunk, norms, ipiv, lusmall, qin, pout,
       fld_src, fld_out, fld_pts;
   hipEvent_t ev[6] = {};
+  double* trig = nullptr;  // (sin, cos)(m pi/32) table of the FP64 phase
 };
 
 struct beat_mesh {
@@ -78,6 +80,7 @@ struct beat_mesh {
   int32_t* face_perm = nullptr;
   int32_t* vert_inv = nullptr;
   int64_t regular_pairs = 0;
+  double diameter = 0;  // bounding-box diagonal: bounds every |y - x| of the assembly
 };
 
 namespace {
@@ -464,6 +467,7 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   ra.krev = (T)(k / (2.0 * M_PI));
   ra.eta_re = (T)er;
   ra.eta_im = (T)ei;
+  ra.trig = ctx->trig;
   for (int i = 0; i < 6; ++i) {
     const bool on = i < m->h.nq;
     const double xi = on ? m->h.rule_pts[2 * i] : 0, eta = on ? m->h.rule_pts[2 * i + 1] : 0;
@@ -473,7 +477,8 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
     ra.phi[i][2] = (T)eta;
   }
   HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
-  HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, ctx->stream));
+  const bool wide = !(std::fabs(k) * m->diameter < SINCOS_TABLE_LIMIT);
+  HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, ctx->stream));
   HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
   SingularArgs<T> sa;
   sa.rec = m->rec64;
@@ -666,6 +671,15 @@ int beat_ctx_create(int device, beat_ctx** out) {
     delete c;
     return BEAT_ERR_HIP;
   }
+  {
+    double t[128];
+    fill_trig_table(t);
+    if (hipMalloc((void**)&c->trig, sizeof(t)) != hipSuccess ||
+        hipMemcpy(c->trig, t, sizeof(t), hipMemcpyHostToDevice) != hipSuccess) {
+      delete c;
+      return BEAT_ERR_HIP;
+    }
+  }
   if (const char* v = std::getenv("BEAT_LU_LOOKAHEAD"))
     if (v[0] == '0') {
       (void)hipStreamDestroy(c->side);
@@ -689,6 +703,7 @@ void beat_ctx_destroy(beat_ctx* c) {
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_fact) (void)hipEventDestroy(c->ev_fact);
   if (c->ev_far) (void)hipEventDestroy(c->ev_far);
+  if (c->trig) (void)hipFree(c->trig);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -738,6 +753,17 @@ int beat_mesh_upload(beat_ctx* ctx, const double* verts, int64_t nv, const int32
     return fail(ctx, BEAT_ERR_HIP, std::string("mesh upload: ") + hipGetErrorString(e));
   }
   m->regular_pairs = nf * nf - (int64_t)m->h.spairs.size();
+  {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t v = 0; v < nv; ++v)
+      for (int c = 0; c < 3; ++c) {
+        lo[c] = std::fmin(lo[c], verts[3 * v + c]);
+        hi[c] = std::fmax(hi[c], verts[3 * v + c]);
+      }
+    double d2 = 0;
+    for (int c = 0; c < 3; ++c) d2 += (hi[c] - lo[c]) * (hi[c] - lo[c]);
+    m->diameter = std::sqrt(d2);
+  }
   *out = m;
   return BEAT_OK;
 }

@@ -37,17 +37,24 @@ __device__ __forceinline__ void gadd(T* p, T v) {
 }
 
 // sin and cos of k*r. FP32: the hardware sin/cos take revolutions; the phase is
-// reduced to [-0.5, 0.5] revolutions first. FP64: library sincos.
-__device__ __forceinline__ void phase_sincos(float r, float krev, float kk, float& s, float& c) {
+// reduced to [-0.5, 0.5] revolutions first. FP64: the pi/32 table form when the
+// host has bounded |k| r below its limit (WIDE false), else the pi/2 form with
+// the library fallback.
+template <bool WIDE>
+__device__ __forceinline__ void phase_sincos(float r, float krev, float kk, const double* trig, float& s, float& c) {
   (void)kk;
+  (void)trig;
   float t = r * krev;
   t = t - rintf(t);
   s = __builtin_amdgcn_sinf(t);
   c = __builtin_amdgcn_cosf(t);
 }
-__device__ __forceinline__ void phase_sincos(double r, double krev, double kk, double& s, double& c) {
+template <bool WIDE>
+__device__ __forceinline__ void phase_sincos(double r, double krev, double kk, const double* trig, double& s,
+                                             double& c) {
   (void)krev;
-  sincos_phase(kk * r, &s, &c);
+  if (WIDE) sincos_phase(kk * r, &s, &c);
+  else sincos_table(kk * r, trig, &s, &c);
 }
 
 __device__ __forceinline__ float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -67,11 +74,13 @@ __device__ __forceinline__ double inv_sqrt(double x) {
 // addresses (scalar loads); 3x3 blocks go to the LDS tile accA[test vertex slot]
 // [trial vertex slot] and the P1 x DP0 blocks to accC[test vertex slot][trial
 // face]; both are flushed once with global atomics. Two workgroups fit per CU.
-template <typename T, int NQ, bool PURE>
+template <typename T, int NQ, bool PURE, bool WIDE>
 __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T> a) {
   __shared__ T accA[TILE_V * TILE_VS * 2];
   __shared__ T accC[TILE_V * TILE_FS * 2];
   __shared__ T xs[NQ * 3 * 64];  // test points, [point*3 + c][lane]
+  __shared__ T sphw[6 * 4];      // per rule point: phi_0, phi_1, phi_2, w
+  __shared__ double trig[128];   // (sin, cos)(m pi/32)
   __shared__ int grow[TILE_V], gcol[TILE_VS];
 
   const int64_t total = (int64_t)a.ntiles * a.nstiles;
@@ -90,6 +99,16 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
   for (int i = tid; i < TILE_V * TILE_FS * 2; i += blockDim.x) accC[i] = T(0);
   if (tid < TILE_V) grow[tid] = a.tile_verts[(int64_t)tt * TILE_V + tid];
   if (tid < TILE_VS) gcol[tid] = a.stile_verts[(int64_t)st * TILE_VS + tid];
+  if (tid < 128) trig[tid] = a.trig[tid];
+  if (tid == 0) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      sphw[4 * i] = a.phi[i][0];
+      sphw[4 * i + 1] = a.phi[i][1];
+      sphw[4 * i + 2] = a.phi[i][2];
+      sphw[4 * i + 3] = a.w[i];
+    }
+  }
 
   const int lane = tid & 63, wave = tid >> 6;
   const bool have_test = lane < nT;
@@ -142,10 +161,8 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
       const T PS = dS - (x0 * nS0 + x1 * nS1 + x2 * nS2);
       T phi_i[3];
 #pragma unroll
-      for (int p = 0; p < 3; ++p) phi_i[p] = i == 0 ? T(a.phi[0][p]) : i == 1 ? T(a.phi[1][p]) : i == 2 ? T(a.phi[2][p])
-                                          : i == 3 ? T(a.phi[3][p]) : i == 4 ? T(a.phi[4][p]) : T(a.phi[5][p]);
-      const T wi = i == 0 ? a.w[0] : i == 1 ? a.w[1] : i == 2 ? a.w[2] : i == 3 ? a.w[3] : i == 4 ? a.w[4] : a.w[5];
-      const T cwi = cw * wi;
+      for (int p = 0; p < 3; ++p) phi_i[p] = sphw[4 * i + p];
+      const T cwi = cw * sphw[4 * i + 3];
       T Mr[3] = {0, 0, 0}, Mi[3] = {0, 0, 0}, Sr = 0, Si = 0;
 #pragma unroll
       for (int j = 0; j < NQ; ++j) {
@@ -154,8 +171,8 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
         const T ir = inv_sqrt(r2);
         const T r = r2 * ir;
         T s, c;
-        phase_sincos(r, krev, k, s, c);
-        const T g = ir * (cwi * T(a.w[j]));
+        phase_sincos<WIDE>(r, krev, k, trig, s, c);
+        const T g = ir * (cwi * sphw[4 * j + 3]);
         const T Gr = c * g, Gi = s * g;
         const T Dr = -ir * Gr - k * Gi, Di = -ir * Gi + k * Gr;  // G' w
         const T t = PS * ir;      // (r.n_y)/r
@@ -182,7 +199,7 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
         Si += hi;
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-          const T ph = T(a.phi[j][b]);
+          const T ph = sphw[4 * j + b];
           Mr[b] += ph * lr;
           Mi[b] += ph * li;
         }
@@ -391,21 +408,34 @@ __global__ void downcast_kernel(const double* src, float* dst, int64_t n) {
 
 // ---- launchers -------------------------------------------------------------------
 
+template <typename T, bool WIDE>
+static void launch_regular_nq(const RegularArgs<T>& a, int nq, bool pure, dim3 grid, dim3 block, hipStream_t s) {
+  if (nq == 6) {
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 6, true, WIDE>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 6, false, WIDE>), grid, block, 0, s, a);
+  } else if (nq == 3) {
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 3, true, WIDE>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 3, false, WIDE>), grid, block, 0, s, a);
+  } else {
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 1, true, WIDE>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 1, false, WIDE>), grid, block, 0, s, a);
+  }
+}
+
 template <typename T>
-hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure, hipStream_t s) {
+hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure, bool wide, hipStream_t s) {
   const int64_t total = (int64_t)a.ntiles * a.nstiles;
   const int64_t per = (total + 7) / 8;
   dim3 grid((unsigned)(per * 8)), block(256);
-  if (nq == 6) {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 6, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 6, false>), grid, block, 0, s, a);
-  } else if (nq == 3) {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 3, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 3, false>), grid, block, 0, s, a);
-  } else {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 1, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 1, false>), grid, block, 0, s, a);
+  // the FP32 phase needs no range bound
+  if constexpr (sizeof(T) == 8) {
+    if (wide) {
+      launch_regular_nq<T, true>(a, nq, pure, grid, block, s);
+      return hipGetLastError();
+    }
   }
+  (void)wide;
+  launch_regular_nq<T, false>(a, nq, pure, grid, block, s);
   return hipGetLastError();
 }
 
@@ -439,8 +469,8 @@ hipError_t launch_downcast(const double* src, float* dst, int64_t n, hipStream_t
   return hipGetLastError();
 }
 
-template hipError_t launch_regular<float>(const RegularArgs<float>&, int, bool, hipStream_t);
-template hipError_t launch_regular<double>(const RegularArgs<double>&, int, bool, hipStream_t);
+template hipError_t launch_regular<float>(const RegularArgs<float>&, int, bool, bool, hipStream_t);
+template hipError_t launch_regular<double>(const RegularArgs<double>&, int, bool, bool, hipStream_t);
 template hipError_t launch_singular<float>(const SingularArgs<float>&, hipStream_t);
 template hipError_t launch_singular<double>(const SingularArgs<double>&, hipStream_t);
 template hipError_t launch_identity<float>(const IdentityArgs<float>&, hipStream_t);

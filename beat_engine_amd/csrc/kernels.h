@@ -33,6 +33,7 @@ struct RegularArgs {
   T eta_re, eta_im;
   T w[6];                  // rule weights
   T phi[6][3];             // P1 basis at the rule points
+  const double* trig;      // 128 doubles: (sin, cos)(m pi/32), m = 0..63
 };
 
 template <typename TO>
@@ -69,7 +70,9 @@ struct IdentityArgs {
 };
 
 template <typename T>
-hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure_imag_eta, hipStream_t s);
+// wide: |k| times the mesh diameter may reach SINCOS_TABLE_LIMIT (FP64 phase by
+// the pi/2 reduction with the library fallback instead of the table form)
+hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure_imag_eta, bool wide, hipStream_t s);
 template <typename TO>
 hipError_t launch_singular(const SingularArgs<TO>& a, hipStream_t s);
 template <typename TO>
