@@ -76,8 +76,11 @@ __device__ __forceinline__ double inv_sqrt(double x) {
 // face]; both are flushed once with global atomics. Two workgroups fit per CU.
 template <typename T, int NQ, bool PURE, bool WIDE>
 __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T> a) {
-  __shared__ T accA[TILE_V * TILE_VS * 2];
-  __shared__ T accC[TILE_V * TILE_FS * 2];
+  // rows padded by one scalar: the 64 lanes add into rows tl (varying) at the same
+  // column, which without the pad would all fall into one LDS bank
+  constexpr int LDA_ = 2 * TILE_VS + 1, LDC_ = 2 * TILE_FS + 1;
+  __shared__ T accA[TILE_V * LDA_];
+  __shared__ T accC[TILE_V * LDC_];
   __shared__ T xs[NQ * 3 * 64];  // test points, [point*3 + c][lane]
   __shared__ T sphw[6 * 4];      // per rule point: phi_0, phi_1, phi_2, w
   __shared__ double trig[128];   // (sin, cos)(m pi/32)
@@ -95,8 +98,8 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
   const int32_t* __restrict__ fverts = a.fverts;
   const uint8_t* __restrict__ flocs = a.flocs;
 
-  for (int i = tid; i < TILE_V * TILE_VS * 2; i += blockDim.x) accA[i] = T(0);
-  for (int i = tid; i < TILE_V * TILE_FS * 2; i += blockDim.x) accC[i] = T(0);
+  for (int i = tid; i < TILE_V * LDA_; i += blockDim.x) accA[i] = T(0);
+  for (int i = tid; i < TILE_V * LDC_; i += blockDim.x) accC[i] = T(0);
   if (tid < TILE_V) grow[tid] = a.tile_verts[(int64_t)tt * TILE_V + tid];
   if (tid < TILE_VS) gcol[tid] = a.stile_verts[(int64_t)st * TILE_VS + tid];
   if (tid < 128) trig[tid] = a.trig[tid];
@@ -226,11 +229,11 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
         const T cc = c0 * sr[REC_CURL + 3 * b] + c1 * sr[REC_CURL + 3 * b + 1] + c2 * sr[REC_CURL + 3 * b + 2];
-        T* dst = accA + 2 * (tl[p] * TILE_VS + sl[b]);
+        T* dst = accA + tl[p] * LDA_ + 2 * sl[b];
         lds_add(dst, Lr[p][b] + cc * ecr);
         lds_add(dst + 1, Li[p][b] + cc * eci);
       }
-      T* dc = accC + 2 * (tl[p] * TILE_FS + sj);
+      T* dc = accC + tl[p] * LDC_ + 2 * sj;
       lds_add(dc, Rr[p]);
       lds_add(dc + 1, Ri[p]);
     }
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
   // flush the P1 x P1 block
   for (int idx = tid; idx < nvT * nvS; idx += blockDim.x) {
     const int r = idx / nvS, c = idx - r * nvS;
-    const T vr = accA[2 * (r * TILE_VS + c)], vi = accA[2 * (r * TILE_VS + c) + 1];
+    const T vr = accA[r * LDA_ + 2 * c], vi = accA[r * LDA_ + 2 * c + 1];
     if (vr != T(0) || vi != T(0)) {
       T* dst = a.A + 2 * ((int64_t)grow[r] * a.lda + gcol[c]);
       gadd(dst, vr);
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
     const int r = idx / a.nrhs, d = idx - r * a.nrhs;
     T br = 0, bi = 0;
     for (int s = 0; s < nS; ++s) {
-      const T cr = accC[2 * (r * TILE_FS + s)], ci = accC[2 * (r * TILE_FS + s) + 1];
+      const T cr = accC[r * LDC_ + 2 * s], ci = accC[r * LDC_ + 2 * s + 1];
       const T* qq = a.q + 2 * ((int64_t)(s0 + s) * a.nrhs + d);
       const T qr = qq[0], qi = qq[1];
       br += cr * qr - ci * qi;
