@@ -110,9 +110,12 @@ __device__ __forceinline__ bool better(double v2, int r2, double v, int r) {
   return r2 >= 0 && (r < 0 || v2 > v || (v2 == v && r2 < r));
 }
 
-// One wavefront per workgroup; lane r owns panel row r0 + r. The panel block is
-// kept column-major in LDS (col[c][r], rows padded to RPW + 1) so that the
-// column operations are bank-conflict free.
+// Four wavefronts per workgroup; lane r of every wavefront stands for panel row
+// r0 + r. The panel block is kept column-major in LDS (col[c][r], rows padded to
+// RPW + 1) so that the column operations are bank-conflict free. Per column,
+// wavefront 0 does the pivot search and the cross-workgroup exchange, applies
+// the row swap and forms the multipliers; then the four wavefronts update the
+// remaining columns (wavefront w takes the columns c = w mod 4).
 //
 // Cross-workgroup exchange per column, without a shared counter: every
 // workgroup writes its candidate row content (and the diagonal row, if it owns
@@ -125,147 +128,153 @@ __device__ __forceinline__ bool better(double v2, int r2, double v, int r) {
 // buffer ended. Slot words are cleared before each factorization, and the tag of
 // a column is its index + 1.
 template <typename T>
-__global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
+__global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
   constexpr int LDR = RPW + 1;
-  constexpr int U = 8;  // unroll of the LDS-bound loops (independent reads in flight)
+  constexpr int U = 4;  // unroll of the LDS-bound loops (independent reads in flight)
   __shared__ T col[PNB * LDR * 2];
   __shared__ T piv[PNB * 2];
-  const int lane = threadIdx.x, g = blockIdx.x, G = gridDim.x;
+  __shared__ T part[4 * PNB * 2];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = blockIdx.x, G = gridDim.x;
   const int64_t r0 = a.j0 + (int64_t)g * RPW;
   const int nr = (int)min((int64_t)RPW, a.n - r0);
   const int nb = a.nb;
-  // stage the block: lane = column, 16 rows in flight
-  for (int rb = 0; rb < nr; rb += 16) {
-    T vr[16], vi[16];
+  // stage the block: lane = column, wavefront w takes rows w, w + 4, ...
+  for (int rb = wv; rb < nr; rb += 16) {
+    T vr[4], vi[4];
 #pragma unroll
-    for (int u = 0; u < 16; ++u)
-      if (rb + u < nr && lane < nb) {
-        const T* src = a.A + 2 * ((r0 + rb + u) * a.lda + a.j0 + lane);
+    for (int u = 0; u < 4; ++u)
+      if (rb + 4 * u < nr && lane < nb) {
+        const T* src = a.A + 2 * ((r0 + rb + 4 * u) * a.lda + a.j0 + lane);
         vr[u] = src[0];
         vi[u] = src[1];
       }
 #pragma unroll
-    for (int u = 0; u < 16; ++u)
-      if (rb + u < nr && lane < nb) {
-        col[2 * (lane * LDR + rb + u)] = vr[u];
-        col[2 * (lane * LDR + rb + u) + 1] = vi[u];
+    for (int u = 0; u < 4; ++u)
+      if (rb + 4 * u < nr && lane < nb) {
+        col[2 * (lane * LDR + rb + 4 * u)] = vr[u];
+        col[2 * (lane * LDR + rb + 4 * u) + 1] = vi[u];
       }
   }
   __syncthreads();
   for (int jj = 0; jj < nb; ++jj) {
     const int64_t cg = a.j0 + jj;
-    const unsigned tag = (unsigned)(cg + 1);
-    const int par = jj & 1;
-    const bool mine = lane < nr && r0 + lane >= cg;
-    const double mag =
-        mine ? fabs((double)col[2 * (jj * LDR + lane)]) + fabs((double)col[2 * (jj * LDR + lane) + 1]) : 0.0;
-    double v;
-    int row;
-    wave_pick(mag, (int)(r0 + lane), mine, v, row);
-    // publish the candidate row content and the diagonal row content, then the slot
-    if (row >= 0 && lane < nb) {
-      const int lr = (int)(row - r0);
-      st_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + g) * PNB + lane), col[2 * (lane * LDR + lr)],
-              col[2 * (lane * LDR + lr) + 1]);
-    }
-    const bool own_diag = cg >= r0 && cg < r0 + nr;
-    if (own_diag && lane < nb) {
-      const int lr = (int)(cg - r0);
-      st_cplx(a.slot_diag + 2 * (par * PNB + lane), col[2 * (lane * LDR + lr)], col[2 * (lane * LDR + lr) + 1]);
-    }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    if (lane == 0) {
-      unsigned long long* slot = a.slots + 2 * (par * a.maxg + g);
-      const unsigned long long vb = ((unsigned long long)__double_as_longlong(v) & ~0xffffull) | (tag & 0xffffu);
-      st_u64(slot + 1, vb);
-      st_u64(slot, ((unsigned long long)tag << 32) | (unsigned long long)(unsigned)row);
-    }
-    // gather every slot of this column: lanes poll disjoint slots, four slots'
-    // loads in flight per lane before any wait
-    double bv = -1.0;
-    int brow = -1;
-    for (int base = lane; base < G; base += 4 * 64) {
-      unsigned long long t[4], vb[4];
+    if (wv == 0) {
+      const unsigned tag = (unsigned)(cg + 1);
+      const int par = jj & 1;
+      const bool mine = lane < nr && r0 + lane >= cg;
+      const double mag =
+          mine ? fabs((double)col[2 * (jj * LDR + lane)]) + fabs((double)col[2 * (jj * LDR + lane) + 1]) : 0.0;
+      double v;
+      int row;
+      wave_pick(mag, (int)(r0 + lane), mine, v, row);
+      // publish the candidate row content and the diagonal row content, then the slot
+      if (row >= 0 && lane < nb) {
+        const int lr = (int)(row - r0);
+        st_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + g) * PNB + lane), col[2 * (lane * LDR + lr)],
+                col[2 * (lane * LDR + lr) + 1]);
+      }
+      const bool own_diag = cg >= r0 && cg < r0 + nr;
+      if (own_diag && lane < nb) {
+        const int lr = (int)(cg - r0);
+        st_cplx(a.slot_diag + 2 * (par * PNB + lane), col[2 * (lane * LDR + lr)], col[2 * (lane * LDR + lr) + 1]);
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+      if (lane == 0) {
+        unsigned long long* slot = a.slots + 2 * (par * a.maxg + g);
+        const unsigned long long vb = ((unsigned long long)__double_as_longlong(v) & ~0xffffull) | (tag & 0xffffu);
+        st_u64(slot + 1, vb);
+        st_u64(slot, ((unsigned long long)tag << 32) | (unsigned long long)(unsigned)row);
+      }
+      // gather every slot of this column: lanes poll disjoint slots, four slots'
+      // loads in flight per lane before any wait
+      double bv = -1.0;
+      int brow = -1;
+      for (int base = lane; base < G; base += 4 * 64) {
+        unsigned long long t[4], vb[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        t[u] = vb[u] = 0;
-        if (base + 64 * u < G) {
+        for (int u = 0; u < 4; ++u) {
+          t[u] = vb[u] = 0;
+          if (base + 64 * u < G) {
+            const unsigned long long* slot = a.slots + 2 * (par * a.maxg + base + 64 * u);
+            t[u] = ld_u64(slot);
+            vb[u] = ld_u64(slot + 1);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (base + 64 * u >= G) break;
           const unsigned long long* slot = a.slots + 2 * (par * a.maxg + base + 64 * u);
-          t[u] = ld_u64(slot);
-          vb[u] = ld_u64(slot + 1);
+          while ((t[u] >> 32) != tag || (vb[u] & 0xffffu) != (tag & 0xffffu)) {
+            __builtin_amdgcn_s_sleep(1);
+            t[u] = ld_u64(slot);
+            vb[u] = ld_u64(slot + 1);
+          }
+          const int ri = (int)(unsigned)(t[u] & 0xffffffffu);
+          const double vi = __longlong_as_double((long long)(vb[u] & ~0xffffull));
+          if (better(vi, ri, bv, brow)) {
+            bv = vi;
+            brow = ri;
+          }
         }
       }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (base + 64 * u >= G) break;
-        const unsigned long long* slot = a.slots + 2 * (par * a.maxg + base + 64 * u);
-        while ((t[u] >> 32) != tag || (vb[u] & 0xffffu) != (tag & 0xffffu)) {
-          __builtin_amdgcn_s_sleep(1);
-          t[u] = ld_u64(slot);
-          vb[u] = ld_u64(slot + 1);
+      double pv;
+      int p;
+      wave_pick(bv, brow, brow >= 0, pv, p);
+      const int pwg = (int)((p - a.j0) / RPW);
+      T pr = 0, pim = 0;
+      if (lane < nb) {
+        ld_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + pwg) * PNB + lane), pr, pim);
+        piv[2 * lane] = pr;
+        piv[2 * lane + 1] = pim;
+      }
+      if (g == 0 && lane == 0) a.ipiv[cg] = p;
+      if (p != cg && lane < nb) {
+        if (p >= r0 && p < r0 + nr) {
+          T dr, di;
+          ld_cplx(a.slot_diag + 2 * (par * PNB + lane), dr, di);
+          col[2 * (lane * LDR + (p - r0))] = dr;
+          col[2 * (lane * LDR + (p - r0)) + 1] = di;
         }
-        const int ri = (int)(unsigned)(t[u] & 0xffffffffu);
-        const double vi = __longlong_as_double((long long)(vb[u] & ~0xffffull));
-        if (better(vi, ri, bv, brow)) {
-          bv = vi;
-          brow = ri;
+        if (own_diag) {
+          col[2 * (lane * LDR + (cg - r0))] = pr;
+          col[2 * (lane * LDR + (cg - r0)) + 1] = pim;
         }
       }
-    }
-    double pv;
-    int p;
-    wave_pick(bv, brow, brow >= 0, pv, p);
-    const int pwg = (int)((p - a.j0) / RPW);
-    T pr = 0, pim = 0;
-    if (lane < nb) {
-      ld_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + pwg) * PNB + lane), pr, pim);
-      piv[2 * lane] = pr;
-      piv[2 * lane + 1] = pim;
-    }
-    if (g == 0 && lane == 0) a.ipiv[cg] = p;
-    if (p != cg && lane < nb) {
-      if (p >= r0 && p < r0 + nr) {
-        T dr, di;
-        ld_cplx(a.slot_diag + 2 * (par * PNB + lane), dr, di);
-        col[2 * (lane * LDR + (p - r0))] = dr;
-        col[2 * (lane * LDR + (p - r0)) + 1] = di;
-      }
-      if (own_diag) {
-        col[2 * (lane * LDR + (cg - r0))] = pr;
-        col[2 * (lane * LDR + (cg - r0)) + 1] = pim;
+      // multipliers of this column (pr, pim of lane jj is the pivot)
+      const T vr = __shfl(pr, jj, 64), vi = __shfl(pim, jj, 64);
+      if (vr == T(0) && vi == T(0)) {
+        if (g == 0 && lane == 0 && *a.info == 0) *a.info = (int)cg + 1;
+      } else if (lane < nr && r0 + lane > cg) {
+        const T den = vr * vr + vi * vi;
+        const T ir = vr / den, ii = -vi / den;
+        const T ar = col[2 * (jj * LDR + lane)], ai = col[2 * (jj * LDR + lane) + 1];
+        col[2 * (jj * LDR + lane)] = ar * ir - ai * ii;
+        col[2 * (jj * LDR + lane) + 1] = ar * ii + ai * ir;
       }
     }
     __syncthreads();
     const T vr = piv[2 * jj], vi = piv[2 * jj + 1];
-    if (vr == T(0) && vi == T(0)) {
-      if (g == 0 && lane == 0 && *a.info == 0) *a.info = (int)cg + 1;
-      continue;  // LAPACK leaves the column unscaled and goes on
-    }
-    const T den = vr * vr + vi * vi;
-    const T ir = vr / den, ii = -vi / den;
+    if (vr == T(0) && vi == T(0)) continue;  // LAPACK leaves the column unscaled and goes on
     if (lane < nr && r0 + lane > cg) {
-      const T ar = col[2 * (jj * LDR + lane)], ai = col[2 * (jj * LDR + lane) + 1];
-      const T lr = ar * ir - ai * ii, li = ar * ii + ai * ir;
-      col[2 * (jj * LDR + lane)] = lr;
-      col[2 * (jj * LDR + lane) + 1] = li;
-      int c = jj + 1;
-      for (; c + U <= nb; c += U) {
+      const T lr = col[2 * (jj * LDR + lane)], li = col[2 * (jj * LDR + lane) + 1];
+      // this wavefront's columns c > jj with c = wv (mod 4)
+      int c = jj + 1 + ((wv - (jj + 1)) & 3);
+      for (; c + 4 * (U - 1) < nb; c += 4 * U) {
         T xr[U], xi[U], ur[U], ui[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          xr[u] = col[2 * ((c + u) * LDR + lane)];
-          xi[u] = col[2 * ((c + u) * LDR + lane) + 1];
-          ur[u] = piv[2 * (c + u)];
-          ui[u] = piv[2 * (c + u) + 1];
+          xr[u] = col[2 * ((c + 4 * u) * LDR + lane)];
+          xi[u] = col[2 * ((c + 4 * u) * LDR + lane) + 1];
+          ur[u] = piv[2 * (c + 4 * u)];
+          ui[u] = piv[2 * (c + 4 * u) + 1];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          col[2 * ((c + u) * LDR + lane)] = xr[u] - (lr * ur[u] - li * ui[u]);
-          col[2 * ((c + u) * LDR + lane) + 1] = xi[u] - (lr * ui[u] + li * ur[u]);
+          col[2 * ((c + 4 * u) * LDR + lane)] = xr[u] - (lr * ur[u] - li * ui[u]);
+          col[2 * ((c + 4 * u) * LDR + lane) + 1] = xi[u] - (lr * ui[u] + li * ur[u]);
         }
       }
-      for (; c < nb; ++c) {
+      for (; c < nb; c += 4) {
         const T ur = piv[2 * c], ui = piv[2 * c + 1];
         col[2 * (c * LDR + lane)] -= lr * ur - li * ui;
         col[2 * (c * LDR + lane) + 1] -= lr * ui + li * ur;
@@ -273,7 +282,7 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
     }
     __syncthreads();
   }
-  for (int r = 0; r < nr; ++r)
+  for (int r = wv; r < nr; r += 4)
     if (lane < nb) {
       T* dst = a.A + 2 * ((r0 + r) * a.lda + a.j0 + lane);
       dst[0] = col[2 * (lane * LDR + r)];
@@ -282,13 +291,14 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
   // The first workgroup holds the leaf's diagonal block (already stored back):
   // invert its unit-lower part in place, columns right to left as in LAPACK's
   // trti2. With the strict upper part of col cleared and its diagonal set to 1,
-  // lane i forms X(i,j) = -sum_{k=j+1}^{nb-1} col[k][i] L(k,j) over a uniform k
-  // range: col[k][i] holds X(i,k) for j < k < i, 1 for k = i and 0 for k > i,
+  // row i of X forms X(i,j) = -sum_{k=j+1}^{nb-1} col[k][i] L(k,j) over a uniform
+  // k range: col[k][i] holds X(i,k) for j < k < i, 1 for k = i and 0 for k > i,
   // and L(k,j) = col[j][k] is still intact while column j is being formed.
+  // Wavefront w sums the terms k = w (mod 4); wavefront 0 adds the four parts.
   if (g == 0 && a.linv) {
     const int i = lane;
     __syncthreads();
-    if (i < nb)
+    if (wv == 0 && i < nb)
       for (int c = i; c < nb; ++c) {
         col[2 * (c * LDR + i)] = c == i ? T(1) : T(0);
         col[2 * (c * LDR + i) + 1] = T(0);
@@ -296,15 +306,15 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
     __syncthreads();
     for (int j = nb - 2; j >= 0; --j) {
       T sr = 0, si = 0;
-      int k = j + 1;
-      for (; k + U <= nb; k += U) {
+      int k = j + 1 + ((wv - (j + 1)) & 3);
+      for (; k + 4 * (U - 1) < nb; k += 4 * U) {
         T xr[U], xi[U], lr[U], li[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          xr[u] = col[2 * ((k + u) * LDR + i)];
-          xi[u] = col[2 * ((k + u) * LDR + i) + 1];
-          lr[u] = col[2 * (j * LDR + k + u)];
-          li[u] = col[2 * (j * LDR + k + u) + 1];
+          xr[u] = col[2 * ((k + 4 * u) * LDR + i)];
+          xi[u] = col[2 * ((k + 4 * u) * LDR + i) + 1];
+          lr[u] = col[2 * (j * LDR + k + 4 * u)];
+          li[u] = col[2 * (j * LDR + k + 4 * u) + 1];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -312,20 +322,23 @@ __global__ __launch_bounds__(64) void panel_kernel(const PanelArgs<T> a) {
           si += xr[u] * li[u] + xi[u] * lr[u];
         }
       }
-      for (; k < nb; ++k) {
+      for (; k < nb; k += 4) {
         const T xr = col[2 * (k * LDR + i)], xi = col[2 * (k * LDR + i) + 1];
         const T lr = col[2 * (j * LDR + k)], li = col[2 * (j * LDR + k) + 1];
         sr += xr * lr - xi * li;
         si += xr * li + xi * lr;
       }
+      part[2 * (wv * PNB + i)] = sr;
+      part[2 * (wv * PNB + i) + 1] = si;
       __syncthreads();
-      if (i > j && i < nb) {
-        col[2 * (j * LDR + i)] = -sr;
-        col[2 * (j * LDR + i) + 1] = -si;
+      if (wv == 0 && i > j && i < nb) {
+        col[2 * (j * LDR + i)] = -(part[2 * i] + part[2 * (PNB + i)] + part[2 * (2 * PNB + i)] + part[2 * (3 * PNB + i)]);
+        col[2 * (j * LDR + i) + 1] =
+            -(part[2 * i + 1] + part[2 * (PNB + i) + 1] + part[2 * (2 * PNB + i) + 1] + part[2 * (3 * PNB + i) + 1]);
       }
       __syncthreads();
     }
-    for (int r = 0; r < nb; ++r)
+    for (int r = wv; r < nb; r += 4)
       if (lane < nb) {
         a.linv[2 * (r * PNB + lane)] = col[2 * (lane * LDR + r)];
         a.linv[2 * (r * PNB + lane) + 1] = col[2 * (lane * LDR + r) + 1];
@@ -910,7 +923,7 @@ static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, 
   pa.info = w.info;
   pa.linv = linv;
   void* args[] = {&pa};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G), dim3(64), args, 0, s);
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G), dim3(256), args, 0, s);
 }
 
 template <typename T>
