@@ -185,36 +185,44 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
         st_u64(slot + 1, vb);
         st_u64(slot, ((unsigned long long)tag << 32) | (unsigned long long)(unsigned)row);
       }
-      // gather every slot of this column: lanes poll disjoint slots, four slots'
-      // loads in flight per lane before any wait
+      // gather every slot of this column: lanes poll disjoint slots (up to four
+      // each per round); every round re-reads all of a lane's pending slots at
+      // once, so one round trip covers them all
       double bv = -1.0;
       int brow = -1;
       for (int base = lane; base < G; base += 4 * 64) {
-        unsigned long long t[4], vb[4];
+        bool pend[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          t[u] = vb[u] = 0;
-          if (base + 64 * u < G) {
-            const unsigned long long* slot = a.slots + 2 * (par * a.maxg + base + 64 * u);
-            t[u] = ld_u64(slot);
-            vb[u] = ld_u64(slot + 1);
-          }
-        }
+        for (int u = 0; u < 4; ++u) pend[u] = base + 64 * u < G;
+        bool any = true;
+        while (any) {
+          unsigned long long t[4], vb[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (base + 64 * u >= G) break;
-          const unsigned long long* slot = a.slots + 2 * (par * a.maxg + base + 64 * u);
-          while ((t[u] >> 32) != tag || (vb[u] & 0xffffu) != (tag & 0xffffu)) {
-            __builtin_amdgcn_s_sleep(1);
-            t[u] = ld_u64(slot);
-            vb[u] = ld_u64(slot + 1);
+          for (int u = 0; u < 4; ++u) {
+            t[u] = vb[u] = 0;
+            if (pend[u]) {
+              const unsigned long long* slot = a.slots + 2 * (par * a.maxg + base + 64 * u);
+              t[u] = ld_u64(slot);
+              vb[u] = ld_u64(slot + 1);
+            }
           }
-          const int ri = (int)(unsigned)(t[u] & 0xffffffffu);
-          const double vi = __longlong_as_double((long long)(vb[u] & ~0xffffull));
-          if (better(vi, ri, bv, brow)) {
-            bv = vi;
-            brow = ri;
+          any = false;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (!pend[u]) continue;
+            if ((t[u] >> 32) == tag && (vb[u] & 0xffffu) == (tag & 0xffffu)) {
+              pend[u] = false;
+              const int ri = (int)(unsigned)(t[u] & 0xffffffffu);
+              const double vi = __longlong_as_double((long long)(vb[u] & ~0xffffull));
+              if (better(vi, ri, bv, brow)) {
+                bv = vi;
+                brow = ri;
+              }
+            } else {
+              any = true;
+            }
           }
+          if (any) __builtin_amdgcn_s_sleep(1);
         }
       }
       double pv;
@@ -346,21 +354,82 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
   }
 }
 
-// Row exchanges ipiv[k] <-> k for k in [k0, k1), in order, on columns [c0, c1).
+// Row exchanges ipiv[k] <-> k for k in [k0, k1), in order (k1 - k0 <= MAXBW),
+// on columns [c0, c1), as one permutation: the rows that change are k0..k1-1 and
+// the targets ipiv[k] >= k1; the final content of row r is the original row
+// src(r) found by applying the exchanges to r in reverse order. Each workgroup
+// composes the list itself, then gathers all moved values of its 16 columns into
+// registers before any is written back (so the in-place update is race-free).
+constexpr int SWP_COLS = 16;
 template <typename T>
-__global__ void laswp_cols_kernel(T* A, int64_t lda, int64_t c0, int64_t c1, const int* ipiv, int64_t k0, int64_t k1) {
-  const int64_t c = c0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= c1) return;
-  for (int64_t k = k0; k < k1; ++k) {
-    const int64_t p = ipiv[k];
-    if (p == k) continue;
-    T* x = A + 2 * (k * lda + c);
-    T* y = A + 2 * (p * lda + c);
-    const T xr = x[0], xi = x[1];
-    x[0] = y[0];
-    x[1] = y[1];
-    y[0] = xr;
-    y[1] = xi;
+__global__ __launch_bounds__(256) void laswp_kernel(T* A, int64_t lda, int64_t c0, int64_t c1, const int* ipiv,
+                                                    int64_t k0, int64_t k1) {
+  __shared__ int sdst[2 * MAXBW], ssrc[2 * MAXBW];
+  __shared__ int4 spiv[MAXBW / 4];
+  const int tid = threadIdx.x;
+  const int K = (int)(k1 - k0);
+  // the exchanges, padded to a multiple of 4 with no-ops (p = k)
+  for (int k = tid; k < MAXBW; k += 256) reinterpret_cast<int*>(spiv)[k] = k < K ? ipiv[k0 + k] : (int)(k0 + k);
+  __syncthreads();
+  // thread t composes entries t and t + 256 (rows k0 + t, and the target of
+  // exchange t when it lies beyond k1), walking the exchanges in reverse
+  int q[2], dst[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = tid + 256 * h;
+    dst[h] = -1;
+    if (e < K) dst[h] = (int)(k0 + e);
+    else if (e >= MAXBW && e - MAXBW < K) {
+      const int p = reinterpret_cast<const int*>(spiv)[e - MAXBW];
+      if (p >= k1) dst[h] = p;
+    }
+    q[h] = dst[h];
+  }
+  for (int k4 = (K + 3) / 4 - 1; k4 >= 0; --k4) {
+    const int4 p4 = spiv[k4];
+    const int ps[4] = {p4.x, p4.y, p4.z, p4.w};
+#pragma unroll
+    for (int u = 3; u >= 0; --u) {
+      const int k = (int)k0 + 4 * k4 + u;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (q[h] == k) q[h] = ps[u];
+        else if (q[h] == ps[u]) q[h] = k;
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = tid + 256 * h;
+    sdst[e] = q[h] == dst[h] ? -1 : dst[h];  // unchanged rows are skipped
+    ssrc[e] = q[h];
+  }
+  __syncthreads();
+  const int cl = tid % SWP_COLS, eg = tid / SWP_COLS;
+  const int64_t c = c0 + (int64_t)blockIdx.x * SWP_COLS + cl;
+  const bool act = c < c1;
+  constexpr int NE = 2 * MAXBW / (256 / SWP_COLS);
+  T vr[NE], vi[NE];
+#pragma unroll
+  for (int j = 0; j < NE; ++j) {
+    const int e = eg + j * (256 / SWP_COLS);
+    vr[j] = vi[j] = T(0);
+    if (act && sdst[e] >= 0) {
+      const T* s = A + 2 * ((int64_t)ssrc[e] * lda + c);
+      vr[j] = s[0];
+      vi[j] = s[1];
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NE; ++j) {
+    const int e = eg + j * (256 / SWP_COLS);
+    if (act && sdst[e] >= 0) {
+      T* d = A + 2 * ((int64_t)sdst[e] * lda + c);
+      d[0] = vr[j];
+      d[1] = vi[j];
+    }
   }
 }
 
@@ -930,9 +999,13 @@ template <typename T>
 static hipError_t launch_laswp(T* A, int64_t lda, int64_t c0, int64_t c1, const int* ipiv, int64_t k0, int64_t k1,
                                hipStream_t s) {
   if (c1 <= c0 || k1 <= k0) return hipSuccess;
-  hipLaunchKernelGGL((laswp_cols_kernel<T>), dim3((unsigned)((c1 - c0 + 255) / 256)), dim3(256), 0, s, A, lda, c0, c1,
-                     ipiv, k0, k1);
-  return hipGetLastError();
+  for (int64_t kb = k0; kb < k1; kb += MAXBW) {
+    hipLaunchKernelGGL((laswp_kernel<T>), dim3((unsigned)((c1 - c0 + SWP_COLS - 1) / SWP_COLS)), dim3(256), 0, s, A, lda,
+                       c0, c1, ipiv, kb, std::min<int64_t>(k1, kb + MAXBW));
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // Outer blocks of LU_BW columns factored as LU_NB-wide leaves (panel, swaps of
