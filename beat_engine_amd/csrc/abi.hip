@@ -54,6 +54,8 @@ struct beat_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // look-ahead stream of the LU
   hipEvent_t ev_fact = nullptr, ev_far = nullptr;
+  hipStream_t inb = nullptr;  // in-block stream of the LU
+  hipEvent_t ev_panel = nullptr, ev_inb = nullptr;
   std::string err;
   std::mutex mu;
   DevBuf a64, a32, b64, b32, q64, q32, x64, r64, n64, rn64, xn64, d32, work, chunk, norms, ipiv, lusmall, qin, pout,
@@ -236,6 +238,9 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   w.side = ctx->side;
   w.ev_fact = ctx->ev_fact;
   w.ev_far = ctx->ev_far;
+  w.inb = ctx->inb;
+  w.ev_panel = ctx->ev_panel;
+  w.ev_inb = ctx->ev_inb;
   w.maxg = maxg;
   (void)sizeof(T);
   return BEAT_OK;
@@ -667,7 +672,10 @@ int beat_ctx_create(int device, beat_ctx** out) {
     }
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fact, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_far, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_far, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->inb, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_panel, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_inb, hipEventDisableTiming) != hipSuccess) {
     delete c;
     return BEAT_ERR_HIP;
   }
@@ -680,10 +688,13 @@ int beat_ctx_create(int device, beat_ctx** out) {
       return BEAT_ERR_HIP;
     }
   }
+  // BEAT_LU_LOOKAHEAD=0 runs the whole factorization in order on one stream
   if (const char* v = std::getenv("BEAT_LU_LOOKAHEAD"))
     if (v[0] == '0') {
       (void)hipStreamDestroy(c->side);
+      (void)hipStreamDestroy(c->inb);
       c->side = nullptr;
+      c->inb = nullptr;
     }
   *out = c;
   return BEAT_OK;
@@ -703,6 +714,9 @@ void beat_ctx_destroy(beat_ctx* c) {
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_fact) (void)hipEventDestroy(c->ev_fact);
   if (c->ev_far) (void)hipEventDestroy(c->ev_far);
+  if (c->inb) (void)hipStreamDestroy(c->inb);
+  if (c->ev_panel) (void)hipEventDestroy(c->ev_panel);
+  if (c->ev_inb) (void)hipEventDestroy(c->ev_inb);
   if (c->trig) (void)hipFree(c->trig);
   (void)hipStreamDestroy(c->stream);
   delete c;

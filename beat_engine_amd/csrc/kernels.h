@@ -92,6 +92,8 @@ struct LuWork {
   int maxg;
   hipStream_t side;   // look-ahead stream (nullptr: no look-ahead)
   hipEvent_t ev_fact, ev_far;
+  hipStream_t inb;    // in-block stream (nullptr: in-block work in order on the main stream)
+  hipEvent_t ev_panel, ev_inb;
 };
 
 template <typename T>

@@ -974,6 +974,36 @@ __global__ void colnorm_final_kernel(const double* part, int nparts, int nrhs, d
 
 }  // namespace
 
+// Grids whose workgroups wait on each other must be resident all at once. A
+// plain launch on the caller's stream keeps such a kernel on that stream's
+// hardware queue (no cross-queue dependency before and after it); it is used
+// when the grid fits on the device at the kernel's occupancy, so every
+// workgroup is placed once the (finite, non-waiting) work of other streams that
+// holds resources drains. Larger grids go through the cooperative launch, which
+// checks residency itself.
+template <typename Args>
+static hipError_t launch_resident(const void* kernel, void (*fn)(Args), int grid, int block, Args& args,
+                                  hipStream_t s) {
+  static std::atomic<int> cap[16];
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  int c = dev < 16 ? cap[dev].load() : 0;
+  if (c <= 0) {
+    int cus = 0, per = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0)) != hipSuccess) return e;
+    c = cus * per;
+    if (dev < 16) cap[dev].store(c);
+  }
+  if (grid <= c) {
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(block), 0, s, args);
+    return hipGetLastError();
+  }
+  void* p[] = {&args};
+  return hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(block), p, 0, s);
+}
+
 template <typename T>
 static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, LuWork& w, T* linv, hipStream_t s) {
   const int G = (int)((n - j0 + RPW - 1) / RPW);
@@ -991,8 +1021,7 @@ static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, 
   pa.maxg = w.maxg;
   pa.info = w.info;
   pa.linv = linv;
-  void* args[] = {&pa};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&panel_kernel<T>), dim3(G), dim3(256), args, 0, s);
+  return launch_resident<PanelArgs<T>>(reinterpret_cast<const void*>(&panel_kernel<T>), panel_kernel<T>, G, 256, pa, s);
 }
 
 template <typename T>
@@ -1008,14 +1037,22 @@ static hipError_t launch_laswp(T* A, int64_t lda, int64_t c0, int64_t c1, const 
   return hipSuccess;
 }
 
-// Outer blocks of LU_BW columns factored as LU_NB-wide leaves (panel, swaps of
-// the block's other columns, leaf TRSM and in-block GEMM); the block's swaps,
-// TRSM and one K = LU_BW GEMM then update the trailing matrix.
-// Look-ahead: with a side stream, block J's update of the columns beyond the
-// next block runs there while the main stream updates the next block's columns
-// and factors it; the main stream waits for that far update before it updates
-// the block after. Leaf inverses alternate between two buffers by block parity,
-// so a far update still reading block J's inverses never sees block J+1's.
+// Outer blocks of MAXBW columns factored as nb-wide leaves; the block's swaps,
+// U12 by leaf inverses and one K = MAXBW GEMM then update the trailing matrix.
+//
+// Streams (when the context has them; otherwise everything runs in order on s):
+//  s      panels, each leaf's update of the next leaf's columns (swaps, U12,
+//         GEMM: the only in-block work between two panels), and the block's
+//         update of the next block's columns;
+//  inb    the rest of each leaf's in-block work, concurrent with the next panel:
+//         its swaps on the block's earlier columns and its update of the leaves
+//         after the next one. Updates of one column range stay in leaf order: s
+//         waits for inb's work of leaf t-1 before it applies leaf t to leaf t+1.
+//  side   block J's update of the columns beyond the next block (look-ahead),
+//         while s updates the next block's columns and factors it; s waits for
+//         that far update before it updates the block after.
+// Leaf inverses alternate between two buffers by block parity, so a far update
+// still reading block J's inverses never sees block J+1's.
 template <typename T>
 hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_t s, int* panels_out) {
   if (nb > PNB) nb = PNB;
@@ -1027,8 +1064,20 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   const T* cA = A;
   T* linv = reinterpret_cast<T*>(w.linv);
   const bool look_ahead = w.side != nullptr;
+  const bool split = w.inb != nullptr;
   auto leaf_inv = [&](int64_t j0, int64_t J) {
     return linv + 2 * ((((J / bw) & 1) * (bw / nb)) + (j0 - J) / nb) * PNB * PNB;
+  };
+  // leaf [j0, j0 + b) of block J: its update of columns [c0, c1) (swaps, U12, GEMM)
+  auto leaf_update = [&](int64_t J, int64_t j0, int b, int64_t c0, int64_t c1, hipStream_t st) -> hipError_t {
+    if (c1 <= c0) return hipSuccess;
+    hipError_t r;
+    if ((r = launch_laswp<T>(A, lda, c0, c1, w.ipiv, j0, j0 + b, st)) != hipSuccess) return r;
+    r = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + c0), lda, A + 2 * (j0 * lda + c0), lda, b, c1 - c0, b,
+                    st);
+    if (r != hipSuccess) return r;
+    return gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + c0), lda, A + 2 * ((j0 + b) * lda + c0),
+                       lda, n - j0 - b, c1 - c0, b, st);
   };
   // block [J, Je)'s update of columns [c0, c1): swaps, U12 by leaf inverses, GEMM
   auto update = [&](int64_t J, int64_t Je, int64_t c0, int64_t c1, hipStream_t st) -> hipError_t {
@@ -1053,20 +1102,24 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
     const int64_t Je = std::min<int64_t>(n, J + bw);
     for (int64_t j0 = J; j0 < Je; j0 += nb) {
       const int b = (int)std::min<int64_t>(nb, Je - j0);
+      const int64_t nx = std::min<int64_t>(Je, j0 + b + nb);  // end of the next leaf's columns
       if ((e = launch_panel<T>(A, n, lda, j0, b, w, leaf_inv(j0, J), s)) != hipSuccess) return e;
       ++panels;
-      if ((e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, s)) != hipSuccess) return e;
-      if (j0 + b < Je) {
-        // U12 = L11^-1 P A12 on the block's remaining columns, then the in-block update
-        if ((e = launch_laswp<T>(A, lda, j0 + b, Je, w.ipiv, j0, j0 + b, s)) != hipSuccess) return e;
-        e = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + j0 + b), lda, A + 2 * (j0 * lda + j0 + b), lda, b,
-                        Je - j0 - b, b, s);
-        if (e != hipSuccess) return e;
-        e = gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + j0 + b), lda,
-                        A + 2 * ((j0 + b) * lda + j0 + b), lda, n - j0 - b, Je - j0 - b, b, s);
-        if (e != hipSuccess) return e;
+      if (!split) {
+        if ((e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, s)) != hipSuccess) return e;
+        if ((e = leaf_update(J, j0, b, j0 + b, Je, s)) != hipSuccess) return e;
+        continue;
       }
+      if ((e = hipEventRecord(w.ev_panel, s)) != hipSuccess) return e;
+      if (j0 > J && nx > j0 + b)  // inb's work of the previous leaf touched these columns
+        if ((e = hipStreamWaitEvent(s, w.ev_inb, 0)) != hipSuccess) return e;
+      if ((e = leaf_update(J, j0, b, j0 + b, nx, s)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(w.inb, w.ev_panel, 0)) != hipSuccess) return e;
+      if ((e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, w.inb)) != hipSuccess) return e;
+      if ((e = leaf_update(J, j0, b, nx, Je, w.inb)) != hipSuccess) return e;
+      if ((e = hipEventRecord(w.ev_inb, w.inb)) != hipSuccess) return e;
     }
+    if (split && (e = hipStreamWaitEvent(s, w.ev_inb, 0)) != hipSuccess) return e;
     if (Je >= n) continue;
     const int64_t Kn = std::min<int64_t>(n, Je + bw);
     if (look_ahead) {
@@ -1168,24 +1221,17 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work
   ta.src = work + 2 * L.b0;
   ta.pub = work + 2 * L.yf;
   ta.flags = flags;
-  {
-    void* args[] = {&ta};
-    e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&tri_solve_kernel<T, true>), dim3(L.np), dim3(256),
-                                   args, 0, s);
-    if (e != hipSuccess) return e;
-  }
+  e = launch_resident<TriArgs<T>>(reinterpret_cast<const void*>(&tri_solve_kernel<T, true>), tri_solve_kernel<T, true>,
+                                  L.np, 256, ta, s);
+  if (e != hipSuccess) return e;
   // backward
   ta.dinv = work + 2 * L.uinv;
   ta.gather = nullptr;
   ta.src = work + 2 * L.yf;
   ta.pub = work + 2 * L.xb;
   ta.flags = flags + L.np;
-  {
-    void* args[] = {&ta};
-    e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&tri_solve_kernel<T, false>), dim3(L.np), dim3(256),
-                                   args, 0, s);
-  }
-  return e;
+  return launch_resident<TriArgs<T>>(reinterpret_cast<const void*>(&tri_solve_kernel<T, false>),
+                                     tri_solve_kernel<T, false>, L.np, 256, ta, s);
 }
 
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
