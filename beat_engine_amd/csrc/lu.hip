@@ -127,13 +127,27 @@ __device__ __forceinline__ bool better(double v2, int r2, double v, int r) {
 // workgroup has published the following column, that is after all reads of the
 // buffer ended. Slot words are cleared before each factorization, and the tag of
 // a column is its index + 1.
+// Leaf inverse: complex64 builds it during the factorization (wavefront 1 forms
+// row jj - 1 of X = L11^-1 while wavefront 0 runs column jj's exchange, from row
+// jj - 1 of L, final once column jj - 1's swap is done), in its own LDS array;
+// complex128 keeps the LDS for occupancy and inverts after the last column.
+template <typename T>
+struct PanelInv {
+  static constexpr bool inc = sizeof(T) == 4;
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
   constexpr int LDR = RPW + 1;
   constexpr int U = 4;  // unroll of the LDS-bound loops (independent reads in flight)
+  constexpr bool INC = PanelInv<T>::inc;
   __shared__ T col[PNB * LDR * 2];
   __shared__ T piv[PNB * 2];
   __shared__ T part[4 * PNB * 2];
+  __shared__ T xs[INC ? PNB * PNB * 2 : 2];  // X[r][m], lane m
+  // the panel is the factorization's critical path: its waves issue first on
+  // CUs shared with trailing-update GEMMs
+  __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, g = blockIdx.x, G = gridDim.x;
   const int64_t r0 = a.j0 + (int64_t)g * RPW;
   const int nr = (int)min((int64_t)RPW, a.n - r0);
@@ -156,8 +170,37 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
       }
   }
   __syncthreads();
+  // X row r = row r of L11^-1: X(r,m) = [m == r] - sum_{k<r} L(r,k) X(k,m), lane m
+  auto inverse_row = [&](int r) {
+    T sr = 0, si = 0;
+    int k = 0;
+    for (; k + 8 <= r; k += 8) {
+      T lr[8], li[8], xr[8], xi[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        lr[u] = col[2 * ((k + u) * LDR + r)];
+        li[u] = col[2 * ((k + u) * LDR + r) + 1];
+        xr[u] = xs[2 * ((k + u) * PNB + lane)];
+        xi[u] = xs[2 * ((k + u) * PNB + lane) + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        sr += lr[u] * xr[u] - li[u] * xi[u];
+        si += lr[u] * xi[u] + li[u] * xr[u];
+      }
+    }
+    for (; k < r; ++k) {
+      const T lr = col[2 * (k * LDR + r)], li = col[2 * (k * LDR + r) + 1];
+      const T xr = xs[2 * (k * PNB + lane)], xi = xs[2 * (k * PNB + lane) + 1];
+      sr += lr * xr - li * xi;
+      si += lr * xi + li * xr;
+    }
+    xs[2 * (r * PNB + lane)] = (lane == r ? T(1) : T(0)) - sr;
+    xs[2 * (r * PNB + lane) + 1] = -si;
+  };
   for (int jj = 0; jj < nb; ++jj) {
     const int64_t cg = a.j0 + jj;
+    if (INC && g == 0 && wv == 1 && jj > 0 && a.linv) inverse_row(jj - 1);
     if (wv == 0) {
       const unsigned tag = (unsigned)(cg + 1);
       const int par = jj & 1;
@@ -296,6 +339,18 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
       dst[0] = col[2 * (lane * LDR + r)];
       dst[1] = col[2 * (lane * LDR + r) + 1];
     }
+  if (INC) {
+    if (g == 0 && a.linv) {
+      if (wv == 1) inverse_row(nb - 1);
+      __syncthreads();
+      for (int r = wv; r < nb; r += 4)
+        if (lane < nb) {
+          a.linv[2 * (r * PNB + lane)] = xs[2 * (r * PNB + lane)];
+          a.linv[2 * (r * PNB + lane) + 1] = xs[2 * (r * PNB + lane) + 1];
+        }
+    }
+    return;
+  }
   // The first workgroup holds the leaf's diagonal block (already stored back):
   // invert its unit-lower part in place, columns right to left as in LAPACK's
   // trti2. With the strict upper part of col cleared and its diagonal set to 1,
