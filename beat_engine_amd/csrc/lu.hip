@@ -698,51 +698,94 @@ hipError_t gemm_set(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int6
 
 // ---- solves ----------------------------------------------------------------------
 
-// Inverses of the unit-lower and upper diagonal blocks of every panel.
+// Inverses of the unit-lower and upper diagonal blocks of every panel; thread c
+// forms column c of each inverse by substitution, the block and the partial
+// columns kept in LDS (reads of the block are broadcast, of the columns per lane).
 template <typename T>
 __global__ __launch_bounds__(64) void diag_inverse_kernel(const T* LU, int64_t n, int64_t lda, int nbmax, T* Linv,
                                                           T* Uinv) {
-  __shared__ T D[PNB * PNB * 2];
-  const int k = blockIdx.x, tid = threadIdx.x;
+  constexpr int DL = PNB + 1;
+  __shared__ T D[PNB * DL * 2];  // D[i][q]
+  __shared__ T X[PNB * PNB * 2];  // X[i][c], lane c
+  const int k = blockIdx.x, c = threadIdx.x;
   const int64_t j0 = (int64_t)k * nbmax;
   const int nb = (int)min((int64_t)nbmax, n - j0);
-  for (int idx = tid; idx < nb * nb; idx += 64) {
-    const int i = idx / nb, c = idx - i * nb;
-    D[2 * (i * PNB + c)] = LU[2 * ((j0 + i) * lda + j0 + c)];
-    D[2 * (i * PNB + c) + 1] = LU[2 * ((j0 + i) * lda + j0 + c) + 1];
-  }
+  for (int i = 0; i < nb; ++i)
+    if (c < nb) {
+      D[2 * (i * DL + c)] = LU[2 * ((j0 + i) * lda + j0 + c)];
+      D[2 * (i * DL + c) + 1] = LU[2 * ((j0 + i) * lda + j0 + c) + 1];
+    }
   __syncthreads();
   T* Li = Linv + 2 * (int64_t)k * PNB * PNB;
   T* Ui = Uinv + 2 * (int64_t)k * PNB * PNB;
-  if (tid < nb) {
-    const int c = tid;
-    // unit lower: L X = I, column c
-    for (int i = 0; i < nb; ++i) {
-      T sr = (i == c) ? T(1) : T(0), si = 0;
-      for (int q = 0; q < i; ++q) {
-        const T lr = D[2 * (i * PNB + q)], li = D[2 * (i * PNB + q) + 1];
-        const T xr = Li[2 * (q * PNB + c)], xi = Li[2 * (q * PNB + c) + 1];
-        sr -= lr * xr - li * xi;
-        si -= lr * xi + li * xr;
+  // unit lower: L X = I, column c; X(i,c) = [i == c] - sum_{q<i} D(i,q) X(q,c)
+  for (int i = 0; i < nb; ++i) {
+    T sr = 0, si = 0;
+    int q = 0;
+    for (; q + 8 <= i; q += 8) {
+      T dr[8], di[8], xr[8], xi[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        dr[u] = D[2 * (i * DL + q + u)];
+        di[u] = D[2 * (i * DL + q + u) + 1];
+        xr[u] = X[2 * ((q + u) * PNB + c)];
+        xi[u] = X[2 * ((q + u) * PNB + c) + 1];
       }
-      Li[2 * (i * PNB + c)] = sr;
-      Li[2 * (i * PNB + c) + 1] = si;
-    }
-    // upper: U X = I, column c
-    for (int i = nb - 1; i >= 0; --i) {
-      T sr = (i == c) ? T(1) : T(0), si = 0;
-      for (int q = i + 1; q < nb; ++q) {
-        const T ur = D[2 * (i * PNB + q)], ui = D[2 * (i * PNB + q) + 1];
-        const T xr = Ui[2 * (q * PNB + c)], xi = Ui[2 * (q * PNB + c) + 1];
-        sr -= ur * xr - ui * xi;
-        si -= ur * xi + ui * xr;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        sr += dr[u] * xr[u] - di[u] * xi[u];
+        si += dr[u] * xi[u] + di[u] * xr[u];
       }
-      const T dr = D[2 * (i * PNB + i)], di = D[2 * (i * PNB + i) + 1];
-      const T den = dr * dr + di * di;
-      Ui[2 * (i * PNB + c)] = (sr * dr + si * di) / den;
-      Ui[2 * (i * PNB + c) + 1] = (si * dr - sr * di) / den;
     }
+    for (; q < i; ++q) {
+      const T dr = D[2 * (i * DL + q)], di = D[2 * (i * DL + q) + 1];
+      const T xr = X[2 * (q * PNB + c)], xi = X[2 * (q * PNB + c) + 1];
+      sr += dr * xr - di * xi;
+      si += dr * xi + di * xr;
+    }
+    X[2 * (i * PNB + c)] = (i == c ? T(1) : T(0)) - sr;
+    X[2 * (i * PNB + c) + 1] = -si;
   }
+  for (int i = 0; i < nb; ++i)
+    if (c < nb) {
+      Li[2 * (i * PNB + c)] = X[2 * (i * PNB + c)];
+      Li[2 * (i * PNB + c) + 1] = X[2 * (i * PNB + c) + 1];
+    }
+  // upper: U X = I, column c; X(i,c) = ([i == c] - sum_{q>i} D(i,q) X(q,c)) / D(i,i)
+  for (int i = nb - 1; i >= 0; --i) {
+    T sr = (i == c) ? T(1) : T(0), si = 0;
+    int q = i + 1;
+    for (; q + 8 <= nb; q += 8) {
+      T dr[8], di[8], xr[8], xi[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        dr[u] = D[2 * (i * DL + q + u)];
+        di[u] = D[2 * (i * DL + q + u) + 1];
+        xr[u] = X[2 * ((q + u) * PNB + c)];
+        xi[u] = X[2 * ((q + u) * PNB + c) + 1];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        sr -= dr[u] * xr[u] - di[u] * xi[u];
+        si -= dr[u] * xi[u] + di[u] * xr[u];
+      }
+    }
+    for (; q < nb; ++q) {
+      const T dr = D[2 * (i * DL + q)], di = D[2 * (i * DL + q) + 1];
+      const T xr = X[2 * (q * PNB + c)], xi = X[2 * (q * PNB + c) + 1];
+      sr -= dr * xr - di * xi;
+      si -= dr * xi + di * xr;
+    }
+    const T pr = D[2 * (i * DL + i)], pi = D[2 * (i * DL + i) + 1];
+    const T den = pr * pr + pi * pi;
+    X[2 * (i * PNB + c)] = (sr * pr + si * pi) / den;
+    X[2 * (i * PNB + c) + 1] = (si * pr - sr * pi) / den;
+  }
+  for (int i = 0; i < nb; ++i)
+    if (c < nb) {
+      Ui[2 * (i * PNB + c)] = X[2 * (i * PNB + c)];
+      Ui[2 * (i * PNB + c) + 1] = X[2 * (i * PNB + c) + 1];
+    }
 }
 
 // Row gather tables of the forward solve. The L columns of outer block J are
@@ -750,20 +793,51 @@ __global__ __launch_bounds__(64) void diag_inverse_kernel(const T* LU, int64_t n
 // swap them), while the solve runs in the final order. For final row r,
 // G[J][r] is the row it occupied after block J's exchanges, found by undoing the
 // exchanges of blocks J+1.. in reverse; perm[r] is its original row.
-__global__ void gather_table_kernel(const int* ipiv, int64_t n, int bw, int nJ, int* G, int* perm) {
+constexpr int GT_CHUNK = 8192;  // exchanges staged in LDS per round (a multiple of the block width)
+__global__ __launch_bounds__(256) void gather_table_kernel(const int* ipiv, int64_t n, int bw, int nJ, int* G,
+                                                           int* perm) {
+  __shared__ int4 sp[GT_CHUNK / 4];
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
   int q = (int)r;
-  for (int J = nJ - 1; J >= 0; --J) {
-    G[(int64_t)J * n + r] = q;
-    const int64_t k0 = (int64_t)J * bw, k1 = min(n, k0 + bw);
-    for (int64_t k = k1 - 1; k >= k0; --k) {
-      const int p = ipiv[k];
-      if (q == (int)k) q = p;
-      else if (q == p) q = (int)k;
+  int J = nJ - 1;
+  // chunks from the last exchanges backwards; each holds whole blocks
+  for (int64_t c1 = n; c1 > 0;) {
+    const int64_t c0 = std::max<int64_t>(0, ((c1 - 1) / GT_CHUNK) * GT_CHUNK);
+    __syncthreads();
+    for (int64_t k = c0 + threadIdx.x; k < c0 + GT_CHUNK; k += blockDim.x)
+      reinterpret_cast<int*>(sp)[k - c0] = k < c1 ? ipiv[k] : (int)k;  // padding: no-op exchanges
+    __syncthreads();
+    for (; J >= 0 && (int64_t)J * bw >= c0; --J) {
+      if (r < n) G[(int64_t)J * n + r] = q;
+      const int64_t k0 = (int64_t)J * bw, k1 = min(n, k0 + bw);
+      // exchanges k1-1 .. k0 in reverse; read four at a time from LDS
+      int64_t k = k1;
+      while (k > k0 && (k & 3)) {
+        --k;
+        const int p = reinterpret_cast<const int*>(sp)[k - c0];
+        if (q == (int)k) q = p;
+        else if (q == p) q = (int)k;
+      }
+      for (; k - 4 >= k0; k -= 4) {
+        const int4 p4 = sp[(k - 4 - c0) >> 2];
+        const int ps[4] = {p4.x, p4.y, p4.z, p4.w};
+#pragma unroll
+        for (int u = 3; u >= 0; --u) {
+          const int kk = (int)(k - 4 + u);
+          if (q == kk) q = ps[u];
+          else if (q == ps[u]) q = kk;
+        }
+      }
+      while (k > k0) {
+        --k;
+        const int p = reinterpret_cast<const int*>(sp)[k - c0];
+        if (q == (int)k) q = p;
+        else if (q == p) q = (int)k;
+      }
     }
+    c1 = c0;
   }
-  perm[r] = q;
+  if (r < n) perm[r] = q;
 }
 
 // dst (n x 8, zero-padded) = rows perm[r] of src (n x nrhs)
@@ -931,7 +1005,7 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
 // x staged in LDS ([d][k], conflict-free) in chunks shared by the workgroup's 8
 // wavefronts (32 rows). The 64 partial sums per lane are then reduced across the
 // wavefront by recursive halving (63 exchanges), leaving sum number l in lane l.
-constexpr int RES_WAVES = 8, RES_ROWS = 4, RES_KC = 256;
+constexpr int RES_WAVES = 8, RES_ROWS = 4, RES_KC = 128;
 
 template <int N>
 __device__ __forceinline__ void halve_step(double* v, int lane) {
@@ -952,8 +1026,23 @@ __global__ __launch_bounds__(512) void residual64_kernel(const double* A, int64_
   double acc[RES_ROWS * 8 * 2];
 #pragma unroll
   for (int j = 0; j < RES_ROWS * 8 * 2; ++j) acc[j] = 0;
+  constexpr int SUB = RES_KC / 64;
   for (int64_t k0 = 0; k0 < n; k0 += RES_KC) {
     const int kc = (int)min((int64_t)RES_KC, n - k0);
+    // this chunk's A values first (all loads in flight), then x into LDS
+    double ar[RES_ROWS][SUB], ai[RES_ROWS][SUB];
+#pragma unroll
+    for (int q = 0; q < RES_ROWS; ++q)
+#pragma unroll
+      for (int s = 0; s < SUB; ++s) {
+        const int kk = lane + 64 * s;
+        const int64_t row = row0 + q;
+        ar[q][s] = ai[q][s] = 0;
+        if (row < n && kk < kc) {
+          ar[q][s] = A[2 * (row * lda + k0 + kk)];
+          ai[q][s] = A[2 * (row * lda + k0 + kk) + 1];
+        }
+      }
     __syncthreads();
     for (int idx = tid; idx < RES_KC * 8; idx += 64 * RES_WAVES) {
       const int kk = idx >> 3, d = idx & 7;
@@ -966,7 +1055,9 @@ __global__ __launch_bounds__(512) void residual64_kernel(const double* A, int64_
       xs[2 * (d * RES_KC + kk) + 1] = xi;
     }
     __syncthreads();
-    for (int kk = lane; kk < kc; kk += 64) {
+#pragma unroll
+    for (int s = 0; s < SUB; ++s) {
+      const int kk = lane + 64 * s;
       double xr[8], xi[8];
 #pragma unroll
       for (int d = 0; d < 8; ++d) {
@@ -974,17 +1065,12 @@ __global__ __launch_bounds__(512) void residual64_kernel(const double* A, int64_
         xi[d] = xs[2 * (d * RES_KC + kk) + 1];
       }
 #pragma unroll
-      for (int q = 0; q < RES_ROWS; ++q) {
-        const int64_t row = row0 + q;
-        if (row < n) {
-          const double ar = A[2 * (row * lda + k0 + kk)], ai = A[2 * (row * lda + k0 + kk) + 1];
+      for (int q = 0; q < RES_ROWS; ++q)
 #pragma unroll
-          for (int d = 0; d < 8; ++d) {
-            acc[(q * 8 + d) * 2] = fma(ar, xr[d], fma(-ai, xi[d], acc[(q * 8 + d) * 2]));
-            acc[(q * 8 + d) * 2 + 1] = fma(ar, xi[d], fma(ai, xr[d], acc[(q * 8 + d) * 2 + 1]));
-          }
+        for (int d = 0; d < 8; ++d) {
+          acc[(q * 8 + d) * 2] = fma(ar[q][s], xr[d], fma(-ai[q][s], xi[d], acc[(q * 8 + d) * 2]));
+          acc[(q * 8 + d) * 2 + 1] = fma(ar[q][s], xi[d], fma(ai[q][s], xr[d], acc[(q * 8 + d) * 2 + 1]));
         }
-      }
     }
   }
   halve_step<32>(acc, lane);
