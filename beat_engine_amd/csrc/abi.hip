@@ -61,7 +61,7 @@ struct beat_ctx {
   DevBuf a64, a32, b64, b32, q64, q32, x64, r64, n64, rn64, xn64, d32, work, chunk, norms, ipiv, lusmall, qin, pout,
       fld_src, fld_out, fld_pts;
   hipEvent_t ev[6] = {};
-  double* trig = nullptr;  // (sin, cos)(m pi/32) table of the FP64 phase
+  double* trig = nullptr;  // (sin, cos)(m pi/64) table of the FP64 phase
 };
 
 struct beat_mesh {
@@ -680,7 +680,7 @@ int beat_ctx_create(int device, beat_ctx** out) {
     return BEAT_ERR_HIP;
   }
   {
-    double t[128];
+    double t[2 * TRIG_N];
     fill_trig_table(t);
     if (hipMalloc((void**)&c->trig, sizeof(t)) != hipSuccess ||
         hipMemcpy(c->trig, t, sizeof(t), hipMemcpyHostToDevice) != hipSuccess) {

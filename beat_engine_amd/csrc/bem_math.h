@@ -51,28 +51,27 @@ BEAT_HD void sincos_reduced(double x, double* s, double* c) {
   *c = cc;
 }
 
-// Table form for |x| < SINCOS_TABLE_LIMIT (below 2^22 pi/32, where n * part stays
-// exact): n = rint(x 32/pi), y = x - n pi/32 with the pi/2 split above scaled by
-// 1/16 (exact), |y| <= pi/64; Taylor polynomials of sin to y^9 and cos to y^8
-// (first omitted terms below 1e-20 absolute), then the rotation by the table
-// entry (sin, cos)(m pi/32), m = n mod 64: sin x = S cos y + C sin y,
+// Table form for |x| < SINCOS_TABLE_LIMIT (below 2^23 pi/64, where n * part stays
+// exact): n = rint(x 64/pi), y = x - n pi/64 with the pi/2 split above scaled by
+// 1/32 (exact), |y| <= pi/128; Taylor polynomials of sin to y^7 and cos to y^6
+// (first omitted terms below 1e-17 absolute), then the rotation by the table
+// entry (sin, cos)(m pi/64), m = n mod 128: sin x = S cos y + C sin y,
 // cos x = C cos y - S sin y.
 constexpr double SINCOS_TABLE_LIMIT = 4.0e5;
+constexpr int TRIG_N = 128;  // table entries (sin, cos) over one period
 
 BEAT_HD void sincos_table(double x, const double* trig, double* s, double* c) {
-  const double n = rint(x * 10.185916357881302);  // 32/pi
-  double y = fma(-n, 1.5707963276654482 / 16, x);
-  y = fma(-n, -8.705515692000731e-10 / 16, y);
-  y = fma(-n, -3.50343439808993e-19 / 16, y);
-  const int m = (int)n & 63;
+  const double n = rint(x * 20.371832715762604);  // 64/pi
+  double y = fma(-n, 1.5707963276654482 / 32, x);
+  y = fma(-n, -8.705515692000731e-10 / 32, y);
+  y = fma(-n, -3.50343439808993e-19 / 32, y);
+  const int m = (int)n & (TRIG_N - 1);
   const double y2 = y * y;
-  double ps = 2.7557319223985893e-06;        // 1/9!
-  ps = fma(ps, y2, -0.0001984126984126984);  // -1/7!
+  double ps = -0.0001984126984126984;        // -1/7!
   ps = fma(ps, y2, 0.008333333333333333);    // 1/5!
   ps = fma(ps, y2, -0.16666666666666666);    // -1/3!
   const double sy = fma(ps * y2, y, y);
-  double pc = 2.48015873015873e-05;          // 1/8!
-  pc = fma(pc, y2, -0.001388888888888889);   // -1/6!
+  double pc = -0.001388888888888889;         // -1/6!
   pc = fma(pc, y2, 0.041666666666666664);    // 1/4!
   pc = fma(pc, y2, -0.5);
   const double cy = fma(pc, y2, 1.0);
@@ -81,12 +80,12 @@ BEAT_HD void sincos_table(double x, const double* trig, double* s, double* c) {
   *c = fma(C, cy, -(S * sy));
 }
 
-// (sin, cos)(m pi/32) for m = 0..63, interleaved, evaluated in long double and
-// rounded once (host code).
+// (sin, cos)(m pi/64) for m = 0..TRIG_N-1, interleaved, evaluated in long double
+// and rounded once (host code).
 inline void fill_trig_table(double* t) {
   const long double pi = 3.141592653589793238462643383279502884L;
-  for (int m = 0; m < 64; ++m) {
-    const long double a = (long double)m * (pi / 32.0L);
+  for (int m = 0; m < TRIG_N; ++m) {
+    const long double a = (long double)m * (pi / 64.0L);
     t[2 * m] = (double)sinl(a);
     t[2 * m + 1] = (double)cosl(a);
   }

@@ -33,7 +33,7 @@ struct RegularArgs {
   T eta_re, eta_im;
   T w[6];                  // rule weights
   T phi[6][3];             // P1 basis at the rule points
-  const double* trig;      // 128 doubles: (sin, cos)(m pi/32), m = 0..63
+  const double* trig;      // 2 TRIG_N doubles: (sin, cos)(m pi/64)
 };
 
 template <typename TO>

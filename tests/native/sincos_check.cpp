@@ -27,8 +27,8 @@ int main() {
     check(std::nextafter(b, -1e9));
   }
   check(0.0);
-  // table form, over its range and across the table's pi/32 steps
-  double trig[128];
+  // table form, over its range
+  double trig[2 * beat::TRIG_N];
   beat::fill_trig_table(trig);
   double wt_s = 0, wt_c = 0;
   auto check_t = [&](double x) {
@@ -40,12 +40,12 @@ int main() {
   for (int i = 0; i < 2000000; ++i) check_t(u(rng) * 2000.0);
   for (int i = 0; i < 200000; ++i) check_t(-u(rng) * 50.0);
   for (int i = 0; i < 200000; ++i) check_t((2.0 * u(rng) - 1.0) * beat::SINCOS_TABLE_LIMIT);
-  for (int k = -200; k <= 200; ++k) {
-    const double b = k * (3.141592653589793 / 32.0);
+  for (int k = -400; k <= 400; ++k) {  // across the table's pi/64 steps and half-steps
+    const double b = k * (3.141592653589793 / 64.0);
     check_t(b);
     check_t(std::nextafter(b, 1e9));
     check_t(std::nextafter(b, -1e9));
-    check_t(b + 3.141592653589793 / 64.0);
+    check_t(b + 3.141592653589793 / 128.0);
   }
   check_t(0.0);
   std::printf("%.3e %.3e %.3e %.3e\n", worst_s, worst_c, wt_s, wt_c);
