@@ -26,7 +26,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "freq-solves/s @N=20480,8 RHS: FP32-MFMA+IR vs FP64 GPU at ≤1e-8 rel error"
 FP64_VALU_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 FLOP_PER_QPAIR = 72  # fused Burton-Miller flops per quadrature-point pair
+# HBM bytes per launch of the FP64 regular kernel at L5, from two rocprofv3 --pmc
+# runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors make a 1 GiB device copy
+# read as 1 GiB each way); profiles/bm_regular_L5_pmc_traffic.md
+REGULAR_L5_TRAFFIC_BYTES = 21.12e9
 
 
 def parse():
@@ -184,6 +189,9 @@ def main():
     qpairs = counts["regular_pairs"] * counts["rule_points"] ** 2
     flops = qpairs * FLOP_PER_QPAIR
     achieved = flops / reg_s / 1e12
+    n_p1 = mesh.num_vertices
+    lu_flop = 8.0 * n_p1 ** 3 / 3.0
+    lu_s = float(np.mean([r.factor_s for r in reps]))
     out = {
         "metric": METRIC,
         "value": 1.0 / sec,
@@ -222,8 +230,15 @@ def main():
             "peak": FP64_VALU_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_VALU_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": REGULAR_L5_TRAFFIC_BYTES if args.level == 5 else None,
             "algorithmic_flop_per_launch": flops,
+        },
+        "lu_c64_mfma": {
+            "flop": lu_flop,
+            "achieved": lu_flop / lu_s / 1e12,
+            "peak": FP32_MFMA_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": lu_flop / lu_s / 1e12 / FP32_MFMA_PEAK_TFLOPS,
         },
     }
     if not args.no_cpu_baseline:
