@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--nrhs", type=int, default=8)
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--counter-run", action="store_true",
+                    help="for rocprofv3 --pmc: a 1 GiB device copy (counter calibration), then one complex128 "
+                         "freq-solve of the workload; prints no JSON line")
     ap.add_argument("--cpu-sample-faces", type=int, default=192)
     ap.add_argument("--cpu-lu-n", type=int, default=3072)
     return ap.parse_args()
@@ -121,6 +124,20 @@ def main():
         raise RuntimeError("bench.py needs a GPU")
     mesh = jittered_icosphere(args.level, 0.1, 0.02, seed=20480)
     nrhs = args.nrhs
+    if args.counter_run:
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        dm = Engine(0).upload(mesh)
+        rng = np.random.default_rng(20480)
+        v = rng.standard_normal((mesh.num_faces, nrhs)) + 1j * rng.standard_normal((mesh.num_faces, nrhs))
+        k = 2 * np.pi * 1000.0 / 343.0
+        q = neumann_scale(1.21, 2 * np.pi * 1000.0) * v
+        _, rep = dm.solve(k, burton_miller_coupling(k), q, precision="float64")
+        log(f"counter run: regular kernel {rep['regular_kernel_s']:.4f} s")
+        return
     freqs = np.logspace(np.log10(100.0), np.log10(5000.0), 20)
     rng = np.random.default_rng(20480)
     v = rng.normal(size=(mesh.num_faces, nrhs)) + 1j * rng.normal(size=(mesh.num_faces, nrhs))
