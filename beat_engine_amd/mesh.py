@@ -276,3 +276,61 @@ def morphed_icosphere(level: int = 4, radius: float = 0.1, n_harmonics: int = 5,
     r = radius + disp
     r = np.maximum(r, 0.25 * radius)
     return BoundaryMesh(u * r[:, None], base.faces, base.physical_tags)
+
+
+# ---- symmetry planes (src/beat_engine/julia_local/src/BeatEngineCore.jl:448-580) ----
+SYMMETRY_MODES = ("off", "x", "xy", "ground")
+
+
+def normalized_symmetry_mode(mode) -> str:
+    """normalized_symmetry_mode (BeatEngineCore.jl:448-454)."""
+    m = str(mode).strip().lower()
+    if m not in SYMMETRY_MODES:
+        raise ValueError(f"Unsupported image mode: {mode}. Expected off, x, xy, or ground.")
+    return m
+
+
+def symmetry_active_axes(mode) -> tuple:
+    m = normalized_symmetry_mode(mode)
+    return (0,) if m == "x" else (0, 1) if m == "xy" else ()
+
+
+def symmetry_plane_tolerance(vertices, absolute_floor: float = 1e-9, relative_tolerance: float = 1e-6) -> float:
+    """max(floor, bounding-box diagonal x relative) (BeatEngineCore.jl:497-517)."""
+    v = np.asarray(vertices, dtype=np.float64)
+    scale = 0.0 if len(v) == 0 else float(np.linalg.norm(v.max(0) - v.min(0)))
+    return max(absolute_floor, scale * relative_tolerance)
+
+
+def snap_symmetry_planes(mesh: BoundaryMesh, mode, tolerance: float | None = None) -> BoundaryMesh:
+    """Vertices within the tolerance of an active symmetry plane moved onto it
+    (snap_symmetry_plane_vertices / snap_symmetry_planes, BeatEngineCore.jl:519-542)."""
+    tol = symmetry_plane_tolerance(mesh.vertices) if tolerance is None else tolerance
+    v = mesh.vertices.copy()
+    for axis in symmetry_active_axes(mode):
+        v[np.abs(v[:, axis]) <= tol, axis] = 0.0
+    if np.array_equal(v, mesh.vertices):
+        return mesh
+    return BoundaryMesh(v, mesh.faces, mesh.physical_tags)
+
+
+def validate_symmetry_fundamental_domain(mesh: BoundaryMesh, mode, tolerance: float | None = None) -> None:
+    """The mesh must lie in the positive half-space of every active plane
+    (validate_symmetry_fundamental_domain!, BeatEngineCore.jl:544-561)."""
+    m = normalized_symmetry_mode(mode)
+    tol = symmetry_plane_tolerance(mesh.vertices) if tolerance is None else tolerance
+    for axis in symmetry_active_axes(m):
+        bad = np.nonzero(mesh.vertices[:, axis] < -tol)[0]
+        if len(bad):
+            name = "XYZ"[axis]
+            raise ValueError(f"Mesh is not in the positive {name} fundamental domain for {m.upper()} symmetry. "
+                             f"Vertex {bad[0] + 1} has {name.lower()}={mesh.vertices[bad[0], axis]} m.")
+
+
+def p1_symmetry_orbit_weights(mesh: BoundaryMesh, mode, tolerance: float | None = None) -> np.ndarray:
+    """2 per active plane a vertex lies on (p1_symmetry_orbit_weights, BeatEngineCore.jl:563-580)."""
+    tol = symmetry_plane_tolerance(mesh.vertices) if tolerance is None else tolerance
+    w = np.ones(mesh.num_vertices)
+    for axis in symmetry_active_axes(mode):
+        w[np.abs(mesh.vertices[:, axis]) <= tol] *= 2.0
+    return w

@@ -52,10 +52,14 @@ def lib():
         L.oracle_singular_pairs.argtypes = [P, I64, P, I64, I, P]
         L.oracle_singular_pairs.restype = I64
         for s in ("f64", "f32"):
-            getattr(L, f"oracle_bm_assemble_{s}").argtypes = [P, I64, P, I64, I, I, D, D, D, P, I64, P, P, I, I, I, I]
-            getattr(L, f"oracle_operators_{s}").argtypes = [P, I64, P, I64, I, I, D, P, P, P, P]
-            getattr(L, f"oracle_field_{s}").argtypes = [P, I64, P, I64, I, D, P, P, I64, P, I64, P]
-        L.oracle_identity_f64.argtypes = [P, I64, P, I64, I, P, P]
+            getattr(L, f"oracle_bm_assemble_{s}").argtypes = [P, I64, P, I64, I, I, D, D, D, P, I64, P, P, I, I, I, I,
+                                                              I]
+            getattr(L, f"oracle_operators_{s}").argtypes = [P, I64, P, I64, I, I, D, P, P, P, P, I]
+            getattr(L, f"oracle_field_{s}").argtypes = [P, I64, P, I64, I, D, P, P, I64, P, I64, P, I]
+        L.oracle_identity_f64.argtypes = [P, I64, P, I64, I, P, P, I]
+        L.oracle_orbit_weights.argtypes = [P, I64, P, I64, I, P]
+        L.oracle_image_singular_pairs.argtypes = [P, I64, P, I64, I, I, I, P]
+        L.oracle_image_singular_pairs.restype = I64
         L.oracle_surface_curls.argtypes = [P, I64, P, I64, P]
         L.oracle_singular_pair_points.argtypes = [P, I64, P, I64, I, I64, P, P, P]
         L.oracle_singular_pair_points.restype = I64
@@ -113,6 +117,17 @@ def duffy_rule(order, kind):
     return tp, sp, w
 
 
+# normalized_symmetry_mode (JL/src/BeatEngineCore.jl:448-454): off, x, xy, ground
+SYMMETRY_MODES = {"off": 0, "x": 1, "xy": 2, "ground": 3}
+
+
+def _sym(mode):
+    m = str(mode).strip().lower()
+    if m not in SYMMETRY_MODES:
+        raise ValueError(f"Unsupported image mode: {mode}. Expected off, x, xy, or ground.")
+    return SYMMETRY_MODES[m]
+
+
 def _mesh_args(vertices, faces):
     V = np.ascontiguousarray(vertices, dtype=np.float64)
     F = np.ascontiguousarray(faces, dtype=np.int32)
@@ -139,6 +154,25 @@ def singular_pair_points(vertices, faces, index, order=4):
     return x, y, w
 
 
+def orbit_weights(vertices, faces, symmetry="off"):
+    """p1_symmetry_orbit_weights: 2 per active symmetry plane a vertex lies on."""
+    V, F = _mesh_args(vertices, faces)
+    out = np.zeros(len(V))
+    lib().oracle_orbit_weights(_p(V), len(V), _p(F), len(F), _sym(symmetry), _p(out))
+    return out
+
+
+def image_singular_pairs(vertices, faces, symmetry, image, order=4):
+    """(test, trial, kind, rule) of the image-singular pairs of one image transform."""
+    V, F = _mesh_args(vertices, faces)
+    n = lib().oracle_image_singular_pairs(_p(V), len(V), _p(F), len(F), order, _sym(symmetry), image, None)
+    if n < 0:
+        raise ValueError("no such image transform")
+    out = np.zeros((n, 4), dtype=np.int32)
+    lib().oracle_image_singular_pairs(_p(V), len(V), _p(F), len(F), order, _sym(symmetry), image, _p(out))
+    return out
+
+
 def surface_curls(vertices, faces):
     V, F = _mesh_args(vertices, faces)
     out = np.zeros((len(F), 3, 3))
@@ -148,7 +182,7 @@ def surface_curls(vertices, faces):
 
 # ---- assembly ---------------------------------------------------------------------
 def bm_assemble(vertices, faces, k, coupling, q, quad_order=4, singular_order=4, precision="f64",
-                test_range=None, singular=True, identity=True):
+                test_range=None, singular=True, identity=True, symmetry="off"):
     """Fused Burton-Miller system (A, B): A = 0.5 M - D + eta H, B = (-S - eta(K' + 0.5 M)) q.
 
     k is the signed (outgoing) wavenumber, coupling the signed eta; q is F x nrhs.
@@ -168,12 +202,13 @@ def bm_assemble(vertices, faces, k, coupling, q, quad_order=4, singular_order=4,
     lo, hi = (0, nf) if test_range is None else test_range
     fn = lib().oracle_bm_assemble_f64 if precision == "f64" else lib().oracle_bm_assemble_f32
     fn(_p(V), nv, _p(F), nf, quad_order, singular_order, float(k), float(coupling.real), float(coupling.imag),
-       _p(qf), nrhs, _p(A), _p(B), int(lo), int(hi), int(bool(singular)), int(bool(identity)))
+       _p(qf), nrhs, _p(A), _p(B), int(lo), int(hi), int(bool(singular)), int(bool(identity)), _sym(symmetry))
     return A, B
 
 
-def operators(vertices, faces, k, quad_order=4, singular_order=4, precision="f64"):
-    """S (N x F), D (N x N), K' (N x F), H (N x N) with the singular pass."""
+def operators(vertices, faces, k, quad_order=4, singular_order=4, precision="f64", symmetry="off"):
+    """S (N x F), D (N x N), K' (N x F), H (N x N) with the singular pass (and the
+    image passes, image-singular deltas and orbit row weights of a symmetry mode)."""
     V, F = _mesh_args(vertices, faces)
     nv, nf = len(V), len(F)
     ct = np.complex128 if precision == "f64" else np.complex64
@@ -182,16 +217,16 @@ def operators(vertices, faces, k, quad_order=4, singular_order=4, precision="f64
     D = np.zeros((nv, nv), dtype=ct, order="F")
     H = np.zeros((nv, nv), dtype=ct, order="F")
     fn = lib().oracle_operators_f64 if precision == "f64" else lib().oracle_operators_f32
-    fn(_p(V), nv, _p(F), nf, quad_order, singular_order, float(k), _p(S), _p(D), _p(K), _p(H))
+    fn(_p(V), nv, _p(F), nf, quad_order, singular_order, float(k), _p(S), _p(D), _p(K), _p(H), _sym(symmetry))
     return S, D, K, H
 
 
-def identity(vertices, faces, quad_order=4):
+def identity(vertices, faces, quad_order=4, symmetry="off"):
     V, F = _mesh_args(vertices, faces)
     nv, nf = len(V), len(F)
     Mpp = np.zeros((nv, nv), order="F")
     Mpd = np.zeros((nv, nf), order="F")
-    lib().oracle_identity_f64(_p(V), nv, _p(F), nf, quad_order, _p(Mpp), _p(Mpd))
+    lib().oracle_identity_f64(_p(V), nv, _p(F), nf, quad_order, _p(Mpp), _p(Mpd), _sym(symmetry))
     return Mpp, Mpd
 
 
@@ -205,7 +240,7 @@ def bm_from_operators(S, D, K, H, Mpp, Mpd, coupling, q):
     return lhs, rhs
 
 
-def field(vertices, faces, k, pressure, q, points, quad_order=4, precision="f64"):
+def field(vertices, faces, k, pressure, q, points, quad_order=4, precision="f64", symmetry="off"):
     V, F = _mesh_args(vertices, faces)
     nv, nf = len(V), len(F)
     ct = np.complex128 if precision == "f64" else np.complex64
@@ -220,7 +255,8 @@ def field(vertices, faces, k, pressure, q, points, quad_order=4, precision="f64"
     pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
     out = np.zeros((len(pts), p.shape[1]), dtype=ct, order="F")
     fn = lib().oracle_field_f64 if precision == "f64" else lib().oracle_field_f32
-    fn(_p(V), nv, _p(F), nf, quad_order, float(k), _p(p), _p(qq), p.shape[1], _p(pts), len(pts), _p(out))
+    fn(_p(V), nv, _p(F), nf, quad_order, float(k), _p(p), _p(qq), p.shape[1], _p(pts), len(pts), _p(out),
+       _sym(symmetry))
     return out
 
 

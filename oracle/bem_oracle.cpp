@@ -231,6 +231,39 @@ template <class T> struct SingularCache {
   std::vector<std::array<int, 5>> keys;
 };
 
+// singular_orientation_key / rule_for_singular_orientation! / remapped_duffy_rule:
+// JL/src/BeatEngineCore.jl:994-1047 (rules deduplicated by orientation key).
+template <class T>
+int rule_for_orientation(SingularCache<T>& c, std::map<std::array<int, 5>, int>& rule_index, const Duffy<T> base[3],
+                         const AdjInfo& info) {
+  std::array<int, 5> key;
+  if (info.kind == 1) key = {1, 0, 0, 0, 0};
+  else if (info.kind == 2) key = {2, info.ta, info.tb, info.sa, info.sb};
+  else key = {3, info.ta, 0, info.sa, 0};
+  auto it = rule_index.find(key);
+  if (it != rule_index.end()) return it->second;
+  const Duffy<T>& b = base[info.kind - 1];
+  Duffy<T> r;
+  r.w = b.w;
+  for (size_t q = 0; q < b.w.size(); ++q) {
+    P2<T> a = b.tp[q], s2 = b.sp[q];
+    if (info.kind == 2) {
+      a = remap_edge(a, key[1], key[2]);
+      s2 = remap_edge(s2, key[3], key[4]);
+    } else if (info.kind == 3) {
+      a = remap_vertex(a, key[1]);
+      s2 = remap_vertex(s2, key[3]);
+    }
+    r.tp.push_back(a);
+    r.sp.push_back(s2);
+  }
+  c.rules.push_back(r);
+  c.keys.push_back(key);
+  const int ri = (int)c.rules.size() - 1;
+  rule_index[key] = ri;
+  return ri;
+}
+
 template <class T> SingularCache<T> build_singular_cache(const Mesh<T>& m, int order) {
   const int nf = (int)m.faces.size();
   std::map<int, std::vector<int>> v2e;
@@ -252,38 +285,117 @@ template <class T> SingularCache<T> build_singular_cache(const Mesh<T>& m, int o
     for (int s : cand) {
       AdjInfo info = adjacency_info(m.faces[t], m.faces[s]);
       if (info.kind == 0) continue;
-      std::array<int, 5> key;
-      if (info.kind == 1) key = {1, 0, 0, 0, 0};
-      else if (info.kind == 2) key = {2, info.ta, info.tb, info.sa, info.sb};
-      else key = {3, info.ta, 0, info.sa, 0};
-      auto it = rule_index.find(key);
-      int ri;
-      if (it == rule_index.end()) {
-        const Duffy<T>& b = base[info.kind - 1];
-        Duffy<T> r;
-        r.w = b.w;
-        for (size_t q = 0; q < b.w.size(); ++q) {
-          P2<T> a = b.tp[q], s2 = b.sp[q];
-          if (info.kind == 2) {
-            a = remap_edge(a, key[1], key[2]);
-            s2 = remap_edge(s2, key[3], key[4]);
-          } else if (info.kind == 3) {
-            a = remap_vertex(a, key[1]);
-            s2 = remap_vertex(s2, key[3]);
-          }
-          r.tp.push_back(a);
-          r.sp.push_back(s2);
-        }
-        c.rules.push_back(r);
-        c.keys.push_back(key);
-        ri = (int)c.rules.size() - 1;
-        rule_index[key] = ri;
-      } else {
-        ri = it->second;
-      }
+      const int ri = rule_for_orientation(c, rule_index, base, info);
       T jac = (T(2.0) * m.areas[t]) * (T(2.0) * m.areas[s]);
       c.by_test[t].push_back({t, s, ri, jac, dot(m.normals[t], m.normals[s])});
     }
+  }
+  return c;
+}
+
+// ---- symmetry images ------------------------------------------------------------
+// normalized_symmetry_mode / symmetry_transforms / reflect_point / reflect_normal /
+// reflect_curl: JL/src/BeatEngineCore.jl:448-495. Modes: 0 off, 1 x, 2 xy, 3 ground.
+struct Xf {
+  int s[3];
+  int det;
+};
+inline std::vector<Xf> image_transforms(int mode) {
+  std::vector<Xf> t;
+  if (mode == 1) {
+    t.push_back({{-1, 1, 1}, -1});
+  } else if (mode == 2) {
+    t.push_back({{-1, 1, 1}, -1});
+    t.push_back({{1, -1, 1}, -1});
+    t.push_back({{-1, -1, 1}, 1});
+  } else if (mode == 3) {
+    t.push_back({{1, -1, 1}, -1});  // rigid_ground_transform
+  }
+  return t;
+}
+template <class T> V3<T> reflect(const Xf& x, V3<T> p) { return {T(x.s[0]) * p.x, T(x.s[1]) * p.y, T(x.s[2]) * p.z}; }
+template <class T> V3<T> reflect_curl(const Xf& x, V3<T> c) {
+  return {T(x.det * x.s[0]) * c.x, T(x.det * x.s[1]) * c.y, T(x.det * x.s[2]) * c.z};
+}
+
+// symmetry_plane_tolerance / p1_symmetry_orbit_weights: JL/src/BeatEngineCore.jl:497-517,563-580.
+template <class T> std::vector<T> orbit_weights(const Mesh<T>& m, int mode) {
+  std::vector<T> w(m.verts.size(), T(1));
+  if ((mode != 1 && mode != 2) || m.verts.empty()) return w;
+  V3<T> lo = m.verts[0], hi = m.verts[0];
+  for (auto& v : m.verts) {
+    lo = {std::min(lo.x, v.x), std::min(lo.y, v.y), std::min(lo.z, v.z)};
+    hi = {std::max(hi.x, v.x), std::max(hi.y, v.y), std::max(hi.z, v.z)};
+  }
+  const T tol = std::max(T(1e-9), norm(hi - lo) * T(1e-6));
+  for (size_t i = 0; i < m.verts.size(); ++i) {
+    T wt = T(1);
+    if (std::abs(m.verts[i].x) <= tol) wt *= T(2);
+    if (mode == 2 && std::abs(m.verts[i].y) <= tol) wt *= T(2);
+    w[i] = wt;
+  }
+  return w;
+}
+
+// geometry_key / geometric_adjacency_info: JL/src/BeatEngineCore.jl:624-658.
+template <class T> std::array<long long, 3> geometry_key(V3<T> p, T tol) {
+  return {(long long)std::nearbyint(p.x / tol), (long long)std::nearbyint(p.y / tol),
+          (long long)std::nearbyint(p.z / tol)};
+}
+template <class T> AdjInfo geometric_adjacency_info(const V3<T> a[3], const V3<T> b[3], T tol) {
+  int sa[9], sb[9], n = 0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      if (norm(a[i] - b[j]) <= tol) {
+        sa[n] = i + 1;
+        sb[n] = j + 1;
+        ++n;
+      }
+  if (n == 3) return {1, sa[0], sa[1], sb[0], sb[1]};
+  if (n == 2) {
+    if (sb[1] < sb[0]) {
+      std::swap(sa[0], sa[1]);
+      std::swap(sb[0], sb[1]);
+    }
+    return {2, sa[0], sa[1], sb[0], sb[1]};
+  }
+  if (n == 1) return {3, sa[0], 0, sb[0], 0};
+  return {0, 0, 0, 0, 0};
+}
+
+// image_singular_candidates + _beat_cpu_image_singular_cache: JL/src/BeatEngineCore.jl:1139-1166,
+// JL/src/BeatEngineCpuAssembly.jl:488-523 (test face touching a reflected trial face).
+template <class T> SingularCache<T> build_image_singular_cache(const Mesh<T>& m, int order, const Xf& x) {
+  const T tol = T(1e-8);
+  const int nf = (int)m.faces.size();
+  std::map<std::array<long long, 3>, std::vector<int>> by_key;
+  for (int t = 0; t < nf; ++t)
+    for (int v : m.faces[t]) by_key[geometry_key(m.verts[v], tol)].push_back(t);
+  std::vector<std::pair<int, int>> cand;
+  std::set<std::pair<int, int>> seen;
+  for (int s = 0; s < nf; ++s)
+    for (int v : m.faces[s]) {
+      auto it = by_key.find(geometry_key(reflect(x, m.verts[v]), tol));
+      if (it == by_key.end()) continue;
+      for (int t : it->second)
+        if (seen.insert({t, s}).second) cand.push_back({t, s});
+    }
+  Duffy<T> base[3] = {duffy_rule<T>(order, 1), duffy_rule<T>(order, 2), duffy_rule<T>(order, 3)};
+  SingularCache<T> c;
+  c.by_test.resize(nf);
+  std::map<std::array<int, 5>, int> rule_index;
+  for (auto& ts : cand) {
+    const int t = ts.first, s = ts.second;
+    V3<T> tv[3], sv[3];
+    for (int i = 0; i < 3; ++i) {
+      tv[i] = m.verts[m.faces[t][i]];
+      sv[i] = reflect(x, m.verts[m.faces[s][i]]);
+    }
+    AdjInfo info = geometric_adjacency_info(tv, sv, tol);
+    if (info.kind == 0) continue;
+    const int ri = rule_for_orientation(c, rule_index, base, info);
+    T jac = (T(2.0) * m.areas[t]) * (T(2.0) * m.areas[s]);
+    c.by_test[t].push_back({t, s, ri, jac, dot(m.normals[t], reflect(x, m.normals[s]))});
   }
   return c;
 }
@@ -336,6 +448,17 @@ std::vector<Elem<T>> element_data(const Mesh<T>& m, const std::vector<P2<T>>& rp
     }
   }
   return el;
+}
+
+// _beat_cpu_reflect_element_data / _beat_cpu_reflect_regular_quadrature_data:
+// JL/src/BeatEngineCpuAssembly.jl:61-100 (same dofs, reflected geometry).
+template <class T> Elem<T> reflect_elem(const Elem<T>& e, const Xf& x) {
+  Elem<T> r = e;
+  for (int i = 0; i < 3; ++i) r.v[i] = reflect(x, e.v[i]);
+  r.n = reflect(x, e.n);
+  for (int a = 0; a < 3; ++a) r.curl[a] = reflect_curl(x, e.curl[a]);
+  for (auto& p : r.qp) p = reflect(x, p);
+  return r;
 }
 
 // _beat_cpu_element_color_groups: JL/src/BeatEngineCpuAssembly.jl:102-130.
@@ -475,16 +598,16 @@ void set_threads() {
 #endif
 }
 
-// assemble_burton_miller_neumann_system_cpu (symmetry off):
-// JL/src/BeatEngineCpuBurtonMiller.jl:276-469. A is N x N column-major, B is
-// N x nrhs column-major, q is F x nrhs column-major (all complex interleaved).
-// test_lo/test_hi restrict the regular pass to a range of test elements (the
-// bounded CPU-baseline sample); the full system uses [0, F).
+// assemble_burton_miller_neumann_system_cpu: JL/src/BeatEngineCpuBurtonMiller.jl:276-469.
+// A is N x N column-major, B is N x nrhs column-major, q is F x nrhs column-major
+// (all complex interleaved). test_lo/test_hi restrict the regular passes to a
+// range of test elements (the bounded CPU-baseline sample); the full system uses
+// [0, F). sym: 0 off, 1 x, 2 xy, 3 ground.
 template <class T>
 void bm_assemble(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int quad_order, int sing_order,
                  double k_signed, double eta_re, double eta_im, const std::complex<T>* q, int64_t nrhs,
                  std::complex<T>* A, std::complex<T>* B, int test_lo, int test_hi, int do_singular,
-                 int do_identity) {
+                 int do_identity, int sym) {
   typedef std::complex<T> C;
   set_threads();
   Mesh<T> m = make_mesh<T>(xyz, nv, f, nf);
@@ -498,14 +621,18 @@ void bm_assemble(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, in
   const int64_t N = nv;
   std::memset((void*)A, 0, sizeof(C) * N * N);
   std::memset((void*)B, 0, sizeof(C) * N * nrhs);
-  auto scatter = [&](const Elem<T>& te, const Elem<T>& se, int sdof, const BMBlock<T>& b) {
-    // _beat_cpu_bm_scatter!: JL/src/BeatEngineCpuBurtonMiller.jl:26-51
+  auto scatter = [&](const Elem<T>& te, const Elem<T>& se, int sdof, const BMBlock<T>& b, T sign = T(1)) {
+    // _beat_cpu_bm_scatter!: JL/src/BeatEngineCpuBurtonMiller.jl:26-51 (sign -1: the subtracting form)
     for (int a = 0; a < 3; ++a) {
       int row = te.p1[a];
-      for (int64_t d = 0; d < nrhs; ++d) B[row + d * N] += b.rhs[a] * q[sdof + d * nf];
-      for (int c = 0; c < 3; ++c) A[row + (int64_t)se.p1[c] * N] += b.lhs[a][c];
+      for (int64_t d = 0; d < nrhs; ++d) B[row + d * N] += sign * b.rhs[a] * q[sdof + d * nf];
+      for (int c = 0; c < 3; ++c) A[row + (int64_t)se.p1[c] * N] += sign * b.lhs[a][c];
     }
   };
+  const std::vector<Xf> xfs = image_transforms(sym);
+  std::vector<std::vector<Elem<T>>> iel(xfs.size());
+  for (size_t x = 0; x < xfs.size(); ++x)
+    for (auto& e : el) iel[x].push_back(reflect_elem(e, xfs[x]));
   for (auto& g : groups) {
 #pragma omp parallel for schedule(dynamic, 1)
     for (int gi = 0; gi < (int)g.size(); ++gi) {
@@ -519,6 +646,20 @@ void bm_assemble(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, in
       }
     }
   }
+  // image passes: every pair, the trial element reflected (_beat_cpu_bm_regular_image_test!, :190-210)
+  for (size_t x = 0; x < xfs.size(); ++x)
+    for (auto& g : groups) {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int gi = 0; gi < (int)g.size(); ++gi) {
+        int t = g[gi];
+        if (t < test_lo || t >= test_hi) continue;
+        BMBlock<T> b;
+        for (int s = 0; s < nf; ++s) {
+          bm_regular_pair(el[t], iel[x][s], rw, k, eta, b);
+          scatter(el[t], iel[x][s], s, b);
+        }
+      }
+    }
   if (do_singular) {
     SingularCache<T> sc = build_singular_cache(m, sing_order);
     for (auto& g : groups) {
@@ -532,7 +673,32 @@ void bm_assemble(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, in
         }
       }
     }
+    // image-singular deltas: Duffy value minus the regular value the image pass added
+    // (_beat_cpu_bm_image_singular_delta_test!, :232-265)
+    for (size_t x = 0; x < xfs.size(); ++x) {
+      SingularCache<T> ic = build_image_singular_cache(m, sing_order, xfs[x]);
+      for (auto& g : groups) {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int gi = 0; gi < (int)g.size(); ++gi) {
+          int t = g[gi];
+          BMBlock<T> b;
+          for (auto& p : ic.by_test[t]) {
+            bm_singular_pair(el[p.test], iel[x][p.trial], p, ic.rules[p.rule], k, eta, b);
+            scatter(el[p.test], iel[x][p.trial], p.trial, b);
+            bm_regular_pair(el[p.test], iel[x][p.trial], rw, k, eta, b);
+            scatter(el[p.test], iel[x][p.trial], p.trial, b, T(-1));
+          }
+        }
+      }
+    }
   }
+  // orbit row weights scale the operator part (:443-449); the identity blocks carry them too
+  const std::vector<T> ow = orbit_weights(m, sym);
+  if (sym != 0)
+    for (int64_t r = 0; r < N; ++r) {
+      for (int64_t c = 0; c < N; ++c) A[r + c * N] *= ow[r];
+      for (int64_t d = 0; d < nrhs; ++d) B[r + d * N] *= ow[r];
+    }
   if (do_identity) {
     // lhs += 0.5 I_p1p1; rhs += (I_p1dp0 q) * (-0.5 eta): BeatEngineCpuBurtonMiller.jl:450-453
     for (int64_t e = 0; e < nf; ++e) {
@@ -540,9 +706,9 @@ void bm_assemble(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, in
       identity_blocks(rp, rw, m.areas[e], pp, pd);
       for (int a = 0; a < 3; ++a) {
         int row = m.faces[e][a];
-        for (int c = 0; c < 3; ++c) A[row + (int64_t)m.faces[e][c] * N] += C(T(0.5)) * C(pp[a][c]);
+        for (int c = 0; c < 3; ++c) A[row + (int64_t)m.faces[e][c] * N] += C(T(0.5)) * C(pp[a][c] * ow[row]);
         for (int64_t d = 0; d < nrhs; ++d)
-          B[row + d * N] += (C(pd[a]) * q[e + d * nf]) * (-C(T(0.5)) * eta);
+          B[row + d * N] += (C(pd[a] * ow[row]) * q[e + d * nf]) * (-C(T(0.5)) * eta);
       }
     }
   }
@@ -553,7 +719,8 @@ void bm_assemble(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, in
 // D, H are N x N, column-major complex.
 template <class T>
 void operators(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int quad_order, int sing_order,
-               double k_signed, std::complex<T>* S, std::complex<T>* D, std::complex<T>* K, std::complex<T>* H) {
+               double k_signed, std::complex<T>* S, std::complex<T>* D, std::complex<T>* K, std::complex<T>* H,
+               int sym) {
   typedef std::complex<T> C;
   set_threads();
   Mesh<T> m = make_mesh<T>(xyz, nv, f, nf);
@@ -570,7 +737,8 @@ void operators(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int 
   std::memset((void*)H, 0, sizeof(C) * N * N);
   // _beat_cpu_accumulate_pair! over a point list
   auto pair = [&](const Elem<T>& te, const Elem<T>& se, int sdof, T nprod, int nq, const V3<T>* xs,
-                  const std::array<T, 3>* tb, const V3<T>* ys, const std::array<T, 3>* sb, const T* w) {
+                  const std::array<T, 3>* tb, const V3<T>* ys, const std::array<T, 3>* sb, const T* w,
+                  T sign = T(1)) {
     C sl[3] = {}, ad[3] = {}, dl[3][3] = {}, hy[3][3] = {};
     T cp[3][3];
     for (int a = 0; a < 3; ++a)
@@ -601,36 +769,59 @@ void operators(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int 
     }
     for (int a = 0; a < 3; ++a) {
       int row = te.p1[a];
-      S[row + sdof * N] += sl[a];
-      K[row + sdof * N] += ad[a];
+      S[row + sdof * N] += sign * sl[a];
+      K[row + sdof * N] += sign * ad[a];
       for (int b = 0; b < 3; ++b) {
-        D[row + (int64_t)se.p1[b] * N] += dl[a][b];
-        H[row + (int64_t)se.p1[b] * N] += hy[a][b];
+        D[row + (int64_t)se.p1[b] * N] += sign * dl[a][b];
+        H[row + (int64_t)se.p1[b] * N] += sign * hy[a][b];
       }
     }
   };
   const int n = (int)rw.size();
+  // regular pair of te with se over the tensor rule
+  auto regular = [&](const Elem<T>& te, const Elem<T>& se, int sdof, T sign) {
+    V3<T> xs[36], ys[36];
+    std::array<T, 3> tb[36], sb[36];
+    T w[36];
+    T jac = T(4.0) * te.area * se.area;
+    int qq = 0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        xs[qq] = te.qp[i];
+        tb[qq] = te.qb[i];
+        ys[qq] = se.qp[j];
+        sb[qq] = se.qb[j];
+        w[qq] = rw[i] * rw[j] * jac;
+        ++qq;
+      }
+    pair(te, se, sdof, dot(te.n, se.n), qq, xs, tb, ys, sb, w, sign);
+  };
+  // Duffy pair
+  auto duffy_pair = [&](const Elem<T>& te, const Elem<T>& se, const SingularPair<T>& p, const Duffy<T>& d) {
+    const int nq = (int)d.w.size();
+    std::vector<V3<T>> xs(nq), ys(nq);
+    std::vector<std::array<T, 3>> tb(nq), sb(nq);
+    std::vector<T> w(nq);
+    for (int qq = 0; qq < nq; ++qq) {
+      tb[qq] = p1_values(d.tp[qq]);
+      sb[qq] = p1_values(d.sp[qq]);
+      xs[qq] = local_to_global(te.v, d.tp[qq]);
+      ys[qq] = local_to_global(se.v, d.sp[qq]);
+      w[qq] = d.w[qq] * p.jac;
+    }
+    pair(te, se, p.trial, p.nprod, nq, xs.data(), tb.data(), ys.data(), sb.data(), w.data());
+  };
+  const std::vector<Xf> xfs = image_transforms(sym);
+  std::vector<std::vector<Elem<T>>> iel(xfs.size());
+  for (size_t x = 0; x < xfs.size(); ++x)
+    for (auto& e : el) iel[x].push_back(reflect_elem(e, xfs[x]));
   for (auto& g : groups) {
 #pragma omp parallel for schedule(dynamic, 1)
     for (int gi = 0; gi < (int)g.size(); ++gi) {
       int t = g[gi];
-      V3<T> xs[36], ys[36];
-      std::array<T, 3> tb[36], sb[36];
-      T w[36];
       for (int s = 0; s < (int)F; ++s) {
         if (adjacent(el[t].p1, el[s].p1)) continue;
-        T jac = T(4.0) * el[t].area * el[s].area;
-        int qq = 0;
-        for (int i = 0; i < n; ++i)
-          for (int j = 0; j < n; ++j) {
-            xs[qq] = el[t].qp[i];
-            tb[qq] = el[t].qb[i];
-            ys[qq] = el[s].qp[j];
-            sb[qq] = el[s].qb[j];
-            w[qq] = rw[i] * rw[j] * jac;
-            ++qq;
-          }
-        pair(el[t], el[s], s, dot(el[t].n, el[s].n), qq, xs, tb, ys, sb, w);
+        regular(el[t], el[s], s, T(1));
       }
     }
   }
@@ -639,20 +830,41 @@ void operators(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int 
 #pragma omp parallel for schedule(dynamic, 1)
     for (int gi = 0; gi < (int)g.size(); ++gi) {
       int t = g[gi];
-      for (auto& p : sc.by_test[t]) {
-        const Duffy<T>& d = sc.rules[p.rule];
-        const int nq = (int)d.w.size();
-        std::vector<V3<T>> xs(nq), ys(nq);
-        std::vector<std::array<T, 3>> tb(nq), sb(nq);
-        std::vector<T> w(nq);
-        for (int qq = 0; qq < nq; ++qq) {
-          tb[qq] = p1_values(d.tp[qq]);
-          sb[qq] = p1_values(d.sp[qq]);
-          xs[qq] = local_to_global(el[p.test].v, d.tp[qq]);
-          ys[qq] = local_to_global(el[p.trial].v, d.sp[qq]);
-          w[qq] = d.w[qq] * p.jac;
+      for (auto& p : sc.by_test[t]) duffy_pair(el[p.test], el[p.trial], p, sc.rules[p.rule]);
+    }
+  }
+  // image passes and image-singular deltas (JL/src/BeatEngineCpuAssembly.jl:571-640,827-960)
+  for (size_t x = 0; x < xfs.size(); ++x) {
+    for (auto& g : groups) {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int gi = 0; gi < (int)g.size(); ++gi) {
+        int t = g[gi];
+        for (int s = 0; s < (int)F; ++s) regular(el[t], iel[x][s], s, T(1));
+      }
+    }
+    SingularCache<T> ic = build_image_singular_cache(m, sing_order, xfs[x]);
+    for (auto& g : groups) {
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int gi = 0; gi < (int)g.size(); ++gi) {
+        int t = g[gi];
+        for (auto& p : ic.by_test[t]) {
+          duffy_pair(el[p.test], iel[x][p.trial], p, ic.rules[p.rule]);
+          regular(el[p.test], iel[x][p.trial], p.trial, T(-1));
         }
-        pair(el[p.test], el[p.trial], p.trial, p.nprod, nq, xs.data(), tb.data(), ys.data(), sb.data(), w.data());
+      }
+    }
+  }
+  // _beat_cpu_apply_operator_p1_row_weights!: JL/src/BeatEngineCpuAssembly.jl:674-680
+  if (sym != 0) {
+    const std::vector<T> ow = orbit_weights(m, sym);
+    for (int64_t r = 0; r < N; ++r) {
+      for (int64_t c = 0; c < F; ++c) {
+        S[r + c * N] *= ow[r];
+        K[r + c * N] *= ow[r];
+      }
+      for (int64_t c = 0; c < N; ++c) {
+        D[r + c * N] *= ow[r];
+        H[r + c * N] *= ow[r];
       }
     }
   }
@@ -663,7 +875,7 @@ void operators(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int 
 template <class T>
 void field(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int quad_order, double k_signed,
            const std::complex<T>* p, const std::complex<T>* q, int64_t nrhs, const double* pts, int64_t np,
-           std::complex<T>* out) {
+           std::complex<T>* out, int sym) {
   typedef std::complex<T> C;
   set_threads();
   Mesh<T> m = make_mesh<T>(xyz, nv, f, nf);
@@ -671,18 +883,26 @@ void field(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int quad
   std::vector<T> rw;
   triangle_rule<T>(quad_order, rp, rw);
   const int nq = (int)rw.size();
-  const int64_t ns = nf * nq;
+  // sources per element for the identity, then each image (symmetry_transforms order)
+  std::vector<Xf> xfs = {{{1, 1, 1}, 1}};
+  for (auto& x : image_transforms(sym)) xfs.push_back(x);
+  const int nt = (int)xfs.size();
+  const int64_t ns = nf * nt * nq;
   std::vector<V3<T>> sy(ns), sn(ns);
   std::vector<T> sw(ns);
   std::vector<std::array<T, 3>> sb(ns);
   for (int64_t e = 0; e < nf; ++e) {
-    V3<T> v[3] = {m.verts[m.faces[e][0]], m.verts[m.faces[e][1]], m.verts[m.faces[e][2]]};
     T jac = T(2.0) * m.areas[e];
-    for (int qq = 0; qq < nq; ++qq) {
-      sy[e * nq + qq] = local_to_global(v, rp[qq]);
-      sn[e * nq + qq] = m.normals[e];
-      sw[e * nq + qq] = rw[qq] * jac;
-      sb[e * nq + qq] = p1_values(rp[qq]);
+    for (int x = 0; x < nt; ++x) {
+      V3<T> v[3] = {reflect(xfs[x], m.verts[m.faces[e][0]]), reflect(xfs[x], m.verts[m.faces[e][1]]),
+                    reflect(xfs[x], m.verts[m.faces[e][2]])};
+      for (int qq = 0; qq < nq; ++qq) {
+        const int64_t s = (e * nt + x) * nq + qq;
+        sy[s] = local_to_global(v, rp[qq]);
+        sn[s] = reflect(xfs[x], m.normals[e]);
+        sw[s] = rw[qq] * jac;
+        sb[s] = p1_values(rp[qq]);
+      }
     }
   }
   T k = T(k_signed);
@@ -700,7 +920,7 @@ void field(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int quad
         C G(std::cos(phase) * gs, std::sin(phase) * gs);
         T proj = dot(r, sn[s]) / radius;
         C dl = G * (C(T(0), k) - T(1) / radius) * proj;
-        int64_t e = s / nq;
+        int64_t e = s / (nq * nt);
         const std::array<int, 3>& fc = m.faces[e];
         C ps = (sb[s][0] * p[fc[0] + d * nv] + sb[s][1] * p[fc[1] + d * nv] + sb[s][2] * p[fc[2] + d * nv]) * sw[s];
         C qs = q[e + d * nf] * sw[s];
@@ -771,35 +991,37 @@ int64_t oracle_singular_pairs(const double* xyz, int64_t nv, const int32_t* f, i
   return n;
 }
 
+// sym (all entry points below): 0 off, 1 x, 2 xy, 3 ground.
 void oracle_bm_assemble_f64(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int qo, int so, double k,
                             double eta_re, double eta_im, const double* q, int64_t nrhs, double* A, double* B,
-                            int test_lo, int test_hi, int do_singular, int do_identity) {
+                            int test_lo, int test_hi, int do_singular, int do_identity, int sym) {
   bm_assemble<double>(xyz, nv, f, nf, qo, so, k, eta_re, eta_im, (const cd*)q, nrhs, (cd*)A, (cd*)B, test_lo, test_hi,
-                      do_singular, do_identity);
+                      do_singular, do_identity, sym);
 }
 
 void oracle_bm_assemble_f32(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int qo, int so, double k,
                             double eta_re, double eta_im, const float* q, int64_t nrhs, float* A, float* B,
-                            int test_lo, int test_hi, int do_singular, int do_identity) {
+                            int test_lo, int test_hi, int do_singular, int do_identity, int sym) {
   typedef std::complex<float> cf;
   bm_assemble<float>(xyz, nv, f, nf, qo, so, k, eta_re, eta_im, (const cf*)q, nrhs, (cf*)A, (cf*)B, test_lo, test_hi,
-                     do_singular, do_identity);
+                     do_singular, do_identity, sym);
 }
 
 void oracle_operators_f64(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int qo, int so, double k,
-                          double* S, double* D, double* K, double* H) {
-  operators<double>(xyz, nv, f, nf, qo, so, k, (cd*)S, (cd*)D, (cd*)K, (cd*)H);
+                          double* S, double* D, double* K, double* H, int sym) {
+  operators<double>(xyz, nv, f, nf, qo, so, k, (cd*)S, (cd*)D, (cd*)K, (cd*)H, sym);
 }
 
 void oracle_operators_f32(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int qo, int so, double k,
-                          float* S, float* D, float* K, float* H) {
+                          float* S, float* D, float* K, float* H, int sym) {
   typedef std::complex<float> cf;
-  operators<float>(xyz, nv, f, nf, qo, so, k, (cf*)S, (cf*)D, (cf*)K, (cf*)H);
+  operators<float>(xyz, nv, f, nf, qo, so, k, (cf*)S, (cf*)D, (cf*)K, (cf*)H, sym);
 }
 
-// assemble_l2_identity_matrix (:p1,:p1) N x N and (:p1,:dp0) N x F, real, column-major.
+// assemble_l2_identity_matrix (:p1,:p1) N x N and (:p1,:dp0) N x F, real, column-major,
+// rows scaled by the orbit weights (JL/src/BeatEngineCore.jl:1187-1214).
 void oracle_identity_f64(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int qo, double* Mpp,
-                         double* Mpd) {
+                         double* Mpd, int sym) {
   Mesh<double> m = make_mesh<double>(xyz, nv, f, nf);
   std::vector<P2<double>> rp;
   std::vector<double> rw;
@@ -814,17 +1036,51 @@ void oracle_identity_f64(const double* xyz, int64_t nv, const int32_t* f, int64_
       for (int b = 0; b < 3; ++b) Mpp[m.faces[e][a] + (int64_t)m.faces[e][b] * nv] += pp[a][b];
     }
   }
+  const std::vector<double> ow = orbit_weights(m, sym);
+  for (int64_t r = 0; r < nv; ++r) {
+    for (int64_t c = 0; c < nv; ++c) Mpp[r + c * nv] *= ow[r];
+    for (int64_t c = 0; c < nf; ++c) Mpd[r + c * nv] *= ow[r];
+  }
 }
 
 void oracle_field_f64(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int qo, double k, const double* p,
-                      const double* q, int64_t nrhs, const double* pts, int64_t np, double* out) {
-  field<double>(xyz, nv, f, nf, qo, k, (const cd*)p, (const cd*)q, nrhs, pts, np, (cd*)out);
+                      const double* q, int64_t nrhs, const double* pts, int64_t np, double* out, int sym) {
+  field<double>(xyz, nv, f, nf, qo, k, (const cd*)p, (const cd*)q, nrhs, pts, np, (cd*)out, sym);
 }
 
 void oracle_field_f32(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int qo, double k, const float* p,
-                      const float* q, int64_t nrhs, const double* pts, int64_t np, float* out) {
+                      const float* q, int64_t nrhs, const double* pts, int64_t np, float* out, int sym) {
   typedef std::complex<float> cf;
-  field<float>(xyz, nv, f, nf, qo, k, (const cf*)p, (const cf*)q, nrhs, pts, np, (cf*)out);
+  field<float>(xyz, nv, f, nf, qo, k, (const cf*)p, (const cf*)q, nrhs, pts, np, (cf*)out, sym);
+}
+
+// p1_symmetry_orbit_weights per vertex.
+void oracle_orbit_weights(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int sym, double* out) {
+  Mesh<double> m = make_mesh<double>(xyz, nv, f, nf);
+  const std::vector<double> ow = orbit_weights(m, sym);
+  for (int64_t i = 0; i < nv; ++i) out[i] = ow[i];
+}
+
+// Image-singular pairs of image transform `image` (0-based, symmetry_transforms order
+// without the identity): (test, trial, kind, rule) per pair in the reference's order.
+int64_t oracle_image_singular_pairs(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int order, int sym,
+                                    int image, int32_t* out) {
+  Mesh<double> m = make_mesh<double>(xyz, nv, f, nf);
+  const std::vector<Xf> xfs = image_transforms(sym);
+  if (image < 0 || image >= (int)xfs.size()) return -1;
+  SingularCache<double> c = build_image_singular_cache(m, order, xfs[image]);
+  int64_t n = 0;
+  for (auto& v : c.by_test)
+    for (auto& p : v) {
+      if (out) {
+        out[4 * n] = p.test;
+        out[4 * n + 1] = p.trial;
+        out[4 * n + 2] = c.keys[p.rule][0];
+        out[4 * n + 3] = p.rule;
+      }
+      ++n;
+    }
+  return n;
 }
 
 // Surface curls per face (3 x 3 per face), for checks of the P1 gradient lift.
