@@ -21,8 +21,10 @@ REFINE_STATUS = {0: "none", 1: "converged", 2: "stagnated", 3: "step_limit", 4: 
 EXPORTS = (
     "beat_ctx_create", "beat_ctx_destroy", "beat_last_error", "beat_mesh_upload", "beat_mesh_destroy",
     "beat_mesh_counts", "beat_bm_assemble", "beat_bm_solve", "beat_bm_solve_device", "beat_field_eval",
-    "beat_dense_lu_solve", "beat_refine_lu",
+    "beat_dense_lu_solve", "beat_refine_lu", "beat_mesh_upload_symmetric", "beat_mesh_symmetry",
 )
+
+SYMMETRY_CODES = {"off": 0, "x": 1, "xy": 2, "ground": 3}
 
 
 class RefineReport(ctypes.Structure):
@@ -84,6 +86,8 @@ def load(path: str = LIB_PATH):
     L.beat_mesh_destroy.argtypes = [P]
     L.beat_mesh_destroy.restype = None
     L.beat_mesh_counts.argtypes = [P, P]
+    L.beat_mesh_upload_symmetric.argtypes = [P, P, I64, P, I64, I, I, I, ctypes.POINTER(P)]
+    L.beat_mesh_symmetry.argtypes = [P, P]
     L.beat_bm_assemble.argtypes = [P, P, I, D, D, D, P, I64, P, P]
     L.beat_bm_solve.argtypes = [P, P, I, D, D, D, P, I64, P, ctypes.POINTER(SolveOptions), ctypes.POINTER(SolveReport)]
     L.beat_bm_solve_device.argtypes = [P, P, I, D, D, D, P, I64, P, ctypes.POINTER(SolveOptions),

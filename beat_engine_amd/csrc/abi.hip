@@ -81,8 +81,10 @@ struct beat_mesh {
   double* rule3 = nullptr;  // regular rule: xi, eta, w per point
   int32_t* face_perm = nullptr;
   int32_t* vert_inv = nullptr;
+  int32_t* wrows = nullptr;  // rows on a mirror plane and their orbit weights
+  double* wvals = nullptr;
   int64_t regular_pairs = 0;
-  double diameter = 0;  // bounding-box diagonal: bounds every |y - x| of the assembly
+  double diameter = 0;  // bounding-box diagonal: bounds every |y - x| of the assembly (images included)
 };
 
 namespace {
@@ -481,9 +483,17 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
     ra.phi[i][1] = (T)xi;
     ra.phi[i][2] = (T)eta;
   }
+  ra.nimg = m->h.nimg;
+  for (int x = 0; x < MAX_IMAGES; ++x)
+    for (int c = 0; c < 3; ++c) {
+      const bool on = x < m->h.nimg;
+      ra.img[x][c] = on ? (T)m->h.img[x].s[c] : T(1);
+      ra.img[x][3 + c] = on ? (T)(m->h.img[x].det * m->h.img[x].s[c]) : T(1);
+    }
   HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
   const bool wide = !(std::fabs(k) * m->diameter < SINCOS_TABLE_LIMIT);
-  HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, ctx->stream));
+  HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, false, ctx->stream));
+  if (m->h.nimg > 0) HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, true, ctx->stream));
   HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
   SingularArgs<T> sa;
   sa.rec = m->rec64;
@@ -502,6 +512,12 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   sa.k = k;
   sa.eta_re = er;
   sa.eta_im = ei;
+  for (int x = 0; x <= MAX_IMAGES; ++x)
+    for (int c = 0; c < 3; ++c) {
+      const bool on = x >= 1 && x <= m->h.nimg;
+      sa.img[x][c] = on ? m->h.img[x - 1].s[c] : 1.0;
+      sa.img[x][3 + c] = on ? m->h.img[x - 1].det * m->h.img[x - 1].s[c] : 1.0;
+    }
   HIPCHK(launch_singular<T>(sa, ctx->stream));
   IdentityArgs<T> ia;
   ia.rec = m->rec64;
@@ -519,6 +535,7 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
     for (int b = 0; b < 3; ++b) ia.pp[p][b] = m->h.id_pp[p][b];
   }
   HIPCHK(launch_identity<T>(ia, ctx->stream));
+  HIPCHK(launch_row_scale<T>(A, lda, n, B, nrhs, m->wrows, m->wvals, (int64_t)m->h.weighted_rows.size(), ctx->stream));
   return BEAT_OK;
 }
 
@@ -724,15 +741,15 @@ void beat_ctx_destroy(beat_ctx* c) {
 
 const char* beat_last_error(const beat_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
-int beat_mesh_upload(beat_ctx* ctx, const double* verts, int64_t nv, const int32_t* faces, int64_t nf,
-                     int quad_order, int singular_order, beat_mesh** out) {
+int beat_mesh_upload_symmetric(beat_ctx* ctx, const double* verts, int64_t nv, const int32_t* faces, int64_t nf,
+                               int quad_order, int singular_order, int symmetry, beat_mesh** out) {
   if (!ctx || !out || !verts || !faces) return fail(ctx, BEAT_ERR_ARG, "null argument");
   std::lock_guard<std::mutex> lock(ctx->mu);
   *out = nullptr;
   HIPCHK(hipSetDevice(ctx->device));
   beat_mesh* m = new beat_mesh();
   m->ctx = ctx;
-  std::string msg = build_host_mesh(verts, nv, faces, nf, quad_order, singular_order, m->h);
+  std::string msg = build_host_mesh(verts, nv, faces, nf, quad_order, singular_order, symmetry, m->h);
   if (!msg.empty()) {
     delete m;
     return fail(ctx, BEAT_ERR_ARG, msg);
@@ -762,11 +779,13 @@ int beat_mesh_upload(beat_ctx* ctx, const double* verts, int64_t nv, const int32
   if (e == hipSuccess) e = upload(&m->rule3, rule3);
   if (e == hipSuccess) e = upload(&m->face_perm, m->h.face_perm);
   if (e == hipSuccess) e = upload(&m->vert_inv, m->h.vert_inv);
+  if (e == hipSuccess) e = upload(&m->wrows, m->h.weighted_rows);
+  if (e == hipSuccess) e = upload(&m->wvals, m->h.row_weights);
   if (e != hipSuccess) {
     beat_mesh_destroy(m);
     return fail(ctx, BEAT_ERR_HIP, std::string("mesh upload: ") + hipGetErrorString(e));
   }
-  m->regular_pairs = nf * nf - (int64_t)m->h.spairs.size();
+  m->regular_pairs = nf * nf - (int64_t)(m->h.spairs.size() - m->h.n_image_singular) + m->h.nimg * nf * nf;
   {
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int64_t v = 0; v < nv; ++v)
@@ -774,19 +793,29 @@ int beat_mesh_upload(beat_ctx* ctx, const double* verts, int64_t nv, const int32
         lo[c] = std::fmin(lo[c], verts[3 * v + c]);
         hi[c] = std::fmax(hi[c], verts[3 * v + c]);
       }
+    // a reflected trial face lies in the mirrored box: the union spans at most
+    // twice the largest coordinate magnitude per axis
     double d2 = 0;
-    for (int c = 0; c < 3; ++c) d2 += (hi[c] - lo[c]) * (hi[c] - lo[c]);
+    for (int c = 0; c < 3; ++c) {
+      const double span = m->h.nimg > 0 ? 2.0 * std::fmax(std::fabs(lo[c]), std::fabs(hi[c])) : hi[c] - lo[c];
+      d2 += span * span;
+    }
     m->diameter = std::sqrt(d2);
   }
   *out = m;
   return BEAT_OK;
 }
 
+int beat_mesh_upload(beat_ctx* ctx, const double* verts, int64_t nv, const int32_t* faces, int64_t nf,
+                     int quad_order, int singular_order, beat_mesh** out) {
+  return beat_mesh_upload_symmetric(ctx, verts, nv, faces, nf, quad_order, singular_order, 0, out);
+}
+
 void beat_mesh_destroy(beat_mesh* m) {
   if (!m) return;
   void* ptrs[] = {m->rec64, m->rec32, m->verts, m->fverts, m->floc, m->tile_start, m->tile_count, m->tile_nv,
-                  m->tile_verts, m->stile_start, m->stile_count, m->stile_nv, m->stile_verts, m->flocs, m->spairs, m->rule_off, m->rule_cnt, m->rule_data, m->rule3, m->face_perm,
-                  m->vert_inv};
+                  m->tile_verts, m->stile_start, m->stile_count, m->stile_nv, m->stile_verts, m->flocs, m->spairs,
+                  m->rule_off, m->rule_cnt, m->rule_data, m->rule3, m->face_perm, m->vert_inv, m->wrows, m->wvals};
   if (m->ctx) (void)hipSetDevice(m->ctx->device);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -801,6 +830,15 @@ int beat_mesh_counts(const beat_mesh* m, int64_t* counts) {
   counts[3] = m->h.n_vertex;
   counts[4] = (int64_t)m->h.tile_start.size();
   counts[5] = m->h.nq;
+  return BEAT_OK;
+}
+
+int beat_mesh_symmetry(const beat_mesh* m, int64_t* info) {
+  if (!m || !info) return BEAT_ERR_ARG;
+  info[0] = m->h.symmetry;
+  info[1] = m->h.nimg;
+  info[2] = m->h.n_image_singular;
+  info[3] = (int64_t)m->h.weighted_rows.size();
   return BEAT_OK;
 }
 
@@ -897,7 +935,7 @@ int beat_field_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, 
   std::lock_guard<std::mutex> lock(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
-  const int64_t n = m->h.nv, nf = m->h.nf, ns = nf * m->h.nq;
+  const int64_t n = m->h.nv, nf = m->h.nf, ns = nf * m->h.nq * (1 + m->h.nimg);
   // stage: pressure (n x nrhs, caller col-major), q (nf x nrhs) -> internal row-major
   HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * (nf + n) * nrhs));
   double* qin = as<double>(ctx->qin);
@@ -927,7 +965,8 @@ int beat_field_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, 
     double* sn = sp + 3 * ns;
     double* pw = sn + 3 * ns;
     double* qw = pw + 2 * ns * nrhs;
-    HIPCHK(field_sources<double>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, pint, qint, (int)nrhs, sp, sn, pw, qw, s));
+    HIPCHK(field_sources<double>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, pint, qint, (int)nrhs, m->h.nimg, m->h.img,
+                                 sp, sn, pw, qw, s));
     HIPCHK(field_eval<double>(sp, sn, pw, qw, ns, (int)nrhs, as<double>(ctx->fld_pts), npts, as<double>(ctx->fld_out),
                               k, s));
     HIPCHK(hipMemcpyAsync(res.data(), ctx->fld_out.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, s));
@@ -946,7 +985,8 @@ int beat_field_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, 
     for (int64_t i = 0; i < 3 * npts; ++i) hp[i] = (float)pts[i];
     HIPCHK(hipMemcpyAsync(pts32, hp.data(), sizeof(float) * hp.size(), hipMemcpyHostToDevice, s));
     float* o32 = as<float>(ctx->fld_out);
-    HIPCHK(field_sources<float>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, p32, q32, (int)nrhs, sp, sn, pw, qw, s));
+    HIPCHK(field_sources<float>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, p32, q32, (int)nrhs, m->h.nimg, m->h.img,
+                                sp, sn, pw, qw, s));
     HIPCHK(field_eval<float>(sp, sn, pw, qw, ns, (int)nrhs, pts32, npts, o32, (float)k, s));
     std::vector<float> r32(res.size());
     HIPCHK(hipMemcpyAsync(r32.data(), o32, sizeof(float) * r32.size(), hipMemcpyDeviceToHost, s));

@@ -74,7 +74,7 @@ __device__ __forceinline__ double inv_sqrt(double x) {
 // addresses (scalar loads); 3x3 blocks go to the LDS tile accA[test vertex slot]
 // [trial vertex slot] and the P1 x DP0 blocks to accC[test vertex slot][trial
 // face]; both are flushed once with global atomics. Two workgroups fit per CU.
-template <typename T, int NQ, bool PURE, bool WIDE>
+template <typename T, int NQ, bool PURE, bool WIDE, bool IMAGE>
 __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T> a) {
   // rows padded by one scalar: the 64 lanes add into rows tl (varying) at the same
   // column, which without the pad would all fall into one LDS bank
@@ -88,7 +88,19 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
 
   const int64_t total = (int64_t)a.ntiles * a.nstiles;
   const int64_t per = (total + 7) / 8;
-  const int64_t L = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  int64_t bid = blockIdx.x;
+  // image passes: the grid holds one copy of the tile-pair range per transform
+  T sg[6] = {T(1), T(1), T(1), T(1), T(1), T(1)};
+  if (IMAGE) {
+    const int xi = (int)(bid / (per * 8));
+    bid -= (int64_t)xi * per * 8;
+#pragma unroll
+    for (int x = 0; x < MAX_IMAGES; ++x)
+      if (x == xi)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sg[c] = a.img[x][c];
+  }
+  const int64_t L = (bid & 7) * per + (bid >> 3);
   if (L >= total) return;
   const int tt = (int)(L / a.nstiles), st = (int)(L % a.nstiles);
   const int tid = threadIdx.x;
@@ -134,14 +146,17 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
     const int sv0 = fverts[sf * 3], sv1 = fverts[sf * 3 + 1], sv2 = fverts[sf * 3 + 2];
     const bool adj = (tv0 == sv0) | (tv0 == sv1) | (tv0 == sv2) | (tv1 == sv0) | (tv1 == sv1) | (tv1 == sv2) |
                      (tv2 == sv0) | (tv2 == sv1) | (tv2 == sv2);
-    if (!have_test || adj) continue;
+    if (!have_test || (!IMAGE && adj)) continue;
     const T* sr = rec + sf * FREC;
     T y[NQ][3];
 #pragma unroll
     for (int j = 0; j < NQ; ++j)
 #pragma unroll
-      for (int c = 0; c < 3; ++c) y[j][c] = sr[3 * j + c];
-    const T nS0 = sr[REC_N], nS1 = sr[REC_N + 1], nS2 = sr[REC_N + 2];
+      for (int c = 0; c < 3; ++c) y[j][c] = IMAGE ? sr[3 * j + c] * sg[c] : sr[3 * j + c];
+    // the reflected plane keeps its offset: (s v1).(s n) = v1.n
+    const T nS0 = IMAGE ? sr[REC_N] * sg[0] : sr[REC_N];
+    const T nS1 = IMAGE ? sr[REC_N + 1] * sg[1] : sr[REC_N + 1];
+    const T nS2 = IMAGE ? sr[REC_N + 2] * sg[2] : sr[REC_N + 2];
     const T dS = sr[REC_D];
     const T cw = T(4) * areaT * sr[REC_AREA] * T(INV_FOUR_PI);
     const T nprod = nT0 * nS0 + nT1 * nS1 + nT2 * nS2;
@@ -228,7 +243,8 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
       const T c0 = tr[REC_CURL + 3 * p], c1 = tr[REC_CURL + 3 * p + 1], c2 = tr[REC_CURL + 3 * p + 2];
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
-        const T cc = c0 * sr[REC_CURL + 3 * b] + c1 * sr[REC_CURL + 3 * b + 1] + c2 * sr[REC_CURL + 3 * b + 2];
+        const T b0 = sr[REC_CURL + 3 * b], b1 = sr[REC_CURL + 3 * b + 1], b2 = sr[REC_CURL + 3 * b + 2];
+        const T cc = IMAGE ? c0 * (b0 * sg[3]) + c1 * (b1 * sg[4]) + c2 * (b2 * sg[5]) : c0 * b0 + c1 * b1 + c2 * b2;
         T* dst = accA + tl[p] * LDA_ + 2 * sl[b];
         lds_add(dst, Lr[p][b] + cc * ecr);
         lds_add(dst + 1, Li[p][b] + cc * eci);
@@ -279,6 +295,15 @@ __global__ __launch_bounds__(256) void bm_singular_kernel(const SingularArgs<TO>
   const SingularPair P = a.pairs[pidx];
   const double* tr = a.rec + (int64_t)P.test * FREC;
   const double* sr = a.rec + (int64_t)P.trial * FREC;
+  // trial-face reflection of an image pair (identity signs otherwise)
+  double sg[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) sg[c] = a.img[0][c];
+#pragma unroll
+  for (int x = 1; x <= MAX_IMAGES; ++x)
+    if (P.xform == x)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) sg[c] = a.img[x][c];
   double tv[3][3], sv[3][3];
   int tvi[3], svi[3];
   for (int v = 0; v < 3; ++v) {
@@ -286,11 +311,11 @@ __global__ __launch_bounds__(256) void bm_singular_kernel(const SingularArgs<TO>
     svi[v] = a.fverts[(int64_t)P.trial * 3 + v];
     for (int c = 0; c < 3; ++c) {
       tv[v][c] = a.verts[3 * (int64_t)tvi[v] + c];
-      sv[v][c] = a.verts[3 * (int64_t)svi[v] + c];
+      sv[v][c] = a.verts[3 * (int64_t)svi[v] + c] * sg[c];
     }
   }
   const double nT[3] = {tr[REC_N], tr[REC_N + 1], tr[REC_N + 2]};
-  const double nS[3] = {sr[REC_N], sr[REC_N + 1], sr[REC_N + 2]};
+  const double nS[3] = {sr[REC_N] * sg[0], sr[REC_N + 1] * sg[1], sr[REC_N + 2] * sg[2]};
   const double k = a.k, er = a.eta_re, ei = a.eta_im;
   const double knr = er * (k * k * P.nprod), kni = ei * (k * k * P.nprod);
   const int off = a.rule_off[P.rule], cnt = a.rule_cnt[P.rule];
@@ -347,8 +372,9 @@ __global__ __launch_bounds__(256) void bm_singular_kernel(const SingularArgs<TO>
   const double ecr = er * acc[24] - ei * acc[25], eci = er * acc[25] + ei * acc[24];
   if (lane < 9) {
     const int p = lane / 3, b = lane % 3;
-    const double cc = tr[REC_CURL + 3 * p] * sr[REC_CURL + 3 * b] + tr[REC_CURL + 3 * p + 1] * sr[REC_CURL + 3 * b + 1] +
-                      tr[REC_CURL + 3 * p + 2] * sr[REC_CURL + 3 * b + 2];
+    const double cc = tr[REC_CURL + 3 * p] * (sr[REC_CURL + 3 * b] * sg[3]) +
+                      tr[REC_CURL + 3 * p + 1] * (sr[REC_CURL + 3 * b + 1] * sg[4]) +
+                      tr[REC_CURL + 3 * p + 2] * (sr[REC_CURL + 3 * b + 2] * sg[5]);
     double vr = 0, vi = 0;
 #pragma unroll
     for (int i = 0; i < 9; ++i)
@@ -398,6 +424,19 @@ __global__ void bm_identity_kernel(const IdentityArgs<TO> a) {
   }
 }
 
+// Orbit weights of the rows on a mirror plane (the reference scales the assembled
+// rows, BeatEngineCpuBurtonMiller.jl:443-449, and its identity blocks carry the
+// same weights, BeatEngineCore.jl:1187-1214): one workgroup per listed row.
+template <typename TO>
+__global__ void row_scale_kernel(TO* A, int64_t lda, int64_t n, TO* B, int nrhs, const int32_t* rows, const double* w) {
+  const int64_t r = rows[blockIdx.x];
+  const TO s = (TO)w[blockIdx.x];
+  TO* a = A + 2 * r * lda;
+  for (int64_t i = threadIdx.x; i < 2 * n; i += blockDim.x) a[i] *= s;
+  TO* b = B + 2 * r * nrhs;
+  for (int i = threadIdx.x; i < 2 * nrhs; i += blockDim.x) b[i] *= s;
+}
+
 template <typename T>
 __global__ void fill_kernel(T* p, int64_t n, T v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -411,34 +450,44 @@ __global__ void downcast_kernel(const double* src, float* dst, int64_t n) {
 
 // ---- launchers -------------------------------------------------------------------
 
-template <typename T, bool WIDE>
+template <typename T, bool WIDE, bool IMAGE>
 static void launch_regular_nq(const RegularArgs<T>& a, int nq, bool pure, dim3 grid, dim3 block, hipStream_t s) {
   if (nq == 6) {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 6, true, WIDE>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 6, false, WIDE>), grid, block, 0, s, a);
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 6, true, WIDE, IMAGE>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 6, false, WIDE, IMAGE>), grid, block, 0, s, a);
   } else if (nq == 3) {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 3, true, WIDE>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 3, false, WIDE>), grid, block, 0, s, a);
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 3, true, WIDE, IMAGE>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 3, false, WIDE, IMAGE>), grid, block, 0, s, a);
   } else {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 1, true, WIDE>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 1, false, WIDE>), grid, block, 0, s, a);
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 1, true, WIDE, IMAGE>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 1, false, WIDE, IMAGE>), grid, block, 0, s, a);
   }
 }
 
-template <typename T>
-hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure, bool wide, hipStream_t s) {
-  const int64_t total = (int64_t)a.ntiles * a.nstiles;
-  const int64_t per = (total + 7) / 8;
-  dim3 grid((unsigned)(per * 8)), block(256);
+template <typename T, bool IMAGE>
+static void launch_regular_range(const RegularArgs<T>& a, int nq, bool pure, bool wide, dim3 grid, dim3 block,
+                                 hipStream_t s) {
   // the FP32 phase needs no range bound
   if constexpr (sizeof(T) == 8) {
     if (wide) {
-      launch_regular_nq<T, true>(a, nq, pure, grid, block, s);
-      return hipGetLastError();
+      launch_regular_nq<T, true, IMAGE>(a, nq, pure, grid, block, s);
+      return;
     }
   }
   (void)wide;
-  launch_regular_nq<T, false>(a, nq, pure, grid, block, s);
+  launch_regular_nq<T, false, IMAGE>(a, nq, pure, grid, block, s);
+}
+
+template <typename T>
+hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure, bool wide, bool image, hipStream_t s) {
+  const int64_t total = (int64_t)a.ntiles * a.nstiles;
+  const int64_t per = (total + 7) / 8;
+  if (image && (a.nimg < 1 || a.nimg > MAX_IMAGES)) return hipErrorInvalidValue;
+  const int64_t blocks = per * 8 * (image ? a.nimg : 1);
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  dim3 grid((unsigned)blocks), block(256);
+  if (image) launch_regular_range<T, true>(a, nq, pure, wide, grid, block, s);
+  else launch_regular_range<T, false>(a, nq, pure, wide, grid, block, s);
   return hipGetLastError();
 }
 
@@ -457,6 +506,14 @@ hipError_t launch_identity(const IdentityArgs<TO>& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename TO>
+hipError_t launch_row_scale(TO* A, int64_t lda, int64_t n, TO* B, int nrhs, const int32_t* rows, const double* w,
+                            int64_t nrows, hipStream_t s) {
+  if (nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL((row_scale_kernel<TO>), dim3((unsigned)nrows), dim3(256), 0, s, A, lda, n, B, nrhs, rows, w);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
@@ -472,8 +529,12 @@ hipError_t launch_downcast(const double* src, float* dst, int64_t n, hipStream_t
   return hipGetLastError();
 }
 
-template hipError_t launch_regular<float>(const RegularArgs<float>&, int, bool, bool, hipStream_t);
-template hipError_t launch_regular<double>(const RegularArgs<double>&, int, bool, bool, hipStream_t);
+template hipError_t launch_regular<float>(const RegularArgs<float>&, int, bool, bool, bool, hipStream_t);
+template hipError_t launch_regular<double>(const RegularArgs<double>&, int, bool, bool, bool, hipStream_t);
+template hipError_t launch_row_scale<float>(float*, int64_t, int64_t, float*, int, const int32_t*, const double*,
+                                            int64_t, hipStream_t);
+template hipError_t launch_row_scale<double>(double*, int64_t, int64_t, double*, int, const int32_t*, const double*,
+                                             int64_t, hipStream_t);
 template hipError_t launch_singular<float>(const SingularArgs<float>&, hipStream_t);
 template hipError_t launch_singular<double>(const SingularArgs<double>&, hipStream_t);
 template hipError_t launch_identity<float>(const IdentityArgs<float>&, hipStream_t);

@@ -25,11 +25,40 @@ constexpr int REC_D = 31;
 
 constexpr double INV_FOUR_PI = 0.079577471545947667884441881686257181;
 
-// Singular (touching) pair: internal test/trial face, rule, jacobian and normal product.
+// Singular (touching) pair: internal test/trial face, rule, image transform of
+// the trial face (0: none), jacobian and normal product.
 struct SingularPair {
-  int32_t test, trial, rule, pad;
+  int32_t test, trial, rule, xform;
   double jac, nprod;
 };
+
+// Symmetry modes and their image transforms (the reference's
+// symmetry_image_transforms, BeatEngineCore.jl:448-495): a point maps to
+// (s0 x, s1 y, s2 z), a normal likewise, a surface curl to det * (s0, s1, s2) c.
+constexpr int SYM_OFF = 0, SYM_X = 1, SYM_XY = 2, SYM_GROUND = 3;
+constexpr int MAX_IMAGES = 3;
+struct ImageXf {
+  int s[3];
+  int det;
+};
+// image transforms of a mode, in the reference's order; returns their count
+inline int image_transforms(int mode, ImageXf* out) {
+  if (mode == SYM_X) {
+    out[0] = {{-1, 1, 1}, -1};
+    return 1;
+  }
+  if (mode == SYM_XY) {
+    out[0] = {{-1, 1, 1}, -1};
+    out[1] = {{1, -1, 1}, -1};
+    out[2] = {{-1, -1, 1}, 1};
+    return 3;
+  }
+  if (mode == SYM_GROUND) {
+    out[0] = {{1, -1, 1}, -1};
+    return 1;
+  }
+  return 0;
+}
 
 // Host-side mesh after preprocessing; all arrays in internal (tile) order
 // except where noted.
@@ -54,12 +83,20 @@ struct HostMesh {
   std::vector<int32_t> rule_off, rule_cnt, rule_kind;
   std::vector<double> rule_data;  // 5 per rule point: test xi, eta, trial xi, eta, weight
   int64_t n_coincident = 0, n_edge = 0, n_vertex = 0;
+  // symmetry: image transforms, the image-singular pairs among spairs (xform > 0,
+  // placed after the ordinary pairs) and the P1 orbit weights of the rows that
+  // lie on a symmetry plane (internal vertex ids)
+  int symmetry = SYM_OFF, nimg = 0;
+  ImageXf img[MAX_IMAGES];
+  int64_t n_image_singular = 0;
+  std::vector<int32_t> weighted_rows;
+  std::vector<double> row_weights;
   double id_pp[3][3];  // unit-area P1 x P1 mass block from the rule
   double id_pd[3];     // unit-area P1 x DP0 block
 };
 
 std::string build_host_mesh(const double* verts, int64_t nv, const int32_t* faces, int64_t nf, int quad_order,
-                            int singular_order, HostMesh& out);
+                            int singular_order, int symmetry, HostMesh& out);
 
 // Reference rules (restated from BeatEngineCore.jl, see mesh_setup.cpp).
 int triangle_rule(int order, double* pts, double* w);

@@ -1,9 +1,12 @@
 // Exterior field from the solved traces (the reference's
 // evaluate_galerkin_field_cpu, src/beat_engine/julia_local/src/BeatEngineCpuField.jl:1-49):
 //   p(x) = sum_s [ G (ik - 1/r) (r.n_s)/r  p_s w_s  -  G q_s w_s ],  r = y_s - x
-// over the quadrature sources y_s of every face. field_sources forms the weighted
-// source values once per call; field_kernel stages sources through LDS and
-// splits the source range across workgroups, each adding its partial sums.
+// over the quadrature sources y_s of every face, and with a symmetry mode of
+// every face's images (build_field_evaluation_cache, BeatEngineCore.jl:1770-1810:
+// reflected points and normals, the same weighted trace values). field_sources
+// forms the weighted source values once per call; field_kernel stages sources
+// through LDS and splits the source range across workgroups, each adding its
+// partial sums.
 #include <hip/hip_runtime.h>
 
 #include "bem_common.h"
@@ -17,18 +20,31 @@ constexpr int FIELD_PTS = 128;   // points per workgroup (one per thread)
 constexpr int FIELD_CHUNK = 128; // sources staged per LDS round
 constexpr int FIELD_RHS = 8;     // right-hand sides per pass
 
+struct FieldImages {
+  int s[MAX_IMAGES + 1][3];  // point signs per transform; row 0 the identity
+};
+
 template <typename T>
 __global__ void field_sources_kernel(const double* rec, const int32_t* fverts, int64_t nf, int nq,
-                                     const double* __restrict__ rule, const T* p, const T* q, int nrhs, T* sp, T* sn,
-                                     T* pw, T* qw) {
+                                     const double* __restrict__ rule, const T* p, const T* q, int nrhs, int ntr,
+                                     const FieldImages im, T* sp, T* sn, T* pw, T* qw) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nf * nq) return;
-  const int64_t f = s / nq;
-  const int qi = (int)(s - f * nq);
+  const int64_t nb = nf * nq;
+  if (s >= nb * ntr) return;
+  const int x = (int)(s / nb);
+  const int64_t sb = s - (int64_t)x * nb;
+  const int64_t f = sb / nq;
+  const int qi = (int)(sb - f * nq);
+  double sg[3] = {1.0, 1.0, 1.0};
+#pragma unroll
+  for (int t = 1; t <= MAX_IMAGES; ++t)
+    if (t == x)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sg[c] = (double)im.s[t][c];
   const double* r = rec + f * FREC;
   for (int c = 0; c < 3; ++c) {
-    sp[3 * s + c] = (T)r[3 * qi + c];
-    sn[3 * s + c] = (T)r[REC_N + c];
+    sp[3 * s + c] = (T)(r[3 * qi + c] * sg[c]);
+    sn[3 * s + c] = (T)(r[REC_N + c] * sg[c]);
   }
   const double xi = rule[3 * qi], eta = rule[3 * qi + 1], wq = rule[3 * qi + 2];
   const double w = wq * (2.0 * r[REC_AREA]);
@@ -128,10 +144,15 @@ __global__ __launch_bounds__(FIELD_PTS) void field_kernel(const T* sp, const T* 
 
 template <typename T>
 hipError_t field_sources(const double* rec, const int32_t* fverts, int64_t nf, int nq, const double* rule, const T* p,
-                         const T* q, int nrhs, T* src_pts, T* src_nrm, T* pw, T* qw, hipStream_t s) {
-  const int64_t ns = nf * nq;
+                         const T* q, int nrhs, int nimg, const ImageXf* img, T* src_pts, T* src_nrm, T* pw, T* qw,
+                         hipStream_t s) {
+  if (nimg < 0 || nimg > MAX_IMAGES) return hipErrorInvalidValue;
+  FieldImages im;
+  for (int t = 0; t <= MAX_IMAGES; ++t)
+    for (int c = 0; c < 3; ++c) im.s[t][c] = (t >= 1 && t <= nimg) ? img[t - 1].s[c] : 1;
+  const int64_t ns = nf * nq * (1 + nimg);
   hipLaunchKernelGGL((field_sources_kernel<T>), dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, rec, fverts, nf,
-                     nq, rule, p, q, nrhs, src_pts, src_nrm, pw, qw);
+                     nq, rule, p, q, nrhs, 1 + nimg, im, src_pts, src_nrm, pw, qw);
   return hipGetLastError();
 }
 
@@ -155,9 +176,11 @@ hipError_t field_eval(const T* src_pts, const T* src_nrm, const T* pw, const T* 
 }
 
 template hipError_t field_sources<float>(const double*, const int32_t*, int64_t, int, const double*, const float*,
-                                         const float*, int, float*, float*, float*, float*, hipStream_t);
+                                         const float*, int, int, const ImageXf*, float*, float*, float*, float*,
+                                         hipStream_t);
 template hipError_t field_sources<double>(const double*, const int32_t*, int64_t, int, const double*, const double*,
-                                          const double*, int, double*, double*, double*, double*, hipStream_t);
+                                          const double*, int, int, const ImageXf*, double*, double*, double*, double*,
+                                          hipStream_t);
 template hipError_t field_eval<float>(const float*, const float*, const float*, const float*, int64_t, int,
                                       const float*, int64_t, float*, float, hipStream_t);
 template hipError_t field_eval<double>(const double*, const double*, const double*, const double*, int64_t, int,

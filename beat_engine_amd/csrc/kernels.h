@@ -34,6 +34,10 @@ struct RegularArgs {
   T w[6];                  // rule weights
   T phi[6][3];             // P1 basis at the rule points
   const double* trig;      // 2 TRIG_N doubles: (sin, cos)(m pi/64)
+  // image passes: nimg transforms, per transform the point signs (s0, s1, s2)
+  // followed by the curl signs det * (s0, s1, s2)
+  int nimg;
+  T img[MAX_IMAGES][6];
 };
 
 template <typename TO>
@@ -52,6 +56,8 @@ struct SingularArgs {
   const double* q;         // F x nrhs row-major complex128
   int nrhs;
   double k, eta_re, eta_im;
+  // signs by SingularPair::xform (row 0: identity): point signs, then curl signs
+  double img[MAX_IMAGES + 1][6];
 };
 
 template <typename TO>
@@ -71,12 +77,18 @@ struct IdentityArgs {
 
 template <typename T>
 // wide: |k| times the mesh diameter may reach SINCOS_TABLE_LIMIT (FP64 phase by
-// the pi/2 reduction with the library fallback instead of the table form)
-hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure_imag_eta, bool wide, hipStream_t s);
+// the pi/2 reduction with the library fallback instead of the table form).
+// image: the a.nimg image passes (every pair, trial face reflected) instead of
+// the direct pass (pairs without a shared vertex).
+hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure_imag_eta, bool wide, bool image, hipStream_t s);
 template <typename TO>
 hipError_t launch_singular(const SingularArgs<TO>& a, hipStream_t s);
 template <typename TO>
 hipError_t launch_identity(const IdentityArgs<TO>& a, hipStream_t s);
+// rows[i] of A (n columns, lda) and B (nrhs columns) times w[i]
+template <typename TO>
+hipError_t launch_row_scale(TO* A, int64_t lda, int64_t n, TO* B, int nrhs, const int32_t* rows, const double* w,
+                            int64_t nrows, hipStream_t s);
 template <typename T>
 hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s);
 hipError_t launch_downcast(const double* src, float* dst, int64_t n, hipStream_t s);
@@ -122,9 +134,12 @@ template <typename T>
 hipError_t field_eval(const T* src_pts, const T* src_nrm, const T* pw, const T* qw, int64_t nsrc, int nrhs,
                       const T* pts, int64_t npts, T* out, T k, hipStream_t s);
 // rule: 3 doubles per regular rule point (xi, eta, weight); p is N x nrhs and q
-// F x nrhs (row-major, internal order); outputs are per source (F * nq).
+// F x nrhs (row-major, internal order); outputs are per source: F * nq for the
+// mesh, then F * nq for each of the nimg image transforms (points and normals
+// reflected by the signs sg[i][0..2], same weighted values).
 template <typename T>
 hipError_t field_sources(const double* rec, const int32_t* fverts, int64_t nf, int nq, const double* rule, const T* p,
-                         const T* q, int nrhs, T* src_pts, T* src_nrm, T* pw, T* qw, hipStream_t s);
+                         const T* q, int nrhs, int nimg, const ImageXf* img, T* src_pts, T* src_nrm, T* pw, T* qw,
+                         hipStream_t s);
 
 }  // namespace beat

@@ -153,8 +153,9 @@ int triangle_rule(int order, double* pts, double* w) {
 }
 
 std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64_t nf, int quad_order,
-                            int singular_order, HostMesh& m) {
+                            int singular_order, int symmetry, HostMesh& m) {
   if (nv < 3 || nf < 1) return "mesh needs at least 3 vertices and 1 face";
+  if (symmetry < SYM_OFF || symmetry > SYM_GROUND) return "symmetry must be 0 (off), 1 (x), 2 (xy) or 3 (ground)";
   if (nv > (int64_t)1 << 30 || nf > (int64_t)1 << 30) return "mesh too large";
   if (singular_order < 1 || singular_order > 4) return "singular_order must be in 1..4";
   for (int64_t e = 0; e < 3 * nf; ++e)
@@ -383,13 +384,141 @@ std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64
       p.test = internal_of[t];
       p.trial = internal_of[s];
       p.rule = ri;
-      p.pad = kind;
+      p.xform = 0;
       p.jac = (2.0 * area[t]) * (2.0 * area[s]);
       p.nprod = nrm[3 * t] * nrm[3 * s] + nrm[3 * t + 1] * nrm[3 * s + 1] + nrm[3 * t + 2] * nrm[3 * s + 2];
       m.spairs.push_back(p);
       if (kind == 1) ++m.n_coincident;
       else if (kind == 2) ++m.n_edge;
       else ++m.n_vertex;
+    }
+  }
+
+  // --- symmetry images: pairs whose reflected trial face touches the test face
+  // get the Duffy value minus the regular value the image pass adds for them
+  // (image_singular_candidates, BeatEngineCore.jl:1139-1166, geometric_adjacency_info
+  // :624-658, _beat_cpu_image_singular_cache, BeatEngineCpuAssembly.jl:488-523,
+  // _beat_cpu_bm_image_singular_delta_test!, BeatEngineCpuBurtonMiller.jl:232-265).
+  // The delta rule of an orientation key is the remapped Duffy rule followed by
+  // the regular product rule with negated weights, so one pass over it forms the
+  // difference.
+  m.symmetry = symmetry;
+  m.nimg = image_transforms(symmetry, m.img);
+  m.n_image_singular = 0;
+  m.weighted_rows.clear();
+  m.row_weights.clear();
+  if (m.nimg > 0) {
+    const double gtol = 1e-8;
+    auto key_of = [&](const double* p, const int* sg) {
+      return std::array<long long, 3>{(long long)std::nearbyint(sg[0] * p[0] / gtol),
+                                      (long long)std::nearbyint(sg[1] * p[1] / gtol),
+                                      (long long)std::nearbyint(sg[2] * p[2] / gtol)};
+    };
+    const int ident[3] = {1, 1, 1};
+    std::map<std::array<long long, 3>, std::vector<int32_t>> by_key;
+    for (int64_t t = 0; t < nf; ++t)
+      for (int k = 0; k < 3; ++k) by_key[key_of(V + 3 * F[3 * t + k], ident)].push_back((int32_t)t);
+    for (int x = 0; x < m.nimg; ++x) {
+      const ImageXf& X = m.img[x];
+      std::map<std::array<int, 5>, int> delta_index;
+      std::set<std::pair<int32_t, int32_t>> seen;
+      for (int64_t s = 0; s < nf; ++s)
+        for (int k = 0; k < 3; ++k) {
+          auto it = by_key.find(key_of(V + 3 * F[3 * s + k], X.s));
+          if (it == by_key.end()) continue;
+          for (int32_t t : it->second) {
+            if (!seen.insert({t, (int32_t)s}).second) continue;
+            int sa[9], sb[9], ns = 0;
+            for (int i = 0; i < 3; ++i)
+              for (int j = 0; j < 3; ++j) {
+                const double* a = V + 3 * F[3 * t + i];
+                const double* b = V + 3 * F[3 * s + j];
+                const double d0 = a[0] - X.s[0] * b[0], d1 = a[1] - X.s[1] * b[1], d2 = a[2] - X.s[2] * b[2];
+                if (std::sqrt(d0 * d0 + d1 * d1 + d2 * d2) <= gtol) {
+                  sa[ns] = i + 1;
+                  sb[ns] = j + 1;
+                  ++ns;
+                }
+              }
+            if (ns == 0) continue;
+            std::array<int, 5> k5;
+            int kind;
+            if (ns >= 3) {
+              kind = 1;
+              k5 = {1, 0, 0, 0, 0};
+            } else if (ns == 2) {
+              if (sb[1] < sb[0]) {
+                std::swap(sa[0], sa[1]);
+                std::swap(sb[0], sb[1]);
+              }
+              kind = 2;
+              k5 = {2, sa[0], sa[1], sb[0], sb[1]};
+            } else {
+              kind = 3;
+              k5 = {3, sa[0], 0, sb[0], 0};
+            }
+            auto dit = delta_index.find(k5);
+            int ri;
+            if (dit == delta_index.end()) {
+              const Rule& b = base[kind - 1];
+              ri = (int)m.rule_off.size();
+              m.rule_off.push_back((int32_t)(m.rule_data.size() / 5));
+              m.rule_cnt.push_back((int32_t)(b.w.size() + (size_t)m.nq * m.nq));
+              m.rule_kind.push_back(kind);
+              for (size_t q = 0; q < b.w.size(); ++q) {
+                P2 a = b.tp[q], c = b.sp[q];
+                if (kind == 2) {
+                  a = remap_edge(a, k5[1], k5[2]);
+                  c = remap_edge(c, k5[3], k5[4]);
+                } else if (kind == 3) {
+                  a = remap_vertex(a, k5[1]);
+                  c = remap_vertex(c, k5[3]);
+                }
+                m.rule_data.insert(m.rule_data.end(), {a.x, a.y, c.x, c.y, b.w[q]});
+              }
+              for (int i = 0; i < m.nq; ++i)
+                for (int j = 0; j < m.nq; ++j)
+                  m.rule_data.insert(m.rule_data.end(), {m.rule_pts[2 * i], m.rule_pts[2 * i + 1], m.rule_pts[2 * j],
+                                                         m.rule_pts[2 * j + 1], -(m.rule_w[i] * m.rule_w[j])});
+              delta_index[k5] = ri;
+            } else {
+              ri = dit->second;
+            }
+            SingularPair p;
+            p.test = internal_of[t];
+            p.trial = internal_of[s];
+            p.rule = ri;
+            p.xform = x + 1;
+            p.jac = (2.0 * area[t]) * (2.0 * area[s]);
+            p.nprod = nrm[3 * t] * (X.s[0] * nrm[3 * s]) + nrm[3 * t + 1] * (X.s[1] * nrm[3 * s + 1]) +
+                      nrm[3 * t + 2] * (X.s[2] * nrm[3 * s + 2]);
+            m.spairs.push_back(p);
+            ++m.n_image_singular;
+          }
+        }
+    }
+  }
+  // P1 orbit weights: rows on a mirror plane stand for one physical dof shared
+  // with its image (symmetry_plane_tolerance / p1_symmetry_orbit_weights,
+  // BeatEngineCore.jl:497-517,563-580); the rigid ground has none.
+  if (symmetry == SYM_X || symmetry == SYM_XY) {
+    double blo[3] = {V[0], V[1], V[2]}, bhi[3] = {V[0], V[1], V[2]};
+    for (int64_t v = 0; v < nv; ++v)
+      for (int k = 0; k < 3; ++k) {
+        blo[k] = std::min(blo[k], V[3 * v + k]);
+        bhi[k] = std::max(bhi[k], V[3 * v + k]);
+      }
+    const double dg = std::sqrt((bhi[0] - blo[0]) * (bhi[0] - blo[0]) + (bhi[1] - blo[1]) * (bhi[1] - blo[1]) +
+                                (bhi[2] - blo[2]) * (bhi[2] - blo[2]));
+    const double ptol = std::max(1e-9, dg * 1e-6);
+    for (int64_t v = 0; v < nv; ++v) {
+      double w = 1.0;
+      if (std::fabs(V[3 * v]) <= ptol) w *= 2.0;
+      if (symmetry == SYM_XY && std::fabs(V[3 * v + 1]) <= ptol) w *= 2.0;
+      if (w != 1.0) {
+        m.weighted_rows.push_back(m.vert_inv[v]);
+        m.row_weights.push_back(w);
+      }
     }
   }
 

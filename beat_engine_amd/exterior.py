@@ -11,7 +11,11 @@ gfx950 library.
   dense_solve_diagnostics          <- beat_dense_solve_diagnostics (src/BeatEngineDenseSolve.jl:1160-1196)
 
 Arguments keep the reference's meaning: k is the wavenumber (normalised by
-outgoing_wavenumber), q_neumann is F x drives, pressures come back per drive.
+outgoing_wavenumber), q_neumann is F x drives, pressures come back per drive,
+symmetry_mode is :off / :x / :xy / :ground (normalized_symmetry_mode,
+src/BeatEngineCore.jl:448-460) with the mesh as the fundamental domain, used as
+given (the reference's driver snaps it first, BeatEngineDriver.jl:927-928;
+mesh.snap_symmetry_planes does the same here).
 """
 from __future__ import annotations
 
@@ -21,7 +25,7 @@ import weakref
 import numpy as np
 
 from . import _native
-from .mesh import BoundaryMesh
+from .mesh import BoundaryMesh, normalized_symmetry_mode
 from .phasor import NEGATIVE_TIME_PHASOR, burton_miller_coupling, outgoing_wavenumber
 
 PRECISIONS = {
@@ -70,11 +74,13 @@ class Engine:
         self._meshes.clear()
         self._fin()
 
-    def upload(self, mesh: BoundaryMesh, quadrature_order: int = 4, singular_order: int = 4) -> "DeviceMesh":
-        key = (id(mesh), quadrature_order, singular_order)
+    def upload(self, mesh: BoundaryMesh, quadrature_order: int = 4, singular_order: int = 4,
+               symmetry_mode="off") -> "DeviceMesh":
+        sym = normalized_symmetry_mode(symmetry_mode)
+        key = (id(mesh), quadrature_order, singular_order, sym)
         dm = self._meshes.get(key)
         if dm is None or dm._mesh_ref() is not mesh:
-            dm = DeviceMesh(self, mesh, quadrature_order, singular_order)
+            dm = DeviceMesh(self, mesh, quadrature_order, singular_order, sym)
             self._meshes[key] = dm
         return dm
 
@@ -129,19 +135,23 @@ class Engine:
 
 
 class DeviceMesh:
-    """Mesh resident on the device with its tiles, touching pairs and Duffy rules."""
+    """Mesh resident on the device with its tiles, touching pairs and Duffy rules
+    (and, with a symmetry mode, its image-touching pairs and orbit weights)."""
 
-    def __init__(self, engine: Engine, mesh: BoundaryMesh, quadrature_order=4, singular_order=4):
+    def __init__(self, engine: Engine, mesh: BoundaryMesh, quadrature_order=4, singular_order=4, symmetry_mode="off"):
         self.engine = engine
         self._mesh_ref = weakref.ref(mesh)
         self.num_vertices = mesh.num_vertices
         self.num_faces = mesh.num_faces
         self.quadrature_order = quadrature_order
         self.singular_order = singular_order
+        self.symmetry_mode = normalized_symmetry_mode(symmetry_mode)
         h = ctypes.c_void_p()
-        rc = engine._lib.beat_mesh_upload(engine._h, _ptr(mesh.vertices), mesh.num_vertices, _ptr(mesh.faces),
-                                          mesh.num_faces, int(quadrature_order), int(singular_order), ctypes.byref(h))
-        engine._check(rc, "beat_mesh_upload")
+        rc = engine._lib.beat_mesh_upload_symmetric(engine._h, _ptr(mesh.vertices), mesh.num_vertices,
+                                                    _ptr(mesh.faces), mesh.num_faces, int(quadrature_order),
+                                                    int(singular_order), _native.SYMMETRY_CODES[self.symmetry_mode],
+                                                    ctypes.byref(h))
+        engine._check(rc, "beat_mesh_upload_symmetric")
         self._h = h
         self._fin = weakref.finalize(self, engine._lib.beat_mesh_destroy, h)
 
@@ -151,8 +161,11 @@ class DeviceMesh:
     def counts(self) -> dict:
         c = np.zeros(6, dtype=np.int64)
         self.engine._check(self.engine._lib.beat_mesh_counts(self._h, _ptr(c)), "beat_mesh_counts")
+        s = np.zeros(4, dtype=np.int64)
+        self.engine._check(self.engine._lib.beat_mesh_symmetry(self._h, _ptr(s)), "beat_mesh_symmetry")
         return dict(regular_pairs=int(c[0]), coincident=int(c[1]), edge=int(c[2]), vertex=int(c[3]),
-                    tiles=int(c[4]), rule_points=int(c[5]))
+                    tiles=int(c[4]), rule_points=int(c[5]), image_transforms=int(s[1]),
+                    image_singular_pairs=int(s[2]), orbit_weighted_rows=int(s[3]))
 
     def _q(self, q):
         q = np.asarray(q)
@@ -232,10 +245,11 @@ def _rule_order(rule) -> int:
 
 
 def assemble_burton_miller_neumann_system_hip(mesh: BoundaryMesh, q_neumann, k, rule=4, *, singular_order=4,
-                                              precision="float64", convention=NEGATIVE_TIME_PHASOR, engine=None):
+                                              precision="float64", symmetry_mode="off",
+                                              convention=NEGATIVE_TIME_PHASOR, engine=None):
     """(matrix, rhs) of the fused system; k is normalised to the outgoing wavenumber."""
     eng = engine or default_engine()
-    dm = eng.upload(mesh, _rule_order(rule), singular_order)
+    dm = eng.upload(mesh, _rule_order(rule), singular_order, symmetry_mode)
     ks = outgoing_wavenumber(k, convention)
     A, B = dm.assemble(ks, burton_miller_coupling(ks, convention), q_neumann, precision)
     return dict(matrix=A, rhs=B, assembly_mode="hip_fused_burton_miller", on_gpu=True, **dm.counts())
@@ -243,14 +257,15 @@ def assemble_burton_miller_neumann_system_hip(mesh: BoundaryMesh, q_neumann, k, 
 
 def solve_exterior_direct_hip(mesh: BoundaryMesh, p1_space, dp0_space, neumann_values, wavenumber, rule=4, *,
                               precision="float32_refined", singular_order=4, refinement_steps=3,
-                              refinement_rtol=1e-10, convention=NEGATIVE_TIME_PHASOR, engine=None):
+                              refinement_rtol=1e-10, symmetry_mode="off", convention=NEGATIVE_TIME_PHASOR,
+                              engine=None):
     """Same contract as solve_exterior_direct_cuda: one fused assembly and one
     factorization for every excitation; returns (pressures, assembly_s, solve_s,
     rhs) with rhs = (mode, operator_bytes), plus the dense-solve diagnostics
     as a fifth element."""
     del p1_space, dp0_space  # P1 = vertices and DP0 = faces, as in the reference
     eng = engine or default_engine()
-    dm = eng.upload(mesh, _rule_order(rule), singular_order)
+    dm = eng.upload(mesh, _rule_order(rule), singular_order, symmetry_mode)
     q = np.stack([np.asarray(v, dtype=np.complex128) for v in neumann_values], axis=1)
     ks = outgoing_wavenumber(wavenumber, convention)
     P, rep = dm.solve(ks, burton_miller_coupling(ks, convention), q, precision, refinement_steps, refinement_rtol)
@@ -260,9 +275,10 @@ def solve_exterior_direct_hip(mesh: BoundaryMesh, p1_space, dp0_space, neumann_v
 
 
 def evaluate_galerkin_field_hip(eval_points, mesh: BoundaryMesh, pressure, q_neumann, k, rule=4, *,
-                                precision="float64", convention=NEGATIVE_TIME_PHASOR, engine=None):
+                                precision="float64", symmetry_mode="off", convention=NEGATIVE_TIME_PHASOR,
+                                engine=None):
     eng = engine or default_engine()
-    dm = eng.upload(mesh, _rule_order(rule), 4)
+    dm = eng.upload(mesh, _rule_order(rule), 4, symmetry_mode)
     out = dm.field(outgoing_wavenumber(k, convention), pressure, q_neumann, eval_points, precision)
     return out[:, 0] if np.ndim(pressure) == 1 else out
 

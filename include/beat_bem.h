@@ -99,6 +99,28 @@ void beat_mesh_destroy(beat_mesh* mesh);
 /* counts[0..5] = regular pairs, coincident, edge, vertex pairs, tiles, regular-rule points */
 int beat_mesh_counts(const beat_mesh* mesh, int64_t* counts);
 
+/* Symmetry modes: the mesh is the fundamental domain and the solve adds its
+ * mirror images (normalized_symmetry_mode / symmetry_image_transforms,
+ * BeatEngineCore.jl:448-495). X: the plane x = 0; XY: x = 0 and y = 0 (three
+ * images); GROUND: a rigid ground plane y = 0. */
+#define BEAT_SYM_OFF 0
+#define BEAT_SYM_X 1
+#define BEAT_SYM_XY 2
+#define BEAT_SYM_GROUND 3
+
+/* beat_mesh_upload with a symmetry mode: image passes over every (test,
+ * reflected trial) pair, Duffy-minus-regular deltas for the image pairs that
+ * touch (image_singular_candidates, BeatEngineCore.jl:1152-1166;
+ * _beat_cpu_bm_image_singular_delta_test!, BeatEngineCpuBurtonMiller.jl:232-265),
+ * P1 orbit weights on the rows of plane vertices (p1_symmetry_orbit_weights,
+ * BeatEngineCore.jl:566-580) and image sources in the field. The vertices are
+ * used as given: snap plane vertices to the plane first, as the reference's
+ * driver does with snap_symmetry_planes (BeatEngineDriver.jl:927). */
+int beat_mesh_upload_symmetric(beat_ctx* ctx, const double* verts, int64_t nv, const int32_t* faces, int64_t nf,
+                               int quad_order, int singular_order, int symmetry, beat_mesh** out);
+/* info[0..3] = symmetry mode, image transforms, image-singular pairs, rows with an orbit weight */
+int beat_mesh_symmetry(const beat_mesh* mesh, int64_t* info);
+
 /* The assembled system A (nv x nv) and B (nv x nrhs), complex128, without
  * solving: the operator-level parity point. Replaces
  * assemble_burton_miller_neumann_system_cpu (BeatEngineCpuBurtonMiller.jl:276-469)
