@@ -56,6 +56,8 @@ def lib():
                                                               I]
             getattr(L, f"oracle_operators_{s}").argtypes = [P, I64, P, I64, I, I, D, P, P, P, P, I]
             getattr(L, f"oracle_field_{s}").argtypes = [P, I64, P, I64, I, D, P, P, I64, P, I64, P, I]
+            getattr(L, f"oracle_field_gradient_{s}").argtypes = [P, I64, P, I64, I, D, P, P, I64, P, I64, P, I]
+            getattr(L, f"oracle_field_gradient_{s}").restype = I64
         L.oracle_identity_f64.argtypes = [P, I64, P, I64, I, P, P, I]
         L.oracle_orbit_weights.argtypes = [P, I64, P, I64, I, P]
         L.oracle_image_singular_pairs.argtypes = [P, I64, P, I64, I, I, I, P]
@@ -261,6 +263,32 @@ def field(vertices, faces, k, pressure, q, points, quad_order=4, precision="f64"
 
 
 # ---- dense solve ------------------------------------------------------------------
+def field_gradient(vertices, faces, k, pressure, q, points, quad_order=4, precision="f64", symmetry="off"):
+    """evaluate_galerkin_field_gradient_cpu (JL/src/BeatEngineFieldGradient.jl:45-92): array
+    (points, 4, rhs) of pressure and its x, y, z derivatives; k is the signed wavenumber.
+    Raises like the reference when a point is too close to the boundary."""
+    V, F = _mesh_args(vertices, faces)
+    nv, nf = len(V), len(F)
+    ct = np.complex128 if precision == "f64" else np.complex64
+    p = np.asarray(pressure)
+    qq = np.asarray(q)
+    vec = p.ndim == 1
+    if p.ndim == 1:
+        p = p[:, None]
+    if qq.ndim == 1:
+        qq = qq[:, None]
+    p = np.asfortranarray(p.astype(ct))
+    qq = np.asfortranarray(qq.astype(ct))
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((len(pts), 4, p.shape[1]), dtype=ct, order="F")
+    fn = lib().oracle_field_gradient_f64 if precision == "f64" else lib().oracle_field_gradient_f32
+    failed = fn(_p(V), nv, _p(F), nf, quad_order, float(k), _p(p), _p(qq), p.shape[1], _p(pts), len(pts), _p(out),
+                _sym(symmetry))
+    if failed:
+        raise ValueError("BEM gradient sample is too close to a boundary for reliable evaluation.")
+    return out[:, :, 0] if vec else out
+
+
 def lu_solve(A, B):
     """LAPACK getrf/getrs with partial pivoting, one factorization for all columns
     (_beat_dense_lu_solve, BeatEngineDenseSolve.jl:1104-1110)."""

@@ -931,6 +931,161 @@ void field(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int quad
   }
 }
 
+// ---- field value and observation gradient -----------------------------------------
+// field_value_gradient: JL/src/BeatEngineFieldGradient.jl:2-13. Value and x-gradient
+// of the integrand of D p - S q at source y with normal n (r = y - x, signed k).
+template <class T> void field_value_gradient(V3<T> x, V3<T> y, V3<T> n, std::complex<T> p, std::complex<T> q, T k,
+                                             std::complex<T> out[4]) {
+  typedef std::complex<T> C;
+  V3<T> r = y - x;
+  T radius = norm(r);
+  V3<T> u = {r.x / radius, r.y / radius, r.z / radius};
+  C green = std::exp(C(T(0), k * radius)) / (T(4.0 * M_PI) * radius);
+  C a(-T(1) / radius, k);
+  C b(T(3) / (radius * radius) - k * k, -T(3) * k / radius);
+  T proj = dot(u, n);
+  out[0] = green * (a * proj * p - q);
+  const C aq = a * q, bp = b * proj, ar = a / radius;
+  out[1] = green * (aq * u.x - p * (bp * u.x + ar * n.x));
+  out[2] = green * (aq * u.y - p * (bp * u.y + ar * n.y));
+  out[3] = green * (aq * u.z - p * (bp * u.z + ar * n.z));
+}
+
+// _gradient_triangle: JL/src/BeatEngineFieldGradient.jl:15-24 (jac = |cross|).
+template <class T> void gradient_triangle(V3<T> x, const V3<T> v[3], V3<T> n, const std::complex<T> p[3],
+                                          std::complex<T> q, T k, const std::vector<P2<T>>& rp,
+                                          const std::vector<T>& rw, std::complex<T> acc[4]) {
+  const T jac = norm(cross(v[1] - v[0], v[2] - v[0]));
+  for (size_t j = 0; j < rw.size(); ++j) {
+    const T b0 = T(1) - rp[j].x - rp[j].y, b1 = rp[j].x, b2 = rp[j].y;
+    const V3<T> y = local_to_global(v, rp[j]);
+    std::complex<T> f[4];
+    field_value_gradient(x, y, n, b0 * p[0] + b1 * p[1] + b2 * p[2], q, k, f);
+    for (int d = 0; d < 4; ++d) acc[d] += f[d] * (jac * rw[j]);
+  }
+}
+
+// _adaptive_gradient_triangle: JL/src/BeatEngineFieldGradient.jl:26-43. Returns false
+// where depth 12 is reached without meeting the size/distance criterion.
+inline bool adaptive_gradient_triangle(V3<double> x, const V3<double> v[3], V3<double> n,
+                                       const std::complex<double> p[3], std::complex<double> q, double k,
+                                       const std::vector<P2<double>>& rp, const std::vector<double>& rw, int depth,
+                                       std::complex<double> acc[4]) {
+  const V3<double> c = {(v[0].x + v[1].x + v[2].x) / 3, (v[0].y + v[1].y + v[2].y) / 3, (v[0].z + v[1].z + v[2].z) / 3};
+  const double diameter = std::max({norm(v[0] - v[1]), norm(v[1] - v[2]), norm(v[2] - v[0])});
+  const double distance = norm(x - c);
+  if (diameter <= 0.4 * distance && std::abs(k) * diameter <= 0.7) {
+    gradient_triangle<double>(x, v, n, p, q, k, rp, rw, acc);
+    return true;
+  }
+  if (depth >= 12) return false;
+  auto mid = [](V3<double> a, V3<double> b) { return V3<double>{(a.x + b.x) / 2, (a.y + b.y) / 2, (a.z + b.z) / 2}; };
+  const V3<double> ab = mid(v[0], v[1]), bc = mid(v[1], v[2]), ca = mid(v[2], v[0]);
+  const std::complex<double> pab = (p[0] + p[1]) / 2.0, pbc = (p[1] + p[2]) / 2.0, pca = (p[2] + p[0]) / 2.0;
+  const V3<double> t0[3] = {v[0], ab, ca}, t1[3] = {ab, v[1], bc}, t2[3] = {ca, bc, v[2]}, t3[3] = {ab, bc, ca};
+  const std::complex<double> p0[3] = {p[0], pab, pca}, p1[3] = {pab, p[1], pbc}, p2[3] = {pca, pbc, p[2]},
+                             p3[3] = {pab, pbc, pca};
+  std::complex<double> s[4][4] = {};
+  bool ok = adaptive_gradient_triangle(x, t0, n, p0, q, k, rp, rw, depth + 1, s[0]) &&
+            adaptive_gradient_triangle(x, t1, n, p1, q, k, rp, rw, depth + 1, s[1]) &&
+            adaptive_gradient_triangle(x, t2, n, p2, q, k, rp, rw, depth + 1, s[2]) &&
+            adaptive_gradient_triangle(x, t3, n, p3, q, k, rp, rw, depth + 1, s[3]);
+  for (int d = 0; d < 4; ++d) acc[d] += s[0][d] + s[1][d] + s[2][d] + s[3][d];
+  return ok;
+}
+
+// evaluate_galerkin_field_gradient_cpu with correct_field_gradient!:
+// JL/src/BeatEngineFieldGradient.jl:45-92. Values (np x 4 x nrhs, column-major:
+// pressure, d/dx, d/dy, d/dz) in precision T from the cache sources (rule of
+// quad_order, images of the symmetry mode), then for every face (and image)
+// whose centroid lies within 3 diameters of the point the FP64 adaptive integral
+// minus the order-4 rule in T. Returns the number of (point, face) pairs that hit
+// the depth limit (the reference raises an error for them).
+template <class T>
+int64_t field_gradient(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int quad_order, double k_signed,
+                       const std::complex<T>* p, const std::complex<T>* q, int64_t nrhs, const double* pts, int64_t np,
+                       std::complex<T>* out, int sym) {
+  typedef std::complex<T> C;
+  set_threads();
+  Mesh<T> m = make_mesh<T>(xyz, nv, f, nf);
+  std::vector<P2<T>> rp, rp4;
+  std::vector<T> rw, rw4;
+  triangle_rule<T>(quad_order, rp, rw);
+  triangle_rule<T>(4, rp4, rw4);
+  std::vector<P2<double>> fp;
+  std::vector<double> fw;
+  triangle_rule<double>(4, fp, fw);
+  const int nq = (int)rw.size();
+  std::vector<Xf> xfs = {{{1, 1, 1}, 1}};
+  for (auto& x : image_transforms(sym)) xfs.push_back(x);
+  const int nt = (int)xfs.size();
+  const int64_t ns = nf * nt * nq;
+  std::vector<V3<T>> sy(ns), sn(ns);
+  std::vector<T> sw(ns);
+  std::vector<std::array<T, 3>> sb(ns);
+  for (int64_t e = 0; e < nf; ++e) {
+    T jac = T(2.0) * m.areas[e];
+    for (int x = 0; x < nt; ++x) {
+      V3<T> v[3] = {reflect(xfs[x], m.verts[m.faces[e][0]]), reflect(xfs[x], m.verts[m.faces[e][1]]),
+                    reflect(xfs[x], m.verts[m.faces[e][2]])};
+      for (int qq = 0; qq < nq; ++qq) {
+        const int64_t s = (e * nt + x) * nq + qq;
+        sy[s] = local_to_global(v, rp[qq]);
+        sn[s] = reflect(xfs[x], m.normals[e]);
+        sw[s] = rw[qq] * jac;
+        sb[s] = p1_values(rp[qq]);
+      }
+    }
+  }
+  const T k = T(k_signed);
+  int64_t failed = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : failed)
+  for (int64_t i = 0; i < np; ++i) {
+    const V3<T> x = {T(pts[3 * i]), T(pts[3 * i + 1]), T(pts[3 * i + 2])};
+    const V3<double> xd = {double(x.x), double(x.y), double(x.z)};
+    for (int64_t d = 0; d < nrhs; ++d) {
+      C acc[4] = {};
+      for (int64_t s = 0; s < ns; ++s) {
+        const int64_t e = s / (nq * nt);
+        const std::array<int, 3>& fc = m.faces[e];
+        const C ps = sb[s][0] * p[fc[0] + d * nv] + sb[s][1] * p[fc[1] + d * nv] + sb[s][2] * p[fc[2] + d * nv];
+        C fv[4];
+        field_value_gradient(x, sy[s], sn[s], ps, q[e + d * nf], k, fv);
+        for (int c = 0; c < 4; ++c) acc[c] += fv[c] * sw[s];
+      }
+      std::complex<double> corr[4] = {};
+      for (int64_t e = 0; e < nf; ++e)
+        for (int t = 0; t < nt; ++t) {
+          V3<T> v[3];
+          V3<double> vd[3];
+          for (int a = 0; a < 3; ++a) {
+            v[a] = reflect(xfs[t], m.verts[m.faces[e][a]]);
+            vd[a] = {double(v[a].x), double(v[a].y), double(v[a].z)};
+          }
+          const V3<T> c = {(v[0].x + v[1].x + v[2].x) / T(3), (v[0].y + v[1].y + v[2].y) / T(3),
+                           (v[0].z + v[1].z + v[2].z) / T(3)};
+          const T diameter = std::max({norm(v[0] - v[1]), norm(v[1] - v[2]), norm(v[2] - v[0])});
+          if (!(norm(x - c) <= T(3) * diameter)) continue;
+          const V3<T> n = reflect(xfs[t], m.normals[e]);
+          const V3<double> nd = {double(n.x), double(n.y), double(n.z)};
+          const C pv[3] = {p[m.faces[e][0] + d * nv], p[m.faces[e][1] + d * nv], p[m.faces[e][2] + d * nv]};
+          C coarse[4] = {};
+          gradient_triangle<T>(x, v, n, pv, q[e + d * nf], k, rp4, rw4, coarse);
+          const std::complex<double> pd[3] = {std::complex<double>(pv[0]), std::complex<double>(pv[1]),
+                                              std::complex<double>(pv[2])};
+          std::complex<double> fine[4] = {};
+          if (!adaptive_gradient_triangle(xd, vd, nd, pd, std::complex<double>(q[e + d * nf]), double(k), fp, fw, 0,
+                                          fine))
+            ++failed;
+          for (int c2 = 0; c2 < 4; ++c2) corr[c2] += fine[c2] - std::complex<double>(coarse[c2]);
+        }
+      // the FP64 correction is added to the precision-T value (values[i,d] += correction[d])
+      for (int c = 0; c < 4; ++c) out[i + c * np + d * np * 4] = C(std::complex<double>(acc[c]) + corr[c]);
+    }
+  }
+  return failed;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1052,6 +1207,20 @@ void oracle_field_f32(const double* xyz, int64_t nv, const int32_t* f, int64_t n
                       const float* q, int64_t nrhs, const double* pts, int64_t np, float* out, int sym) {
   typedef std::complex<float> cf;
   field<float>(xyz, nv, f, nf, qo, k, (const cf*)p, (const cf*)q, nrhs, pts, np, (cf*)out, sym);
+}
+
+// Field value and gradient (out: np x 4 x nrhs complex, column-major); returns the
+// count of (point, face) pairs beyond the adaptive depth limit.
+int64_t oracle_field_gradient_f64(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int qo, double k,
+                                  const double* p, const double* q, int64_t nrhs, const double* pts, int64_t np,
+                                  double* out, int sym) {
+  return field_gradient<double>(xyz, nv, f, nf, qo, k, (const cd*)p, (const cd*)q, nrhs, pts, np, (cd*)out, sym);
+}
+int64_t oracle_field_gradient_f32(const double* xyz, int64_t nv, const int32_t* f, int64_t nf, int qo, double k,
+                                  const float* p, const float* q, int64_t nrhs, const double* pts, int64_t np,
+                                  float* out, int sym) {
+  typedef std::complex<float> cf;
+  return field_gradient<float>(xyz, nv, f, nf, qo, k, (const cf*)p, (const cf*)q, nrhs, pts, np, (cf*)out, sym);
 }
 
 // p1_symmetry_orbit_weights per vertex.
