@@ -59,7 +59,7 @@ struct beat_ctx {
   std::string err;
   std::mutex mu;
   DevBuf a64, a32, b64, b32, q64, q32, x64, r64, n64, rn64, xn64, d32, work, chunk, norms, ipiv, lusmall, qin, pout,
-      fld_src, fld_out, fld_pts;
+      fld_src, fld_out, fld_pts, grad;
   hipEvent_t ev[6] = {};
   double* trig = nullptr;  // (sin, cos)(m pi/64) table of the FP64 phase
 };
@@ -79,6 +79,7 @@ struct beat_mesh {
   int32_t *rule_off = nullptr, *rule_cnt = nullptr;
   double* rule_data = nullptr;
   double* rule3 = nullptr;  // regular rule: xi, eta, w per point
+  double* rule4 = nullptr;  // the 6-point rule (xi, eta, w) of the near-panel gradient correction
   int32_t* face_perm = nullptr;
   int32_t* vert_inv = nullptr;
   int32_t* wrows = nullptr;  // rows on a mirror plane and their orbit weights
@@ -666,6 +667,138 @@ hipError_t upload(T** dst, const std::vector<T>& v) {
   return hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
 }
 
+
+// Near-panel correction of the gradient (correct_field_gradient!,
+// BeatEngineFieldGradient.jl:45-73): list the (point, face image) pairs within 3
+// face diameters (count, then fill), then integrate each pair adaptively.
+template <typename T>
+int near_panel_correction(beat_ctx* ctx, const beat_mesh* m, const T* pts, int64_t npts, const T* p, const T* q,
+                          int nrhs, double k, T* out) {
+  hipStream_t s = ctx->stream;
+  HIPCHK(ensure(ctx->grad, 256));
+  unsigned long long* count = as<unsigned long long>(ctx->grad);
+  int* failed = reinterpret_cast<int*>(count + 1);
+  HIPCHK(hipMemsetAsync(ctx->grad.p, 0, 16, s));
+  HIPCHK(gradient_near_pairs<T>(m->verts, m->fverts, m->h.nf, m->h.nimg, m->h.img, pts, npts, count, nullptr, 0, false,
+                                s));
+  unsigned long long npairs = 0;
+  HIPCHK(hipMemcpyAsync(&npairs, count, sizeof(npairs), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (npairs == 0) return BEAT_OK;
+  if (npairs > (1ull << 31)) return fail(ctx, BEAT_ERR_ARG, "too many near-panel gradient pairs");
+  HIPCHK(ensure(ctx->grad, 256 + sizeof(int2) * npairs));
+  count = as<unsigned long long>(ctx->grad);
+  failed = reinterpret_cast<int*>(count + 1);
+  int2* list = reinterpret_cast<int2*>(reinterpret_cast<char*>(ctx->grad.p) + 256);
+  HIPCHK(hipMemsetAsync(ctx->grad.p, 0, 16, s));
+  HIPCHK(gradient_near_pairs<T>(m->verts, m->fverts, m->h.nf, m->h.nimg, m->h.img, pts, npts, count, list, npairs, true,
+                                s));
+  HIPCHK(gradient_correction<T>(m->verts, m->fverts, m->rec64, m->h.nimg, m->h.img, list, (int64_t)npairs, pts, p, q,
+                                nrhs, k, m->rule4, out, failed, s));
+  int nfail = 0;
+  HIPCHK(hipMemcpyAsync(&nfail, failed, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (nfail > 0)
+    return fail(ctx, BEAT_ERR_ARG, "BEM gradient sample is too close to a boundary for reliable evaluation.");
+  return BEAT_OK;
+}
+
+// Field (gradient false: npts x nrhs) or value and gradient (gradient true:
+// npts x 4 x nrhs) at the caller's points, column-major complex128.
+int field_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const double* pressure, const double* q,
+               int64_t nrhs, const double* pts, int64_t npts, double* out, bool gradient) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (nrhs < 1 || npts < 0 || (npts > 0 && (!pressure || !q || !pts || !out))) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (precision != BEAT_PREC_FP64 && precision != BEAT_PREC_FP32) return fail(ctx, BEAT_ERR_ARG, "precision");
+  if (npts == 0) return BEAT_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t n = m->h.nv, nf = m->h.nf, ns = nf * m->h.nq * (1 + m->h.nimg);
+  // stage: pressure (n x nrhs, caller col-major), q (nf x nrhs) -> internal row-major
+  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * (nf + n) * nrhs));
+  double* qin = as<double>(ctx->qin);
+  double* pin = qin + 2 * nf * nrhs;
+  HIPCHK(hipMemcpyAsync(qin, q, sizeof(double) * 2 * nf * nrhs, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(pin, pressure, sizeof(double) * 2 * n * nrhs, hipMemcpyHostToDevice, s));
+  HIPCHK(ensure(ctx->q64, sizeof(double) * 2 * (nf + n) * nrhs));
+  double* qint = as<double>(ctx->q64);
+  double* pint = qint + 2 * nf * nrhs;
+  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, s, (const double*)qin, nf,
+                     (int)nrhs, (const int32_t*)m->face_perm, qint);
+  // pressure: internal row v takes caller row vert_perm[v]; gather with the vertex permutation
+  std::vector<int32_t> vp(m->h.vert_perm);
+  HIPCHK(ensure(ctx->lusmall, sizeof(int32_t) * n + 256));
+  HIPCHK(hipMemcpyAsync(ctx->lusmall.p, vp.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s, (const double*)pin, n,
+                     (int)nrhs, (const int32_t*)as<int32_t>(ctx->lusmall), pint);
+  HIPCHK(hipGetLastError());
+  const size_t es = precision == BEAT_PREC_FP64 ? sizeof(double) : sizeof(float);
+  HIPCHK(ensure(ctx->fld_src, es * (ns * 6 + 4 * ns * nrhs + 2 * (nf + n) * nrhs)));
+  HIPCHK(ensure(ctx->fld_pts, sizeof(double) * 3 * npts));
+  const int ncomp = gradient ? 4 : 1;
+  HIPCHK(ensure(ctx->fld_out, sizeof(double) * 2 * npts * nrhs * ncomp));
+  HIPCHK(hipMemcpyAsync(ctx->fld_pts.p, pts, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, s));
+  std::vector<double> res((size_t)2 * npts * nrhs * ncomp);
+  if (precision == BEAT_PREC_FP64) {
+    double* sp = as<double>(ctx->fld_src);
+    double* sn = sp + 3 * ns;
+    double* pw = sn + 3 * ns;
+    double* qw = pw + 2 * ns * nrhs;
+    HIPCHK(field_sources<double>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, pint, qint, (int)nrhs, m->h.nimg, m->h.img,
+                                 sp, sn, pw, qw, s));
+    if (gradient) {
+      HIPCHK(field_gradient_eval<double>(sp, sn, pw, qw, ns, (int)nrhs, as<double>(ctx->fld_pts), npts,
+                                         as<double>(ctx->fld_out), k, s));
+      if (int rc = near_panel_correction<double>(ctx, m, as<double>(ctx->fld_pts), npts, pint, qint, (int)nrhs, k,
+                                                 as<double>(ctx->fld_out)))
+        return rc;
+    } else {
+      HIPCHK(field_eval<double>(sp, sn, pw, qw, ns, (int)nrhs, as<double>(ctx->fld_pts), npts,
+                                as<double>(ctx->fld_out), k, s));
+    }
+    HIPCHK(hipMemcpyAsync(res.data(), ctx->fld_out.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, s));
+  } else {
+    float* sp = as<float>(ctx->fld_src);
+    float* sn = sp + 3 * ns;
+    float* pw = sn + 3 * ns;
+    float* qw = pw + 2 * ns * nrhs;
+    float* p32 = qw + 2 * ns * nrhs;
+    float* q32 = p32 + 2 * n * nrhs;
+    HIPCHK(launch_downcast(pint, p32, 2 * n * nrhs, s));
+    HIPCHK(launch_downcast(qint, q32, 2 * nf * nrhs, s));
+    // points in float
+    float* pts32 = reinterpret_cast<float*>(as<double>(ctx->fld_pts) + 0);
+    std::vector<float> hp(3 * npts);
+    for (int64_t i = 0; i < 3 * npts; ++i) hp[i] = (float)pts[i];
+    HIPCHK(hipMemcpyAsync(pts32, hp.data(), sizeof(float) * hp.size(), hipMemcpyHostToDevice, s));
+    float* o32 = as<float>(ctx->fld_out);
+    HIPCHK(field_sources<float>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, p32, q32, (int)nrhs, m->h.nimg, m->h.img,
+                                sp, sn, pw, qw, s));
+    if (gradient) {
+      HIPCHK(field_gradient_eval<float>(sp, sn, pw, qw, ns, (int)nrhs, pts32, npts, o32, (float)k, s));
+      if (int rc = near_panel_correction<float>(ctx, m, pts32, npts, p32, q32, (int)nrhs, k, o32)) return rc;
+    } else {
+      HIPCHK(field_eval<float>(sp, sn, pw, qw, ns, (int)nrhs, pts32, npts, o32, (float)k, s));
+    }
+    std::vector<float> r32(res.size());
+    HIPCHK(hipMemcpyAsync(r32.data(), o32, sizeof(float) * r32.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::copy(r32.begin(), r32.end(), res.begin());
+  }
+  HIPCHK(hipStreamSynchronize(s));
+  // internal (point, rhs, component) -> caller (point, component, rhs), column-major
+  for (int64_t d = 0; d < nrhs; ++d)
+    for (int c = 0; c < ncomp; ++c)
+      for (int64_t i = 0; i < npts; ++i) {
+        const size_t src = 2 * ((size_t)(i * nrhs + d) * ncomp + c);
+        const size_t dst = 2 * (size_t)(i + npts * (c + (int64_t)ncomp * d));
+        out[dst] = res[src];
+        out[dst + 1] = res[src + 1];
+      }
+  return BEAT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -723,7 +856,7 @@ void beat_ctx_destroy(beat_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64,
                     &c->rn64, &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
-                    &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts};
+                    &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (auto& e : c->ev)
@@ -777,6 +910,13 @@ int beat_mesh_upload_symmetric(beat_ctx* ctx, const double* verts, int64_t nv, c
   if (e == hipSuccess) e = upload(&m->rule_cnt, m->h.rule_cnt);
   if (e == hipSuccess) e = upload(&m->rule_data, m->h.rule_data);
   if (e == hipSuccess) e = upload(&m->rule3, rule3);
+  {
+    double rp[12], rw[6];
+    triangle_rule(4, rp, rw);
+    std::vector<double> r4;
+    for (int q = 0; q < 6; ++q) r4.insert(r4.end(), {rp[2 * q], rp[2 * q + 1], rw[q]});
+    if (e == hipSuccess) e = upload(&m->rule4, r4);
+  }
   if (e == hipSuccess) e = upload(&m->face_perm, m->h.face_perm);
   if (e == hipSuccess) e = upload(&m->vert_inv, m->h.vert_inv);
   if (e == hipSuccess) e = upload(&m->wrows, m->h.weighted_rows);
@@ -815,7 +955,8 @@ void beat_mesh_destroy(beat_mesh* m) {
   if (!m) return;
   void* ptrs[] = {m->rec64, m->rec32, m->verts, m->fverts, m->floc, m->tile_start, m->tile_count, m->tile_nv,
                   m->tile_verts, m->stile_start, m->stile_count, m->stile_nv, m->stile_verts, m->flocs, m->spairs,
-                  m->rule_off, m->rule_cnt, m->rule_data, m->rule3, m->face_perm, m->vert_inv, m->wrows, m->wvals};
+                  m->rule_off, m->rule_cnt, m->rule_data, m->rule3, m->rule4, m->face_perm, m->vert_inv, m->wrows,
+                  m->wvals};
   if (m->ctx) (void)hipSetDevice(m->ctx->device);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -928,78 +1069,12 @@ int beat_bm_solve_device(beat_ctx* ctx, const beat_mesh* m, int precision, doubl
 
 int beat_field_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const double* pressure,
                     const double* q, int64_t nrhs, const double* pts, int64_t npts, double* out) {
-  if (int rc = check_mesh(ctx, m)) return rc;
-  if (nrhs < 1 || npts < 0 || (npts > 0 && (!pressure || !q || !pts || !out))) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
-  if (precision != BEAT_PREC_FP64 && precision != BEAT_PREC_FP32) return fail(ctx, BEAT_ERR_ARG, "precision");
-  if (npts == 0) return BEAT_OK;
-  std::lock_guard<std::mutex> lock(ctx->mu);
-  HIPCHK(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  const int64_t n = m->h.nv, nf = m->h.nf, ns = nf * m->h.nq * (1 + m->h.nimg);
-  // stage: pressure (n x nrhs, caller col-major), q (nf x nrhs) -> internal row-major
-  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * (nf + n) * nrhs));
-  double* qin = as<double>(ctx->qin);
-  double* pin = qin + 2 * nf * nrhs;
-  HIPCHK(hipMemcpyAsync(qin, q, sizeof(double) * 2 * nf * nrhs, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(pin, pressure, sizeof(double) * 2 * n * nrhs, hipMemcpyHostToDevice, s));
-  HIPCHK(ensure(ctx->q64, sizeof(double) * 2 * (nf + n) * nrhs));
-  double* qint = as<double>(ctx->q64);
-  double* pint = qint + 2 * nf * nrhs;
-  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, s, (const double*)qin, nf,
-                     (int)nrhs, (const int32_t*)m->face_perm, qint);
-  // pressure: internal row v takes caller row vert_perm[v]; gather with the vertex permutation
-  std::vector<int32_t> vp(m->h.vert_perm);
-  HIPCHK(ensure(ctx->lusmall, sizeof(int32_t) * n + 256));
-  HIPCHK(hipMemcpyAsync(ctx->lusmall.p, vp.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s, (const double*)pin, n,
-                     (int)nrhs, (const int32_t*)as<int32_t>(ctx->lusmall), pint);
-  HIPCHK(hipGetLastError());
-  const size_t es = precision == BEAT_PREC_FP64 ? sizeof(double) : sizeof(float);
-  HIPCHK(ensure(ctx->fld_src, es * (ns * 6 + 4 * ns * nrhs + 2 * (nf + n) * nrhs)));
-  HIPCHK(ensure(ctx->fld_pts, sizeof(double) * 3 * npts));
-  HIPCHK(ensure(ctx->fld_out, sizeof(double) * 2 * npts * nrhs));
-  HIPCHK(hipMemcpyAsync(ctx->fld_pts.p, pts, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, s));
-  std::vector<double> res((size_t)2 * npts * nrhs);
-  if (precision == BEAT_PREC_FP64) {
-    double* sp = as<double>(ctx->fld_src);
-    double* sn = sp + 3 * ns;
-    double* pw = sn + 3 * ns;
-    double* qw = pw + 2 * ns * nrhs;
-    HIPCHK(field_sources<double>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, pint, qint, (int)nrhs, m->h.nimg, m->h.img,
-                                 sp, sn, pw, qw, s));
-    HIPCHK(field_eval<double>(sp, sn, pw, qw, ns, (int)nrhs, as<double>(ctx->fld_pts), npts, as<double>(ctx->fld_out),
-                              k, s));
-    HIPCHK(hipMemcpyAsync(res.data(), ctx->fld_out.p, sizeof(double) * res.size(), hipMemcpyDeviceToHost, s));
-  } else {
-    float* sp = as<float>(ctx->fld_src);
-    float* sn = sp + 3 * ns;
-    float* pw = sn + 3 * ns;
-    float* qw = pw + 2 * ns * nrhs;
-    float* p32 = qw + 2 * ns * nrhs;
-    float* q32 = p32 + 2 * n * nrhs;
-    HIPCHK(launch_downcast(pint, p32, 2 * n * nrhs, s));
-    HIPCHK(launch_downcast(qint, q32, 2 * nf * nrhs, s));
-    // points in float
-    float* pts32 = reinterpret_cast<float*>(as<double>(ctx->fld_pts) + 0);
-    std::vector<float> hp(3 * npts);
-    for (int64_t i = 0; i < 3 * npts; ++i) hp[i] = (float)pts[i];
-    HIPCHK(hipMemcpyAsync(pts32, hp.data(), sizeof(float) * hp.size(), hipMemcpyHostToDevice, s));
-    float* o32 = as<float>(ctx->fld_out);
-    HIPCHK(field_sources<float>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, p32, q32, (int)nrhs, m->h.nimg, m->h.img,
-                                sp, sn, pw, qw, s));
-    HIPCHK(field_eval<float>(sp, sn, pw, qw, ns, (int)nrhs, pts32, npts, o32, (float)k, s));
-    std::vector<float> r32(res.size());
-    HIPCHK(hipMemcpyAsync(r32.data(), o32, sizeof(float) * r32.size(), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    std::copy(r32.begin(), r32.end(), res.begin());
-  }
-  HIPCHK(hipStreamSynchronize(s));
-  for (int64_t d = 0; d < nrhs; ++d)
-    for (int64_t i = 0; i < npts; ++i) {
-      out[2 * (i + d * npts)] = res[2 * (i * nrhs + d)];
-      out[2 * (i + d * npts) + 1] = res[2 * (i * nrhs + d) + 1];
-    }
-  return BEAT_OK;
+  return field_impl(ctx, m, precision, k, pressure, q, nrhs, pts, npts, out, false);
+}
+
+int beat_field_gradient_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const double* pressure,
+                             const double* q, int64_t nrhs, const double* pts, int64_t npts, double* out) {
+  return field_impl(ctx, m, precision, k, pressure, q, nrhs, pts, npts, out, true);
 }
 
 int beat_dense_lu_solve(beat_ctx* ctx, int precision, const double* a, int64_t n, const double* b, int64_t nrhs,

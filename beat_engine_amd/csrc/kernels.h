@@ -142,4 +142,25 @@ hipError_t field_sources(const double* rec, const int32_t* fverts, int64_t nf, i
                          const T* q, int nrhs, int nimg, const ImageXf* img, T* src_pts, T* src_nrm, T* pw, T* qw,
                          hipStream_t s);
 
+// Value and x, y, z derivatives of the field (out: npts x nrhs x 4 complex,
+// internal row-major) from the same sources as field_eval.
+template <typename T>
+hipError_t field_gradient_eval(const T* src_pts, const T* src_nrm, const T* pw, const T* qw, int64_t nsrc, int nrhs,
+                               const T* pts, int64_t npts, T* out, T k, hipStream_t s);
+// (point, face image) pairs with the image's centroid within 3 diameters of the
+// point: counted into *count (zeroed by the caller) and, with fill, stored in list
+// while the count stays below cap (entry = point, face * (1 + nimg) + transform).
+template <typename T>
+hipError_t gradient_near_pairs(const double* verts, const int32_t* fverts, int64_t nf, int nimg, const ImageXf* img,
+                               const T* pts, int64_t npts, unsigned long long* count, int2* list,
+                               unsigned long long cap, bool fill, hipStream_t s);
+// Adds the adaptive FP64 integral minus the 6-point rule (rule4: xi, eta, w per
+// point) of every listed pair to out; *failed (zeroed by the caller) counts pairs
+// that reach the depth limit without meeting the refinement criterion.
+template <typename T>
+hipError_t gradient_correction(const double* verts, const int32_t* fverts, const double* rec, int nimg,
+                               const ImageXf* img, const int2* list, int64_t npairs, const T* pts, const T* p,
+                               const T* q, int nrhs, double k, const double* rule4, T* out, int* failed,
+                               hipStream_t s);
+
 }  // namespace beat

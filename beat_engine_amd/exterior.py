@@ -7,6 +7,9 @@ gfx950 library.
                                    <- assemble_burton_miller_neumann_system_cpu
                                       (src/BeatEngineCpuBurtonMiller.jl:276-469)
   evaluate_galerkin_field_hip      <- evaluate_galerkin_field_cpu (src/BeatEngineCpuField.jl:1-49)
+  evaluate_galerkin_field_gradient_hip
+                                   <- evaluate_galerkin_field_gradient_cpu / _rocm
+                                      (src/BeatEngineFieldGradient.jl:45-92)
   beat_refine_lu                   <- beat_refine_cpu_lu (src/BeatEngineCpuRefinement.jl:5-78)
   dense_solve_diagnostics          <- beat_dense_solve_diagnostics (src/BeatEngineDenseSolve.jl:1160-1196)
 
@@ -227,6 +230,20 @@ class DeviceMesh:
         self.engine._check(rc, "beat_field_eval")
         return out
 
+    def field_gradient(self, k, pressure, q, points, precision="float64"):
+        """(points, 4, rhs): pressure and its x, y, z derivatives."""
+        p = np.asarray(pressure)
+        if p.ndim == 1:
+            p = p[:, None]
+        p = np.asfortranarray(p, dtype=np.complex128)
+        qq = self._q(q)
+        pts = np.ascontiguousarray(np.reshape(points, (-1, 3)), dtype=np.float64)
+        out = np.zeros((len(pts), 4, p.shape[1]), dtype=np.complex128, order="F")
+        rc = self.engine._lib.beat_field_gradient_eval(self.engine._h, self._h, _precision(precision), float(k),
+                                                       _ptr(p), _ptr(qq), p.shape[1], _ptr(pts), len(pts), _ptr(out))
+        self.engine._check(rc, "beat_field_gradient_eval")
+        return out
+
 
 _default_engine = None
 
@@ -281,6 +298,17 @@ def evaluate_galerkin_field_hip(eval_points, mesh: BoundaryMesh, pressure, q_neu
     dm = eng.upload(mesh, _rule_order(rule), 4, symmetry_mode)
     out = dm.field(outgoing_wavenumber(k, convention), pressure, q_neumann, eval_points, precision)
     return out[:, 0] if np.ndim(pressure) == 1 else out
+
+
+def evaluate_galerkin_field_gradient_hip(points, mesh: BoundaryMesh, pressure, neumann, k, *, precision="float64",
+                                         symmetry_mode="off", convention=NEGATIVE_TIME_PHASOR, engine=None):
+    """Same contract as evaluate_galerkin_field_gradient_rocm: a (points, 4) array of
+    pressure and its x, y, z derivatives (a third axis per column of pressure), with
+    the 4th-order rule and the near-panel correction."""
+    eng = engine or default_engine()
+    dm = eng.upload(mesh, 4, 4, symmetry_mode)
+    out = dm.field_gradient(outgoing_wavenumber(k, convention), pressure, neumann, points, precision)
+    return out[:, :, 0] if np.ndim(pressure) == 1 else out
 
 
 def beat_refine_lu(matrix, rhs, residual_matrix=None, residual_rhs=None, max_steps=3, rtol=1e-10, engine=None):

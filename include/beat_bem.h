@@ -150,6 +150,21 @@ int beat_bm_solve_device(beat_ctx* ctx, const beat_mesh* mesh, int precision, do
 int beat_field_eval(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, const double* pressure,
                     const double* q, int64_t nrhs, const double* pts, int64_t npts, double* out);
 
+/* Pressure and its x, y, z derivatives at npts points (pressure_gradient_v1):
+ * evaluate_galerkin_field_gradient_cpu / _rocm (BeatEngineFieldGradient.jl:45-92,
+ * BeatEngineRocmFieldGradient.jl:20-47), field_value_gradient (:2-13) over the
+ * field sources (with the mesh's symmetry images), plus the near-panel correction
+ * (correct_field_gradient!, :45-73): for every face image whose centroid lies
+ * within 3 diameters of a point, the FP64 adaptive integral (subdivision until
+ * diameter <= 0.4 distance and |k| diameter <= 0.7, at most 12 levels) replaces
+ * the 6-point rule. Arguments as beat_field_eval; out: npts x 4 x nrhs
+ * (pressure, d/dx, d/dy, d/dz), column-major. A point the 12 levels cannot
+ * resolve returns BEAT_ERR_ARG with the reference's message. The reference
+ * requires the 4th-order rule for gradients (BeatEngineCompiledDriver.jl:4168-4171):
+ * upload the mesh with quad_order 4. */
+int beat_field_gradient_eval(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, const double* pressure,
+                             const double* q, int64_t nrhs, const double* pts, int64_t npts, double* out);
+
 /* Dense LU with partial pivoting and solve of nrhs columns, the library routine
  * behind _beat_dense_lu_solve (BeatEngineDenseSolve.jl:1104-1110) and
  * rocm_dense_lu! (BeatEngineRocmOperators.jl:132-159). a: n x n, b and x:
