@@ -445,13 +445,9 @@ float* mesh_rec<float>(const beat_mesh* m) {
 
 // Assemble A (row-major, lda) and B (n x nrhs row-major) in precision T from the
 // internal-order q64 (and q32 for T = float).
+// Geometry, tiles, rule, wavenumber and images of the regular pass.
 template <typename T>
-int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, const double* q64, const T* qT,
-             int nrhs, T* A, int64_t lda, T* B) {
-  const int64_t n = m->h.nv;
-  HIPCHK(launch_fill<T>(A, 2 * n * lda, T(0), ctx->stream));
-  HIPCHK(launch_fill<T>(B, 2 * n * nrhs, T(0), ctx->stream));
-  RegularArgs<T> ra;
+void regular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, RegularArgs<T>& ra) {
   ra.rec = mesh_rec<T>(m);
   ra.fverts = m->fverts;
   ra.floc = m->floc;
@@ -466,15 +462,8 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   ra.stile_nv = m->stile_nv;
   ra.stile_verts = m->stile_verts;
   ra.nstiles = (int)m->h.stile_start.size();
-  ra.A = A;
-  ra.lda = lda;
-  ra.B = B;
-  ra.q = qT;
-  ra.nrhs = nrhs;
   ra.k = (T)k;
   ra.krev = (T)(k / (2.0 * M_PI));
-  ra.eta_re = (T)er;
-  ra.eta_im = (T)ei;
   ra.trig = ctx->trig;
   for (int i = 0; i < 6; ++i) {
     const bool on = i < m->h.nq;
@@ -491,12 +480,11 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
       ra.img[x][c] = on ? (T)m->h.img[x].s[c] : T(1);
       ra.img[x][3 + c] = on ? (T)(m->h.img[x].det * m->h.img[x].s[c]) : T(1);
     }
-  HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
-  const bool wide = !(std::fabs(k) * m->diameter < SINCOS_TABLE_LIMIT);
-  HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, false, ctx->stream));
-  if (m->h.nimg > 0) HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, true, ctx->stream));
-  HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
-  SingularArgs<T> sa;
+}
+
+// Pairs, rules, wavenumber and reflection table of the singular pass.
+template <typename TO>
+void singular_geometry(const beat_mesh* m, double k, SingularArgs<TO>& sa) {
   sa.rec = m->rec64;
   sa.verts = m->verts;
   sa.fverts = m->fverts;
@@ -505,20 +493,46 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   sa.rule_off = m->rule_off;
   sa.rule_cnt = m->rule_cnt;
   sa.rule_data = m->rule_data;
-  sa.A = A;
-  sa.lda = lda;
-  sa.B = B;
-  sa.q = q64;
-  sa.nrhs = nrhs;
   sa.k = k;
-  sa.eta_re = er;
-  sa.eta_im = ei;
   for (int x = 0; x <= MAX_IMAGES; ++x)
     for (int c = 0; c < 3; ++c) {
       const bool on = x >= 1 && x <= m->h.nimg;
       sa.img[x][c] = on ? m->h.img[x - 1].s[c] : 1.0;
       sa.img[x][3 + c] = on ? m->h.img[x - 1].det * m->h.img[x - 1].s[c] : 1.0;
     }
+}
+
+// Assemble A (row-major, lda) and B (n x nrhs row-major) in precision T from the
+// internal-order q64 (and q32 for T = float).
+template <typename T>
+int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, const double* q64, const T* qT,
+             int nrhs, T* A, int64_t lda, T* B) {
+  const int64_t n = m->h.nv;
+  HIPCHK(launch_fill<T>(A, 2 * n * lda, T(0), ctx->stream));
+  HIPCHK(launch_fill<T>(B, 2 * n * nrhs, T(0), ctx->stream));
+  RegularArgs<T> ra;
+  regular_geometry<T>(ctx, m, k, ra);
+  ra.A = A;
+  ra.lda = lda;
+  ra.B = B;
+  ra.q = qT;
+  ra.nrhs = nrhs;
+  ra.eta_re = (T)er;
+  ra.eta_im = (T)ei;
+  HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+  const bool wide = !(std::fabs(k) * m->diameter < SINCOS_TABLE_LIMIT);
+  HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, false, ctx->stream));
+  if (m->h.nimg > 0) HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, true, ctx->stream));
+  HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+  SingularArgs<T> sa;
+  singular_geometry<T>(m, k, sa);
+  sa.A = A;
+  sa.lda = lda;
+  sa.B = B;
+  sa.q = q64;
+  sa.nrhs = nrhs;
+  sa.eta_re = er;
+  sa.eta_im = ei;
   HIPCHK(launch_singular<T>(sa, ctx->stream));
   IdentityArgs<T> ia;
   ia.rec = m->rec64;
@@ -537,6 +551,54 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   }
   HIPCHK(launch_identity<T>(ia, ctx->stream));
   HIPCHK(launch_row_scale<T>(A, lda, n, B, nrhs, m->wrows, m->wvals, (int64_t)m->h.weighted_rows.size(), ctx->stream));
+  return BEAT_OK;
+}
+
+// The four operators S, K' (n x ldf) and D, H (n x ldd), row-major, internal
+// numbering, in precision T: regular and image passes, touching and image-touching
+// pairs, orbit row weights.
+template <typename T>
+int assemble_operators(beat_ctx* ctx, const beat_mesh* m, double k, T* S, T* D, T* K, T* H, int64_t ldd,
+                       int64_t ldf) {
+  const int64_t n = m->h.nv;
+  hipStream_t s = ctx->stream;
+  HIPCHK(launch_fill<T>(S, 2 * n * ldf, T(0), s));
+  HIPCHK(launch_fill<T>(K, 2 * n * ldf, T(0), s));
+  HIPCHK(launch_fill<T>(D, 2 * n * ldd, T(0), s));
+  HIPCHK(launch_fill<T>(H, 2 * n * ldd, T(0), s));
+  OperatorArgs<T> oa;
+  regular_geometry<T>(ctx, m, k, oa.g);
+  oa.g.A = oa.g.B = nullptr;
+  oa.g.q = nullptr;
+  oa.g.lda = 0;
+  oa.g.nrhs = 0;
+  oa.g.eta_re = oa.g.eta_im = T(0);
+  oa.S = S;
+  oa.D = D;
+  oa.K = K;
+  oa.H = H;
+  oa.ldd = ldd;
+  oa.ldf = ldf;
+  const bool wide = !(std::fabs(k) * m->diameter < SINCOS_TABLE_LIMIT);
+  HIPCHK(launch_operators<T>(oa, m->h.nq, wide, false, s));
+  if (m->h.nimg > 0) HIPCHK(launch_operators<T>(oa, m->h.nq, wide, true, s));
+  SingularOperatorArgs<T> so;
+  singular_geometry<T>(m, k, so.g);
+  so.g.A = so.g.B = nullptr;
+  so.g.q = nullptr;
+  so.g.lda = 0;
+  so.g.nrhs = 0;
+  so.g.eta_re = so.g.eta_im = 0;
+  so.S = S;
+  so.D = D;
+  so.K = K;
+  so.H = H;
+  so.ldd = ldd;
+  so.ldf = ldf;
+  HIPCHK(launch_singular_operators<T>(so, s));
+  const int64_t nw = (int64_t)m->h.weighted_rows.size();
+  HIPCHK(launch_row_scale<T>(D, ldd, n, S, (int)ldf, m->wrows, m->wvals, nw, s));
+  HIPCHK(launch_row_scale<T>(H, ldd, n, K, (int)ldf, m->wrows, m->wvals, nw, s));
   return BEAT_OK;
 }
 
@@ -1036,6 +1098,57 @@ int beat_bm_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k,
       const size_t src = 2 * ((size_t)inv[r] * nrhs + d);
       b_out[2 * (r + d * n)] = B[src];
       b_out[2 * (r + d * n) + 1] = B[src + 1];
+    }
+  return BEAT_OK;
+}
+
+int beat_operators_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double* s_out, double* d_out,
+                            double* k_out, double* h_out) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (!s_out || !d_out || !k_out || !h_out) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (precision != BEAT_PREC_FP64 && precision != BEAT_PREC_FP32) return fail(ctx, BEAT_ERR_ARG, "precision");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  const int64_t n = m->h.nv, nf = m->h.nf, ldd = lda_for(n), ldf = lda_for(nf);
+  hipStream_t s = ctx->stream;
+  const size_t elems = (size_t)2 * (2 * n * ldd + 2 * n * ldf);
+  std::vector<double> host(elems);
+  if (precision == BEAT_PREC_FP64) {
+    HIPCHK(ensure(ctx->a64, sizeof(double) * elems));
+    double* base = as<double>(ctx->a64);
+    double *S = base, *K = S + 2 * n * ldf, *D = K + 2 * n * ldf, *H = D + 2 * n * ldd;
+    if (int rc = assemble_operators<double>(ctx, m, k, S, D, K, H, ldd, ldf)) return rc;
+    HIPCHK(hipMemcpyAsync(host.data(), base, sizeof(double) * elems, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  } else {
+    HIPCHK(ensure(ctx->a32, sizeof(float) * elems));
+    float* base = as<float>(ctx->a32);
+    float *S = base, *K = S + 2 * n * ldf, *D = K + 2 * n * ldf, *H = D + 2 * n * ldd;
+    if (int rc = assemble_operators<float>(ctx, m, k, S, D, K, H, ldd, ldf)) return rc;
+    std::vector<float> h32(elems);
+    HIPCHK(hipMemcpyAsync(h32.data(), base, sizeof(float) * elems, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::copy(h32.begin(), h32.end(), host.begin());
+  }
+  const double *S = host.data(), *K = S + 2 * n * ldf, *D = K + 2 * n * ldf, *H = D + 2 * n * ldd;
+  const auto& vinv = m->h.vert_inv;
+  std::vector<int32_t> finv(nf);
+  for (int64_t i = 0; i < nf; ++i) finv[m->h.face_perm[i]] = (int32_t)i;
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t r = 0; r < n; ++r) {
+      const size_t src = 2 * ((size_t)vinv[r] * ldd + vinv[c]), dst = 2 * (size_t)(r + c * n);
+      d_out[dst] = D[src];
+      d_out[dst + 1] = D[src + 1];
+      h_out[dst] = H[src];
+      h_out[dst + 1] = H[src + 1];
+    }
+  for (int64_t f = 0; f < nf; ++f)
+    for (int64_t r = 0; r < n; ++r) {
+      const size_t src = 2 * ((size_t)vinv[r] * ldf + finv[f]), dst = 2 * (size_t)(r + f * n);
+      s_out[dst] = S[src];
+      s_out[dst + 1] = S[src + 1];
+      k_out[dst] = K[src];
+      k_out[dst + 1] = K[src + 1];
     }
   return BEAT_OK;
 }

@@ -403,6 +403,330 @@ __global__ __launch_bounds__(256) void bm_singular_kernel(const SingularArgs<TO>
   }
 }
 
+
+// ---- four separate operators (operator_matrices) -----------------------------------
+// _beat_cpu_accumulate_pair! / _beat_cpu_accumulate_regular_pair_signed!
+// (BeatEngineCpuAssembly.jl:151-218,278-373): per point pair, with G' = G (-1/r + i k),
+//   S[a, s]  += phi_a G w              K'[a, s] += phi_a G' (-r.n_x / r) w
+//   D[a, b]  += phi_a phi_b G' (r.n_y / r) w
+//   H[a, b]  += (curl_a . curl_b - k^2 phi_a phi_b (n_x . n_y)) G w
+// Same tiling as bm_regular_kernel; four LDS tiles (D, H by trial vertex slot, S, K'
+// by trial face), so one workgroup per CU in FP64.
+template <typename T, int NQ, bool WIDE, bool IMAGE>
+__global__ __launch_bounds__(256, 1) void bm_operators_kernel(const OperatorArgs<T> o) {
+  constexpr int LDA_ = 2 * TILE_VS + 1, LDC_ = 2 * TILE_FS + 1;
+  __shared__ T accD[TILE_V * LDA_];
+  __shared__ T accH[TILE_V * LDA_];
+  __shared__ T accS[TILE_V * LDC_];
+  __shared__ T accK[TILE_V * LDC_];
+  __shared__ T xs[NQ * 3 * 64];
+  __shared__ T sphw[6 * 4];
+  __shared__ double trig[2 * TRIG_N];
+  __shared__ int grow[TILE_V], gcol[TILE_VS];
+  const RegularArgs<T>& a = o.g;
+
+  const int64_t total = (int64_t)a.ntiles * a.nstiles;
+  const int64_t per = (total + 7) / 8;
+  int64_t bid = blockIdx.x;
+  T sg[6] = {T(1), T(1), T(1), T(1), T(1), T(1)};
+  if (IMAGE) {
+    const int xi = (int)(bid / (per * 8));
+    bid -= (int64_t)xi * per * 8;
+#pragma unroll
+    for (int x = 0; x < MAX_IMAGES; ++x)
+      if (x == xi)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sg[c] = a.img[x][c];
+  }
+  const int64_t L = (bid & 7) * per + (bid >> 3);
+  if (L >= total) return;
+  const int tt = (int)(L / a.nstiles), st = (int)(L % a.nstiles);
+  const int tid = threadIdx.x;
+  const int t0 = a.tile_start[tt], nT = a.tile_count[tt], nvT = a.tile_nv[tt];
+  const int s0 = a.stile_start[st], nS = a.stile_count[st], nvS = a.stile_nv[st];
+  const T* __restrict__ rec = a.rec;
+  const int32_t* __restrict__ fverts = a.fverts;
+  const uint8_t* __restrict__ flocs = a.flocs;
+
+  for (int i = tid; i < TILE_V * LDA_; i += blockDim.x) accD[i] = accH[i] = T(0);
+  for (int i = tid; i < TILE_V * LDC_; i += blockDim.x) accS[i] = accK[i] = T(0);
+  if (tid < TILE_V) grow[tid] = a.tile_verts[(int64_t)tt * TILE_V + tid];
+  if (tid < TILE_VS) gcol[tid] = a.stile_verts[(int64_t)st * TILE_VS + tid];
+  for (int i = tid; i < 2 * TRIG_N; i += blockDim.x) trig[i] = a.trig[i];
+  if (tid == 0) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      sphw[4 * i] = a.phi[i][0];
+      sphw[4 * i + 1] = a.phi[i][1];
+      sphw[4 * i + 2] = a.phi[i][2];
+      sphw[4 * i + 3] = a.w[i];
+    }
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  const bool have_test = lane < nT;
+  const int tf = t0 + (have_test ? lane : 0);
+  const T* tr = rec + (int64_t)tf * FREC;
+  if (wave == 0)
+    for (int i = 0; i < NQ * 3; ++i) xs[i * 64 + lane] = tr[i];
+  __syncthreads();
+  const T nT0 = tr[REC_N], nT1 = tr[REC_N + 1], nT2 = tr[REC_N + 2];
+  const T dT = tr[REC_D], areaT = tr[REC_AREA];
+  const int tv0 = fverts[(int64_t)tf * 3], tv1 = fverts[(int64_t)tf * 3 + 1], tv2 = fverts[(int64_t)tf * 3 + 2];
+  const int tl[3] = {a.floc[(int64_t)tf * 3], a.floc[(int64_t)tf * 3 + 1], a.floc[(int64_t)tf * 3 + 2]};
+  const T k = a.k, krev = a.krev;
+  const int wu = __builtin_amdgcn_readfirstlane(wave);
+
+  for (int sj = wu; sj < nS; sj += 4) {
+    const int64_t sf = s0 + sj;
+    const int sv0 = fverts[sf * 3], sv1 = fverts[sf * 3 + 1], sv2 = fverts[sf * 3 + 2];
+    const bool adj = (tv0 == sv0) | (tv0 == sv1) | (tv0 == sv2) | (tv1 == sv0) | (tv1 == sv1) | (tv1 == sv2) |
+                     (tv2 == sv0) | (tv2 == sv1) | (tv2 == sv2);
+    if (!have_test || (!IMAGE && adj)) continue;
+    const T* sr = rec + sf * FREC;
+    T y[NQ][3];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) y[j][c] = IMAGE ? sr[3 * j + c] * sg[c] : sr[3 * j + c];
+    const T nS0 = IMAGE ? sr[REC_N] * sg[0] : sr[REC_N];
+    const T nS1 = IMAGE ? sr[REC_N + 1] * sg[1] : sr[REC_N + 1];
+    const T nS2 = IMAGE ? sr[REC_N + 2] * sg[2] : sr[REC_N + 2];
+    const T dS = sr[REC_D];
+    const T cw = T(4) * areaT * sr[REC_AREA] * T(INV_FOUR_PI);
+    const T nprod = nT0 * nS0 + nT1 * nS1 + nT2 * nS2;
+    T PT[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) PT[j] = dT - (y[j][0] * nT0 + y[j][1] * nT1 + y[j][2] * nT2);
+    T Dr[3][3], Di[3][3], Hr[3][3], Hi[3][3], Sr[3], Si[3], Kr[3], Ki[3], csr = 0, csi = 0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      Sr[p] = Si[p] = Kr[p] = Ki[p] = 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) Dr[p][b] = Di[p][b] = Hr[p][b] = Hi[p][b] = 0;
+    }
+#pragma unroll 1
+    for (int i = 0; i < NQ; ++i) {
+      const T x0 = xs[(3 * i) * 64 + lane], x1 = xs[(3 * i + 1) * 64 + lane], x2 = xs[(3 * i + 2) * 64 + lane];
+      const T PS = dS - (x0 * nS0 + x1 * nS1 + x2 * nS2);
+      const T cwi = cw * sphw[4 * i + 3];
+      T mDr[3] = {0, 0, 0}, mDi[3] = {0, 0, 0}, mHr[3] = {0, 0, 0}, mHi[3] = {0, 0, 0};
+      T gr = 0, gi = 0, kr = 0, ki = 0;
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const T rx = y[j][0] - x0, ry = y[j][1] - x1, rz = y[j][2] - x2;
+        const T r2 = rx * rx + ry * ry + rz * rz;
+        const T ir = inv_sqrt(r2);
+        const T r = r2 * ir;
+        T s, c;
+        phase_sincos<WIDE>(r, krev, k, trig, s, c);
+        const T g = ir * (cwi * sphw[4 * j + 3]);
+        const T Gr = c * g, Gi = s * g;                            // G w
+        const T Pr = -ir * Gr - k * Gi, Pi = -ir * Gi + k * Gr;  // G' w
+        const T t = PS * ir;     // (r.n_y)/r
+        const T u = PT[j] * ir;  // -(r.n_x)/r
+        gr += Gr;
+        gi += Gi;
+        kr += Pr * u;
+        ki += Pi * u;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const T ph = sphw[4 * j + b];
+          mDr[b] += ph * (Pr * t);
+          mDi[b] += ph * (Pi * t);
+          mHr[b] += ph * Gr;
+          mHi[b] += ph * Gi;
+        }
+      }
+      csr += gr;
+      csi += gi;
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const T ph = sphw[4 * i + p];
+        Sr[p] += ph * gr;
+        Si[p] += ph * gi;
+        Kr[p] += ph * kr;
+        Ki[p] += ph * ki;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          Dr[p][b] += ph * mDr[b];
+          Di[p][b] += ph * mDi[b];
+          Hr[p][b] += ph * mHr[b];
+          Hi[p][b] += ph * mHi[b];
+        }
+      }
+    }
+    const T kn = -k * k * nprod;
+    const int sl[3] = {flocs[sf * 3], flocs[sf * 3 + 1], flocs[sf * 3 + 2]};
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const T c0 = tr[REC_CURL + 3 * p], c1 = tr[REC_CURL + 3 * p + 1], c2 = tr[REC_CURL + 3 * p + 2];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const T b0 = sr[REC_CURL + 3 * b], b1 = sr[REC_CURL + 3 * b + 1], b2 = sr[REC_CURL + 3 * b + 2];
+        const T cc = IMAGE ? c0 * (b0 * sg[3]) + c1 * (b1 * sg[4]) + c2 * (b2 * sg[5]) : c0 * b0 + c1 * b1 + c2 * b2;
+        const int o2 = tl[p] * LDA_ + 2 * sl[b];
+        lds_add(accD + o2, Dr[p][b]);
+        lds_add(accD + o2 + 1, Di[p][b]);
+        lds_add(accH + o2, cc * csr + kn * Hr[p][b]);
+        lds_add(accH + o2 + 1, cc * csi + kn * Hi[p][b]);
+      }
+      const int oc = tl[p] * LDC_ + 2 * sj;
+      lds_add(accS + oc, Sr[p]);
+      lds_add(accS + oc + 1, Si[p]);
+      lds_add(accK + oc, Kr[p]);
+      lds_add(accK + oc + 1, Ki[p]);
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < nvT * nvS; idx += blockDim.x) {
+    const int r = idx / nvS, c = idx - r * nvS;
+    const int64_t dst = 2 * ((int64_t)grow[r] * o.ldd + gcol[c]);
+    const T dr = accD[r * LDA_ + 2 * c], di = accD[r * LDA_ + 2 * c + 1];
+    if (dr != T(0) || di != T(0)) {
+      gadd(o.D + dst, dr);
+      gadd(o.D + dst + 1, di);
+    }
+    const T hr = accH[r * LDA_ + 2 * c], hi = accH[r * LDA_ + 2 * c + 1];
+    if (hr != T(0) || hi != T(0)) {
+      gadd(o.H + dst, hr);
+      gadd(o.H + dst + 1, hi);
+    }
+  }
+  for (int idx = tid; idx < nvT * nS; idx += blockDim.x) {
+    const int r = idx / nS, c = idx - r * nS;
+    const int64_t dst = 2 * ((int64_t)grow[r] * o.ldf + s0 + c);
+    const T sr_ = accS[r * LDC_ + 2 * c], si_ = accS[r * LDC_ + 2 * c + 1];
+    if (sr_ != T(0) || si_ != T(0)) {
+      gadd(o.S + dst, sr_);
+      gadd(o.S + dst + 1, si_);
+    }
+    const T kr_ = accK[r * LDC_ + 2 * c], ki_ = accK[r * LDC_ + 2 * c + 1];
+    if (kr_ != T(0) || ki_ != T(0)) {
+      gadd(o.K + dst, kr_);
+      gadd(o.K + dst + 1, ki_);
+    }
+  }
+}
+
+// Touching pairs of the four operators over the Duffy (or image delta) rules, FP64,
+// one wavefront per pair.
+template <typename TO>
+__global__ __launch_bounds__(256) void bm_singular_operators_kernel(const SingularOperatorArgs<TO> o) {
+  const SingularArgs<TO>& a = o.g;
+  const int lane = threadIdx.x & 63;
+  const int64_t pidx = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (pidx >= a.npairs) return;
+  const SingularPair P = a.pairs[pidx];
+  const double* tr = a.rec + (int64_t)P.test * FREC;
+  const double* sr = a.rec + (int64_t)P.trial * FREC;
+  double sg[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) sg[c] = a.img[0][c];
+#pragma unroll
+  for (int x = 1; x <= MAX_IMAGES; ++x)
+    if (P.xform == x)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) sg[c] = a.img[x][c];
+  double tv[3][3], sv[3][3];
+  int tvi[3], svi[3];
+  for (int v = 0; v < 3; ++v) {
+    tvi[v] = a.fverts[(int64_t)P.test * 3 + v];
+    svi[v] = a.fverts[(int64_t)P.trial * 3 + v];
+    for (int c = 0; c < 3; ++c) {
+      tv[v][c] = a.verts[3 * (int64_t)tvi[v] + c];
+      sv[v][c] = a.verts[3 * (int64_t)svi[v] + c] * sg[c];
+    }
+  }
+  const double nT[3] = {tr[REC_N], tr[REC_N + 1], tr[REC_N + 2]};
+  const double nS[3] = {sr[REC_N] * sg[0], sr[REC_N + 1] * sg[1], sr[REC_N + 2] * sg[2]};
+  const double k = a.k;
+  const int off = a.rule_off[P.rule], cnt = a.rule_cnt[P.rule];
+  // acc: D 0..17, Hphi 18..35 (sum phi phi G w), S 36..41, K 42..47, sum G w 48..49
+  double acc[50];
+#pragma unroll
+  for (int i = 0; i < 50; ++i) acc[i] = 0;
+  for (int q = lane; q < cnt; q += 64) {
+    const double* rd = a.rule_data + 5 * (int64_t)(off + q);
+    const double tx = rd[0], ty = rd[1], sx = rd[2], sy = rd[3];
+    const double ta = 1.0 - tx - ty, sa = 1.0 - sx - sy;
+    double x[3], y[3];
+    for (int c = 0; c < 3; ++c) {
+      x[c] = ta * tv[0][c] + tx * tv[1][c] + ty * tv[2][c];
+      y[c] = sa * sv[0][c] + sx * sv[1][c] + sy * sv[2][c];
+    }
+    const double rx = y[0] - x[0], ry = y[1] - x[1], rz = y[2] - x[2];
+    const double radius = sqrt(rx * rx + ry * ry + rz * rz);
+    if (radius == 0.0) continue;
+    const double ir = 1.0 / radius;
+    double s, c;
+    sincos_phase(k * radius, &s, &c);
+    const double w = rd[4] * P.jac;
+    const double Gr = c * INV_FOUR_PI * ir * w, Gi = s * INV_FOUR_PI * ir * w;  // G w
+    const double Pr = Gr * -ir - Gi * k, Pi = Gr * k + Gi * -ir;                // G' w
+    const double tdot = (rx * nS[0] + ry * nS[1] + rz * nS[2]) * ir;
+    const double sdot = -(rx * nT[0] + ry * nT[1] + rz * nT[2]) * ir;
+    const double pt[3] = {ta, tx, ty}, ps[3] = {sa, sx, sy};
+    for (int p = 0; p < 3; ++p) {
+      acc[36 + 2 * p] += pt[p] * Gr;
+      acc[37 + 2 * p] += pt[p] * Gi;
+      acc[42 + 2 * p] += pt[p] * (Pr * sdot);
+      acc[43 + 2 * p] += pt[p] * (Pi * sdot);
+      for (int b = 0; b < 3; ++b) {
+        const double bp = pt[p] * ps[b];
+        acc[2 * (3 * p + b)] += bp * (Pr * tdot);
+        acc[2 * (3 * p + b) + 1] += bp * (Pi * tdot);
+        acc[18 + 2 * (3 * p + b)] += bp * Gr;
+        acc[19 + 2 * (3 * p + b)] += bp * Gi;
+      }
+    }
+    acc[48] += Gr;
+    acc[49] += Gi;
+  }
+#pragma unroll
+  for (int i = 0; i < 50; ++i) {
+    double v = acc[i];
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    acc[i] = v;
+  }
+  const double kn = -k * k * P.nprod;
+  if (lane < 9) {
+    const int p = lane / 3, b = lane % 3;
+    const double cc = tr[REC_CURL + 3 * p] * (sr[REC_CURL + 3 * b] * sg[3]) +
+                      tr[REC_CURL + 3 * p + 1] * (sr[REC_CURL + 3 * b + 1] * sg[4]) +
+                      tr[REC_CURL + 3 * p + 2] * (sr[REC_CURL + 3 * b + 2] * sg[5]);
+    double dr = 0, di = 0, hr = 0, hi = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      if (i == lane) {
+        dr = acc[2 * i];
+        di = acc[2 * i + 1];
+        hr = cc * acc[48] + kn * acc[18 + 2 * i];
+        hi = cc * acc[49] + kn * acc[19 + 2 * i];
+      }
+    const int64_t dst = 2 * ((int64_t)tvi[p] * o.ldd + svi[b]);
+    gadd(o.D + dst, (TO)dr);
+    gadd(o.D + dst + 1, (TO)di);
+    gadd(o.H + dst, (TO)hr);
+    gadd(o.H + dst + 1, (TO)hi);
+  } else if (lane < 12) {
+    const int p = lane - 9;
+    double s_r = 0, s_i = 0, k_r = 0, k_i = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (i == p) {
+        s_r = acc[36 + 2 * i];
+        s_i = acc[37 + 2 * i];
+        k_r = acc[42 + 2 * i];
+        k_i = acc[43 + 2 * i];
+      }
+    const int64_t dst = 2 * ((int64_t)tvi[p] * o.ldf + P.trial);
+    gadd(o.S + dst, (TO)s_r);
+    gadd(o.S + dst + 1, (TO)s_i);
+    gadd(o.K + dst, (TO)k_r);
+    gadd(o.K + dst + 1, (TO)k_i);
+  }
+}
+
 // Identity blocks: A += 0.5 M_p1p1, B += -0.5 eta M_p1dp0 q (BeatEngineCpuBurtonMiller.jl:450-453).
 template <typename TO>
 __global__ void bm_identity_kernel(const IdentityArgs<TO> a) {
@@ -491,6 +815,46 @@ hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure, bool wide,
   return hipGetLastError();
 }
 
+template <typename T, int NQ, bool IMAGE>
+static void launch_operators_nq(const OperatorArgs<T>& a, bool wide, dim3 grid, dim3 block, hipStream_t s) {
+  if constexpr (sizeof(T) == 8) {
+    if (wide) {
+      hipLaunchKernelGGL((bm_operators_kernel<T, NQ, true, IMAGE>), grid, block, 0, s, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((bm_operators_kernel<T, NQ, false, IMAGE>), grid, block, 0, s, a);
+}
+
+template <typename T>
+hipError_t launch_operators(const OperatorArgs<T>& a, int nq, bool wide, bool image, hipStream_t s) {
+  const int64_t total = (int64_t)a.g.ntiles * a.g.nstiles;
+  const int64_t per = (total + 7) / 8;
+  if (image && (a.g.nimg < 1 || a.g.nimg > MAX_IMAGES)) return hipErrorInvalidValue;
+  const int64_t blocks = per * 8 * (image ? a.g.nimg : 1);
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  dim3 grid((unsigned)blocks), block(256);
+  if (nq == 6) {
+    if (image) launch_operators_nq<T, 6, true>(a, wide, grid, block, s);
+    else launch_operators_nq<T, 6, false>(a, wide, grid, block, s);
+  } else if (nq == 3) {
+    if (image) launch_operators_nq<T, 3, true>(a, wide, grid, block, s);
+    else launch_operators_nq<T, 3, false>(a, wide, grid, block, s);
+  } else {
+    if (image) launch_operators_nq<T, 1, true>(a, wide, grid, block, s);
+    else launch_operators_nq<T, 1, false>(a, wide, grid, block, s);
+  }
+  return hipGetLastError();
+}
+
+template <typename TO>
+hipError_t launch_singular_operators(const SingularOperatorArgs<TO>& a, hipStream_t s) {
+  if (a.g.npairs == 0) return hipSuccess;
+  dim3 grid((unsigned)((a.g.npairs + 3) / 4)), block(256);
+  hipLaunchKernelGGL((bm_singular_operators_kernel<TO>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
 template <typename TO>
 hipError_t launch_singular(const SingularArgs<TO>& a, hipStream_t s) {
   if (a.npairs == 0) return hipSuccess;
@@ -535,6 +899,10 @@ template hipError_t launch_row_scale<float>(float*, int64_t, int64_t, float*, in
                                             int64_t, hipStream_t);
 template hipError_t launch_row_scale<double>(double*, int64_t, int64_t, double*, int, const int32_t*, const double*,
                                              int64_t, hipStream_t);
+template hipError_t launch_operators<float>(const OperatorArgs<float>&, int, bool, bool, hipStream_t);
+template hipError_t launch_operators<double>(const OperatorArgs<double>&, int, bool, bool, hipStream_t);
+template hipError_t launch_singular_operators<float>(const SingularOperatorArgs<float>&, hipStream_t);
+template hipError_t launch_singular_operators<double>(const SingularOperatorArgs<double>&, hipStream_t);
 template hipError_t launch_singular<float>(const SingularArgs<float>&, hipStream_t);
 template hipError_t launch_singular<double>(const SingularArgs<double>&, hipStream_t);
 template hipError_t launch_identity<float>(const IdentityArgs<float>&, hipStream_t);

@@ -60,6 +60,23 @@ struct SingularArgs {
   double img[MAX_IMAGES + 1][6];
 };
 
+// Four separate operators (operator_matrices): S, K' (n x ldf, P1 rows x DP0
+// columns) and D, H (n x ldd), row-major complex, internal numbering. g carries
+// the tiles, geometry, rule, wavenumber and images of the regular pass (its
+// A, B, q and eta are unused); sg those of the singular pass.
+template <typename T>
+struct OperatorArgs {
+  RegularArgs<T> g;
+  T *S, *D, *K, *H;
+  int64_t ldd, ldf;
+};
+template <typename TO>
+struct SingularOperatorArgs {
+  SingularArgs<TO> g;
+  TO *S, *D, *K, *H;
+  int64_t ldd, ldf;
+};
+
 template <typename TO>
 struct IdentityArgs {
   const double* rec;
@@ -83,6 +100,10 @@ template <typename T>
 hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure_imag_eta, bool wide, bool image, hipStream_t s);
 template <typename TO>
 hipError_t launch_singular(const SingularArgs<TO>& a, hipStream_t s);
+template <typename T>
+hipError_t launch_operators(const OperatorArgs<T>& a, int nq, bool wide, bool image, hipStream_t s);
+template <typename TO>
+hipError_t launch_singular_operators(const SingularOperatorArgs<TO>& a, hipStream_t s);
 template <typename TO>
 hipError_t launch_identity(const IdentityArgs<TO>& a, hipStream_t s);
 // rows[i] of A (n columns, lda) and B (nrhs columns) times w[i]

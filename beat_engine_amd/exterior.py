@@ -7,6 +7,10 @@ gfx950 library.
                                    <- assemble_burton_miller_neumann_system_cpu
                                       (src/BeatEngineCpuBurtonMiller.jl:276-469)
   evaluate_galerkin_field_hip      <- evaluate_galerkin_field_cpu (src/BeatEngineCpuField.jl:1-49)
+  assemble_regular_galerkin_operators_hip
+                                   <- assemble_regular_galerkin_operators_cpu / _rocm
+                                      (src/BeatEngineCpuAssembly.jl:739-1031,
+                                       src/BeatEngineRocmAssembly.jl:16-161)
   evaluate_galerkin_field_gradient_hip
                                    <- evaluate_galerkin_field_gradient_cpu / _rocm
                                       (src/BeatEngineFieldGradient.jl:45-92)
@@ -189,6 +193,18 @@ class DeviceMesh:
         self.engine._check(rc, "beat_bm_assemble")
         return A, B
 
+    def operators(self, k, precision="float64"):
+        """(S, D, K', H): nv x nf, nv x nv, nv x nf, nv x nv complex128."""
+        n, f = self.num_vertices, self.num_faces
+        S = np.zeros((n, f), dtype=np.complex128, order="F")
+        K = np.zeros((n, f), dtype=np.complex128, order="F")
+        D = np.zeros((n, n), dtype=np.complex128, order="F")
+        H = np.zeros((n, n), dtype=np.complex128, order="F")
+        rc = self.engine._lib.beat_operators_assemble(self.engine._h, self._h, _precision(precision), float(k), _ptr(S),
+                                                      _ptr(D), _ptr(K), _ptr(H))
+        self.engine._check(rc, "beat_operators_assemble")
+        return S, D, K, H
+
     def solve(self, k, coupling, q, precision="float32_refined", max_steps=3, rtol=1e-10):
         qq = self._q(q)
         n, nrhs = self.num_vertices, qq.shape[1]
@@ -270,6 +286,20 @@ def assemble_burton_miller_neumann_system_hip(mesh: BoundaryMesh, q_neumann, k, 
     ks = outgoing_wavenumber(k, convention)
     A, B = dm.assemble(ks, burton_miller_coupling(ks, convention), q_neumann, precision)
     return dict(matrix=A, rhs=B, assembly_mode="hip_fused_burton_miller", on_gpu=True, **dm.counts())
+
+
+def assemble_regular_galerkin_operators_hip(mesh: BoundaryMesh, k, rule=4, *, singular_order=4, precision="float64",
+                                            symmetry_mode="off", convention=NEGATIVE_TIME_PHASOR, engine=None):
+    """The operators of the reference's operator_matrices mode as a named tuple-like
+    dict (single_layer, double_layer, adjoint_double_layer, hypersingular) plus the
+    pair counts; k is normalised to the outgoing wavenumber."""
+    eng = engine or default_engine()
+    dm = eng.upload(mesh, _rule_order(rule), singular_order, symmetry_mode)
+    S, D, K, H = dm.operators(outgoing_wavenumber(k, convention), precision)
+    c = dm.counts()
+    return dict(single_layer=S, double_layer=D, adjoint_double_layer=K, hypersingular=H,
+                regular_pairs=c["regular_pairs"], singular_pairs=c["coincident"] + c["edge"] + c["vertex"],
+                image_singular_pairs=c["image_singular_pairs"], on_gpu=True, gpu_backend="rocm")
 
 
 def solve_exterior_direct_hip(mesh: BoundaryMesh, p1_space, dp0_space, neumann_values, wavenumber, rule=4, *,

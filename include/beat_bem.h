@@ -129,6 +129,17 @@ int beat_mesh_symmetry(const beat_mesh* mesh, int64_t* info);
 int beat_bm_assemble(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, double eta_re, double eta_im,
                      const double* q, int64_t nrhs, double* a_out, double* b_out);
 
+/* The four Galerkin operators separately (operator_matrices): S and K' (nv x nf,
+ * P1 rows, DP0 columns), D and H (nv x nv), complex128 column-major in the
+ * caller's numbering, with the mesh's symmetry images and orbit row weights.
+ * Replaces assemble_regular_galerkin_operators_cpu with the singular pass
+ * (BeatEngineCpuAssembly.jl:739-1031) and the ROCm native operator assembly
+ * (_assemble_regular_galerkin_operators_rocm_native, BeatEngineRocmAssembly.jl:16-161;
+ * its optional near-field corrections are not part of this entry point).
+ * precision: BEAT_PREC_FP64 or BEAT_PREC_FP32 (the singular pass is FP64). */
+int beat_operators_assemble(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, double* s_out,
+                            double* d_out, double* kp_out, double* h_out);
+
 /* Assemble, factor once and solve every right-hand side: the body of
  * solve_exterior_direct_cuda (BeatEngineCompiledDriver.jl:870-915) and
  * solve_burton_miller_neumann_system_cpu_with_report (BeatEngineCpuBurtonMiller.jl:490-508).
