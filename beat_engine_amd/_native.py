@@ -22,7 +22,8 @@ EXPORTS = (
     "beat_ctx_create", "beat_ctx_destroy", "beat_last_error", "beat_mesh_upload", "beat_mesh_destroy",
     "beat_mesh_counts", "beat_bm_assemble", "beat_bm_solve", "beat_bm_solve_device", "beat_field_eval",
     "beat_dense_lu_solve", "beat_refine_lu", "beat_mesh_upload_symmetric", "beat_mesh_symmetry",
-    "beat_field_gradient_eval", "beat_operators_assemble",
+    "beat_field_gradient_eval", "beat_operators_assemble", "beat_bm_sweep_device",
+    "beat_bm_sweep",
 )
 
 SYMMETRY_CODES = {"off": 0, "x": 1, "xy": 2, "ground": 3}
@@ -96,6 +97,8 @@ def load(path: str = LIB_PATH):
     L.beat_field_eval.argtypes = [P, P, I, D, P, P, I64, P, I64, P]
     L.beat_field_gradient_eval.argtypes = [P, P, I, D, P, P, I64, P, I64, P]
     L.beat_operators_assemble.argtypes = [P, P, I, D, P, P, P, P]
+    L.beat_bm_sweep_device.argtypes = [P, P, I, I64, P, P, P, P, I64, P, ctypes.POINTER(SolveOptions), P]
+    L.beat_bm_sweep.argtypes = [P, P, I, I64, P, P, P, P, I64, P, ctypes.POINTER(SolveOptions), P]
     L.beat_dense_lu_solve.argtypes = [P, I, P, I64, P, I64, P]
     L.beat_refine_lu.argtypes = [P, P, I64, P, I64, P, P, I, D, P, ctypes.POINTER(RefineReport)]
     if path == LIB_PATH:

@@ -60,7 +60,16 @@ struct beat_ctx {
   std::mutex mu;
   DevBuf a64, a32, b64, b32, q64, q32, x64, r64, n64, rn64, xn64, d32, work, chunk, norms, ipiv, lusmall, qin, pout,
       fld_src, fld_out, fld_pts, grad;
-  hipEvent_t ev[6] = {};
+  hipEvent_t ev[7] = {};
+  // frequency sweeps: the assembly stream, two systems in flight, their events
+  // (assembly start, regular start, regular end, assembly end) and release events
+  hipStream_t asm_stream = nullptr;
+  DevBuf sw_q64[2], sw_q32[2], sw_a64[2], sw_b64[2], sw_a32[2], sw_b32[2];
+  hipEvent_t sw_ev[2][4] = {};
+  hipEvent_t sw_free[2] = {};
+  // the regular kernel at one workgroup per CU while it shares the device with
+  // an LU (BEAT_SWEEP_ASM_ONE_PER_CU=0 lets it take every slot)
+  bool sweep_asm_one_per_cu = true;
   double* trig = nullptr;  // (sin, cos)(m pi/64) table of the FP64 phase
 };
 
@@ -480,6 +489,7 @@ void regular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, Regular
       ra.img[x][c] = on ? (T)m->h.img[x].s[c] : T(1);
       ra.img[x][3 + c] = on ? (T)(m->h.img[x].det * m->h.img[x].s[c]) : T(1);
     }
+  ra.one_per_cu = 0;
 }
 
 // Pairs, rules, wavenumber and reflection table of the singular pass.
@@ -504,12 +514,15 @@ void singular_geometry(const beat_mesh* m, double k, SingularArgs<TO>& sa) {
 
 // Assemble A (row-major, lda) and B (n x nrhs row-major) in precision T from the
 // internal-order q64 (and q32 for T = float).
+// st: the stream; ev_reg0 / ev_reg1 (may be null) bracket the regular passes;
+// one_per_cu: run the regular kernel at one workgroup per CU.
 template <typename T>
 int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, const double* q64, const T* qT,
-             int nrhs, T* A, int64_t lda, T* B) {
+             int nrhs, T* A, int64_t lda, T* B, hipStream_t st, hipEvent_t ev_reg0, hipEvent_t ev_reg1,
+             bool one_per_cu = false) {
   const int64_t n = m->h.nv;
-  HIPCHK(launch_fill<T>(A, 2 * n * lda, T(0), ctx->stream));
-  HIPCHK(launch_fill<T>(B, 2 * n * nrhs, T(0), ctx->stream));
+  HIPCHK(launch_fill<T>(A, 2 * n * lda, T(0), st));
+  HIPCHK(launch_fill<T>(B, 2 * n * nrhs, T(0), st));
   RegularArgs<T> ra;
   regular_geometry<T>(ctx, m, k, ra);
   ra.A = A;
@@ -519,11 +532,12 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   ra.nrhs = nrhs;
   ra.eta_re = (T)er;
   ra.eta_im = (T)ei;
-  HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+  ra.one_per_cu = one_per_cu ? 1 : 0;
+  if (ev_reg0) HIPCHK(hipEventRecord(ev_reg0, st));
   const bool wide = !(std::fabs(k) * m->diameter < SINCOS_TABLE_LIMIT);
-  HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, false, ctx->stream));
-  if (m->h.nimg > 0) HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, true, ctx->stream));
-  HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+  HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, false, st));
+  if (m->h.nimg > 0) HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, true, st));
+  if (ev_reg1) HIPCHK(hipEventRecord(ev_reg1, st));
   SingularArgs<T> sa;
   singular_geometry<T>(m, k, sa);
   sa.A = A;
@@ -533,7 +547,7 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   sa.nrhs = nrhs;
   sa.eta_re = er;
   sa.eta_im = ei;
-  HIPCHK(launch_singular<T>(sa, ctx->stream));
+  HIPCHK(launch_singular<T>(sa, st));
   IdentityArgs<T> ia;
   ia.rec = m->rec64;
   ia.fverts = m->fverts;
@@ -549,8 +563,8 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
     ia.pd[p] = m->h.id_pd[p];
     for (int b = 0; b < 3; ++b) ia.pp[p][b] = m->h.id_pp[p][b];
   }
-  HIPCHK(launch_identity<T>(ia, ctx->stream));
-  HIPCHK(launch_row_scale<T>(A, lda, n, B, nrhs, m->wrows, m->wvals, (int64_t)m->h.weighted_rows.size(), ctx->stream));
+  HIPCHK(launch_identity<T>(ia, st));
+  HIPCHK(launch_row_scale<T>(A, lda, n, B, nrhs, m->wrows, m->wvals, (int64_t)m->h.weighted_rows.size(), st));
   return BEAT_OK;
 }
 
@@ -608,96 +622,113 @@ int check_mesh(beat_ctx* ctx, const beat_mesh* m) {
   return BEAT_OK;
 }
 
-// q_dev: caller column-major complex128 (device); p_dev: caller column-major out.
-int solve_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei, const double* q_dev,
-               int nrhs, double* p_dev, const beat_solve_options* opts, beat_solve_report* rep) {
-  const int64_t n = m->h.nv, nf = m->h.nf;
+// Device buffers of one system in flight: q (internal order), the assembled
+// matrix and right-hand sides. a32 / b32 hold the complex64 system of the FP32
+// path, or the complex64 copy factored by the refined path.
+struct SystemBufs {
+  DevBuf *q64, *q32, *a64, *b64, *a32, *b32;
+};
+
+// Timing events of one system: assembly start, regular passes, assembly end.
+struct AsmEvents {
+  hipEvent_t start, reg0, reg1, end;
+};
+
+// Stage 1 (stream st): gather q_dev (caller column-major complex128) into internal
+// order and assemble the system of the precision: complex128 for FP64 and the
+// refined path, complex64 for FP32.
+int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei,
+                   const double* q_dev, int nrhs, SystemBufs& sb, hipStream_t st, const AsmEvents& ev,
+                   bool one_per_cu = false) {
+  const int64_t n = m->h.nv, nf = m->h.nf, lda = lda_for(n);
+  HIPCHK(hipEventRecord(ev.start, st));
+  HIPCHK(ensure(*sb.q64, sizeof(double) * 2 * nf * nrhs));
+  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, st, q_dev, nf, nrhs,
+                     (const int32_t*)m->face_perm, as<double>(*sb.q64));
+  HIPCHK(hipGetLastError());
+  if (precision == BEAT_PREC_FP32) {
+    HIPCHK(ensure(*sb.q32, sizeof(float) * 2 * nf * nrhs));
+    HIPCHK(launch_downcast(as<double>(*sb.q64), as<float>(*sb.q32), 2 * nf * nrhs, st));
+    HIPCHK(ensure(*sb.a32, sizeof(float) * 2 * n * lda));
+    HIPCHK(ensure(*sb.b32, sizeof(float) * 2 * n * nrhs));
+    if (int rc = assemble<float>(ctx, m, k, er, ei, as<double>(*sb.q64), as<float>(*sb.q32), nrhs, as<float>(*sb.a32),
+                                 lda, as<float>(*sb.b32), st, ev.reg0, ev.reg1, one_per_cu))
+      return rc;
+  } else {
+    HIPCHK(ensure(*sb.a64, sizeof(double) * 2 * n * lda));
+    HIPCHK(ensure(*sb.b64, sizeof(double) * 2 * n * nrhs));
+    if (int rc = assemble<double>(ctx, m, k, er, ei, as<double>(*sb.q64), as<double>(*sb.q64), nrhs,
+                                  as<double>(*sb.a64), lda, as<double>(*sb.b64), st, ev.reg0, ev.reg1, one_per_cu))
+      return rc;
+  }
+  HIPCHK(hipEventRecord(ev.end, st));
+  return BEAT_OK;
+}
+
+// Stage 2 (context stream, after the assembly): factor once, solve every column
+// (and refine), write the pressures (caller column-major) to p_dev, then wait for
+// the stream and report. The assembled buffers of sb are consumed.
+int stage_solve(beat_ctx* ctx, const beat_mesh* m, int precision, int nrhs, SystemBufs& sb, double* p_dev,
+                const beat_solve_options* opts, const AsmEvents& asm_ev, beat_solve_report* rep) {
+  const int64_t n = m->h.nv;
   const int64_t lda = lda_for(n);
   hipStream_t s = ctx->stream;
   int max_steps = opts ? opts->max_steps : 3;
   double rtol = opts ? opts->rtol : 1e-10;
-  if (max_steps < 1 || max_steps > 8) return fail(ctx, BEAT_ERR_ARG, "max_steps must be in 1..8");
-  if (!(rtol > 0) || !std::isfinite(rtol)) return fail(ctx, BEAT_ERR_ARG, "rtol must be finite and positive");
-  HIPCHK(ensure(ctx->q64, sizeof(double) * 2 * nf * nrhs));
-  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, s, q_dev, nf, nrhs,
-                     (const int32_t*)m->face_perm, as<double>(ctx->q64));
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(ctx->ev[0], s));
+  HIPCHK(hipStreamWaitEvent(s, asm_ev.end, 0));
   LuWork lw;
   if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
   int panels = 0;
   Refine R;
+  HIPCHK(hipEventRecord(ctx->ev[1], s));
   if (precision == BEAT_PREC_FP64) {
-    HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
-    HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n * nrhs));
-    int rc = assemble<double>(ctx, m, k, er, ei, as<double>(ctx->q64), as<double>(ctx->q64), nrhs, as<double>(ctx->a64),
-                              lda, as<double>(ctx->b64));
-    if (rc) return rc;
-    HIPCHK(hipEventRecord(ctx->ev[1], s));
-    HIPCHK(lu_factor<double>(as<double>(ctx->a64), n, lda, lw, LU_NB, s, &panels));
+    HIPCHK(lu_factor<double>(as<double>(*sb.a64), n, lda, lw, LU_NB, s, &panels));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    rc = solve_all<double>(ctx, as<double>(ctx->a64), n, lda, lw.ipiv, as<double>(ctx->b64), nrhs, true);
+    int rc = solve_all<double>(ctx, as<double>(*sb.a64), n, lda, lw.ipiv, as<double>(*sb.b64), nrhs, true);
     if (rc) return rc;
     hipLaunchKernelGGL((scatter_p_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
-                       (const double*)as<double>(ctx->b64), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
+                       (const double*)as<double>(*sb.b64), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
     HIPCHK(hipGetLastError());
   } else if (precision == BEAT_PREC_FP32) {
-    HIPCHK(ensure(ctx->q32, sizeof(float) * 2 * nf * nrhs));
-    HIPCHK(launch_downcast(as<double>(ctx->q64), as<float>(ctx->q32), 2 * nf * nrhs, s));
-    HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
-    HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * n * nrhs));
-    int rc = assemble<float>(ctx, m, k, er, ei, as<double>(ctx->q64), as<float>(ctx->q32), nrhs, as<float>(ctx->a32),
-                             lda, as<float>(ctx->b32));
-    if (rc) return rc;
-    HIPCHK(hipEventRecord(ctx->ev[1], s));
-    HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
+    HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), nrhs, true);
+    int rc = solve_all<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs, true);
     if (rc) return rc;
     hipLaunchKernelGGL((scatter_p_kernel<float>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
-                       (const float*)as<float>(ctx->b32), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
+                       (const float*)as<float>(*sb.b32), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
     HIPCHK(hipGetLastError());
-  } else if (precision == BEAT_PREC_FP32_REFINED) {
-    HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
-    HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n * nrhs));
-    HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
-    HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * n * nrhs));
+  } else {
+    HIPCHK(ensure(*sb.a32, sizeof(float) * 2 * n * lda));
+    HIPCHK(ensure(*sb.b32, sizeof(float) * 2 * n * nrhs));
     HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * n * nrhs));
-    int rc = assemble<double>(ctx, m, k, er, ei, as<double>(ctx->q64), as<double>(ctx->q64), nrhs, as<double>(ctx->a64),
-                              lda, as<double>(ctx->b64));
-    if (rc) return rc;
-    HIPCHK(hipEventRecord(ctx->ev[1], s));
-    HIPCHK(launch_downcast(as<double>(ctx->a64), as<float>(ctx->a32), 2 * n * lda, s));
-    HIPCHK(launch_downcast(as<double>(ctx->b64), as<float>(ctx->b32), 2 * n * nrhs, s));
-    HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
+    HIPCHK(launch_downcast(as<double>(*sb.a64), as<float>(*sb.a32), 2 * n * lda, s));
+    HIPCHK(launch_downcast(as<double>(*sb.b64), as<float>(*sb.b32), 2 * n * nrhs, s));
+    HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    rc = solve_all<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, as<float>(ctx->b32), nrhs, true);
+    int rc = solve_all<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs, true);
     if (rc) return rc;
-    hipLaunchKernelGGL(upcast_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, s, (const float*)as<float>(ctx->b32),
+    hipLaunchKernelGGL(upcast_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, s, (const float*)as<float>(*sb.b32),
                        as<double>(ctx->x64), 2 * n * nrhs);
     HIPCHK(hipGetLastError());
-    rc = refine_loop(ctx, as<float>(ctx->a32), lda, lw.ipiv, as<double>(ctx->a64), lda, as<double>(ctx->b64),
+    rc = refine_loop(ctx, as<float>(*sb.a32), lda, lw.ipiv, as<double>(*sb.a64), lda, as<double>(*sb.b64),
                      as<double>(ctx->x64), n, nrhs, max_steps, rtol, false, R);
     if (rc) return rc;
     hipLaunchKernelGGL((scatter_p_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
                        (const double*)as<double>(ctx->x64), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
     HIPCHK(hipGetLastError());
-  } else {
-    return fail(ctx, BEAT_ERR_ARG, "unknown precision");
   }
   HIPCHK(hipEventRecord(ctx->ev[3], s));
   HIPCHK(hipEventSynchronize(ctx->ev[3]));
   int info = 0;
   HIPCHK(hipMemcpy(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost));
   if (rep) {
-    float t01, t12, t23;
-    HIPCHK(hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]));
+    float ta, tr, t12, t23;
+    HIPCHK(hipEventElapsedTime(&ta, asm_ev.start, asm_ev.end));
+    HIPCHK(hipEventElapsedTime(&tr, asm_ev.reg0, asm_ev.reg1));
     HIPCHK(hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]));
     HIPCHK(hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]));
-    float t45;
-    HIPCHK(hipEventElapsedTime(&t45, ctx->ev[4], ctx->ev[5]));
-    rep->regular_kernel_s = t45 * 1e-3;
-    rep->assembly_s = t01 * 1e-3;
+    rep->regular_kernel_s = tr * 1e-3;
+    rep->assembly_s = ta * 1e-3;
     rep->factor_s = t12 * 1e-3;
     rep->solve_s = t23 * 1e-3;
     rep->factorization_count = 1;
@@ -716,6 +747,70 @@ int solve_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, doubl
   }
   if (info != 0) return fail(ctx, BEAT_ERR_SINGULAR, "LU found an exactly zero pivot at column " + std::to_string(info));
   return BEAT_OK;
+}
+
+int check_solve_options(beat_ctx* ctx, int precision, const beat_solve_options* opts) {
+  const int max_steps = opts ? opts->max_steps : 3;
+  const double rtol = opts ? opts->rtol : 1e-10;
+  if (max_steps < 1 || max_steps > 8) return fail(ctx, BEAT_ERR_ARG, "max_steps must be in 1..8");
+  if (!(rtol > 0) || !std::isfinite(rtol)) return fail(ctx, BEAT_ERR_ARG, "rtol must be finite and positive");
+  if (precision != BEAT_PREC_FP64 && precision != BEAT_PREC_FP32 && precision != BEAT_PREC_FP32_REFINED)
+    return fail(ctx, BEAT_ERR_ARG, "unknown precision");
+  return BEAT_OK;
+}
+
+// q_dev: caller column-major complex128 (device); p_dev: caller column-major out.
+int solve_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei, const double* q_dev,
+               int nrhs, double* p_dev, const beat_solve_options* opts, beat_solve_report* rep) {
+  if (int rc = check_solve_options(ctx, precision, opts)) return rc;
+  SystemBufs sb{&ctx->q64, &ctx->q32, &ctx->a64, &ctx->b64, &ctx->a32, &ctx->b32};
+  const AsmEvents ev{ctx->ev[0], ctx->ev[4], ctx->ev[5], ctx->ev[6]};
+  if (int rc = stage_assemble(ctx, m, precision, k, er, ei, q_dev, nrhs, sb, ctx->stream, ev)) return rc;
+  return stage_solve(ctx, m, precision, nrhs, sb, p_dev, opts, ev, rep);
+}
+
+// Frequency sweep with two systems in flight: the assembly of frequency i + 1 runs
+// on the assembly stream while frequency i is factored and solved on the context
+// stream; a system's buffers are reassembled only after its solve has finished.
+int sweep_impl(beat_ctx* ctx, const beat_mesh* m, int precision, int64_t nfreq, const double* ks, const double* ers,
+               const double* eis, const double* q_dev, int nrhs, double* p_dev, const beat_solve_options* opts,
+               beat_solve_report* reps) {
+  if (int rc = check_solve_options(ctx, precision, opts)) return rc;
+  const int64_t n = m->h.nv, nf = m->h.nf;
+  SystemBufs sb[2];
+  for (int s2 = 0; s2 < 2; ++s2) {
+    sb[s2] = SystemBufs{&ctx->sw_q64[s2], &ctx->sw_q32[s2], &ctx->sw_a64[s2], &ctx->sw_b64[s2],
+                        precision == BEAT_PREC_FP32 ? &ctx->sw_a32[s2] : &ctx->a32,
+                        precision == BEAT_PREC_FP32 ? &ctx->sw_b32[s2] : &ctx->b32};
+  }
+  AsmEvents ev[2];
+  for (int s2 = 0; s2 < 2; ++s2)
+    ev[s2] = AsmEvents{ctx->sw_ev[s2][0], ctx->sw_ev[s2][1], ctx->sw_ev[s2][2], ctx->sw_ev[s2][3]};
+  auto enqueue = [&](int64_t i) -> int {
+    const int s2 = (int)(i & 1);
+    // the system's previous solve (frequency i - 2) must be done with its buffers
+    if (i >= 2) HIPCHK(hipStreamWaitEvent(ctx->asm_stream, ctx->sw_free[s2], 0));
+    return stage_assemble(ctx, m, precision, ks[i], ers[i], eis[i], q_dev + 2 * i * nf * nrhs, nrhs, sb[s2],
+                          ctx->asm_stream, ev[s2], ctx->sweep_asm_one_per_cu);
+  };
+  if (nfreq <= 0) return BEAT_OK;
+  if (int rc = enqueue(0)) return rc;
+  int result = BEAT_OK;
+  for (int64_t i = 0; i < nfreq; ++i) {
+    const int s2 = (int)(i & 1);
+    if (i + 1 < nfreq)
+      if (int rc = enqueue(i + 1)) return rc;
+    int rc = stage_solve(ctx, m, precision, nrhs, sb[s2], p_dev + 2 * i * n * nrhs, opts, ev[s2],
+                         reps ? reps + i : nullptr);
+    if (rc != BEAT_OK && rc != BEAT_ERR_SINGULAR) {
+      (void)hipStreamSynchronize(ctx->asm_stream);
+      return rc;
+    }
+    if (rc == BEAT_ERR_SINGULAR) result = rc;
+    HIPCHK(hipEventRecord(ctx->sw_free[s2], ctx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->asm_stream));
+  return result;
 }
 
 template <typename T>
@@ -882,6 +977,21 @@ int beat_ctx_create(int device, beat_ctx** out) {
       delete c;
       return BEAT_ERR_HIP;
     }
+  for (auto& row : c->sw_ev)
+    for (auto& e : row)
+      if (hipEventCreate(&e) != hipSuccess) {
+        delete c;
+        return BEAT_ERR_HIP;
+      }
+  for (auto& e : c->sw_free)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      delete c;
+      return BEAT_ERR_HIP;
+    }
+  if (hipStreamCreateWithFlags(&c->asm_stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return BEAT_ERR_HIP;
+  }
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fact, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_far, hipEventDisableTiming) != hipSuccess ||
@@ -900,6 +1010,7 @@ int beat_ctx_create(int device, beat_ctx** out) {
       return BEAT_ERR_HIP;
     }
   }
+  if (const char* v = std::getenv("BEAT_SWEEP_ASM_ONE_PER_CU")) c->sweep_asm_one_per_cu = v[0] != '0';
   // BEAT_LU_LOOKAHEAD=0 runs the whole factorization in order on one stream
   if (const char* v = std::getenv("BEAT_LU_LOOKAHEAD"))
     if (v[0] == '0') {
@@ -921,6 +1032,16 @@ void beat_ctx_destroy(beat_ctx* c) {
                     &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
+  if (c->asm_stream) (void)hipStreamSynchronize(c->asm_stream);
+  for (int s2 = 0; s2 < 2; ++s2) {
+    DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2], &c->sw_b32[s2]};
+    for (DevBuf* b : sw)
+      if (b->p) (void)hipFree(b->p);
+    for (auto& e : c->sw_ev[s2])
+      if (e) (void)hipEventDestroy(e);
+    if (c->sw_free[s2]) (void)hipEventDestroy(c->sw_free[s2]);
+  }
+  if (c->asm_stream) (void)hipStreamDestroy(c->asm_stream);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->side) (void)hipStreamDestroy(c->side);
@@ -1066,7 +1187,7 @@ int beat_bm_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k,
     HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
     HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n * nrhs));
     if (int rc = assemble<double>(ctx, m, k, er, ei, as<double>(ctx->q64), as<double>(ctx->q64), (int)nrhs,
-                                  as<double>(ctx->a64), lda, as<double>(ctx->b64)))
+                                  as<double>(ctx->a64), lda, as<double>(ctx->b64), s, nullptr, nullptr))
     return rc;
     HIPCHK(hipMemcpyAsync(A.data(), ctx->a64.p, sizeof(double) * A.size(), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(B.data(), ctx->b64.p, sizeof(double) * B.size(), hipMemcpyDeviceToHost, s));
@@ -1076,7 +1197,7 @@ int beat_bm_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k,
     HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
     HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * n * nrhs));
     if (int rc = assemble<float>(ctx, m, k, er, ei, as<double>(ctx->q64), as<float>(ctx->q32), (int)nrhs,
-                                 as<float>(ctx->a32), lda, as<float>(ctx->b32)))
+                                 as<float>(ctx->a32), lda, as<float>(ctx->b32), s, nullptr, nullptr))
     return rc;
     std::vector<float> A32(A.size()), B32(B.size());
     HIPCHK(hipMemcpyAsync(A32.data(), ctx->a32.p, sizeof(float) * A32.size(), hipMemcpyDeviceToHost, s));
@@ -1178,6 +1299,41 @@ int beat_bm_solve_device(beat_ctx* ctx, const beat_mesh* m, int precision, doubl
   std::lock_guard<std::mutex> lock(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   return solve_impl(ctx, m, precision, k, er, ei, (const double*)q_dev, (int)nrhs, (double*)p_dev, opts, rep);
+}
+
+int beat_bm_sweep_device(beat_ctx* ctx, const beat_mesh* m, int precision, int64_t nfreq, const double* k,
+                         const double* eta_re, const double* eta_im, const void* q_dev, int64_t nrhs, void* p_dev,
+                         const beat_solve_options* opts, beat_solve_report* reps) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (nfreq < 0 || (nfreq > 0 && (!k || !eta_re || !eta_im || !q_dev || !p_dev)) || nrhs < 1 || nrhs > 4096)
+    return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  return sweep_impl(ctx, m, precision, nfreq, k, eta_re, eta_im, (const double*)q_dev, (int)nrhs, (double*)p_dev, opts,
+                    reps);
+}
+
+int beat_bm_sweep(beat_ctx* ctx, const beat_mesh* m, int precision, int64_t nfreq, const double* k,
+                  const double* eta_re, const double* eta_im, const double* q, int64_t nrhs, double* pressure_out,
+                  const beat_solve_options* opts, beat_solve_report* reps) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (nfreq < 0 || (nfreq > 0 && (!k || !eta_re || !eta_im || !q || !pressure_out)) || nrhs < 1 || nrhs > 4096)
+    return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (nfreq == 0) return BEAT_OK;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  const int64_t n = m->h.nv, nf = m->h.nf;
+  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * nf * nrhs * nfreq));
+  HIPCHK(ensure(ctx->pout, sizeof(double) * 2 * n * nrhs * nfreq));
+  HIPCHK(hipMemcpyAsync(ctx->qin.p, q, sizeof(double) * 2 * nf * nrhs * nfreq, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int rc = sweep_impl(ctx, m, precision, nfreq, k, eta_re, eta_im, as<double>(ctx->qin), (int)nrhs,
+                      as<double>(ctx->pout), opts, reps);
+  if (rc != BEAT_OK && rc != BEAT_ERR_SINGULAR) return rc;
+  HIPCHK(hipMemcpyAsync(pressure_out, ctx->pout.p, sizeof(double) * 2 * n * nrhs * nfreq, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return rc;
 }
 
 int beat_field_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const double* pressure,

@@ -776,15 +776,18 @@ __global__ void downcast_kernel(const double* src, float* dst, int64_t n) {
 
 template <typename T, bool WIDE, bool IMAGE>
 static void launch_regular_nq(const RegularArgs<T>& a, int nq, bool pure, dim3 grid, dim3 block, hipStream_t s) {
+  // one workgroup per CU: unused dynamic LDS lifts the workgroup's LDS above half a
+  // CU's 160 KB (static use: about 78 KB in FP64, 40 KB in FP32)
+  const unsigned pad = a.one_per_cu ? (sizeof(T) == 8 ? 4096u : 45056u) : 0u;
   if (nq == 6) {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 6, true, WIDE, IMAGE>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 6, false, WIDE, IMAGE>), grid, block, 0, s, a);
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 6, true, WIDE, IMAGE>), grid, block, pad, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 6, false, WIDE, IMAGE>), grid, block, pad, s, a);
   } else if (nq == 3) {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 3, true, WIDE, IMAGE>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 3, false, WIDE, IMAGE>), grid, block, 0, s, a);
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 3, true, WIDE, IMAGE>), grid, block, pad, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 3, false, WIDE, IMAGE>), grid, block, pad, s, a);
   } else {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 1, true, WIDE, IMAGE>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 1, false, WIDE, IMAGE>), grid, block, 0, s, a);
+    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 1, true, WIDE, IMAGE>), grid, block, pad, s, a);
+    else hipLaunchKernelGGL((bm_regular_kernel<T, 1, false, WIDE, IMAGE>), grid, block, pad, s, a);
   }
 }
 

@@ -38,6 +38,9 @@ struct RegularArgs {
   // followed by the curl signs det * (s0, s1, s2)
   int nimg;
   T img[MAX_IMAGES][6];
+  // nonzero: at most one workgroup per CU (leaves room for kernels of other
+  // streams, e.g. an LU running beside the assembly of the next frequency)
+  int one_per_cu;
 };
 
 template <typename TO>

@@ -233,6 +233,37 @@ class DeviceMesh:
                 history=[hist[i * nrhs:(i + 1) * nrhs].copy() for i in range(rep.refine.history_len)])
         return P, report
 
+    def sweep(self, ks, couplings, qs, precision="float32_refined", max_steps=3, rtol=1e-10):
+        """Solves at every (k, coupling, q) with the next frequency assembled while the
+        current one is factored; qs: (nfreq, F, nrhs). Returns pressures (nfreq, N,
+        nrhs) and the per-frequency reports."""
+        qs = np.asarray(qs, dtype=np.complex128)
+        if qs.ndim == 2:
+            qs = qs[:, :, None]
+        nfreq, nf, nrhs = qs.shape
+        if nf != self.num_faces:
+            raise ValueError("Fused Burton-Miller assembly needs one Neumann row per DP0 dof.")
+        # per frequency a column-major F x nrhs block
+        qh = np.ascontiguousarray(np.transpose(qs, (0, 2, 1)))
+        ph = np.zeros((nfreq, nrhs, self.num_vertices), dtype=np.complex128)
+        k = np.ascontiguousarray(ks, dtype=np.float64)
+        c = np.asarray(couplings, dtype=np.complex128)
+        er = np.ascontiguousarray(c.real)
+        ei = np.ascontiguousarray(c.imag)
+        reps = (_native.SolveReport * nfreq)()
+        work = np.zeros((nfreq, nrhs))
+        for i in range(nfreq):
+            reps[i].refine.working_residuals = work[i].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        opts = _native.SolveOptions(int(max_steps), float(rtol))
+        rc = self.engine._lib.beat_bm_sweep(self.engine._h, self._h, _precision(precision), nfreq, _ptr(k), _ptr(er),
+                                            _ptr(ei), _ptr(qh), nrhs, _ptr(ph), ctypes.byref(opts), reps)
+        self.engine._check(rc, "beat_bm_sweep")
+        P = np.transpose(ph, (0, 2, 1))
+        out = [dict(assembly_s=r.assembly_s, factor_s=r.factor_s, solve_s=r.solve_s,
+                    refinement_status=_native.REFINE_STATUS[r.refine.status], refinement_steps=r.refine.steps,
+                    working_relative_residuals=work[i].copy()) for i, r in enumerate(reps)]
+        return P, out
+
     def field(self, k, pressure, q, points, precision="float64"):
         p = np.asarray(pressure)
         if p.ndim == 1:

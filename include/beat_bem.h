@@ -154,6 +154,24 @@ int beat_bm_solve_device(beat_ctx* ctx, const beat_mesh* mesh, int precision, do
                          double eta_im, const void* q_dev, int64_t nrhs, void* pressure_dev,
                          const beat_solve_options* opts, beat_solve_report* rep);
 
+/* A frequency sweep on one mesh: for each of nfreq frequencies (signed k[i],
+ * coupling eta_re[i] + i eta_im[i]) the body of beat_bm_solve_device, with two
+ * systems in flight: frequency i + 1 is assembled on a second stream while
+ * frequency i is factored and solved (the reference's producer/consumer sweep,
+ * start_sweep_assembly_pipeline / take_sweep_assembly!, BeatEngineCore.jl:1536-1570,
+ * around the per-frequency solve of solve_exterior_request,
+ * BeatEngineCompiledDriver.jl:1337-1346). q_dev: nfreq consecutive complex128 nf x nrhs
+ * column-major blocks; p_dev: nfreq consecutive nv x nrhs blocks (device
+ * pointers). reps: NULL or nfreq reports. Device memory: two assembled systems. */
+int beat_bm_sweep_device(beat_ctx* ctx, const beat_mesh* mesh, int precision, int64_t nfreq, const double* k,
+                         const double* eta_re, const double* eta_im, const void* q_dev, int64_t nrhs, void* p_dev,
+                         const beat_solve_options* opts, beat_solve_report* reps);
+/* Same with host arrays: q nfreq x (nf x nrhs) and pressure_out nfreq x (nv x nrhs),
+ * complex128 column-major blocks. */
+int beat_bm_sweep(beat_ctx* ctx, const beat_mesh* mesh, int precision, int64_t nfreq, const double* k,
+                  const double* eta_re, const double* eta_im, const double* q, int64_t nrhs, double* pressure_out,
+                  const beat_solve_options* opts, beat_solve_report* reps);
+
 /* Field at npts points (npts x 3 row-major) from pressure (nv x nrhs) and q
  * (nf x nrhs): evaluate_galerkin_field_cpu (BeatEngineCpuField.jl:1-49) /
  * evaluate_galerkin_field_rocm (BeatEngineRocmField.jl:150). out: npts x nrhs.
