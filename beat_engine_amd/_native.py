@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(_HERE, "libbeat_bem_hip.so")
 
 BEAT_OK = 0
 BEAT_ERR_SINGULAR = -4
+BEAT_ERR_CANCELLED = -5
 PREC_FP64 = 0
 PREC_FP32_REFINED = 1
 PREC_FP32 = 2
@@ -23,7 +24,7 @@ EXPORTS = (
     "beat_mesh_counts", "beat_bm_assemble", "beat_bm_solve", "beat_bm_solve_device", "beat_field_eval",
     "beat_dense_lu_solve", "beat_refine_lu", "beat_mesh_upload_symmetric", "beat_mesh_symmetry",
     "beat_field_gradient_eval", "beat_operators_assemble", "beat_bm_sweep_device",
-    "beat_bm_sweep",
+    "beat_bm_sweep", "beat_ctx_set_cancel_path", "beat_ctx_reclaim", "beat_ctx_memory_info", "beat_capabilities",
 )
 
 SYMMETRY_CODES = {"off": 0, "x": 1, "xy": 2, "ground": 3}
@@ -67,6 +68,17 @@ class BeatError(RuntimeError):
     pass
 
 
+def capabilities(lib=None, ctx=None) -> dict:
+    """The library's capability object (beat_capabilities)."""
+    import json
+
+    L = lib or load()
+    n = L.beat_capabilities(ctx, None, 0)
+    buf = ctypes.create_string_buffer(int(n))
+    L.beat_capabilities(ctx, buf, n)
+    return json.loads(buf.value.decode())
+
+
 def load(path: str = LIB_PATH):
     global _lib
     if _lib is not None and path == LIB_PATH:
@@ -99,6 +111,11 @@ def load(path: str = LIB_PATH):
     L.beat_operators_assemble.argtypes = [P, P, I, D, P, P, P, P]
     L.beat_bm_sweep_device.argtypes = [P, P, I, I64, P, P, P, P, I64, P, ctypes.POINTER(SolveOptions), P]
     L.beat_bm_sweep.argtypes = [P, P, I, I64, P, P, P, P, I64, P, ctypes.POINTER(SolveOptions), P]
+    L.beat_ctx_set_cancel_path.argtypes = [P, ctypes.c_char_p]
+    L.beat_ctx_reclaim.argtypes = [P]
+    L.beat_ctx_memory_info.argtypes = [P, P]
+    L.beat_capabilities.argtypes = [P, ctypes.c_char_p, I64]
+    L.beat_capabilities.restype = I64
     L.beat_dense_lu_solve.argtypes = [P, I, P, I64, P, I64, P]
     L.beat_refine_lu.argtypes = [P, P, I64, P, I64, P, P, I, D, P, ctypes.POINTER(RefineReport)]
     if path == LIB_PATH:

@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include <unistd.h>
+
 #include "../../include/beat_bem.h"
 #include "bem_common.h"
 #include "bem_math.h"
@@ -70,6 +72,8 @@ struct beat_ctx {
   // the regular kernel at one workgroup per CU while it shares the device with
   // an LU (BEAT_SWEEP_ASM_ONE_PER_CU=0 lets it take every slot)
   bool sweep_asm_one_per_cu = true;
+  // cancellation marker polled between the frequencies of a sweep (empty: none)
+  std::string cancel_path;
   double* trig = nullptr;  // (sin, cos)(m pi/64) table of the FP64 phase
 };
 
@@ -794,10 +798,20 @@ int sweep_impl(beat_ctx* ctx, const beat_mesh* m, int precision, int64_t nfreq, 
                           ctx->asm_stream, ev[s2], ctx->sweep_asm_one_per_cu);
   };
   if (nfreq <= 0) return BEAT_OK;
+  // the reference's marker-file cancellation, checked before each frequency
+  // (cancel_requested, BeatEngineCompiledDriver.jl:1199-1200,1270)
+  auto cancelled = [&]() { return !ctx->cancel_path.empty() && access(ctx->cancel_path.c_str(), F_OK) == 0; };
+  if (reps)
+    for (int64_t i = 0; i < nfreq; ++i) reps[i].factorization_count = 0;
+  if (cancelled()) return fail(ctx, BEAT_ERR_CANCELLED, "cancelled after 0 frequencies");
   if (int rc = enqueue(0)) return rc;
   int result = BEAT_OK;
   for (int64_t i = 0; i < nfreq; ++i) {
     const int s2 = (int)(i & 1);
+    if (i > 0 && cancelled()) {
+      HIPCHK(hipStreamSynchronize(ctx->asm_stream));
+      return fail(ctx, BEAT_ERR_CANCELLED, "cancelled after " + std::to_string(i) + " frequencies");
+    }
     if (i + 1 < nfreq)
       if (int rc = enqueue(i + 1)) return rc;
     int rc = stage_solve(ctx, m, precision, nrhs, sb[s2], p_dev + 2 * i * n * nrhs, opts, ev[s2],
@@ -1056,6 +1070,86 @@ void beat_ctx_destroy(beat_ctx* c) {
 }
 
 const char* beat_last_error(const beat_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int beat_ctx_set_cancel_path(beat_ctx* ctx, const char* path) {
+  if (!ctx) return BEAT_ERR_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->cancel_path = path ? path : "";
+  return BEAT_OK;
+}
+
+int beat_ctx_reclaim(beat_ctx* c) {
+  if (!c) return BEAT_ERR_ARG;
+  std::lock_guard<std::mutex> lock(c->mu);
+  beat_ctx* ctx = c;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->asm_stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->side) HIPCHK(hipStreamSynchronize(c->side));
+  if (c->inb) HIPCHK(hipStreamSynchronize(c->inb));
+  DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64, &c->rn64,
+                    &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin, &c->pout,
+                    &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad};
+  for (DevBuf* b : bufs) {
+    if (b->p) HIPCHK(hipFree(b->p));
+    b->p = nullptr;
+    b->bytes = 0;
+  }
+  for (int s2 = 0; s2 < 2; ++s2) {
+    DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2], &c->sw_b32[s2]};
+    for (DevBuf* b : sw) {
+      if (b->p) HIPCHK(hipFree(b->p));
+      b->p = nullptr;
+      b->bytes = 0;
+    }
+  }
+  return BEAT_OK;
+}
+
+int beat_ctx_memory_info(beat_ctx* c, int64_t* info) {
+  if (!c || !info) return BEAT_ERR_ARG;
+  std::lock_guard<std::mutex> lock(c->mu);
+  beat_ctx* ctx = c;
+  HIPCHK(hipSetDevice(c->device));
+  size_t used = 0;
+  const DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64, &c->rn64,
+                          &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
+                          &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad};
+  for (const DevBuf* b : bufs) used += b->bytes;
+  for (int s2 = 0; s2 < 2; ++s2) {
+    const DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2],
+                          &c->sw_b32[s2]};
+    for (const DevBuf* b : sw) used += b->bytes;
+  }
+  size_t free_b = 0, total_b = 0;
+  HIPCHK(hipMemGetInfo(&free_b, &total_b));
+  info[0] = (int64_t)used;
+  info[1] = (int64_t)free_b;
+  info[2] = (int64_t)total_b;
+  return BEAT_OK;
+}
+
+int64_t beat_capabilities(const beat_ctx* ctx, char* buf, int64_t len) {
+  std::string arch = "gfx950";
+  if (ctx) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) arch = prop.gcnArchName;
+  }
+  const std::string j =
+      std::string("{\"library\":\"beat_bem_hip\",\"abi_version\":1,\"arch\":\"") + arch +
+      "\",\"precisions\":[\"float64\",\"float32_refined\",\"float32\"]"
+      ",\"burton_miller_assembly\":[\"direct_system\",\"operator_matrices\"]"
+      ",\"symmetry_modes\":[\"off\",\"x\",\"xy\",\"ground\"]"
+      ",\"field_quantities\":[\"pressure\",\"pressure_gradient_v1\"]"
+      ",\"phasor_conventions\":[\"exp(-i omega t)\",\"exp(+i omega t)\"]"
+      ",\"cancellation\":\"marker_file\",\"linear_solver\":\"hip_mfma_lu\"}";
+  if (buf && len > 0) {
+    const size_t n = std::min<size_t>((size_t)len - 1, j.size());
+    std::memcpy(buf, j.data(), n);
+    buf[n] = 0;
+  }
+  return (int64_t)j.size() + 1;
+}
 
 int beat_mesh_upload_symmetric(beat_ctx* ctx, const double* verts, int64_t nv, const int32_t* faces, int64_t nf,
                                int quad_order, int singular_order, int symmetry, beat_mesh** out) {
@@ -1329,7 +1423,8 @@ int beat_bm_sweep(beat_ctx* ctx, const beat_mesh* m, int precision, int64_t nfre
   HIPCHK(hipStreamSynchronize(ctx->stream));
   int rc = sweep_impl(ctx, m, precision, nfreq, k, eta_re, eta_im, as<double>(ctx->qin), (int)nrhs,
                       as<double>(ctx->pout), opts, reps);
-  if (rc != BEAT_OK && rc != BEAT_ERR_SINGULAR) return rc;
+  // a cancelled sweep still returns the frequencies it solved
+  if (rc != BEAT_OK && rc != BEAT_ERR_SINGULAR && rc != BEAT_ERR_CANCELLED) return rc;
   HIPCHK(hipMemcpyAsync(pressure_out, ctx->pout.p, sizeof(double) * 2 * n * nrhs * nfreq, hipMemcpyDeviceToHost,
                         ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));

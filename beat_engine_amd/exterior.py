@@ -81,6 +81,24 @@ class Engine:
         self._meshes.clear()
         self._fin()
 
+    # -- worker integration -----------------------------------------------------------
+    def set_cancel_path(self, path):
+        """Marker file that stops a sweep before its next frequency (None clears)."""
+        self._check(self._lib.beat_ctx_set_cancel_path(self._h, None if path is None else str(path).encode()),
+                    "beat_ctx_set_cancel_path")
+
+    def reclaim(self):
+        """Frees the context's workspaces (reclaim_accelerator_memory!)."""
+        self._check(self._lib.beat_ctx_reclaim(self._h), "beat_ctx_reclaim")
+
+    def memory_info(self) -> dict:
+        v = np.zeros(3, dtype=np.int64)
+        self._check(self._lib.beat_ctx_memory_info(self._h, _ptr(v)), "beat_ctx_memory_info")
+        return dict(workspace_bytes=int(v[0]), free_bytes=int(v[1]), total_bytes=int(v[2]))
+
+    def capabilities(self) -> dict:
+        return _native.capabilities(self._lib, self._h)
+
     def upload(self, mesh: BoundaryMesh, quadrature_order: int = 4, singular_order: int = 4,
                symmetry_mode="off") -> "DeviceMesh":
         sym = normalized_symmetry_mode(symmetry_mode)
@@ -236,7 +254,8 @@ class DeviceMesh:
     def sweep(self, ks, couplings, qs, precision="float32_refined", max_steps=3, rtol=1e-10):
         """Solves at every (k, coupling, q) with the next frequency assembled while the
         current one is factored; qs: (nfreq, F, nrhs). Returns pressures (nfreq, N,
-        nrhs) and the per-frequency reports."""
+        nrhs) and the per-frequency reports; after a cancellation (the engine's
+        marker file) the unsolved frequencies report solved=False."""
         qs = np.asarray(qs, dtype=np.complex128)
         if qs.ndim == 2:
             qs = qs[:, :, None]
@@ -257,11 +276,13 @@ class DeviceMesh:
         opts = _native.SolveOptions(int(max_steps), float(rtol))
         rc = self.engine._lib.beat_bm_sweep(self.engine._h, self._h, _precision(precision), nfreq, _ptr(k), _ptr(er),
                                             _ptr(ei), _ptr(qh), nrhs, _ptr(ph), ctypes.byref(opts), reps)
-        self.engine._check(rc, "beat_bm_sweep")
+        if rc != _native.BEAT_ERR_CANCELLED:
+            self.engine._check(rc, "beat_bm_sweep")
         P = np.transpose(ph, (0, 2, 1))
-        out = [dict(assembly_s=r.assembly_s, factor_s=r.factor_s, solve_s=r.solve_s,
-                    refinement_status=_native.REFINE_STATUS[r.refine.status], refinement_steps=r.refine.steps,
-                    working_relative_residuals=work[i].copy()) for i, r in enumerate(reps)]
+        out = [dict(solved=r.factorization_count == 1, assembly_s=r.assembly_s, factor_s=r.factor_s,
+                    solve_s=r.solve_s, refinement_status=_native.REFINE_STATUS[r.refine.status],
+                    refinement_steps=r.refine.steps, working_relative_residuals=work[i].copy())
+               for i, r in enumerate(reps)]
         return P, out
 
     def field(self, k, pressure, q, points, precision="float64"):
@@ -374,6 +395,22 @@ def evaluate_galerkin_field_gradient_hip(points, mesh: BoundaryMesh, pressure, n
 
 def beat_refine_lu(matrix, rhs, residual_matrix=None, residual_rhs=None, max_steps=3, rtol=1e-10, engine=None):
     return (engine or default_engine()).refine_lu(matrix, rhs, residual_matrix, residual_rhs, max_steps, rtol)
+
+
+def worker_backend_availability(engine=None) -> dict:
+    """The worker's backend announcement (worker_backend_availability.jl:1-26) for the
+    ROCm backend this library provides: available when a device context opens."""
+    try:
+        eng = engine or default_engine()
+        caps = eng.capabilities()
+        rocm = {"available": True, "reason": "", "phasor_conventions": caps["phasor_conventions"],
+                "arch": caps["arch"], "linear_solver": caps["linear_solver"]}
+        field_quantities = {"rocm": caps["field_quantities"]}
+    except Exception as exc:  # noqa: BLE001 - reported, as the reference does
+        rocm = {"available": False, "reason": str(exc),
+                "phasor_conventions": [NEGATIVE_TIME_PHASOR, "exp(+i omega t)"]}
+        field_quantities = {"rocm": []}
+    return {"backends": {"rocm": rocm}, "field_quantities": field_quantities}
 
 
 def dense_solve_diagnostics(report: dict, precision="float32_refined") -> dict:

@@ -29,6 +29,7 @@ extern "C" {
 #define BEAT_ERR_HIP (-2)
 #define BEAT_ERR_ALLOC (-3)
 #define BEAT_ERR_SINGULAR (-4)
+#define BEAT_ERR_CANCELLED (-5)
 
 /* Arithmetic of a solve. FP64: assembly, LU and solve in complex128.
  * FP32_REFINED: complex128 assembly held for the residual, complex64 LU
@@ -87,6 +88,23 @@ typedef struct {
 int beat_ctx_create(int device, beat_ctx** out);
 void beat_ctx_destroy(beat_ctx* ctx);
 const char* beat_last_error(const beat_ctx* ctx);
+
+/* Worker integration (the compiled worker's cancellation, cleanup and capability
+ * advertisement: cancel_requested, BeatEngineCompiledDriver.jl:1199-1200,1270;
+ * reclaim_accelerator_memory!, :3951-3971 and the cleanup policy, :4308-4349;
+ * worker_backend_availability, worker_backend_availability.jl:1-26).
+ * beat_ctx_set_cancel_path: a marker file whose existence stops a sweep before its
+ * next frequency (BEAT_ERR_CANCELLED; the reports of the solved frequencies carry
+ * factorization_count 1, the rest 0); NULL clears it.
+ * beat_ctx_reclaim: waits for the context's streams and frees its workspaces
+ * (meshes stay resident). beat_ctx_memory_info: info[0..2] = workspace bytes,
+ * device free bytes, device total bytes. beat_capabilities: a JSON object of what
+ * this library evaluates (ctx may be NULL: no device query); returns the buffer
+ * length needed including the terminating zero and writes at most len bytes. */
+int beat_ctx_set_cancel_path(beat_ctx* ctx, const char* path);
+int beat_ctx_reclaim(beat_ctx* ctx);
+int beat_ctx_memory_info(beat_ctx* ctx, int64_t* info);
+int64_t beat_capabilities(const beat_ctx* ctx, char* buf, int64_t len);
 
 /* Mesh, P1/DP0 spaces, regular rule, touching-pair cache and Duffy rules, once
  * per mesh. Replaces BoundaryMesh (BeatEngineCore.jl:344-364), build_p1_space /

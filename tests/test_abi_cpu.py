@@ -33,3 +33,11 @@ def test_library_exports_every_declared_symbol():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError, match="not built"):
         _native.load(str(tmp_path / "nope.so"))
+
+
+def test_capabilities_without_a_device():
+    caps = _native.capabilities()
+    assert caps["library"] == "beat_bem_hip" and caps["arch"] == "gfx950"
+    assert "pressure_gradient_v1" in caps["field_quantities"]
+    assert caps["cancellation"] == "marker_file"
+    assert set(caps["symmetry_modes"]) == {"off", "x", "xy", "ground"}

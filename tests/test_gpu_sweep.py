@@ -52,3 +52,29 @@ def test_sweep_matches_oracle_and_single_solves(case, precision, tol):
         if precision == "float32_refined":
             assert reps[i]["refinement_status"] == "converged"
         assert reps[i]["assembly_s"] > 0 and reps[i]["factor_s"] > 0
+
+
+def test_cancel_marker_reclaim_and_capabilities(case, tmp_path):
+    mesh, ks, etas, qs, refs, eng = case
+    dm = eng.upload(mesh)
+    marker = tmp_path / "cancel"
+    marker.write_text("")
+    eng.set_cancel_path(marker)
+    P, reps = dm.sweep(ks, etas, qs, "float64")
+    assert not any(r["solved"] for r in reps)
+    marker.unlink()
+    P, reps = dm.sweep(ks[:2], etas[:2], qs[:2], "float64")
+    assert all(r["solved"] for r in reps)
+    assert rel(P[1], refs[1]) < 1e-10
+    eng.set_cancel_path(None)
+    used = eng.memory_info()
+    assert used["workspace_bytes"] > 0 and 0 < used["free_bytes"] <= used["total_bytes"]
+    eng.reclaim()
+    assert eng.memory_info()["workspace_bytes"] == 0
+    P, reps = dm.sweep(ks[:1], etas[:1], qs[:1], "float32_refined")
+    assert rel(P[0], refs[0]) < 1e-8
+    caps = eng.capabilities()
+    assert caps["arch"].startswith("gfx950")
+    avail = exterior.worker_backend_availability(eng)
+    assert avail["backends"]["rocm"]["available"] is True
+    assert "pressure_gradient_v1" in avail["field_quantities"]["rocm"]
