@@ -61,7 +61,7 @@ struct beat_ctx {
   std::string err;
   std::mutex mu;
   DevBuf a64, a32, b64, b32, q64, q32, x64, r64, n64, rn64, xn64, d32, work, chunk, norms, ipiv, lusmall, qin, pout,
-      fld_src, fld_out, fld_pts, grad;
+      fld_src, fld_out, fld_pts, grad, adj;
   hipEvent_t ev[7] = {};
   // frequency sweeps: the assembly stream, two systems in flight, their events
   // (assembly start, regular start, regular end, assembly end) and release events
@@ -1043,7 +1043,7 @@ void beat_ctx_destroy(beat_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64,
                     &c->rn64, &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
-                    &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad};
+                    &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   if (c->asm_stream) (void)hipStreamSynchronize(c->asm_stream);
@@ -1089,7 +1089,7 @@ int beat_ctx_reclaim(beat_ctx* c) {
   if (c->inb) HIPCHK(hipStreamSynchronize(c->inb));
   DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64, &c->rn64,
                     &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin, &c->pout,
-                    &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad};
+                    &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj};
   for (DevBuf* b : bufs) {
     if (b->p) HIPCHK(hipFree(b->p));
     b->p = nullptr;
@@ -1114,7 +1114,7 @@ int beat_ctx_memory_info(beat_ctx* c, int64_t* info) {
   size_t used = 0;
   const DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64, &c->rn64,
                           &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
-                          &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad};
+                          &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj};
   for (const DevBuf* b : bufs) used += b->bytes;
   for (int s2 = 0; s2 < 2; ++s2) {
     const DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2],
@@ -1429,6 +1429,107 @@ int beat_bm_sweep(beat_ctx* ctx, const beat_mesh* m, int precision, int64_t nfre
                         ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   return rc;
+}
+
+
+int beat_adjoint_spl_gradient(beat_ctx* ctx, const beat_mesh* m, int64_t nfreq, const double* k, const double* eta_re,
+                              const double* eta_im, const double* q, const double* pts, int64_t npts, double target_db,
+                              double* loss_out, double* spl_out, double* grad_out) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (nfreq < 1 || npts < 1 || !k || !eta_re || !eta_im || !q || !pts || !loss_out || !grad_out)
+    return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (m->h.nimg != 0) return fail(ctx, BEAT_ERR_ARG, "the adjoint gradient supports meshes without symmetry images");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t n = m->h.nv, nf = m->h.nf, lda = lda_for(n);
+  const int nq = m->h.nq;
+  // device layout (doubles): gV n*3 | seeds nf*34 | fi nf*38 | mu 2n | rhs 2n | v 2n | cw 2npts | pts 3npts | zero 2n
+  const size_t o_gv = 0, o_sd = o_gv + 3 * n, o_fi = o_sd + 34 * nf, o_mu = o_fi + 38 * nf, o_rhs = o_mu + 2 * n,
+               o_v = o_rhs + 2 * n, o_cw = o_v + 2 * n, o_pts = o_cw + 2 * npts, o_zero = o_pts + 3 * npts,
+               o_end = o_zero + 2 * n;
+  HIPCHK(ensure(ctx->adj, sizeof(double) * o_end));
+  double* base = as<double>(ctx->adj);
+  double *gV = base + o_gv, *seeds = base + o_sd, *fi = base + o_fi, *mu = base + o_mu, *rhs = base + o_rhs,
+         *v = base + o_v, *cw = base + o_cw, *dpts = base + o_pts, *zero = base + o_zero;
+  HIPCHK(launch_fill<double>(gV, 3 * n, 0.0, s));
+  HIPCHK(launch_fill<double>(zero, 2 * n, 0.0, s));
+  HIPCHK(hipMemcpyAsync(dpts, pts, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, s));
+  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * nf));
+  HIPCHK(ensure(ctx->q64, sizeof(double) * 2 * nf));
+  HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
+  HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n));
+  const int64_t ns = nf * nq;
+  HIPCHK(ensure(ctx->fld_src, sizeof(double) * (ns * 6 + 4 * ns)));
+  HIPCHK(ensure(ctx->fld_out, sizeof(double) * 2 * npts));
+  IdentityBlocks idb;
+  for (int a = 0; a < 3; ++a) {
+    idb.pd[a] = m->h.id_pd[a];
+    for (int b = 0; b < 3; ++b) idb.pp[a][b] = m->h.id_pp[a][b];
+  }
+  double loss = 0;
+  std::vector<double> u(2 * npts), c(2 * npts);
+  const double ln10 = std::log(10.0);
+  for (int64_t f = 0; f < nfreq; ++f) {
+    // forward: complex128 system, LU, solve
+    HIPCHK(hipMemcpyAsync(ctx->qin.p, q + 2 * f * nf, sizeof(double) * 2 * nf, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf)), dim3(256), 0, s,
+                       (const double*)as<double>(ctx->qin), nf, 1, (const int32_t*)m->face_perm, as<double>(ctx->q64));
+    HIPCHK(hipGetLastError());
+    const double* qd = as<double>(ctx->q64);
+    double* A = as<double>(ctx->a64);
+    double* p = as<double>(ctx->b64);
+    if (int rc = assemble<double>(ctx, m, k[f], eta_re[f], eta_im[f], qd, qd, 1, A, lda, p, s, nullptr, nullptr))
+      return rc;
+    LuWork lw;
+    if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
+    int panels = 0;
+    HIPCHK(lu_factor<double>(A, n, lda, lw, LU_NB, s, &panels));
+    if (int rc = solve_all<double>(ctx, A, n, lda, lw.ipiv, p, 1, true)) return rc;
+    // field at the observation points
+    double* sp = as<double>(ctx->fld_src);
+    double *sn = sp + 3 * ns, *pw = sn + 3 * ns, *qw = pw + 2 * ns;
+    HIPCHK(field_sources<double>(m->rec64, m->fverts, nf, nq, m->rule3, p, qd, 1, 0, m->h.img, sp, sn, pw, qw, s));
+    HIPCHK(field_eval<double>(sp, sn, pw, qw, ns, 1, dpts, npts, as<double>(ctx->fld_out), k[f], s));
+    HIPCHK(hipMemcpyAsync(u.data(), ctx->fld_out.p, sizeof(double) * 2 * npts, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    int info = 0;
+    HIPCHK(hipMemcpy(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost));
+    if (info != 0) return fail(ctx, BEAT_ERR_SINGULAR, "LU found an exactly zero pivot at column " + std::to_string(info));
+    // loss and c = conj(dL/du*) = 2 (S - target) (10 / ln 10) conj(u) / |u|^2
+    for (int64_t d = 0; d < npts; ++d) {
+      const double ur = u[2 * d], ui = u[2 * d + 1], a2 = ur * ur + ui * ui;
+      const double spl = 20.0 * std::log10(std::sqrt(a2) / 20e-6);
+      if (spl_out) spl_out[d + f * npts] = spl;
+      loss += (spl - target_db) * (spl - target_db);
+      const double g = 2.0 * (spl - target_db) * (10.0 / ln10) / a2;
+      c[2 * d] = g * ur;
+      c[2 * d + 1] = -g * ui;
+    }
+    if (!std::isfinite(loss)) return fail(ctx, BEAT_ERR_ARG, "the field vanishes at an observation point");
+    HIPCHK(hipMemcpyAsync(cw, c.data(), sizeof(double) * 2 * npts, hipMemcpyHostToDevice, s));
+    // field part and the adjoint right-hand side F^T c
+    HIPCHK(launch_fill<double>(seeds, 34 * nf, 0.0, s));
+    HIPCHK(launch_fill<double>(rhs, 2 * n, 0.0, s));
+    HIPCHK(adjoint_face_interp(m->rec64, m->fverts, nf, nq, m->rule3, zero, p, qd, fi, s));
+    HIPCHK(adjoint_field(m->rec64, m->fverts, nf, nq, m->rule3, fi, dpts, cw, npts, k[f], seeds, rhs, s));
+    // A^T mu = F^T c with the same factors
+    HIPCHK(hipMemcpyAsync(v, rhs, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(lu_solve_transposed<double>(A, n, lda, v, mu, as<double>(ctx->work), s));
+    // system part and the chain to the vertices
+    HIPCHK(adjoint_face_interp(m->rec64, m->fverts, nf, nq, m->rule3, mu, p, qd, fi, s));
+    HIPCHK(adjoint_pairs(m->rec64, m->verts, m->fverts, nf, nq, m->rule3, m->spairs, (int64_t)m->h.spairs.size(),
+                         m->rule_off, m->rule_cnt, m->rule_data, fi, mu, p, k[f], eta_re[f], eta_im[f], seeds, gV, s));
+    HIPCHK(adjoint_chain(m->verts, m->fverts, nf, nq, m->rule3, seeds, mu, p, qd, eta_re[f], eta_im[f], idb, gV, s));
+  }
+  std::vector<double> g(3 * n);
+  HIPCHK(hipMemcpyAsync(g.data(), gV, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const auto& inv = m->h.vert_inv;
+  for (int64_t vtx = 0; vtx < n; ++vtx)
+    for (int cc = 0; cc < 3; ++cc) grad_out[3 * vtx + cc] = 2.0 * g[3 * (size_t)inv[vtx] + cc];
+  *loss_out = loss;
+  return BEAT_OK;
 }
 
 int beat_field_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const double* pressure,

@@ -145,6 +145,11 @@ hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv
 template <typename T>
 hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, hipStream_t s);
 size_t lu_solve_work_bytes(int64_t n, size_t elem);
+// A^T x = c with the same factors and prepared work area (after lu_solve_prepare):
+// v (n complex, internal row order) holds c and is overwritten; x receives the
+// solution.
+template <typename T>
+hipError_t lu_solve_transposed(const T* LU, int64_t n, int64_t lda, T* v, T* x, const T* work, hipStream_t s);
 
 // r = b - A x (complex128), A row-major n x n, x and b n x nrhs row-major.
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
@@ -152,6 +157,29 @@ hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, 
 // per-column 2-norms of an n x nrhs complex128 row-major matrix into out[nrhs]
 // (out holds nrhs norms followed by 64*nrhs partial sums)
 hipError_t column_norms64(const double* X, int64_t n, int nrhs, double* out, hipStream_t s);
+
+// ---- adjoint shape gradient (FP64, complex interleaved, internal numbering) ---------
+struct IdentityBlocks {
+  double pp[3][3];  // unit-area P1 x P1 mass block
+  double pd[3];     // unit-area P1 x DP0 block
+};
+// per-face interpolants of mu (may be all zero), p and q; fi: 38 doubles per face
+hipError_t adjoint_face_interp(const double* rec, const int32_t* fverts, int64_t nf, int nq, const double* rule,
+                               const double* mu, const double* p, const double* q, double* fi, hipStream_t s);
+// field part: seeds (34 doubles per face, accumulated) and the adjoint right-hand
+// side F^T cw (n complex, accumulated); cw: conj(dL/du*) per observation point
+hipError_t adjoint_field(const double* rec, const int32_t* fverts, int64_t nf, int nq, const double* rule,
+                         const double* fi, const double* pts, const double* cw, int64_t npts, double k, double* seeds,
+                         double* rhs, hipStream_t s);
+// system part: regular pairs (two role passes) and touching pairs; gV: n x 3 real
+hipError_t adjoint_pairs(const double* rec, const double* verts, const int32_t* fverts, int64_t nf, int nq,
+                         const double* rule, const SingularPair* pairs, int64_t npairs, const int32_t* rule_off,
+                         const int32_t* rule_cnt, const double* rule_data, const double* fi, const double* mu,
+                         const double* p, double k, double er, double ei, double* seeds, double* gV, hipStream_t s);
+// identity blocks and the chain from face seeds to vertex coordinates (into gV)
+hipError_t adjoint_chain(const double* verts, const int32_t* fverts, int64_t nf, int nq, const double* rule,
+                         const double* seeds, const double* mu, const double* p, const double* q, double er, double ei,
+                         const IdentityBlocks& idb, double* gV, hipStream_t s);
 
 // ---- field -----------------------------------------------------------------------
 template <typename T>

@@ -1397,6 +1397,149 @@ hipError_t column_norms64(const double* X, int64_t n, int nrhs, double* out, hip
 
 template hipError_t lu_factor<float>(float*, int64_t, int64_t, LuWork&, int, hipStream_t, int*);
 template hipError_t lu_factor<double>(double*, int64_t, int64_t, LuWork&, int, hipStream_t, int*);
+
+// Solve A^T x = c with the factors of lu_factor (A = P L U, so U^T L^T P^T x = c),
+// one workgroup of 1024 threads: U^T z = c forward and L^T y = z backward by
+// 64-column blocks (diagonal block in LDS, solved by one wavefront; the rest of
+// the vector updated from contiguous rows of U and of L in its final row order,
+// read through the gather tables of lu_solve_prepare), then x[perm[r]] = y[r].
+// v (n complex) holds c on entry and is overwritten.
+template <typename T>
+__global__ __launch_bounds__(1024) void lu_solve_t_kernel(const T* LU, int64_t n, int64_t lda, const int* G,
+                                                          const int* perm, T* v, T* x) {
+  __shared__ T blk[2 * PNB * PNB];
+  __shared__ T zb[2 * PNB];
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  // forward: U^T z = c
+  for (int64_t jb = 0; jb < n; jb += PNB) {
+    const int nb = (int)min((int64_t)PNB, n - jb);
+    for (int e = t; e < PNB * PNB; e += blockDim.x) {
+      const int i = e / PNB, j = e - i * PNB;  // blk[i][j] = U[jb+i][jb+j]
+      T re = 0, im = 0;
+      if (i < nb && j < nb) {
+        re = LU[2 * ((jb + i) * lda + jb + j)];
+        im = LU[2 * ((jb + i) * lda + jb + j) + 1];
+      }
+      blk[2 * e] = re;
+      blk[2 * e + 1] = im;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      T vr = lane < nb ? v[2 * (jb + lane)] : T(0), vi = lane < nb ? v[2 * (jb + lane) + 1] : T(0);
+      for (int j = 0; j < nb; ++j) {
+        // z_j = v_j / U_jj on lane j, broadcast
+        T zr = 0, zi = 0;
+        if (lane == j) {
+          const T dr = blk[2 * (j * PNB + j)], di = blk[2 * (j * PNB + j) + 1];
+          const T den = dr * dr + di * di;
+          zr = (vr * dr + vi * di) / den;
+          zi = (vi * dr - vr * di) / den;
+          vr = zr;
+          vi = zi;
+        }
+        zr = __shfl(zr, j, 64);
+        zi = __shfl(zi, j, 64);
+        if (lane > j && lane < nb) {
+          const T ur = blk[2 * (j * PNB + lane)], ui = blk[2 * (j * PNB + lane) + 1];
+          vr -= ur * zr - ui * zi;
+          vi -= ur * zi + ui * zr;
+        }
+      }
+      if (lane < nb) {
+        zb[2 * lane] = vr;
+        zb[2 * lane + 1] = vi;
+        v[2 * (jb + lane)] = vr;
+        v[2 * (jb + lane) + 1] = vi;
+      }
+    }
+    __syncthreads();
+    for (int64_t i = jb + nb + t; i < n; i += blockDim.x) {
+      T ar = 0, ai = 0;
+      for (int j = 0; j < nb; ++j) {
+        const T ur = LU[2 * ((jb + j) * lda + i)], ui = LU[2 * ((jb + j) * lda + i) + 1];
+        const T zr = zb[2 * j], zi = zb[2 * j + 1];
+        ar += ur * zr - ui * zi;
+        ai += ur * zi + ui * zr;
+      }
+      v[2 * i] -= ar;
+      v[2 * i + 1] -= ai;
+    }
+    __syncthreads();
+  }
+  // backward: L^T y = z, L unit lower in its final row order: L[r][c] is stored in
+  // row G[c / MAXBW][r]
+  const int64_t nblk = (n + PNB - 1) / PNB;
+  for (int64_t b = nblk - 1; b >= 0; --b) {
+    const int64_t jb = b * PNB;
+    const int nb = (int)min((int64_t)PNB, n - jb);
+    const int* Gj = G + (jb / MAXBW) * n;
+    for (int e = t; e < PNB * PNB; e += blockDim.x) {
+      const int i = e / PNB, j = e - i * PNB;  // blk[i][j] = L[jb+i][jb+j], i > j
+      T re = 0, im = 0;
+      if (i < nb && j < i) {
+        const int64_t sr = Gj[jb + i];
+        re = LU[2 * (sr * lda + jb + j)];
+        im = LU[2 * (sr * lda + jb + j) + 1];
+      }
+      blk[2 * e] = re;
+      blk[2 * e + 1] = im;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      T vr = lane < nb ? v[2 * (jb + lane)] : T(0), vi = lane < nb ? v[2 * (jb + lane) + 1] : T(0);
+      for (int j = nb - 1; j >= 0; --j) {
+        const T yr = __shfl(vr, j, 64), yi = __shfl(vi, j, 64);
+        if (lane < j) {
+          const T lr = blk[2 * (j * PNB + lane)], li = blk[2 * (j * PNB + lane) + 1];
+          vr -= lr * yr - li * yi;
+          vi -= lr * yi + li * yr;
+        }
+      }
+      if (lane < nb) {
+        zb[2 * lane] = vr;
+        zb[2 * lane + 1] = vi;
+        v[2 * (jb + lane)] = vr;
+        v[2 * (jb + lane) + 1] = vi;
+      }
+    }
+    __syncthreads();
+    for (int64_t i = t; i < jb; i += blockDim.x) {
+      const int* Gi = G + (i / MAXBW) * n;
+      T ar = 0, ai = 0;
+      for (int j = 0; j < nb; ++j) {
+        const int64_t sr = Gi[jb + j];
+        const T lr = LU[2 * (sr * lda + i)], li = LU[2 * (sr * lda + i) + 1];
+        const T yr = zb[2 * j], yi = zb[2 * j + 1];
+        ar += lr * yr - li * yi;
+        ai += lr * yi + li * yr;
+      }
+      v[2 * i] -= ar;
+      v[2 * i + 1] -= ai;
+    }
+    __syncthreads();
+  }
+  for (int64_t r = t; r < n; r += blockDim.x) {
+    const int64_t o = perm[r];
+    x[2 * o] = v[2 * r];
+    x[2 * o + 1] = v[2 * r + 1];
+  }
+}
+
+template <typename T>
+hipError_t lu_solve_transposed(const T* LU, int64_t n, int64_t lda, T* v, T* x, const T* work, hipStream_t s) {
+  const SolveLayout L = solve_layout(n);
+  const int* G = solve_ints(const_cast<T*>(work), L);
+  const int* perm = G + (size_t)L.nJ * n;
+  hipLaunchKernelGGL((lu_solve_t_kernel<T>), dim3(1), dim3(1024), 0, s, LU, n, lda, G, perm, v, x);
+  return hipGetLastError();
+}
+
+template hipError_t lu_solve_transposed<float>(const float*, int64_t, int64_t, float*, float*, const float*,
+                                               hipStream_t);
+template hipError_t lu_solve_transposed<double>(const double*, int64_t, int64_t, double*, double*, const double*,
+                                                hipStream_t);
+
 template hipError_t lu_solve_prepare<float>(const float*, int64_t, int64_t, const int*, float*, hipStream_t);
 template hipError_t lu_solve_prepare<double>(const double*, int64_t, int64_t, const int*, double*, hipStream_t);
 template hipError_t lu_solve<float>(const float*, int64_t, int64_t, float*, int, float*, hipStream_t);

@@ -33,7 +33,7 @@ import numpy as np
 
 from . import _native
 from .mesh import BoundaryMesh, normalized_symmetry_mode
-from .phasor import NEGATIVE_TIME_PHASOR, burton_miller_coupling, outgoing_wavenumber
+from .phasor import NEGATIVE_TIME_PHASOR, burton_miller_coupling, neumann_scale, outgoing_wavenumber
 
 PRECISIONS = {
     "float64": _native.PREC_FP64,
@@ -285,6 +285,29 @@ class DeviceMesh:
                for i, r in enumerate(reps)]
         return P, out
 
+    def spl_loss_gradient(self, ks, couplings, qs, points, target_db=90.0):
+        """(loss, gradient (nv x 3), SPL (points x nfreq)) of
+        sum_f sum_d (SPL_fd - target_db)^2 with one adjoint solve per frequency."""
+        qs = np.asarray(qs, dtype=np.complex128)
+        if qs.ndim == 1:
+            qs = qs[None, :]
+        nfreq = qs.shape[0]
+        if qs.shape[1] != self.num_faces:
+            raise ValueError("one Neumann value per face and frequency")
+        qh = np.ascontiguousarray(qs)
+        k = np.ascontiguousarray(ks, dtype=np.float64).reshape(nfreq)
+        c = np.asarray(couplings, dtype=np.complex128).reshape(nfreq)
+        er, ei = np.ascontiguousarray(c.real), np.ascontiguousarray(c.imag)
+        pts = np.ascontiguousarray(np.reshape(points, (-1, 3)), dtype=np.float64)
+        loss = ctypes.c_double(0.0)
+        spl = np.zeros((len(pts), nfreq), order="F")
+        grad = np.zeros((self.num_vertices, 3))
+        rc = self.engine._lib.beat_adjoint_spl_gradient(self.engine._h, self._h, nfreq, _ptr(k), _ptr(er), _ptr(ei),
+                                                        _ptr(qh), _ptr(pts), len(pts), float(target_db),
+                                                        ctypes.byref(loss), _ptr(spl), _ptr(grad))
+        self.engine._check(rc, "beat_adjoint_spl_gradient")
+        return loss.value, grad, spl
+
     def field(self, k, pressure, q, points, precision="float64"):
         p = np.asarray(pressure)
         if p.ndim == 1:
@@ -391,6 +414,31 @@ def evaluate_galerkin_field_gradient_hip(points, mesh: BoundaryMesh, pressure, n
     dm = eng.upload(mesh, 4, 4, symmetry_mode)
     out = dm.field_gradient(outgoing_wavenumber(k, convention), pressure, neumann, points, precision)
     return out[:, :, 0] if np.ndim(pressure) == 1 else out
+
+
+def far_field_spl_loss_and_gradient(mesh: BoundaryMesh, frequencies_hz, normal_velocity, points, *,
+                                    target_db=90.0, sound_speed=343.0, density=1.21, quadrature_order=4,
+                                    singular_order=4, convention=NEGATIVE_TIME_PHASOR, engine=None):
+    """Loss sum_f sum_d (SPL_fd - target_db)^2 at the observation points and its
+    gradient with respect to every vertex coordinate (nv x 3), for a fixed normal
+    velocity per face (F, or nfreq x F). The reverse rule of the mesh -> loss map."""
+    eng = engine or default_engine()
+    dm = DeviceMesh(eng, mesh, quadrature_order, singular_order)
+    try:
+        freqs = np.atleast_1d(np.asarray(frequencies_hz, dtype=np.float64))
+        vn = np.asarray(normal_velocity, dtype=np.complex128)
+        if vn.ndim == 1:
+            vn = np.broadcast_to(vn, (len(freqs), mesh.num_faces))
+        ks, etas, qs = [], [], []
+        for f, v in zip(freqs, vn):
+            w = 2 * np.pi * f
+            k = outgoing_wavenumber(w / sound_speed, convention)
+            ks.append(k)
+            etas.append(burton_miller_coupling(k, convention))
+            qs.append(neumann_scale(density, w, convention) * v)
+        return dm.spl_loss_gradient(ks, etas, np.stack(qs), points, target_db)
+    finally:
+        dm.close()
 
 
 def beat_refine_lu(matrix, rhs, residual_matrix=None, residual_rhs=None, max_steps=3, rtol=1e-10, engine=None):

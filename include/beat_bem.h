@@ -212,6 +212,22 @@ int beat_field_eval(beat_ctx* ctx, const beat_mesh* mesh, int precision, double 
 int beat_field_gradient_eval(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, const double* pressure,
                              const double* q, int64_t nrhs, const double* pts, int64_t npts, double* out);
 
+/* Far-field SPL loss and its gradient with respect to the mesh vertex positions
+ * (no counterpart in the reference, which has no shape derivative; its SPL is
+ * spl_for_points, BeatEngineDriver.jl:657-663). For each frequency f (signed k[f],
+ * coupling eta[f], Neumann data q: nf complex128 per frequency, nfreq consecutive
+ * blocks) the complex128 system is assembled, factored and solved, the field is
+ * evaluated at the npts observation points (npts x 3 row-major), and
+ *   loss = sum_f sum_d (20 log10(|u_fd| / 20e-6) - target_db)^2.
+ * The gradient comes from one adjoint solve per frequency with the same LU
+ * (A^T mu = F^T conj(dL/du*)) and kernels that contract mu^T (dA p - db) and the
+ * field derivative over element pairs without forming dA; q is held fixed.
+ * spl_out (npts x nfreq column-major) may be NULL; grad_out: nv x 3 row-major,
+ * caller's vertex order. Meshes uploaded with a symmetry mode are rejected. */
+int beat_adjoint_spl_gradient(beat_ctx* ctx, const beat_mesh* mesh, int64_t nfreq, const double* k,
+                              const double* eta_re, const double* eta_im, const double* q, const double* pts,
+                              int64_t npts, double target_db, double* loss_out, double* spl_out, double* grad_out);
+
 /* Dense LU with partial pivoting and solve of nrhs columns, the library routine
  * behind _beat_dense_lu_solve (BeatEngineDenseSolve.jl:1104-1110) and
  * rocm_dense_lu! (BeatEngineRocmOperators.jl:132-159). a: n x n, b and x:
