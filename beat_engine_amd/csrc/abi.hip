@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -74,6 +75,13 @@ struct beat_ctx {
   bool sweep_asm_one_per_cu = true;
   // cancellation marker polled between the frequencies of a sweep (empty: none)
   std::string cancel_path;
+  // autotuned launch strategy per size class (floor(log2 n)): the factorization's
+  // stream layout (-1 not tuned, 0 look-ahead streams, 1 one stream) and the
+  // batch assembly occupancy (-1 not tuned, 0 every slot, 1 one workgroup per CU)
+  signed char lu_layout[40], asm_occ[40];
+  beat_ctx() {
+    for (int i = 0; i < 40; ++i) lu_layout[i] = asm_occ[i] = -1;
+  }
   double* trig = nullptr;  // (sin, cos)(m pi/64) table of the FP64 phase
 };
 
@@ -102,6 +110,12 @@ struct beat_mesh {
 };
 
 namespace {
+
+inline int size_class(int64_t n) {
+  int b = 0;
+  while (b < 39 && (int64_t(2) << b) <= n) ++b;
+  return b;
+}
 
 int fail(beat_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -682,6 +696,10 @@ int stage_solve(beat_ctx* ctx, const beat_mesh* m, int precision, int nrhs, Syst
   HIPCHK(hipStreamWaitEvent(s, asm_ev.end, 0));
   LuWork lw;
   if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
+  if (ctx->lu_layout[size_class(n)] == 1) {  // tuned: one stream for this size class
+    lw.side = nullptr;
+    lw.inb = nullptr;
+  }
   int panels = 0;
   Refine R;
   HIPCHK(hipEventRecord(ctx->ev[1], s));
@@ -773,48 +791,83 @@ int solve_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, doubl
   return stage_solve(ctx, m, precision, nrhs, sb, p_dev, opts, ev, rep);
 }
 
-// Frequency sweep with two systems in flight: the assembly of frequency i + 1 runs
-// on the assembly stream while frequency i is factored and solved on the context
-// stream; a system's buffers are reassembled only after its solve has finished.
-int sweep_impl(beat_ctx* ctx, const beat_mesh* m, int precision, int64_t nfreq, const double* ks, const double* ers,
-               const double* eis, const double* q_dev, int nrhs, double* p_dev, const beat_solve_options* opts,
-               beat_solve_report* reps) {
+// Jobs (mesh, wavenumber, coupling, q) with two systems in flight: the assembly
+// of job i + 1 runs on the assembly stream while job i is factored and solved on
+// the context stream; a system's buffers are reassembled only after its solve has
+// finished. q_dev / p_dev: per job a column-major block (nf_j x nrhs, nv_j x nrhs)
+// at the offsets q_off / p_off (complex entries).
+int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int precision, const double* ks,
+               const double* ers, const double* eis, const double* q_dev, const int64_t* q_off, int nrhs, double* p_dev,
+               const int64_t* p_off, const beat_solve_options* opts, beat_solve_report* reps) {
   if (int rc = check_solve_options(ctx, precision, opts)) return rc;
-  const int64_t n = m->h.nv, nf = m->h.nf;
+  if (njobs <= 0) return BEAT_OK;
   SystemBufs sb[2];
   for (int s2 = 0; s2 < 2; ++s2) {
     sb[s2] = SystemBufs{&ctx->sw_q64[s2], &ctx->sw_q32[s2], &ctx->sw_a64[s2], &ctx->sw_b64[s2],
                         precision == BEAT_PREC_FP32 ? &ctx->sw_a32[s2] : &ctx->a32,
                         precision == BEAT_PREC_FP32 ? &ctx->sw_b32[s2] : &ctx->b32};
   }
+  // size both systems for the largest job up front (no reallocation while in flight)
+  int64_t nmax = 0, fmax = 0;
+  for (int64_t i = 0; i < njobs; ++i) {
+    nmax = std::max<int64_t>(nmax, meshes[i]->h.nv);
+    fmax = std::max<int64_t>(fmax, meshes[i]->h.nf);
+  }
+  for (int s2 = 0; s2 < 2; ++s2) {
+    HIPCHK(ensure(*sb[s2].q64, sizeof(double) * 2 * fmax * nrhs));
+    if (precision == BEAT_PREC_FP32) {
+      HIPCHK(ensure(*sb[s2].q32, sizeof(float) * 2 * fmax * nrhs));
+      HIPCHK(ensure(*sb[s2].a32, sizeof(float) * 2 * nmax * lda_for(nmax)));
+      HIPCHK(ensure(*sb[s2].b32, sizeof(float) * 2 * nmax * nrhs));
+    } else {
+      HIPCHK(ensure(*sb[s2].a64, sizeof(double) * 2 * nmax * lda_for(nmax)));
+      HIPCHK(ensure(*sb[s2].b64, sizeof(double) * 2 * nmax * nrhs));
+    }
+  }
+  {
+    // the factorization, solve and refinement workspaces too
+    LuWork lw;
+    if (ensure_lu_work<double>(ctx, nmax, lw) != BEAT_OK) return BEAT_ERR_HIP;
+    HIPCHK(ensure(ctx->work, lu_solve_work_bytes(nmax, sizeof(double))));
+    if (precision == BEAT_PREC_FP32_REFINED) {
+      HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * nmax * lda_for(nmax)));
+      HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * nmax * nrhs));
+      HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * nmax * nrhs));
+      HIPCHK(ensure(ctx->r64, sizeof(double) * 2 * nmax * nrhs));
+      HIPCHK(ensure(ctx->n64, sizeof(double) * 2 * nmax * nrhs));
+      HIPCHK(ensure(ctx->rn64, sizeof(double) * 2 * nmax * nrhs));
+      HIPCHK(ensure(ctx->d32, sizeof(float) * 2 * nmax * nrhs + sizeof(int) * 64 + 4096));
+    }
+  }
   AsmEvents ev[2];
   for (int s2 = 0; s2 < 2; ++s2)
     ev[s2] = AsmEvents{ctx->sw_ev[s2][0], ctx->sw_ev[s2][1], ctx->sw_ev[s2][2], ctx->sw_ev[s2][3]};
   auto enqueue = [&](int64_t i) -> int {
     const int s2 = (int)(i & 1);
-    // the system's previous solve (frequency i - 2) must be done with its buffers
+    // the system's previous solve (job i - 2) must be done with its buffers
     if (i >= 2) HIPCHK(hipStreamWaitEvent(ctx->asm_stream, ctx->sw_free[s2], 0));
-    return stage_assemble(ctx, m, precision, ks[i], ers[i], eis[i], q_dev + 2 * i * nf * nrhs, nrhs, sb[s2],
-                          ctx->asm_stream, ev[s2], ctx->sweep_asm_one_per_cu);
+    const signed char occ = ctx->asm_occ[size_class(meshes[i]->h.nv)];
+    const bool one_per_cu = occ < 0 ? ctx->sweep_asm_one_per_cu : occ == 1;
+    return stage_assemble(ctx, meshes[i], precision, ks[i], ers[i], eis[i], q_dev + 2 * q_off[i], nrhs, sb[s2],
+                          ctx->asm_stream, ev[s2], one_per_cu);
   };
-  if (nfreq <= 0) return BEAT_OK;
-  // the reference's marker-file cancellation, checked before each frequency
+  // the reference's marker-file cancellation, checked before each job
   // (cancel_requested, BeatEngineCompiledDriver.jl:1199-1200,1270)
   auto cancelled = [&]() { return !ctx->cancel_path.empty() && access(ctx->cancel_path.c_str(), F_OK) == 0; };
   if (reps)
-    for (int64_t i = 0; i < nfreq; ++i) reps[i].factorization_count = 0;
+    for (int64_t i = 0; i < njobs; ++i) reps[i].factorization_count = 0;
   if (cancelled()) return fail(ctx, BEAT_ERR_CANCELLED, "cancelled after 0 frequencies");
   if (int rc = enqueue(0)) return rc;
   int result = BEAT_OK;
-  for (int64_t i = 0; i < nfreq; ++i) {
+  for (int64_t i = 0; i < njobs; ++i) {
     const int s2 = (int)(i & 1);
     if (i > 0 && cancelled()) {
       HIPCHK(hipStreamSynchronize(ctx->asm_stream));
       return fail(ctx, BEAT_ERR_CANCELLED, "cancelled after " + std::to_string(i) + " frequencies");
     }
-    if (i + 1 < nfreq)
+    if (i + 1 < njobs)
       if (int rc = enqueue(i + 1)) return rc;
-    int rc = stage_solve(ctx, m, precision, nrhs, sb[s2], p_dev + 2 * i * n * nrhs, opts, ev[s2],
+    int rc = stage_solve(ctx, meshes[i], precision, nrhs, sb[s2], p_dev + 2 * p_off[i], opts, ev[s2],
                          reps ? reps + i : nullptr);
     if (rc != BEAT_OK && rc != BEAT_ERR_SINGULAR) {
       (void)hipStreamSynchronize(ctx->asm_stream);
@@ -825,6 +878,20 @@ int sweep_impl(beat_ctx* ctx, const beat_mesh* m, int precision, int64_t nfreq, 
   }
   HIPCHK(hipStreamSynchronize(ctx->asm_stream));
   return result;
+}
+
+// A frequency sweep on one mesh: consecutive q and p blocks.
+int sweep_impl(beat_ctx* ctx, const beat_mesh* m, int precision, int64_t nfreq, const double* ks, const double* ers,
+               const double* eis, const double* q_dev, int nrhs, double* p_dev, const beat_solve_options* opts,
+               beat_solve_report* reps) {
+  std::vector<const beat_mesh*> ms(std::max<int64_t>(nfreq, 0), m);
+  std::vector<int64_t> qo(ms.size()), po(ms.size());
+  for (size_t i = 0; i < ms.size(); ++i) {
+    qo[i] = (int64_t)i * m->h.nf * nrhs;
+    po[i] = (int64_t)i * m->h.nv * nrhs;
+  }
+  return batch_impl(ctx, nfreq, ms.data(), precision, ks, ers, eis, q_dev, qo.data(), nrhs, p_dev, po.data(), opts,
+                    reps);
 }
 
 template <typename T>
@@ -1530,6 +1597,104 @@ int beat_adjoint_spl_gradient(beat_ctx* ctx, const beat_mesh* m, int64_t nfreq, 
     for (int cc = 0; cc < 3; ++cc) grad_out[3 * vtx + cc] = 2.0 * g[3 * (size_t)inv[vtx] + cc];
   *loss_out = loss;
   return BEAT_OK;
+}
+
+int beat_autotune(beat_ctx* ctx, const beat_mesh* m, int precision, double k, int64_t nrhs, double* timings,
+                  int* choice) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (nrhs < 1 || nrhs > 64 || !(k > 0) || !choice) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (int rc = check_solve_options(ctx, precision, nullptr)) return rc;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  const int64_t n = m->h.nv, nf = m->h.nf;
+  const int b = size_class(n);
+  // synthetic drive: unit normal velocity on every face and right-hand side
+  std::vector<double> qh(2 * nf * nrhs * 3);
+  for (int64_t i = 0; i < nf * nrhs * 3; ++i) {
+    qh[2 * i] = 0.0;
+    qh[2 * i + 1] = 1.0 + 0.01 * (double)(i % 7);
+  }
+  HIPCHK(ensure(ctx->qin, sizeof(double) * qh.size()));
+  HIPCHK(ensure(ctx->pout, sizeof(double) * 2 * n * nrhs * 3));
+  HIPCHK(hipMemcpy(ctx->qin.p, qh.data(), sizeof(double) * qh.size(), hipMemcpyHostToDevice));
+  const double eta_re = 0.0, eta_im = 1.0 / k;
+  auto run_solve = [&](double* best) -> int {
+    *best = INFINITY;
+    for (int r = 0; r < 3; ++r) {
+      const auto t0 = std::chrono::steady_clock::now();
+      int rc = solve_impl(ctx, m, precision, k, eta_re, eta_im, as<double>(ctx->qin), (int)nrhs,
+                          as<double>(ctx->pout), nullptr, nullptr);
+      if (rc) return rc;
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (r > 0) *best = std::min(*best, dt);  // the first run warms up
+    }
+    return BEAT_OK;
+  };
+  double t[4];
+  // factorization stream layout
+  for (int layout = 0; layout < 2; ++layout) {
+    ctx->lu_layout[b] = (signed char)layout;
+    if (layout == 0 && (!ctx->side || !ctx->inb)) {
+      t[layout] = INFINITY;
+      continue;
+    }
+    if (int rc = run_solve(&t[layout])) return rc;
+  }
+  ctx->lu_layout[b] = t[1] < t[0] ? 1 : 0;
+  // assembly occupancy inside a batch of three solves
+  const beat_mesh* ms[3] = {m, m, m};
+  const double ks[3] = {k, k, k}, ers[3] = {eta_re, eta_re, eta_re}, eis[3] = {eta_im, eta_im, eta_im};
+  const int64_t qo[3] = {0, nf * nrhs, 2 * nf * nrhs}, po[3] = {0, n * nrhs, 2 * n * nrhs};
+  for (int occ = 0; occ < 2; ++occ) {
+    ctx->asm_occ[b] = (signed char)occ;
+    double best = INFINITY;
+    for (int r = 0; r < 2; ++r) {
+      const auto t0 = std::chrono::steady_clock::now();
+      int rc = batch_impl(ctx, 3, ms, precision, ks, ers, eis, as<double>(ctx->qin), qo, (int)nrhs,
+                          as<double>(ctx->pout), po, nullptr, nullptr);
+      if (rc) return rc;
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (r > 0) best = std::min(best, dt);
+    }
+    t[2 + occ] = best;
+  }
+  ctx->asm_occ[b] = t[3] < t[2] ? 1 : 0;
+  choice[0] = ctx->lu_layout[b];
+  choice[1] = ctx->asm_occ[b];
+  if (timings)
+    for (int i = 0; i < 4; ++i) timings[i] = t[i];
+  return BEAT_OK;
+}
+
+int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int precision, const double* k,
+                  const double* eta_re, const double* eta_im, const double* q, int64_t nrhs, double* pressure_out,
+                  const beat_solve_options* opts, beat_solve_report* reps) {
+  if (!ctx) return BEAT_ERR_ARG;
+  if (njobs < 0 || (njobs > 0 && (!meshes || !k || !eta_re || !eta_im || !q || !pressure_out)) || nrhs < 1 ||
+      nrhs > 4096)
+    return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (njobs == 0) return BEAT_OK;
+  std::vector<int64_t> qo(njobs), po(njobs);
+  int64_t qt = 0, pt = 0;
+  for (int64_t i = 0; i < njobs; ++i) {
+    if (int rc = check_mesh(ctx, meshes[i])) return rc;
+    qo[i] = qt;
+    po[i] = pt;
+    qt += meshes[i]->h.nf * nrhs;
+    pt += meshes[i]->h.nv * nrhs;
+  }
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * qt));
+  HIPCHK(ensure(ctx->pout, sizeof(double) * 2 * pt));
+  HIPCHK(hipMemcpyAsync(ctx->qin.p, q, sizeof(double) * 2 * qt, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int rc = batch_impl(ctx, njobs, meshes, precision, k, eta_re, eta_im, as<double>(ctx->qin), qo.data(), (int)nrhs,
+                      as<double>(ctx->pout), po.data(), opts, reps);
+  if (rc != BEAT_OK && rc != BEAT_ERR_SINGULAR && rc != BEAT_ERR_CANCELLED) return rc;
+  HIPCHK(hipMemcpyAsync(pressure_out, ctx->pout.p, sizeof(double) * 2 * pt, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return rc;
 }
 
 int beat_field_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const double* pressure,

@@ -99,6 +99,47 @@ class Engine:
     def capabilities(self) -> dict:
         return _native.capabilities(self._lib, self._h)
 
+    # -- batching ---------------------------------------------------------------------
+    def autotune(self, device_mesh: "DeviceMesh", precision="float32_refined", k=10.0, nrhs=8) -> dict:
+        """Times the launch strategies on this mesh's size class and keeps the faster."""
+        t = np.zeros(4)
+        ch = np.zeros(2, dtype=np.int32)
+        rc = self._lib.beat_autotune(self._h, device_mesh._h, _precision(precision), float(k), int(nrhs), _ptr(t),
+                                     _ptr(ch))
+        self._check(rc, "beat_autotune")
+        return dict(lu_layout=("lookahead", "one_stream")[int(ch[0])],
+                    batch_assembly=("every_slot", "one_per_cu")[int(ch[1])],
+                    seconds=dict(lookahead=t[0], one_stream=t[1], every_slot=t[2], one_per_cu=t[3]))
+
+    def batch(self, meshes, ks, couplings, qs, precision="float32_refined", max_steps=3, rtol=1e-10):
+        """Solves job i = (meshes[i], ks[i], couplings[i], qs[i] (F_i x nrhs)) with the
+        next job assembled while the current one is factored; returns the pressures
+        (N_i x nrhs each) and the per-job reports."""
+        nj = len(meshes)
+        qs = [np.asarray(q, dtype=np.complex128).reshape(m.num_faces, -1) for m, q in zip(meshes, qs)]
+        nrhs = qs[0].shape[1]
+        if any(q.shape[1] != nrhs for q in qs):
+            raise ValueError("every job needs the same number of right-hand sides")
+        qh = np.concatenate([np.asfortranarray(q).ravel(order="F") for q in qs])
+        ph = np.zeros(sum(m.num_vertices for m in meshes) * nrhs, dtype=np.complex128)
+        handles = (ctypes.c_void_p * nj)(*[m._h.value for m in meshes])
+        k = np.ascontiguousarray(ks, dtype=np.float64)
+        c = np.asarray(couplings, dtype=np.complex128)
+        er, ei = np.ascontiguousarray(c.real), np.ascontiguousarray(c.imag)
+        reps = (_native.SolveReport * nj)()
+        opts = _native.SolveOptions(int(max_steps), float(rtol))
+        rc = self._lib.beat_bm_batch(self._h, nj, ctypes.cast(handles, ctypes.c_void_p), _precision(precision),
+                                     _ptr(k), _ptr(er), _ptr(ei), _ptr(qh), nrhs, _ptr(ph), ctypes.byref(opts), reps)
+        self._check(rc, "beat_bm_batch")
+        out, off = [], 0
+        for m in meshes:
+            n = m.num_vertices * nrhs
+            out.append(ph[off:off + n].reshape((m.num_vertices, nrhs), order="F"))
+            off += n
+        info = [dict(assembly_s=r.assembly_s, factor_s=r.factor_s, solve_s=r.solve_s,
+                     refinement_status=_native.REFINE_STATUS[r.refine.status]) for r in reps]
+        return out, info
+
     def upload(self, mesh: BoundaryMesh, quadrature_order: int = 4, singular_order: int = 4,
                symmetry_mode="off") -> "DeviceMesh":
         sym = normalized_symmetry_mode(symmetry_mode)

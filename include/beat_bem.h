@@ -190,6 +190,27 @@ int beat_bm_sweep(beat_ctx* ctx, const beat_mesh* mesh, int precision, int64_t n
                   const double* eta_re, const double* eta_im, const double* q, int64_t nrhs, double* pressure_out,
                   const beat_solve_options* opts, beat_solve_report* reps);
 
+/* Autotuner (no counterpart in the reference): times, on this mesh's size class
+ * (floor(log2 nv)) and precision, the factorization with the look-ahead streams
+ * against one stream, and a three-solve batch with the next assembly at one
+ * workgroup per CU against every slot, and keeps the faster of each in the
+ * context for later solves, sweeps and batches of that size class. k: a
+ * wavenumber for the trial solves. timings (may be NULL): 4 seconds
+ * (look-ahead, one stream, every slot, one per CU; INFINITY when not available);
+ * choice[0..1] = stream layout (0 look-ahead, 1 one stream), occupancy (0 every
+ * slot, 1 one workgroup per CU). */
+int beat_autotune(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, int64_t nrhs, double* timings,
+                  int* choice);
+
+/* A batch of solves over several meshes (batching form: many meshes of uneven
+ * size, each at its frequencies): job i solves meshes[i] at k[i], eta[i] with the
+ * next job assembled while job i is factored, as in beat_bm_sweep. q: per job an
+ * nf_i x nrhs complex128 column-major block, consecutive; pressure_out: per job
+ * nv_i x nrhs, consecutive. reps: NULL or njobs reports. */
+int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int precision, const double* k,
+                  const double* eta_re, const double* eta_im, const double* q, int64_t nrhs, double* pressure_out,
+                  const beat_solve_options* opts, beat_solve_report* reps);
+
 /* Field at npts points (npts x 3 row-major) from pressure (nv x nrhs) and q
  * (nf x nrhs): evaluate_galerkin_field_cpu (BeatEngineCpuField.jl:1-49) /
  * evaluate_galerkin_field_rocm (BeatEngineRocmField.jl:150). out: npts x nrhs.
