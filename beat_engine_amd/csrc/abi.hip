@@ -11,6 +11,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <unistd.h>
@@ -73,12 +74,17 @@ struct beat_ctx {
   // the regular kernel at one workgroup per CU while it shares the device with
   // an LU (BEAT_SWEEP_ASM_ONE_PER_CU=0 lets it take every slot)
   bool sweep_asm_one_per_cu = true;
+  // contexts a batch may use at once (BEAT_BATCH_WORKERS, default 4)
+  int batch_workers = 4;
   // cancellation marker polled between the frequencies of a sweep (empty: none)
   std::string cancel_path;
   // autotuned launch strategy per size class (floor(log2 n)): the factorization's
   // stream layout (-1 not tuned, 0 look-ahead streams, 1 one stream) and the
   // batch assembly occupancy (-1 not tuned, 0 every slot, 1 one workgroup per CU)
   signed char lu_layout[40], asm_occ[40];
+  // further contexts on the same device that run batch jobs of small systems
+  // concurrently (created on first use, owned by this context)
+  std::vector<beat_ctx*> aux;
   beat_ctx() {
     for (int i = 0; i < 40; ++i) lu_layout[i] = asm_occ[i] = -1;
   }
@@ -1092,6 +1098,7 @@ int beat_ctx_create(int device, beat_ctx** out) {
     }
   }
   if (const char* v = std::getenv("BEAT_SWEEP_ASM_ONE_PER_CU")) c->sweep_asm_one_per_cu = v[0] != '0';
+  if (const char* v = std::getenv("BEAT_BATCH_WORKERS")) c->batch_workers = std::max(1, std::atoi(v));
   // BEAT_LU_LOOKAHEAD=0 runs the whole factorization in order on one stream
   if (const char* v = std::getenv("BEAT_LU_LOOKAHEAD"))
     if (v[0] == '0') {
@@ -1106,6 +1113,8 @@ int beat_ctx_create(int device, beat_ctx** out) {
 
 void beat_ctx_destroy(beat_ctx* c) {
   if (!c) return;
+  for (beat_ctx* a : c->aux) beat_ctx_destroy(a);
+  c->aux.clear();
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64,
@@ -1147,6 +1156,8 @@ int beat_ctx_set_cancel_path(beat_ctx* ctx, const char* path) {
 
 int beat_ctx_reclaim(beat_ctx* c) {
   if (!c) return BEAT_ERR_ARG;
+  for (beat_ctx* a : c->aux)
+    if (int rc = beat_ctx_reclaim(a)) return rc;
   std::lock_guard<std::mutex> lock(c->mu);
   beat_ctx* ctx = c;
   HIPCHK(hipSetDevice(c->device));
@@ -1689,8 +1700,77 @@ int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, 
   HIPCHK(ensure(ctx->pout, sizeof(double) * 2 * pt));
   HIPCHK(hipMemcpyAsync(ctx->qin.p, q, sizeof(double) * 2 * qt, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
-  int rc = batch_impl(ctx, njobs, meshes, precision, k, eta_re, eta_im, as<double>(ctx->qin), qo.data(), (int)nrhs,
-                      as<double>(ctx->pout), po.data(), opts, reps);
+  // Small systems leave most of the device idle, so several contexts run jobs at
+  // once. Their factorizations and triangular solves keep all their workgroups
+  // waiting on each other; the number of contexts is bounded so that those
+  // workgroups of every context together never exceed the CU count, which lets
+  // each of them find a CU once the other (finite) kernels drain.
+  int64_t nmax = 0;
+  for (int64_t i = 0; i < njobs; ++i) nmax = std::max<int64_t>(nmax, meshes[i]->h.nv);
+  int cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  const int64_t spin = (nmax + 63) / 64;  // workgroups of one factorization panel or block solve
+  int workers = (int)std::min<int64_t>({(int64_t)ctx->batch_workers, (int64_t)cus / std::max<int64_t>(spin, 1),
+                                        njobs});
+  if (workers < 1) workers = 1;
+  while ((int)ctx->aux.size() < workers - 1) {
+    beat_ctx* a = nullptr;
+    if (beat_ctx_create(ctx->device, &a) != BEAT_OK) return fail(ctx, BEAT_ERR_HIP, "could not create a batch context");
+    ctx->aux.push_back(a);
+  }
+  std::vector<beat_ctx*> cs{ctx};
+  for (int w = 1; w < workers; ++w) {
+    beat_ctx* a = ctx->aux[w - 1];
+    a->cancel_path = ctx->cancel_path;
+    a->sweep_asm_one_per_cu = ctx->sweep_asm_one_per_cu;
+    std::memcpy(a->lu_layout, ctx->lu_layout, sizeof(ctx->lu_layout));
+    std::memcpy(a->asm_occ, ctx->asm_occ, sizeof(ctx->asm_occ));
+    cs.push_back(a);
+  }
+  // worker w takes jobs w, w + workers, ... (interleaved sizes)
+  std::vector<int> rcs(workers, BEAT_OK);
+  auto run = [&](int w) {
+    beat_ctx* c = cs[w];
+    if (hipSetDevice(c->device) != hipSuccess) {
+      rcs[w] = BEAT_ERR_HIP;
+      return;
+    }
+    std::vector<const beat_mesh*> ms;
+    std::vector<double> kk, er, ei;
+    std::vector<int64_t> qq, pp, idx;
+    for (int64_t i = w; i < njobs; i += workers) {
+      ms.push_back(meshes[i]);
+      kk.push_back(k[i]);
+      er.push_back(eta_re[i]);
+      ei.push_back(eta_im[i]);
+      qq.push_back(qo[i]);
+      pp.push_back(po[i]);
+      idx.push_back(i);
+    }
+    std::vector<beat_solve_report> rr(reps ? ms.size() : 0);
+    if (reps)
+      for (size_t j = 0; j < ms.size(); ++j) rr[j] = reps[idx[j]];
+    rcs[w] = batch_impl(c, (int64_t)ms.size(), ms.data(), precision, kk.data(), er.data(), ei.data(),
+                        as<double>(ctx->qin), qq.data(), (int)nrhs, as<double>(ctx->pout), pp.data(), opts,
+                        reps ? rr.data() : nullptr);
+    if (reps)
+      for (size_t j = 0; j < ms.size(); ++j) reps[idx[j]] = rr[j];
+  };
+  if (workers == 1) {
+    run(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int w = 1; w < workers; ++w) th.emplace_back(run, w);
+    run(0);
+    for (auto& t : th) t.join();
+  }
+  int rc = BEAT_OK;
+  for (int w = 0; w < workers; ++w) {
+    const int r = rcs[w];
+    if (r == BEAT_OK) continue;
+    if (w > 0 && r != BEAT_OK) ctx->err = cs[w]->err;
+    if (rc == BEAT_OK || (rc == BEAT_ERR_SINGULAR || rc == BEAT_ERR_CANCELLED)) rc = r;
+  }
   if (rc != BEAT_OK && rc != BEAT_ERR_SINGULAR && rc != BEAT_ERR_CANCELLED) return rc;
   HIPCHK(hipMemcpyAsync(pressure_out, ctx->pout.p, sizeof(double) * 2 * pt, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--nrhs", type=int, default=8)
     ap.add_argument("--fp64-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-form", action="store_true",
+                    help="time the batching form (32 meshes x 128 frequencies, 4 RHS) instead; no JSON line")
+    ap.add_argument("--batch-freqs", type=int, default=128)
     ap.add_argument("--counter-run", action="store_true",
                     help="for rocprofv3 --pmc: a 1 GiB device copy (counter calibration), then one complex128 "
                          "freq-solve of the workload; prints no JSON line")
@@ -108,6 +111,58 @@ def cpu_baseline(mesh, nrhs, sample_faces, lu_n):
     }
 
 
+def batch_form(args):
+    """32 meshes of uneven size (jittered icospheres and graded capped cylinders, 500-6000
+    triangles, from seeds) x log-spaced frequencies 50 Hz - 8 kHz, 4 RHS each, as one
+    batch per precision; prints jobs/s per precision and the refined-vs-FP64 error."""
+    from beat_engine_amd.exterior import Engine
+    from beat_engine_amd.mesh import capped_cylinder, jittered_icosphere
+    from beat_engine_amd.phasor import burton_miller_coupling, neumann_scale
+
+    rng = np.random.default_rng(32)
+    meshes = []
+    for i in range(32):
+        target = int(rng.integers(500, 6001))
+        if i % 2 == 0:
+            level = 2 if target < 900 else 3 if target < 2500 else 4
+            m = jittered_icosphere(level, 0.1, 0.02, seed=100 + i)
+            if m.num_faces > 6000:
+                m = jittered_icosphere(3, 0.1, 0.02, seed=100 + i)
+        else:
+            m = capped_cylinder(target_faces=target, seed=200 + i)
+        meshes.append(m)
+    freqs = np.logspace(np.log10(50.0), np.log10(8000.0), args.batch_freqs)
+    eng = Engine(0)
+    dms = [eng.upload(m) for m in meshes]
+    tune = eng.autotune(dms[-1], "float32_refined", 2 * np.pi * 1000 / 343.0, 4)
+    log(f"autotune (size class of {meshes[-1].num_vertices} dofs): {tune}")
+    jobs = []
+    for m, dm in zip(meshes, dms):
+        v = rng.normal(size=(m.num_faces, 4)) + 1j * rng.normal(size=(m.num_faces, 4))
+        for f in freqs:
+            w = 2 * np.pi * f
+            k = w / 343.0
+            jobs.append((dm, k, burton_miller_coupling(k), neumann_scale(1.21, w) * v))
+    faces = [m.num_faces for m in meshes]
+    log(f"{len(jobs)} jobs, faces {min(faces)}-{max(faces)}, dofs {sum(m.num_vertices for m in meshes)} total")
+    res = {}
+    for prec in ("float32_refined", "float64"):
+        t = time.perf_counter()
+        P, info = eng.batch([j[0] for j in jobs], [j[1] for j in jobs], [j[2] for j in jobs], [j[3] for j in jobs],
+                            prec)
+        el = time.perf_counter() - t
+        res[prec] = (P, el)
+        bad = sum(1 for r in info if prec == "float32_refined" and r["refinement_status"] != "converged")
+        log(f"{prec}: {len(jobs)} jobs in {el:.2f} s = {len(jobs) / el:.1f} jobs/s; not converged: {bad}")
+    err = max(float(np.linalg.norm(a - b) / np.linalg.norm(b))
+              for a, b in zip(res["float32_refined"][0], res["float64"][0]))
+    print(json.dumps({"batch_form": {"jobs": len(jobs), "meshes": 32, "frequencies": int(args.batch_freqs), "nrhs": 4,
+                                     "refined_jobs_per_s": len(jobs) / res["float32_refined"][1],
+                                     "fp64_jobs_per_s": len(jobs) / res["float64"][1],
+                                     "max_rel_err_refined_vs_fp64": err, "autotune": tune}}), flush=True)
+    eng.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -125,6 +180,9 @@ def main():
 
     if not torch.cuda.is_available():
         raise RuntimeError("bench.py needs a GPU")
+    if args.batch_form:
+        batch_form(args)
+        return
     mesh = jittered_icosphere(args.level, 0.1, 0.02, seed=20480)
     nrhs = args.nrhs
     if args.counter_run:
