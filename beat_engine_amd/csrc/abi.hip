@@ -852,8 +852,9 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
     const int s2 = (int)(i & 1);
     // the system's previous solve (job i - 2) must be done with its buffers
     if (i >= 2) HIPCHK(hipStreamWaitEvent(ctx->asm_stream, ctx->sw_free[s2], 0));
+    // the first job's assembly has nothing beside it: every slot
     const signed char occ = ctx->asm_occ[size_class(meshes[i]->h.nv)];
-    const bool one_per_cu = occ < 0 ? ctx->sweep_asm_one_per_cu : occ == 1;
+    const bool one_per_cu = i > 0 && (occ < 0 ? ctx->sweep_asm_one_per_cu : occ == 1);
     return stage_assemble(ctx, meshes[i], precision, ks[i], ers[i], eis[i], q_dev + 2 * q_off[i], nrhs, sb[s2],
                           ctx->asm_stream, ev[s2], one_per_cu);
   };
