@@ -254,10 +254,12 @@ int64_t lda_for(int64_t n) { return (n + 63) / 64 * 64; }
 template <typename T>
 int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   const int maxg = (int)((n + 63) / 64) + 1;
-  const size_t slot_rows = (size_t)2 * maxg * 64 * 2 * sizeof(double);
+  // published panel rows: up to four tagged 64-bit words per complex element
+  const size_t slot_rows = (size_t)2 * maxg * 64 * 4 * sizeof(unsigned long long);
+  const size_t slot_diag = (size_t)2 * 64 * 4 * sizeof(unsigned long long);
   const size_t linv_bytes = (size_t)8 * 64 * 64 * 2 * sizeof(double);
-  const size_t bytes = sizeof(int) * (n + 8) + sizeof(long long) * 4 * maxg + slot_rows + 2 * 64 * 2 * sizeof(double) +
-                       linv_bytes + 4096;
+  const size_t bytes =
+      sizeof(int) * (n + 8) + sizeof(long long) * 32 * maxg + slot_rows + slot_diag + linv_bytes + 4096;
   HIPCHK(ensure(ctx->ipiv, bytes));
   char* p = reinterpret_cast<char*>(ctx->ipiv.p);
   auto take = [&](size_t b) {
@@ -267,9 +269,9 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   };
   w.ipiv = reinterpret_cast<int*>(take(sizeof(int) * (n + 8)));
   w.info = reinterpret_cast<int*>(take(sizeof(int) * 4));
-  w.slots = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 4 * maxg));
+  w.slots = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 32 * maxg));  // 2 x maxg slots of 16 words
   w.slot_rows = take(slot_rows);
-  w.slot_diag = take(2 * 64 * 2 * sizeof(double));
+  w.slot_diag = take(slot_diag);
   w.linv = take(linv_bytes);
   w.side = ctx->side;
   w.ev_fact = ctx->ev_fact;

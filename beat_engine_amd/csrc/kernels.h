@@ -120,9 +120,9 @@ hipError_t launch_downcast(const double* src, float* dst, int64_t n, hipStream_t
 // ---- dense LU (row-major complex, leading dimension lda) -------------------------
 struct LuWork {
   int* ipiv;          // n, global row index chosen at each column
-  unsigned long long* slots;  // 2 x maxg slots of two words (tag << 32 | row, tagged value)
-  void* slot_rows;    // 2 x maxg x nb complex T
-  void* slot_diag;    // 2 x nb complex T
+  unsigned long long* slots;  // 2 x maxg slots of 16 words (tag << 32 | row, tagged value, padding)
+  void* slot_rows;    // 2 x maxg x 64 published rows, 4 tagged 64-bit words per element
+  void* slot_diag;    // 2 x 64 published elements, same form
   int* info;          // first zero pivot (1-based), 0 if none
   void* linv;         // 2 x 4 x 64 x 64 complex (double-sized): leaf inverses of two outer blocks
   int maxg;

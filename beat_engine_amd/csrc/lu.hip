@@ -6,10 +6,14 @@
 //   panel_kernel     cooperative grid; each workgroup holds a block of panel rows
 //                    in LDS; per column a grid-wide FP64 |re|+|im| pivot search
 //                    (the LAPACK izamax measure), swap, scale and rank-1 update
-//   swap_trsm_kernel row swaps of the trailing columns and U12 = L11^-1 A12
-//   gemm kernels     A22 -= L21 U12 as one real GEMM on MFMA: complex operands
+//   laswp_kernel     row swaps of the trailing columns
+//   gemm kernels     U12 = L11^-1 A12 with the leaf inverse the panel formed,
+//                    and A22 -= L21 U12, as real GEMMs on MFMA: complex operands
 //                    are expanded to 2x2 real blocks [[br, bi], [-bi, br]]
 //                    while staged (the 4-real-multiply complex product)
+// The factorization and solve kernels raise their wave priority above the
+// assembly kernels' (which a sweep runs beside them); the panel, the critical
+// path, raises its own higher still.
 // Columns left of the panel are never swapped; the forward solve applies each
 // panel's swaps to the right-hand sides just before that panel's block, which is
 // equivalent to LAPACK's getrs on fully swapped factors.
@@ -29,6 +33,7 @@ namespace {
 constexpr int PNB = 64;   // max panel width
 constexpr int RPW = 64;   // panel rows per workgroup
 constexpr int MAXBW = 256; // outer block width (columns factored before the wide trailing update)
+constexpr int SLOT_WORDS = 16;  // one exchange slot per 128-byte line
 
 template <typename T>
 struct PanelArgs {
@@ -36,9 +41,9 @@ struct PanelArgs {
   int64_t n, lda, j0;
   int nb;
   int* ipiv;
-  unsigned long long* slots;  // 2 x maxg slots of two words: (tag << 32 | row), tagged value
-  T* slot_rows;
-  T* slot_diag;
+  unsigned long long* slots;  // 2 x maxg slots of SLOT_WORDS words: (tag << 32 | row), tagged value
+  unsigned long long* slot_rows;  // 2 x maxg x PNB tagged elements
+  unsigned long long* slot_diag;  // 2 x PNB tagged elements
   int maxg;
   int* info;
   T* linv;  // PNB x PNB row-major: inverse of the leaf's unit-lower diagonal block
@@ -66,6 +71,52 @@ __device__ __forceinline__ void st_cplx(double* p, double re, double im) {
 __device__ __forceinline__ void ld_cplx(const double* p, double& re, double& im) {
   re = __longlong_as_double((long long)ld_u64(reinterpret_cast<const unsigned long long*>(p)));
   im = __longlong_as_double((long long)ld_u64(reinterpret_cast<const unsigned long long*>(p) + 1));
+}
+
+// Panel rows published to other workgroups: every 32-bit half of a complex
+// element's parts travels in its own 64-bit word as (tag << 32 | bits), TW words
+// per element (complex64 2, complex128 4). A reader checks the tag of each word it
+// loads, so the writer needs no fence between the row and the slot announcing it.
+template <typename T>
+struct TaggedRow {
+  static constexpr int TW = sizeof(T) / 2;
+};
+__device__ __forceinline__ void st_tagged(unsigned long long* q, float re, float im, unsigned tag) {
+  const unsigned long long t = (unsigned long long)tag << 32;
+  st_u64(q, t | __float_as_uint(re));
+  st_u64(q + 1, t | __float_as_uint(im));
+}
+__device__ __forceinline__ void st_tagged(unsigned long long* q, double re, double im, unsigned tag) {
+  const unsigned long long t = (unsigned long long)tag << 32;
+  const unsigned long long a = (unsigned long long)__double_as_longlong(re), b = (unsigned long long)__double_as_longlong(im);
+  st_u64(q, t | (a & 0xffffffffull));
+  st_u64(q + 1, t | (a >> 32));
+  st_u64(q + 2, t | (b & 0xffffffffull));
+  st_u64(q + 3, t | (b >> 32));
+}
+// Loads one published element, re-reading until every word carries the tag.
+__device__ __forceinline__ void ld_tagged(const unsigned long long* q, float& re, float& im, unsigned tag) {
+  for (;;) {
+    const unsigned long long a = ld_u64(q), b = ld_u64(q + 1);
+    if ((unsigned)(a >> 32) == tag && (unsigned)(b >> 32) == tag) {
+      re = __uint_as_float((unsigned)a);
+      im = __uint_as_float((unsigned)b);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+__device__ __forceinline__ void ld_tagged(const unsigned long long* q, double& re, double& im, unsigned tag) {
+  for (;;) {
+    const unsigned long long a = ld_u64(q), b = ld_u64(q + 1), c = ld_u64(q + 2), d = ld_u64(q + 3);
+    if ((unsigned)(a >> 32) == tag && (unsigned)(b >> 32) == tag && (unsigned)(c >> 32) == tag &&
+        (unsigned)(d >> 32) == tag) {
+      re = __longlong_as_double((long long)((b << 32) | (a & 0xffffffffull)));
+      im = __longlong_as_double((long long)((d << 32) | (c & 0xffffffffull)));
+      return;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
 }
 
 // Wavefront reductions of 32-bit keys (full wavefront active): DPP exchanges
@@ -112,25 +163,34 @@ __device__ __forceinline__ bool better(double v2, int r2, double v, int r) {
 
 // Four wavefronts per workgroup; lane r of every wavefront stands for panel row
 // r0 + r. The panel block is kept column-major in LDS (col[c][r], rows padded to
-// RPW + 1) so that the column operations are bank-conflict free. Per column,
-// wavefront 0 does the pivot search and the cross-workgroup exchange, applies
-// the row swap and forms the multipliers; then the four wavefronts update the
-// remaining columns (wavefront w takes the columns c = w mod 4).
+// RPW + 1) so that the column operations are bank-conflict free. Column jj runs
+// in three phases separated by workgroup barriers:
+//  A1  wavefront 0 applies column jj - 1's elimination to column jj, picks the
+//      workgroup's pivot candidate and copies the candidate row and the diagonal
+//      row into registers;
+//  A2  wavefront 0 publishes the copies and runs the cross-workgroup exchange
+//      while wavefronts 1-3 apply column jj - 1's elimination to the columns
+//      > jj (so the elimination is hidden behind the exchange latency). The
+//      copies lack that elimination in their columns > jj; the readers complete
+//      them from the multiplier in the copy and the previous pivot row;
+//  B   wavefront 0 exchanges the rows, forms the multipliers of column jj and
+//      keeps the pivot row for the next step.
 //
 // Cross-workgroup exchange per column, without a shared counter: every
-// workgroup writes its candidate row content (and the diagonal row, if it owns
-// it) to its slot area, waits for those stores, then writes the slot's two
-// words: (column tag << 32 | row) and the candidate value with the tag in its
-// low 16 mantissa bits (a relative change below 2^-36). Readers poll both words
-// of every slot until both carry the column's tag. Slots alternate between two
+// workgroup writes its candidate row (and the diagonal row, if it owns it) as
+// tagged words, then the slot's two words: (column tag << 32 | row) and the
+// candidate value with the tag in its low 16 mantissa bits (a relative change
+// below 2^-36). Readers poll both words of every slot until both carry the
+// column's tag, then load the winning row, re-reading words until each carries
+// the tag. Slots sit on their own 128-byte lines and alternate between two
 // buffers by column parity; a workgroup can only overwrite a buffer after every
 // workgroup has published the following column, that is after all reads of the
-// buffer ended. Slot words are cleared before each factorization, and the tag of
-// a column is its index + 1.
+// buffer ended. Slot words and published rows are cleared before each
+// factorization, and the tag of a column is its index + 1.
 // Leaf inverse: complex64 builds it during the factorization (wavefront 1 forms
-// row jj - 1 of X = L11^-1 while wavefront 0 runs column jj's exchange, from row
-// jj - 1 of L, final once column jj - 1's swap is done), in its own LDS array;
-// complex128 keeps the LDS for occupancy and inverts after the last column.
+// row jj - 1 of X = L11^-1 in phase A2 of column jj, from row jj - 1 of L, final
+// once column jj - 1's swap is done), in its own LDS array; complex128 keeps the
+// LDS for occupancy and inverts after the last column.
 template <typename T>
 struct PanelInv {
   static constexpr bool inc = sizeof(T) == 4;
@@ -141,6 +201,7 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
   constexpr int LDR = RPW + 1;
   constexpr int U = 4;  // unroll of the LDS-bound loops (independent reads in flight)
   constexpr bool INC = PanelInv<T>::inc;
+  constexpr int TW = TaggedRow<T>::TW;
   __shared__ T col[PNB * LDR * 2];
   __shared__ T piv[PNB * 2];
   __shared__ T part[4 * PNB * 2];
@@ -200,30 +261,58 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
   };
   for (int jj = 0; jj < nb; ++jj) {
     const int64_t cg = a.j0 + jj;
-    if (INC && g == 0 && wv == 1 && jj > 0 && a.linv) inverse_row(jj - 1);
+    const unsigned tag = (unsigned)(cg + 1);
+    const int par = jj & 1;
+    const bool own_diag = cg >= r0 && cg < r0 + nr;
+    // column jj - 1's elimination (rows >= cg) is applied during this step: a
+    // zero pivot leaves a zero multiplier column, so it is skipped
+    const bool upd = jj > 0 && !(piv[2 * (jj - 1)] == T(0) && piv[2 * (jj - 1) + 1] == T(0));
+    const bool below = lane < nr && r0 + lane >= cg;
+    // A1 (wavefront 0): bring column jj up to date, pick this workgroup's
+    // candidate, and copy the candidate row and the diagonal row (lane = column)
+    // before the other wavefronts touch them. Columns > jj of the copies still
+    // lack column jj - 1's elimination.
+    T cr = 0, ci = 0, dr = 0, di = 0;
+    int row = -1;
+    double v = 0.0;
     if (wv == 0) {
-      const unsigned tag = (unsigned)(cg + 1);
-      const int par = jj & 1;
-      const bool mine = lane < nr && r0 + lane >= cg;
-      const double mag =
-          mine ? fabs((double)col[2 * (jj * LDR + lane)]) + fabs((double)col[2 * (jj * LDR + lane) + 1]) : 0.0;
-      double v;
-      int row;
-      wave_pick(mag, (int)(r0 + lane), mine, v, row);
-      // publish the candidate row content and the diagonal row content, then the slot
-      if (row >= 0 && lane < nb) {
-        const int lr = (int)(row - r0);
-        st_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + g) * PNB + lane), col[2 * (lane * LDR + lr)],
-                col[2 * (lane * LDR + lr) + 1]);
+      T xr = 0, xi = 0;
+      if (below) {
+        xr = col[2 * (jj * LDR + lane)];
+        xi = col[2 * (jj * LDR + lane) + 1];
+        if (upd) {
+          const T lr = col[2 * ((jj - 1) * LDR + lane)], li = col[2 * ((jj - 1) * LDR + lane) + 1];
+          const T ur = piv[2 * jj], ui = piv[2 * jj + 1];
+          xr -= lr * ur - li * ui;
+          xi -= lr * ui + li * ur;
+          col[2 * (jj * LDR + lane)] = xr;
+          col[2 * (jj * LDR + lane) + 1] = xi;
+        }
       }
-      const bool own_diag = cg >= r0 && cg < r0 + nr;
-      if (own_diag && lane < nb) {
-        const int lr = (int)(cg - r0);
-        st_cplx(a.slot_diag + 2 * (par * PNB + lane), col[2 * (lane * LDR + lr)], col[2 * (lane * LDR + lr) + 1]);
+      wave_pick(below ? fabs((double)xr) + fabs((double)xi) : 0.0, (int)(r0 + lane), below, v, row);
+      const int lr = (int)(row - r0), ld = (int)(cg - r0);
+      const T sxr = __shfl(xr, lr, 64), sxi = __shfl(xi, lr, 64);
+      const T dxr = __shfl(xr, own_diag ? ld : 0, 64), dxi = __shfl(xi, own_diag ? ld : 0, 64);
+      if (lane < nb) {
+        cr = lane == jj ? sxr : col[2 * (lane * LDR + lr)];
+        ci = lane == jj ? sxi : col[2 * (lane * LDR + lr) + 1];
+        if (own_diag) {
+          dr = lane == jj ? dxr : col[2 * (lane * LDR + ld)];
+          di = lane == jj ? dxi : col[2 * (lane * LDR + ld) + 1];
+        }
       }
-      __builtin_amdgcn_s_waitcnt(0);
+    }
+    __syncthreads();
+    // A2: wavefront 0 runs the cross-workgroup exchange while wavefronts 1-3
+    // apply column jj - 1's elimination to the columns > jj (and, complex64,
+    // wavefront 1 forms row jj - 1 of the leaf inverse).
+    T pr = 0, pim = 0, qr = 0, qi = 0;
+    int p = -1;
+    if (wv == 0) {
+      if (lane < nb) st_tagged(a.slot_rows + TW * (((int64_t)par * a.maxg + g) * PNB + lane), cr, ci, tag);
+      if (own_diag && lane < nb) st_tagged(a.slot_diag + TW * (par * PNB + lane), dr, di, tag);
       if (lane == 0) {
-        unsigned long long* slot = a.slots + 2 * (par * a.maxg + g);
+        unsigned long long* slot = a.slots + SLOT_WORDS * (par * a.maxg + g);
         const unsigned long long vb = ((unsigned long long)__double_as_longlong(v) & ~0xffffull) | (tag & 0xffffu);
         st_u64(slot + 1, vb);
         st_u64(slot, ((unsigned long long)tag << 32) | (unsigned long long)(unsigned)row);
@@ -244,7 +333,7 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
           for (int u = 0; u < 4; ++u) {
             t[u] = vb[u] = 0;
             if (pend[u]) {
-              const unsigned long long* slot = a.slots + 2 * (par * a.maxg + base + 64 * u);
+              const unsigned long long* slot = a.slots + SLOT_WORDS * (par * a.maxg + base + 64 * u);
               t[u] = ld_u64(slot);
               vb[u] = ld_u64(slot + 1);
             }
@@ -269,66 +358,82 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
         }
       }
       double pv;
-      int p;
       wave_pick(bv, brow, brow >= 0, pv, p);
       const int pwg = (int)((p - a.j0) / RPW);
-      T pr = 0, pim = 0;
+      const bool own_p = p != cg && p >= r0 && p < r0 + nr;
       if (lane < nb) {
-        ld_cplx(a.slot_rows + 2 * (((int64_t)par * a.maxg + pwg) * PNB + lane), pr, pim);
+        ld_tagged(a.slot_rows + TW * (((int64_t)par * a.maxg + pwg) * PNB + lane), pr, pim, tag);
+        if (own_p) ld_tagged(a.slot_diag + TW * (par * PNB + lane), qr, qi, tag);
+      }
+      // complete the copies: x[c] -= x[jj - 1] u[c] for c > jj (u = pivot row jj - 1)
+      if (upd) {
+        const T plr = __shfl(pr, jj - 1, 64), pli = __shfl(pim, jj - 1, 64);
+        const T qlr = __shfl(qr, jj - 1, 64), qli = __shfl(qi, jj - 1, 64);
+        if (lane > jj && lane < nb) {
+          const T ur = piv[2 * lane], ui = piv[2 * lane + 1];
+          pr -= plr * ur - pli * ui;
+          pim -= plr * ui + pli * ur;
+          qr -= qlr * ur - qli * ui;
+          qi -= qlr * ui + qli * ur;
+        }
+      }
+    } else {
+      if (upd && below) {
+        const T lr = col[2 * ((jj - 1) * LDR + lane)], li = col[2 * ((jj - 1) * LDR + lane) + 1];
+        // this wavefront's columns c > jj with c = jj + wv (mod 3)
+        int c = jj + wv;
+        for (; c + 3 * (U - 1) < nb; c += 3 * U) {
+          T xr[U], xi[U], ur[U], ui[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            xr[u] = col[2 * ((c + 3 * u) * LDR + lane)];
+            xi[u] = col[2 * ((c + 3 * u) * LDR + lane) + 1];
+            ur[u] = piv[2 * (c + 3 * u)];
+            ui[u] = piv[2 * (c + 3 * u) + 1];
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            col[2 * ((c + 3 * u) * LDR + lane)] = xr[u] - (lr * ur[u] - li * ui[u]);
+            col[2 * ((c + 3 * u) * LDR + lane) + 1] = xi[u] - (lr * ui[u] + li * ur[u]);
+          }
+        }
+        for (; c < nb; c += 3) {
+          const T ur = piv[2 * c], ui = piv[2 * c + 1];
+          col[2 * (c * LDR + lane)] -= lr * ur - li * ui;
+          col[2 * (c * LDR + lane) + 1] -= lr * ui + li * ur;
+        }
+      }
+      if (INC && g == 0 && wv == 1 && jj > 0 && a.linv) inverse_row(jj - 1);
+    }
+    __syncthreads();
+    // B (wavefront 0): row exchange, multipliers of column jj, pivot row to piv
+    if (wv == 0) {
+      if (lane < nb) {
         piv[2 * lane] = pr;
         piv[2 * lane + 1] = pim;
       }
       if (g == 0 && lane == 0) a.ipiv[cg] = p;
       if (p != cg && lane < nb) {
         if (p >= r0 && p < r0 + nr) {
-          T dr, di;
-          ld_cplx(a.slot_diag + 2 * (par * PNB + lane), dr, di);
-          col[2 * (lane * LDR + (p - r0))] = dr;
-          col[2 * (lane * LDR + (p - r0)) + 1] = di;
+          col[2 * (lane * LDR + (p - r0))] = qr;
+          col[2 * (lane * LDR + (p - r0)) + 1] = qi;
         }
         if (own_diag) {
           col[2 * (lane * LDR + (cg - r0))] = pr;
           col[2 * (lane * LDR + (cg - r0)) + 1] = pim;
         }
       }
-      // multipliers of this column (pr, pim of lane jj is the pivot)
       const T vr = __shfl(pr, jj, 64), vi = __shfl(pim, jj, 64);
+      const T qjr = __shfl(qr, jj, 64), qji = __shfl(qi, jj, 64);  // row p's new column-jj entry
       if (vr == T(0) && vi == T(0)) {
         if (g == 0 && lane == 0 && *a.info == 0) *a.info = (int)cg + 1;
       } else if (lane < nr && r0 + lane > cg) {
         const T den = vr * vr + vi * vi;
         const T ir = vr / den, ii = -vi / den;
-        const T ar = col[2 * (jj * LDR + lane)], ai = col[2 * (jj * LDR + lane) + 1];
+        const bool at_p = r0 + lane == p;
+        const T ar = at_p ? qjr : col[2 * (jj * LDR + lane)], ai = at_p ? qji : col[2 * (jj * LDR + lane) + 1];
         col[2 * (jj * LDR + lane)] = ar * ir - ai * ii;
         col[2 * (jj * LDR + lane) + 1] = ar * ii + ai * ir;
-      }
-    }
-    __syncthreads();
-    const T vr = piv[2 * jj], vi = piv[2 * jj + 1];
-    if (vr == T(0) && vi == T(0)) continue;  // LAPACK leaves the column unscaled and goes on
-    if (lane < nr && r0 + lane > cg) {
-      const T lr = col[2 * (jj * LDR + lane)], li = col[2 * (jj * LDR + lane) + 1];
-      // this wavefront's columns c > jj with c = wv (mod 4)
-      int c = jj + 1 + ((wv - (jj + 1)) & 3);
-      for (; c + 4 * (U - 1) < nb; c += 4 * U) {
-        T xr[U], xi[U], ur[U], ui[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          xr[u] = col[2 * ((c + 4 * u) * LDR + lane)];
-          xi[u] = col[2 * ((c + 4 * u) * LDR + lane) + 1];
-          ur[u] = piv[2 * (c + 4 * u)];
-          ui[u] = piv[2 * (c + 4 * u) + 1];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          col[2 * ((c + 4 * u) * LDR + lane)] = xr[u] - (lr * ur[u] - li * ui[u]);
-          col[2 * ((c + 4 * u) * LDR + lane) + 1] = xi[u] - (lr * ui[u] + li * ur[u]);
-        }
-      }
-      for (; c < nb; c += 4) {
-        const T ur = piv[2 * c], ui = piv[2 * c + 1];
-        col[2 * (c * LDR + lane)] -= lr * ur - li * ui;
-        col[2 * (c * LDR + lane) + 1] -= lr * ui + li * ur;
       }
     }
     __syncthreads();
@@ -419,6 +524,9 @@ constexpr int SWP_COLS = 16;
 template <typename T>
 __global__ __launch_bounds__(256) void laswp_kernel(T* A, int64_t lda, int64_t c0, int64_t c1, const int* ipiv,
                                                     int64_t k0, int64_t k1) {
+  // in a sweep the factorization and solves run beside the next system's
+  // assembly: their waves issue ahead of the assembly's
+  __builtin_amdgcn_s_setprio(2);
   __shared__ int sdst[2 * MAXBW], ssrc[2 * MAXBW];
   __shared__ int4 spiv[MAXBW / 4];
   const int tid = threadIdx.x;
@@ -515,6 +623,9 @@ struct GemmCfg<double> {
 template <typename T, bool SET>
 __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
                                                     int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n, int kc) {
+  // in a sweep the factorization and solves run beside the next system's
+  // assembly: their waves issue ahead of the assembly's
+  __builtin_amdgcn_s_setprio(2);
   constexpr int BM = 128, BN = 64, KB = 16;
   constexpr int VEC = GemmCfg<T>::VEC;
   constexpr int AP = 2 * KB + VEC;  // padded row lengths in scalars
@@ -887,6 +998,9 @@ struct TriArgs {
 // awaited, so only the flag latency and the block product sit on the chain.
 template <typename T, bool FWD>
 __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
+  // in a sweep the factorization and solves run beside the next system's
+  // assembly: their waves issue ahead of the assembly's
+  __builtin_amdgcn_s_setprio(2);
   constexpr int YS = PNB * 8 + 6;  // staged block result; 2-complex pad per 16-row group
   __shared__ T ys[2 * YS];
   __shared__ T bs[2 * PNB * 8];
@@ -1020,6 +1134,9 @@ __device__ __forceinline__ void halve_step(double* v, int lane) {
 
 __global__ __launch_bounds__(512) void residual64_kernel(const double* A, int64_t n, int64_t lda, const double* x,
                                                          const double* b, double* r, int nrhs) {
+  // in a sweep the factorization and solves run beside the next system's
+  // assembly: their waves issue ahead of the assembly's
+  __builtin_amdgcn_s_setprio(2);
   __shared__ double xs[8 * RES_KC * 2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int64_t row0 = ((int64_t)blockIdx.x * RES_WAVES + wave) * RES_ROWS;
@@ -1157,8 +1274,8 @@ static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, 
   pa.nb = b;
   pa.ipiv = w.ipiv;
   pa.slots = w.slots;
-  pa.slot_rows = reinterpret_cast<T*>(w.slot_rows);
-  pa.slot_diag = reinterpret_cast<T*>(w.slot_diag);
+  pa.slot_rows = reinterpret_cast<unsigned long long*>(w.slot_rows);
+  pa.slot_diag = reinterpret_cast<unsigned long long*>(w.slot_diag);
   pa.maxg = w.maxg;
   pa.info = w.info;
   pa.linv = linv;
@@ -1200,7 +1317,12 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   const int bw = MAXBW;
   hipError_t e;
   if ((e = launch_fill<int>(w.info, 1, 0, s)) != hipSuccess) return e;
-  if ((e = launch_fill<int>(reinterpret_cast<int*>(w.slots), 8 * (int64_t)w.maxg, 0, s)) != hipSuccess) return e;
+  // slot words and published rows carry column tags: clear both
+  if ((e = launch_fill<int>(reinterpret_cast<int*>(w.slots), 2 * SLOT_WORDS * 2 * (int64_t)w.maxg, 0, s)) != hipSuccess)
+    return e;
+  if ((e = launch_fill<int>(reinterpret_cast<int*>(w.slot_rows), (int64_t)2 * w.maxg * PNB * 4 * 2, 0, s)) != hipSuccess)
+    return e;
+  if ((e = launch_fill<int>(reinterpret_cast<int*>(w.slot_diag), (int64_t)2 * PNB * 4 * 2, 0, s)) != hipSuccess) return e;
   int panels = 0;
   const T* cA = A;
   T* linv = reinterpret_cast<T*>(w.linv);

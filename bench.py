@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--counter-run", action="store_true",
                     help="for rocprofv3 --pmc: a 1 GiB device copy (counter calibration), then one complex128 "
                          "freq-solve of the workload; prints no JSON line")
+    ap.add_argument("--no-rocsolver", action="store_true",
+                    help="skip timing rocSOLVER getrf (the vendor LU) at the workload's size")
     ap.add_argument("--cpu-sample-faces", type=int, default=192)
     ap.add_argument("--cpu-lu-n", type=int, default=3072)
     return ap.parse_args()
@@ -109,6 +111,60 @@ def cpu_baseline(mesh, nrhs, sample_faces, lu_n):
                    f"{cores} threads"),
         "s_per_freq": {"assembly": asm, "lu": t_lu, "solve": t_sv},
     }
+
+
+def rocsolver_getrf_ms(n, reps=2):
+    """Best of reps timed calls of rocSOLVER cgetrf and zgetrf on an n x n N(0,1)
+    complex matrix (after one untimed call). rocSOLVER getrf is the factorization
+    the reference's ROCm path calls (rocm_dense_lu!, BeatEngineRocmOperators.jl:
+    132-159); it is timed here for comparison only and is no part of this path."""
+    import ctypes
+
+    import torch
+    libdir = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    try:
+        rb = ctypes.CDLL(os.path.join(libdir, "librocblas.so"))
+        rs = ctypes.CDLL(os.path.join(libdir, "librocsolver.so"))
+    except OSError as e:
+        return {"error": str(e)}
+    h = ctypes.c_void_p()
+    if rb.rocblas_create_handle(ctypes.byref(h)) != 0:
+        return {"error": "rocblas_create_handle failed"}
+    rb.rocblas_set_stream.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    rb.rocblas_set_stream(h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    ipiv = torch.empty(n, dtype=torch.int32, device="cuda")
+    info = torch.empty(1, dtype=torch.int32, device="cuda")
+    out = {"n": n}
+    for name, dt in (("cgetrf", torch.complex64), ("zgetrf", torch.complex128)):
+        fn = getattr(rs, "rocsolver_" + name)
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                       ctypes.c_void_p]
+        fn.restype = ctypes.c_int
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(10242)
+        a0 = torch.randn(n, n, dtype=dt, device="cuda", generator=gen)
+        a = torch.empty_like(a0)
+        times = []
+        for r in range(reps + 1):
+            a.copy_(a0)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            st = fn(h, n, n, ctypes.c_void_p(a.data_ptr()), n, ctypes.c_void_p(ipiv.data_ptr()),
+                    ctypes.c_void_p(info.data_ptr()))
+            e1.record()
+            e1.synchronize()
+            if st != 0 or int(info.item()) != 0:
+                out[name + "_error"] = f"status {st} info {int(info.item())}"
+                break
+            if r:
+                times.append(e0.elapsed_time(e1))
+        if times:
+            out[name + "_ms"] = min(times)
+        del a0, a
+    rb.rocblas_destroy_handle(h)
+    torch.cuda.empty_cache()
+    return out
 
 
 def batch_form(args):
@@ -283,7 +339,7 @@ def main():
     sec64 = el64 / max(1, args.fp64_steps)
     # refined vs FP64 at the same frequency
     step(last, _native.PREC_FP32_REFINED, p_mixed)
-    step(last, _native.PREC_FP64, p_fp64)
+    rep64 = step(last, _native.PREC_FP64, p_fp64)
     torch.cuda.synchronize()
     a = p_mixed.cpu().numpy()
     b = p_fp64.cpu().numpy()
@@ -354,6 +410,16 @@ def main():
             "frac": lu_flop / lu_s / 1e12 / FP32_MFMA_PEAK_TFLOPS,
         },
     }
+    if not args.no_rocsolver:
+        rs = rocsolver_getrf_ms(n_p1)
+        rs["ours_c64_ms"] = lu_s * 1e3
+        rs["ours_c128_ms"] = rep64.factor_s * 1e3
+        if "cgetrf_ms" in rs:
+            rs["c64_speedup"] = rs["cgetrf_ms"] / rs["ours_c64_ms"]
+        if "zgetrf_ms" in rs:
+            rs["c128_speedup"] = rs["zgetrf_ms"] / rs["ours_c128_ms"]
+        log("rocSOLVER getrf comparison:", rs)
+        out["lu_vs_rocsolver"] = rs
     if not args.no_cpu_baseline:
         cb = cpu_baseline(mesh, nrhs, args.cpu_sample_faces, args.cpu_lu_n)
         out["cpu_baseline"] = cb
