@@ -602,8 +602,8 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // C[m x n] (-)= A[m x kc] B[kc x n] for row-major interleaved complex matrices
 // (leading dimensions in complex elements), as one real GEMM on MFMA: the right
 // operand's complex entries are expanded to 2x2 real blocks [[br, bi], [-bi, br]]
-// when read from LDS. Workgroup tile 128 rows x 64 complex columns, 4 waves of
-// 64 x 64 real outputs; K staged through LDS in chunks of 16 complex with the
+// when read from LDS. Workgroup tile BM rows x 64 complex columns, 4 waves of
+// BM/2 x 64 real outputs; K staged through LDS in chunks of KB complex with the
 // next chunk prefetched into registers while the current one is multiplied.
 //   complex64:  v_mfma_f32_32x32x2_f32, lane l holds A[l%32][l/32], B[l/32][l%32],
 //               output item v of lane l is D[8(v/4) + 4(l/32) + v%4][l%32]
@@ -620,13 +620,17 @@ struct GemmCfg<double> {
   static constexpr int VEC = 2;
 };
 
-template <typename T, bool SET>
+// Two shapes: BM = 128 rows with K staged in chunks of KB = 16 (the wide
+// trailing updates), and BM = 64 with all of K <= 64 staged at once (the
+// tall-skinny updates of one leaf, on the factorization's critical path: one
+// load latency per workgroup and twice the workgroups).
+template <typename T, bool SET, int BM, int KB>
 __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
                                                     int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n, int kc) {
   // in a sweep the factorization and solves run beside the next system's
   // assembly: their waves issue ahead of the assembly's
   __builtin_amdgcn_s_setprio(2);
-  constexpr int BM = 128, BN = 64, KB = 16;
+  constexpr int BN = 64;
   constexpr int VEC = GemmCfg<T>::VEC;
   constexpr int AP = 2 * KB + VEC;  // padded row lengths in scalars
   constexpr int BP = 2 * BN + VEC;
@@ -689,10 +693,12 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
     }
   };
 
+  constexpr int WR = BM / 2;  // rows per wave (two waves along the rows, two along the columns)
   if constexpr (sizeof(T) == 4) {
-    f32x16 acc[2][2];
+    constexpr int MI = WR / 32;
+    f32x16 acc[MI][2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
       for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -705,9 +711,9 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
       if (k0 + KB < kc) load(k0 + KB);
 #pragma unroll 4
       for (int kk = 0; kk < KB; ++kk) {
-        float av[2], bv[2];
+        float av[MI], bv[2];
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi) av[mi] = As[(wm * 64 + mi * 32 + l32) * AP + 2 * kk + h];
+        for (int mi = 0; mi < MI; ++mi) av[mi] = As[(wm * WR + mi * 32 + l32) * AP + 2 * kk + h];
 #pragma unroll
         for (int ni = 0; ni < 2; ++ni) {
           const int j = wn * 64 + ni * 32 + l32;
@@ -716,7 +722,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
           bv[ni] = h == 0 ? (part == 0 ? br : bi) : (part == 0 ? -bi : br);
         }
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi)
+        for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
           for (int ni = 0; ni < 2; ++ni)
             acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
@@ -724,12 +730,12 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
       __syncthreads();
     }
 #pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
+    for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
       for (int ni = 0; ni < 2; ++ni)
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
-          const int r = wm * 64 + mi * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+          const int r = wm * WR + mi * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
           const int j = wn * 64 + ni * 32 + l32;
           const int64_t gr = row0 + r, gcr = 2 * col0 + j;
           if (gr < m && gcr < 2 * n) {
@@ -738,9 +744,10 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
           }
         }
   } else {
-    f64x4 acc[4][4];
+    constexpr int MI = WR / 16;
+    f64x4 acc[MI][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -754,9 +761,9 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
 #pragma unroll 2
       for (int kk = 0; kk < KB; kk += 2) {
         const int kcx = kk + (kq >> 1), hp = kq & 1;
-        double av[4], bv[4];
+        double av[MI], bv[4];
 #pragma unroll
-        for (int mi = 0; mi < 4; ++mi) av[mi] = As[(wm * 64 + mi * 16 + l16) * AP + 2 * kcx + hp];
+        for (int mi = 0; mi < MI; ++mi) av[mi] = As[(wm * WR + mi * 16 + l16) * AP + 2 * kcx + hp];
 #pragma unroll
         for (int ni = 0; ni < 4; ++ni) {
           const int j = wn * 64 + ni * 16 + l16;
@@ -765,7 +772,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
           bv[ni] = hp == 0 ? (part == 0 ? br : bi) : (part == 0 ? -bi : br);
         }
 #pragma unroll
-        for (int mi = 0; mi < 4; ++mi)
+        for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
           for (int ni = 0; ni < 4; ++ni)
             acc[mi][ni] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[mi], bv[ni], acc[mi][ni], 0, 0, 0);
@@ -773,12 +780,12 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
       __syncthreads();
     }
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+    for (int mi = 0; mi < MI; ++mi)
 #pragma unroll
       for (int ni = 0; ni < 4; ++ni)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int r = wm * 64 + mi * 16 + kq + 4 * v;
+          const int r = wm * WR + mi * 16 + kq + 4 * v;
           const int j = wn * 64 + ni * 16 + l16;
           const int64_t gr = row0 + r, gcr = 2 * col0 + j;
           if (gr < m && gcr < 2 * n) {
@@ -789,22 +796,31 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
   }
 }
 
+template <typename T, bool SET>
+hipError_t gemm_launch(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
+                       int kc, hipStream_t s) {
+  if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
+  if (kc <= 64) {
+    constexpr int KS = sizeof(T) == 4 ? 64 : 32;  // complex128: two chunks (LDS for two workgroups per CU)
+    dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64));
+    hipLaunchKernelGGL((zgemm_kernel<T, SET, 64, KS>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
+  } else {
+    dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 127) / 128));
+    hipLaunchKernelGGL((zgemm_kernel<T, SET, 128, 16>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t gemm_sub(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
                     int kc, hipStream_t s) {
-  if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
-  dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 127) / 128));
-  hipLaunchKernelGGL((zgemm_kernel<T, false>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
-  return hipGetLastError();
+  return gemm_launch<T, false>(A, lda, B, ldb, C, ldc, m, n, kc, s);
 }
 
 template <typename T>
 hipError_t gemm_set(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
                     int kc, hipStream_t s) {
-  if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
-  dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 127) / 128));
-  hipLaunchKernelGGL((zgemm_kernel<T, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
-  return hipGetLastError();
+  return gemm_launch<T, true>(A, lda, B, ldb, C, ldc, m, n, kc, s);
 }
 
 // ---- solves ----------------------------------------------------------------------
