@@ -22,7 +22,7 @@ REFINE_STATUS = {0: "none", 1: "converged", 2: "stagnated", 3: "step_limit", 4: 
 EXPORTS = (
     "beat_ctx_create", "beat_ctx_destroy", "beat_last_error", "beat_mesh_upload", "beat_mesh_destroy",
     "beat_mesh_counts", "beat_bm_assemble", "beat_bm_solve", "beat_bm_solve_device", "beat_field_eval",
-    "beat_dense_lu_solve", "beat_refine_lu", "beat_mesh_upload_symmetric", "beat_mesh_symmetry",
+    "beat_dense_lu_solve", "beat_dense_lu_factor", "beat_refine_lu", "beat_mesh_upload_symmetric", "beat_mesh_symmetry",
     "beat_field_gradient_eval", "beat_operators_assemble", "beat_bm_sweep_device",
     "beat_bm_sweep", "beat_ctx_set_cancel_path", "beat_ctx_reclaim", "beat_ctx_memory_info", "beat_capabilities",
     "beat_adjoint_spl_gradient", "beat_bm_batch", "beat_autotune",
@@ -121,6 +121,7 @@ def load(path: str = LIB_PATH):
     L.beat_bm_batch.argtypes = [P, I64, P, I, P, P, P, P, I64, P, ctypes.POINTER(SolveOptions), P]
     L.beat_adjoint_spl_gradient.argtypes = [P, P, I64, P, P, P, P, P, I64, D, P, P, P]
     L.beat_dense_lu_solve.argtypes = [P, I, P, I64, P, I64, P]
+    L.beat_dense_lu_factor.argtypes = [P, I, P, I64, P, P, P]
     L.beat_refine_lu.argtypes = [P, P, I64, P, I64, P, P, I, D, P, ctypes.POINTER(RefineReport)]
     if path == LIB_PATH:
         _lib = L

@@ -1844,6 +1844,73 @@ int beat_dense_lu_solve(beat_ctx* ctx, int precision, const double* a, int64_t n
   return BEAT_OK;
 }
 
+extern "C++" {
+namespace {
+// Factors of lu_factor in LAPACK getrf form: the exchanges of every later outer
+// block applied to the earlier blocks' columns, then column-major complex128 and
+// 1-based pivots.
+template <typename T>
+int dense_factor_out(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const int* ipiv_dev, double* lu_out,
+                     int32_t* ipiv_out) {
+  std::vector<T> h((size_t)2 * n * lda);
+  std::vector<int> piv((size_t)n);
+  HIPCHK(hipMemcpyAsync(h.data(), LU, sizeof(T) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(piv.data(), ipiv_dev, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t p = piv[k], cend = k / LU_BLOCK_COLS * LU_BLOCK_COLS;
+    if (p < k || p >= n) return fail(ctx, BEAT_ERR_HIP, "pivot index out of range");
+    if (p != k)
+      for (int64_t c = 0; c < 2 * cend; ++c) std::swap(h[2 * k * lda + c], h[2 * p * lda + c]);
+    ipiv_out[k] = (int32_t)(p + 1);
+  }
+  for (int64_t c = 0; c < n; ++c)
+    for (int64_t r = 0; r < n; ++r) {
+      lu_out[2 * (c * n + r)] = (double)h[2 * (r * lda + c)];
+      lu_out[2 * (c * n + r) + 1] = (double)h[2 * (r * lda + c) + 1];
+    }
+  return BEAT_OK;
+}
+}  // namespace
+}  // extern "C++"
+
+int beat_dense_lu_factor(beat_ctx* ctx, int precision, const double* a, int64_t n, double* lu_out, int32_t* ipiv_out,
+                         int32_t* info_out) {
+  if (!ctx) return BEAT_ERR_ARG;
+  if (!a || !lu_out || !ipiv_out || !info_out || n < 1) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (precision != BEAT_PREC_FP64 && precision != BEAT_PREC_FP32) return fail(ctx, BEAT_ERR_ARG, "precision");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  const int64_t lda = lda_for(n);
+  HIPCHK(ensure(ctx->qin, sizeof(double) * 2 * n * n));
+  double* ain = as<double>(ctx->qin);
+  HIPCHK(hipMemcpyAsync(ain, a, sizeof(double) * 2 * n * n, hipMemcpyHostToDevice, s));
+  LuWork lw;
+  if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
+  int panels = 0, rc;
+  if (precision == BEAT_PREC_FP64) {
+    HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
+    hipLaunchKernelGGL((transpose_in_kernel<double>), dim3(blocks_for(n * n)), dim3(256), 0, s, (const double*)ain, n,
+                       lda, as<double>(ctx->a64));
+    HIPCHK(hipGetLastError());
+    HIPCHK(lu_factor<double>(as<double>(ctx->a64), n, lda, lw, LU_NB, s, &panels));
+    rc = dense_factor_out<double>(ctx, as<double>(ctx->a64), n, lda, lw.ipiv, lu_out, ipiv_out);
+  } else {
+    HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
+    hipLaunchKernelGGL((transpose_in_kernel<float>), dim3(blocks_for(n * n)), dim3(256), 0, s, (const double*)ain, n,
+                       lda, as<float>(ctx->a32));
+    HIPCHK(hipGetLastError());
+    HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
+    rc = dense_factor_out<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, lu_out, ipiv_out);
+  }
+  if (rc != BEAT_OK) return rc;
+  int info = 0;
+  HIPCHK(hipMemcpy(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost));
+  *info_out = info;
+  return BEAT_OK;
+}
+
 int beat_refine_lu(beat_ctx* ctx, const float* matrix, int64_t n, const float* rhs, int64_t nrhs,
                    const double* residual_matrix, const double* residual_rhs, int max_steps, double rtol, float* x_out,
                    beat_refine_report* rep) {

@@ -118,6 +118,10 @@ hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s);
 hipError_t launch_downcast(const double* src, float* dst, int64_t n, hipStream_t s);
 
 // ---- dense LU (row-major complex, leading dimension lda) -------------------------
+// Outer block width of lu_factor: the row exchanges chosen in an outer block are
+// applied to that block's columns and every column right of it, never to the
+// columns of earlier blocks (the solves read those through gather tables).
+constexpr int LU_BLOCK_COLS = 256;
 struct LuWork {
   int* ipiv;          // n, global row index chosen at each column
   unsigned long long* slots;  // 2 x maxg slots of 16 words (tag << 32 | row, tagged value, padding)

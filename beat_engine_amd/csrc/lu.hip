@@ -32,7 +32,7 @@ namespace {
 
 constexpr int PNB = 64;   // max panel width
 constexpr int RPW = 64;   // panel rows per workgroup
-constexpr int MAXBW = 256; // outer block width (columns factored before the wide trailing update)
+constexpr int MAXBW = LU_BLOCK_COLS; // outer block width (columns factored before the wide trailing update)
 constexpr int SLOT_WORDS = 16;  // one exchange slot per 128-byte line
 
 template <typename T>

@@ -163,6 +163,20 @@ class Engine:
         self._check(rc, "beat_dense_lu_solve")
         return X[:, 0] if vec else X
 
+    def dense_lu_factor(self, A, precision="float64"):
+        """getrf on the GPU: (lu, piv, info) with LAPACK's layout (scipy.linalg.lu_factor's
+        lu and 0-based piv) and info (first exactly zero pivot, 1-based, or 0)."""
+        A = np.asfortranarray(A, dtype=np.complex128)
+        n = A.shape[0]
+        if A.ndim != 2 or A.shape[1] != n:
+            raise ValueError("matrix must be square")
+        lu = np.zeros((n, n), dtype=np.complex128, order="F")
+        piv = np.zeros(n, dtype=np.int32)
+        info = np.zeros(1, dtype=np.int32)
+        rc = self._lib.beat_dense_lu_factor(self._h, _precision(precision), _ptr(A), n, _ptr(lu), _ptr(piv), _ptr(info))
+        self._check(rc, "beat_dense_lu_factor")
+        return lu, piv.astype(np.int64) - 1, int(info[0])
+
     def refine_lu(self, matrix, rhs, residual_matrix=None, residual_rhs=None, max_steps=3, rtol=1e-10):
         if not (1 <= max_steps <= 3):
             raise ValueError("max_steps must be in 1:3")

@@ -256,6 +256,17 @@ int beat_adjoint_spl_gradient(beat_ctx* ctx, const beat_mesh* mesh, int64_t nfre
 int beat_dense_lu_solve(beat_ctx* ctx, int precision, const double* a, int64_t n, const double* b, int64_t nrhs,
                         double* x);
 
+/* LU factorization with partial pivoting alone, in LAPACK getrf form: the
+ * factorization rocm_dense_lu! (BeatEngineRocmOperators.jl:160-191) obtains from
+ * lu!(working), i.e. rocSOLVER getrf. a: n x n complex128 column-major; lu_out: the
+ * unit-lower L and U in place (n x n complex128 column-major, L and U rounded from
+ * complex64 for precision FP32); ipiv_out: n 1-based row exchanges (row k with
+ * ipiv_out[k]); info_out: LAPACK's info (first exactly zero pivot, 1-based, or 0;
+ * the factorization still completes). Pivot rows are chosen by |re| + |im|, the
+ * first maximum on ties (izamax). */
+int beat_dense_lu_factor(beat_ctx* ctx, int precision, const double* a, int64_t n, double* lu_out, int32_t* ipiv_out,
+                         int32_t* info_out);
+
 /* beat_refine_cpu_lu (BeatEngineCpuRefinement.jl:5-78) on the GPU: complex64 LU of
  * `matrix`, complex128 residuals with residual_matrix / residual_rhs (NULL: the
  * complex64 inputs widened), per-column acceptance. matrix, rhs, x_out are

@@ -56,6 +56,62 @@ def test_dense_lu_needs_pivoting(eng):
     assert np.allclose(A @ X, B, atol=1e-13)
 
 
+def _swap_rows(A, piv):
+    P = np.array(A, copy=True)
+    for k, p in enumerate(piv):
+        if p != k:
+            P[[k, p]] = P[[p, k]]
+    return P
+
+
+@pytest.mark.parametrize("n", [3, 64, 65, 200, 333, 777, 1300])
+def test_dense_lu_factor_matches_lapack_getrf(eng, n):
+    """complex128 factors against LAPACK zgetrf (scipy.linalg.lu_factor; OpenBLAS
+    0.3.29, the version the reference pins): the same pivot sequence and factors
+    within 1e-12. n = 777 and 1300 span several 256-column outer blocks."""
+    import scipy.linalg
+    rng = np.random.default_rng(100 + n)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    lu, piv, info = eng.dense_lu_factor(A, "float64")
+    lu_r, piv_r = scipy.linalg.lu_factor(A)
+    assert info == 0
+    assert np.array_equal(piv, piv_r)
+    assert rel(lu, lu_r) < 1e-12
+
+
+@pytest.mark.parametrize("n", [200, 777, 1300])
+def test_dense_lu_factor_complex64_backward_error(eng, n):
+    """complex64 factors: P A = L U to FP32 backward error (pivot ties at FP32
+    rounding may resolve differently from cgetrf, so the sequence is not compared)."""
+    rng = np.random.default_rng(200 + n)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    lu, piv, info = eng.dense_lu_factor(A, "float32")
+    assert info == 0
+    L = np.tril(lu, -1) + np.eye(n)
+    U = np.triu(lu)
+    assert rel(L @ U, _swap_rows(A.astype(np.complex64).astype(np.complex128), piv)) < 2e-6 * np.sqrt(n)
+    # pivots maximise |re| + |im|, which bounds every multiplier's modulus by sqrt(2)
+    assert np.max(np.abs(np.tril(lu, -1))) <= np.sqrt(2.0) * (1 + 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_dense_lu_factor_pivot_measure_and_zero_pivot(eng, precision):
+    """izamax's |re| + |im| with the first maximum on ties: 3+4i and 7 tie (|z| would
+    pick 7); an exactly zero column gives LAPACK's info and still completes."""
+    import scipy.linalg
+    A = np.array([[3 + 4j, 1, 2], [7, 2, 1], [1, 5, 3]], dtype=complex)
+    lu, piv, info = eng.dense_lu_factor(A, precision)
+    assert piv[0] == 0 and info == 0
+    assert np.array_equal(piv, scipy.linalg.lu_factor(A)[1])
+    Z = np.array([[1, 2, 0], [3, 4, 0], [5, 6, 0]], dtype=complex)
+    lu, piv, info = eng.dense_lu_factor(Z, precision)
+    assert info == 3
+    with pytest.warns(scipy.linalg.LinAlgWarning):
+        lu_r, piv_r = scipy.linalg.lu_factor(Z)
+    assert np.array_equal(piv, piv_r)
+    assert np.allclose(lu, lu_r, atol=1e-6)
+
+
 def _case(mesh, f=700.0, nrhs=3, seed=2):
     w = 2 * np.pi * f
     k = w / 343.0
