@@ -72,8 +72,10 @@ struct beat_ctx {
   hipEvent_t sw_ev[2][4] = {};
   hipEvent_t sw_free[2] = {};
   // the regular kernel at one workgroup per CU while it shares the device with
-  // an LU (BEAT_SWEEP_ASM_ONE_PER_CU=0 lets it take every slot)
-  bool sweep_asm_one_per_cu = true;
+  // an LU (BEAT_SWEEP_ASM_ONE_PER_CU=1). Off by default: the LU's kernels issue at
+  // a higher wave priority, so the assembly may take every slot the LU leaves
+  // (measured at L5: 11.2 vs 10.7 freq-solves/s)
+  bool sweep_asm_one_per_cu = false;
   // contexts a batch may use at once (BEAT_BATCH_WORKERS, default 4)
   int batch_workers = 4;
   // cancellation marker polled between the frequencies of a sweep (empty: none)
