@@ -104,6 +104,8 @@ struct beat_mesh {
   int32_t *tile_start = nullptr, *tile_count = nullptr, *tile_nv = nullptr, *tile_verts = nullptr;
   int32_t *stile_start = nullptr, *stile_count = nullptr, *stile_nv = nullptr, *stile_verts = nullptr;
   uint8_t* flocs = nullptr;
+  int32_t *ntile_start = nullptr, *ntile_count = nullptr, *ntile_nv = nullptr, *ntile_verts = nullptr;
+  uint8_t* flocn = nullptr;
   SingularPair* spairs = nullptr;
   int32_t *rule_off = nullptr, *rule_cnt = nullptr;
   double* rule_data = nullptr;
@@ -480,8 +482,19 @@ float* mesh_rec<float>(const beat_mesh* m) {
   return m->rec32;
 }
 
-// Assemble A (row-major, lda) and B (n x nrhs row-major) in precision T from the
-// internal-order q64 (and q32 for T = float).
+// The trial tile partition of the regular pass: wide, or narrow (beside a
+// factorization).
+template <typename T>
+void set_trial_tiles(const beat_mesh* m, bool narrow, RegularArgs<T>& ra) {
+  ra.narrow = narrow ? 1 : 0;
+  ra.flocs = narrow ? m->flocn : m->flocs;
+  ra.stile_start = narrow ? m->ntile_start : m->stile_start;
+  ra.stile_count = narrow ? m->ntile_count : m->stile_count;
+  ra.stile_nv = narrow ? m->ntile_nv : m->stile_nv;
+  ra.stile_verts = narrow ? m->ntile_verts : m->stile_verts;
+  ra.nstiles = (int)(narrow ? m->h.ntile_start.size() : m->h.stile_start.size());
+}
+
 // Geometry, tiles, rule, wavenumber and images of the regular pass.
 template <typename T>
 void regular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, RegularArgs<T>& ra) {
@@ -493,12 +506,7 @@ void regular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, Regular
   ra.tile_nv = m->tile_nv;
   ra.tile_verts = m->tile_verts;
   ra.ntiles = (int)m->h.tile_start.size();
-  ra.flocs = m->flocs;
-  ra.stile_start = m->stile_start;
-  ra.stile_count = m->stile_count;
-  ra.stile_nv = m->stile_nv;
-  ra.stile_verts = m->stile_verts;
-  ra.nstiles = (int)m->h.stile_start.size();
+  set_trial_tiles(m, false, ra);
   ra.k = (T)k;
   ra.krev = (T)(k / (2.0 * M_PI));
   ra.trig = ctx->trig;
@@ -543,11 +551,12 @@ void singular_geometry(const beat_mesh* m, double k, SingularArgs<TO>& sa) {
 // Assemble A (row-major, lda) and B (n x nrhs row-major) in precision T from the
 // internal-order q64 (and q32 for T = float).
 // st: the stream; ev_reg0 / ev_reg1 (may be null) bracket the regular passes;
-// one_per_cu: run the regular kernel at one workgroup per CU.
+// one_per_cu: run the regular kernel at one workgroup per CU; narrow: its narrow
+// trial tiles (the assembly runs beside a factorization).
 template <typename T>
 int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, const double* q64, const T* qT,
              int nrhs, T* A, int64_t lda, T* B, hipStream_t st, hipEvent_t ev_reg0, hipEvent_t ev_reg1,
-             bool one_per_cu = false) {
+             bool one_per_cu = false, bool narrow = false) {
   const int64_t n = m->h.nv;
   HIPCHK(launch_fill<T>(A, 2 * n * lda, T(0), st));
   HIPCHK(launch_fill<T>(B, 2 * n * nrhs, T(0), st));
@@ -563,7 +572,9 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   ra.one_per_cu = one_per_cu ? 1 : 0;
   if (ev_reg0) HIPCHK(hipEventRecord(ev_reg0, st));
   const bool wide = !(std::fabs(k) * m->diameter < SINCOS_TABLE_LIMIT);
+  set_trial_tiles(m, narrow, ra);
   HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, false, st));
+  set_trial_tiles(m, false, ra);  // image passes: the wide partition
   if (m->h.nimg > 0) HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, true, st));
   if (ev_reg1) HIPCHK(hipEventRecord(ev_reg1, st));
   SingularArgs<T> sa;
@@ -667,7 +678,7 @@ struct AsmEvents {
 // refined path, complex64 for FP32.
 int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei,
                    const double* q_dev, int nrhs, SystemBufs& sb, hipStream_t st, const AsmEvents& ev,
-                   bool one_per_cu = false) {
+                   bool one_per_cu = false, bool narrow = false) {
   const int64_t n = m->h.nv, nf = m->h.nf, lda = lda_for(n);
   HIPCHK(hipEventRecord(ev.start, st));
   HIPCHK(ensure(*sb.q64, sizeof(double) * 2 * nf * nrhs));
@@ -680,13 +691,14 @@ int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, d
     HIPCHK(ensure(*sb.a32, sizeof(float) * 2 * n * lda));
     HIPCHK(ensure(*sb.b32, sizeof(float) * 2 * n * nrhs));
     if (int rc = assemble<float>(ctx, m, k, er, ei, as<double>(*sb.q64), as<float>(*sb.q32), nrhs, as<float>(*sb.a32),
-                                 lda, as<float>(*sb.b32), st, ev.reg0, ev.reg1, one_per_cu))
+                                 lda, as<float>(*sb.b32), st, ev.reg0, ev.reg1, one_per_cu, narrow))
       return rc;
   } else {
     HIPCHK(ensure(*sb.a64, sizeof(double) * 2 * n * lda));
     HIPCHK(ensure(*sb.b64, sizeof(double) * 2 * n * nrhs));
     if (int rc = assemble<double>(ctx, m, k, er, ei, as<double>(*sb.q64), as<double>(*sb.q64), nrhs,
-                                  as<double>(*sb.a64), lda, as<double>(*sb.b64), st, ev.reg0, ev.reg1, one_per_cu))
+                                  as<double>(*sb.a64), lda, as<double>(*sb.b64), st, ev.reg0, ev.reg1, one_per_cu,
+                                  narrow))
       return rc;
   }
   HIPCHK(hipEventRecord(ev.end, st));
@@ -856,11 +868,12 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
     const int s2 = (int)(i & 1);
     // the system's previous solve (job i - 2) must be done with its buffers
     if (i >= 2) HIPCHK(hipStreamWaitEvent(ctx->asm_stream, ctx->sw_free[s2], 0));
-    // the first job's assembly has nothing beside it: every slot
+    // the first job's assembly has nothing beside it: every slot, wide tiles;
+    // later ones run beside a factorization: narrow tiles
     const signed char occ = ctx->asm_occ[size_class(meshes[i]->h.nv)];
     const bool one_per_cu = i > 0 && (occ < 0 ? ctx->sweep_asm_one_per_cu : occ == 1);
     return stage_assemble(ctx, meshes[i], precision, ks[i], ers[i], eis[i], q_dev + 2 * q_off[i], nrhs, sb[s2],
-                          ctx->asm_stream, ev[s2], one_per_cu);
+                          ctx->asm_stream, ev[s2], one_per_cu, i > 0);
   };
   // the reference's marker-file cancellation, checked before each job
   // (cancel_requested, BeatEngineCompiledDriver.jl:1199-1200,1270)
@@ -1265,6 +1278,11 @@ int beat_mesh_upload_symmetric(beat_ctx* ctx, const double* verts, int64_t nv, c
   if (e == hipSuccess) e = upload(&m->stile_nv, m->h.stile_nv);
   if (e == hipSuccess) e = upload(&m->stile_verts, m->h.stile_verts);
   if (e == hipSuccess) e = upload(&m->flocs, m->h.flocs);
+  if (e == hipSuccess) e = upload(&m->ntile_start, m->h.ntile_start);
+  if (e == hipSuccess) e = upload(&m->ntile_count, m->h.ntile_count);
+  if (e == hipSuccess) e = upload(&m->ntile_nv, m->h.ntile_nv);
+  if (e == hipSuccess) e = upload(&m->ntile_verts, m->h.ntile_verts);
+  if (e == hipSuccess) e = upload(&m->flocn, m->h.flocn);
   if (e == hipSuccess) e = upload(&m->spairs, m->h.spairs);
   if (e == hipSuccess) e = upload(&m->rule_off, m->h.rule_off);
   if (e == hipSuccess) e = upload(&m->rule_cnt, m->h.rule_cnt);
@@ -1314,7 +1332,8 @@ int beat_mesh_upload(beat_ctx* ctx, const double* verts, int64_t nv, const int32
 void beat_mesh_destroy(beat_mesh* m) {
   if (!m) return;
   void* ptrs[] = {m->rec64, m->rec32, m->verts, m->fverts, m->floc, m->tile_start, m->tile_count, m->tile_nv,
-                  m->tile_verts, m->stile_start, m->stile_count, m->stile_nv, m->stile_verts, m->flocs, m->spairs,
+                  m->tile_verts, m->stile_start, m->stile_count, m->stile_nv, m->stile_verts, m->flocs,
+                  m->ntile_start, m->ntile_count, m->ntile_nv, m->ntile_verts, m->flocn, m->spairs,
                   m->rule_off, m->rule_cnt, m->rule_data, m->rule3, m->rule4, m->face_perm, m->vert_inv, m->wrows,
                   m->wvals};
   if (m->ctx) (void)hipSetDevice(m->ctx->device);

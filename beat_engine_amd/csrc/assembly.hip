@@ -74,17 +74,17 @@ __device__ __forceinline__ double inv_sqrt(double x) {
 // addresses (scalar loads); 3x3 blocks go to the LDS tile accA[test vertex slot]
 // [trial vertex slot] and the P1 x DP0 blocks to accC[test vertex slot][trial
 // face]; both are flushed once with global atomics. Two workgroups fit per CU.
-template <typename T, int NQ, bool PURE, bool WIDE, bool IMAGE>
+template <typename T, int NQ, bool PURE, bool WIDE, bool IMAGE, int TFS = TILE_FS, int TVS = TILE_VS>
 __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T> a) {
   // rows padded by one scalar: the 64 lanes add into rows tl (varying) at the same
   // column, which without the pad would all fall into one LDS bank
-  constexpr int LDA_ = 2 * TILE_VS + 1, LDC_ = 2 * TILE_FS + 1;
+  constexpr int LDA_ = 2 * TVS + 1, LDC_ = 2 * TFS + 1;
   __shared__ T accA[TILE_V * LDA_];
   __shared__ T accC[TILE_V * LDC_];
   __shared__ T xs[NQ * 3 * 64];  // test points, [point*3 + c][lane]
   __shared__ T sphw[6 * 4];      // per rule point: phi_0, phi_1, phi_2, w
   __shared__ double trig[2 * TRIG_N];  // (sin, cos)(m pi/64)
-  __shared__ int grow[TILE_V], gcol[TILE_VS];
+  __shared__ int grow[TILE_V], gcol[TVS];
 
   const int64_t total = (int64_t)a.ntiles * a.nstiles;
   const int64_t per = (total + 7) / 8;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
   for (int i = tid; i < TILE_V * LDA_; i += blockDim.x) accA[i] = T(0);
   for (int i = tid; i < TILE_V * LDC_; i += blockDim.x) accC[i] = T(0);
   if (tid < TILE_V) grow[tid] = a.tile_verts[(int64_t)tt * TILE_V + tid];
-  if (tid < TILE_VS) gcol[tid] = a.stile_verts[(int64_t)st * TILE_VS + tid];
+  if (tid < TVS) gcol[tid] = a.stile_verts[(int64_t)st * TVS + tid];
   for (int i = tid; i < 2 * TRIG_N; i += blockDim.x) trig[i] = a.trig[i];
   if (tid == 0) {
 #pragma unroll
@@ -774,21 +774,45 @@ __global__ void downcast_kernel(const double* src, float* dst, int64_t n) {
 
 // ---- launchers -------------------------------------------------------------------
 
+// Dynamic LDS that lifts a workgroup's LDS above half of a CU's 160 KB, so that
+// one workgroup of the kernel runs per CU (0 if the query fails: every slot).
+template <typename K>
+static unsigned one_per_cu_pad(K kernel) {
+  hipFuncAttributes at;
+  if (hipFuncGetAttributes(&at, reinterpret_cast<const void*>(kernel)) != hipSuccess) return 0u;
+  constexpr size_t half = 80 * 1024;
+  return at.sharedSizeBytes < half + 1024 ? (unsigned)(half + 1024 - at.sharedSizeBytes) : 0u;
+}
+
+template <typename T, int NQ, bool WIDE, bool IMAGE, int TFS, int TVS>
+static void launch_regular_t(const RegularArgs<T>& a, bool pure, dim3 grid, dim3 block, hipStream_t s) {
+  if (pure) {
+    const auto k = bm_regular_kernel<T, NQ, true, WIDE, IMAGE, TFS, TVS>;
+    hipLaunchKernelGGL(k, grid, block, a.one_per_cu ? one_per_cu_pad(k) : 0u, s, a);
+  } else {
+    const auto k = bm_regular_kernel<T, NQ, false, WIDE, IMAGE, TFS, TVS>;
+    hipLaunchKernelGGL(k, grid, block, a.one_per_cu ? one_per_cu_pad(k) : 0u, s, a);
+  }
+}
+
+// the kernel's tile shape follows the trial partition the arguments carry (the
+// image passes always carry the wide one)
+template <typename T, int NQ, bool WIDE, bool IMAGE>
+static void launch_regular_q(const RegularArgs<T>& a, bool pure, dim3 grid, dim3 block, hipStream_t s) {
+  if constexpr (!IMAGE) {
+    if (a.narrow) {
+      launch_regular_t<T, NQ, WIDE, IMAGE, TILE_FS_N, TILE_VS_N>(a, pure, grid, block, s);
+      return;
+    }
+  }
+  launch_regular_t<T, NQ, WIDE, IMAGE, TILE_FS, TILE_VS>(a, pure, grid, block, s);
+}
+
 template <typename T, bool WIDE, bool IMAGE>
 static void launch_regular_nq(const RegularArgs<T>& a, int nq, bool pure, dim3 grid, dim3 block, hipStream_t s) {
-  // one workgroup per CU: unused dynamic LDS lifts the workgroup's LDS above half a
-  // CU's 160 KB (static use: about 78 KB in FP64, 40 KB in FP32)
-  const unsigned pad = a.one_per_cu ? (sizeof(T) == 8 ? 4096u : 45056u) : 0u;
-  if (nq == 6) {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 6, true, WIDE, IMAGE>), grid, block, pad, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 6, false, WIDE, IMAGE>), grid, block, pad, s, a);
-  } else if (nq == 3) {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 3, true, WIDE, IMAGE>), grid, block, pad, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 3, false, WIDE, IMAGE>), grid, block, pad, s, a);
-  } else {
-    if (pure) hipLaunchKernelGGL((bm_regular_kernel<T, 1, true, WIDE, IMAGE>), grid, block, pad, s, a);
-    else hipLaunchKernelGGL((bm_regular_kernel<T, 1, false, WIDE, IMAGE>), grid, block, pad, s, a);
-  }
+  if (nq == 6) launch_regular_q<T, 6, WIDE, IMAGE>(a, pure, grid, block, s);
+  else if (nq == 3) launch_regular_q<T, 3, WIDE, IMAGE>(a, pure, grid, block, s);
+  else launch_regular_q<T, 1, WIDE, IMAGE>(a, pure, grid, block, s);
 }
 
 template <typename T, bool IMAGE>
@@ -807,6 +831,7 @@ static void launch_regular_range(const RegularArgs<T>& a, int nq, bool pure, boo
 
 template <typename T>
 hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure, bool wide, bool image, hipStream_t s) {
+  if (image && a.narrow) return hipErrorInvalidValue;
   const int64_t total = (int64_t)a.ntiles * a.nstiles;
   const int64_t per = (total + 7) / 8;
   if (image && (a.nimg < 1 || a.nimg > MAX_IMAGES)) return hipErrorInvalidValue;

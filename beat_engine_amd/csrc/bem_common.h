@@ -12,9 +12,14 @@ namespace beat {
 constexpr int TILE_F = 64;
 constexpr int TILE_V = 64;
 // Trial tiles of the regular assembly: up to TILE_FS faces touching at most
-// TILE_VS distinct vertices (a second partition of the same face order).
+// TILE_VS distinct vertices (a second partition of the same face order). A third,
+// narrow partition (TILE_FS_N / TILE_VS_N) halves the regular kernel's LDS, so
+// that a factorization's panel workgroup fits on a CU beside two assembly
+// workgroups when a sweep assembles the next system during a factorization.
 constexpr int TILE_FS = 32;
 constexpr int TILE_VS = 32;
+constexpr int TILE_FS_N = 16;
+constexpr int TILE_VS_N = 16;
 // Per-face geometry record (FREC scalars): regular quadrature points (q*3+c, up
 // to 6 points), unit normal, surface curls (a*3+c), area, plane offset v1.n.
 constexpr int FREC = 32;
@@ -79,6 +84,9 @@ struct HostMesh {
   std::vector<int32_t> stile_start, stile_count, stile_nv;
   std::vector<int32_t> stile_verts;  // TILE_VS per trial tile
   std::vector<uint8_t> flocs;        // 3 per internal face: vertex slot in its trial tile
+  std::vector<int32_t> ntile_start, ntile_count, ntile_nv;
+  std::vector<int32_t> ntile_verts;  // TILE_VS_N per narrow trial tile
+  std::vector<uint8_t> flocn;        // 3 per internal face: vertex slot in its narrow trial tile
   std::vector<SingularPair> spairs;
   std::vector<int32_t> rule_off, rule_cnt, rule_kind;
   std::vector<double> rule_data;  // 5 per rule point: test xi, eta, trial xi, eta, weight

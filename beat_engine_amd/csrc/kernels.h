@@ -22,8 +22,9 @@ struct RegularArgs {
   const int32_t* stile_start;
   const int32_t* stile_count;
   const int32_t* stile_nv;
-  const int32_t* stile_verts;  // TILE_VS per trial tile
+  const int32_t* stile_verts;  // TILE_VS (narrow: TILE_VS_N) per trial tile
   int nstiles;
+  int narrow;              // the trial tile arrays are the narrow partition
   T* A;                    // row-major complex, lda
   int64_t lda;
   T* B;                    // N x nrhs row-major complex

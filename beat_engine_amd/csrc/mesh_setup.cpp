@@ -229,7 +229,8 @@ std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64
     for (int k = 0; k < 3; ++k) m.fverts[3 * i + k] = m.vert_inv[F[3 * m.face_perm[i] + k]];
 
   // --- tiles: test tiles (TILE_F faces / TILE_V vertices) and trial tiles
-  // (TILE_FS / TILE_VS), both greedy along the face order
+  // (TILE_FS / TILE_VS, and the narrow TILE_FS_N / TILE_VS_N), all greedy along
+  // the face order
   auto tiles = [&](int maxf, int maxv, std::vector<int32_t>& st, std::vector<int32_t>& cnt,
                    std::vector<int32_t>& tnv, std::vector<int32_t>& tverts, std::vector<uint8_t>& loc) {
     st.clear();
@@ -272,6 +273,7 @@ std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64
   };
   tiles(TILE_F, TILE_V, m.tile_start, m.tile_count, m.tile_nv, m.tile_verts, m.floc);
   tiles(TILE_FS, TILE_VS, m.stile_start, m.stile_count, m.stile_nv, m.stile_verts, m.flocs);
+  tiles(TILE_FS_N, TILE_VS_N, m.ntile_start, m.ntile_count, m.ntile_nv, m.ntile_verts, m.flocn);
 
   // --- face records (internal order)
   m.rec.assign((size_t)FREC * nf, 0.0);
