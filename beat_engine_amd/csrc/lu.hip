@@ -622,10 +622,13 @@ struct GemmCfg<double> {
 
 template <typename T, bool SET, int BM, int KB>
 __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
-                                                    int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n, int kc) {
+                                                    int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n, int kc,
+                                                    int behind) {
   // in a sweep the factorization and solves run beside the next system's
-  // assembly: their waves issue ahead of the assembly's
-  __builtin_amdgcn_s_setprio(2);
+  // assembly: their waves issue ahead of the assembly's; the look-ahead update
+  // (behind) issues after the factorization's critical path
+  if (behind) __builtin_amdgcn_s_setprio(1);
+  else __builtin_amdgcn_s_setprio(2);
   constexpr int BN = 64;
   constexpr int VEC = GemmCfg<T>::VEC;
   constexpr int AP = 2 * KB + VEC;  // padded row lengths in scalars
@@ -794,7 +797,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
 
 template <typename T, bool SET>
 hipError_t gemm_launch(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
-                       int kc, hipStream_t s) {
+                       int kc, hipStream_t s, bool behind) {
   if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
   // 64 x 64-complex tiles, K in chunks of 16. Measured on gfx950 at the LU's
   // update shapes against 128-row tiles (chunks of 16 or 32), 256-row tiles and
@@ -803,20 +806,21 @@ hipError_t gemm_launch(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, i
   // and chunks of 16 keep the LDS small enough that a panel workgroup still fits
   // on a CU that runs the update's workgroups at full occupancy.
   dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64));
-  hipLaunchKernelGGL((zgemm_kernel<T, SET, 64, 16>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc);
+  hipLaunchKernelGGL((zgemm_kernel<T, SET, 64, 16>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc,
+                     behind ? 1 : 0);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t gemm_sub(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
-                    int kc, hipStream_t s) {
-  return gemm_launch<T, false>(A, lda, B, ldb, C, ldc, m, n, kc, s);
+                    int kc, hipStream_t s, bool behind = false) {
+  return gemm_launch<T, false>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind);
 }
 
 template <typename T>
 hipError_t gemm_set(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
-                    int kc, hipStream_t s) {
-  return gemm_launch<T, true>(A, lda, B, ldb, C, ldc, m, n, kc, s);
+                    int kc, hipStream_t s, bool behind = false) {
+  return gemm_launch<T, true>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind);
 }
 
 // ---- solves ----------------------------------------------------------------------
@@ -1357,21 +1361,22 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   // block [J, Je)'s update of columns [c0, c1): swaps, U12 by leaf inverses, GEMM
   auto update = [&](int64_t J, int64_t Je, int64_t c0, int64_t c1, hipStream_t st) -> hipError_t {
     if (c1 <= c0) return hipSuccess;
+    const bool behind = look_ahead && st == w.side;  // the look-ahead's far update
     hipError_t r;
     if ((r = launch_laswp<T>(A, lda, c0, c1, w.ipiv, J, Je, st)) != hipSuccess) return r;
     for (int64_t j0 = J; j0 < Je; j0 += nb) {
       const int b = (int)std::min<int64_t>(nb, Je - j0);
       r = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + c0), lda, A + 2 * (j0 * lda + c0), lda, b, c1 - c0,
-                      b, st);
+                      b, st, behind);
       if (r != hipSuccess) return r;
       if (j0 + b < Je) {
         r = gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + c0), lda,
-                        A + 2 * ((j0 + b) * lda + c0), lda, Je - j0 - b, c1 - c0, b, st);
+                        A + 2 * ((j0 + b) * lda + c0), lda, Je - j0 - b, c1 - c0, b, st, behind);
         if (r != hipSuccess) return r;
       }
     }
     return gemm_sub<T>(cA + 2 * (Je * lda + J), lda, cA + 2 * (J * lda + c0), lda, A + 2 * (Je * lda + c0), lda,
-                       n - Je, c1 - c0, (int)(Je - J), st);
+                       n - Je, c1 - c0, (int)(Je - J), st, behind);
   };
   for (int64_t J = 0; J < n; J += bw) {
     const int64_t Je = std::min<int64_t>(n, J + bw);
