@@ -675,7 +675,8 @@ struct AsmEvents {
 
 // Stage 1 (stream st): gather q_dev (caller column-major complex128) into internal
 // order and assemble the system of the precision: complex128 for FP64 and the
-// refined path, complex64 for FP32.
+// refined path (plus the complex64 copy the refined path factors), complex64 for
+// FP32.
 int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei,
                    const double* q_dev, int nrhs, SystemBufs& sb, hipStream_t st, const AsmEvents& ev,
                    bool one_per_cu = false, bool narrow = false) {
@@ -700,6 +701,14 @@ int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, d
                                   as<double>(*sb.a64), lda, as<double>(*sb.b64), st, ev.reg0, ev.reg1, one_per_cu,
                                   narrow))
       return rc;
+    if (precision == BEAT_PREC_FP32_REFINED) {
+      // the complex64 copy the refined path factors, made here so that in a sweep
+      // it is off the factorization's stream
+      HIPCHK(ensure(*sb.a32, sizeof(float) * 2 * n * lda));
+      HIPCHK(ensure(*sb.b32, sizeof(float) * 2 * n * nrhs));
+      HIPCHK(launch_downcast(as<double>(*sb.a64), as<float>(*sb.a32), 2 * n * lda, st));
+      HIPCHK(launch_downcast(as<double>(*sb.b64), as<float>(*sb.b32), 2 * n * nrhs, st));
+    }
   }
   HIPCHK(hipEventRecord(ev.end, st));
   return BEAT_OK;
@@ -742,11 +751,7 @@ int stage_solve(beat_ctx* ctx, const beat_mesh* m, int precision, int nrhs, Syst
                        (const float*)as<float>(*sb.b32), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
     HIPCHK(hipGetLastError());
   } else {
-    HIPCHK(ensure(*sb.a32, sizeof(float) * 2 * n * lda));
-    HIPCHK(ensure(*sb.b32, sizeof(float) * 2 * n * nrhs));
     HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * n * nrhs));
-    HIPCHK(launch_downcast(as<double>(*sb.a64), as<float>(*sb.a32), 2 * n * lda, s));
-    HIPCHK(launch_downcast(as<double>(*sb.b64), as<float>(*sb.b32), 2 * n * nrhs, s));
     HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
     int rc = solve_all<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs, true);
@@ -825,9 +830,8 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
   if (njobs <= 0) return BEAT_OK;
   SystemBufs sb[2];
   for (int s2 = 0; s2 < 2; ++s2) {
-    sb[s2] = SystemBufs{&ctx->sw_q64[s2], &ctx->sw_q32[s2], &ctx->sw_a64[s2], &ctx->sw_b64[s2],
-                        precision == BEAT_PREC_FP32 ? &ctx->sw_a32[s2] : &ctx->a32,
-                        precision == BEAT_PREC_FP32 ? &ctx->sw_b32[s2] : &ctx->b32};
+    sb[s2] = SystemBufs{&ctx->sw_q64[s2], &ctx->sw_q32[s2], &ctx->sw_a64[s2],
+                        &ctx->sw_b64[s2], &ctx->sw_a32[s2], &ctx->sw_b32[s2]};
   }
   // size both systems for the largest job up front (no reallocation while in flight)
   int64_t nmax = 0, fmax = 0;
@@ -837,11 +841,12 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
   }
   for (int s2 = 0; s2 < 2; ++s2) {
     HIPCHK(ensure(*sb[s2].q64, sizeof(double) * 2 * fmax * nrhs));
-    if (precision == BEAT_PREC_FP32) {
-      HIPCHK(ensure(*sb[s2].q32, sizeof(float) * 2 * fmax * nrhs));
+    if (precision == BEAT_PREC_FP32) HIPCHK(ensure(*sb[s2].q32, sizeof(float) * 2 * fmax * nrhs));
+    if (precision != BEAT_PREC_FP64) {
       HIPCHK(ensure(*sb[s2].a32, sizeof(float) * 2 * nmax * lda_for(nmax)));
       HIPCHK(ensure(*sb[s2].b32, sizeof(float) * 2 * nmax * nrhs));
-    } else {
+    }
+    if (precision != BEAT_PREC_FP32) {
       HIPCHK(ensure(*sb[s2].a64, sizeof(double) * 2 * nmax * lda_for(nmax)));
       HIPCHK(ensure(*sb[s2].b64, sizeof(double) * 2 * nmax * nrhs));
     }
@@ -852,8 +857,6 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
     if (ensure_lu_work<double>(ctx, nmax, lw) != BEAT_OK) return BEAT_ERR_HIP;
     HIPCHK(ensure(ctx->work, lu_solve_work_bytes(nmax, sizeof(double))));
     if (precision == BEAT_PREC_FP32_REFINED) {
-      HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * nmax * lda_for(nmax)));
-      HIPCHK(ensure(ctx->b32, sizeof(float) * 2 * nmax * nrhs));
       HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * nmax * nrhs));
       HIPCHK(ensure(ctx->r64, sizeof(double) * 2 * nmax * nrhs));
       HIPCHK(ensure(ctx->n64, sizeof(double) * 2 * nmax * nrhs));

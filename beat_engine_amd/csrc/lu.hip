@@ -920,51 +920,49 @@ __global__ __launch_bounds__(64) void diag_inverse_kernel(const T* LU, int64_t n
 // swap them), while the solve runs in the final order. For final row r,
 // G[J][r] is the row it occupied after block J's exchanges, found by undoing the
 // exchanges of blocks J+1.. in reverse; perm[r] is its original row.
-constexpr int GT_CHUNK = 8192;  // exchanges staged in LDS per round (a multiple of the block width)
-__global__ __launch_bounds__(256) void gather_table_kernel(const int* ipiv, int64_t n, int bw, int nJ, int* G,
-                                                           int* perm) {
-  __shared__ int4 sp[GT_CHUNK / 4];
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int q = (int)r;
-  int J = nJ - 1;
-  // chunks from the last exchanges backwards; each holds whole blocks
-  for (int64_t c1 = n; c1 > 0;) {
-    const int64_t c0 = std::max<int64_t>(0, ((c1 - 1) / GT_CHUNK) * GT_CHUNK);
-    __syncthreads();
-    for (int64_t k = c0 + threadIdx.x; k < c0 + GT_CHUNK; k += blockDim.x)
-      reinterpret_cast<int*>(sp)[k - c0] = k < c1 ? ipiv[k] : (int)k;  // padding: no-op exchanges
-    __syncthreads();
-    for (; J >= 0 && (int64_t)J * bw >= c0; --J) {
-      if (r < n) G[(int64_t)J * n + r] = q;
-      const int64_t k0 = (int64_t)J * bw, k1 = min(n, k0 + bw);
-      // exchanges k1-1 .. k0 in reverse; read four at a time from LDS
-      int64_t k = k1;
-      while (k > k0 && (k & 3)) {
-        --k;
-        const int p = reinterpret_cast<const int*>(sp)[k - c0];
-        if (q == (int)k) q = p;
-        else if (q == p) q = (int)k;
-      }
-      for (; k - 4 >= k0; k -= 4) {
-        const int4 p4 = sp[(k - 4 - c0) >> 2];
-        const int ps[4] = {p4.x, p4.y, p4.z, p4.w};
+// Two steps: each block's exchanges composed into one sparse map R[J] (row after
+// the block's exchanges -> row before them; identity elsewhere), then per row a
+// walk through the nJ maps (instead of through all n exchanges).
+__global__ __launch_bounds__(256) void block_map_kernel(const int* ipiv, int64_t n, int bw, int* R) {
+  __shared__ int sp[MAXBW];
+  const int tid = threadIdx.x, J = blockIdx.x;
+  const int64_t k0 = (int64_t)J * bw, k1 = min(n, k0 + bw);
+  const int K = (int)(k1 - k0);
+  for (int k = tid; k < K; k += blockDim.x) sp[k] = ipiv[k0 + k];
+  __syncthreads();
+  // entry e < K: row k0 + e; entry MAXBW + e: the target of exchange e beyond k1
 #pragma unroll
-        for (int u = 3; u >= 0; --u) {
-          const int kk = (int)(k - 4 + u);
-          if (q == kk) q = ps[u];
-          else if (q == ps[u]) q = kk;
-        }
-      }
-      while (k > k0) {
-        --k;
-        const int p = reinterpret_cast<const int*>(sp)[k - c0];
-        if (q == (int)k) q = p;
-        else if (q == p) q = (int)k;
-      }
+  for (int h = 0; h < 2; ++h) {
+    const int e = tid + 256 * h;
+    int dst = -1;
+    if (e < K) dst = (int)(k0 + e);
+    else if (e >= MAXBW && e - MAXBW < K && sp[e - MAXBW] >= k1) dst = sp[e - MAXBW];
+    if (dst < 0) continue;
+    int q = dst;
+    for (int k = K - 1; k >= 0; --k) {
+      const int kk = (int)k0 + k, p = sp[k];
+      if (q == kk) q = p;
+      else if (q == p) q = kk;
     }
-    c1 = c0;
+    R[(int64_t)J * n + dst] = q;
   }
-  if (r < n) perm[r] = q;
+}
+
+__global__ __launch_bounds__(256) void identity_maps_kernel(int64_t n, int nJ, int* R) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)nJ * n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    R[i] = (int)(i % n);
+}
+
+__global__ __launch_bounds__(256) void gather_table_kernel(const int* R, int64_t n, int nJ, int* G, int* perm) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int q = (int)r;
+  for (int J = nJ - 1; J >= 0; --J) {
+    G[(int64_t)J * n + r] = q;
+    q = R[(int64_t)J * n + q];
+  }
+  perm[r] = q;
 }
 
 // dst (n x 8, zero-padded) = rows perm[r] of src (n x nrhs)
@@ -1422,8 +1420,9 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
 
 // Solve workspace (complex T units, then ints): Linv and Uinv per leaf (PNB x PNB
 // each), three n x 8 blocks (gathered rhs, forward results, backward results),
-// the row gather tables (nJ x n), the total row permutation (n) and the forward
-// and backward flags (np each).
+// the row gather tables (nJ x n), the total row permutation (n), the forward
+// and backward flags (np each) and the per-block row maps the tables are built
+// from (nJ x n).
 struct SolveLayout {
   int np, nJ;
   size_t linv, uinv, b0, yf, xb, cend;  // complex offsets
@@ -1443,7 +1442,7 @@ static SolveLayout solve_layout(int64_t n) {
 
 size_t lu_solve_work_bytes(int64_t n, size_t elem) {
   const SolveLayout L = solve_layout(n);
-  return elem * 2 * L.cend + 256 + sizeof(int) * ((size_t)L.nJ * n + n + 2 * (size_t)L.np) + 256;
+  return elem * 2 * L.cend + 256 + sizeof(int) * (2 * (size_t)L.nJ * n + n + 2 * (size_t)L.np) + 256;
 }
 
 template <typename T>
@@ -1468,7 +1467,11 @@ hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv
                      work + 2 * L.uinv);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(gather_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ipiv, n, MAXBW, L.nJ, G,
+  int* R = perm + n + 2 * L.np;  // after the flags: nJ x n block maps
+  hipLaunchKernelGGL(identity_maps_kernel, dim3((unsigned)std::min<int64_t>(4096, ((int64_t)L.nJ * n + 255) / 256)),
+                     dim3(256), 0, s, n, L.nJ, R);
+  hipLaunchKernelGGL(block_map_kernel, dim3((unsigned)L.nJ), dim3(256), 0, s, ipiv, n, MAXBW, R);
+  hipLaunchKernelGGL(gather_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const int*)R, n, L.nJ, G,
                      perm);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   return launch_fill<int>(flags, 2 * (int64_t)L.np, 0, s);
