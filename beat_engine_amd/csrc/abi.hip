@@ -509,6 +509,7 @@ void regular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, Regular
   set_trial_tiles(m, false, ra);
   ra.k = (T)k;
   ra.krev = (T)(k / (2.0 * M_PI));
+  ra.ksteps = phase_steps_per_unit(k);
   ra.trig = ctx->trig;
   for (int i = 0; i < 6; ++i) {
     const bool on = i < m->h.nq;

@@ -37,11 +37,13 @@ __device__ __forceinline__ void gadd(T* p, T v) {
 }
 
 // sin and cos of k*r. FP32: the hardware sin/cos take revolutions; the phase is
-// reduced to [-0.5, 0.5] revolutions first. FP64: the pi/64 table form when the
-// host has bounded |k| r below its limit (WIDE false), else the pi/2 form with
-// the library fallback.
+// reduced to [-0.5, 0.5] revolutions first. FP64: the pi/64 table form on the
+// phase in table steps (r times k 64/pi) when the host has bounded |k| r below
+// its limit (WIDE false), else the pi/2 form with the library fallback.
 template <bool WIDE>
-__device__ __forceinline__ void phase_sincos(float r, float krev, float kk, const double* trig, float& s, float& c) {
+__device__ __forceinline__ void phase_sincos(float r, float krev, double ksteps, float kk, const double* trig,
+                                             float& s, float& c) {
+  (void)ksteps;
   (void)kk;
   (void)trig;
   float t = r * krev;
@@ -50,11 +52,11 @@ __device__ __forceinline__ void phase_sincos(float r, float krev, float kk, cons
   c = __builtin_amdgcn_cosf(t);
 }
 template <bool WIDE>
-__device__ __forceinline__ void phase_sincos(double r, double krev, double kk, const double* trig, double& s,
-                                             double& c) {
+__device__ __forceinline__ void phase_sincos(double r, double krev, double ksteps, double kk, const double* trig,
+                                             double& s, double& c) {
   (void)krev;
   if (WIDE) sincos_phase(kk * r, &s, &c);
-  else sincos_table(kk * r, trig, &s, &c);
+  else sincos_steps(r * ksteps, trig, &s, &c);
 }
 
 __device__ __forceinline__ float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -82,7 +84,7 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
   __shared__ T accA[TILE_V * LDA_];
   __shared__ T accC[TILE_V * LDC_];
   __shared__ T xs[NQ * 3 * 64];  // test points, [point*3 + c][lane]
-  __shared__ T sphw[6 * 4];      // per rule point: phi_0, phi_1, phi_2, w
+  __shared__ T sphw[6 * 8];      // per rule point: phi_0..2, w, phi_0..2 w, unused
   __shared__ double trig[2 * TRIG_N];  // (sin, cos)(m pi/64)
   __shared__ int grow[TILE_V], gcol[TVS];
 
@@ -118,10 +120,13 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
   if (tid == 0) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
-      sphw[4 * i] = a.phi[i][0];
-      sphw[4 * i + 1] = a.phi[i][1];
-      sphw[4 * i + 2] = a.phi[i][2];
-      sphw[4 * i + 3] = a.w[i];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        sphw[8 * i + b] = a.phi[i][b];
+        sphw[8 * i + 4 + b] = a.phi[i][b] * a.w[i];
+      }
+      sphw[8 * i + 3] = a.w[i];
+      sphw[8 * i + 7] = T(0);
     }
   }
 
@@ -161,11 +166,18 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
     const T cw = T(4) * areaT * sr[REC_AREA] * T(INV_FOUR_PI);
     const T nprod = nT0 * nS0 + nT1 * nS1 + nT2 * nS2;
     const T kk2n = k * k * nprod;
+    // Every accumulated quantity is linear in G w, w = cw w_i w_j, so the loop
+    // works with G = e^{ikr} / r and the weights sit in the factors instead: w_j
+    // in the trial basis (sphw columns 4..6) and in PW, cw w_i in the test basis.
     // (r.n_x) = y.n_x - d_x is a per-trial-point constant; (r.n_y) = d_y - x.n_y
-    // is a per-test-point constant (both faces are flat).
-    T PT[NQ];
+    // is a per-test-point constant (both faces are flat). PW[j] = -(r.n_x) w_j,
+    // times eta_im when eta = i eta_im.
+    T PW[NQ];
 #pragma unroll
-    for (int j = 0; j < NQ; ++j) PT[j] = dT - (y[j][0] * nT0 + y[j][1] * nT1 + y[j][2] * nT2);
+    for (int j = 0; j < NQ; ++j) {
+      const T pt = (dT - (y[j][0] * nT0 + y[j][1] * nT1 + y[j][2] * nT2)) * sphw[8 * j + 3];
+      PW[j] = PURE ? ei * pt : pt;
+    }
     T Lr[3][3], Li[3][3], Rr[3], Ri[3], csr = 0, csi = 0;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
@@ -177,11 +189,10 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
     for (int i = 0; i < NQ; ++i) {
       const T x0 = xs[(3 * i) * 64 + lane], x1 = xs[(3 * i + 1) * 64 + lane], x2 = xs[(3 * i + 2) * 64 + lane];
       const T PS = dS - (x0 * nS0 + x1 * nS1 + x2 * nS2);
-      T phi_i[3];
-#pragma unroll
-      for (int p = 0; p < 3; ++p) phi_i[p] = sphw[4 * i + p];
-      const T cwi = cw * sphw[4 * i + 3];
-      T Mr[3] = {0, 0, 0}, Mi[3] = {0, 0, 0}, Sr = 0, Si = 0;
+      const T cwi = cw * sphw[8 * i + 3];
+      // M: sum_j phi_b(y_j) w_j lhs_s; E - Gs: sum_j w_j rhs_s, split as the
+      // eta G' term (E) and the G term (Gs, also the curl term's sum)
+      T Mr[3] = {0, 0, 0}, Mi[3] = {0, 0, 0}, Er = 0, Ei = 0, Gsr = 0, Gsi = 0;
 #pragma unroll
       for (int j = 0; j < NQ; ++j) {
         const T rx = y[j][0] - x0, ry = y[j][1] - x1, rz = y[j][2] - x2;
@@ -189,42 +200,43 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
         const T ir = inv_sqrt(r2);
         const T r = r2 * ir;
         T s, c;
-        phase_sincos<WIDE>(r, krev, k, trig, s, c);
-        const T g = ir * (cwi * sphw[4 * j + 3]);
-        const T Gr = c * g, Gi = s * g;
-        const T Dr = -ir * Gr - k * Gi, Di = -ir * Gi + k * Gr;  // G' w
-        const T t = PS * ir;      // (r.n_y)/r
-        const T u = PT[j] * ir;   // -(r.n_x)/r
-        T lr, li, hr, hi;
+        phase_sincos<WIDE>(r, krev, a.ksteps, k, trig, s, c);
+        const T Gr = c * ir, Gi = s * ir;
+        const T Dr = -ir * Gr - k * Gi, Di = -ir * Gi + k * Gr;  // G'
+        const T t = PS * ir;  // (r.n_y)/r
+        const T wj = sphw[8 * j + 3];
+        T lr, li;
         if (PURE) {
           // eta = i ei
-          const T kap = ei * kk2n, eu = ei * u;
+          const T kap = ei * kk2n, eu = PW[j] * ir;  // eu = ei w_j (-(r.n_x)/r)
           lr = -t * Dr + kap * Gi;
           li = -t * Di - kap * Gr;
-          hr = -Gr + eu * Di;
-          hi = -Gi - eu * Dr;
+          Er += eu * Di;
+          Ei -= eu * Dr;
         } else {
           const T knr = er * kk2n, kni = ei * kk2n;
           lr = -t * Dr - (knr * Gr - kni * Gi);
           li = -t * Di - (knr * Gi + kni * Gr);
+          const T u = PW[j] * ir;  // w_j (-(r.n_x)/r)
           const T eur = er * u, eui = ei * u;
-          hr = -Gr - (eur * Dr - eui * Di);
-          hi = -Gi - (eur * Di + eui * Dr);
+          Er -= eur * Dr - eui * Di;
+          Ei -= eur * Di + eui * Dr;
         }
-        csr += Gr;
-        csi += Gi;
-        Sr += hr;
-        Si += hi;
+        Gsr += wj * Gr;
+        Gsi += wj * Gi;
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-          const T ph = sphw[4 * j + b];
+          const T ph = sphw[8 * j + 4 + b];
           Mr[b] += ph * lr;
           Mi[b] += ph * li;
         }
       }
+      const T Sr = Er - Gsr, Si = Ei - Gsi;
+      csr += cwi * Gsr;
+      csi += cwi * Gsi;
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
-        const T ph = phi_i[p];
+        const T ph = cwi * sphw[8 * i + p];
         Rr[p] += ph * Sr;
         Ri[p] += ph * Si;
 #pragma unroll
@@ -518,7 +530,7 @@ __global__ __launch_bounds__(256, 1) void bm_operators_kernel(const OperatorArgs
         const T ir = inv_sqrt(r2);
         const T r = r2 * ir;
         T s, c;
-        phase_sincos<WIDE>(r, krev, k, trig, s, c);
+        phase_sincos<WIDE>(r, krev, a.ksteps, k, trig, s, c);
         const T g = ir * (cwi * sphw[4 * j + 3]);
         const T Gr = c * g, Gi = s * g;                            // G w
         const T Pr = -ir * Gr - k * Gi, Pi = -ir * Gi + k * Gr;  // G' w

@@ -56,16 +56,13 @@ BEAT_HD void sincos_reduced(double x, double* s, double* c) {
 // 1/32 (exact), |y| <= pi/128; Taylor polynomials of sin to y^7 and cos to y^6
 // (first omitted terms below 1e-17 absolute), then the rotation by the table
 // entry (sin, cos)(m pi/64), m = n mod 128: sin x = S cos y + C sin y,
-// cos x = C cos y - S sin y.
+// cos x = C cos y - S sin y. The reduction and the rotation are separate
+// functions, so that sincos_steps below shares the rotation.
 constexpr double SINCOS_TABLE_LIMIT = 4.0e5;
 constexpr int TRIG_N = 128;  // table entries (sin, cos) over one period
 
-BEAT_HD void sincos_table(double x, const double* trig, double* s, double* c) {
-  const double n = rint(x * 20.371832715762604);  // 64/pi
-  double y = fma(-n, 1.5707963276654482 / 32, x);
-  y = fma(-n, -8.705515692000731e-10 / 32, y);
-  y = fma(-n, -3.50343439808993e-19 / 32, y);
-  const int m = (int)n & (TRIG_N - 1);
+// sin y, cos y for |y| <= pi/128, rotated by the table entry m.
+BEAT_HD void sincos_rotate(double y, int m, const double* trig, double* s, double* c) {
   const double y2 = y * y;
   double ps = -0.0001984126984126984;        // -1/7!
   ps = fma(ps, y2, 0.008333333333333333);    // 1/5!
@@ -78,6 +75,31 @@ BEAT_HD void sincos_table(double x, const double* trig, double* s, double* c) {
   const double S = trig[2 * m], C = trig[2 * m + 1];
   *s = fma(S, cy, C * sy);
   *c = fma(C, cy, -(S * sy));
+}
+
+BEAT_HD void sincos_table(double x, const double* trig, double* s, double* c) {
+  const double n = rint(x * 20.371832715762604);  // 64/pi
+  double y = fma(-n, 1.5707963276654482 / 32, x);
+  y = fma(-n, -8.705515692000731e-10 / 32, y);
+  y = fma(-n, -3.50343439808993e-19 / 32, y);
+  sincos_rotate(y, (int)n & (TRIG_N - 1), trig, s, c);
+}
+
+// The table form on a phase given in table steps, t = x 64/pi with |t| below
+// SINCOS_TABLE_LIMIT 64/pi: n = rint(t), y = (t - n) pi/64, where t - n is exact
+// and the product adds a relative rounding of y alone (|y| <= pi/128). Callers
+// fold 64/pi into the wavenumber, so forming t is one product: the phase then
+// carries the rounding of t (|x| 2^-53) and of the folded wavenumber (another
+// |x| 2^-53), against |x| 2^-53 for rounding k r in sincos_table.
+BEAT_HD void sincos_steps(double t, const double* trig, double* s, double* c) {
+  const double n = rint(t);
+  const double y = (t - n) * 0.04908738521234052;  // pi/64
+  sincos_rotate(y, (int)n & (TRIG_N - 1), trig, s, c);
+}
+
+// Wavenumber scale for sincos_steps: k 64/pi, rounded once (host code).
+inline double phase_steps_per_unit(double k) {
+  return (double)((long double)k * 64.0L / 3.141592653589793238462643383279502884L);
 }
 
 // (sin, cos)(m pi/64) for m = 0..TRIG_N-1, interleaved, evaluated in long double

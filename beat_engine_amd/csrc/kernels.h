@@ -31,6 +31,7 @@ struct RegularArgs {
   const T* q;              // F x nrhs row-major complex (internal face order)
   int nrhs;
   T k, krev;               // signed wavenumber, k / (2 pi)
+  double ksteps;           // k 64/pi: the FP64 table phase in steps per unit length
   T eta_re, eta_im;
   T w[6];                  // rule weights
   T phi[6][3];             // P1 basis at the rule points

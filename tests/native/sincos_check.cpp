@@ -48,6 +48,40 @@ int main() {
     check_t(b + 3.141592653589793 / 128.0);
   }
   check_t(0.0);
-  std::printf("%.3e %.3e %.3e %.3e\n", worst_s, worst_c, wt_s, wt_c);
-  return (worst_s < 4e-16 && worst_c < 4e-16 && wt_s < 4e-16 && wt_c < 4e-16) ? 0 : 1;
+  // step form: the phase given in table steps t = x 64/pi, checked against the
+  // exact phase t pi/64 of that t, evaluated in long double after removing whole
+  // periods exactly (t = n + f with f = t - rint(t) exact; n pi/64 is a whole
+  // number of periods plus (n mod 128) pi/64)
+  const long double pi_ld = 3.141592653589793238462643383279502884L;
+  double ws_s = 0, ws_c = 0;
+  auto check_s = [&](double t) {
+    double s, c;
+    beat::sincos_steps(t, trig, &s, &c);
+    const double n = std::rint(t);
+    const long double m = (long double)(((long long)n % 128 + 128) % 128);
+    const long double x = (m + (long double)(t - n)) * (pi_ld / 64.0L);
+    ws_s = std::fmax(ws_s, std::fabs(s - (double)sinl(x)));
+    ws_c = std::fmax(ws_c, std::fabs(c - (double)cosl(x)));
+  };
+  const double lim_t = beat::SINCOS_TABLE_LIMIT * 64.0 / 3.141592653589793;
+  for (int i = 0; i < 2000000; ++i) check_s(u(rng) * 2000.0 * 64.0 / 3.141592653589793);
+  for (int i = 0; i < 200000; ++i) check_s(-u(rng) * 1000.0);
+  for (int i = 0; i < 200000; ++i) check_s((2.0 * u(rng) - 1.0) * lim_t);
+  for (int k = -400; k <= 400; ++k) {  // integer steps, their neighbours and half-steps
+    check_s(k);
+    check_s(std::nextafter((double)k, 1e9));
+    check_s(std::nextafter((double)k, -1e9));
+    check_s(k + 0.5);
+    check_s(std::nextafter(k + 0.5, 1e9));
+  }
+  check_s(0.0);
+  // the wavenumber scale is k 64/pi rounded once
+  const double ks = beat::phase_steps_per_unit(91.6);
+  const long double ks_exact = 91.6L * 64.0L / pi_ld;
+  const double ks_rel = (double)std::fabs(((long double)ks - ks_exact) / ks_exact);
+  std::printf("%.3e %.3e %.3e %.3e %.3e %.3e %.3e\n", worst_s, worst_c, wt_s, wt_c, ws_s, ws_c, ks_rel);
+  return (worst_s < 4e-16 && worst_c < 4e-16 && wt_s < 4e-16 && wt_c < 4e-16 && ws_s < 4e-16 && ws_c < 4e-16 &&
+          ks_rel <= 1.2e-16)
+             ? 0
+             : 1;
 }

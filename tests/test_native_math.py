@@ -1,6 +1,8 @@
 """CPU check of the FP64 phase sin/cos used by the assembly kernels: the same
 header compiled for the host, compared with the C library over the phase range
-the kernels see and across quadrant boundaries (absolute error < 4e-16)."""
+the kernels see and across quadrant boundaries (absolute error < 4e-16); the
+form that takes the phase in pi/64 steps is compared with the exact phase of its
+argument, and the wavenumber scale k 64/pi with its exact value (one rounding)."""
 import os
 import subprocess
 
