@@ -313,6 +313,17 @@ int solve_all(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const int* ipi
   return BEAT_OK;
 }
 
+// First solve after a factorization that carried B's columns (lu_factor with
+// nc = n + nrhs): the factorization did the forward solve, only the backward one
+// runs. Prepares the per-factorization solve data as solve_all(prepare) does.
+template <typename T>
+int solve_eliminated(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, int nrhs) {
+  HIPCHK(ensure(ctx->work, lu_solve_work_bytes(n, sizeof(T))));
+  HIPCHK(lu_solve_prepare<T>(LU, n, lda, ipiv, as<T>(ctx->work), ctx->stream));
+  HIPCHK(lu_solve_eliminated<T>(LU, n, lda, B, nrhs, as<T>(ctx->work), ctx->stream));
+  return BEAT_OK;
+}
+
 // r = b - A x for all columns; A64 row-major.
 int residual_all(beat_ctx* ctx, const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r,
                  int nrhs) {
@@ -734,28 +745,38 @@ int stage_solve(beat_ctx* ctx, const beat_mesh* m, int precision, int nrhs, Syst
   }
   int panels = 0;
   Refine R;
+  // the right-hand sides ride along in the padding columns of the factored
+  // matrix when they fit: the factorization then does the first forward solve
+  const bool carry = nrhs <= SOLVE_CHUNK && n + nrhs <= lda;
+  const int64_t nc = carry ? n + nrhs : 0;
   HIPCHK(hipEventRecord(ctx->ev[1], s));
   if (precision == BEAT_PREC_FP64) {
-    HIPCHK(lu_factor<double>(as<double>(*sb.a64), n, lda, lw, LU_NB, s, &panels));
+    if (carry) HIPCHK(lu_append_rhs<double>(as<double>(*sb.a64), n, lda, as<double>(*sb.b64), nrhs, s));
+    HIPCHK(lu_factor<double>(as<double>(*sb.a64), n, lda, lw, LU_NB, s, &panels, nc));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    int rc = solve_all<double>(ctx, as<double>(*sb.a64), n, lda, lw.ipiv, as<double>(*sb.b64), nrhs, true);
+    int rc = carry ? solve_eliminated<double>(ctx, as<double>(*sb.a64), n, lda, lw.ipiv, as<double>(*sb.b64), nrhs)
+                   : solve_all<double>(ctx, as<double>(*sb.a64), n, lda, lw.ipiv, as<double>(*sb.b64), nrhs, true);
     if (rc) return rc;
     hipLaunchKernelGGL((scatter_p_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
                        (const double*)as<double>(*sb.b64), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
     HIPCHK(hipGetLastError());
   } else if (precision == BEAT_PREC_FP32) {
-    HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels));
+    if (carry) HIPCHK(lu_append_rhs<float>(as<float>(*sb.a32), n, lda, as<float>(*sb.b32), nrhs, s));
+    HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels, nc));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    int rc = solve_all<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs, true);
+    int rc = carry ? solve_eliminated<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs)
+                   : solve_all<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs, true);
     if (rc) return rc;
     hipLaunchKernelGGL((scatter_p_kernel<float>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
                        (const float*)as<float>(*sb.b32), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
     HIPCHK(hipGetLastError());
   } else {
     HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * n * nrhs));
-    HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels));
+    if (carry) HIPCHK(lu_append_rhs<float>(as<float>(*sb.a32), n, lda, as<float>(*sb.b32), nrhs, s));
+    HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels, nc));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
-    int rc = solve_all<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs, true);
+    int rc = carry ? solve_eliminated<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs)
+                   : solve_all<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs, true);
     if (rc) return rc;
     hipLaunchKernelGGL(upcast_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, s, (const float*)as<float>(*sb.b32),
                        as<double>(ctx->x64), 2 * n * nrhs);

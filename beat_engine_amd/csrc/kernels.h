@@ -138,8 +138,11 @@ struct LuWork {
   hipEvent_t ev_panel, ev_inb;
 };
 
+// Columns [n, nc) (n < nc <= lda) may carry right-hand sides: on return they
+// hold L^-1 P b (nc = 0: none).
 template <typename T>
-hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_t s, int* panels_out);
+hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_t s, int* panels_out,
+                     int64_t nc = 0);
 
 // Forward/backward solves with the factors of lu_factor. lu_solve_prepare runs
 // once per factorization (diagonal-block inverses, row gather tables) into work
@@ -151,6 +154,14 @@ hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv
 template <typename T>
 hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, hipStream_t s);
 size_t lu_solve_work_bytes(int64_t n, size_t elem);
+// Right-hand sides carried by lu_factor: lu_append_rhs copies B (n x nrhs) to the
+// columns [n, n + nrhs) before the factorization (n + nrhs <= lda); after it and
+// lu_solve_prepare, lu_solve_eliminated runs the backward solve alone on those
+// columns and writes the solution to B.
+template <typename T>
+hipError_t lu_append_rhs(T* A, int64_t n, int64_t lda, const T* B, int nrhs, hipStream_t s);
+template <typename T>
+hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, hipStream_t s);
 // A^T x = c with the same factors and prepared work area (after lu_solve_prepare):
 // v (n complex, internal row order) holds c and is overwritten; x receives the
 // solution.

@@ -1325,10 +1325,15 @@ static hipError_t launch_laswp(T* A, int64_t lda, int64_t c0, int64_t c1, const 
 //         that far update before it updates the block after.
 // Leaf inverses alternate between two buffers by block parity, so a far update
 // still reading block J's inverses never sees block J+1's.
+// Columns [n, nc) (nc <= lda) are right-hand sides carried along: every block's
+// swaps, U12 and trailing update reach them as they reach the trailing columns,
+// so on return they hold L^-1 P b, the forward solve done by the factorization.
 template <typename T>
-hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_t s, int* panels_out) {
+hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_t s, int* panels_out, int64_t nc) {
   if (nb > PNB) nb = PNB;
   const int bw = MAXBW;
+  const int64_t ncol = nc > n ? nc : n;
+  if (ncol > lda) return hipErrorInvalidValue;
   hipError_t e;
   if ((e = launch_fill<int>(w.info, 1, 0, s)) != hipSuccess) return e;
   // slot words and published rows carry column tags: clear both
@@ -1398,19 +1403,27 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
       if ((e = hipEventRecord(w.ev_inb, w.inb)) != hipSuccess) return e;
     }
     if (split && (e = hipStreamWaitEvent(s, w.ev_inb, 0)) != hipSuccess) return e;
-    if (Je >= n) continue;
+    if (Je >= n) {
+      // the last block's update of the carried right-hand sides, after the
+      // previous block's far update of them
+      if (ncol > n) {
+        if (look_ahead && (e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;
+        if ((e = update(J, Je, n, ncol, s)) != hipSuccess) return e;
+      }
+      continue;
+    }
     const int64_t Kn = std::min<int64_t>(n, Je + bw);
     if (look_ahead) {
       if ((e = hipEventRecord(w.ev_fact, s)) != hipSuccess) return e;
       if ((e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;  // previous block's far update
-      if (Kn < n) {
+      if (Kn < ncol) {
         if ((e = hipStreamWaitEvent(w.side, w.ev_fact, 0)) != hipSuccess) return e;
-        if ((e = update(J, Je, Kn, n, w.side)) != hipSuccess) return e;
+        if ((e = update(J, Je, Kn, ncol, w.side)) != hipSuccess) return e;
         if ((e = hipEventRecord(w.ev_far, w.side)) != hipSuccess) return e;
       }
       if ((e = update(J, Je, Je, Kn, s)) != hipSuccess) return e;
     } else {
-      if ((e = update(J, Je, Je, n, s)) != hipSuccess) return e;
+      if ((e = update(J, Je, Je, ncol, s)) != hipSuccess) return e;
     }
   }
   if (look_ahead && (e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;
@@ -1517,6 +1530,70 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work
                                      tri_solve_kernel<T, false>, L.np, 256, ta, s);
 }
 
+// Right-hand sides carried by the factorization: in (n x nrhs) to the LU's
+// columns [n, n + nrhs) before it, and L^-1 P b from there to the backward
+// solve's input (n x 8, zero-padded) after it.
+template <typename T>
+__global__ void rhs_to_cols_kernel(const T* B, int64_t n, int nrhs, T* A, int64_t lda) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * nrhs) return;
+  const int64_t r = idx / nrhs;
+  const int d = (int)(idx % nrhs);
+  A[2 * (r * lda + n + d)] = B[2 * idx];
+  A[2 * (r * lda + n + d) + 1] = B[2 * idx + 1];
+}
+
+template <typename T>
+__global__ void cols_to_rhs8_kernel(const T* A, int64_t n, int64_t lda, int nrhs, T* Y) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * 8) return;
+  const int64_t r = idx >> 3;
+  const int d = (int)(idx & 7);
+  Y[2 * idx] = d < nrhs ? A[2 * (r * lda + n + d)] : T(0);
+  Y[2 * idx + 1] = d < nrhs ? A[2 * (r * lda + n + d) + 1] : T(0);
+}
+
+template <typename T>
+hipError_t lu_append_rhs(T* A, int64_t n, int64_t lda, const T* B, int nrhs, hipStream_t s) {
+  if (nrhs < 1 || n + nrhs > lda) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((rhs_to_cols_kernel<T>), dim3((unsigned)((n * nrhs + 255) / 256)), dim3(256), 0, s, B, n, nrhs,
+                     A, lda);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, hipStream_t s) {
+  if (nrhs > 8 || nrhs < 1 || n + nrhs > lda) return hipErrorInvalidValue;
+  const SolveLayout L = solve_layout(n);
+  int* G = solve_ints(work, L);
+  int* flags = G + (size_t)L.nJ * n + n;
+  hipLaunchKernelGGL((cols_to_rhs8_kernel<T>), dim3((unsigned)((n * 8 + 255) / 256)), dim3(256), 0, s, LU, n, lda,
+                     nrhs, work + 2 * L.yf);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  TriArgs<T> ta;
+  ta.LU = LU;
+  ta.n = n;
+  ta.lda = lda;
+  ta.bw = MAXBW;
+  ta.epoch = next_epoch();
+  ta.np = L.np;
+  ta.nrhs = nrhs;
+  ta.out = B;
+  ta.dinv = work + 2 * L.uinv;
+  ta.gather = nullptr;
+  ta.src = work + 2 * L.yf;
+  ta.pub = work + 2 * L.xb;
+  ta.flags = flags + L.np;
+  return launch_resident<TriArgs<T>>(reinterpret_cast<const void*>(&tri_solve_kernel<T, false>),
+                                     tri_solve_kernel<T, false>, L.np, 256, ta, s);
+}
+
+template hipError_t lu_append_rhs<float>(float*, int64_t, int64_t, const float*, int, hipStream_t);
+template hipError_t lu_append_rhs<double>(double*, int64_t, int64_t, const double*, int, hipStream_t);
+template hipError_t lu_solve_eliminated<float>(const float*, int64_t, int64_t, float*, int, float*, hipStream_t);
+template hipError_t lu_solve_eliminated<double>(const double*, int64_t, int64_t, double*, int, double*, hipStream_t);
+
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
                       hipStream_t s) {
   if (nrhs > 8) return hipErrorInvalidValue;
@@ -1537,8 +1614,8 @@ hipError_t column_norms64(const double* X, int64_t n, int nrhs, double* out, hip
   return hipGetLastError();
 }
 
-template hipError_t lu_factor<float>(float*, int64_t, int64_t, LuWork&, int, hipStream_t, int*);
-template hipError_t lu_factor<double>(double*, int64_t, int64_t, LuWork&, int, hipStream_t, int*);
+template hipError_t lu_factor<float>(float*, int64_t, int64_t, LuWork&, int, hipStream_t, int*, int64_t);
+template hipError_t lu_factor<double>(double*, int64_t, int64_t, LuWork&, int, hipStream_t, int*, int64_t);
 
 // Solve A^T x = c with the factors of lu_factor (A = P L U, so U^T L^T P^T x = c),
 // one workgroup of 1024 threads: U^T z = c forward and L^T y = z backward by
