@@ -358,6 +358,22 @@ int col_norms(beat_ctx* ctx, const double* X, int64_t n, int nrhs, std::vector<d
   return BEAT_OK;
 }
 
+// Column norms of X and Y read back with one wait on the stream.
+int col_norms2(beat_ctx* ctx, const double* X, const double* Y, int64_t n, int nrhs, std::vector<double>& ox,
+               std::vector<double>& oy) {
+  HIPCHK(ensure(ctx->norms, sizeof(double) * 2 * (nrhs + 64 * nrhs)));
+  double* nx = as<double>(ctx->norms);
+  double* ny = nx + nrhs + 64 * nrhs;
+  HIPCHK(column_norms64(X, n, nrhs, nx, ctx->stream));
+  HIPCHK(column_norms64(Y, n, nrhs, ny, ctx->stream));
+  ox.resize(nrhs);
+  oy.resize(nrhs);
+  HIPCHK(hipMemcpyAsync(ox.data(), nx, sizeof(double) * nrhs, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(oy.data(), ny, sizeof(double) * nrhs, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return BEAT_OK;
+}
+
 struct Refine {
   int steps = 0, status = BEAT_REFINE_NONE, working = BEAT_REFINE_NONE, hist_len = 0;
   std::vector<double> working_res, returned_res, history;
@@ -383,7 +399,6 @@ int refine_loop(beat_ctx* ctx, const float* LU32, int64_t lda32, const int* ipiv
   HIPCHK(ensure(ctx->lusmall, sizeof(int) * nrhs + 256));
   int* dmask = as<int>(ctx->lusmall);
   std::vector<double> bn, rn;
-  if (col_norms(ctx, b64, n, nrhs, bn) != BEAT_OK) return BEAT_ERR_HIP;
   auto errors_of = [&](const std::vector<double>& nrm) {
     std::vector<double> e(nrhs);
     for (int j = 0; j < nrhs; ++j)
@@ -391,7 +406,7 @@ int refine_loop(beat_ctx* ctx, const float* LU32, int64_t lda32, const int* ipiv
     return e;
   };
   if (residual_all(ctx, A64, n, lda64, x64, b64, r, nrhs) != BEAT_OK) return BEAT_ERR_HIP;
-  if (col_norms(ctx, r, n, nrhs, rn) != BEAT_OK) return BEAT_ERR_HIP;
+  if (col_norms2(ctx, b64, r, n, nrhs, bn, rn) != BEAT_OK) return BEAT_ERR_HIP;
   std::vector<double> errors = errors_of(rn);
   out.history = errors;
   out.hist_len = 1;
@@ -789,9 +804,10 @@ int stage_solve(beat_ctx* ctx, const beat_mesh* m, int precision, int nrhs, Syst
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ctx->ev[3], s));
-  HIPCHK(hipEventSynchronize(ctx->ev[3]));
+  // the factorization's zero-pivot report, read with the same wait as the rest
   int info = 0;
-  HIPCHK(hipMemcpy(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpyAsync(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
   if (rep) {
     float ta, tr, t12, t23;
     HIPCHK(hipEventElapsedTime(&ta, asm_ev.start, asm_ev.end));
@@ -884,6 +900,8 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
       HIPCHK(ensure(ctx->n64, sizeof(double) * 2 * nmax * nrhs));
       HIPCHK(ensure(ctx->rn64, sizeof(double) * 2 * nmax * nrhs));
       HIPCHK(ensure(ctx->d32, sizeof(float) * 2 * nmax * nrhs + sizeof(int) * 64 + 4096));
+      HIPCHK(ensure(ctx->norms, sizeof(double) * 2 * (nrhs + 64 * nrhs)));
+      HIPCHK(ensure(ctx->lusmall, sizeof(int) * nrhs + 256));
     }
   }
   AsmEvents ev[2];
