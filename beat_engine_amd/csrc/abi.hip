@@ -788,6 +788,9 @@ int stage_solve(beat_ctx* ctx, const beat_mesh* m, int precision, int nrhs, Syst
   } else {
     HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * n * nrhs));
     if (carry) HIPCHK(lu_append_rhs<float>(as<float>(*sb.a32), n, lda, as<float>(*sb.b32), nrhs, s));
+    // these factors only precondition the complex128 refinement: the trailing
+    // updates may use the three-real-product complex form
+    lw.gauss3 = true;
     HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels, nc));
     HIPCHK(hipEventRecord(ctx->ev[2], s));
     int rc = carry ? solve_eliminated<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs)

@@ -136,6 +136,9 @@ struct LuWork {
   hipEvent_t ev_fact, ev_far;
   hipStream_t inb;    // in-block stream (nullptr: in-block work in order on the main stream)
   hipEvent_t ev_panel, ev_inb;
+  // complex64 trailing updates as three real products instead of four (normwise
+  // stable; for factors that only precondition a refinement)
+  bool gauss3 = false;
 };
 
 // Columns [n, nc) (n < nc <= lda) may carry right-hand sides: on return they

@@ -609,6 +609,10 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 //               output item v of lane l is D[8(v/4) + 4(l/32) + v%4][l%32]
 //   complex128: v_mfma_f64_16x16x4_f64, lane l holds A[l%16][l/16], B[l/16][l%16],
 //               output item v of lane l is D[l/16 + 4v][l%16]
+// G3 (complex64): the three-real-product form of the complex product (Gauss):
+// T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi), then Cr = T1 - T2 and
+// Ci = T3 - T1 - T2. Three 32x32x2 products per two steps of k instead of four;
+// normwise stable, with Ci's error bounded by |A||B| rather than componentwise.
 template <typename T>
 struct GemmCfg;
 template <>
@@ -620,7 +624,7 @@ struct GemmCfg<double> {
   static constexpr int VEC = 2;
 };
 
-template <typename T, bool SET, int BM, int KB>
+template <typename T, bool SET, int BM, int KB, bool G3 = false>
 __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
                                                     int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n, int kc,
                                                     int behind) {
@@ -693,7 +697,48 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
   };
 
   constexpr int WR = BM / 2;  // rows per wave (two waves along the rows, two along the columns)
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 4 && G3) {
+    // each wave: 32 rows x 32 complex columns of T1, T2, T3; lane (h, l) takes
+    // step kk + h of k, row l of the wave's rows and complex column l of its columns
+    static_assert(BM == 64 && KB % 2 == 0, "G3 tiles: 64 rows, even K chunks");
+    f32x16 t1, t2, t3;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) t1[v] = t2[v] = t3[v] = 0.f;
+    const int h = lane >> 5, l32 = lane & 31;
+    const int arow = (wm * 32 + l32) * AP;
+    const int jc = wn * 32 + l32;
+    load(0);
+    for (int k0 = 0; k0 < kc; k0 += KB) {
+      stash();
+      __syncthreads();
+      if (k0 + KB < kc) load(k0 + KB);
+#pragma unroll 4
+      for (int kk = 0; kk < KB; kk += 2) {
+        const int kx = kk + h;
+        const float2 av = *reinterpret_cast<const float2*>(&As[arow + 2 * kx]);
+        const float2 bv = *reinterpret_cast<const float2*>(&Bs[kx * BP + 2 * jc]);
+        t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bv.x, t1, 0, 0, 0);
+        t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bv.y, t2, 0, 0, 0);
+        t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x + av.y, bv.x + bv.y, t3, 0, 0, 0);
+      }
+      __syncthreads();
+    }
+    const int64_t gc = col0 + jc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int64_t gr = row0 + wm * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+      if (gr < m && gc < n) {
+        const float cr = t1[v] - t2[v], ci = t3[v] - t1[v] - t2[v];
+        float2* dst = reinterpret_cast<float2*>(C + 2 * (gr * ldc + gc));
+        if (SET) {
+          *dst = make_float2(cr, ci);
+        } else {
+          const float2 o = *dst;
+          *dst = make_float2(o.x - cr, o.y - ci);
+        }
+      }
+    }
+  } else if constexpr (sizeof(T) == 4) {
     constexpr int MI = WR / 32;
     f32x16 acc[MI][2];
 #pragma unroll
@@ -797,7 +842,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
 
 template <typename T, bool SET>
 hipError_t gemm_launch(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
-                       int kc, hipStream_t s, bool behind) {
+                       int kc, hipStream_t s, bool behind, bool g3) {
   if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
   // 64 x 64-complex tiles, K in chunks of 16. Measured on gfx950 at the LU's
   // update shapes against 128-row tiles (chunks of 16 or 32), 256-row tiles and
@@ -806,6 +851,13 @@ hipError_t gemm_launch(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, i
   // and chunks of 16 keep the LDS small enough that a panel workgroup still fits
   // on a CU that runs the update's workgroups at full occupancy.
   dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64));
+  if constexpr (sizeof(T) == 4) {
+    if (g3) {
+      hipLaunchKernelGGL((zgemm_kernel<T, SET, 64, 16, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc,
+                         behind ? 1 : 0);
+      return hipGetLastError();
+    }
+  }
   hipLaunchKernelGGL((zgemm_kernel<T, SET, 64, 16>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc,
                      behind ? 1 : 0);
   return hipGetLastError();
@@ -813,14 +865,14 @@ hipError_t gemm_launch(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, i
 
 template <typename T>
 hipError_t gemm_sub(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
-                    int kc, hipStream_t s, bool behind = false) {
-  return gemm_launch<T, false>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind);
+                    int kc, hipStream_t s, bool behind = false, bool g3 = false) {
+  return gemm_launch<T, false>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind, g3);
 }
 
 template <typename T>
 hipError_t gemm_set(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
-                    int kc, hipStream_t s, bool behind = false) {
-  return gemm_launch<T, true>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind);
+                    int kc, hipStream_t s, bool behind = false, bool g3 = false) {
+  return gemm_launch<T, true>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind, g3);
 }
 
 // ---- solves ----------------------------------------------------------------------
@@ -1334,6 +1386,7 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   const int bw = MAXBW;
   const int64_t ncol = nc > n ? nc : n;
   if (ncol > lda) return hipErrorInvalidValue;
+  const bool g3 = w.gauss3;  // complex64 updates in the three-real-product form
   hipError_t e;
   if ((e = launch_fill<int>(w.info, 1, 0, s)) != hipSuccess) return e;
   // slot words and published rows carry column tags: clear both
@@ -1356,10 +1409,10 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
     hipError_t r;
     if ((r = launch_laswp<T>(A, lda, c0, c1, w.ipiv, j0, j0 + b, st)) != hipSuccess) return r;
     r = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + c0), lda, A + 2 * (j0 * lda + c0), lda, b, c1 - c0, b,
-                    st);
+                    st, false, g3);
     if (r != hipSuccess) return r;
     return gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + c0), lda, A + 2 * ((j0 + b) * lda + c0),
-                       lda, n - j0 - b, c1 - c0, b, st);
+                       lda, n - j0 - b, c1 - c0, b, st, false, g3);
   };
   // block [J, Je)'s update of columns [c0, c1): swaps, U12 by leaf inverses, GEMM
   auto update = [&](int64_t J, int64_t Je, int64_t c0, int64_t c1, hipStream_t st) -> hipError_t {
@@ -1370,16 +1423,16 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
     for (int64_t j0 = J; j0 < Je; j0 += nb) {
       const int b = (int)std::min<int64_t>(nb, Je - j0);
       r = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + c0), lda, A + 2 * (j0 * lda + c0), lda, b, c1 - c0,
-                      b, st, behind);
+                      b, st, behind, g3);
       if (r != hipSuccess) return r;
       if (j0 + b < Je) {
         r = gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + c0), lda,
-                        A + 2 * ((j0 + b) * lda + c0), lda, Je - j0 - b, c1 - c0, b, st, behind);
+                        A + 2 * ((j0 + b) * lda + c0), lda, Je - j0 - b, c1 - c0, b, st, behind, g3);
         if (r != hipSuccess) return r;
       }
     }
     return gemm_sub<T>(cA + 2 * (Je * lda + J), lda, cA + 2 * (J * lda + c0), lda, A + 2 * (Je * lda + c0), lda,
-                       n - Je, c1 - c0, (int)(Je - J), st, behind);
+                       n - Je, c1 - c0, (int)(Je - J), st, behind, g3);
   };
   for (int64_t J = 0; J < n; J += bw) {
     const int64_t Je = std::min<int64_t>(n, J + bw);
