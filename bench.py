@@ -34,7 +34,7 @@ FLOP_PER_QPAIR = 72  # fused Burton-Miller flops per quadrature-point pair
 # HBM bytes per launch of the FP64 regular kernel at L5, from two rocprofv3 --pmc
 # runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors make a 1 GiB device copy
 # read as 1 GiB each way); profiles/bm_regular_L5_pmc_traffic.md
-REGULAR_L5_TRAFFIC_BYTES = 21.12e9
+REGULAR_L5_TRAFFIC_BYTES = 19.17e9
 
 
 def parse():
