@@ -74,7 +74,8 @@ struct beat_ctx {
   // the regular kernel at one workgroup per CU while it shares the device with
   // an LU (BEAT_SWEEP_ASM_ONE_PER_CU=1). Off by default: the LU's kernels issue at
   // a higher wave priority, so the assembly may take every slot the LU leaves
-  // (measured at L5: 11.2 vs 10.7 freq-solves/s)
+  // (measured at L5, 8 RHS: 12.9 vs 10.7 freq-solves/s; one per CU the assembly
+  // takes 90 ms per frequency instead of 70, the LU 64.8 ms instead of 69.7)
   bool sweep_asm_one_per_cu = false;
   // contexts a batch may use at once (BEAT_BATCH_WORKERS, default 4)
   int batch_workers = 4;
