@@ -13,7 +13,7 @@ import pytest
 
 import oracle
 from beat_engine_amd import exterior
-from beat_engine_amd.mesh import fibonacci_sphere, icosphere, jittered_icosphere, load_gmsh22
+from beat_engine_amd.mesh import capped_cylinder, fibonacci_sphere, icosphere, jittered_icosphere, load_gmsh22
 from beat_engine_amd.phasor import burton_miller_coupling, neumann_scale
 
 pytestmark = pytest.mark.gpu
@@ -185,6 +185,26 @@ def test_solve_matches_oracle(eng, l3_system, precision, tol):
         # recovered from it
         first = np.asarray(rep["refinement"]["history"][0])
         assert first.max() <= 1e-5 and rep["refinement"]["steps"] <= 2
+
+
+@pytest.mark.parametrize("nrhs", [7, 8])
+def test_solve_rhs_in_padding_columns(eng, nrhs):
+    """The right-hand sides ride through the factorization in the padding columns
+    of the factored matrix when n + nrhs <= lda: 1017 dofs (lda 1024) with 7 fill
+    the padding exactly, with 8 the solve falls back to the full forward solve.
+    Both against the oracle, complex128 and refined."""
+    say("padding columns", nrhs)
+    mesh = capped_cylinder(0.3, 0.08, 2064, 4.0, seed=7)
+    assert mesh.num_vertices == 1017
+    k, eta, q = _case(mesh, 1200.0, nrhs, seed=5)
+    Ar, Br = oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q)
+    pr = oracle.lu_solve(Ar, Br)
+    dm = eng.upload(mesh)
+    for precision, tol in (("float64", 1e-10), ("float32_refined", 1e-8)):
+        P, rep = dm.solve(k, eta, q, precision)
+        assert rel(P, pr) <= tol, (precision, rel(P, pr))
+        if precision == "float32_refined":
+            assert rep["refinement"]["status"] == "converged"
 
 
 def test_solve_many_rhs(eng):
