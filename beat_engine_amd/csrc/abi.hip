@@ -92,6 +92,11 @@ struct beat_ctx {
     for (int i = 0; i < 40; ++i) lu_layout[i] = asm_occ[i] = -1;
   }
   double* trig = nullptr;  // (sin, cos)(m pi/64) table of the FP64 phase
+  // wait control of the last factorization (LuWork::ctl), passed to its solves
+  int* lu_ctl = nullptr;
+  // test hook (beat_ctx_debug_abort_waits): factorizations start with their abort
+  // word set, so every cross-workgroup wait gives up at once
+  int debug_abort_waits = 0;
 };
 
 struct beat_mesh {
@@ -274,6 +279,9 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   };
   w.ipiv = reinterpret_cast<int*>(take(sizeof(int) * (n + 8)));
   w.info = reinterpret_cast<int*>(take(sizeof(int) * 4));
+  w.ctl = w.info + 1;
+  w.abort_init = ctx->debug_abort_waits;
+  ctx->lu_ctl = w.ctl;
   w.slots = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 32 * maxg));  // 2 x maxg slots of 16 words
   w.slot_rows = take(slot_rows);
   w.slot_diag = take(slot_diag);
@@ -289,6 +297,27 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   return BEAT_OK;
 }
 
+// The factorization's zero-pivot report and wait control, read after the stream
+// has drained: BEAT_ERR_HIP when a cross-workgroup wait of the factorization or of
+// its solves gave up (the results are then abandoned), else *info = the first
+// zero pivot (1-based, 0 if none).
+int lu_outcome(beat_ctx* ctx, const LuWork& w, int* info) {
+  int st[3] = {0, 0, 0};
+  HIPCHK(hipMemcpy(st, w.info, sizeof(st), hipMemcpyDeviceToHost));
+  *info = st[0];
+  if (st[1] != 0 || st[2] != 0) {
+    std::string where;
+    if (st[2] == 0) where = "the abort word was set";
+    if (st[2] & 1) where += "panel slot poll ";
+    if (st[2] & 2) where += "panel row fetch ";
+    if (st[2] & 4) where += "forward solve ";
+    if (st[2] & 8) where += "backward solve ";
+    return fail(ctx, BEAT_ERR_HIP, "a cross-workgroup wait of the factorization or its solves gave up (" + where +
+                                       "); the results of this call are abandoned");
+  }
+  return BEAT_OK;
+}
+
 // Solve with LU factors for every column of the n x nrhs row-major B (in place).
 // prepare: first solve after a factorization (builds the per-factorization solve
 // data in ctx->work; later solves with the same factors pass false).
@@ -297,7 +326,7 @@ int solve_all(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const int* ipi
   HIPCHK(ensure(ctx->work, lu_solve_work_bytes(n, sizeof(T))));
   if (prepare) HIPCHK(lu_solve_prepare<T>(LU, n, lda, ipiv, as<T>(ctx->work), ctx->stream));
   if (nrhs <= SOLVE_CHUNK) {
-    HIPCHK(lu_solve<T>(LU, n, lda, B, nrhs, as<T>(ctx->work), ctx->stream));
+    HIPCHK(lu_solve<T>(LU, n, lda, B, nrhs, as<T>(ctx->work), ctx->lu_ctl, ctx->stream));
     return BEAT_OK;
   }
   HIPCHK(ensure(ctx->chunk, sizeof(T) * 2 * n * SOLVE_CHUNK));
@@ -306,7 +335,7 @@ int solve_all(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const int* ipi
     hipLaunchKernelGGL((extract_cols_kernel<T>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream, (const T*)B, n,
                        nrhs, c0, nc, as<T>(ctx->chunk));
     HIPCHK(hipGetLastError());
-    HIPCHK(lu_solve<T>(LU, n, lda, as<T>(ctx->chunk), nc, as<T>(ctx->work), ctx->stream));
+    HIPCHK(lu_solve<T>(LU, n, lda, as<T>(ctx->chunk), nc, as<T>(ctx->work), ctx->lu_ctl, ctx->stream));
     hipLaunchKernelGGL((insert_cols_kernel<T>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream,
                        (const T*)as<T>(ctx->chunk), n, nrhs, c0, nc, B);
     HIPCHK(hipGetLastError());
@@ -321,7 +350,7 @@ template <typename T>
 int solve_eliminated(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const int* ipiv, T* B, int nrhs) {
   HIPCHK(ensure(ctx->work, lu_solve_work_bytes(n, sizeof(T))));
   HIPCHK(lu_solve_prepare<T>(LU, n, lda, ipiv, as<T>(ctx->work), ctx->stream));
-  HIPCHK(lu_solve_eliminated<T>(LU, n, lda, B, nrhs, as<T>(ctx->work), ctx->stream));
+  HIPCHK(lu_solve_eliminated<T>(LU, n, lda, B, nrhs, as<T>(ctx->work), ctx->lu_ctl, ctx->stream));
   return BEAT_OK;
 }
 
@@ -808,10 +837,10 @@ int stage_solve(beat_ctx* ctx, const beat_mesh* m, int precision, int nrhs, Syst
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ctx->ev[3], s));
-  // the factorization's zero-pivot report, read with the same wait as the rest
-  int info = 0;
-  HIPCHK(hipMemcpyAsync(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost, s));
+  // the factorization's zero-pivot report and wait control, after the stream drained
   HIPCHK(hipStreamSynchronize(s));
+  int info = 0;
+  if (int rc = lu_outcome(ctx, lw, &info)) return rc;
   if (rep) {
     float ta, tr, t12, t23;
     HIPCHK(hipEventElapsedTime(&ta, asm_ev.start, asm_ev.end));
@@ -1216,6 +1245,13 @@ int beat_ctx_set_cancel_path(beat_ctx* ctx, const char* path) {
   if (!ctx) return BEAT_ERR_ARG;
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->cancel_path = path ? path : "";
+  return BEAT_OK;
+}
+
+int beat_ctx_debug_abort_waits(beat_ctx* ctx, int on) {
+  if (!ctx) return BEAT_ERR_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->debug_abort_waits = on ? 1 : 0;
   return BEAT_OK;
 }
 
@@ -1643,7 +1679,7 @@ int beat_adjoint_spl_gradient(beat_ctx* ctx, const beat_mesh* m, int64_t nfreq, 
     HIPCHK(hipMemcpyAsync(u.data(), ctx->fld_out.p, sizeof(double) * 2 * npts, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     int info = 0;
-    HIPCHK(hipMemcpy(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost));
+    if (int rc = lu_outcome(ctx, lw, &info)) return rc;
     if (info != 0) return fail(ctx, BEAT_ERR_SINGULAR, "LU found an exactly zero pivot at column " + std::to_string(info));
     // loss and c = conj(dL/du*) = 2 (S - target) (10 / ln 10) conj(u) / |u|^2
     for (int64_t d = 0; d < npts; ++d) {
@@ -1793,6 +1829,7 @@ int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, 
   for (int w = 1; w < workers; ++w) {
     beat_ctx* a = ctx->aux[w - 1];
     a->cancel_path = ctx->cancel_path;
+    a->debug_abort_waits = ctx->debug_abort_waits;
     a->sweep_asm_one_per_cu = ctx->sweep_asm_one_per_cu;
     std::memcpy(a->lu_layout, ctx->lu_layout, sizeof(ctx->lu_layout));
     std::memcpy(a->asm_occ, ctx->asm_occ, sizeof(ctx->asm_occ));
@@ -1907,7 +1944,7 @@ int beat_dense_lu_solve(beat_ctx* ctx, int precision, const double* a, int64_t n
   HIPCHK(hipMemcpyAsync(x, ctx->pout.p, sizeof(double) * 2 * n * nrhs, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   int info = 0;
-  HIPCHK(hipMemcpy(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost));
+  if (int rc = lu_outcome(ctx, lw, &info)) return rc;
   if (info != 0) return fail(ctx, BEAT_ERR_SINGULAR, "LU found an exactly zero pivot at column " + std::to_string(info));
   return BEAT_OK;
 }
@@ -1974,7 +2011,7 @@ int beat_dense_lu_factor(beat_ctx* ctx, int precision, const double* a, int64_t 
   }
   if (rc != BEAT_OK) return rc;
   int info = 0;
-  HIPCHK(hipMemcpy(&info, lw.info, sizeof(int), hipMemcpyDeviceToHost));
+  if ((rc = lu_outcome(ctx, lw, &info)) != BEAT_OK) return rc;
   *info_out = info;
   return BEAT_OK;
 }
@@ -2036,6 +2073,8 @@ int beat_refine_lu(beat_ctx* ctx, const float* matrix, int64_t n, const float* r
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(x_out, ctx->pout.p, sizeof(float) * 2 * n * nrhs, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  int info = 0;
+  if (int rc = lu_outcome(ctx, lw, &info)) return rc;
   fill_report(rep, R, rtol, (int)nrhs);
   return BEAT_OK;
 }

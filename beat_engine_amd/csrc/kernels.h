@@ -126,10 +126,16 @@ hipError_t launch_downcast(const double* src, float* dst, int64_t n, hipStream_t
 constexpr int LU_BLOCK_COLS = 256;
 struct LuWork {
   int* ipiv;          // n, global row index chosen at each column
-  unsigned long long* slots;  // 2 x maxg slots of 16 words (tag << 32 | row, tagged value, padding)
+  unsigned long long* slots;  // 2 x maxg slots of 16 words (two tagged halves of the candidate magnitude, padding)
   void* slot_rows;    // 2 x maxg x 64 published rows, 4 tagged 64-bit words per element
   void* slot_diag;    // 2 x 64 published elements, same form
   int* info;          // first zero pivot (1-based), 0 if none
+  // wait control of the factorization and of the solves with its factors (ctl =
+  // info + 1): [0] abort word (nonzero: every cross-workgroup wait gives up at
+  // once), [1] the kernels whose wait timed out (SpinSite bits), 0 if none.
+  // lu_factor clears both, then sets the abort word to abort_init.
+  int* ctl;
+  int abort_init = 0;
   void* linv;         // 2 x 4 x 64 x 64 complex (double-sized): leaf inverses of two outer blocks
   int maxg;
   hipStream_t side;   // look-ahead stream (nullptr: no look-ahead)
@@ -150,12 +156,12 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
 // Forward/backward solves with the factors of lu_factor. lu_solve_prepare runs
 // once per factorization (diagonal-block inverses, row gather tables) into work
 // (lu_solve_work_bytes); lu_solve then overwrites B (n x nrhs row-major complex
-// T, nrhs <= 8) with the solution. Solves sharing one work area run in stream
-// order.
+// T, nrhs <= 8) with the solution; ctl is the factorization's wait control
+// (LuWork::ctl). Solves sharing one work area run in stream order.
 template <typename T>
 hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv, T* work, hipStream_t s);
 template <typename T>
-hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, hipStream_t s);
+hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, int* ctl, hipStream_t s);
 size_t lu_solve_work_bytes(int64_t n, size_t elem);
 // Right-hand sides carried by lu_factor: lu_append_rhs copies B (n x nrhs) to the
 // columns [n, n + nrhs) before the factorization (n + nrhs <= lda); after it and
@@ -164,7 +170,8 @@ size_t lu_solve_work_bytes(int64_t n, size_t elem);
 template <typename T>
 hipError_t lu_append_rhs(T* A, int64_t n, int64_t lda, const T* B, int nrhs, hipStream_t s);
 template <typename T>
-hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, hipStream_t s);
+hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, int* ctl,
+                               hipStream_t s);
 // A^T x = c with the same factors and prepared work area (after lu_solve_prepare):
 // v (n complex, internal row order) holds c and is overwritten; x receives the
 // solution.

@@ -46,6 +46,7 @@ struct PanelArgs {
   unsigned long long* slot_diag;  // 2 x PNB tagged elements
   int maxg;
   int* info;
+  int* ctl;  // wait control: [0] abort word, [1] give-up record (see spin_expired)
   T* linv;  // PNB x PNB row-major: inverse of the leaf's unit-lower diagonal block
 };
 
@@ -72,6 +73,37 @@ __device__ __forceinline__ void ld_cplx(const double* p, double& re, double& im)
   re = __longlong_as_double((long long)ld_u64(reinterpret_cast<const unsigned long long*>(p)));
   im = __longlong_as_double((long long)ld_u64(reinterpret_cast<const unsigned long long*>(p) + 1));
 }
+__device__ __forceinline__ void st_flag(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ __forceinline__ int ld_flag(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Bounded cross-workgroup waits. Every spin loop of the factorization and of the
+// triangular solves counts its rounds; every 64th round it reads the abort word
+// ctl[0] and the real-time counter (s_memrealtime, 100 MHz). A wait gives up when
+// the abort word is set or after SPIN_LIMIT_TICKS (4 s, far beyond any wait of a
+// resident grid, whose longest is the placement of a workgroup behind finite
+// kernels of other streams); a wait that times out records its kernel in ctl[1]
+// and sets the abort word, so that every other wait of the call gives up within
+// 64 rounds. The kernels then leave their loops together and end normally; the
+// host reads ctl after the call and reports the abandoned wait as an error.
+constexpr unsigned long long SPIN_LIMIT_TICKS = 400000000ull;
+enum SpinSite : int { SPIN_PANEL_SLOT = 1, SPIN_PANEL_ROW = 2, SPIN_TRI_FWD = 4, SPIN_TRI_BWD = 8 };
+struct Spin {
+  unsigned long long t0 = 0;
+  unsigned n = 0;
+};
+__device__ __forceinline__ bool spin_expired(int* ctl, Spin& sp, int site) {
+  if ((++sp.n & 63u) != 0) return false;
+  if (ld_flag(ctl) != 0) return true;
+  const unsigned long long t = __builtin_amdgcn_s_memrealtime();
+  if (sp.t0 == 0) {
+    sp.t0 = t;
+    return false;
+  }
+  if (t - sp.t0 < SPIN_LIMIT_TICKS) return false;
+  __hip_atomic_fetch_or(ctl + 1, site, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  st_flag(ctl, 1);
+  return true;
+}
 
 // Panel rows published to other workgroups: every 32-bit half of a complex
 // element's parts travels in its own 64-bit word as (tag << 32 | bits), TW words
@@ -94,26 +126,37 @@ __device__ __forceinline__ void st_tagged(unsigned long long* q, double re, doub
   st_u64(q + 2, t | (b & 0xffffffffull));
   st_u64(q + 3, t | (b >> 32));
 }
-// Loads one published element, re-reading until every word carries the tag.
-__device__ __forceinline__ void ld_tagged(const unsigned long long* q, float& re, float& im, unsigned tag) {
+// Loads one published element, re-reading until every word carries the tag;
+// false when the wait gave up (the element is then zero).
+__device__ __forceinline__ bool ld_tagged(const unsigned long long* q, float& re, float& im, unsigned tag, int* ctl) {
+  Spin sp;
   for (;;) {
     const unsigned long long a = ld_u64(q), b = ld_u64(q + 1);
     if ((unsigned)(a >> 32) == tag && (unsigned)(b >> 32) == tag) {
       re = __uint_as_float((unsigned)a);
       im = __uint_as_float((unsigned)b);
-      return;
+      return true;
+    }
+    if (spin_expired(ctl, sp, SPIN_PANEL_ROW)) {
+      re = im = 0.f;
+      return false;
     }
     __builtin_amdgcn_s_sleep(1);
   }
 }
-__device__ __forceinline__ void ld_tagged(const unsigned long long* q, double& re, double& im, unsigned tag) {
+__device__ __forceinline__ bool ld_tagged(const unsigned long long* q, double& re, double& im, unsigned tag, int* ctl) {
+  Spin sp;
   for (;;) {
     const unsigned long long a = ld_u64(q), b = ld_u64(q + 1), c = ld_u64(q + 2), d = ld_u64(q + 3);
     if ((unsigned)(a >> 32) == tag && (unsigned)(b >> 32) == tag && (unsigned)(c >> 32) == tag &&
         (unsigned)(d >> 32) == tag) {
       re = __longlong_as_double((long long)((b << 32) | (a & 0xffffffffull)));
       im = __longlong_as_double((long long)((d << 32) | (c & 0xffffffffull)));
-      return;
+      return true;
+    }
+    if (spin_expired(ctl, sp, SPIN_PANEL_ROW)) {
+      re = im = 0.0;
+      return false;
     }
     __builtin_amdgcn_s_sleep(1);
   }
@@ -156,11 +199,6 @@ __device__ __forceinline__ void wave_pick(double v, int row, bool valid, double&
   vbest = __longlong_as_double((long long)(((unsigned long long)(H - 1u) << 32) | Lo));
 }
 
-// Candidate order inside one lane: larger value first, then the smaller row.
-__device__ __forceinline__ bool better(double v2, int r2, double v, int r) {
-  return r2 >= 0 && (r < 0 || v2 > v || (v2 == v && r2 < r));
-}
-
 // Four wavefronts per workgroup; lane r of every wavefront stands for panel row
 // r0 + r. The panel block is kept column-major in LDS (col[c][r], rows padded to
 // RPW + 1) so that the column operations are bank-conflict free. Column jj runs
@@ -178,11 +216,17 @@ __device__ __forceinline__ bool better(double v2, int r2, double v, int r) {
 //
 // Cross-workgroup exchange per column, without a shared counter: every
 // workgroup writes its candidate row (and the diagonal row, if it owns it) as
-// tagged words, then the slot's two words: (column tag << 32 | row) and the
-// candidate value with the tag in its low 16 mantissa bits (a relative change
-// below 2^-36). Readers poll both words of every slot until both carry the
-// column's tag, then load the winning row, re-reading words until each carries
-// the tag. Slots sit on their own 128-byte lines and alternate between two
+// tagged words, then the slot's two words: the candidate's FP64 magnitude as two
+// tagged 32-bit halves (tag << 32 | half), so candidates of different workgroups
+// compare exactly, as izamax compares them (-1: no candidate). Readers poll both
+// words of every slot until both carry the column's tag and take the first
+// maximum in workgroup order, then load the winning row, re-reading words until
+// each carries the tag. The row index itself never travels: only the winning
+// workgroup needs it (it is its own candidate), and that workgroup records the
+// pivot and swaps with the diagonal row. Every poll is
+// bounded (spin_expired): a wait that gives up sets the workgroup's abort flag in
+// LDS, and after the phase's barrier all four wavefronts leave the column loop
+// together. Slots sit on their own 128-byte lines and alternate between two
 // buffers by column parity; a workgroup can only overwrite a buffer after every
 // workgroup has published the following column, that is after all reads of the
 // buffer ended. Slot words and published rows are cleared before each
@@ -206,6 +250,7 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
   __shared__ T piv[PNB * 2];
   __shared__ T part[4 * PNB * 2];
   __shared__ T xs[INC ? PNB * PNB * 2 : 2];  // X[r][m], lane m
+  __shared__ int gave_up;  // a cross-workgroup wait of this workgroup gave up
   // the panel is the factorization's critical path: its waves issue first on
   // CUs shared with trailing-update GEMMs
   __builtin_amdgcn_s_setprio(3);
@@ -213,6 +258,7 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
   const int64_t r0 = a.j0 + (int64_t)g * RPW;
   const int nr = (int)min((int64_t)RPW, a.n - r0);
   const int nb = a.nb;
+  if (tid == 0) gave_up = 0;
   // stage the block: lane = column, wavefront w takes rows w, w + 4, ...
   for (int rb = wv; rb < nr; rb += 16) {
     T vr[4], vi[4];
@@ -307,66 +353,90 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
     // apply column jj - 1's elimination to the columns > jj (and, complex64,
     // wavefront 1 forms row jj - 1 of the leaf inverse).
     T pr = 0, pim = 0, qr = 0, qi = 0;
-    int p = -1;
+    int pw = -1;                   // the workgroup holding the pivot row
+    bool mine = false, swap = true;  // this workgroup holds it; it is not row cg
     if (wv == 0) {
       if (lane < nb) st_tagged(a.slot_rows + TW * (((int64_t)par * a.maxg + g) * PNB + lane), cr, ci, tag);
       if (own_diag && lane < nb) st_tagged(a.slot_diag + TW * (par * PNB + lane), dr, di, tag);
       if (lane == 0) {
+        // no candidate in this workgroup (every row above the diagonal): -1
+        const double vp = row >= 0 ? v : -1.0;
         unsigned long long* slot = a.slots + SLOT_WORDS * (par * a.maxg + g);
-        const unsigned long long vb = ((unsigned long long)__double_as_longlong(v) & ~0xffffull) | (tag & 0xffffu);
-        st_u64(slot + 1, vb);
-        st_u64(slot, ((unsigned long long)tag << 32) | (unsigned long long)(unsigned)row);
+        const unsigned long long vb = (unsigned long long)__double_as_longlong(vp), tt = (unsigned long long)tag << 32;
+        st_u64(slot, tt | (vb & 0xffffffffull));
+        st_u64(slot + 1, tt | (vb >> 32));
       }
       // gather every slot of this column: lanes poll disjoint slots (up to four
       // each per round); every round re-reads all of a lane's pending slots at
-      // once, so one round trip covers them all
+      // once, so one round trip covers them all. A lane visits its slots in
+      // increasing workgroup order and keeps the first maximum; workgroups hold
+      // increasing row ranges, so the smallest workgroup among equal maxima holds
+      // the first maximum row, as izamax chooses
       double bv = -1.0;
-      int brow = -1;
-      for (int base = lane; base < G; base += 4 * 64) {
+      int bw = -1;
+      bool quit = false;
+      for (int base = lane; base < G && !quit; base += 4 * 64) {
         bool pend[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) pend[u] = base + 64 * u < G;
         bool any = true;
+        Spin sp;
         while (any) {
-          unsigned long long t[4], vb[4];
+          unsigned long long lo[4], hi[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
-            t[u] = vb[u] = 0;
+            lo[u] = hi[u] = 0;
             if (pend[u]) {
               const unsigned long long* slot = a.slots + SLOT_WORDS * (par * a.maxg + base + 64 * u);
-              t[u] = ld_u64(slot);
-              vb[u] = ld_u64(slot + 1);
+              lo[u] = ld_u64(slot);
+              hi[u] = ld_u64(slot + 1);
             }
           }
           any = false;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             if (!pend[u]) continue;
-            if ((t[u] >> 32) == tag && (vb[u] & 0xffffu) == (tag & 0xffffu)) {
+            if ((lo[u] >> 32) == tag && (hi[u] >> 32) == tag) {
               pend[u] = false;
-              const int ri = (int)(unsigned)(t[u] & 0xffffffffu);
-              const double vi = __longlong_as_double((long long)(vb[u] & ~0xffffull));
-              if (better(vi, ri, bv, brow)) {
+              const double vi = __longlong_as_double((long long)((hi[u] << 32) | (lo[u] & 0xffffffffull)));
+              if (vi >= 0.0 && (bw < 0 || vi > bv)) {
                 bv = vi;
-                brow = ri;
+                bw = base + 64 * u;
               }
             } else {
               any = true;
             }
           }
-          if (any) __builtin_amdgcn_s_sleep(1);
+          if (any) {
+            if (spin_expired(a.ctl, sp, SPIN_PANEL_SLOT)) {
+              quit = true;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
         }
       }
-      double pv;
-      wave_pick(bv, brow, brow >= 0, pv, p);
-      const int pwg = (int)((p - a.j0) / RPW);
-      const bool own_p = p != cg && p >= r0 && p < r0 + nr;
-      if (lane < nb) {
-        ld_tagged(a.slot_rows + TW * (((int64_t)par * a.maxg + pwg) * PNB + lane), pr, pim, tag);
-        if (own_p) ld_tagged(a.slot_diag + TW * (par * PNB + lane), qr, qi, tag);
+      // a lane whose wait gave up leaves the exchange to the whole wavefront
+      bool stop = __ballot(quit) != 0;
+      if (!stop) {
+        double pv;
+        wave_pick(bv, bw, bw >= 0, pv, pw);
+        stop = pw < 0;  // every slot was read, so a candidate exists: pw < 0 only on a give-up
       }
+      // the winning workgroup's candidate is its own row
+      mine = pw == g;
+      swap = !(mine && row == cg);
+      if (!stop) {
+        bool ok = true;
+        if (lane < nb) {
+          ok = ld_tagged(a.slot_rows + TW * (((int64_t)par * a.maxg + pw) * PNB + lane), pr, pim, tag, a.ctl);
+          if (mine && swap && ok) ok = ld_tagged(a.slot_diag + TW * (par * PNB + lane), qr, qi, tag, a.ctl);
+        }
+        stop = __ballot(!ok) != 0;
+      }
+      if (stop && lane == 0) gave_up = 1;
       // complete the copies: x[c] -= x[jj - 1] u[c] for c > jj (u = pivot row jj - 1)
-      if (upd) {
+      if (upd && !stop) {
         const T plr = __shfl(pr, jj - 1, 64), pli = __shfl(pim, jj - 1, 64);
         const T qlr = __shfl(qr, jj - 1, 64), qli = __shfl(qi, jj - 1, 64);
         if (lane > jj && lane < nb) {
@@ -406,17 +476,18 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
       if (INC && g == 0 && wv == 1 && jj > 0 && a.linv) inverse_row(jj - 1);
     }
     __syncthreads();
+    if (gave_up) break;  // uniform: read after the barrier by every wavefront
     // B (wavefront 0): row exchange, multipliers of column jj, pivot row to piv
     if (wv == 0) {
       if (lane < nb) {
         piv[2 * lane] = pr;
         piv[2 * lane + 1] = pim;
       }
-      if (g == 0 && lane == 0) a.ipiv[cg] = p;
-      if (p != cg && lane < nb) {
-        if (p >= r0 && p < r0 + nr) {
-          col[2 * (lane * LDR + (p - r0))] = qr;
-          col[2 * (lane * LDR + (p - r0)) + 1] = qi;
+      if (mine && lane == 0) a.ipiv[cg] = row;
+      if (swap && lane < nb) {
+        if (mine) {
+          col[2 * (lane * LDR + (row - r0))] = qr;
+          col[2 * (lane * LDR + (row - r0)) + 1] = qi;
         }
         if (own_diag) {
           col[2 * (lane * LDR + (cg - r0))] = pr;
@@ -430,7 +501,7 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
       } else if (lane < nr && r0 + lane > cg) {
         const T den = vr * vr + vi * vi;
         const T ir = vr / den, ii = -vi / den;
-        const bool at_p = r0 + lane == p;
+        const bool at_p = mine && r0 + lane == row;
         const T ar = at_p ? qjr : col[2 * (jj * LDR + lane)], ai = at_p ? qji : col[2 * (jj * LDR + lane) + 1];
         col[2 * (jj * LDR + lane)] = ar * ir - ai * ii;
         col[2 * (jj * LDR + lane) + 1] = ar * ii + ai * ir;
@@ -438,6 +509,7 @@ __global__ __launch_bounds__(256) void panel_kernel(const PanelArgs<T> a) {
     }
     __syncthreads();
   }
+  if (gave_up) return;  // the factors are abandoned; the host reports the give-up
   for (int r = wv; r < nr; r += 4)
     if (lane < nb) {
       T* dst = a.A + 2 * ((r0 + r) * a.lda + a.j0 + lane);
@@ -625,7 +697,10 @@ struct GemmCfg<double> {
 };
 
 template <typename T, bool SET, int BM, int KB, bool G3 = false>
-__global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int64_t lda, const T* __restrict__ B,
+// B may alias C (U12 = L11^-1 A12 in place): every workgroup then reads all of
+// its rows' K before its epilogue writes them, which needs the whole of m in one
+// row tile (m <= BM, checked by gemm_launch), and B carries no __restrict__.
+__global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int64_t lda, const T* B,
                                                     int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n, int kc,
                                                     int behind) {
   // in a sweep the factorization and solves run beside the next system's
@@ -844,6 +919,8 @@ template <typename T, bool SET>
 hipError_t gemm_launch(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
                        int kc, hipStream_t s, bool behind, bool g3) {
   if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
+  // in place (B == C): one row tile only, so no workgroup writes rows another reads
+  if (static_cast<const void*>(B) == static_cast<const void*>(C) && (m > 64 || ldb != ldc)) return hipErrorInvalidValue;
   // 64 x 64-complex tiles, K in chunks of 16. Measured on gfx950 at the LU's
   // update shapes against 128-row tiles (chunks of 16 or 32), 256-row tiles and
   // K <= 64 staged at once, 64-row tiles are the fastest for every shape in both
@@ -1034,9 +1111,6 @@ __global__ void gather_rhs_kernel(const T* src, int64_t n, int nrhs, const int* 
   dst[2 * idx + 1] = im;
 }
 
-__device__ __forceinline__ void st_flag(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-__device__ __forceinline__ int ld_flag(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-
 template <typename T>
 struct TriArgs {
   const T* LU;
@@ -1047,6 +1121,7 @@ struct TriArgs {
   const T* src;       // n x 8: forward P b, backward y
   T* pub;             // n x 8: block results, written with coherent stores for the other workgroups
   int* flags;         // per leaf: epoch once the leaf's result is published
+  int* ctl;           // wait control of the factorization (abort word, give-up record)
   int epoch;
   int np;
   T* out;             // backward: caller's n x nrhs solution
@@ -1070,7 +1145,9 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   constexpr int YS = PNB * 8 + 6;  // staged block result; 2-complex pad per 16-row group
   __shared__ T ys[2 * YS];
   __shared__ T bs[2 * PNB * 8];
+  __shared__ int gave_up;
   const int t = threadIdx.x, rl = t >> 2, kq = t & 3;
+  if (t == 0) gave_up = 0;
   const int i = FWD ? (int)blockIdx.x : a.np - 1 - (int)blockIdx.x;
   const int64_t base = (int64_t)i * PNB;
   const int nbi = (int)min((int64_t)PNB, a.n - base);
@@ -1107,9 +1184,18 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
         mi[u] = ok ? M[2 * u + 1] : T(0);
       }
     }
-    if (t == 0)
-      while (ld_flag(a.flags + k) != a.epoch) __builtin_amdgcn_s_sleep(1);
+    if (t == 0) {
+      Spin sp;
+      while (ld_flag(a.flags + k) != a.epoch) {
+        if (spin_expired(a.ctl, sp, FWD ? SPIN_TRI_FWD : SPIN_TRI_BWD)) {
+          gave_up = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
     __syncthreads();
+    if (gave_up) return;  // uniform after the barrier; the host reports the give-up
     for (int idx = t; idx < PNB * 8; idx += 256) {
       const int kk = idx >> 3, d = idx & 7;
       T zr = 0, zi = 0;
@@ -1305,6 +1391,11 @@ __global__ void colnorm_final_kernel(const double* part, int nparts, int nrhs, d
 // workgroup is placed once the (finite, non-waiting) work of other streams that
 // holds resources drains. Larger grids go through the cooperative launch, which
 // checks residency itself.
+__global__ void iota_kernel(int* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (int)i;
+}
+
 template <typename Args>
 static hipError_t launch_resident(const void* kernel, void (*fn)(Args), int grid, int block, Args& args,
                                   hipStream_t s) {
@@ -1344,6 +1435,7 @@ static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, 
   pa.slot_diag = reinterpret_cast<unsigned long long*>(w.slot_diag);
   pa.maxg = w.maxg;
   pa.info = w.info;
+  pa.ctl = w.ctl;
   pa.linv = linv;
   return launch_resident<PanelArgs<T>>(reinterpret_cast<const void*>(&panel_kernel<T>), panel_kernel<T>, G, 256, pa, s);
 }
@@ -1388,7 +1480,13 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   if (ncol > lda) return hipErrorInvalidValue;
   const bool g3 = w.gauss3;  // complex64 updates in the three-real-product form
   hipError_t e;
-  if ((e = launch_fill<int>(w.info, 1, 0, s)) != hipSuccess) return e;
+  // zero-pivot report and wait control (info, abort word, give-up record)
+  if ((e = launch_fill<int>(w.info, 3, 0, s)) != hipSuccess) return e;
+  if (w.abort_init && (e = launch_fill<int>(w.ctl, 1, w.abort_init, s)) != hipSuccess) return e;
+  // pivots start as no-op exchanges: columns a give-up leaves unfactored then swap
+  // nothing in the row exchanges and gather tables that follow
+  hipLaunchKernelGGL(iota_kernel, dim3((unsigned)std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0, s, w.ipiv, n);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
   // slot words and published rows carry column tags: clear both
   if ((e = launch_fill<int>(reinterpret_cast<int*>(w.slots), 2 * SLOT_WORDS * 2 * (int64_t)w.maxg, 0, s)) != hipSuccess)
     return e;
@@ -1544,7 +1642,7 @@ hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv
 }
 
 template <typename T>
-hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, hipStream_t s) {
+hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, int* ctl, hipStream_t s) {
   if (nrhs > 8 || nrhs < 1) return hipErrorInvalidValue;
   const SolveLayout L = solve_layout(n);
   int* G = solve_ints(work, L);
@@ -1561,6 +1659,7 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work
   ta.lda = lda;
   ta.bw = MAXBW;
   ta.epoch = epoch;
+  ta.ctl = ctl;
   ta.np = L.np;
   ta.nrhs = nrhs;
   ta.out = B;
@@ -1615,7 +1714,8 @@ hipError_t lu_append_rhs(T* A, int64_t n, int64_t lda, const T* B, int nrhs, hip
 }
 
 template <typename T>
-hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, hipStream_t s) {
+hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work, int* ctl,
+                               hipStream_t s) {
   if (nrhs > 8 || nrhs < 1 || n + nrhs > lda) return hipErrorInvalidValue;
   const SolveLayout L = solve_layout(n);
   int* G = solve_ints(work, L);
@@ -1630,6 +1730,7 @@ hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nr
   ta.lda = lda;
   ta.bw = MAXBW;
   ta.epoch = next_epoch();
+  ta.ctl = ctl;
   ta.np = L.np;
   ta.nrhs = nrhs;
   ta.out = B;
@@ -1644,8 +1745,9 @@ hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nr
 
 template hipError_t lu_append_rhs<float>(float*, int64_t, int64_t, const float*, int, hipStream_t);
 template hipError_t lu_append_rhs<double>(double*, int64_t, int64_t, const double*, int, hipStream_t);
-template hipError_t lu_solve_eliminated<float>(const float*, int64_t, int64_t, float*, int, float*, hipStream_t);
-template hipError_t lu_solve_eliminated<double>(const double*, int64_t, int64_t, double*, int, double*, hipStream_t);
+template hipError_t lu_solve_eliminated<float>(const float*, int64_t, int64_t, float*, int, float*, int*, hipStream_t);
+template hipError_t lu_solve_eliminated<double>(const double*, int64_t, int64_t, double*, int, double*, int*,
+                                                hipStream_t);
 
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
                       hipStream_t s) {
@@ -1814,7 +1916,7 @@ template hipError_t lu_solve_transposed<double>(const double*, int64_t, int64_t,
 
 template hipError_t lu_solve_prepare<float>(const float*, int64_t, int64_t, const int*, float*, hipStream_t);
 template hipError_t lu_solve_prepare<double>(const double*, int64_t, int64_t, const int*, double*, hipStream_t);
-template hipError_t lu_solve<float>(const float*, int64_t, int64_t, float*, int, float*, hipStream_t);
-template hipError_t lu_solve<double>(const double*, int64_t, int64_t, double*, int, double*, hipStream_t);
+template hipError_t lu_solve<float>(const float*, int64_t, int64_t, float*, int, float*, int*, hipStream_t);
+template hipError_t lu_solve<double>(const double*, int64_t, int64_t, double*, int, double*, int*, hipStream_t);
 
 }  // namespace beat

@@ -87,6 +87,11 @@ class Engine:
         self._check(self._lib.beat_ctx_set_cancel_path(self._h, None if path is None else str(path).encode()),
                     "beat_ctx_set_cancel_path")
 
+    def debug_abort_waits(self, on: bool):
+        """Test hook: while on, every factorization's cross-workgroup waits give up at
+        once and the call raises (the error path of a wait that times out)."""
+        self._check(self._lib.beat_ctx_debug_abort_waits(self._h, 1 if on else 0), "beat_ctx_debug_abort_waits")
+
     def reclaim(self):
         """Frees the context's workspaces (reclaim_accelerator_memory!)."""
         self._check(self._lib.beat_ctx_reclaim(self._h), "beat_ctx_reclaim")
@@ -116,15 +121,25 @@ class Engine:
         next job assembled while the current one is factored; returns the pressures
         (N_i x nrhs each) and the per-job reports."""
         nj = len(meshes)
-        qs = [np.asarray(q, dtype=np.complex128).reshape(m.num_faces, -1) for m, q in zip(meshes, qs)]
+        if nj == 0:
+            raise ValueError("a batch needs at least one job")
+        if len(qs) != nj:
+            raise ValueError(f"qs must hold one Neumann block per mesh ({nj}), got {len(qs)}")
+        qs = [np.asarray(q, dtype=np.complex128) for q in qs]
+        for m, q in zip(meshes, qs):
+            if q.shape[0] != m.num_faces or q.ndim > 2:
+                raise ValueError("Fused Burton-Miller assembly needs one Neumann row per DP0 dof.")
+        qs = [q.reshape(m.num_faces, -1) for m, q in zip(meshes, qs)]
         nrhs = qs[0].shape[1]
         if any(q.shape[1] != nrhs for q in qs):
             raise ValueError("every job needs the same number of right-hand sides")
+        k = np.ascontiguousarray(ks, dtype=np.float64)
+        c = np.asarray(couplings, dtype=np.complex128)
+        if k.shape != (nj,) or c.shape != (nj,):
+            raise ValueError(f"ks and couplings must have shape ({nj},), got {k.shape} and {c.shape}")
         qh = np.concatenate([np.asfortranarray(q).ravel(order="F") for q in qs])
         ph = np.zeros(sum(m.num_vertices for m in meshes) * nrhs, dtype=np.complex128)
         handles = (ctypes.c_void_p * nj)(*[m._h.value for m in meshes])
-        k = np.ascontiguousarray(ks, dtype=np.float64)
-        c = np.asarray(couplings, dtype=np.complex128)
         er, ei = np.ascontiguousarray(c.real), np.ascontiguousarray(c.imag)
         reps = (_native.SolveReport * nj)()
         opts = _native.SolveOptions(int(max_steps), float(rtol))
@@ -314,6 +329,8 @@ class DeviceMesh:
         qs = np.asarray(qs, dtype=np.complex128)
         if qs.ndim == 2:
             qs = qs[:, :, None]
+        if qs.ndim != 3:
+            raise ValueError("qs must have shape (nfreq, F) or (nfreq, F, nrhs)")
         nfreq, nf, nrhs = qs.shape
         if nf != self.num_faces:
             raise ValueError("Fused Burton-Miller assembly needs one Neumann row per DP0 dof.")
@@ -322,6 +339,8 @@ class DeviceMesh:
         ph = np.zeros((nfreq, nrhs, self.num_vertices), dtype=np.complex128)
         k = np.ascontiguousarray(ks, dtype=np.float64)
         c = np.asarray(couplings, dtype=np.complex128)
+        if k.shape != (nfreq,) or c.shape != (nfreq,):
+            raise ValueError(f"ks and couplings must have shape ({nfreq},), got {k.shape} and {c.shape}")
         er = np.ascontiguousarray(c.real)
         ei = np.ascontiguousarray(c.imag)
         reps = (_native.SolveReport * nfreq)()

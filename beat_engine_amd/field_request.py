@@ -148,8 +148,8 @@ def retained_bem_field_evaluation_cache(request, request_path, precision, backen
     fv, fshape = read_field_binary_array(request, request_path, "boundary_triangles", "int64")
     if not (len(fshape) == 2 and fshape[1] == 3):
         raise ValueError("BEM boundary triangles must have shape (bem_face, 3).")
-    mesh = BoundaryMesh(vv.reshape(-1, 3).astype(np.float64), fv.reshape(-1, 3).astype(np.int32),
-                        np.ones(fshape[0]))
+    # int64 indices go to BoundaryMesh as they are: it range-checks them before narrowing
+    mesh = BoundaryMesh(vv.reshape(-1, 3).astype(np.float64), fv.reshape(-1, 3), np.ones(fshape[0]))
     dm = exterior.DeviceMesh(engine, mesh, quadrature_order, 4, symmetry)
     entry = (mesh, dm)
     caches.put(key, entry)

@@ -25,14 +25,18 @@ class BoundaryMesh:
 
     def __post_init__(self):
         self.vertices = np.ascontiguousarray(self.vertices, dtype=np.float64)
-        self.faces = np.ascontiguousarray(self.faces, dtype=np.int32)
-        self.physical_tags = np.ascontiguousarray(self.physical_tags, dtype=np.int64)
+        faces = np.asarray(self.faces)
+        if not np.issubdtype(faces.dtype, np.integer):
+            raise ValueError("faces must hold integer vertex indices")
         if self.vertices.ndim != 2 or self.vertices.shape[1] != 3:
             raise ValueError("vertices must be (nv, 3)")
-        if self.faces.ndim != 2 or self.faces.shape[1] != 3:
+        if faces.ndim != 2 or faces.shape[1] != 3:
             raise ValueError("faces must be (nf, 3)")
-        if self.faces.size and (self.faces.min() < 0 or self.faces.max() >= len(self.vertices)):
+        # range-checked in the caller's integer type, before the narrowing to int32
+        if faces.size and (int(faces.min()) < 0 or int(faces.max()) >= len(self.vertices)):
             raise ValueError("face vertex index out of range")
+        self.faces = np.ascontiguousarray(faces, dtype=np.int32)
+        self.physical_tags = np.ascontiguousarray(self.physical_tags, dtype=np.int64)
         v1 = self.vertices[self.faces[:, 0]]
         v2 = self.vertices[self.faces[:, 1]]
         v3 = self.vertices[self.faces[:, 2]]

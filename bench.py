@@ -11,7 +11,9 @@ steps run as one frequency sweep in which the next frequency is assembled while
 the current one is factored (beat_bm_sweep_device); the same solves one call at a
 time are reported beside it. The plain complex128 GPU path is timed on the same
 inputs for the ratio, and the refined pressures are compared with it (relative
-L2 per drive).
+L2 per drive). The CPU baseline solves one frequency of the workload whole on the
+host (the oracle's fused FP64 assembly of every pair, LAPACK getrf/getrs), and the
+GPU's pressures at that frequency are compared with it (rel_err_vs_cpu_oracle).
 
 Prints one JSON line. --gpus above 1 is not supported: this build runs on one GPU.
 """
@@ -54,8 +56,8 @@ def parse():
                          "freq-solve of the workload; prints no JSON line")
     ap.add_argument("--no-rocsolver", action="store_true",
                     help="skip timing rocSOLVER getrf (the vendor LU) at the workload's size")
-    ap.add_argument("--cpu-sample-faces", type=int, default=192)
-    ap.add_argument("--cpu-lu-n", type=int, default=3072)
+    ap.add_argument("--cpu-freq-index", type=int, default=11,
+                    help="which of the 20 frequencies the CPU baseline solves (and the GPU result is checked at)")
     return ap.parse_args()
 
 
@@ -63,54 +65,56 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mesh, nrhs, sample_faces, lu_n):
-    """Oracle (C++ port of the reference CPU path, OpenMP) on a bounded sample:
-    regular pairs of `sample_faces` test faces against all trial faces, scaled
-    to all test faces; the full singular pass; complex128 LAPACK getrf (scipy's
-    OpenBLAS) at n = lu_n scaled by n^3; solve of all drives at full size."""
+def host_threads():
+    """Host threads for the CPU baseline: the process's CPU affinity, capped by
+    OMP_NUM_THREADS when it is set (the GPU box sets it to its CPU share)."""
+    cores = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        cores = min(cores, int(env))
+    return cores
+
+
+def cpu_baseline(mesh, k, eta, q):
+    """One whole freq-solve of the workload on the host, unextrapolated: the
+    oracle's fused FP64 assembly over every test face (the C++ restatement of the
+    reference's CPU direct_system path, OpenMP over coloured test elements), then
+    LAPACK zgetrf and zgetrs for every drive (scipy's OpenBLAS 0.3.29, the version
+    the reference pins). Returns the timing record and the pressures."""
     import scipy.linalg as sl
+    from threadpoolctl import threadpool_info, threadpool_limits
 
     import oracle
 
-    cores = len(os.sched_getaffinity(0))
+    cores = host_threads()
     oracle.set_threads(cores)
-    w = 2 * math.pi * 1000.0
-    k = w / 343.0
-    eta = complex(0.0, 1.0 / k)
-    rng = np.random.default_rng(5)
-    q = 1j * 1.21 * w * (rng.normal(size=(mesh.num_faces, nrhs)) + 1j * rng.normal(size=(mesh.num_faces, nrhs)))
     t = time.perf_counter()
-    oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q, test_range=(0, 0), singular=False, identity=False)
-    t_alloc = time.perf_counter() - t
-    t = time.perf_counter()
-    oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q, test_range=(0, sample_faces), singular=False,
-                       identity=False)
-    t_reg = max(time.perf_counter() - t - t_alloc, 1e-9)
-    t = time.perf_counter()
-    oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q, test_range=(0, 0), singular=True, identity=True)
-    t_sing = max(time.perf_counter() - t - t_alloc, 0.0)
-    asm = t_reg * mesh.num_faces / sample_faces + t_sing
-    n = mesh.num_vertices
-    A = rng.normal(size=(lu_n, lu_n)) + 1j * rng.normal(size=(lu_n, lu_n)) + 40 * np.eye(lu_n)
-    t = time.perf_counter()
-    lu = sl.lu_factor(A, check_finite=False, overwrite_a=True)
-    t_lu = (time.perf_counter() - t) * (n / lu_n) ** 3
-    B = rng.normal(size=(lu_n, nrhs)) + 0j
-    t = time.perf_counter()
-    sl.lu_solve(lu, B, check_finite=False)
-    t_sv = (time.perf_counter() - t) * (n / lu_n) ** 2
-    total = asm + t_lu + t_sv
+    A, B = oracle.bm_assemble(mesh.vertices, mesh.faces, k, eta, q)
+    t_asm = time.perf_counter() - t
+    with threadpool_limits(cores):
+        blas = [(i.get("internal_api"), i.get("version"), i.get("num_threads")) for i in threadpool_info()
+                if i.get("user_api") == "blas"]
+        t = time.perf_counter()
+        lu = sl.lu_factor(A, check_finite=False, overwrite_a=True)
+        t_lu = time.perf_counter() - t
+        t = time.perf_counter()
+        p = sl.lu_solve(lu, B, check_finite=False)
+        t_sv = time.perf_counter() - t
+    del A, lu
+    total = t_asm + t_lu + t_sv
     return {
         "value": 1.0 / total,
         "unit": "freq-solves/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"fused FP64 assembly of {sample_faces} of {mesh.num_faces} test faces x all trial faces "
-                   f"({t_reg:.2f} s) scaled to all, full singular pass ({t_sing:.2f} s), complex128 getrf at "
-                   f"n={lu_n} scaled by (n/{lu_n})^3 ({t_lu:.2f} s), getrs x{nrhs} ({t_sv:.3f} s); "
-                   f"{cores} threads"),
-        "s_per_freq": {"assembly": asm, "lu": t_lu, "solve": t_sv},
-    }
+        "sample": (f"one whole freq-solve of the workload at k = {k:.4f} (unextrapolated): oracle fused FP64 "
+                   f"assembly of all {mesh.num_faces} test faces with {cores} OpenMP threads ({t_asm:.1f} s), "
+                   f"complex128 getrf at n = {mesh.num_vertices} ({t_lu:.2f} s) and getrs of {q.shape[1]} drives "
+                   f"({t_sv:.3f} s) with BLAS {blas}"),
+        "s_per_freq": {"assembly": t_asm, "lu": t_lu, "solve": t_sv},
+        "omp_threads": cores,
+        "blas": [{"api": b[0], "version": b[1], "threads": b[2]} for b in blas],
+    }, p
 
 
 def rocsolver_getrf_ms(n, reps=2):
@@ -425,10 +429,33 @@ def main():
         log("rocSOLVER getrf comparison:", rs)
         out["lu_vs_rocsolver"] = rs
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(mesh, nrhs, args.cpu_sample_faces, args.cpu_lu_n)
+        # one frequency of the workload solved whole on the host, and the GPU's
+        # refined and complex128 pressures at that frequency compared with it
+        ci = args.cpu_freq_index % len(freqs)
+        fc = freqs[ci]
+        kc = 2 * math.pi * fc / 343.0
+        etac = burton_miller_coupling(kc)
+        step(ci, _native.PREC_FP32_REFINED, p_mixed)
+        step(ci, _native.PREC_FP64, p_fp64)
+        torch.cuda.synchronize()
+        g_ref = p_mixed.cpu().numpy().T
+        g_64 = p_fp64.cpu().numpy().T
+        eng.close()
+        del qs, p_mixed, p_fp64, p_sweep
+        torch.cuda.empty_cache()
+        log(f"CPU baseline: one whole freq-solve at {fc:.1f} Hz on {host_threads()} threads")
+        cb, p_cpu = cpu_baseline(mesh, kc, etac, neumann_scale(1.21, 2 * math.pi * fc) * v)
+        errs = [float(np.linalg.norm(g_ref[:, j] - p_cpu[:, j]) / np.linalg.norm(p_cpu[:, j])) for j in range(nrhs)]
+        errs64 = [float(np.linalg.norm(g_64[:, j] - p_cpu[:, j]) / np.linalg.norm(p_cpu[:, j])) for j in range(nrhs)]
+        log(f"CPU baseline {cb['s_per_freq']}; refined GPU vs CPU oracle max rel L2 {max(errs):.2e}, "
+            f"complex128 GPU {max(errs64):.2e}")
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = out["value"] / cb["value"]
-    eng.close()
+        out["rel_err_vs_cpu_oracle"] = max(errs)
+        out["rel_err_vs_cpu_oracle_detail"] = {"frequency_hz": float(fc), "refined_per_rhs": errs,
+                                               "fp64_gpu_max": max(errs64)}
+    else:
+        eng.close()
     print(json.dumps(out), flush=True)
 
 

@@ -102,6 +102,12 @@ const char* beat_last_error(const beat_ctx* ctx);
  * this library evaluates (ctx may be NULL: no device query); returns the buffer
  * length needed including the terminating zero and writes at most len bytes. */
 int beat_ctx_set_cancel_path(beat_ctx* ctx, const char* path);
+/* Test hook (no counterpart in the reference): while on, every factorization
+ * starts with its device abort word set, so each bounded cross-workgroup wait of
+ * the factorization and of its triangular solves gives up at once; the kernels
+ * end normally and the call returns BEAT_ERR_HIP with a message (the same path a
+ * wait that times out after 4 s takes). For tests of that error path only. */
+int beat_ctx_debug_abort_waits(beat_ctx* ctx, int on);
 int beat_ctx_reclaim(beat_ctx* ctx);
 int beat_ctx_memory_info(beat_ctx* ctx, int64_t* info);
 int64_t beat_capabilities(const beat_ctx* ctx, char* buf, int64_t len);
