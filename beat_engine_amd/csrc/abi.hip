@@ -63,12 +63,12 @@ struct beat_ctx {
   std::string err;
   std::mutex mu;
   DevBuf a64, a32, b64, b32, q64, q32, x64, r64, n64, rn64, xn64, d32, work, chunk, norms, ipiv, lusmall, qin, pout,
-      fld_src, fld_out, fld_pts, grad, adj;
+      fld_src, fld_out, fld_pts, grad, adj, af;
   hipEvent_t ev[7] = {};
   // frequency sweeps: the assembly stream, two systems in flight, their events
   // (assembly start, regular start, regular end, assembly end) and release events
   hipStream_t asm_stream = nullptr;
-  DevBuf sw_q64[2], sw_q32[2], sw_a64[2], sw_b64[2], sw_a32[2], sw_b32[2];
+  DevBuf sw_q64[2], sw_q32[2], sw_a64[2], sw_b64[2], sw_a32[2], sw_b32[2], sw_af[2];
   hipEvent_t sw_ev[2][4] = {};
   hipEvent_t sw_free[2] = {};
   // the regular kernel at one workgroup per CU while it shares the device with
@@ -97,6 +97,9 @@ struct beat_ctx {
   // test hook (beat_ctx_debug_abort_waits): factorizations start with their abort
   // word set, so every cross-workgroup wait gives up at once
   int debug_abort_waits = 0;
+  // test hook (beat_ctx_debug_force_exchange): speculative factorizations report
+  // an exchange, so every solve takes its second, exchange factorization
+  bool debug_force_exchange = false;
 };
 
 struct beat_mesh {
@@ -281,6 +284,7 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   w.info = reinterpret_cast<int*>(take(sizeof(int) * 4));
   w.ctl = w.info + 1;
   w.abort_init = ctx->debug_abort_waits;
+  w.force_exchange = ctx->debug_force_exchange;
   ctx->lu_ctl = w.ctl;
   w.slots = reinterpret_cast<unsigned long long*>(take(sizeof(unsigned long long) * 32 * maxg));  // 2 x maxg slots of 16 words
   w.slot_rows = take(slot_rows);
@@ -300,11 +304,13 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
 // The factorization's zero-pivot report and wait control, read after the stream
 // has drained: BEAT_ERR_HIP when a cross-workgroup wait of the factorization or of
 // its solves gave up (the results are then abandoned), else *info = the first
-// zero pivot (1-based, 0 if none).
-int lu_outcome(beat_ctx* ctx, const LuWork& w, int* info) {
-  int st[3] = {0, 0, 0};
+// zero pivot (1-based, 0 if none) and *exchange_needed = a speculative
+// factorization found a column where partial pivoting exchanges rows.
+int lu_outcome(beat_ctx* ctx, const LuWork& w, int* info, bool* exchange_needed = nullptr) {
+  int st[4] = {0, 0, 0, 0};
   HIPCHK(hipMemcpy(st, w.info, sizeof(st), hipMemcpyDeviceToHost));
   *info = st[0];
+  if (exchange_needed) *exchange_needed = st[3] != 0;
   if (st[1] != 0 || st[2] != 0) {
     std::string where;
     if (st[2] == 0) where = "the abort word was set";
@@ -723,6 +729,10 @@ int check_mesh(beat_ctx* ctx, const beat_mesh* m) {
 // path, or the complex64 copy factored by the refined path.
 struct SystemBufs {
   DevBuf *q64, *q32, *a64, *b64, *a32, *b32;
+  // the matrix the FP64 and FP32 paths factor: a copy of the assembled one (a64,
+  // a32), which stays as assembled for a second factorization with exchanges
+  // (the refined path factors a32, its copy of a64)
+  DevBuf* af;
 };
 
 // Timing events of one system: assembly start, regular passes, assembly end.
@@ -751,6 +761,8 @@ int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, d
     if (int rc = assemble<float>(ctx, m, k, er, ei, as<double>(*sb.q64), as<float>(*sb.q32), nrhs, as<float>(*sb.a32),
                                  lda, as<float>(*sb.b32), st, ev.reg0, ev.reg1, one_per_cu, narrow))
       return rc;
+    HIPCHK(ensure(*sb.af, sizeof(float) * 2 * n * lda));
+    HIPCHK(hipMemcpyAsync(sb.af->p, sb.a32->p, sizeof(float) * 2 * n * lda, hipMemcpyDeviceToDevice, st));
   } else {
     HIPCHK(ensure(*sb.a64, sizeof(double) * 2 * n * lda));
     HIPCHK(ensure(*sb.b64, sizeof(double) * 2 * n * nrhs));
@@ -758,6 +770,10 @@ int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, d
                                   as<double>(*sb.a64), lda, as<double>(*sb.b64), st, ev.reg0, ev.reg1, one_per_cu,
                                   narrow))
       return rc;
+    if (precision == BEAT_PREC_FP64) {
+      HIPCHK(ensure(*sb.af, sizeof(double) * 2 * n * lda));
+      HIPCHK(hipMemcpyAsync(sb.af->p, sb.a64->p, sizeof(double) * 2 * n * lda, hipMemcpyDeviceToDevice, st));
+    }
     if (precision == BEAT_PREC_FP32_REFINED) {
       // the complex64 copy the refined path factors, made here so that in a sweep
       // it is off the factorization's stream
@@ -788,59 +804,91 @@ int stage_solve(beat_ctx* ctx, const beat_mesh* m, int precision, int nrhs, Syst
     lw.side = nullptr;
     lw.inb = nullptr;
   }
-  int panels = 0;
+  int panels = 0, info = 0, factorizations = 0;
   Refine R;
   // the right-hand sides ride along in the padding columns of the factored
   // matrix when they fit: the factorization then does the first forward solve
   const bool carry = nrhs <= SOLVE_CHUNK && n + nrhs <= lda;
   const int64_t nc = carry ? n + nrhs : 0;
-  HIPCHK(hipEventRecord(ctx->ev[1], s));
-  if (precision == BEAT_PREC_FP64) {
-    if (carry) HIPCHK(lu_append_rhs<double>(as<double>(*sb.a64), n, lda, as<double>(*sb.b64), nrhs, s));
-    HIPCHK(lu_factor<double>(as<double>(*sb.a64), n, lda, lw, LU_NB, s, &panels, nc));
-    HIPCHK(hipEventRecord(ctx->ev[2], s));
-    int rc = carry ? solve_eliminated<double>(ctx, as<double>(*sb.a64), n, lda, lw.ipiv, as<double>(*sb.b64), nrhs)
-                   : solve_all<double>(ctx, as<double>(*sb.a64), n, lda, lw.ipiv, as<double>(*sb.b64), nrhs, true);
-    if (rc) return rc;
-    hipLaunchKernelGGL((scatter_p_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
-                       (const double*)as<double>(*sb.b64), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
-    HIPCHK(hipGetLastError());
-  } else if (precision == BEAT_PREC_FP32) {
-    if (carry) HIPCHK(lu_append_rhs<float>(as<float>(*sb.a32), n, lda, as<float>(*sb.b32), nrhs, s));
-    HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels, nc));
-    HIPCHK(hipEventRecord(ctx->ev[2], s));
-    int rc = carry ? solve_eliminated<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs)
-                   : solve_all<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs, true);
-    if (rc) return rc;
-    hipLaunchKernelGGL((scatter_p_kernel<float>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
-                       (const float*)as<float>(*sb.b32), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
-    HIPCHK(hipGetLastError());
-  } else {
-    HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * n * nrhs));
-    if (carry) HIPCHK(lu_append_rhs<float>(as<float>(*sb.a32), n, lda, as<float>(*sb.b32), nrhs, s));
-    // these factors only precondition the complex128 refinement: the trailing
-    // updates may use the three-real-product complex form
-    lw.gauss3 = true;
-    HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels, nc));
-    HIPCHK(hipEventRecord(ctx->ev[2], s));
-    int rc = carry ? solve_eliminated<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs)
-                   : solve_all<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, as<float>(*sb.b32), nrhs, true);
-    if (rc) return rc;
-    hipLaunchKernelGGL(upcast_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, s, (const float*)as<float>(*sb.b32),
-                       as<double>(ctx->x64), 2 * n * nrhs);
-    HIPCHK(hipGetLastError());
-    rc = refine_loop(ctx, as<float>(*sb.a32), lda, lw.ipiv, as<double>(*sb.a64), lda, as<double>(*sb.b64),
-                     as<double>(ctx->x64), n, nrhs, max_steps, rtol, false, R);
-    if (rc) return rc;
-    hipLaunchKernelGGL((scatter_p_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
-                       (const double*)as<double>(ctx->x64), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
-    HIPCHK(hipGetLastError());
+  // First the speculative factorization (no row exchanges, every column checked
+  // against partial pivoting's choice); if a column would have exchanged rows,
+  // the matrix is taken again as assembled and factored with the exchange panels.
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    lw.nopiv = attempt == 0;
+    if (attempt == 1) {
+      // the factored buffer back to the assembled matrix
+      if (precision == BEAT_PREC_FP64)
+        HIPCHK(hipMemcpyAsync(sb.af->p, sb.a64->p, sizeof(double) * 2 * n * lda, hipMemcpyDeviceToDevice, s));
+      else if (precision == BEAT_PREC_FP32)
+        HIPCHK(hipMemcpyAsync(sb.af->p, sb.a32->p, sizeof(float) * 2 * n * lda, hipMemcpyDeviceToDevice, s));
+      else
+        HIPCHK(launch_downcast(as<double>(*sb.a64), as<float>(*sb.a32), 2 * n * lda, s));
+    }
+    ++factorizations;
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    if (precision == BEAT_PREC_FP64) {
+      double* F = as<double>(*sb.af);
+      if (carry) HIPCHK(lu_append_rhs<double>(F, n, lda, as<double>(*sb.b64), nrhs, s));
+      HIPCHK(lu_factor<double>(F, n, lda, lw, LU_NB, s, &panels, nc));
+      HIPCHK(hipEventRecord(ctx->ev[2], s));
+      // the solution overwrites b64 (kept for a second attempt: lu_append_rhs copied it)
+      HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * n * nrhs));
+      double* X = as<double>(ctx->x64);
+      HIPCHK(hipMemcpyAsync(X, sb.b64->p, sizeof(double) * 2 * n * nrhs, hipMemcpyDeviceToDevice, s));
+      int rc = carry ? solve_eliminated<double>(ctx, F, n, lda, lw.ipiv, X, nrhs)
+                     : solve_all<double>(ctx, F, n, lda, lw.ipiv, X, nrhs, true);
+      if (rc) return rc;
+      hipLaunchKernelGGL((scatter_p_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s, (const double*)X, n,
+                         nrhs, (const int32_t*)m->vert_inv, p_dev);
+      HIPCHK(hipGetLastError());
+    } else if (precision == BEAT_PREC_FP32) {
+      float* F = as<float>(*sb.af);
+      if (carry) HIPCHK(lu_append_rhs<float>(F, n, lda, as<float>(*sb.b32), nrhs, s));
+      HIPCHK(lu_factor<float>(F, n, lda, lw, LU_NB, s, &panels, nc));
+      HIPCHK(hipEventRecord(ctx->ev[2], s));
+      HIPCHK(ensure(ctx->d32, sizeof(float) * 2 * n * nrhs + sizeof(int) * 64 + 4096));
+      float* X = as<float>(ctx->d32);
+      HIPCHK(hipMemcpyAsync(X, sb.b32->p, sizeof(float) * 2 * n * nrhs, hipMemcpyDeviceToDevice, s));
+      int rc = carry ? solve_eliminated<float>(ctx, F, n, lda, lw.ipiv, X, nrhs)
+                     : solve_all<float>(ctx, F, n, lda, lw.ipiv, X, nrhs, true);
+      if (rc) return rc;
+      hipLaunchKernelGGL((scatter_p_kernel<float>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s, (const float*)X, n,
+                         nrhs, (const int32_t*)m->vert_inv, p_dev);
+      HIPCHK(hipGetLastError());
+    } else {
+      HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * n * nrhs));
+      if (carry) HIPCHK(lu_append_rhs<float>(as<float>(*sb.a32), n, lda, as<float>(*sb.b32), nrhs, s));
+      // these factors only precondition the complex128 refinement: the trailing
+      // updates may use the three-real-product complex form
+      lw.gauss3 = true;
+      HIPCHK(lu_factor<float>(as<float>(*sb.a32), n, lda, lw, LU_NB, s, &panels, nc));
+      HIPCHK(hipEventRecord(ctx->ev[2], s));
+      // the first solution in x64 through d32 (b32 stays for a second attempt; the
+      // refinement's residuals are complex128, from b64)
+      HIPCHK(ensure(ctx->d32, sizeof(float) * 2 * n * nrhs + sizeof(int) * 64 + 4096));
+      float* X = as<float>(ctx->d32);
+      HIPCHK(hipMemcpyAsync(X, sb.b32->p, sizeof(float) * 2 * n * nrhs, hipMemcpyDeviceToDevice, s));
+      int rc = carry ? solve_eliminated<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, X, nrhs)
+                     : solve_all<float>(ctx, as<float>(*sb.a32), n, lda, lw.ipiv, X, nrhs, true);
+      if (rc) return rc;
+      hipLaunchKernelGGL(upcast_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, s, (const float*)X,
+                         as<double>(ctx->x64), 2 * n * nrhs);
+      HIPCHK(hipGetLastError());
+      rc = refine_loop(ctx, as<float>(*sb.a32), lda, lw.ipiv, as<double>(*sb.a64), lda, as<double>(*sb.b64),
+                       as<double>(ctx->x64), n, nrhs, max_steps, rtol, false, R);
+      if (rc) return rc;
+      hipLaunchKernelGGL((scatter_p_kernel<double>), dim3(blocks_for(n * nrhs)), dim3(256), 0, s,
+                         (const double*)as<double>(ctx->x64), n, nrhs, (const int32_t*)m->vert_inv, p_dev);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(ctx->ev[3], s));
+    // the factorization's zero-pivot report, wait control and speculation check,
+    // after the stream drained
+    HIPCHK(hipStreamSynchronize(s));
+    bool exchange = false;
+    if (int rc = lu_outcome(ctx, lw, &info, &exchange)) return rc;
+    if (!exchange) break;
   }
-  HIPCHK(hipEventRecord(ctx->ev[3], s));
-  // the factorization's zero-pivot report and wait control, after the stream drained
-  HIPCHK(hipStreamSynchronize(s));
-  int info = 0;
-  if (int rc = lu_outcome(ctx, lw, &info)) return rc;
   if (rep) {
     float ta, tr, t12, t23;
     HIPCHK(hipEventElapsedTime(&ta, asm_ev.start, asm_ev.end));
@@ -851,7 +899,7 @@ int stage_solve(beat_ctx* ctx, const beat_mesh* m, int precision, int nrhs, Syst
     rep->assembly_s = ta * 1e-3;
     rep->factor_s = t12 * 1e-3;
     rep->solve_s = t23 * 1e-3;
-    rep->factorization_count = 1;
+    rep->factorization_count = factorizations;
     rep->lu_panels = panels;
     rep->regular_pairs = m->regular_pairs;
     rep->singular_pairs = (int64_t)m->h.spairs.size();
@@ -883,7 +931,7 @@ int check_solve_options(beat_ctx* ctx, int precision, const beat_solve_options* 
 int solve_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei, const double* q_dev,
                int nrhs, double* p_dev, const beat_solve_options* opts, beat_solve_report* rep) {
   if (int rc = check_solve_options(ctx, precision, opts)) return rc;
-  SystemBufs sb{&ctx->q64, &ctx->q32, &ctx->a64, &ctx->b64, &ctx->a32, &ctx->b32};
+  SystemBufs sb{&ctx->q64, &ctx->q32, &ctx->a64, &ctx->b64, &ctx->a32, &ctx->b32, &ctx->af};
   const AsmEvents ev{ctx->ev[0], ctx->ev[4], ctx->ev[5], ctx->ev[6]};
   if (int rc = stage_assemble(ctx, m, precision, k, er, ei, q_dev, nrhs, sb, ctx->stream, ev)) return rc;
   return stage_solve(ctx, m, precision, nrhs, sb, p_dev, opts, ev, rep);
@@ -901,8 +949,8 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
   if (njobs <= 0) return BEAT_OK;
   SystemBufs sb[2];
   for (int s2 = 0; s2 < 2; ++s2) {
-    sb[s2] = SystemBufs{&ctx->sw_q64[s2], &ctx->sw_q32[s2], &ctx->sw_a64[s2],
-                        &ctx->sw_b64[s2], &ctx->sw_a32[s2], &ctx->sw_b32[s2]};
+    sb[s2] = SystemBufs{&ctx->sw_q64[s2], &ctx->sw_q32[s2], &ctx->sw_a64[s2], &ctx->sw_b64[s2],
+                        &ctx->sw_a32[s2], &ctx->sw_b32[s2], &ctx->sw_af[s2]};
   }
   // size both systems for the largest job up front (no reallocation while in flight)
   int64_t nmax = 0, fmax = 0;
@@ -921,6 +969,9 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
       HIPCHK(ensure(*sb[s2].a64, sizeof(double) * 2 * nmax * lda_for(nmax)));
       HIPCHK(ensure(*sb[s2].b64, sizeof(double) * 2 * nmax * nrhs));
     }
+    if (precision != BEAT_PREC_FP32_REFINED)
+      HIPCHK(ensure(*sb[s2].af, (precision == BEAT_PREC_FP64 ? sizeof(double) : sizeof(float)) * 2 * nmax *
+                                    lda_for(nmax)));
   }
   {
     // the factorization, solve and refinement workspaces too
@@ -1213,12 +1264,12 @@ void beat_ctx_destroy(beat_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64,
                     &c->rn64, &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
-                    &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj};
+                    &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   if (c->asm_stream) (void)hipStreamSynchronize(c->asm_stream);
   for (int s2 = 0; s2 < 2; ++s2) {
-    DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2], &c->sw_b32[s2]};
+    DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2], &c->sw_b32[s2], &c->sw_af[s2]};
     for (DevBuf* b : sw)
       if (b->p) (void)hipFree(b->p);
     for (auto& e : c->sw_ev[s2])
@@ -1255,6 +1306,13 @@ int beat_ctx_debug_abort_waits(beat_ctx* ctx, int on) {
   return BEAT_OK;
 }
 
+int beat_ctx_debug_force_exchange(beat_ctx* ctx, int on) {
+  if (!ctx) return BEAT_ERR_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->debug_force_exchange = on != 0;
+  return BEAT_OK;
+}
+
 int beat_ctx_reclaim(beat_ctx* c) {
   if (!c) return BEAT_ERR_ARG;
   for (beat_ctx* a : c->aux)
@@ -1268,14 +1326,14 @@ int beat_ctx_reclaim(beat_ctx* c) {
   if (c->inb) HIPCHK(hipStreamSynchronize(c->inb));
   DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64, &c->rn64,
                     &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin, &c->pout,
-                    &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj};
+                    &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af};
   for (DevBuf* b : bufs) {
     if (b->p) HIPCHK(hipFree(b->p));
     b->p = nullptr;
     b->bytes = 0;
   }
   for (int s2 = 0; s2 < 2; ++s2) {
-    DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2], &c->sw_b32[s2]};
+    DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2], &c->sw_b32[s2], &c->sw_af[s2]};
     for (DevBuf* b : sw) {
       if (b->p) HIPCHK(hipFree(b->p));
       b->p = nullptr;
@@ -1293,11 +1351,11 @@ int beat_ctx_memory_info(beat_ctx* c, int64_t* info) {
   size_t used = 0;
   const DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64, &c->rn64,
                           &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
-                          &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj};
+                          &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af};
   for (const DevBuf* b : bufs) used += b->bytes;
   for (int s2 = 0; s2 < 2; ++s2) {
     const DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2],
-                          &c->sw_b32[s2]};
+                          &c->sw_b32[s2], &c->sw_af[s2]};
     for (const DevBuf* b : sw) used += b->bytes;
   }
   size_t free_b = 0, total_b = 0;
@@ -1830,6 +1888,7 @@ int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, 
     beat_ctx* a = ctx->aux[w - 1];
     a->cancel_path = ctx->cancel_path;
     a->debug_abort_waits = ctx->debug_abort_waits;
+    a->debug_force_exchange = ctx->debug_force_exchange;
     a->sweep_asm_one_per_cu = ctx->sweep_asm_one_per_cu;
     std::memcpy(a->lu_layout, ctx->lu_layout, sizeof(ctx->lu_layout));
     std::memcpy(a->asm_occ, ctx->asm_occ, sizeof(ctx->asm_occ));
@@ -1912,7 +1971,10 @@ int beat_dense_lu_solve(beat_ctx* ctx, int precision, const double* a, int64_t n
   LuWork lw;
   if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
   HIPCHK(ensure(ctx->pout, sizeof(double) * 2 * n * nrhs));
-  int panels = 0;
+  int panels = 0, info = 0;
+  // speculative factorization first; with exchanges when a column needs them
+  for (int attempt = 0; attempt < 2; ++attempt) {
+  lw.nopiv = attempt == 0;
   if (precision == BEAT_PREC_FP64) {
     HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
     HIPCHK(ensure(ctx->b64, sizeof(double) * 2 * n * nrhs));
@@ -1943,8 +2005,10 @@ int beat_dense_lu_solve(beat_ctx* ctx, int precision, const double* a, int64_t n
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(x, ctx->pout.p, sizeof(double) * 2 * n * nrhs, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
-  int info = 0;
-  if (int rc = lu_outcome(ctx, lw, &info)) return rc;
+  bool exchange = false;
+  if (int rc = lu_outcome(ctx, lw, &info, &exchange)) return rc;
+  if (!exchange) break;
+  }
   if (info != 0) return fail(ctx, BEAT_ERR_SINGULAR, "LU found an exactly zero pivot at column " + std::to_string(info));
   return BEAT_OK;
 }
@@ -1993,25 +2057,33 @@ int beat_dense_lu_factor(beat_ctx* ctx, int precision, const double* a, int64_t 
   HIPCHK(hipMemcpyAsync(ain, a, sizeof(double) * 2 * n * n, hipMemcpyHostToDevice, s));
   LuWork lw;
   if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
-  int panels = 0, rc;
-  if (precision == BEAT_PREC_FP64) {
-    HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
-    hipLaunchKernelGGL((transpose_in_kernel<double>), dim3(blocks_for(n * n)), dim3(256), 0, s, (const double*)ain, n,
-                       lda, as<double>(ctx->a64));
-    HIPCHK(hipGetLastError());
-    HIPCHK(lu_factor<double>(as<double>(ctx->a64), n, lda, lw, LU_NB, s, &panels));
-    rc = dense_factor_out<double>(ctx, as<double>(ctx->a64), n, lda, lw.ipiv, lu_out, ipiv_out);
-  } else {
-    HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
-    hipLaunchKernelGGL((transpose_in_kernel<float>), dim3(blocks_for(n * n)), dim3(256), 0, s, (const double*)ain, n,
-                       lda, as<float>(ctx->a32));
-    HIPCHK(hipGetLastError());
-    HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
-    rc = dense_factor_out<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, lu_out, ipiv_out);
+  int panels = 0, rc, info = 0;
+  // speculative factorization first (its factors are getrf's whenever its check
+  // passes); with exchanges when a column needs them
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    lw.nopiv = attempt == 0;
+    if (precision == BEAT_PREC_FP64) {
+      HIPCHK(ensure(ctx->a64, sizeof(double) * 2 * n * lda));
+      hipLaunchKernelGGL((transpose_in_kernel<double>), dim3(blocks_for(n * n)), dim3(256), 0, s, (const double*)ain,
+                         n, lda, as<double>(ctx->a64));
+      HIPCHK(hipGetLastError());
+      HIPCHK(lu_factor<double>(as<double>(ctx->a64), n, lda, lw, LU_NB, s, &panels));
+    } else {
+      HIPCHK(ensure(ctx->a32, sizeof(float) * 2 * n * lda));
+      hipLaunchKernelGGL((transpose_in_kernel<float>), dim3(blocks_for(n * n)), dim3(256), 0, s, (const double*)ain, n,
+                         lda, as<float>(ctx->a32));
+      HIPCHK(hipGetLastError());
+      HIPCHK(lu_factor<float>(as<float>(ctx->a32), n, lda, lw, LU_NB, s, &panels));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    bool exchange = false;
+    if ((rc = lu_outcome(ctx, lw, &info, &exchange)) != BEAT_OK) return rc;
+    if (!exchange) break;
   }
+  rc = precision == BEAT_PREC_FP64
+           ? dense_factor_out<double>(ctx, as<double>(ctx->a64), n, lda, lw.ipiv, lu_out, ipiv_out)
+           : dense_factor_out<float>(ctx, as<float>(ctx->a32), n, lda, lw.ipiv, lu_out, ipiv_out);
   if (rc != BEAT_OK) return rc;
-  int info = 0;
-  if ((rc = lu_outcome(ctx, lw, &info)) != BEAT_OK) return rc;
   *info_out = info;
   return BEAT_OK;
 }

@@ -132,10 +132,15 @@ struct LuWork {
   int* info;          // first zero pivot (1-based), 0 if none
   // wait control of the factorization and of the solves with its factors (ctl =
   // info + 1): [0] abort word (nonzero: every cross-workgroup wait gives up at
-  // once), [1] the kernels whose wait timed out (SpinSite bits), 0 if none.
-  // lu_factor clears both, then sets the abort word to abort_init.
+  // once), [1] the kernels whose wait timed out (SpinSite bits), 0 if none,
+  // [2] nonzero when a speculative leaf found that partial pivoting would
+  // exchange rows. lu_factor clears all three, then sets the abort word to
+  // abort_init.
   int* ctl;
   int abort_init = 0;
+  // test hook: a speculative factorization reports an exchange (ctl[2]) whatever
+  // its check finds
+  bool force_exchange = false;
   void* linv;         // 2 x 4 x 64 x 64 complex (double-sized): leaf inverses of two outer blocks
   int maxg;
   hipStream_t side;   // look-ahead stream (nullptr: no look-ahead)
@@ -145,6 +150,11 @@ struct LuWork {
   // complex64 trailing updates as three real products instead of four (normwise
   // stable; for factors that only precondition a refinement)
   bool gauss3 = false;
+  // speculative leaves: each leaf factored without row exchanges while every
+  // column is checked against partial pivoting's choice (ctl[2] set where an
+  // exchange would have been made; the factors are then those of an unpivoted LU
+  // and the caller factors again with nopiv false)
+  bool nopiv = false;
 };
 
 // Columns [n, nc) (n < nc <= lda) may carry right-hand sides: on return they

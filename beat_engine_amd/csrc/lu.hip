@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 #include <vector>
 
 #include "bem_common.h"
@@ -1382,6 +1383,230 @@ __global__ void colnorm_final_kernel(const double* part, int nparts, int nrhs, d
   out[d] = sqrt(s);
 }
 
+// ---- speculative panels: no row exchange, verified --------------------------------
+// Partial pivoting exchanges rows only where a row below the diagonal holds a
+// larger |re| + |im| in the current column than the diagonal. Burton-Miller
+// Galerkin matrices have a dominant diagonal, and the factorization keeps it, so
+// LAPACK's getrf makes no exchange on them. The speculative panel factors a leaf
+// without exchanges and checks, at every column, izamax's choice on every row
+// (the diagonal must be the first maximum of its column); the check is exact, so
+// whenever it passes the factors are those of partial pivoting with the identity
+// pivot sequence. No workgroup waits for another. A failed check raises a flag
+// (LuWork::ctl[2]) and the caller factors the matrix again with the exchange
+// panels.
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+// lane K of every group of four lanes to the whole group (DPP quad_perm [K,K,K,K])
+template <int K>
+__device__ __forceinline__ float quad_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, false));
+}
+template <int K>
+__device__ __forceinline__ double quad_bcast(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffull), K * 0x55, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), K * 0x55, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <typename T>
+__device__ __forceinline__ double cabs1(T re, T im) {
+  return fabs((double)re) + fabs((double)im);
+}
+
+// Multiplier arithmetic of both speculative kernels: the pivot's reciprocal
+// (conj(u) / |u|^2) once per column, then products, as getf2 scales a column by
+// 1 / u. A zero pivot gives zero multipliers (the column is left as it is).
+template <typename T>
+__device__ __forceinline__ void pivot_reciprocal(T ur, T ui, T& ir, T& ii) {
+  const T den = ur * ur + ui * ui;
+  const T rd = den != T(0) ? T(1) / den : T(0);
+  ir = ur * rd;
+  ii = -ui * rd;
+}
+
+// Diagonal block of a leaf (one workgroup): the b x b block at (j0, j0) factored
+// in place without exchanges, and the inverse of its unit-lower factor (the leaf
+// inverse of the U12 products) formed alongside by applying the same row
+// operations to the identity. Thread t owns row t / 4 and, of it, the columns
+// c = 4s + t % 4 (s = 0..15) in registers; row j travels through LDS (two buffers
+// by step parity, one barrier per step), the multiplier of a row through its four
+// lanes. Rows and columns beyond b are the identity. The empty asm statements
+// with register operands make each step's updates happen in that step (the
+// compiler would otherwise defer them to their first use and keep every step's
+// operands live).
+template <typename T>
+__global__ __launch_bounds__(256) void diag_nopiv_kernel(T* A, int64_t lda, int64_t j0, int b, T* linv, int* info,
+                                                         int* ctl) {
+  __builtin_amdgcn_s_setprio(3);
+  __shared__ T prow[2][4][16][2];  // row j of U by column class [parity][c % 4][c / 4][re, im]
+  __shared__ T plow[2][4][16][2];  // row j of L^-1
+  __shared__ T prcp[2][2];         // 1 / u_jj
+  const int t = threadIdx.x, r = t >> 2, q = t & 3;
+  T xr[16], xi[16], yr[16], yi[16];
+  {
+    const T* src = A + 2 * ((j0 + min(r, b - 1)) * lda + j0 + q);  // rows beyond b: an in-bounds row, masked
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      xr[s] = src[8 * s];
+      xi[s] = src[8 * s + 1];
+    }
+  }
+  // (every load issued first: the asm statements order what follows them)
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int c = 4 * s + q;
+    const bool in = r < b && c < b;
+    xr[s] = in ? xr[s] : (r == c ? T(1) : T(0));
+    xi[s] = in ? xi[s] : T(0);
+    yr[s] = r == c ? T(1) : T(0);
+    yi[s] = T(0);
+    __asm__ __volatile__("" : "+v"(xr[s]), "+v"(xi[s]), "+v"(yr[s]), "+v"(yi[s]));
+  }
+  bool viol = false;
+  int zero = 0;
+  static_for<0, PNB>([&](auto J) {
+    constexpr int j = decltype(J)::value, S = j >> 2, k = j & 3, par = j & 1;
+    if (r == j) {
+#pragma unroll
+      for (int s = S; s < 16; ++s) {
+        prow[par][q][s][0] = xr[s];
+        prow[par][q][s][1] = xi[s];
+      }
+#pragma unroll
+      for (int s = 0; s <= S; ++s) {
+        plow[par][q][s][0] = yr[s];
+        plow[par][q][s][1] = yi[s];
+      }
+      if (q == k) pivot_reciprocal(xr[S], xi[S], prcp[par][0], prcp[par][1]);
+    }
+    __syncthreads();
+    const T ur = prow[par][k][S][0], ui = prow[par][k][S][1];
+    const T ir = prcp[par][0], ii = prcp[par][1];
+    // the lane of class k holds column j of the row
+    const T mr = quad_bcast<k>(xr[S] * ir - xi[S] * ii), mi = quad_bcast<k>(xr[S] * ii + xi[S] * ir);
+    if (r > j) {
+      if (q == k) {
+        viol |= cabs1(xr[S], xi[S]) > cabs1(ur, ui);
+        xr[S] = mr;
+        xi[S] = mi;
+      }
+#pragma unroll
+      for (int s = S; s < 16; ++s)
+        if (s > S || q > k) {
+          const T pr = prow[par][q][s][0], pi = prow[par][q][s][1];
+          xr[s] -= mr * pr - mi * pi;
+          xi[s] -= mr * pi + mi * pr;
+        }
+#pragma unroll
+      for (int s = 0; s <= S; ++s)
+        if (s < S || q <= k) {
+          const T pr = plow[par][q][s][0], pi = plow[par][q][s][1];
+          yr[s] -= mr * pr - mi * pi;
+          yi[s] -= mr * pi + mi * pr;
+        }
+    }
+#pragma unroll
+    for (int s = S; s < 16; ++s) __asm__ __volatile__("" : "+v"(xr[s]), "+v"(xi[s]));
+#pragma unroll
+    for (int s = 0; s <= S; ++s) __asm__ __volatile__("" : "+v"(yr[s]), "+v"(yi[s]));
+    if (ur == T(0) && ui == T(0) && zero == 0 && j < b) zero = j + 1;
+  });
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int c = 4 * s + q;
+    if (r < b && c < b) {
+      T* dst = A + 2 * ((j0 + r) * lda + j0 + c);
+      dst[0] = xr[s];
+      dst[1] = xi[s];
+    }
+    linv[2 * (r * PNB + c)] = yr[s];
+    linv[2 * (r * PNB + c) + 1] = yi[s];
+  }
+  if (__ballot(viol) != 0 && (t & 63) == 0) __hip_atomic_fetch_or(ctl + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t == 0 && zero != 0 && *info == 0) *info = (int)j0 + zero;
+}
+
+// Rows below the diagonal block of a leaf: L21 = A21 U11^-1 by substitution
+// (U11 staged in LDS by column class; no barrier inside the column loop), each
+// column checked against izamax's choice: no row's |re| + |im| may exceed the
+// diagonal's. 64 rows per workgroup, threads laid out as in diag_nopiv_kernel.
+template <typename T>
+__global__ __launch_bounds__(256) void below_nopiv_kernel(T* A, int64_t n, int64_t lda, int64_t j0, int b, int* ctl) {
+  __builtin_amdgcn_s_setprio(3);
+  __shared__ T us[PNB][4][16][2];  // U11 row j by column class
+  __shared__ T rcp[PNB][2];        // 1 / u_jj
+  const int t = threadIdx.x, r = t >> 2, q = t & 3;
+  const int64_t row = j0 + b + (int64_t)blockIdx.x * PNB + r;
+  T xr[16], xi[16];
+  {
+    const T* src = A + 2 * ((row < n ? row : j0) * lda + j0 + q);  // rows beyond n: an in-bounds row, masked
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      xr[s] = src[8 * s];
+      xi[s] = src[8 * s + 1];
+    }
+  }
+  for (int e = t; e < PNB * PNB; e += 256) {
+    const int i = e >> 6, c = e & 63;
+    T vr = i == c ? T(1) : T(0), vi = T(0);
+    if (i < b && c < b && c >= i) {
+      vr = A[2 * ((j0 + i) * lda + j0 + c)];
+      vi = A[2 * ((j0 + i) * lda + j0 + c) + 1];
+    }
+    us[i][c & 3][c >> 2][0] = vr;
+    us[i][c & 3][c >> 2][1] = vi;
+    if (i == c) pivot_reciprocal(vr, vi, rcp[i][0], rcp[i][1]);
+  }
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const bool in = row < n && 4 * s + q < b;
+    xr[s] = in ? xr[s] : T(0);
+    xi[s] = in ? xi[s] : T(0);
+    __asm__ __volatile__("" : "+v"(xr[s]), "+v"(xi[s]));  // the masks applied here, not at first use
+  }
+  __syncthreads();
+  bool viol = false;
+  static_for<0, PNB>([&](auto J) {
+    constexpr int j = decltype(J)::value, S = j >> 2, k = j & 3;
+    const T ur = us[j][k][S][0], ui = us[j][k][S][1];
+    const T ir = rcp[j][0], ii = rcp[j][1];
+    const T mr = quad_bcast<k>(xr[S] * ir - xi[S] * ii), mi = quad_bcast<k>(xr[S] * ii + xi[S] * ir);
+    if (q == k) {
+      viol |= cabs1(xr[S], xi[S]) > cabs1(ur, ui);
+      xr[S] = mr;
+      xi[S] = mi;
+    }
+#pragma unroll
+    for (int s = S; s < 16; ++s)
+      if (s > S || q > k) {
+        const T pr = us[j][q][s][0], pi = us[j][q][s][1];
+        xr[s] -= mr * pr - mi * pi;
+        xi[s] -= mr * pi + mi * pr;
+      }
+#pragma unroll
+    for (int s = S; s < 16; ++s) __asm__ __volatile__("" : "+v"(xr[s]), "+v"(xi[s]));
+    __asm__ __volatile__("" ::: "memory");  // and the next step's LDS reads stay in that step
+  });
+  if (row < n) {
+    T* dst = A + 2 * (row * lda + j0 + q);
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+      if (4 * s + q < b) {
+        dst[8 * s] = xr[s];
+        dst[8 * s + 1] = xi[s];
+      }
+  }
+  if (__ballot(viol) != 0 && (t & 63) == 0) __hip_atomic_fetch_or(ctl + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace
 
 // Grids whose workgroups wait on each other must be resident all at once. A
@@ -1440,6 +1665,20 @@ static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, 
   return launch_resident<PanelArgs<T>>(reinterpret_cast<const void*>(&panel_kernel<T>), panel_kernel<T>, G, 256, pa, s);
 }
 
+// The speculative leaf: diagonal block, then the rows below it.
+template <typename T>
+static hipError_t launch_nopiv_leaf(T* A, int64_t n, int64_t lda, int64_t j0, int b, LuWork& w, T* linv,
+                                    hipStream_t s) {
+  hipLaunchKernelGGL((diag_nopiv_kernel<T>), dim3(1), dim3(256), 0, s, A, lda, j0, b, linv, w.info, w.ctl);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t below = n - j0 - b;
+  if (below <= 0) return hipSuccess;
+  hipLaunchKernelGGL((below_nopiv_kernel<T>), dim3((unsigned)((below + PNB - 1) / PNB)), dim3(256), 0, s, A, n, lda, j0,
+                     b, w.ctl);
+  return hipGetLastError();
+}
+
 template <typename T>
 static hipError_t launch_laswp(T* A, int64_t lda, int64_t c0, int64_t c1, const int* ipiv, int64_t k0, int64_t k1,
                                hipStream_t s) {
@@ -1479,10 +1718,12 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   const int64_t ncol = nc > n ? nc : n;
   if (ncol > lda) return hipErrorInvalidValue;
   const bool g3 = w.gauss3;  // complex64 updates in the three-real-product form
+  const bool nopiv = w.nopiv;  // speculative leaves: no exchanges (row swaps are no-ops)
   hipError_t e;
   // zero-pivot report and wait control (info, abort word, give-up record)
-  if ((e = launch_fill<int>(w.info, 3, 0, s)) != hipSuccess) return e;
+  if ((e = launch_fill<int>(w.info, 4, 0, s)) != hipSuccess) return e;
   if (w.abort_init && (e = launch_fill<int>(w.ctl, 1, w.abort_init, s)) != hipSuccess) return e;
+  if (w.nopiv && w.force_exchange && (e = launch_fill<int>(w.ctl + 2, 1, 1, s)) != hipSuccess) return e;
   // pivots start as no-op exchanges: columns a give-up leaves unfactored then swap
   // nothing in the row exchanges and gather tables that follow
   hipLaunchKernelGGL(iota_kernel, dim3((unsigned)std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0, s, w.ipiv, n);
@@ -1505,7 +1746,7 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   auto leaf_update = [&](int64_t J, int64_t j0, int b, int64_t c0, int64_t c1, hipStream_t st) -> hipError_t {
     if (c1 <= c0) return hipSuccess;
     hipError_t r;
-    if ((r = launch_laswp<T>(A, lda, c0, c1, w.ipiv, j0, j0 + b, st)) != hipSuccess) return r;
+    if (!nopiv && (r = launch_laswp<T>(A, lda, c0, c1, w.ipiv, j0, j0 + b, st)) != hipSuccess) return r;
     r = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + c0), lda, A + 2 * (j0 * lda + c0), lda, b, c1 - c0, b,
                     st, false, g3);
     if (r != hipSuccess) return r;
@@ -1517,7 +1758,7 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
     if (c1 <= c0) return hipSuccess;
     const bool behind = look_ahead && st == w.side;  // the look-ahead's far update
     hipError_t r;
-    if ((r = launch_laswp<T>(A, lda, c0, c1, w.ipiv, J, Je, st)) != hipSuccess) return r;
+    if (!nopiv && (r = launch_laswp<T>(A, lda, c0, c1, w.ipiv, J, Je, st)) != hipSuccess) return r;
     for (int64_t j0 = J; j0 < Je; j0 += nb) {
       const int b = (int)std::min<int64_t>(nb, Je - j0);
       r = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + c0), lda, A + 2 * (j0 * lda + c0), lda, b, c1 - c0,
@@ -1537,10 +1778,12 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
     for (int64_t j0 = J; j0 < Je; j0 += nb) {
       const int b = (int)std::min<int64_t>(nb, Je - j0);
       const int64_t nx = std::min<int64_t>(Je, j0 + b + nb);  // end of the next leaf's columns
-      if ((e = launch_panel<T>(A, n, lda, j0, b, w, leaf_inv(j0, J), s)) != hipSuccess) return e;
+      e = nopiv ? launch_nopiv_leaf<T>(A, n, lda, j0, b, w, leaf_inv(j0, J), s)
+                : launch_panel<T>(A, n, lda, j0, b, w, leaf_inv(j0, J), s);
+      if (e != hipSuccess) return e;
       ++panels;
       if (!split) {
-        if ((e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, s)) != hipSuccess) return e;
+        if (!nopiv && (e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, s)) != hipSuccess) return e;
         if ((e = leaf_update(J, j0, b, j0 + b, Je, s)) != hipSuccess) return e;
         continue;
       }
@@ -1549,7 +1792,7 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
         if ((e = hipStreamWaitEvent(s, w.ev_inb, 0)) != hipSuccess) return e;
       if ((e = leaf_update(J, j0, b, j0 + b, nx, s)) != hipSuccess) return e;
       if ((e = hipStreamWaitEvent(w.inb, w.ev_panel, 0)) != hipSuccess) return e;
-      if ((e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, w.inb)) != hipSuccess) return e;
+      if (!nopiv && (e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, w.inb)) != hipSuccess) return e;
       if ((e = leaf_update(J, j0, b, nx, Je, w.inb)) != hipSuccess) return e;
       if ((e = hipEventRecord(w.ev_inb, w.inb)) != hipSuccess) return e;
     }

@@ -92,6 +92,11 @@ class Engine:
         once and the call raises (the error path of a wait that times out)."""
         self._check(self._lib.beat_ctx_debug_abort_waits(self._h, 1 if on else 0), "beat_ctx_debug_abort_waits")
 
+    def debug_force_exchange(self, on: bool):
+        """Test hook: while on, every solve's speculative factorization reports that
+        partial pivoting exchanges rows, so the solve factors again with exchanges."""
+        self._check(self._lib.beat_ctx_debug_force_exchange(self._h, 1 if on else 0), "beat_ctx_debug_force_exchange")
+
     def reclaim(self):
         """Frees the context's workspaces (reclaim_accelerator_memory!)."""
         self._check(self._lib.beat_ctx_reclaim(self._h), "beat_ctx_reclaim")
@@ -353,7 +358,7 @@ class DeviceMesh:
         if rc != _native.BEAT_ERR_CANCELLED:
             self.engine._check(rc, "beat_bm_sweep")
         P = np.transpose(ph, (0, 2, 1))
-        out = [dict(solved=r.factorization_count == 1, assembly_s=r.assembly_s, factor_s=r.factor_s,
+        out = [dict(solved=r.factorization_count >= 1, assembly_s=r.assembly_s, factor_s=r.factor_s,
                     solve_s=r.solve_s, refinement_status=_native.REFINE_STATUS[r.refine.status],
                     refinement_steps=r.refine.steps, working_relative_residuals=work[i].copy())
                for i, r in enumerate(reps)]

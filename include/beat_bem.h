@@ -71,7 +71,10 @@ typedef struct {
   double factor_s;   /* LU (and the complex128 -> complex64 copy when refined) */
   double solve_s;    /* triangular solves and refinement */
   double regular_kernel_s; /* the regular-pair assembly kernel alone */
-  int factorization_count;
+  int factorization_count; /* 1; 2 when the first, speculative factorization (no row
+                              exchanges, each column checked against partial pivoting's
+                              choice) found a column where partial pivoting exchanges
+                              rows and the matrix was factored again with exchanges */
   int lu_panels;
   int64_t regular_pairs;
   int64_t singular_pairs;
@@ -95,7 +98,7 @@ const char* beat_last_error(const beat_ctx* ctx);
  * worker_backend_availability, worker_backend_availability.jl:1-26).
  * beat_ctx_set_cancel_path: a marker file whose existence stops a sweep before its
  * next frequency (BEAT_ERR_CANCELLED; the reports of the solved frequencies carry
- * factorization_count 1, the rest 0); NULL clears it.
+ * factorization_count >= 1, the rest 0); NULL clears it.
  * beat_ctx_reclaim: waits for the context's streams and frees its workspaces
  * (meshes stay resident). beat_ctx_memory_info: info[0..2] = workspace bytes,
  * device free bytes, device total bytes. beat_capabilities: a JSON object of what
@@ -108,6 +111,12 @@ int beat_ctx_set_cancel_path(beat_ctx* ctx, const char* path);
  * end normally and the call returns BEAT_ERR_HIP with a message (the same path a
  * wait that times out after 4 s takes). For tests of that error path only. */
 int beat_ctx_debug_abort_waits(beat_ctx* ctx, int on);
+/* Test hook (no counterpart in the reference): while on, the speculative first
+ * factorization of every solve (no row exchanges, each column checked against
+ * partial pivoting's choice) reports that an exchange is needed, so the solve
+ * takes its second factorization with the exchange panels from the matrix as
+ * assembled (factorization_count 2). For tests of that path only. */
+int beat_ctx_debug_force_exchange(beat_ctx* ctx, int on);
 int beat_ctx_reclaim(beat_ctx* ctx);
 int beat_ctx_memory_info(beat_ctx* ctx, int64_t* info);
 int64_t beat_capabilities(const beat_ctx* ctx, char* buf, int64_t len);
