@@ -11,6 +11,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbeat_bem_hip.so")
 
 BEAT_OK = 0
+BEAT_ERR_ARG = -1
+BEAT_ERR_HIP = -2
+BEAT_ERR_ALLOC = -3
 BEAT_ERR_SINGULAR = -4
 BEAT_ERR_CANCELLED = -5
 PREC_FP64 = 0
@@ -26,7 +29,7 @@ EXPORTS = (
     "beat_field_gradient_eval", "beat_operators_assemble", "beat_bm_sweep_device",
     "beat_bm_sweep", "beat_ctx_set_cancel_path", "beat_ctx_reclaim", "beat_ctx_memory_info", "beat_capabilities",
     "beat_adjoint_spl_gradient", "beat_bm_batch", "beat_autotune", "beat_ctx_debug_abort_waits",
-    "beat_ctx_debug_force_exchange",
+    "beat_ctx_debug_force_exchange", "beat_field_eval_device_noalloc",
 )
 
 SYMMETRY_CODES = {"off": 0, "x": 1, "xy": 2, "ground": 3}
@@ -110,6 +113,7 @@ def load(path: str = LIB_PATH):
                                        ctypes.POINTER(SolveReport)]
     L.beat_field_eval.argtypes = [P, P, I, D, P, P, I64, P, I64, P]
     L.beat_field_gradient_eval.argtypes = [P, P, I, D, P, P, I64, P, I64, P]
+    L.beat_field_eval_device_noalloc.argtypes = [P, P, I, D, P, P, I64, P, I64, P]
     L.beat_operators_assemble.argtypes = [P, P, I, D, P, P, P, P]
     L.beat_bm_sweep_device.argtypes = [P, P, I, I64, P, P, P, P, I64, P, ctypes.POINTER(SolveOptions), P]
     L.beat_bm_sweep.argtypes = [P, P, I, I64, P, P, P, P, I64, P, ctypes.POINTER(SolveOptions), P]

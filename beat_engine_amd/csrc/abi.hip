@@ -28,6 +28,14 @@ namespace {
 constexpr int LU_NB = 64;
 constexpr int SOLVE_CHUNK = 8;
 
+// Device scratch of beat_field_eval_device_noalloc (callers that cannot
+// allocate): the staged inputs, the field sources and the accumulated output.
+// One copy per GPU, alive until the process ends; every call on a GPU shares it.
+// Host code never passes the name itself to a kernel or copy: it uses the
+// address hipGetSymbolAddress gives on each device (beat_ctx::scratch).
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -100,6 +108,8 @@ struct beat_ctx {
   // test hook (beat_ctx_debug_force_exchange): speculative factorizations report
   // an exchange, so every solve takes its second, exchange factorization
   bool debug_force_exchange = false;
+  // this device's copy of the no-allocation scratch array (set at creation)
+  unsigned char* scratch = nullptr;
 };
 
 struct beat_mesh {
@@ -157,6 +167,17 @@ __global__ void gather_q_kernel(const double* qc, int64_t nf, int nrhs, const in
   const int64_t src = (int64_t)perm[f] + (int64_t)d * nf;
   qi[2 * i] = (T)qc[2 * src];
   qi[2 * i + 1] = (T)qc[2 * src + 1];
+}
+
+// caller column-major nv x nrhs complex128 -> internal row-major (row inv[v] of caller row v)
+__global__ void p_to_internal_kernel(const double* pc, int64_t nv, int nrhs, const int32_t* inv, double* pi) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv * nrhs) return;
+  const int64_t v = i % nv;
+  const int d = (int)(i / nv);
+  const int64_t dst = (int64_t)inv[v] * nrhs + d;
+  pi[2 * dst] = pc[2 * i];
+  pi[2 * dst + 1] = pc[2 * i + 1];
 }
 
 template <typename T>
@@ -1188,6 +1209,87 @@ int field_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const
   return BEAT_OK;
 }
 
+// Bytes of the no-allocation scratch a field evaluation needs, in the order the
+// buffers are laid out (each rounded to 256 bytes).
+size_t noalloc_field_bytes(int64_t nv, int64_t nf, int64_t ns, int64_t nrhs, int64_t npts, bool fp32) {
+  auto r = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t e = fp32 ? sizeof(float) : sizeof(double);
+  size_t b = r(sizeof(double) * 2 * nf * nrhs) + r(sizeof(double) * 2 * nv * nrhs);  // q, p internal (complex128)
+  if (fp32) b += r(e * 2 * nf * nrhs) + r(e * 2 * nv * nrhs) + r(e * 3 * npts);     // complex64 copies, points
+  b += r(e * 3 * ns) + r(e * 3 * ns) + r(e * 2 * ns * nrhs) + r(e * 2 * ns * nrhs);  // sources
+  b += r(e * 2 * npts * nrhs);                                                       // accumulated field
+  return b;
+}
+
+// beat_field_eval on device pointers with every buffer in the scratch array.
+int field_noalloc_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const double* p_dev,
+                       const double* q_dev, int64_t nrhs, const double* pts_dev, int64_t npts, double* out_dev) {
+  if (int rc = check_mesh(ctx, m)) return rc;
+  if (nrhs < 1 || npts < 0 || (npts > 0 && (!p_dev || !q_dev || !pts_dev || !out_dev)))
+    return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (precision != BEAT_PREC_FP64 && precision != BEAT_PREC_FP32) return fail(ctx, BEAT_ERR_ARG, "precision");
+  if (npts == 0) return BEAT_OK;
+  const int64_t n = m->h.nv, nf = m->h.nf, ns = nf * m->h.nq * (1 + m->h.nimg);
+  const bool fp32 = precision == BEAT_PREC_FP32;
+  const size_t need = noalloc_field_bytes(n, nf, ns, nrhs, npts, fp32);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, BEAT_ERR_ARG,
+                "beat_field_eval_device_noalloc needs " + std::to_string(need) + " bytes of scratch for this mesh, " +
+                    std::to_string(nrhs) + " right-hand sides and " + std::to_string(npts) + " points; it holds " +
+                    std::to_string(NOALLOC_SCRATCH_BYTES) + " (use beat_field_eval, or fewer points per call)");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  unsigned char* base = ctx->scratch;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + off;
+    off += (bytes + 255) / 256 * 256;
+    return p;
+  };
+  double* qint = reinterpret_cast<double*>(take(sizeof(double) * 2 * nf * nrhs));
+  double* pint = reinterpret_cast<double*>(take(sizeof(double) * 2 * n * nrhs));
+  hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, s, q_dev, nf, (int)nrhs,
+                     (const int32_t*)m->face_perm, qint);
+  hipLaunchKernelGGL(p_to_internal_kernel, dim3(blocks_for(n * nrhs)), dim3(256), 0, s, p_dev, n, (int)nrhs,
+                     (const int32_t*)m->vert_inv, pint);
+  HIPCHK(hipGetLastError());
+  if (!fp32) {
+    double* sp = reinterpret_cast<double*>(take(sizeof(double) * 3 * ns));
+    double* sn = reinterpret_cast<double*>(take(sizeof(double) * 3 * ns));
+    double* pw = reinterpret_cast<double*>(take(sizeof(double) * 2 * ns * nrhs));
+    double* qw = reinterpret_cast<double*>(take(sizeof(double) * 2 * ns * nrhs));
+    double* o = reinterpret_cast<double*>(take(sizeof(double) * 2 * npts * nrhs));
+    HIPCHK(field_sources<double>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, pint, qint, (int)nrhs, m->h.nimg, m->h.img,
+                                 sp, sn, pw, qw, s));
+    // field_eval clears o (a fill kernel) before accumulating into it
+    HIPCHK(field_eval<double>(sp, sn, pw, qw, ns, (int)nrhs, pts_dev, npts, o, k, s));
+    hipLaunchKernelGGL((cols_out_kernel<double, double>), dim3(blocks_for(npts * nrhs)), dim3(256), 0, s,
+                       (const double*)o, npts, (int)nrhs, out_dev);
+  } else {
+    float* qq = reinterpret_cast<float*>(take(sizeof(float) * 2 * nf * nrhs));
+    float* pp = reinterpret_cast<float*>(take(sizeof(float) * 2 * n * nrhs));
+    float* pts32 = reinterpret_cast<float*>(take(sizeof(float) * 3 * npts));
+    HIPCHK(launch_downcast(pint, pp, 2 * n * nrhs, s));
+    HIPCHK(launch_downcast(qint, qq, 2 * nf * nrhs, s));
+    HIPCHK(launch_downcast(pts_dev, pts32, 3 * npts, s));
+    float* sp = reinterpret_cast<float*>(take(sizeof(float) * 3 * ns));
+    float* sn = reinterpret_cast<float*>(take(sizeof(float) * 3 * ns));
+    float* pw = reinterpret_cast<float*>(take(sizeof(float) * 2 * ns * nrhs));
+    float* qw = reinterpret_cast<float*>(take(sizeof(float) * 2 * ns * nrhs));
+    float* o = reinterpret_cast<float*>(take(sizeof(float) * 2 * npts * nrhs));
+    HIPCHK(field_sources<float>(m->rec64, m->fverts, nf, m->h.nq, m->rule3, pp, qq, (int)nrhs, m->h.nimg, m->h.img, sp,
+                                sn, pw, qw, s));
+    HIPCHK(field_eval<float>(sp, sn, pw, qw, ns, (int)nrhs, pts32, npts, o, (float)k, s));
+    hipLaunchKernelGGL((cols_out_kernel<float, double>), dim3(blocks_for(npts * nrhs)), dim3(256), 0, s,
+                       (const float*)o, npts, (int)nrhs, out_dev);
+  }
+  HIPCHK(hipGetLastError());
+  if (off > NOALLOC_SCRATCH_BYTES) return fail(ctx, BEAT_ERR_HIP, "no-allocation scratch layout overflow");
+  HIPCHK(hipStreamSynchronize(s));
+  return BEAT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1241,6 +1343,11 @@ int beat_ctx_create(int device, beat_ctx** out) {
       delete c;
       return BEAT_ERR_HIP;
     }
+  }
+  // the no-allocation entry point's scratch on this device (the device is current)
+  if (hipGetSymbolAddress(reinterpret_cast<void**>(&c->scratch), HIP_SYMBOL(scratch)) != hipSuccess || !c->scratch) {
+    delete c;
+    return BEAT_ERR_HIP;
   }
   if (const char* v = std::getenv("BEAT_SWEEP_ASM_ONE_PER_CU")) c->sweep_asm_one_per_cu = v[0] != '0';
   if (const char* v = std::getenv("BEAT_BATCH_WORKERS")) c->batch_workers = std::max(1, std::atoi(v));
@@ -1947,6 +2054,13 @@ int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, 
 int beat_field_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const double* pressure,
                     const double* q, int64_t nrhs, const double* pts, int64_t npts, double* out) {
   return field_impl(ctx, m, precision, k, pressure, q, nrhs, pts, npts, out, false);
+}
+
+int beat_field_eval_device_noalloc(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const void* pressure_dev,
+                                   const void* q_dev, int64_t nrhs, const void* pts_dev, int64_t npts, void* out_dev) {
+  return field_noalloc_impl(ctx, m, precision, k, static_cast<const double*>(pressure_dev),
+                            static_cast<const double*>(q_dev), nrhs, static_cast<const double*>(pts_dev), npts,
+                            static_cast<double*>(out_dev));
 }
 
 int beat_field_gradient_eval(beat_ctx* ctx, const beat_mesh* m, int precision, double k, const double* pressure,

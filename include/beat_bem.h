@@ -233,6 +233,22 @@ int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, 
 int beat_field_eval(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, const double* pressure,
                     const double* q, int64_t nrhs, const double* pts, int64_t npts, double* out);
 
+/* beat_field_eval on device pointers, allocating nothing (no counterpart in the
+ * reference; for callers that cannot allocate device memory during the call:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). pressure_dev: nv x nrhs, q_dev: nf x nrhs, out_dev: npts x nrhs,
+ * complex128 column-major; pts_dev: npts x 3 doubles row-major; all on the
+ * context's device. Every staging buffer (internal-order inputs, field sources,
+ * the accumulated field) lives in one file-scope device array of 64 MiB per GPU,
+ * held until the process ends; a call that would need more (it takes about 43 MB for a
+ * 20480-face mesh, 8 right-hand sides and 10000 points in FP64) returns
+ * BEAT_ERR_ARG without touching the GPU. All calls on one GPU share that array:
+ * calls that overlap on one GPU (from two contexts, or two threads) race. The
+ * call returns when the output is written. */
+int beat_field_eval_device_noalloc(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k,
+                                   const void* pressure_dev, const void* q_dev, int64_t nrhs, const void* pts_dev,
+                                   int64_t npts, void* out_dev);
+
 /* Pressure and its x, y, z derivatives at npts points (pressure_gradient_v1):
  * evaluate_galerkin_field_gradient_cpu / _rocm (BeatEngineFieldGradient.jl:45-92,
  * BeatEngineRocmFieldGradient.jl:20-47), field_value_gradient (:2-13) over the
