@@ -56,6 +56,8 @@ def parse():
                          "freq-solve of the workload; prints no JSON line")
     ap.add_argument("--no-rocsolver", action="store_true",
                     help="skip timing rocSOLVER getrf (the vendor LU) at the workload's size")
+    ap.add_argument("--no-extra-forms", action="store_true",
+                    help="skip the plain complex64 sweep, the precision form's field and the differentiation form")
     ap.add_argument("--cpu-freq-index", type=int, default=11,
                     help="which of the 20 frequencies the CPU baseline solves (and the GPU result is checked at)")
     return ap.parse_args()
@@ -63,6 +65,12 @@ def parse():
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def counts_flops(dm):
+    """Algorithmic FLOP of one regular-pair kernel launch: pairs x rule points^2 x 72."""
+    c = dm.counts()
+    return c["regular_pairs"] * c["rule_points"] ** 2 * FLOP_PER_QPAIR
 
 
 def host_threads():
@@ -223,6 +231,84 @@ def batch_form(args):
     eng.close()
 
 
+FP32_VALU_TFLOPS = 78.6  # FP32 VALU without packed instructions (this build uses none), as FP64
+FIELD_FLOP_PER_EVAL = 40  # per (point, source) pair and right-hand side, the SURVEY's count
+
+
+def precision_form(eng):
+    """Graded capped cylinder (0.3 m x 0.08 m, ~32768 triangles), 64 drives on the graded
+    cap, refined solve at 2 kHz, then the field of all 64 at 10000 points on a 2 m
+    hemisphere in FP32 and FP64 (wall time of beat_field_eval, host arrays in and out)."""
+    from beat_engine_amd.mesh import capped_cylinder
+    from beat_engine_amd.phasor import burton_miller_coupling, neumann_scale
+
+    mesh = capped_cylinder(target_faces=32768)
+    f = 2000.0
+    w = 2 * math.pi * f
+    k = w / 343.0
+    rng = np.random.default_rng(64)
+    v = np.zeros((mesh.num_faces, 64), complex)
+    cap = mesh.physical_tags == 2
+    v[cap] = rng.normal(size=(cap.sum(), 64)) + 1j * rng.normal(size=(cap.sum(), 64))
+    q = neumann_scale(1.21, w) * v
+    dm = eng.upload(mesh, 4, 4)
+    t = time.perf_counter()
+    P, rep = dm.solve(k, burton_miller_coupling(k), q, "float32_refined", 3, 1e-10)
+    t_solve = time.perf_counter() - t
+    n = 10000
+    i = np.arange(n) + 0.5
+    z = i / n
+    rr = np.sqrt(1 - z * z)
+    phi = math.pi * (3 - math.sqrt(5)) * i
+    pts = 2.0 * np.stack([rr * np.cos(phi), rr * np.sin(phi), z], axis=1) + np.array([0.0, 0.0, 0.15])
+    out = {"workload": (f"capped cylinder {mesh.num_faces} triangles ({mesh.num_vertices} P1), 64 drives on the "
+                        f"graded cap, {f:.0f} Hz, field at {n} points on a 2 m hemisphere"),
+           "refined_solve_s": t_solve, "refinement_status": rep["refinement"]["status"],
+           "refinement_steps": rep["refinement"]["steps"],
+           "stage_s": {"assembly": rep["assembly_s"], "lu_c64": rep["factor_s"], "solve_refine": rep["solve_s"]}}
+    nsrc = mesh.num_faces * 6
+    res = {}
+    for prec in ("float32", "float64"):
+        dm.field(k, P[:, :2], q[:, :2], pts[:64], prec)  # warm
+        t = time.perf_counter()
+        res[prec] = dm.field(k, P, q, pts, prec)
+        el = time.perf_counter() - t
+        evals = n * nsrc * 64
+        out[f"field_{prec}_s"] = el
+        out[f"field_{prec}_evals_per_s"] = evals / el
+        out[f"field_{prec}_tflops"] = evals * FIELD_FLOP_PER_EVAL / el / 1e12
+    out["field_frac_of_valu_peak"] = {"float32": out["field_float32_tflops"] / FP32_VALU_TFLOPS,
+                                      "float64": out["field_float64_tflops"] / FP64_VALU_PEAK_TFLOPS}
+    out["field_float32_vs_float64_rel"] = float(np.linalg.norm(res["float32"] - res["float64"]) /
+                                                np.linalg.norm(res["float64"]))
+    dm.close()
+    return out
+
+
+def differentiation_form(eng):
+    """Level-4 icosphere morphed by 5 seeded spherical harmonics, 8 frequencies
+    1-4 kHz, 64 far-field directions at 2 m: loss and gradient by the adjoint."""
+    from beat_engine_amd.mesh import fibonacci_sphere, morphed_icosphere
+    from beat_engine_amd.phasor import burton_miller_coupling, neumann_scale
+
+    mesh = morphed_icosphere(4, 0.1, 5, 0.01, seed=5120)
+    freqs = np.linspace(1000.0, 4000.0, 8)
+    ks = 2 * np.pi * freqs / 343.0
+    etas = np.array([burton_miller_coupling(k) for k in ks])
+    qs = np.stack([np.full(mesh.num_faces, neumann_scale(1.21, 2 * np.pi * f)) for f in freqs])
+    pts = fibonacci_sphere(64, 2.0)
+    dm = eng.upload(mesh, 4, 4)
+    dm.spl_loss_gradient(ks[:1], etas[:1], qs[:1], pts, 90.0)  # warm
+    t = time.perf_counter()
+    L, g, _ = dm.spl_loss_gradient(ks, etas, qs, pts, 90.0)
+    el = time.perf_counter() - t
+    dm.close()
+    return {"workload": (f"morphed level-4 icosphere ({mesh.num_faces} triangles, {mesh.num_vertices} nodes), "
+                         "8 frequencies 1-4 kHz, 64 directions at 2 m, loss sum (SPL - 90 dB)^2"),
+            "loss_and_gradient_s": el, "s_per_frequency": el / len(freqs), "loss": float(L),
+            "gradient_norm": float(np.linalg.norm(g))}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -349,6 +435,34 @@ def main():
     b = p_fp64.cpu().numpy()
     err = float(max(np.linalg.norm(a[j] - b[j]) / np.linalg.norm(b[j]) for j in range(nrhs)))
     log(f"fp64 path {sec64:.3f} s/freq; refined serial {sec_serial:.3f} s/freq; refined vs fp64 max rel L2 {err:.2e}")
+    extra = {}
+    if not args.no_extra_forms:
+        # the plain complex64 path (the reference's float32 option) on the same inputs
+        el32, reps32 = sweep(0, args.fp64_steps, _native.PREC_FP32, p_sweep)
+        p_32 = torch.zeros_like(p_fp64)
+        rep32 = step(last, _native.PREC_FP32, p_32)
+        torch.cuda.synchronize()
+        c = p_32.cpu().numpy()
+        err32 = float(max(np.linalg.norm(c[j] - b[j]) / np.linalg.norm(b[j]) for j in range(nrhs)))
+        extra["fp32_gpu"] = {
+            "freq_solves_per_s": args.fp64_steps / el32,
+            "rel_err_vs_fp64_gpu": err32,
+            "stage_s": {"assembly": rep32.assembly_s, "regular_kernel": rep32.regular_kernel_s,
+                        "lu_c64": rep32.factor_s, "solve": rep32.solve_s},
+            "sweep_stage_s": {"assembly": float(np.mean([r.assembly_s for r in reps32])),
+                              "lu_c64": float(np.mean([r.factor_s for r in reps32])),
+                              "solve": float(np.mean([r.solve_s for r in reps32]))},
+            "regular_kernel_tflops": counts_flops(dm) / rep32.regular_kernel_s / 1e12,
+        }
+        log("fp32 path:", extra["fp32_gpu"])
+
+    if not args.no_extra_forms:
+        t = time.perf_counter()
+        extra["precision_form"] = precision_form(eng)
+        log(f"precision form ({time.perf_counter() - t:.1f} s):", extra["precision_form"])
+        t = time.perf_counter()
+        extra["differentiation_form"] = differentiation_form(eng)
+        log(f"differentiation form ({time.perf_counter() - t:.1f} s):", extra["differentiation_form"])
 
     reg_s = float(np.mean([r.regular_kernel_s for r in serial]))
     counts = dm.counts()
@@ -418,6 +532,7 @@ def main():
             "complex_product": "three real products (refined path)",
         },
     }
+    out.update(extra)
     if not args.no_rocsolver:
         rs = rocsolver_getrf_ms(n_p1)
         rs["ours_c64_ms"] = lu_s * 1e3  # the refined path's LU: three-real-product updates
