@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -87,6 +88,9 @@ struct beat_ctx {
   bool sweep_asm_one_per_cu = false;
   // contexts a batch may use at once (BEAT_BATCH_WORKERS, default 4)
   int batch_workers = 4;
+  // batches of small systems through the grouped, graph-replayed path
+  // (BEAT_BATCH_GROUPED=0: the per-job pipeline)
+  bool batch_grouped = true;
   // cancellation marker polled between the frequencies of a sweep (empty: none)
   std::string cancel_path;
   // autotuned launch strategy per size class (floor(log2 n)): the factorization's
@@ -110,6 +114,11 @@ struct beat_ctx {
   bool debug_force_exchange = false;
   // this device's copy of the no-allocation scratch array (set at creation)
   unsigned char* scratch = nullptr;
+  // grouped batches (batch_grouped_impl): the group's systems, vectors, refinement
+  // states and descriptors, and the captured factor / solve / refine graphs by
+  // group shape (invalidated whenever these buffers move)
+  DevBuf g_a64, g_a32, g_b64, g_b32, g_q64, g_q32, g_linv, g_vec, g_state, g_desc;
+  std::map<std::string, hipGraphExec_t> g_graphs;
 };
 
 struct beat_mesh {
@@ -1052,6 +1061,503 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
   return result;
 }
 
+// ---- grouped batches: many small systems factored, solved and refined together ----
+//
+// Jobs are taken in groups of up to GROUP_MAX. A group's systems are assembled one
+// by one (eager launches: the wavenumber is in their arguments), then one
+// captured hipGraph does the rest for the whole group: the batched speculative LU
+// (one launch per kernel and leaf for every system), the backward substitutions
+// of the carried right-hand sides, and for the refined path the complex128
+// refinement with its per-column acceptance decided on the device. The graph
+// depends only on the group's shape (sizes, precision, options) and buffers, so
+// groups of the same shape (the frequencies of one mesh) replay it. A system
+// whose speculative factorization would have exchanged rows is solved again on
+// its own by the per-job path.
+constexpr int GROUP_MAX = 32;
+constexpr int64_t GROUP_NMAX = 4096;  // larger systems keep the per-job pipeline
+
+// Device state of one system's refinement (beat_refine_cpu_lu's loop,
+// BeatEngineCpuRefinement.jl:5-78, with its decisions taken on the device).
+struct RefineState {
+  int steps, status, done, skip, hist_len, working, pad[2];
+  int mask[8], acc[8];
+  double bn[8], err[8], hist[9 * 8];
+};
+
+struct BatchJobDesc {
+  int64_t n;
+  const int32_t* vert_inv;
+  double* p_out;  // caller column-major n x nrhs (eager scatter only)
+};
+
+__device__ __forceinline__ bool all_le(const double* e, int nrhs, double rtol) {
+  for (int d = 0; d < nrhs; ++d)
+    if (!(e[d] <= rtol)) return false;
+  return true;
+}
+
+// mode 0: initial residual norms -> errors, history row 0, first decision;
+// mode 1: after correction step `step` -> per-column acceptance, history, decision;
+// mode 2: final statuses. One thread per system. norms: [bn | rn] per system (8 + 8).
+__global__ void refine_control_kernel(RefineState* st, const double* norms, int nsys, int nrhs, double rtol, int mode,
+                                      int step) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nsys) return;
+  RefineState& S = st[j];
+  const double* bn = norms + 16 * j;
+  const double* rn = bn + 8;
+  auto error_of = [&](int d, double nr) {
+    return S.bn[d] == 0 ? (nr == 0 ? 0.0 : INFINITY) : nr / S.bn[d];
+  };
+  if (mode == 0) {
+    for (int d = 0; d < 8; ++d) {
+      S.bn[d] = d < nrhs ? bn[d] : 0.0;
+      S.err[d] = d < nrhs ? error_of(d, rn[d]) : 0.0;
+      S.hist[d] = S.err[d];
+      S.mask[d] = d < nrhs && !(S.err[d] <= rtol);
+      S.acc[d] = 0;
+    }
+    S.hist_len = 1;
+    S.steps = 0;
+    S.status = BEAT_REFINE_STEP_LIMIT;
+    S.done = 0;
+    if (all_le(S.err, nrhs, rtol)) {
+      S.status = BEAT_REFINE_CONVERGED;
+      S.done = 1;
+    }
+    S.skip = S.done;
+    return;
+  }
+  if (mode == 1) {
+    if (S.skip) return;
+    bool improved = false;
+    for (int d = 0; d < nrhs; ++d) {
+      const double ne = error_of(d, rn[d]);
+      S.acc[d] = isfinite(ne) && ne < S.err[d];
+      if (S.acc[d]) {
+        S.err[d] = ne;
+        improved = true;
+      }
+    }
+    S.steps = step;
+    for (int d = 0; d < 8; ++d) S.hist[S.hist_len * 8 + d] = S.err[d];
+    S.hist_len += 1;
+    if (!improved) {
+      S.status = BEAT_REFINE_STAGNATED;
+      S.done = 1;
+    } else if (all_le(S.err, nrhs, rtol)) {
+      S.status = BEAT_REFINE_CONVERGED;
+      S.done = 1;
+    }
+    for (int d = 0; d < nrhs; ++d) S.mask[d] = !(S.err[d] <= rtol);
+    return;
+  }
+  if (all_le(S.err, nrhs, rtol)) S.status = BEAT_REFINE_CONVERGED;
+  bool nonfinite = false;
+  for (int d = 0; d < nrhs; ++d) nonfinite |= !isfinite(S.err[d]);
+  S.working = nonfinite ? BEAT_REFINE_NONFINITE : S.status;
+  // the returned solution is the complex128 working one: returned = working errors
+  if (nonfinite) S.status = BEAT_REFINE_NONFINITE;
+}
+
+// the next step runs only for systems whose refinement has not finished
+__global__ void refine_advance_kernel(RefineState* st, int nsys) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nsys) st[j].skip = st[j].done;
+}
+
+// Column 2-norms of each system's vector block (norms[16 j + off + d]); one
+// workgroup per system. Skipped with the system's skip flag (when given).
+__global__ __launch_bounds__(256) void colnorms_batched_kernel(const BatchVec<double>* vec, int nrhs, double* norms,
+                                                               int off) {
+  const BatchVec<double> V = vec[blockIdx.x];
+  if (V.skip && *V.skip) return;
+  __shared__ double red[8][256];
+  const int t = threadIdx.x;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t r = t; r < V.n; r += 256)
+#pragma unroll
+    for (int d = 0; d < 8; ++d)
+      if (d < nrhs) {
+        const double a = V.v[2 * (r * V.ld + d)], b = V.v[2 * (r * V.ld + d) + 1];
+        acc[d] += a * a + b * b;
+      }
+#pragma unroll
+  for (int d = 0; d < 8; ++d) red[d][t] = acc[d];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+#pragma unroll
+      for (int d = 0; d < 8; ++d) red[d][t] += red[d][t + o];
+    __syncthreads();
+  }
+  if (t < nrhs) norms[16 * blockIdx.x + off + t] = sqrt(red[t][0]);
+}
+
+// Elementwise work on every system's n x nrhs block (blockIdx.y: the system):
+//  0  dst64 = (double) src32            (the first solution, from the carried columns)
+//  1  dst32 = mask ? (float) src64 : 0  (the correction's right-hand side)
+//  2  dst64 = a64 + (double) src32      (corrected solution)
+//  3  a64 = b64, c64 = d64 where accepted (accepted columns of solution and residual)
+//  4  dst64 = src64 (complex128 copy)
+template <int OP>
+__global__ void batch_elementwise_kernel(const BatchVec<double>* d64, const BatchVec<float>* s32,
+                                         const BatchVec<double>* a64, const BatchVec<double>* b64,
+                                         const BatchVec<double>* c64, const BatchVec<double>* e64,
+                                         const RefineState* st, int nrhs) {
+  const int j = blockIdx.y;
+  if (st && OP != 0 && OP != 4 && st[j].skip) return;
+  const int64_t n = (OP == 1 || OP == 0) ? s32[j].n : d64[j].n;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * nrhs) return;
+  const int64_t r = i / nrhs;
+  const int d = (int)(i - r * nrhs);
+  if (OP == 0) {
+    const BatchVec<float> S = s32[j];
+    const BatchVec<double> D = d64[j];
+    D.v[2 * (r * D.ld + d)] = (double)S.v[2 * (r * S.ld + d)];
+    D.v[2 * (r * D.ld + d) + 1] = (double)S.v[2 * (r * S.ld + d) + 1];
+  } else if (OP == 1) {
+    const BatchVec<double> R = d64[j];
+    const BatchVec<float> D = s32[j];
+    const bool on = st[j].mask[d] != 0;
+    D.v[2 * (r * D.ld + d)] = on ? (float)R.v[2 * (r * R.ld + d)] : 0.f;
+    D.v[2 * (r * D.ld + d) + 1] = on ? (float)R.v[2 * (r * R.ld + d) + 1] : 0.f;
+  } else if (OP == 2) {
+    const BatchVec<double> D = d64[j], X = a64[j];
+    const BatchVec<float> S = s32[j];
+    D.v[2 * (r * D.ld + d)] = X.v[2 * (r * X.ld + d)] + (double)S.v[2 * (r * S.ld + d)];
+    D.v[2 * (r * D.ld + d) + 1] = X.v[2 * (r * X.ld + d) + 1] + (double)S.v[2 * (r * S.ld + d) + 1];
+  } else if (OP == 3) {
+    if (!st[j].acc[d]) return;
+    const BatchVec<double> X = d64[j], NX = a64[j], R = b64[j], NR = c64[j];
+    X.v[2 * (r * X.ld + d)] = NX.v[2 * (r * NX.ld + d)];
+    X.v[2 * (r * X.ld + d) + 1] = NX.v[2 * (r * NX.ld + d) + 1];
+    R.v[2 * (r * R.ld + d)] = NR.v[2 * (r * NR.ld + d)];
+    R.v[2 * (r * R.ld + d) + 1] = NR.v[2 * (r * NR.ld + d) + 1];
+  } else {
+    const BatchVec<double> D = d64[j], S = e64[j];
+    D.v[2 * (r * D.ld + d)] = S.v[2 * (r * S.ld + d)];
+    D.v[2 * (r * D.ld + d) + 1] = S.v[2 * (r * S.ld + d) + 1];
+  }
+}
+
+// complex64 variant of OP 0 for the plain complex64 path: dst64 = (double) src32
+// is OP 0; the complex128 path copies with OP 4. Scatter of each system's internal
+// n x nrhs solution to the caller's column-major block.
+__global__ void batch_scatter_kernel(const BatchVec<double>* x, const BatchJobDesc* jobs, int nrhs) {
+  const int j = blockIdx.y;
+  const BatchVec<double> X = x[j];
+  const BatchJobDesc J = jobs[j];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= J.n * nrhs) return;
+  const int64_t v = i % J.n;
+  const int d = (int)(i / J.n);
+  const int64_t r = J.vert_inv[v];
+  J.p_out[2 * i] = X.v[2 * (r * X.ld + d)];
+  J.p_out[2 * i + 1] = X.v[2 * (r * X.ld + d) + 1];
+}
+
+// Group layout: per system the strides of every buffer kind (complex entries), so
+// the descriptors of a shape never change while the buffers stay.
+struct GroupLayout {
+  int J = 0;
+  int64_t nmax = 0, nfmax = 0, lda = 0;
+  size_t mat, vec, qv, linv;
+};
+
+int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int precision, const double* ks,
+                       const double* ers, const double* eis, const double* q_dev, const int64_t* q_off, int nrhs,
+                       double* p_dev, const int64_t* p_off, const beat_solve_options* opts, beat_solve_report* reps) {
+  if (int rc = check_solve_options(ctx, precision, opts)) return rc;
+  const int max_steps = opts ? opts->max_steps : 3;
+  const double rtol = opts ? opts->rtol : 1e-10;
+  hipStream_t s = ctx->stream;
+  const bool refined = precision == BEAT_PREC_FP32_REFINED, fp32 = precision == BEAT_PREC_FP32;
+  // buffers sized once for the largest group of this batch
+  GroupLayout G;
+  for (int64_t i = 0; i < njobs; ++i) {
+    G.nmax = std::max<int64_t>(G.nmax, meshes[i]->h.nv);
+    G.nfmax = std::max<int64_t>(G.nfmax, meshes[i]->h.nf);
+  }
+  G.J = (int)std::min<int64_t>(GROUP_MAX, njobs);
+  G.lda = lda_for(G.nmax + nrhs);
+  G.mat = (size_t)G.nmax * G.lda;
+  G.vec = (size_t)G.nmax * nrhs;
+  G.qv = (size_t)G.nfmax * nrhs;
+  G.linv = (size_t)((G.nmax + 63) / 64) * 64 * 64;
+  const size_t J = (size_t)G.J;
+  const bool want64 = !fp32, want32 = fp32 || refined;
+  // captured graphs hold these buffers' addresses: any reallocation drops them all
+  const void* before[] = {ctx->g_a64.p, ctx->g_a32.p, ctx->g_b64.p, ctx->g_b32.p, ctx->g_q64.p, ctx->g_q32.p,
+                          ctx->g_linv.p, ctx->g_vec.p, ctx->g_state.p, ctx->g_desc.p};
+  HIPCHK(ensure(ctx->g_q64, sizeof(double) * 2 * G.qv * J));
+  if (fp32) HIPCHK(ensure(ctx->g_q32, sizeof(float) * 2 * G.qv * J));
+  if (want64) HIPCHK(ensure(ctx->g_a64, sizeof(double) * 2 * G.mat * J));
+  if (want64) HIPCHK(ensure(ctx->g_b64, sizeof(double) * 2 * G.vec * J));
+  if (want32) HIPCHK(ensure(ctx->g_a32, sizeof(float) * 2 * G.mat * J));
+  if (want32) HIPCHK(ensure(ctx->g_b32, sizeof(float) * 2 * G.vec * J));
+  HIPCHK(ensure(ctx->g_linv, sizeof(double) * 2 * G.linv * J));
+  // vectors (complex128 n x nrhs): x, r, nx, nr, out; d (complex64)
+  HIPCHK(ensure(ctx->g_vec, sizeof(double) * 2 * G.vec * J * 5 + sizeof(float) * 2 * G.vec * J + 4096));
+  HIPCHK(ensure(ctx->g_state, sizeof(RefineState) * J + sizeof(int) * 4 * J + sizeof(double) * 16 * J + 4096));
+  const size_t desc_bytes = 16 * J * sizeof(BatchVec<double>) + 2 * J * sizeof(BatchSystem<double>) +
+                            J * sizeof(BatchJobDesc) + 4096;
+  HIPCHK(ensure(ctx->g_desc, desc_bytes));
+  {
+    const void* after[] = {ctx->g_a64.p, ctx->g_a32.p, ctx->g_b64.p, ctx->g_b32.p, ctx->g_q64.p, ctx->g_q32.p,
+                           ctx->g_linv.p, ctx->g_vec.p, ctx->g_state.p, ctx->g_desc.p};
+    if (!std::equal(std::begin(before), std::end(before), std::begin(after))) {
+      HIPCHK(hipStreamSynchronize(s));
+      for (auto& g : ctx->g_graphs) HIPCHK(hipGraphExecDestroy(g.second));
+      ctx->g_graphs.clear();
+    }
+  }
+  double* v64 = as<double>(ctx->g_vec);
+  double* vx = v64;
+  double* vr = vx + 2 * G.vec * J;
+  double* vnx = vr + 2 * G.vec * J;
+  double* vnr = vnx + 2 * G.vec * J;
+  double* vout = vnr + 2 * G.vec * J;
+  float* vd = reinterpret_cast<float*>(vout + 2 * G.vec * J);
+  RefineState* state = reinterpret_cast<RefineState*>(ctx->g_state.p);
+  int* lustat = reinterpret_cast<int*>(state + J);
+  double* norms = reinterpret_cast<double*>(lustat + 4 * J);
+  // descriptor arrays on the device: [0] systems (complex128 or complex64 A),
+  // then the vector kinds
+  char* dp = reinterpret_cast<char*>(ctx->g_desc.p);
+  BatchSystem<double>* sys64 = reinterpret_cast<BatchSystem<double>*>(dp);
+  BatchSystem<float>* sys32 = reinterpret_cast<BatchSystem<float>*>(sys64 + J);
+  BatchVec<double>* dv = reinterpret_cast<BatchVec<double>*>(sys32 + J);
+  BatchVec<double>* dX = dv, *dR = dv + J, *dNX = dv + 2 * J, *dNR = dv + 3 * J, *dB = dv + 4 * J, *dOut = dv + 5 * J,
+                   *dRs = dv + 6 * J, *dNXs = dv + 7 * J, *dNRs = dv + 8 * J;
+  BatchVec<float>* dD = reinterpret_cast<BatchVec<float>*>(dv + 9 * J);        // correction (skip-gated)
+  BatchVec<float>* dCar32 = dD + J;                                             // carried columns, complex64
+  BatchVec<double>* dCar64 = reinterpret_cast<BatchVec<double>*>(dCar32 + J);  // carried columns, complex128
+  BatchJobDesc* djobs = reinterpret_cast<BatchJobDesc*>(dCar64 + J);
+  std::vector<beat_solve_report> none;
+  int result = BEAT_OK;
+  for (int64_t g0 = 0; g0 < njobs; g0 += G.J) {
+    const int nj = (int)std::min<int64_t>(G.J, njobs - g0);
+    // host descriptors of this group
+    std::vector<BatchSystem<double>> h64(J);
+    std::vector<BatchSystem<float>> h32(J);
+    std::vector<BatchVec<double>> hv(9 * J);
+    std::vector<BatchVec<float>> hd(2 * J);
+    std::vector<BatchVec<double>> hc64(J);
+    std::vector<BatchJobDesc> hj(J);
+    int64_t gmax = 0, gncmax = 0;
+    // the graph's shape: options, strides and the group's sizes (the buffers stay:
+    // a reallocation above drops every graph)
+    std::string key = std::to_string(precision) + "/" + std::to_string(nrhs) + "/" + std::to_string(max_steps) + "/" +
+                      std::to_string(rtol) + "/" + std::to_string(nj) + "/" + std::to_string(G.lda) + "/" +
+                      std::to_string(G.mat) + "/" + std::to_string(G.vec) + "/" + std::to_string(G.linv) + "/" +
+                      std::to_string(G.J);
+    for (int jj = 0; jj < nj; ++jj) {
+      const beat_mesh* m = meshes[g0 + jj];
+      const int64_t n = m->h.nv;
+      gmax = std::max(gmax, n);
+      gncmax = std::max(gncmax, n + nrhs);
+      key += ":" + std::to_string(n);
+      int* stj = lustat + 4 * jj;
+      h64[jj] = BatchSystem<double>{as<double>(ctx->g_a64) + 2 * G.mat * jj, n, G.lda, n + nrhs,
+                                    as<double>(ctx->g_linv) + 2 * G.linv * jj, stj};
+      h32[jj] = BatchSystem<float>{as<float>(ctx->g_a32) + 2 * G.mat * jj, n, G.lda, n + nrhs,
+                                   as<float>(ctx->g_linv) + 2 * G.linv * jj, stj};
+      const int* skip = &state[jj].skip;
+      auto v = [&](double* base, const int* sk) { return BatchVec<double>{base + 2 * G.vec * jj, n, nrhs, sk}; };
+      hv[jj] = v(vx, nullptr);
+      hv[J + jj] = v(vr, nullptr);
+      hv[2 * J + jj] = v(vnx, skip);
+      hv[3 * J + jj] = v(vnr, skip);
+      hv[4 * J + jj] = BatchVec<double>{as<double>(ctx->g_b64) + 2 * G.vec * jj, n, nrhs, nullptr};
+      hv[5 * J + jj] = v(vout, nullptr);
+      hv[6 * J + jj] = v(vr, skip);
+      hv[7 * J + jj] = v(vnx, skip);
+      hv[8 * J + jj] = v(vnr, skip);
+      hd[jj] = BatchVec<float>{vd + 2 * G.vec * jj, n, nrhs, skip};
+      hd[J + jj] = BatchVec<float>{as<float>(ctx->g_a32) + 2 * (G.mat * jj + n), n, G.lda, nullptr};
+      hc64[jj] = BatchVec<double>{as<double>(ctx->g_a64) + 2 * (G.mat * jj + n), n, G.lda, nullptr};
+      hj[jj] = BatchJobDesc{n, m->vert_inv, p_dev + 2 * p_off[g0 + jj]};
+    }
+    HIPCHK(hipMemcpyAsync(sys64, h64.data(), sizeof(BatchSystem<double>) * J, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(sys32, h32.data(), sizeof(BatchSystem<float>) * J, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dv, hv.data(), sizeof(BatchVec<double>) * 9 * J, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dD, hd.data(), sizeof(BatchVec<float>) * 2 * J, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(dCar64, hc64.data(), sizeof(BatchVec<double>) * J, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(djobs, hj.data(), sizeof(BatchJobDesc) * J, hipMemcpyHostToDevice, s));
+    // 1. assembly of every system of the group (eager: the wavenumber is an argument)
+    HIPCHK(hipEventRecord(ctx->ev[0], s));
+    for (int jj = 0; jj < nj; ++jj) {
+      const int64_t i = g0 + jj;
+      const beat_mesh* m = meshes[i];
+      const int64_t n = m->h.nv, nf = m->h.nf;
+      double* q64 = as<double>(ctx->g_q64) + 2 * G.qv * jj;
+      hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, s, q_dev + 2 * q_off[i],
+                         nf, nrhs, (const int32_t*)m->face_perm, q64);
+      HIPCHK(hipGetLastError());
+      if (fp32) {
+        float* q32 = as<float>(ctx->g_q32) + 2 * G.qv * jj;
+        float* A = h32[jj].A;
+        float* B = as<float>(ctx->g_b32) + 2 * G.vec * jj;
+        HIPCHK(launch_downcast(q64, q32, 2 * nf * nrhs, s));
+        if (int rc = assemble<float>(ctx, m, ks[i], ers[i], eis[i], q64, q32, nrhs, A, G.lda, B, s, nullptr, nullptr))
+          return rc;
+        HIPCHK(lu_append_rhs<float>(A, n, G.lda, B, nrhs, s));
+      } else {
+        double* A = h64[jj].A;
+        double* B = as<double>(ctx->g_b64) + 2 * G.vec * jj;
+        if (int rc = assemble<double>(ctx, m, ks[i], ers[i], eis[i], q64, q64, nrhs, A, G.lda, B, s, nullptr, nullptr))
+          return rc;
+        if (refined) {
+          float* A32 = h32[jj].A;
+          float* B32 = as<float>(ctx->g_b32) + 2 * G.vec * jj;
+          HIPCHK(launch_downcast(A, A32, 2 * n * G.lda, s));
+          HIPCHK(launch_downcast(B, B32, 2 * n * nrhs, s));
+          HIPCHK(lu_append_rhs<float>(A32, n, G.lda, B32, nrhs, s));
+        } else {
+          HIPCHK(lu_append_rhs<double>(A, n, G.lda, B, nrhs, s));
+        }
+      }
+    }
+    HIPCHK(hipEventRecord(ctx->ev[1], s));
+    // 2. factorization, solves and refinement of the whole group: one graph
+    hipGraphExec_t exec = nullptr;
+    auto it = ctx->g_graphs.find(key);
+    if (it != ctx->g_graphs.end()) {
+      exec = it->second;
+    } else {
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+      hipError_t ce = hipSuccess;
+      auto ok = [&](hipError_t e) {
+        if (ce == hipSuccess) ce = e;
+      };
+      ok(launch_fill<int>(lustat, 4 * (int64_t)nj, 0, s));
+      const unsigned tb = (unsigned)((nj + 63) / 64);
+      const dim3 ge((unsigned)((gmax * nrhs + 255) / 256), (unsigned)nj);
+      if (want32 && !fp32) {
+        // refined: complex64 speculative LU (three-product updates), first solution
+        ok(lu_factor_batched<float>(sys32, nj, gmax, gncmax, true, s));
+        ok(tri_solve_batched<float>(sys32, dCar32, nj, nrhs, false, true, s));
+        hipLaunchKernelGGL((batch_elementwise_kernel<0>), ge, dim3(256), 0, s, dX, dCar32, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, nrhs);
+        // complex128 refinement with the device-side decisions
+        hipLaunchKernelGGL(colnorms_batched_kernel, dim3((unsigned)nj), dim3(256), 0, s, dB, nrhs, norms, 0);
+        ok(residual64_batched(sys64, dX, dB, dR, nj, gmax, nrhs, s));
+        hipLaunchKernelGGL(colnorms_batched_kernel, dim3((unsigned)nj), dim3(256), 0, s, dR, nrhs, norms, 8);
+        hipLaunchKernelGGL(refine_control_kernel, dim3(tb), dim3(64), 0, s, state, norms, nj, nrhs, rtol, 0, 0);
+        for (int step = 1; step <= max_steps; ++step) {
+          hipLaunchKernelGGL((batch_elementwise_kernel<1>), ge, dim3(256), 0, s, dRs, dD, nullptr, nullptr, nullptr,
+                             nullptr, state, nrhs);
+          ok(tri_solve_batched<float>(sys32, dD, nj, nrhs, true, true, s));
+          hipLaunchKernelGGL((batch_elementwise_kernel<2>), ge, dim3(256), 0, s, dNXs, dD, dX, nullptr, nullptr,
+                             nullptr, state, nrhs);
+          ok(residual64_batched(sys64, dNXs, dB, dNRs, nj, gmax, nrhs, s));
+          hipLaunchKernelGGL(colnorms_batched_kernel, dim3((unsigned)nj), dim3(256), 0, s, dNRs, nrhs, norms, 8);
+          hipLaunchKernelGGL(refine_control_kernel, dim3(tb), dim3(64), 0, s, state, norms, nj, nrhs, rtol, 1, step);
+          hipLaunchKernelGGL((batch_elementwise_kernel<3>), ge, dim3(256), 0, s, dX, nullptr, dNX, dR, dNR, nullptr,
+                             state, nrhs);
+          hipLaunchKernelGGL(refine_advance_kernel, dim3(tb), dim3(64), 0, s, state, nj);
+        }
+        hipLaunchKernelGGL(refine_control_kernel, dim3(tb), dim3(64), 0, s, state, norms, nj, nrhs, rtol, 2, 0);
+        hipLaunchKernelGGL((batch_elementwise_kernel<4>), ge, dim3(256), 0, s, dOut, nullptr, nullptr, nullptr,
+                           nullptr, dX, nullptr, nrhs);
+      } else if (fp32) {
+        ok(lu_factor_batched<float>(sys32, nj, gmax, gncmax, false, s));
+        ok(tri_solve_batched<float>(sys32, dCar32, nj, nrhs, false, true, s));
+        hipLaunchKernelGGL((batch_elementwise_kernel<0>), ge, dim3(256), 0, s, dOut, dCar32, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, nrhs);
+      } else {
+        ok(lu_factor_batched<double>(sys64, nj, gmax, gncmax, false, s));
+        ok(tri_solve_batched<double>(sys64, dCar64, nj, nrhs, false, true, s));
+        hipLaunchKernelGGL((batch_elementwise_kernel<4>), ge, dim3(256), 0, s, dOut, nullptr, nullptr, nullptr,
+                           nullptr, dCar64, nullptr, nrhs);
+      }
+      ok(hipGetLastError());
+      hipError_t ee = hipStreamEndCapture(s, &graph);
+      if (ce != hipSuccess || ee != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(ctx, BEAT_ERR_HIP, std::string("batch graph capture: ") +
+                                           hipGetErrorString(ce != hipSuccess ? ce : ee));
+      }
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) return fail(ctx, BEAT_ERR_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(ie));
+      ctx->g_graphs[key] = exec;
+    }
+    HIPCHK(hipGraphLaunch(exec, s));
+    HIPCHK(hipEventRecord(ctx->ev[2], s));
+    // 3. the pressures to the caller's blocks
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3((unsigned)((gmax * nrhs + 255) / 256), (unsigned)nj), dim3(256), 0, s,
+                       (const BatchVec<double>*)dOut, (const BatchJobDesc*)djobs, nrhs);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(ctx->ev[3], s));
+    std::vector<RefineState> hs(J);
+    std::vector<int> hl(4 * J);
+    HIPCHK(hipMemcpyAsync(hs.data(), state, sizeof(RefineState) * nj, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(hl.data(), lustat, sizeof(int) * 4 * nj, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ta = 0, tf = 0;
+    HIPCHK(hipEventElapsedTime(&ta, ctx->ev[0], ctx->ev[1]));
+    HIPCHK(hipEventElapsedTime(&tf, ctx->ev[1], ctx->ev[3]));
+    for (int jj = 0; jj < nj; ++jj) {
+      const int64_t i = g0 + jj;
+      const bool exchange = hl[4 * jj + 3] != 0 || ctx->debug_force_exchange;
+      if (exchange) {
+        // partial pivoting would exchange rows here: this job again on its own
+        const beat_mesh* m = meshes[i];
+        SystemBufs sb{&ctx->q64, &ctx->q32, &ctx->a64, &ctx->b64, &ctx->a32, &ctx->b32, &ctx->af};
+        const AsmEvents ev{ctx->ev[0], ctx->ev[4], ctx->ev[5], ctx->ev[6]};
+        if (int rc = stage_assemble(ctx, m, precision, ks[i], ers[i], eis[i], q_dev + 2 * q_off[i], nrhs, sb, s, ev))
+          return rc;
+        int rc = stage_solve(ctx, m, precision, nrhs, sb, p_dev + 2 * p_off[i], opts, ev, reps ? reps + i : nullptr);
+        if (rc == BEAT_ERR_SINGULAR) result = rc;
+        else if (rc) return rc;
+        if (reps) reps[i].factorization_count = 2;
+        continue;
+      }
+      if (hl[4 * jj] != 0) {
+        result = BEAT_ERR_SINGULAR;
+        fail(ctx, BEAT_ERR_SINGULAR, "LU found an exactly zero pivot at column " + std::to_string(hl[4 * jj]));
+      }
+      if (!reps) continue;
+      beat_solve_report& R = reps[i];
+      const beat_mesh* m = meshes[i];
+      R.assembly_s = ta * 1e-3 / nj;
+      R.regular_kernel_s = 0;
+      R.factor_s = tf * 1e-3 / nj;  // the group's graph, shared evenly
+      R.solve_s = 0;
+      R.factorization_count = 1;
+      R.lu_panels = (int)((m->h.nv + 63) / 64);
+      R.regular_pairs = m->regular_pairs;
+      R.singular_pairs = (int64_t)m->h.spairs.size();
+      if (refined) {
+        const RefineState& S = hs[jj];
+        R.refine.steps = S.steps;
+        R.refine.status = S.status;
+        R.refine.working_status = S.working;
+        R.refine.history_len = S.hist_len;
+        R.refine.rtol = rtol;
+        if (R.refine.working_residuals)
+          for (int d = 0; d < nrhs; ++d) R.refine.working_residuals[d] = S.err[d];
+        if (R.refine.returned_residuals)
+          for (int d = 0; d < nrhs; ++d) R.refine.returned_residuals[d] = S.err[d];
+        if (R.refine.history)
+          for (int e = 0; e < S.hist_len; ++e)
+            for (int d = 0; d < nrhs; ++d) R.refine.history[e * nrhs + d] = S.hist[e * 8 + d];
+      } else {
+        R.refine.steps = 0;
+        R.refine.status = BEAT_REFINE_NONE;
+        R.refine.working_status = BEAT_REFINE_NONE;
+        R.refine.history_len = 0;
+        R.refine.rtol = rtol;
+      }
+    }
+  }
+  return result;
+}
+
 // A frequency sweep on one mesh: consecutive q and p blocks.
 int sweep_impl(beat_ctx* ctx, const beat_mesh* m, int precision, int64_t nfreq, const double* ks, const double* ers,
                const double* eis, const double* q_dev, int nrhs, double* p_dev, const beat_solve_options* opts,
@@ -1351,6 +1857,7 @@ int beat_ctx_create(int device, beat_ctx** out) {
   }
   if (const char* v = std::getenv("BEAT_SWEEP_ASM_ONE_PER_CU")) c->sweep_asm_one_per_cu = v[0] != '0';
   if (const char* v = std::getenv("BEAT_BATCH_WORKERS")) c->batch_workers = std::max(1, std::atoi(v));
+  if (const char* v = std::getenv("BEAT_BATCH_GROUPED")) c->batch_grouped = v[0] != '0';
   // BEAT_LU_LOOKAHEAD=0 runs the whole factorization in order on one stream
   if (const char* v = std::getenv("BEAT_LU_LOOKAHEAD"))
     if (v[0] == '0') {
@@ -1369,9 +1876,13 @@ void beat_ctx_destroy(beat_ctx* c) {
   c->aux.clear();
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  for (auto& g : c->g_graphs) (void)hipGraphExecDestroy(g.second);
+  c->g_graphs.clear();
   DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64,
                     &c->rn64, &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
-                    &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af};
+                    &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af,
+                    &c->g_a64, &c->g_a32, &c->g_b64, &c->g_b32, &c->g_q64, &c->g_q32, &c->g_linv, &c->g_vec,
+                    &c->g_state, &c->g_desc};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   if (c->asm_stream) (void)hipStreamSynchronize(c->asm_stream);
@@ -1431,9 +1942,13 @@ int beat_ctx_reclaim(beat_ctx* c) {
   HIPCHK(hipStreamSynchronize(c->stream));
   if (c->side) HIPCHK(hipStreamSynchronize(c->side));
   if (c->inb) HIPCHK(hipStreamSynchronize(c->inb));
+  for (auto& g : c->g_graphs) HIPCHK(hipGraphExecDestroy(g.second));
+  c->g_graphs.clear();
   DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64, &c->rn64,
                     &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin, &c->pout,
-                    &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af};
+                    &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af,
+                    &c->g_a64, &c->g_a32, &c->g_b64, &c->g_b32, &c->g_q64, &c->g_q32, &c->g_linv, &c->g_vec,
+                    &c->g_state, &c->g_desc};
   for (DevBuf* b : bufs) {
     if (b->p) HIPCHK(hipFree(b->p));
     b->p = nullptr;
@@ -1458,7 +1973,9 @@ int beat_ctx_memory_info(beat_ctx* c, int64_t* info) {
   size_t used = 0;
   const DevBuf* bufs[] = {&c->a64, &c->a32, &c->b64, &c->b32, &c->q64, &c->q32, &c->x64, &c->r64, &c->n64, &c->rn64,
                           &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
-                          &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af};
+                          &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af,
+                    &c->g_a64, &c->g_a32, &c->g_b64, &c->g_b32, &c->g_q64, &c->g_q32, &c->g_linv, &c->g_vec,
+                    &c->g_state, &c->g_desc};
   for (const DevBuf* b : bufs) used += b->bytes;
   for (int s2 = 0; s2 < 2; ++s2) {
     const DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2],
@@ -1972,6 +2489,20 @@ int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, 
   HIPCHK(ensure(ctx->pout, sizeof(double) * 2 * pt));
   HIPCHK(hipMemcpyAsync(ctx->qin.p, q, sizeof(double) * 2 * qt, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
+  {
+    // systems small enough go through the grouped path: batched factorizations,
+    // solves and refinement, replayed as one graph per group shape
+    bool grouped = nrhs <= SOLVE_CHUNK && ctx->batch_grouped && ctx->cancel_path.empty();
+    for (int64_t i = 0; i < njobs && grouped; ++i) grouped = meshes[i]->h.nv <= GROUP_NMAX;
+    if (grouped) {
+      int rc = batch_grouped_impl(ctx, njobs, meshes, precision, k, eta_re, eta_im, as<double>(ctx->qin), qo.data(),
+                                  (int)nrhs, as<double>(ctx->pout), po.data(), opts, reps);
+      if (rc != BEAT_OK && rc != BEAT_ERR_SINGULAR) return rc;
+      HIPCHK(hipMemcpyAsync(pressure_out, ctx->pout.p, sizeof(double) * 2 * pt, hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      return rc;
+    }
+  }
   // Small systems leave most of the device idle, so several contexts run jobs at
   // once. Their factorizations and triangular solves keep all their workgroups
   // waiting on each other; the number of contexts is bounded so that those

@@ -188,6 +188,44 @@ hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nr
 template <typename T>
 hipError_t lu_solve_transposed(const T* LU, int64_t n, int64_t lda, T* v, T* x, const T* work, hipStream_t s);
 
+// ---- batched factorization and solves (many small systems at once) ----------------
+// One system of a batch, as a device array entry. A: n x lda row-major, factored
+// in place; its columns [n, nc) carry the right-hand sides (nc <= lda); linv: one
+// 64 x 64 complex leaf inverse per 64 columns; status: the LuWork info block
+// ([0] first zero pivot, [1] abort word, [2] give-up record, [3] exchange needed).
+template <typename T>
+struct BatchSystem {
+  T* A;
+  int64_t n, lda, nc;
+  T* linv;
+  int* status;
+};
+// A vector block of a system: n x nrhs complex T with row stride ld; skip: when
+// non-null and nonzero, the batched kernels leave this system alone (its
+// refinement has finished).
+template <typename T>
+struct BatchVec {
+  T* v;
+  int64_t n, ld;
+  const int* skip;
+};
+// Speculative LU (no exchanges, every column checked) of nsys systems at once,
+// leaf by leaf: per 64 columns a diagonal-block kernel, a rows-below kernel and
+// the two trailing products, each one launch for every system. nmax / ncmax: the
+// largest n and nc. Zero pivots and needed exchanges land in each status.
+template <typename T>
+hipError_t lu_factor_batched(const BatchSystem<T>* sys, int nsys, int64_t nmax, int64_t ncmax, bool gauss3,
+                             hipStream_t s);
+// With the factors of lu_factor_batched: forward (L y = v) and/or backward (U x = y)
+// substitution of each system's vector block in place, one workgroup per system.
+template <typename T>
+hipError_t tri_solve_batched(const BatchSystem<T>* sys, const BatchVec<T>* vec, int nsys, int nrhs, bool fwd, bool bwd,
+                             hipStream_t s);
+// r = b - A x for every system (A the complex128 matrices of sys, x / b / r the
+// vector blocks), nrhs <= 8.
+hipError_t residual64_batched(const BatchSystem<double>* sys, const BatchVec<double>* x, const BatchVec<double>* b,
+                              const BatchVec<double>* r, int nsys, int64_t nmax, int nrhs, hipStream_t s);
+
 // r = b - A x (complex128), A row-major n x n, x and b n x nrhs row-major.
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
                       hipStream_t s);

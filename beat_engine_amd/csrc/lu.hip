@@ -701,14 +701,8 @@ template <typename T, bool SET, int BM, int KB, bool G3 = false>
 // B may alias C (U12 = L11^-1 A12 in place): every workgroup then reads all of
 // its rows' K before its epilogue writes them, which needs the whole of m in one
 // row tile (m <= BM, checked by gemm_launch), and B carries no __restrict__.
-__global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int64_t lda, const T* B,
-                                                    int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n, int kc,
-                                                    int behind) {
-  // in a sweep the factorization and solves run beside the next system's
-  // assembly: their waves issue ahead of the assembly's; the look-ahead update
-  // (behind) issues after the factorization's critical path
-  if (behind) __builtin_amdgcn_s_setprio(1);
-  else __builtin_amdgcn_s_setprio(2);
+__device__ __forceinline__ void zgemm_tile(const T* __restrict__ A, int64_t lda, const T* B, int64_t ldb, T* C,
+                                           int64_t ldc, int64_t m, int64_t n, int kc, int64_t row0, int64_t col0) {
   constexpr int BN = 64;
   constexpr int VEC = GemmCfg<T>::VEC;
   constexpr int AP = 2 * KB + VEC;  // padded row lengths in scalars
@@ -720,7 +714,6 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
   __shared__ __attribute__((aligned(16))) T Bs[KB * BP];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int64_t row0 = (int64_t)blockIdx.y * BM, col0 = (int64_t)blockIdx.x * BN;
   vec ra[AV], rb[BV];
 
   auto load = [&](int k0) {
@@ -914,6 +907,174 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
           }
         }
   }
+}
+
+template <typename T, bool SET, int BM, int KB, bool G3 = false>
+__global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int64_t lda, const T* B,
+                                                    int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n, int kc,
+                                                    int behind) {
+  // in a sweep the factorization and solves run beside the next system's
+  // assembly: their waves issue ahead of the assembly's; the look-ahead update
+  // (behind) issues after the factorization's critical path
+  if (behind) __builtin_amdgcn_s_setprio(1);
+  else __builtin_amdgcn_s_setprio(2);
+  zgemm_tile<T, SET, BM, KB, G3>(A, lda, B, ldb, C, ldc, m, n, kc, (int64_t)blockIdx.y * BM, (int64_t)blockIdx.x * 64);
+}
+
+// Leaf t's trailing products for every system of a batch: UPPER, U12 = L11^-1 A12
+// in place over the columns [j0 + b, nc); else A22 -= L21 U12 over the same
+// columns and the rows below the leaf. blockIdx.z: the system.
+template <typename T, bool UPPER, bool G3>
+__global__ __launch_bounds__(256) void zgemm_batched_kernel(const BatchSystem<T>* sys, int t) {
+  const BatchSystem<T> S = sys[blockIdx.z];
+  const int64_t j0 = (int64_t)t * PNB;
+  if (j0 >= S.n) return;
+  const int b = (int)min((int64_t)PNB, S.n - j0);
+  const int64_t cols = S.nc - j0 - b, rows = UPPER ? b : S.n - j0 - b;
+  const int64_t row0 = (int64_t)blockIdx.y * 64, col0 = (int64_t)blockIdx.x * 64;
+  if (cols <= 0 || rows <= 0 || row0 >= rows || col0 >= cols) return;
+  T* const U12 = S.A + 2 * (j0 * S.lda + j0 + b);
+  if (UPPER)
+    zgemm_tile<T, true, 64, 16, G3>(S.linv + 2 * (int64_t)t * PNB * PNB, PNB, U12, S.lda, U12, S.lda, rows, cols, b,
+                                    row0, col0);
+  else
+    zgemm_tile<T, false, 64, 16, G3>(S.A + 2 * ((j0 + b) * S.lda + j0), S.lda, U12, S.lda,
+                                     S.A + 2 * ((j0 + b) * S.lda + j0 + b), S.lda, rows, cols, b, row0, col0);
+}
+
+// Triangular substitutions of one system's vector block by one workgroup (1024
+// threads): forward L y = v (unit lower, below the diagonal of A), then backward
+// U x = y, block of 64 rows by block. Per block the diagonal part is solved by
+// one wavefront per right-hand side (lane = row, the solved entry broadcast by
+// shuffle) from the block staged in LDS; the other rows then subtract the block's
+// contribution, each thread owning rows and reading its row of the block.
+template <typename T, bool FWD, bool BWD>
+__global__ __launch_bounds__(1024) void tri_wg_kernel(const BatchSystem<T>* sys, const BatchVec<T>* vec, int nrhs) {
+  const BatchSystem<T> S = sys[blockIdx.x];
+  const BatchVec<T> V = vec[blockIdx.x];
+  if (V.skip && *V.skip) return;
+  __shared__ T blk[PNB][PNB + 1][2];
+  __shared__ T ys[PNB][8][2];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t n = S.n, lda = S.lda, ld = V.ld;
+  T* v = V.v;
+  const T* A = S.A;
+  const int nblk = (int)((n + PNB - 1) / PNB);
+  if (FWD)
+    for (int kb = 0; kb < nblk; ++kb) {
+      const int64_t jb = (int64_t)kb * PNB;
+      const int nb = (int)min((int64_t)PNB, n - jb);
+      for (int e = t; e < PNB * PNB; e += 1024) {
+        const int i = e / PNB, j = e - i * PNB;
+        const bool in = i < nb && j < i;
+        blk[i][j][0] = in ? A[2 * ((jb + i) * lda + jb + j)] : T(0);
+        blk[i][j][1] = in ? A[2 * ((jb + i) * lda + jb + j) + 1] : T(0);
+      }
+      __syncthreads();
+      if (wave < nrhs) {
+        const int d = wave;
+        T yr = lane < nb ? v[2 * ((jb + lane) * ld + d)] : T(0), yi = lane < nb ? v[2 * ((jb + lane) * ld + d) + 1] : T(0);
+        for (int j = 0; j < nb; ++j) {
+          const T zr = __shfl(yr, j, 64), zi = __shfl(yi, j, 64);
+          if (lane > j) {
+            const T lr = blk[lane][j][0], li = blk[lane][j][1];
+            yr -= lr * zr - li * zi;
+            yi -= lr * zi + li * zr;
+          }
+        }
+        if (lane < nb) {
+          v[2 * ((jb + lane) * ld + d)] = yr;
+          v[2 * ((jb + lane) * ld + d) + 1] = yi;
+        }
+        ys[lane][d][0] = lane < nb ? yr : T(0);
+        ys[lane][d][1] = lane < nb ? yi : T(0);
+      }
+      __syncthreads();
+      for (int64_t i = jb + nb + t; i < n; i += 1024) {
+        T ar[8], ai[8];
+#pragma unroll
+        for (int d = 0; d < 8; ++d) ar[d] = ai[d] = T(0);
+        const T* Li = A + 2 * (i * lda + jb);
+        for (int j = 0; j < nb; ++j) {
+          const T lr = Li[2 * j], li = Li[2 * j + 1];
+#pragma unroll
+          for (int d = 0; d < 8; ++d)
+            if (d < nrhs) {
+              const T zr = ys[j][d][0], zi = ys[j][d][1];
+              ar[d] += lr * zr - li * zi;
+              ai[d] += lr * zi + li * zr;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 8; ++d)
+          if (d < nrhs) {
+            v[2 * (i * ld + d)] -= ar[d];
+            v[2 * (i * ld + d) + 1] -= ai[d];
+          }
+      }
+      __syncthreads();
+    }
+  if (BWD)
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+      const int64_t jb = (int64_t)kb * PNB;
+      const int nb = (int)min((int64_t)PNB, n - jb);
+      for (int e = t; e < PNB * PNB; e += 1024) {
+        const int i = e / PNB, j = e - i * PNB;
+        const bool in = i < nb && j >= i && j < nb;
+        blk[i][j][0] = in ? A[2 * ((jb + i) * lda + jb + j)] : T(0);
+        blk[i][j][1] = in ? A[2 * ((jb + i) * lda + jb + j) + 1] : T(0);
+      }
+      __syncthreads();
+      if (wave < nrhs) {
+        const int d = wave;
+        T xr = lane < nb ? v[2 * ((jb + lane) * ld + d)] : T(0), xi = lane < nb ? v[2 * ((jb + lane) * ld + d) + 1] : T(0);
+        for (int j = nb - 1; j >= 0; --j) {
+          if (lane == j) {
+            const T ur = blk[j][j][0], ui = blk[j][j][1];
+            const T den = ur * ur + ui * ui;
+            const T qr = (xr * ur + xi * ui) / den, qi = (xi * ur - xr * ui) / den;
+            xr = qr;
+            xi = qi;
+          }
+          const T zr = __shfl(xr, j, 64), zi = __shfl(xi, j, 64);
+          if (lane < j) {
+            const T ur = blk[lane][j][0], ui = blk[lane][j][1];
+            xr -= ur * zr - ui * zi;
+            xi -= ur * zi + ui * zr;
+          }
+        }
+        if (lane < nb) {
+          v[2 * ((jb + lane) * ld + d)] = xr;
+          v[2 * ((jb + lane) * ld + d) + 1] = xi;
+        }
+        ys[lane][d][0] = lane < nb ? xr : T(0);
+        ys[lane][d][1] = lane < nb ? xi : T(0);
+      }
+      __syncthreads();
+      for (int64_t i = t; i < jb; i += 1024) {
+        T ar[8], ai[8];
+#pragma unroll
+        for (int d = 0; d < 8; ++d) ar[d] = ai[d] = T(0);
+        const T* Ui = A + 2 * (i * lda + jb);
+        for (int j = 0; j < nb; ++j) {
+          const T ur = Ui[2 * j], ui = Ui[2 * j + 1];
+#pragma unroll
+          for (int d = 0; d < 8; ++d)
+            if (d < nrhs) {
+              const T zr = ys[j][d][0], zi = ys[j][d][1];
+              ar[d] += ur * zr - ui * zi;
+              ai[d] += ur * zi + ui * zr;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 8; ++d)
+          if (d < nrhs) {
+            v[2 * (i * ld + d)] -= ar[d];
+            v[2 * (i * ld + d) + 1] -= ai[d];
+          }
+      }
+      __syncthreads();
+    }
 }
 
 template <typename T, bool SET>
@@ -1285,14 +1446,13 @@ __device__ __forceinline__ void halve_step(double* v, int lane) {
   }
 }
 
-__global__ __launch_bounds__(512) void residual64_kernel(const double* A, int64_t n, int64_t lda, const double* x,
-                                                         const double* b, double* r, int nrhs) {
-  // in a sweep the factorization and solves run beside the next system's
-  // assembly: their waves issue ahead of the assembly's
-  __builtin_amdgcn_s_setprio(2);
+// xld / bld / rld: row strides of x, b and r (nrhs for the single-system form)
+__device__ __forceinline__ void residual64_body(const double* A, int64_t n, int64_t lda, const double* x, int64_t xld,
+                                                const double* b, int64_t bld, double* r, int64_t rld, int nrhs,
+                                                int64_t block) {
   __shared__ double xs[8 * RES_KC * 2];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int64_t row0 = ((int64_t)blockIdx.x * RES_WAVES + wave) * RES_ROWS;
+  const int64_t row0 = (block * RES_WAVES + wave) * RES_ROWS;
   double acc[RES_ROWS * 8 * 2];
 #pragma unroll
   for (int j = 0; j < RES_ROWS * 8 * 2; ++j) acc[j] = 0;
@@ -1318,8 +1478,8 @@ __global__ __launch_bounds__(512) void residual64_kernel(const double* A, int64_
       const int kk = idx >> 3, d = idx & 7;
       double xr = 0, xi = 0;
       if (kk < kc && d < nrhs) {
-        xr = x[2 * ((k0 + kk) * nrhs + d)];
-        xi = x[2 * ((k0 + kk) * nrhs + d) + 1];
+        xr = x[2 * ((k0 + kk) * xld + d)];
+        xi = x[2 * ((k0 + kk) * xld + d) + 1];
       }
       xs[2 * (d * RES_KC + kk)] = xr;
       xs[2 * (d * RES_KC + kk) + 1] = xi;
@@ -1352,7 +1512,26 @@ __global__ __launch_bounds__(512) void residual64_kernel(const double* A, int64_
   // lane l holds sum number l = (q * 8 + d) * 2 + (0: re, 1: im)
   const int q = lane >> 4, d = (lane >> 1) & 7, c = lane & 1;
   const int64_t row = row0 + q;
-  if (row < n && d < nrhs) r[2 * (row * nrhs + d) + c] = b[2 * (row * nrhs + d) + c] - acc[0];
+  if (row < n && d < nrhs) r[2 * (row * rld + d) + c] = b[2 * (row * bld + d) + c] - acc[0];
+}
+
+__global__ __launch_bounds__(512) void residual64_kernel(const double* A, int64_t n, int64_t lda, const double* x,
+                                                         const double* b, double* r, int nrhs) {
+  // in a sweep the factorization and solves run beside the next system's
+  // assembly: their waves issue ahead of the assembly's
+  __builtin_amdgcn_s_setprio(2);
+  residual64_body(A, n, lda, x, nrhs, b, nrhs, r, nrhs, nrhs, (int64_t)blockIdx.x);
+}
+
+__global__ __launch_bounds__(512) void residual64_batched_kernel(const BatchSystem<double>* sys,
+                                                                 const BatchVec<double>* x, const BatchVec<double>* b,
+                                                                 const BatchVec<double>* r, int nrhs) {
+  const int j = blockIdx.y;
+  const BatchSystem<double> S = sys[j];
+  const BatchVec<double> X = x[j], B = b[j], R = r[j];
+  if (R.skip && *R.skip) return;
+  if ((int64_t)blockIdx.x * RES_WAVES * RES_ROWS >= S.n) return;
+  residual64_body(S.A, S.n, S.lda, X.v, X.ld, B.v, B.ld, R.v, R.ld, nrhs, (int64_t)blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void colnorm_partial_kernel(const double* X, int64_t n, int nrhs, double* part) {
@@ -1443,9 +1622,7 @@ __device__ __forceinline__ void pivot_reciprocal(T ur, T ui, T& ir, T& ii) {
 // compiler would otherwise defer them to their first use and keep every step's
 // operands live).
 template <typename T>
-__global__ __launch_bounds__(256) void diag_nopiv_kernel(T* A, int64_t lda, int64_t j0, int b, T* linv, int* info,
-                                                         int* ctl) {
-  __builtin_amdgcn_s_setprio(3);
+__device__ __forceinline__ void diag_nopiv_body(T* A, int64_t lda, int64_t j0, int b, T* linv, int* info, int* ctl) {
   __shared__ T prow[2][4][16][2];  // row j of U by column class [parity][c % 4][c / 4][re, im]
   __shared__ T plow[2][4][16][2];  // row j of L^-1
   __shared__ T prcp[2][2];         // 1 / u_jj
@@ -1534,17 +1711,24 @@ __global__ __launch_bounds__(256) void diag_nopiv_kernel(T* A, int64_t lda, int6
   if (t == 0 && zero != 0 && *info == 0) *info = (int)j0 + zero;
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void diag_nopiv_kernel(T* A, int64_t lda, int64_t j0, int b, T* linv, int* info,
+                                                         int* ctl) {
+  __builtin_amdgcn_s_setprio(3);
+  diag_nopiv_body(A, lda, j0, b, linv, info, ctl);
+}
+
 // Rows below the diagonal block of a leaf: L21 = A21 U11^-1 by substitution
 // (U11 staged in LDS by column class; no barrier inside the column loop), each
 // column checked against izamax's choice: no row's |re| + |im| may exceed the
 // diagonal's. 64 rows per workgroup, threads laid out as in diag_nopiv_kernel.
 template <typename T>
-__global__ __launch_bounds__(256) void below_nopiv_kernel(T* A, int64_t n, int64_t lda, int64_t j0, int b, int* ctl) {
-  __builtin_amdgcn_s_setprio(3);
+__device__ __forceinline__ void below_nopiv_body(T* A, int64_t n, int64_t lda, int64_t j0, int b, int* ctl,
+                                                 int64_t rblock) {
   __shared__ T us[PNB][4][16][2];  // U11 row j by column class
   __shared__ T rcp[PNB][2];        // 1 / u_jj
   const int t = threadIdx.x, r = t >> 2, q = t & 3;
-  const int64_t row = j0 + b + (int64_t)blockIdx.x * PNB + r;
+  const int64_t row = j0 + b + rblock * PNB + r;
   T xr[16], xi[16];
   {
     const T* src = A + 2 * ((row < n ? row : j0) * lda + j0 + q);  // rows beyond n: an in-bounds row, masked
@@ -1605,6 +1789,31 @@ __global__ __launch_bounds__(256) void below_nopiv_kernel(T* A, int64_t n, int64
       }
   }
   if (__ballot(viol) != 0 && (t & 63) == 0) __hip_atomic_fetch_or(ctl + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void below_nopiv_kernel(T* A, int64_t n, int64_t lda, int64_t j0, int b, int* ctl) {
+  __builtin_amdgcn_s_setprio(3);
+  below_nopiv_body(A, n, lda, j0, b, ctl, (int64_t)blockIdx.x);
+}
+
+// Leaf t of every system of a batch (systems with fewer columns exit at once).
+template <typename T>
+__global__ __launch_bounds__(256) void diag_nopiv_batched_kernel(const BatchSystem<T>* sys, int t) {
+  const BatchSystem<T> S = sys[blockIdx.x];
+  const int64_t j0 = (int64_t)t * PNB;
+  if (j0 >= S.n) return;
+  const int b = (int)min((int64_t)PNB, S.n - j0);
+  diag_nopiv_body(S.A, S.lda, j0, b, S.linv + 2 * (int64_t)t * PNB * PNB, S.status, S.status + 1);
+}
+template <typename T>
+__global__ __launch_bounds__(256) void below_nopiv_batched_kernel(const BatchSystem<T>* sys, int t) {
+  const BatchSystem<T> S = sys[blockIdx.y];
+  const int64_t j0 = (int64_t)t * PNB;
+  if (j0 >= S.n) return;
+  const int b = (int)min((int64_t)PNB, S.n - j0);
+  if ((int64_t)blockIdx.x * PNB >= S.n - j0 - b) return;
+  below_nopiv_body(S.A, S.n, S.lda, j0, b, S.status + 1, (int64_t)blockIdx.x);
 }
 
 }  // namespace
@@ -2156,6 +2365,66 @@ template hipError_t lu_solve_transposed<float>(const float*, int64_t, int64_t, f
                                                hipStream_t);
 template hipError_t lu_solve_transposed<double>(const double*, int64_t, int64_t, double*, double*, const double*,
                                                 hipStream_t);
+
+template <typename T>
+hipError_t lu_factor_batched(const BatchSystem<T>* sys, int nsys, int64_t nmax, int64_t ncmax, bool gauss3,
+                             hipStream_t s) {
+  if (nsys <= 0 || nmax <= 0) return hipSuccess;
+  const int leaves = (int)((nmax + PNB - 1) / PNB);
+  hipError_t e;
+  for (int t = 0; t < leaves; ++t) {
+    const int64_t j0 = (int64_t)t * PNB;
+    const int64_t below = std::max<int64_t>(0, nmax - j0 - PNB);
+    const int64_t cols = std::max<int64_t>(0, ncmax - j0 - 1);
+    hipLaunchKernelGGL((diag_nopiv_batched_kernel<T>), dim3((unsigned)nsys), dim3(256), 0, s, sys, t);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (below > 0) {
+      hipLaunchKernelGGL((below_nopiv_batched_kernel<T>), dim3((unsigned)((below + PNB - 1) / PNB), (unsigned)nsys),
+                         dim3(256), 0, s, sys, t);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (cols <= 0) continue;
+    const dim3 gu((unsigned)((cols + 63) / 64), 1, (unsigned)nsys);
+    const dim3 gs((unsigned)((cols + 63) / 64), (unsigned)std::max<int64_t>(1, (below + 63) / 64), (unsigned)nsys);
+    if (sizeof(T) == 4 && gauss3) {
+      hipLaunchKernelGGL((zgemm_batched_kernel<T, true, true>), gu, dim3(256), 0, s, sys, t);
+      if (below > 0) hipLaunchKernelGGL((zgemm_batched_kernel<T, false, true>), gs, dim3(256), 0, s, sys, t);
+    } else {
+      hipLaunchKernelGGL((zgemm_batched_kernel<T, true, false>), gu, dim3(256), 0, s, sys, t);
+      if (below > 0) hipLaunchKernelGGL((zgemm_batched_kernel<T, false, false>), gs, dim3(256), 0, s, sys, t);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t tri_solve_batched(const BatchSystem<T>* sys, const BatchVec<T>* vec, int nsys, int nrhs, bool fwd, bool bwd,
+                             hipStream_t s) {
+  if (nsys <= 0) return hipSuccess;
+  if (nrhs < 1 || nrhs > 8) return hipErrorInvalidValue;
+  if (fwd && bwd) hipLaunchKernelGGL((tri_wg_kernel<T, true, true>), dim3((unsigned)nsys), dim3(1024), 0, s, sys, vec, nrhs);
+  else if (fwd) hipLaunchKernelGGL((tri_wg_kernel<T, true, false>), dim3((unsigned)nsys), dim3(1024), 0, s, sys, vec, nrhs);
+  else if (bwd) hipLaunchKernelGGL((tri_wg_kernel<T, false, true>), dim3((unsigned)nsys), dim3(1024), 0, s, sys, vec, nrhs);
+  return hipGetLastError();
+}
+
+hipError_t residual64_batched(const BatchSystem<double>* sys, const BatchVec<double>* x, const BatchVec<double>* b,
+                              const BatchVec<double>* r, int nsys, int64_t nmax, int nrhs, hipStream_t s) {
+  if (nsys <= 0) return hipSuccess;
+  if (nrhs > 8) return hipErrorInvalidValue;
+  constexpr int rows = RES_WAVES * RES_ROWS;
+  hipLaunchKernelGGL(residual64_batched_kernel, dim3((unsigned)((nmax + rows - 1) / rows), (unsigned)nsys),
+                     dim3(64 * RES_WAVES), 0, s, sys, x, b, r, nrhs);
+  return hipGetLastError();
+}
+
+template hipError_t lu_factor_batched<float>(const BatchSystem<float>*, int, int64_t, int64_t, bool, hipStream_t);
+template hipError_t lu_factor_batched<double>(const BatchSystem<double>*, int, int64_t, int64_t, bool, hipStream_t);
+template hipError_t tri_solve_batched<float>(const BatchSystem<float>*, const BatchVec<float>*, int, int, bool, bool,
+                                             hipStream_t);
+template hipError_t tri_solve_batched<double>(const BatchSystem<double>*, const BatchVec<double>*, int, int, bool, bool,
+                                              hipStream_t);
 
 template hipError_t lu_solve_prepare<float>(const float*, int64_t, int64_t, const int*, float*, hipStream_t);
 template hipError_t lu_solve_prepare<double>(const double*, int64_t, int64_t, const int*, double*, hipStream_t);

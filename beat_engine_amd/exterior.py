@@ -157,6 +157,7 @@ class Engine:
             out.append(ph[off:off + n].reshape((m.num_vertices, nrhs), order="F"))
             off += n
         info = [dict(assembly_s=r.assembly_s, factor_s=r.factor_s, solve_s=r.solve_s,
+                     factorization_count=r.factorization_count, refinement_steps=r.refine.steps,
                      refinement_status=_native.REFINE_STATUS[r.refine.status]) for r in reps]
         return out, info
 
