@@ -1272,9 +1272,11 @@ int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* mes
   if (int rc = check_solve_options(ctx, precision, opts)) return rc;
   const int max_steps = opts ? opts->max_steps : 3;
   const double rtol = opts ? opts->rtol : 1e-10;
-  hipStream_t s = ctx->stream;
+  hipStream_t s = ctx->stream, sa = ctx->asm_stream;
   const bool refined = precision == BEAT_PREC_FP32_REFINED, fp32 = precision == BEAT_PREC_FP32;
-  // buffers sized once for the largest group of this batch
+  // buffers sized once for the largest group of this batch, two slots: group
+  // g + 1 is assembled in one (assembly stream) while group g's graph runs on
+  // the other (context stream)
   GroupLayout G;
   for (int64_t i = 0; i < njobs; ++i) {
     G.nmax = std::max<int64_t>(G.nmax, meshes[i]->h.nv);
@@ -1286,24 +1288,25 @@ int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* mes
   G.vec = (size_t)G.nmax * nrhs;
   G.qv = (size_t)G.nfmax * nrhs;
   G.linv = (size_t)((G.nmax + 63) / 64) * 64 * 64;
-  const size_t J = (size_t)G.J;
+  const size_t J = (size_t)G.J, SJ = 2 * J;  // systems over both slots
   const bool want64 = !fp32, want32 = fp32 || refined;
   // captured graphs hold these buffers' addresses: any reallocation drops them all
   const void* before[] = {ctx->g_a64.p, ctx->g_a32.p, ctx->g_b64.p, ctx->g_b32.p, ctx->g_q64.p, ctx->g_q32.p,
                           ctx->g_linv.p, ctx->g_vec.p, ctx->g_state.p, ctx->g_desc.p};
-  HIPCHK(ensure(ctx->g_q64, sizeof(double) * 2 * G.qv * J));
-  if (fp32) HIPCHK(ensure(ctx->g_q32, sizeof(float) * 2 * G.qv * J));
-  if (want64) HIPCHK(ensure(ctx->g_a64, sizeof(double) * 2 * G.mat * J));
-  if (want64) HIPCHK(ensure(ctx->g_b64, sizeof(double) * 2 * G.vec * J));
-  if (want32) HIPCHK(ensure(ctx->g_a32, sizeof(float) * 2 * G.mat * J));
-  if (want32) HIPCHK(ensure(ctx->g_b32, sizeof(float) * 2 * G.vec * J));
-  HIPCHK(ensure(ctx->g_linv, sizeof(double) * 2 * G.linv * J));
+  HIPCHK(ensure(ctx->g_q64, sizeof(double) * 2 * G.qv * SJ));
+  if (fp32) HIPCHK(ensure(ctx->g_q32, sizeof(float) * 2 * G.qv * SJ));
+  if (want64) HIPCHK(ensure(ctx->g_a64, sizeof(double) * 2 * G.mat * SJ));
+  if (want64) HIPCHK(ensure(ctx->g_b64, sizeof(double) * 2 * G.vec * SJ));
+  if (want32) HIPCHK(ensure(ctx->g_a32, sizeof(float) * 2 * G.mat * SJ));
+  if (want32) HIPCHK(ensure(ctx->g_b32, sizeof(float) * 2 * G.vec * SJ));
+  HIPCHK(ensure(ctx->g_linv, sizeof(double) * 2 * G.linv * SJ));
   // vectors (complex128 n x nrhs): x, r, nx, nr, out; d (complex64)
-  HIPCHK(ensure(ctx->g_vec, sizeof(double) * 2 * G.vec * J * 5 + sizeof(float) * 2 * G.vec * J + 4096));
-  HIPCHK(ensure(ctx->g_state, sizeof(RefineState) * J + sizeof(int) * 4 * J + sizeof(double) * 16 * J + 4096));
-  const size_t desc_bytes = 16 * J * sizeof(BatchVec<double>) + 2 * J * sizeof(BatchSystem<double>) +
-                            J * sizeof(BatchJobDesc) + 4096;
-  HIPCHK(ensure(ctx->g_desc, desc_bytes));
+  HIPCHK(ensure(ctx->g_vec, sizeof(double) * 2 * G.vec * SJ * 5 + sizeof(float) * 2 * G.vec * SJ + 4096));
+  HIPCHK(ensure(ctx->g_state, sizeof(RefineState) * SJ + sizeof(int) * 4 * SJ + sizeof(double) * 16 * SJ + 4096));
+  // per slot: 2 system arrays, 12 vector arrays, job descriptors
+  const size_t desc_slot = 2 * J * sizeof(BatchSystem<double>) + 12 * J * sizeof(BatchVec<double>) +
+                           J * sizeof(BatchJobDesc) + 256;
+  HIPCHK(ensure(ctx->g_desc, 2 * desc_slot + 256));
   {
     const void* after[] = {ctx->g_a64.p, ctx->g_a32.p, ctx->g_b64.p, ctx->g_b32.p, ctx->g_q64.p, ctx->g_q32.p,
                            ctx->g_linv.p, ctx->g_vec.p, ctx->g_state.p, ctx->g_desc.p};
@@ -1315,115 +1318,156 @@ int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* mes
   }
   double* v64 = as<double>(ctx->g_vec);
   double* vx = v64;
-  double* vr = vx + 2 * G.vec * J;
-  double* vnx = vr + 2 * G.vec * J;
-  double* vnr = vnx + 2 * G.vec * J;
-  double* vout = vnr + 2 * G.vec * J;
-  float* vd = reinterpret_cast<float*>(vout + 2 * G.vec * J);
-  RefineState* state = reinterpret_cast<RefineState*>(ctx->g_state.p);
-  int* lustat = reinterpret_cast<int*>(state + J);
-  double* norms = reinterpret_cast<double*>(lustat + 4 * J);
-  // descriptor arrays on the device: [0] systems (complex128 or complex64 A),
-  // then the vector kinds
-  char* dp = reinterpret_cast<char*>(ctx->g_desc.p);
-  BatchSystem<double>* sys64 = reinterpret_cast<BatchSystem<double>*>(dp);
-  BatchSystem<float>* sys32 = reinterpret_cast<BatchSystem<float>*>(sys64 + J);
-  BatchVec<double>* dv = reinterpret_cast<BatchVec<double>*>(sys32 + J);
-  BatchVec<double>* dX = dv, *dR = dv + J, *dNX = dv + 2 * J, *dNR = dv + 3 * J, *dB = dv + 4 * J, *dOut = dv + 5 * J,
-                   *dRs = dv + 6 * J, *dNXs = dv + 7 * J, *dNRs = dv + 8 * J;
-  BatchVec<float>* dD = reinterpret_cast<BatchVec<float>*>(dv + 9 * J);        // correction (skip-gated)
-  BatchVec<float>* dCar32 = dD + J;                                             // carried columns, complex64
-  BatchVec<double>* dCar64 = reinterpret_cast<BatchVec<double>*>(dCar32 + J);  // carried columns, complex128
-  BatchJobDesc* djobs = reinterpret_cast<BatchJobDesc*>(dCar64 + J);
-  std::vector<beat_solve_report> none;
-  int result = BEAT_OK;
-  for (int64_t g0 = 0; g0 < njobs; g0 += G.J) {
-    const int nj = (int)std::min<int64_t>(G.J, njobs - g0);
-    // host descriptors of this group
-    std::vector<BatchSystem<double>> h64(J);
-    std::vector<BatchSystem<float>> h32(J);
-    std::vector<BatchVec<double>> hv(9 * J);
-    std::vector<BatchVec<float>> hd(2 * J);
-    std::vector<BatchVec<double>> hc64(J);
-    std::vector<BatchJobDesc> hj(J);
+  double* vr = vx + 2 * G.vec * SJ;
+  double* vnx = vr + 2 * G.vec * SJ;
+  double* vnr = vnx + 2 * G.vec * SJ;
+  double* vout = vnr + 2 * G.vec * SJ;
+  float* vd = reinterpret_cast<float*>(vout + 2 * G.vec * SJ);
+  RefineState* state_all = reinterpret_cast<RefineState*>(ctx->g_state.p);
+  int* lustat_all = reinterpret_cast<int*>(state_all + SJ);
+  double* norms_all = reinterpret_cast<double*>(lustat_all + 4 * SJ);
+
+  // one slot's device descriptor arrays
+  struct SlotDesc {
+    BatchSystem<double>* sys64;
+    BatchSystem<float>* sys32;
+    BatchVec<double> *dX, *dR, *dNX, *dNR, *dB, *dOut, *dRs, *dNXs, *dNRs, *dCar64;
+    BatchVec<float> *dD, *dCar32;
+    BatchJobDesc* jobs;
+    RefineState* state;
+    int* lustat;
+    double* norms;
+  };
+  auto slot_desc = [&](int sl) {
+    SlotDesc D;
+    char* dp = reinterpret_cast<char*>(ctx->g_desc.p) + (size_t)sl * desc_slot;
+    D.sys64 = reinterpret_cast<BatchSystem<double>*>(dp);
+    D.sys32 = reinterpret_cast<BatchSystem<float>*>(D.sys64 + J);
+    BatchVec<double>* dv = reinterpret_cast<BatchVec<double>*>(D.sys32 + J);
+    D.dX = dv;
+    D.dR = dv + J;
+    D.dNX = dv + 2 * J;
+    D.dNR = dv + 3 * J;
+    D.dB = dv + 4 * J;
+    D.dOut = dv + 5 * J;
+    D.dRs = dv + 6 * J;
+    D.dNXs = dv + 7 * J;
+    D.dNRs = dv + 8 * J;
+    D.dCar64 = dv + 9 * J;
+    D.dD = reinterpret_cast<BatchVec<float>*>(dv + 10 * J);
+    D.dCar32 = D.dD + J;
+    D.jobs = reinterpret_cast<BatchJobDesc*>(dv + 12 * J);
+    D.state = state_all + (size_t)sl * J;
+    D.lustat = lustat_all + (size_t)sl * 4 * J;
+    D.norms = norms_all + (size_t)sl * 16 * J;
+    return D;
+  };
+  struct HostGroup {
+    int64_t g0 = 0;
+    int nj = 0;
     int64_t gmax = 0, gncmax = 0;
-    // the graph's shape: options, strides and the group's sizes (the buffers stay:
-    // a reallocation above drops every graph)
-    std::string key = std::to_string(precision) + "/" + std::to_string(nrhs) + "/" + std::to_string(max_steps) + "/" +
-                      std::to_string(rtol) + "/" + std::to_string(nj) + "/" + std::to_string(G.lda) + "/" +
-                      std::to_string(G.mat) + "/" + std::to_string(G.vec) + "/" + std::to_string(G.linv) + "/" +
-                      std::to_string(G.J);
-    for (int jj = 0; jj < nj; ++jj) {
+    std::string key;
+    std::vector<BatchSystem<double>> h64;
+    std::vector<BatchSystem<float>> h32;
+    std::vector<BatchVec<double>> hv;
+    std::vector<BatchVec<float>> hd;
+    std::vector<BatchJobDesc> hj;
+  };
+  // descriptors and assembly of group g in slot sl (assembly stream)
+  auto prepare = [&](int64_t g0, int sl, HostGroup& H) -> int {
+    const SlotDesc D = slot_desc(sl);
+    H.g0 = g0;
+    H.nj = (int)std::min<int64_t>(G.J, njobs - g0);
+    H.gmax = H.gncmax = 0;
+    H.h64.assign(J, BatchSystem<double>{});
+    H.h32.assign(J, BatchSystem<float>{});
+    H.hv.assign(10 * J, BatchVec<double>{});
+    H.hd.assign(2 * J, BatchVec<float>{});
+    H.hj.assign(J, BatchJobDesc{});
+    // the graph's shape: options, strides, slot and the group's sizes (the buffers
+    // stay: a reallocation above drops every graph)
+    H.key = std::to_string(precision) + "/" + std::to_string(nrhs) + "/" + std::to_string(max_steps) + "/" +
+            std::to_string(rtol) + "/" + std::to_string(H.nj) + "/" + std::to_string(G.lda) + "/" +
+            std::to_string(G.mat) + "/" + std::to_string(G.vec) + "/" + std::to_string(G.linv) + "/" +
+            std::to_string(G.J) + "/slot" + std::to_string(sl);
+    for (int jj = 0; jj < H.nj; ++jj) {
       const beat_mesh* m = meshes[g0 + jj];
       const int64_t n = m->h.nv;
-      gmax = std::max(gmax, n);
-      gncmax = std::max(gncmax, n + nrhs);
-      key += ":" + std::to_string(n);
-      int* stj = lustat + 4 * jj;
-      h64[jj] = BatchSystem<double>{as<double>(ctx->g_a64) + 2 * G.mat * jj, n, G.lda, n + nrhs,
-                                    as<double>(ctx->g_linv) + 2 * G.linv * jj, stj};
-      h32[jj] = BatchSystem<float>{as<float>(ctx->g_a32) + 2 * G.mat * jj, n, G.lda, n + nrhs,
-                                   as<float>(ctx->g_linv) + 2 * G.linv * jj, stj};
-      const int* skip = &state[jj].skip;
-      auto v = [&](double* base, const int* sk) { return BatchVec<double>{base + 2 * G.vec * jj, n, nrhs, sk}; };
-      hv[jj] = v(vx, nullptr);
-      hv[J + jj] = v(vr, nullptr);
-      hv[2 * J + jj] = v(vnx, skip);
-      hv[3 * J + jj] = v(vnr, skip);
-      hv[4 * J + jj] = BatchVec<double>{as<double>(ctx->g_b64) + 2 * G.vec * jj, n, nrhs, nullptr};
-      hv[5 * J + jj] = v(vout, nullptr);
-      hv[6 * J + jj] = v(vr, skip);
-      hv[7 * J + jj] = v(vnx, skip);
-      hv[8 * J + jj] = v(vnr, skip);
-      hd[jj] = BatchVec<float>{vd + 2 * G.vec * jj, n, nrhs, skip};
-      hd[J + jj] = BatchVec<float>{as<float>(ctx->g_a32) + 2 * (G.mat * jj + n), n, G.lda, nullptr};
-      hc64[jj] = BatchVec<double>{as<double>(ctx->g_a64) + 2 * (G.mat * jj + n), n, G.lda, nullptr};
-      hj[jj] = BatchJobDesc{n, m->vert_inv, p_dev + 2 * p_off[g0 + jj]};
+      const size_t js = (size_t)sl * J + jj;  // system index over both slots
+      H.gmax = std::max(H.gmax, n);
+      H.gncmax = std::max(H.gncmax, n + nrhs);
+      H.key += ":" + std::to_string(n);
+      int* stj = D.lustat + 4 * jj;
+      H.h64[jj] = BatchSystem<double>{as<double>(ctx->g_a64) + 2 * G.mat * js, n, G.lda, n + nrhs,
+                                      as<double>(ctx->g_linv) + 2 * G.linv * js, stj};
+      H.h32[jj] = BatchSystem<float>{as<float>(ctx->g_a32) + 2 * G.mat * js, n, G.lda, n + nrhs,
+                                     as<float>(ctx->g_linv) + 2 * G.linv * js, stj};
+      const int* skip = &D.state[jj].skip;
+      auto v = [&](double* base, const int* sk) { return BatchVec<double>{base + 2 * G.vec * js, n, nrhs, sk}; };
+      H.hv[jj] = v(vx, nullptr);
+      H.hv[J + jj] = v(vr, nullptr);
+      H.hv[2 * J + jj] = v(vnx, skip);
+      H.hv[3 * J + jj] = v(vnr, skip);
+      H.hv[4 * J + jj] = BatchVec<double>{as<double>(ctx->g_b64) + 2 * G.vec * js, n, nrhs, nullptr};
+      H.hv[5 * J + jj] = v(vout, nullptr);
+      H.hv[6 * J + jj] = v(vr, skip);
+      H.hv[7 * J + jj] = v(vnx, skip);
+      H.hv[8 * J + jj] = v(vnr, skip);
+      H.hv[9 * J + jj] = BatchVec<double>{as<double>(ctx->g_a64) + 2 * (G.mat * js + n), n, G.lda, nullptr};
+      H.hd[jj] = BatchVec<float>{vd + 2 * G.vec * js, n, nrhs, skip};
+      H.hd[J + jj] = BatchVec<float>{as<float>(ctx->g_a32) + 2 * (G.mat * js + n), n, G.lda, nullptr};
+      H.hj[jj] = BatchJobDesc{n, m->vert_inv, p_dev + 2 * p_off[g0 + jj]};
     }
-    HIPCHK(hipMemcpyAsync(sys64, h64.data(), sizeof(BatchSystem<double>) * J, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(sys32, h32.data(), sizeof(BatchSystem<float>) * J, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dv, hv.data(), sizeof(BatchVec<double>) * 9 * J, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dD, hd.data(), sizeof(BatchVec<float>) * 2 * J, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(dCar64, hc64.data(), sizeof(BatchVec<double>) * J, hipMemcpyHostToDevice, s));
-    HIPCHK(hipMemcpyAsync(djobs, hj.data(), sizeof(BatchJobDesc) * J, hipMemcpyHostToDevice, s));
-    // 1. assembly of every system of the group (eager: the wavenumber is an argument)
-    HIPCHK(hipEventRecord(ctx->ev[0], s));
-    for (int jj = 0; jj < nj; ++jj) {
+    HIPCHK(hipMemcpyAsync(D.sys64, H.h64.data(), sizeof(BatchSystem<double>) * J, hipMemcpyHostToDevice, sa));
+    HIPCHK(hipMemcpyAsync(D.sys32, H.h32.data(), sizeof(BatchSystem<float>) * J, hipMemcpyHostToDevice, sa));
+    HIPCHK(hipMemcpyAsync(D.dX, H.hv.data(), sizeof(BatchVec<double>) * 10 * J, hipMemcpyHostToDevice, sa));
+    HIPCHK(hipMemcpyAsync(D.dD, H.hd.data(), sizeof(BatchVec<float>) * 2 * J, hipMemcpyHostToDevice, sa));
+    HIPCHK(hipMemcpyAsync(D.jobs, H.hj.data(), sizeof(BatchJobDesc) * J, hipMemcpyHostToDevice, sa));
+    HIPCHK(hipEventRecord(ctx->sw_ev[sl][1], sa));
+    for (int jj = 0; jj < H.nj; ++jj) {
       const int64_t i = g0 + jj;
+      const size_t js = (size_t)sl * J + jj;
       const beat_mesh* m = meshes[i];
       const int64_t n = m->h.nv, nf = m->h.nf;
-      double* q64 = as<double>(ctx->g_q64) + 2 * G.qv * jj;
-      hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, s, q_dev + 2 * q_off[i],
+      double* q64 = as<double>(ctx->g_q64) + 2 * G.qv * js;
+      hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, sa, q_dev + 2 * q_off[i],
                          nf, nrhs, (const int32_t*)m->face_perm, q64);
       HIPCHK(hipGetLastError());
       if (fp32) {
-        float* q32 = as<float>(ctx->g_q32) + 2 * G.qv * jj;
-        float* A = h32[jj].A;
-        float* B = as<float>(ctx->g_b32) + 2 * G.vec * jj;
-        HIPCHK(launch_downcast(q64, q32, 2 * nf * nrhs, s));
-        if (int rc = assemble<float>(ctx, m, ks[i], ers[i], eis[i], q64, q32, nrhs, A, G.lda, B, s, nullptr, nullptr))
+        float* q32 = as<float>(ctx->g_q32) + 2 * G.qv * js;
+        float* A = H.h32[jj].A;
+        float* B = as<float>(ctx->g_b32) + 2 * G.vec * js;
+        HIPCHK(launch_downcast(q64, q32, 2 * nf * nrhs, sa));
+        if (int rc = assemble<float>(ctx, m, ks[i], ers[i], eis[i], q64, q32, nrhs, A, G.lda, B, sa, nullptr, nullptr))
           return rc;
-        HIPCHK(lu_append_rhs<float>(A, n, G.lda, B, nrhs, s));
+        HIPCHK(lu_append_rhs<float>(A, n, G.lda, B, nrhs, sa));
       } else {
-        double* A = h64[jj].A;
-        double* B = as<double>(ctx->g_b64) + 2 * G.vec * jj;
-        if (int rc = assemble<double>(ctx, m, ks[i], ers[i], eis[i], q64, q64, nrhs, A, G.lda, B, s, nullptr, nullptr))
+        double* A = H.h64[jj].A;
+        double* B = as<double>(ctx->g_b64) + 2 * G.vec * js;
+        if (int rc = assemble<double>(ctx, m, ks[i], ers[i], eis[i], q64, q64, nrhs, A, G.lda, B, sa, nullptr, nullptr))
           return rc;
         if (refined) {
-          float* A32 = h32[jj].A;
-          float* B32 = as<float>(ctx->g_b32) + 2 * G.vec * jj;
-          HIPCHK(launch_downcast(A, A32, 2 * n * G.lda, s));
-          HIPCHK(launch_downcast(B, B32, 2 * n * nrhs, s));
-          HIPCHK(lu_append_rhs<float>(A32, n, G.lda, B32, nrhs, s));
+          float* A32 = H.h32[jj].A;
+          float* B32 = as<float>(ctx->g_b32) + 2 * G.vec * js;
+          HIPCHK(launch_downcast(A, A32, 2 * n * G.lda, sa));
+          HIPCHK(launch_downcast(B, B32, 2 * n * nrhs, sa));
+          HIPCHK(lu_append_rhs<float>(A32, n, G.lda, B32, nrhs, sa));
         } else {
-          HIPCHK(lu_append_rhs<double>(A, n, G.lda, B, nrhs, s));
+          HIPCHK(lu_append_rhs<double>(A, n, G.lda, B, nrhs, sa));
         }
       }
     }
-    HIPCHK(hipEventRecord(ctx->ev[1], s));
-    // 2. factorization, solves and refinement of the whole group: one graph
+    HIPCHK(hipEventRecord(ctx->sw_ev[sl][2], sa));
+    return BEAT_OK;
+  };
+  // the factor / solve / refine graph of group H in slot sl, then the scatter (context stream)
+  auto launch = [&](int sl, const HostGroup& H) -> int {
+    const SlotDesc D = slot_desc(sl);
+    const int nj = H.nj;
+    HIPCHK(hipStreamWaitEvent(s, ctx->sw_ev[sl][2], 0));
+    HIPCHK(hipEventRecord(ctx->sw_ev[sl][0], s));
     hipGraphExec_t exec = nullptr;
-    auto it = ctx->g_graphs.find(key);
+    auto it = ctx->g_graphs.find(H.key);
     if (it != ctx->g_graphs.end()) {
       exec = it->second;
     } else {
@@ -1433,76 +1477,86 @@ int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* mes
       auto ok = [&](hipError_t e) {
         if (ce == hipSuccess) ce = e;
       };
-      ok(launch_fill<int>(lustat, 4 * (int64_t)nj, 0, s));
+      const int64_t gmax = H.gmax, gncmax = H.gncmax;
+      ok(launch_fill<int>(D.lustat, 4 * (int64_t)nj, 0, s));
       const unsigned tb = (unsigned)((nj + 63) / 64);
       const dim3 ge((unsigned)((gmax * nrhs + 255) / 256), (unsigned)nj);
-      if (want32 && !fp32) {
-        // refined: complex64 speculative LU (three-product updates), first solution
-        ok(lu_factor_batched<float>(sys32, nj, gmax, gncmax, true, s));
-        ok(tri_solve_batched<float>(sys32, dCar32, nj, nrhs, false, true, s));
-        hipLaunchKernelGGL((batch_elementwise_kernel<0>), ge, dim3(256), 0, s, dX, dCar32, nullptr, nullptr, nullptr,
-                           nullptr, nullptr, nrhs);
+      if (refined) {
+        // complex64 speculative LU (three-product updates), first solution
+        ok(lu_factor_batched<float>(D.sys32, nj, gmax, gncmax, true, s));
+        ok(tri_solve_batched<float>(D.sys32, D.dCar32, nj, nrhs, false, true, s));
+        hipLaunchKernelGGL((batch_elementwise_kernel<0>), ge, dim3(256), 0, s, D.dX, D.dCar32, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nrhs);
         // complex128 refinement with the device-side decisions
-        hipLaunchKernelGGL(colnorms_batched_kernel, dim3((unsigned)nj), dim3(256), 0, s, dB, nrhs, norms, 0);
-        ok(residual64_batched(sys64, dX, dB, dR, nj, gmax, nrhs, s));
-        hipLaunchKernelGGL(colnorms_batched_kernel, dim3((unsigned)nj), dim3(256), 0, s, dR, nrhs, norms, 8);
-        hipLaunchKernelGGL(refine_control_kernel, dim3(tb), dim3(64), 0, s, state, norms, nj, nrhs, rtol, 0, 0);
+        hipLaunchKernelGGL(colnorms_batched_kernel, dim3((unsigned)nj), dim3(256), 0, s, D.dB, nrhs, D.norms, 0);
+        ok(residual64_batched(D.sys64, D.dX, D.dB, D.dR, nj, gmax, nrhs, s));
+        hipLaunchKernelGGL(colnorms_batched_kernel, dim3((unsigned)nj), dim3(256), 0, s, D.dR, nrhs, D.norms, 8);
+        hipLaunchKernelGGL(refine_control_kernel, dim3(tb), dim3(64), 0, s, D.state, D.norms, nj, nrhs, rtol, 0, 0);
         for (int step = 1; step <= max_steps; ++step) {
-          hipLaunchKernelGGL((batch_elementwise_kernel<1>), ge, dim3(256), 0, s, dRs, dD, nullptr, nullptr, nullptr,
-                             nullptr, state, nrhs);
-          ok(tri_solve_batched<float>(sys32, dD, nj, nrhs, true, true, s));
-          hipLaunchKernelGGL((batch_elementwise_kernel<2>), ge, dim3(256), 0, s, dNXs, dD, dX, nullptr, nullptr,
-                             nullptr, state, nrhs);
-          ok(residual64_batched(sys64, dNXs, dB, dNRs, nj, gmax, nrhs, s));
-          hipLaunchKernelGGL(colnorms_batched_kernel, dim3((unsigned)nj), dim3(256), 0, s, dNRs, nrhs, norms, 8);
-          hipLaunchKernelGGL(refine_control_kernel, dim3(tb), dim3(64), 0, s, state, norms, nj, nrhs, rtol, 1, step);
-          hipLaunchKernelGGL((batch_elementwise_kernel<3>), ge, dim3(256), 0, s, dX, nullptr, dNX, dR, dNR, nullptr,
-                             state, nrhs);
-          hipLaunchKernelGGL(refine_advance_kernel, dim3(tb), dim3(64), 0, s, state, nj);
+          hipLaunchKernelGGL((batch_elementwise_kernel<1>), ge, dim3(256), 0, s, D.dRs, D.dD, nullptr, nullptr,
+                             nullptr, nullptr, D.state, nrhs);
+          ok(tri_solve_batched<float>(D.sys32, D.dD, nj, nrhs, true, true, s));
+          hipLaunchKernelGGL((batch_elementwise_kernel<2>), ge, dim3(256), 0, s, D.dNXs, D.dD, D.dX, nullptr, nullptr,
+                             nullptr, D.state, nrhs);
+          ok(residual64_batched(D.sys64, D.dNXs, D.dB, D.dNRs, nj, gmax, nrhs, s));
+          hipLaunchKernelGGL(colnorms_batched_kernel, dim3((unsigned)nj), dim3(256), 0, s, D.dNRs, nrhs, D.norms, 8);
+          hipLaunchKernelGGL(refine_control_kernel, dim3(tb), dim3(64), 0, s, D.state, D.norms, nj, nrhs, rtol, 1,
+                             step);
+          hipLaunchKernelGGL((batch_elementwise_kernel<3>), ge, dim3(256), 0, s, D.dX, nullptr, D.dNX, D.dR, D.dNR,
+                             nullptr, D.state, nrhs);
+          hipLaunchKernelGGL(refine_advance_kernel, dim3(tb), dim3(64), 0, s, D.state, nj);
         }
-        hipLaunchKernelGGL(refine_control_kernel, dim3(tb), dim3(64), 0, s, state, norms, nj, nrhs, rtol, 2, 0);
-        hipLaunchKernelGGL((batch_elementwise_kernel<4>), ge, dim3(256), 0, s, dOut, nullptr, nullptr, nullptr,
-                           nullptr, dX, nullptr, nrhs);
+        hipLaunchKernelGGL(refine_control_kernel, dim3(tb), dim3(64), 0, s, D.state, D.norms, nj, nrhs, rtol, 2, 0);
+        hipLaunchKernelGGL((batch_elementwise_kernel<4>), ge, dim3(256), 0, s, D.dOut, nullptr, nullptr, nullptr,
+                           nullptr, D.dX, nullptr, nrhs);
       } else if (fp32) {
-        ok(lu_factor_batched<float>(sys32, nj, gmax, gncmax, false, s));
-        ok(tri_solve_batched<float>(sys32, dCar32, nj, nrhs, false, true, s));
-        hipLaunchKernelGGL((batch_elementwise_kernel<0>), ge, dim3(256), 0, s, dOut, dCar32, nullptr, nullptr, nullptr,
-                           nullptr, nullptr, nrhs);
+        ok(lu_factor_batched<float>(D.sys32, nj, gmax, gncmax, false, s));
+        ok(tri_solve_batched<float>(D.sys32, D.dCar32, nj, nrhs, false, true, s));
+        hipLaunchKernelGGL((batch_elementwise_kernel<0>), ge, dim3(256), 0, s, D.dOut, D.dCar32, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nrhs);
       } else {
-        ok(lu_factor_batched<double>(sys64, nj, gmax, gncmax, false, s));
-        ok(tri_solve_batched<double>(sys64, dCar64, nj, nrhs, false, true, s));
-        hipLaunchKernelGGL((batch_elementwise_kernel<4>), ge, dim3(256), 0, s, dOut, nullptr, nullptr, nullptr,
-                           nullptr, dCar64, nullptr, nrhs);
+        ok(lu_factor_batched<double>(D.sys64, nj, gmax, gncmax, false, s));
+        ok(tri_solve_batched<double>(D.sys64, D.dCar64, nj, nrhs, false, true, s));
+        hipLaunchKernelGGL((batch_elementwise_kernel<4>), ge, dim3(256), 0, s, D.dOut, nullptr, nullptr, nullptr,
+                           nullptr, D.dCar64, nullptr, nrhs);
       }
       ok(hipGetLastError());
       hipError_t ee = hipStreamEndCapture(s, &graph);
       if (ce != hipSuccess || ee != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
-        return fail(ctx, BEAT_ERR_HIP, std::string("batch graph capture: ") +
-                                           hipGetErrorString(ce != hipSuccess ? ce : ee));
+        return fail(ctx, BEAT_ERR_HIP,
+                    std::string("batch graph capture: ") + hipGetErrorString(ce != hipSuccess ? ce : ee));
       }
       hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
-      if (ie != hipSuccess) return fail(ctx, BEAT_ERR_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(ie));
-      ctx->g_graphs[key] = exec;
+      if (ie != hipSuccess)
+        return fail(ctx, BEAT_ERR_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(ie));
+      ctx->g_graphs[H.key] = exec;
     }
     HIPCHK(hipGraphLaunch(exec, s));
-    HIPCHK(hipEventRecord(ctx->ev[2], s));
-    // 3. the pressures to the caller's blocks
-    hipLaunchKernelGGL(batch_scatter_kernel, dim3((unsigned)((gmax * nrhs + 255) / 256), (unsigned)nj), dim3(256), 0, s,
-                       (const BatchVec<double>*)dOut, (const BatchJobDesc*)djobs, nrhs);
+    // the pressures to the caller's blocks
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3((unsigned)((H.gmax * nrhs + 255) / 256), (unsigned)nj), dim3(256), 0,
+                       s, (const BatchVec<double>*)D.dOut, (const BatchJobDesc*)D.jobs, nrhs);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ctx->ev[3], s));
+    HIPCHK(hipEventRecord(ctx->sw_ev[sl][3], s));
+    HIPCHK(hipEventRecord(ctx->sw_free[sl], s));
+    return BEAT_OK;
+  };
+  int result = BEAT_OK;
+  // reports and the per-job re-solve of exchange-flagged systems of group H
+  auto finish = [&](int sl, const HostGroup& H) -> int {
+    const SlotDesc D = slot_desc(sl);
+    const int nj = H.nj;
+    HIPCHK(hipEventSynchronize(ctx->sw_ev[sl][3]));
     std::vector<RefineState> hs(J);
     std::vector<int> hl(4 * J);
-    HIPCHK(hipMemcpyAsync(hs.data(), state, sizeof(RefineState) * nj, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(hl.data(), lustat, sizeof(int) * 4 * nj, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpy(hs.data(), D.state, sizeof(RefineState) * nj, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hl.data(), D.lustat, sizeof(int) * 4 * nj, hipMemcpyDeviceToHost));
     float ta = 0, tf = 0;
-    HIPCHK(hipEventElapsedTime(&ta, ctx->ev[0], ctx->ev[1]));
-    HIPCHK(hipEventElapsedTime(&tf, ctx->ev[1], ctx->ev[3]));
+    HIPCHK(hipEventElapsedTime(&ta, ctx->sw_ev[sl][1], ctx->sw_ev[sl][2]));
+    HIPCHK(hipEventElapsedTime(&tf, ctx->sw_ev[sl][0], ctx->sw_ev[sl][3]));
     for (int jj = 0; jj < nj; ++jj) {
-      const int64_t i = g0 + jj;
+      const int64_t i = H.g0 + jj;
       const bool exchange = hl[4 * jj + 3] != 0 || ctx->debug_force_exchange;
       if (exchange) {
         // partial pivoting would exchange rows here: this job again on its own
@@ -1524,9 +1578,9 @@ int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* mes
       if (!reps) continue;
       beat_solve_report& R = reps[i];
       const beat_mesh* m = meshes[i];
-      R.assembly_s = ta * 1e-3 / nj;
+      R.assembly_s = ta * 1e-3 / nj;  // the group's assembly, shared evenly
       R.regular_kernel_s = 0;
-      R.factor_s = tf * 1e-3 / nj;  // the group's graph, shared evenly
+      R.factor_s = tf * 1e-3 / nj;  // the group's graph and scatter, shared evenly
       R.solve_s = 0;
       R.factorization_count = 1;
       R.lu_panels = (int)((m->h.nv + 63) / 64);
@@ -1554,7 +1608,35 @@ int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* mes
         R.refine.rtol = rtol;
       }
     }
+    return BEAT_OK;
+  };
+  const int64_t ngroups = (njobs + G.J - 1) / G.J;
+  HostGroup hg[2];
+  // the assembly stream starts after everything earlier on the context stream
+  HIPCHK(hipEventRecord(ctx->sw_free[1], s));
+  HIPCHK(hipStreamWaitEvent(sa, ctx->sw_free[1], 0));
+  if (int rc = prepare(0, 0, hg[0])) return rc;
+  for (int64_t g = 0; g < ngroups; ++g) {
+    const int sl = (int)(g & 1);
+    if (g + 1 < ngroups) {
+      // slot 1 - sl is free once group g - 1's graph and scatter are done
+      if (g >= 1) HIPCHK(hipStreamWaitEvent(sa, ctx->sw_free[1 - sl], 0));
+      if (int rc = prepare((g + 1) * G.J, 1 - sl, hg[1 - sl])) {
+        (void)hipStreamSynchronize(sa);
+        return rc;
+      }
+    }
+    if (int rc = launch(sl, hg[sl])) {
+      (void)hipStreamSynchronize(sa);
+      return rc;
+    }
+    if (int rc = finish(sl, hg[sl])) {
+      (void)hipStreamSynchronize(sa);
+      return rc;
+    }
   }
+  HIPCHK(hipStreamSynchronize(sa));
+  HIPCHK(hipStreamSynchronize(s));
   return result;
 }
 
