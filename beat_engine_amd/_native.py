@@ -29,7 +29,7 @@ EXPORTS = (
     "beat_field_gradient_eval", "beat_operators_assemble", "beat_bm_sweep_device",
     "beat_bm_sweep", "beat_ctx_set_cancel_path", "beat_ctx_reclaim", "beat_ctx_memory_info", "beat_capabilities",
     "beat_adjoint_spl_gradient", "beat_bm_batch", "beat_autotune", "beat_ctx_debug_abort_waits",
-    "beat_ctx_debug_force_exchange", "beat_field_eval_device_noalloc",
+    "beat_ctx_debug_force_exchange", "beat_ctx_regular_symmetric", "beat_field_eval_device_noalloc",
 )
 
 SYMMETRY_CODES = {"off": 0, "x": 1, "xy": 2, "ground": 3}
@@ -121,6 +121,7 @@ def load(path: str = LIB_PATH):
     L.beat_ctx_reclaim.argtypes = [P]
     L.beat_ctx_debug_abort_waits.argtypes = [P, I]
     L.beat_ctx_debug_force_exchange.argtypes = [P, I]
+    L.beat_ctx_regular_symmetric.argtypes = [P, I]
     L.beat_ctx_memory_info.argtypes = [P, P]
     L.beat_capabilities.argtypes = [P, ctypes.c_char_p, I64]
     L.beat_capabilities.restype = I64

@@ -86,6 +86,9 @@ struct beat_ctx {
   // (measured at L5, 8 RHS: 12.9 vs 10.7 freq-solves/s; one per CU the assembly
   // takes 90 ms per frequency instead of 70, the LU 64.8 ms instead of 69.7)
   bool sweep_asm_one_per_cu = false;
+  // the regular pass over unordered tile pairs (both orders of a face pair from
+  // one evaluation of the kernel); false: the ordered-pair kernel
+  bool regular_symmetric = true;
   // contexts a batch may use at once (BEAT_BATCH_WORKERS, default 4)
   int batch_workers = 4;
   // batches of small systems through the grouped, graph-replayed path
@@ -134,6 +137,10 @@ struct beat_mesh {
   uint8_t* flocs = nullptr;
   int32_t *ntile_start = nullptr, *ntile_count = nullptr, *ntile_nv = nullptr, *ntile_verts = nullptr;
   uint8_t* flocn = nullptr;
+  int32_t *qtile_start = nullptr, *qtile_count = nullptr, *qtile_nv = nullptr, *qtile_verts = nullptr;
+  uint8_t* flocq = nullptr;
+  int32_t* tile_qfirst = nullptr;
+  int64_t* sym_unit_start = nullptr;
   SingularPair* spairs = nullptr;
   int32_t *rule_off = nullptr, *rule_cnt = nullptr;
   double* rule_data = nullptr;
@@ -599,6 +606,16 @@ void regular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, Regular
   ra.tile_verts = m->tile_verts;
   ra.ntiles = (int)m->h.tile_start.size();
   set_trial_tiles(m, false, ra);
+  ra.qtile_start = m->qtile_start;
+  ra.qtile_count = m->qtile_count;
+  ra.qtile_nv = m->qtile_nv;
+  ra.qtile_verts = m->qtile_verts;
+  ra.flocq = m->flocq;
+  ra.tile_qfirst = m->tile_qfirst;
+  ra.unit_start = m->sym_unit_start;
+  ra.nqtiles = (int)m->h.qtile_start.size();
+  ra.nunits = m->h.sym_unit_start.empty() ? 0 : m->h.sym_unit_start.back();
+  ra.symmetric = ctx->regular_symmetric ? 1 : 0;
   ra.k = (T)k;
   ra.krev = (T)(k / (2.0 * M_PI));
   ra.ksteps = phase_steps_per_unit(k);
@@ -610,6 +627,9 @@ void regular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, Regular
     ra.phi[i][0] = (T)(1.0 - xi - eta);
     ra.phi[i][1] = (T)xi;
     ra.phi[i][2] = (T)eta;
+    ra.pw[i][0] = (T)((1.0 - xi - eta) * (on ? m->h.rule_w[i] : 0.0));
+    ra.pw[i][1] = (T)(xi * (on ? m->h.rule_w[i] : 0.0));
+    ra.pw[i][2] = (T)(eta * (on ? m->h.rule_w[i] : 0.0));
   }
   ra.nimg = m->h.nimg;
   for (int x = 0; x < MAX_IMAGES; ++x)
@@ -1938,6 +1958,7 @@ int beat_ctx_create(int device, beat_ctx** out) {
     return BEAT_ERR_HIP;
   }
   if (const char* v = std::getenv("BEAT_SWEEP_ASM_ONE_PER_CU")) c->sweep_asm_one_per_cu = v[0] != '0';
+  if (const char* v = std::getenv("BEAT_REGULAR_SYMMETRIC")) c->regular_symmetric = v[0] != '0';
   if (const char* v = std::getenv("BEAT_BATCH_WORKERS")) c->batch_workers = std::max(1, std::atoi(v));
   if (const char* v = std::getenv("BEAT_BATCH_GROUPED")) c->batch_grouped = v[0] != '0';
   // BEAT_LU_LOOKAHEAD=0 runs the whole factorization in order on one stream
@@ -2010,6 +2031,13 @@ int beat_ctx_debug_force_exchange(beat_ctx* ctx, int on) {
   if (!ctx) return BEAT_ERR_ARG;
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->debug_force_exchange = on != 0;
+  return BEAT_OK;
+}
+
+int beat_ctx_regular_symmetric(beat_ctx* ctx, int on) {
+  if (!ctx) return BEAT_ERR_ARG;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  ctx->regular_symmetric = on != 0;
   return BEAT_OK;
 }
 
@@ -2130,6 +2158,13 @@ int beat_mesh_upload_symmetric(beat_ctx* ctx, const double* verts, int64_t nv, c
   if (e == hipSuccess) e = upload(&m->ntile_nv, m->h.ntile_nv);
   if (e == hipSuccess) e = upload(&m->ntile_verts, m->h.ntile_verts);
   if (e == hipSuccess) e = upload(&m->flocn, m->h.flocn);
+  if (e == hipSuccess) e = upload(&m->qtile_start, m->h.qtile_start);
+  if (e == hipSuccess) e = upload(&m->qtile_count, m->h.qtile_count);
+  if (e == hipSuccess) e = upload(&m->qtile_nv, m->h.qtile_nv);
+  if (e == hipSuccess) e = upload(&m->qtile_verts, m->h.qtile_verts);
+  if (e == hipSuccess) e = upload(&m->flocq, m->h.flocq);
+  if (e == hipSuccess) e = upload(&m->tile_qfirst, m->h.tile_qfirst);
+  if (e == hipSuccess) e = upload(&m->sym_unit_start, m->h.sym_unit_start);
   if (e == hipSuccess) e = upload(&m->spairs, m->h.spairs);
   if (e == hipSuccess) e = upload(&m->rule_off, m->h.rule_off);
   if (e == hipSuccess) e = upload(&m->rule_cnt, m->h.rule_cnt);
@@ -2180,7 +2215,8 @@ void beat_mesh_destroy(beat_mesh* m) {
   if (!m) return;
   void* ptrs[] = {m->rec64, m->rec32, m->verts, m->fverts, m->floc, m->tile_start, m->tile_count, m->tile_nv,
                   m->tile_verts, m->stile_start, m->stile_count, m->stile_nv, m->stile_verts, m->flocs,
-                  m->ntile_start, m->ntile_count, m->ntile_nv, m->ntile_verts, m->flocn, m->spairs,
+                  m->ntile_start, m->ntile_count, m->ntile_nv, m->ntile_verts, m->flocn, m->qtile_start,
+                  m->qtile_count, m->qtile_nv, m->qtile_verts, m->flocq, m->tile_qfirst, m->sym_unit_start, m->spairs,
                   m->rule_off, m->rule_cnt, m->rule_data, m->rule3, m->rule4, m->face_perm, m->vert_inv, m->wrows,
                   m->wvals};
   if (m->ctx) (void)hipSetDevice(m->ctx->device);
@@ -2609,6 +2645,7 @@ int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, 
     a->cancel_path = ctx->cancel_path;
     a->debug_abort_waits = ctx->debug_abort_waits;
     a->debug_force_exchange = ctx->debug_force_exchange;
+    a->regular_symmetric = ctx->regular_symmetric;
     a->sweep_asm_one_per_cu = ctx->sweep_asm_one_per_cu;
     std::memcpy(a->lu_layout, ctx->lu_layout, sizeof(ctx->lu_layout));
     std::memcpy(a->asm_occ, ctx->asm_occ, sizeof(ctx->asm_occ));

@@ -303,6 +303,359 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
   }
 }
 
+// ---- symmetric regular pass ----------------------------------------------------
+// The regular pairs over unordered tile pairs. A unit is (test tile t, sub-tile
+// s) with s from t's first sub-tile on; sub-tiles nest in the test tiles (at most
+// TILE_FS_N faces / TILE_VS_N vertices). Lane = test face T of tile t; waves stride
+// over the faces S of sub-tile s. When s lies in a later test tile, one evaluation
+// of r, G = e^{ikr} / r and G' = G (-1/r + i k) per point pair serves both orders:
+// (T test, S trial) into fA / fC and (S test, T trial) into rA / rC. With r = y - x,
+// x on T and y on S, the two normal factors swap roles between the orders:
+//   a = (d_S - x.n_S) / r = (r.n_S) / r   forward (r.n_y)/r, reverse -(r'.n_x')/r'
+//   b = (d_T - y.n_T) / r = -(r.n_T) / r  forward -(r.n_x)/r, reverse (r'.n_y')/r'
+// (r' = x - y is the reverse pair's y' - x'). When s lies in t itself, both faces
+// of a pair are test faces of the unit: the pair is taken once, from its lower face.
+// Basis sums: on each side the first basis slot accumulates the sum over the three
+// functions (phi_0 + phi_1 + phi_2 = 1), and phi_0's entry is the difference,
+// formed once per face pair.
+// LDS pointers keep their address space through the call, so that the atomics on
+// them are known not to touch the face records (which then stay scalar loads)
+template <typename T>
+using lds_ptr = __attribute__((address_space(3))) T*;
+
+template <typename T, int NQ, bool PURE, bool WIDE>
+__device__ __forceinline__ void sym_unit(const RegularArgs<T> a, int tt, int st, bool diag, lds_ptr<T> fA,
+                                         lds_ptr<T> fC, lds_ptr<T> rA, lds_ptr<T> rC, lds_ptr<const T> xs,
+                                         lds_ptr<const T> sphw, const double* trig) {
+  constexpr int LF = 2 * TILE_VS_N + 1, LR = 2 * TILE_V + 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int t0 = a.tile_start[tt], nT = a.tile_count[tt];
+  const int s0 = a.qtile_start[st], nS = a.qtile_count[st];
+  const T* __restrict__ rec = a.rec;
+  const int32_t* __restrict__ fverts = a.fverts;
+  const uint8_t* __restrict__ flocq = a.flocq;
+  const bool have_test = lane < nT;
+  const int tf = t0 + (have_test ? lane : 0);
+  const T* tr = rec + (int64_t)tf * FREC;
+  const T nT0 = tr[REC_N], nT1 = tr[REC_N + 1], nT2 = tr[REC_N + 2];
+  const T dT = tr[REC_D], areaT = tr[REC_AREA];
+  const int tv0 = fverts[(int64_t)tf * 3], tv1 = fverts[(int64_t)tf * 3 + 1], tv2 = fverts[(int64_t)tf * 3 + 2];
+  const int tl[3] = {a.floc[(int64_t)tf * 3], a.floc[(int64_t)tf * 3 + 1], a.floc[(int64_t)tf * 3 + 2]};
+  const T k = a.k, krev = a.krev;
+  const T er = a.eta_re, ei = a.eta_im;
+  const int wu = __builtin_amdgcn_readfirstlane(wave);
+
+  for (int sj = wu; sj < nS; sj += 4) {
+    const int64_t sf = __builtin_amdgcn_readfirstlane(s0 + sj);
+    const T* sr = rec + sf * FREC;
+    T y[NQ][3];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) y[j][c] = sr[3 * j + c];
+    const T nS0 = sr[REC_N], nS1 = sr[REC_N + 1], nS2 = sr[REC_N + 2];
+    const T dS = sr[REC_D], areaS = sr[REC_AREA];
+    const int sv0 = fverts[sf * 3], sv1 = fverts[sf * 3 + 1], sv2 = fverts[sf * 3 + 2];
+    const bool adj = (tv0 == sv0) | (tv0 == sv1) | (tv0 == sv2) | (tv1 == sv0) | (tv1 == sv1) | (tv1 == sv2) |
+                     (tv2 == sv0) | (tv2 == sv1) | (tv2 == sv2);
+    if (!have_test || adj || (diag && tf >= sf)) continue;
+    const T cw = T(4) * areaT * areaS * T(INV_FOUR_PI);
+    const T kk2n = k * k * (nT0 * nS0 + nT1 * nS1 + nT2 * nS2);
+    const T knr = er * kk2n, kni = ei * kk2n;
+    // QW[j] = d_T - y_j.n_T: b times r
+    T QW[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) QW[j] = dT - (y[j][0] * nT0 + y[j][1] * nT1 + y[j][2] * nT2);
+    // forward [test basis][trial basis] and right-hand side [test basis]; reverse
+    // [trial-face basis][test-face basis] and [trial-face basis]; slot 0 = sum
+    T Lfr[3][3], Lfi[3][3], Rfr[3], Rfi[3];
+    T Lrr[3][3], Lri[3][3], Rrr[3], Rri[3];
+    T csr = 0, csi = 0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      Rfr[p] = Rfi[p] = Rrr[p] = Rri[p] = 0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) Lfr[p][b] = Lfi[p][b] = Lrr[p][b] = Lri[p][b] = 0;
+    }
+#pragma unroll 1
+    for (int i = 0; i < NQ; ++i) {
+      const T x0 = xs[(3 * i) * 64 + lane], x1 = xs[(3 * i + 1) * 64 + lane], x2 = xs[(3 * i + 2) * 64 + lane];
+      const T PS = dS - (x0 * nS0 + x1 * nS1 + x2 * nS2);  // a times r
+      const T cwi = cw * sphw[8 * i + 3];
+      // forward: M = sum_j phi_b(y_j) w_j lhs, U = sum_j w_j b eta-free G' factor,
+      // Gs = sum_j w_j G; reverse: Mr = sum_j phi_p(y_j) w_j lhs', Fr = the same
+      // sums of rhs'
+      T Mfr[3] = {0, 0, 0}, Mfi[3] = {0, 0, 0}, Ufr = 0, Ufi = 0, Gsr = 0, Gsi = 0;
+      T Mrr[3] = {0, 0, 0}, Mri[3] = {0, 0, 0}, Frr[3] = {0, 0, 0}, Fri[3] = {0, 0, 0};
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        const T rx = y[j][0] - x0, ry = y[j][1] - x1, rz = y[j][2] - x2;
+        const T r2 = rx * rx + ry * ry + rz * rz;
+        const T ir = inv_sqrt(r2);
+        const T r = r2 * ir;
+        T s, c;
+        phase_sincos<WIDE>(r, krev, a.ksteps, k, trig, s, c);
+        const T Gr = c * ir, Gi = s * ir;
+        const T Dr = -ir * Gr - k * Gi, Di = -ir * Gi + k * Gr;  // G'
+        const T at = PS * ir, bt = QW[j] * ir;
+        // rule constants: compile-time, so they live in no register across the loop
+        const double wjd = TriRule<NQ>::w[j], p1d = wjd * TriRule<NQ>::xi[j], p2d = wjd * TriRule<NQ>::eta[j];
+        const T wj = (T)wjd, p1 = (T)p1d, p2 = (T)p2d;
+        // eta k^2 (n_T.n_S) G
+        T Kr, Ki;
+        if (PURE) {
+          Kr = -kni * Gi;
+          Ki = kni * Gr;
+        } else {
+          Kr = knr * Gr - kni * Gi;
+          Ki = knr * Gi + kni * Gr;
+        }
+        const T lfr = -at * Dr - Kr, lfi = -at * Di - Ki;  // forward lhs
+        T hr, hi;                                          // reverse rhs: -G - eta G' a
+        const T ub = wj * bt;
+        if (PURE) {
+          // eta G' = i ei G': the forward sum keeps (Di, Dr); ei and the signs follow
+          Ufr += ub * Di;
+          Ufi += ub * Dr;
+          const T ea = ei * at;
+          hr = ea * Di - Gr;
+          hi = -ea * Dr - Gi;
+        } else {
+          const T EDr = er * Dr - ei * Di, EDi = er * Di + ei * Dr;
+          Ufr += ub * EDr;
+          Ufi += ub * EDi;
+          hr = -Gr - at * EDr;
+          hi = -Gi - at * EDi;
+        }
+        Gsr += wj * Gr;
+        Gsi += wj * Gi;
+        Mfr[0] += wj * lfr;
+        Mfi[0] += wj * lfi;
+        Mfr[1] += p1 * lfr;
+        Mfi[1] += p1 * lfi;
+        Mfr[2] += p2 * lfr;
+        Mfi[2] += p2 * lfi;
+        const T lrr = -bt * Dr - Kr, lri = -bt * Di - Ki;  // reverse lhs
+        Mrr[0] += wj * lrr;
+        Mri[0] += wj * lri;
+        Mrr[1] += p1 * lrr;
+        Mri[1] += p1 * lri;
+        Mrr[2] += p2 * lrr;
+        Mri[2] += p2 * lri;
+        Frr[0] += wj * hr;
+        Fri[0] += wj * hi;
+        Frr[1] += p1 * hr;
+        Fri[1] += p1 * hi;
+        Frr[2] += p2 * hr;
+        Fri[2] += p2 * hi;
+      }
+      // forward rhs sum: -Gs - sum_j w_j b eta G'
+      T Sr, Si;
+      if (PURE) {
+        Sr = ei * Ufr - Gsr;
+        Si = -ei * Ufi - Gsi;
+      } else {
+        Sr = -Gsr - Ufr;
+        Si = -Gsi - Ufi;
+      }
+      csr += cwi * Gsr;
+      csi += cwi * Gsi;
+      const T ph[3] = {cwi, cwi * sphw[8 * i + 1], cwi * sphw[8 * i + 2]};
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        Rfr[p] += ph[p] * Sr;
+        Rfi[p] += ph[p] * Si;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          Lfr[p][b] += ph[p] * Mfr[b];
+          Lfi[p][b] += ph[p] * Mfi[b];
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        Rrr[p] += cwi * Frr[p];
+        Rri[p] += cwi * Fri[p];
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          Lrr[p][b] += ph[b] * Mrr[p];
+          Lri[p][b] += ph[b] * Mri[p];
+        }
+      }
+    }
+    // slot 0 held the basis sums: phi_0's entries are the differences
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      Lfr[p][0] -= Lfr[p][1] + Lfr[p][2];
+      Lfi[p][0] -= Lfi[p][1] + Lfi[p][2];
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      Lfr[0][b] -= Lfr[1][b] + Lfr[2][b];
+      Lfi[0][b] -= Lfi[1][b] + Lfi[2][b];
+    }
+    Rfr[0] -= Rfr[1] + Rfr[2];
+    Rfi[0] -= Rfi[1] + Rfi[2];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      Lrr[p][0] -= Lrr[p][1] + Lrr[p][2];
+      Lri[p][0] -= Lri[p][1] + Lri[p][2];
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      Lrr[0][b] -= Lrr[1][b] + Lrr[2][b];
+      Lri[0][b] -= Lri[1][b] + Lri[2][b];
+    }
+    Rrr[0] -= Rrr[1] + Rrr[2];
+    Rri[0] -= Rri[1] + Rri[2];
+    // hypersingular curl term: (curl_a . curl_b) * eta * sum(G w), the same for both orders
+    const T ecr = er * csr - ei * csi, eci = er * csi + ei * csr;
+    const int sl[3] = {flocq[sf * 3], flocq[sf * 3 + 1], flocq[sf * 3 + 2]};
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+      const T c0 = tr[REC_CURL + 3 * p], c1 = tr[REC_CURL + 3 * p + 1], c2 = tr[REC_CURL + 3 * p + 2];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const T b0 = sr[REC_CURL + 3 * b], b1 = sr[REC_CURL + 3 * b + 1], b2 = sr[REC_CURL + 3 * b + 2];
+        const T cc = c0 * b0 + c1 * b1 + c2 * b2;
+        lds_ptr<T> dst = fA + tl[p] * LF + 2 * sl[b];
+        atomicAdd((T*)dst, Lfr[p][b] + cc * ecr);
+        atomicAdd((T*)(dst + 1), Lfi[p][b] + cc * eci);
+        lds_ptr<T> dr = rA + sl[b] * LR + 2 * tl[p];
+        atomicAdd((T*)dr, Lrr[b][p] + cc * ecr);
+        atomicAdd((T*)(dr + 1), Lri[b][p] + cc * eci);
+      }
+      lds_ptr<T> dc = fC + tl[p] * LF + 2 * sj;
+      atomicAdd((T*)dc, Rfr[p]);
+      atomicAdd((T*)(dc + 1), Rfi[p]);
+      lds_ptr<T> drc = rC + sl[p] * LR + 2 * lane;
+      atomicAdd((T*)drc, Rrr[p]);
+      atomicAdd((T*)(drc + 1), Rri[p]);
+    }
+  }
+}
+
+// One workgroup per unit of the symmetric pass (4 waves); the units of one test
+// tile are consecutive, each XCD takes a contiguous range of them, and odd test
+// tiles walk their sub-tiles backwards (the trial records the previous tile read
+// last are still in L2). Flushes as in bm_regular_kernel, the reverse blocks
+// transposed: rows of sub-tile vertices, columns of test-tile vertices / faces.
+template <typename T, int NQ, bool PURE, bool WIDE>
+__global__ __launch_bounds__(256, 2) void bm_regular_sym_kernel(const RegularArgs<T> a) {
+  constexpr int LF = 2 * TILE_VS_N + 1, LR = 2 * TILE_V + 1;
+  __shared__ T fA[TILE_V * LF];
+  __shared__ T fC[TILE_V * LF];
+  __shared__ T rA[TILE_VS_N * LR];
+  __shared__ T rC[TILE_VS_N * LR];
+  __shared__ T xs[NQ * 3 * 64];
+  __shared__ T sphw[6 * 8];
+  __shared__ double trig[2 * TRIG_N];
+  __shared__ int grow[TILE_V], gcol[TILE_VS_N];
+
+  const int64_t total = a.nunits;
+  const int64_t per = (total + 7) / 8;
+  const int64_t bid = blockIdx.x;
+  const int64_t L = (bid & 7) * per + (bid >> 3);
+  if (L >= total) return;
+  // the test tile: the last one whose first unit is at most L
+  int lo = 0, hi = a.ntiles;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (a.unit_start[mid] <= L) lo = mid;
+    else hi = mid;
+  }
+  // (wave-uniform: the trial records below are then scalar loads)
+  const int tt = __builtin_amdgcn_readfirstlane(lo);
+  const int q0 = a.tile_qfirst[tt], q1 = a.tile_qfirst[tt + 1];
+  const int off = (int)(L - a.unit_start[tt]);
+  const int st = __builtin_amdgcn_readfirstlane((tt & 1) ? a.nqtiles - 1 - off : q0 + off);
+  const bool diag = st < q1;
+  const int tid = threadIdx.x;
+  const int t0 = a.tile_start[tt], nT = a.tile_count[tt], nvT = a.tile_nv[tt];
+  const int s0 = a.qtile_start[st], nS = a.qtile_count[st], nvS = a.qtile_nv[st];
+
+  for (int i = tid; i < TILE_V * LF; i += blockDim.x) fA[i] = fC[i] = T(0);
+  for (int i = tid; i < TILE_VS_N * LR; i += blockDim.x) rA[i] = rC[i] = T(0);
+  if (tid < TILE_V) grow[tid] = a.tile_verts[(int64_t)tt * TILE_V + tid];
+  if (tid < TILE_VS_N) gcol[tid] = a.qtile_verts[(int64_t)st * TILE_VS_N + tid];
+  for (int i = tid; i < 2 * TRIG_N; i += blockDim.x) trig[i] = a.trig[i];
+  if (tid == 0) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        sphw[8 * i + b] = a.phi[i][b];
+        sphw[8 * i + 4 + b] = a.phi[i][b] * a.w[i];
+      }
+      sphw[8 * i + 3] = a.w[i];
+      sphw[8 * i + 7] = T(0);
+    }
+  }
+  {
+    const int lane = tid & 63;
+    const int tf = t0 + (lane < nT ? lane : 0);
+    if ((tid >> 6) == 0)
+      for (int i = 0; i < NQ * 3; ++i) xs[i * 64 + lane] = a.rec[(int64_t)tf * FREC + i];
+  }
+  __syncthreads();
+  const lds_ptr<T> lfA = (lds_ptr<T>)fA, lfC = (lds_ptr<T>)fC, lrA = (lds_ptr<T>)rA, lrC = (lds_ptr<T>)rC;
+  const lds_ptr<const T> lxs = (lds_ptr<const T>)xs, lsp = (lds_ptr<const T>)sphw;
+  sym_unit<T, NQ, PURE, WIDE>(a, tt, st, diag, lfA, lfC, lrA, lrC, lxs, lsp, trig);
+  __syncthreads();
+
+  // forward P1 x P1 block: rows of the test tile, columns of the sub-tile
+  for (int idx = tid; idx < nvT * nvS; idx += blockDim.x) {
+    const int r = idx / nvS, c = idx - r * nvS;
+    const T vr = fA[r * LF + 2 * c], vi = fA[r * LF + 2 * c + 1];
+    if (vr != T(0) || vi != T(0)) {
+      T* dst = a.A + 2 * ((int64_t)grow[r] * a.lda + gcol[c]);
+      gadd(dst, vr);
+      gadd(dst + 1, vi);
+    }
+  }
+  // forward right-hand sides: B[row, d] += sum_s fC[row, s] q[s, d]
+  for (int idx = tid; idx < nvT * a.nrhs; idx += blockDim.x) {
+    const int r = idx / a.nrhs, d = idx - r * a.nrhs;
+    T br = 0, bi = 0;
+    for (int s = 0; s < nS; ++s) {
+      const T cr = fC[r * LF + 2 * s], ci = fC[r * LF + 2 * s + 1];
+      const T* qq = a.q + 2 * ((int64_t)(s0 + s) * a.nrhs + d);
+      br += cr * qq[0] - ci * qq[1];
+      bi += cr * qq[1] + ci * qq[0];
+    }
+    if (br != T(0) || bi != T(0)) {
+      T* dst = a.B + 2 * ((int64_t)grow[r] * a.nrhs + d);
+      gadd(dst, br);
+      gadd(dst + 1, bi);
+    }
+  }
+  // reverse P1 x P1 block: rows of the sub-tile, columns of the test tile
+  for (int idx = tid; idx < nvS * nvT; idx += blockDim.x) {
+    const int r = idx / nvT, c = idx - r * nvT;
+    const T vr = rA[r * LR + 2 * c], vi = rA[r * LR + 2 * c + 1];
+    if (vr != T(0) || vi != T(0)) {
+      T* dst = a.A + 2 * ((int64_t)gcol[r] * a.lda + grow[c]);
+      gadd(dst, vr);
+      gadd(dst + 1, vi);
+    }
+  }
+  // reverse right-hand sides: B[sub-tile row, d] += sum_t rC[row, t] q[t, d]
+  for (int idx = tid; idx < nvS * a.nrhs; idx += blockDim.x) {
+    const int r = idx / a.nrhs, d = idx - r * a.nrhs;
+    T br = 0, bi = 0;
+    for (int t = 0; t < nT; ++t) {
+      const T cr = rC[r * LR + 2 * t], ci = rC[r * LR + 2 * t + 1];
+      const T* qq = a.q + 2 * ((int64_t)(t0 + t) * a.nrhs + d);
+      br += cr * qq[0] - ci * qq[1];
+      bi += cr * qq[1] + ci * qq[0];
+    }
+    if (br != T(0) || bi != T(0)) {
+      T* dst = a.B + 2 * ((int64_t)gcol[r] * a.nrhs + d);
+      gadd(dst, br);
+      gadd(dst + 1, bi);
+    }
+  }
+}
+
 // Touching pairs over the remapped Duffy rules, FP64 arithmetic; one
 // wavefront per pair, lanes stride over the rule points.
 template <typename TO>
@@ -847,8 +1200,44 @@ static void launch_regular_range(const RegularArgs<T>& a, int nq, bool pure, boo
   launch_regular_nq<T, false, IMAGE>(a, nq, pure, grid, block, s);
 }
 
+template <typename T, int NQ, bool WIDE>
+static void launch_regular_sym_q(const RegularArgs<T>& a, bool pure, dim3 grid, hipStream_t s) {
+  if (pure) {
+    const auto k = bm_regular_sym_kernel<T, NQ, true, WIDE>;
+    hipLaunchKernelGGL(k, grid, dim3(256), a.one_per_cu ? one_per_cu_pad(k) : 0u, s, a);
+  } else {
+    const auto k = bm_regular_sym_kernel<T, NQ, false, WIDE>;
+    hipLaunchKernelGGL(k, grid, dim3(256), a.one_per_cu ? one_per_cu_pad(k) : 0u, s, a);
+  }
+}
+
+template <typename T, bool WIDE>
+static void launch_regular_sym_nq(const RegularArgs<T>& a, int nq, bool pure, dim3 grid, hipStream_t s) {
+  if (nq == 6) launch_regular_sym_q<T, 6, WIDE>(a, pure, grid, s);
+  else if (nq == 3) launch_regular_sym_q<T, 3, WIDE>(a, pure, grid, s);
+  else launch_regular_sym_q<T, 1, WIDE>(a, pure, grid, s);
+}
+
+// the symmetric pass: one workgroup per (test tile, sub-tile) unit
+template <typename T>
+static hipError_t launch_regular_sym(const RegularArgs<T>& a, int nq, bool pure, bool wide, hipStream_t s) {
+  if (!a.unit_start || !a.tile_qfirst || a.nqtiles < 1 || a.nunits < 1) return hipErrorInvalidValue;
+  const int64_t blocks = (a.nunits + 7) / 8 * 8;
+  if (blocks > 0x7fffffff) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)blocks);
+  if constexpr (sizeof(T) == 8) {
+    if (wide) {
+      launch_regular_sym_nq<T, true>(a, nq, pure, grid, s);
+      return hipGetLastError();
+    }
+  }
+  launch_regular_sym_nq<T, false>(a, nq, pure, grid, s);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_regular(const RegularArgs<T>& a, int nq, bool pure, bool wide, bool image, hipStream_t s) {
+  if (!image && a.symmetric) return launch_regular_sym<T>(a, nq, pure, wide, s);
   if (image && a.narrow) return hipErrorInvalidValue;
   const int64_t total = (int64_t)a.ntiles * a.nstiles;
   const int64_t per = (total + 7) / 8;

@@ -65,6 +65,34 @@ inline int image_transforms(int mode, ImageXf* out) {
   return 0;
 }
 
+// The regular triangle rules on the reference triangle (area 1/2): per point the
+// (xi, eta) coordinates and the weight. One point (order 1), three points (orders
+// 2 and 3) and the six-point table of order 4, as BeatEngineCore.jl tabulates them.
+// Shared by the host set-up and, as compile-time constants, the regular kernels.
+template <int NQ>
+struct TriRule;
+template <>
+struct TriRule<1> {
+  static constexpr double xi[1] = {1.0 / 3.0};
+  static constexpr double eta[1] = {1.0 / 3.0};
+  static constexpr double w[1] = {0.5};
+};
+template <>
+struct TriRule<3> {
+  static constexpr double xi[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0};
+  static constexpr double eta[3] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
+  static constexpr double w[3] = {1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0};
+};
+template <>
+struct TriRule<6> {
+  static constexpr double xi[6] = {0.4459484909159651, 0.0915762135097710, 0.1081030181680700,
+                                   0.4459484909159651, 0.8168475729804590, 0.0915762135097710};
+  static constexpr double eta[6] = {0.4459484909159651, 0.0915762135097700, 0.4459484909159651,
+                                    0.1081030181680700, 0.0915762135097700, 0.8168475729804580};
+  static constexpr double w[6] = {0.5 * 0.2233815896780110, 0.5 * 0.1099517436553220, 0.5 * 0.2233815896780110,
+                                  0.5 * 0.2233815896780110, 0.5 * 0.1099517436553220, 0.5 * 0.1099517436553220};
+};
+
 // Host-side mesh after preprocessing; all arrays in internal (tile) order
 // except where noted.
 struct HostMesh {
@@ -87,6 +115,16 @@ struct HostMesh {
   std::vector<int32_t> ntile_start, ntile_count, ntile_nv;
   std::vector<int32_t> ntile_verts;  // TILE_VS_N per narrow trial tile
   std::vector<uint8_t> flocn;        // 3 per internal face: vertex slot in its narrow trial tile
+  // sub-tiles of the symmetric regular pass: each test tile cut greedily into
+  // runs of at most TILE_FS_N faces / TILE_VS_N vertices, so that sub-tiles nest
+  // in test tiles; tile_qfirst[t] is test tile t's first sub-tile (ntiles + 1
+  // entries); sym_unit_start[t] is the first (test tile, sub-tile) unit of test
+  // tile t, which pairs it with every sub-tile from its own first one on
+  std::vector<int32_t> qtile_start, qtile_count, qtile_nv;
+  std::vector<int32_t> qtile_verts;  // TILE_VS_N per sub-tile
+  std::vector<uint8_t> flocq;        // 3 per internal face: vertex slot in its sub-tile
+  std::vector<int32_t> tile_qfirst;
+  std::vector<int64_t> sym_unit_start;
   std::vector<SingularPair> spairs;
   std::vector<int32_t> rule_off, rule_cnt, rule_kind;
   std::vector<double> rule_data;  // 5 per rule point: test xi, eta, trial xi, eta, weight

@@ -61,8 +61,8 @@ BEAT_HD void sincos_reduced(double x, double* s, double* c) {
 constexpr double SINCOS_TABLE_LIMIT = 4.0e5;
 constexpr int TRIG_N = 128;  // table entries (sin, cos) over one period
 
-// sin y, cos y for |y| <= pi/128, rotated by the table entry m.
-BEAT_HD void sincos_rotate(double y, int m, const double* trig, double* s, double* c) {
+// sin y, cos y for |y| <= pi/128, rotated by the table entry (S, C).
+BEAT_HD void sincos_rotate(double y, double S, double C, double* s, double* c) {
   const double y2 = y * y;
   double ps = -0.0001984126984126984;        // -1/7!
   ps = fma(ps, y2, 0.008333333333333333);    // 1/5!
@@ -72,7 +72,6 @@ BEAT_HD void sincos_rotate(double y, int m, const double* trig, double* s, doubl
   pc = fma(pc, y2, 0.041666666666666664);    // 1/4!
   pc = fma(pc, y2, -0.5);
   const double cy = fma(pc, y2, 1.0);
-  const double S = trig[2 * m], C = trig[2 * m + 1];
   *s = fma(S, cy, C * sy);
   *c = fma(C, cy, -(S * sy));
 }
@@ -82,7 +81,8 @@ BEAT_HD void sincos_table(double x, const double* trig, double* s, double* c) {
   double y = fma(-n, 1.5707963276654482 / 32, x);
   y = fma(-n, -8.705515692000731e-10 / 32, y);
   y = fma(-n, -3.50343439808993e-19 / 32, y);
-  sincos_rotate(y, (int)n & (TRIG_N - 1), trig, s, c);
+  const int m = (int)n & (TRIG_N - 1);
+  sincos_rotate(y, trig[2 * m], trig[2 * m + 1], s, c);
 }
 
 // The table form on a phase given in table steps, t = x 64/pi with |t| below
@@ -94,7 +94,8 @@ BEAT_HD void sincos_table(double x, const double* trig, double* s, double* c) {
 BEAT_HD void sincos_steps(double t, const double* trig, double* s, double* c) {
   const double n = rint(t);
   const double y = (t - n) * 0.04908738521234052;  // pi/64
-  sincos_rotate(y, (int)n & (TRIG_N - 1), trig, s, c);
+  const int m = (int)n & (TRIG_N - 1);
+  sincos_rotate(y, trig[2 * m], trig[2 * m + 1], s, c);
 }
 
 // Wavenumber scale for sincos_steps: k 64/pi, rounded once (host code).

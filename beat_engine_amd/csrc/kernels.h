@@ -25,6 +25,19 @@ struct RegularArgs {
   const int32_t* stile_verts;  // TILE_VS (narrow: TILE_VS_N) per trial tile
   int nstiles;
   int narrow;              // the trial tile arrays are the narrow partition
+  // symmetric pass: sub-tiles nested in the test tiles (TILE_FS_N faces /
+  // TILE_VS_N vertices), each test tile's first sub-tile (ntiles + 1 entries) and
+  // first unit (ntiles + 1 entries, the last one the unit count)
+  const int32_t* qtile_start;
+  const int32_t* qtile_count;
+  const int32_t* qtile_nv;
+  const int32_t* qtile_verts;
+  const uint8_t* flocq;
+  const int32_t* tile_qfirst;
+  const int64_t* unit_start;
+  int nqtiles;
+  int64_t nunits;
+  int symmetric;           // nonzero: the non-image pass runs the symmetric kernel
   T* A;                    // row-major complex, lda
   int64_t lda;
   T* B;                    // N x nrhs row-major complex
@@ -35,6 +48,7 @@ struct RegularArgs {
   T eta_re, eta_im;
   T w[6];                  // rule weights
   T phi[6][3];             // P1 basis at the rule points
+  T pw[6][3];              // phi[i][b] w[i]
   const double* trig;      // 2 TRIG_N doubles: (sin, cos)(m pi/64)
   // image passes: nimg transforms, per transform the point signs (s0, s1, s2)
   // followed by the curl signs det * (s0, s1, s2)

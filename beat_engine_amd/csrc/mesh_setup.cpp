@@ -28,13 +28,11 @@ struct P2 {
 
 // 6-point degree-4 rule with the reference's truncated literals (weights x 0.5).
 void rule6(double* p, double* w) {
-  const double pts[12] = {0.4459484909159651, 0.4459484909159651, 0.0915762135097710, 0.0915762135097700,
-                          0.1081030181680700, 0.4459484909159651, 0.4459484909159651, 0.1081030181680700,
-                          0.8168475729804590, 0.0915762135097700, 0.0915762135097710, 0.8168475729804580};
-  const double ww[6] = {0.2233815896780110, 0.1099517436553220, 0.2233815896780110,
-                        0.2233815896780110, 0.1099517436553220, 0.1099517436553220};
-  for (int i = 0; i < 12; ++i) p[i] = pts[i];
-  for (int i = 0; i < 6; ++i) w[i] = 0.5 * ww[i];
+  for (int i = 0; i < 6; ++i) {
+    p[2 * i] = TriRule<6>::xi[i];
+    p[2 * i + 1] = TriRule<6>::eta[i];
+    w[i] = TriRule<6>::w[i];
+  }
 }
 
 void gauss01(int order, std::vector<double>& x, std::vector<double>& w) {
@@ -138,17 +136,20 @@ inline uint32_t spread10(uint32_t v) {
 
 int triangle_rule(int order, double* pts, double* w) {
   if (order <= 1) {
-    pts[0] = pts[1] = 1.0 / 3.0;
-    w[0] = 0.5;
+    pts[0] = TriRule<1>::xi[0];
+    pts[1] = TriRule<1>::eta[0];
+    w[0] = TriRule<1>::w[0];
     return 1;
   }
   if (order == 4) {
     rule6(pts, w);
     return 6;
   }
-  const double p[6] = {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0};
-  for (int i = 0; i < 6; ++i) pts[i] = p[i];
-  for (int i = 0; i < 3; ++i) w[i] = 1.0 / 6.0;
+  for (int i = 0; i < 3; ++i) {
+    pts[2 * i] = TriRule<3>::xi[i];
+    pts[2 * i + 1] = TriRule<3>::eta[i];
+    w[i] = TriRule<3>::w[i];
+  }
   return 3;
 }
 
@@ -231,18 +232,15 @@ std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64
   // --- tiles: test tiles (TILE_F faces / TILE_V vertices) and trial tiles
   // (TILE_FS / TILE_VS, and the narrow TILE_FS_N / TILE_VS_N), all greedy along
   // the face order
-  auto tiles = [&](int maxf, int maxv, std::vector<int32_t>& st, std::vector<int32_t>& cnt,
-                   std::vector<int32_t>& tnv, std::vector<int32_t>& tverts, std::vector<uint8_t>& loc) {
-    st.clear();
-    cnt.clear();
-    tnv.clear();
-    tverts.clear();
-    loc.assign(3 * nf, 0);
-    int64_t i = 0;
-    while (i < nf) {
+  // faces [f0, f1) are cut into tiles appended to the arrays
+  auto tiles_in = [&](int64_t f0, int64_t f1, int maxf, int maxv, std::vector<int32_t>& st,
+                      std::vector<int32_t>& cnt, std::vector<int32_t>& tnv, std::vector<int32_t>& tverts,
+                      std::vector<uint8_t>& loc) {
+    int64_t i = f0;
+    while (i < f1) {
       int start = (int)i, count = 0;
       std::vector<int32_t> tv;
-      while (i < nf && count < maxf) {
+      while (i < f1 && count < maxf) {
         int add = 0;
         for (int k = 0; k < 3; ++k) {
           int v = m.fverts[3 * i + k];
@@ -271,9 +269,37 @@ std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64
       tverts.insert(tverts.end(), tv.begin(), tv.end());
     }
   };
+  auto tiles = [&](int maxf, int maxv, std::vector<int32_t>& st, std::vector<int32_t>& cnt,
+                   std::vector<int32_t>& tnv, std::vector<int32_t>& tverts, std::vector<uint8_t>& loc) {
+    st.clear();
+    cnt.clear();
+    tnv.clear();
+    tverts.clear();
+    loc.assign(3 * nf, 0);
+    tiles_in(0, nf, maxf, maxv, st, cnt, tnv, tverts, loc);
+  };
   tiles(TILE_F, TILE_V, m.tile_start, m.tile_count, m.tile_nv, m.tile_verts, m.floc);
   tiles(TILE_FS, TILE_VS, m.stile_start, m.stile_count, m.stile_nv, m.stile_verts, m.flocs);
   tiles(TILE_FS_N, TILE_VS_N, m.ntile_start, m.ntile_count, m.ntile_nv, m.ntile_verts, m.flocn);
+  // sub-tiles nested in the test tiles, and the unit ranges of the symmetric pass
+  {
+    const size_t nt = m.tile_start.size();
+    m.qtile_start.clear();
+    m.qtile_count.clear();
+    m.qtile_nv.clear();
+    m.qtile_verts.clear();
+    m.flocq.assign(3 * nf, 0);
+    m.tile_qfirst.assign(nt + 1, 0);
+    for (size_t t = 0; t < nt; ++t) {
+      m.tile_qfirst[t] = (int32_t)m.qtile_start.size();
+      tiles_in(m.tile_start[t], (int64_t)m.tile_start[t] + m.tile_count[t], TILE_FS_N, TILE_VS_N, m.qtile_start,
+               m.qtile_count, m.qtile_nv, m.qtile_verts, m.flocq);
+    }
+    const int32_t nq = (int32_t)m.qtile_start.size();
+    m.tile_qfirst[nt] = nq;
+    m.sym_unit_start.assign(nt + 1, 0);
+    for (size_t t = 0; t < nt; ++t) m.sym_unit_start[t + 1] = m.sym_unit_start[t] + (nq - m.tile_qfirst[t]);
+  }
 
   // --- face records (internal order)
   m.rec.assign((size_t)FREC * nf, 0.0);

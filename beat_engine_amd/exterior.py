@@ -97,6 +97,11 @@ class Engine:
         partial pivoting exchanges rows, so the solve factors again with exchanges."""
         self._check(self._lib.beat_ctx_debug_force_exchange(self._h, 1 if on else 0), "beat_ctx_debug_force_exchange")
 
+    def regular_symmetric(self, on: bool):
+        """The regular assembly pass over unordered tile pairs (on, the default) or
+        over ordered tile pairs (off); both give the same operators to rounding."""
+        self._check(self._lib.beat_ctx_regular_symmetric(self._h, 1 if on else 0), "beat_ctx_regular_symmetric")
+
     def reclaim(self):
         """Frees the context's workspaces (reclaim_accelerator_memory!)."""
         self._check(self._lib.beat_ctx_reclaim(self._h), "beat_ctx_reclaim")

@@ -117,6 +117,12 @@ int beat_ctx_debug_abort_waits(beat_ctx* ctx, int on);
  * takes its second factorization with the exchange panels from the matrix as
  * assembled (factorization_count 2). For tests of that path only. */
 int beat_ctx_debug_force_exchange(beat_ctx* ctx, int on);
+/* Tuning and test switch (no counterpart in the reference): on (the default, or
+ * BEAT_REGULAR_SYMMETRIC unset / nonzero at context creation), the regular pass
+ * of the fused assembly runs over unordered tile pairs, evaluating the kernel
+ * once per point pair for both orders of a face pair; off, over ordered tile
+ * pairs. The results agree to rounding; image passes always run ordered. */
+int beat_ctx_regular_symmetric(beat_ctx* ctx, int on);
 int beat_ctx_reclaim(beat_ctx* ctx);
 int beat_ctx_memory_info(beat_ctx* ctx, int64_t* info);
 int64_t beat_capabilities(const beat_ctx* ctx, char* buf, int64_t len);
