@@ -643,7 +643,10 @@ void regular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, Regular
 
 // Pairs, rules, wavenumber and reflection table of the singular pass.
 template <typename TO>
-void singular_geometry(const beat_mesh* m, double k, SingularArgs<TO>& sa) {
+void singular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, SingularArgs<TO>& sa) {
+  sa.trig = ctx->trig;
+  sa.ksteps = phase_steps_per_unit(k);
+  sa.wide = !(std::fabs(k) * m->diameter < SINCOS_TABLE_LIMIT) ? 1 : 0;
   sa.rec = m->rec64;
   sa.verts = m->verts;
   sa.fverts = m->fverts;
@@ -691,7 +694,7 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   if (m->h.nimg > 0) HIPCHK(launch_regular<T>(ra, m->h.nq, er == 0.0, wide, true, st));
   if (ev_reg1) HIPCHK(hipEventRecord(ev_reg1, st));
   SingularArgs<T> sa;
-  singular_geometry<T>(m, k, sa);
+  singular_geometry<T>(ctx, m, k, sa);
   sa.A = A;
   sa.lda = lda;
   sa.B = B;
@@ -749,7 +752,7 @@ int assemble_operators(beat_ctx* ctx, const beat_mesh* m, double k, T* S, T* D, 
   HIPCHK(launch_operators<T>(oa, m->h.nq, wide, false, s));
   if (m->h.nimg > 0) HIPCHK(launch_operators<T>(oa, m->h.nq, wide, true, s));
   SingularOperatorArgs<T> so;
-  singular_geometry<T>(m, k, so.g);
+  singular_geometry<T>(ctx, m, k, so.g);
   so.g.A = so.g.B = nullptr;
   so.g.q = nullptr;
   so.g.lda = 0;

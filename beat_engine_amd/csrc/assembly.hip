@@ -59,6 +59,47 @@ __device__ __forceinline__ void phase_sincos(double r, double krev, double kstep
   else sincos_steps(r * ksteps, trig, &s, &c);
 }
 
+// The phase split in two, for software pipelining: phase_begin forms what needs
+// no table value and issues the table read; phase_end finishes sin and cos.
+template <typename T>
+struct PhaseStage {
+  T ir, y, S, C;  // 1/r; the table path: the remainder in steps and the table entry
+};
+template <bool WIDE>
+__device__ __forceinline__ void phase_begin(float r, float krev, double ksteps, float kk, const double* trig,
+                                            PhaseStage<float>& o) {
+  phase_sincos<WIDE>(r, krev, ksteps, kk, trig, o.S, o.C);
+}
+template <bool WIDE>
+__device__ __forceinline__ void phase_end(const PhaseStage<float>& o, float& s, float& c) {
+  s = o.S;
+  c = o.C;
+}
+template <bool WIDE>
+__device__ __forceinline__ void phase_begin(double r, double krev, double ksteps, double kk, const double* trig,
+                                            PhaseStage<double>& o) {
+  if (WIDE) {
+    phase_sincos<WIDE>(r, krev, ksteps, kk, trig, o.S, o.C);
+  } else {
+    // sincos_steps, split: t = r k 64/pi, n = rint(t), the remainder u = t - n
+    const double t = r * ksteps;
+    const double n = rint(t);
+    o.y = t - n;
+    const int m = (int)n & (TRIG_N - 1);
+    o.S = trig[2 * m];
+    o.C = trig[2 * m + 1];
+  }
+}
+template <bool WIDE>
+__device__ __forceinline__ void phase_end(const PhaseStage<double>& o, double& s, double& c) {
+  if (WIDE) {
+    s = o.S;
+    c = o.C;
+  } else {
+    sincos_rotate_steps(o.y, o.S, o.C, &s, &c);
+  }
+}
+
 __device__ __forceinline__ float inv_sqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 // 1/sqrt(x): the hardware estimate (relative error 5.2e-8 measured on gfx950 over
 // r^2 in [1e-8, 1e2]) and one Newton step (measured 4.1e-15), far inside the
@@ -377,9 +418,16 @@ __device__ __forceinline__ void sym_unit(const RegularArgs<T> a, int tt, int st,
 #pragma unroll
       for (int b = 0; b < 3; ++b) Lfr[p][b] = Lfi[p][b] = Lrr[p][b] = Lri[p][b] = 0;
     }
+    // the next test point's coordinates are read one point ahead
+    T nx0 = xs[lane], nx1 = xs[64 + lane], nx2 = xs[128 + lane];
 #pragma unroll 1
     for (int i = 0; i < NQ; ++i) {
-      const T x0 = xs[(3 * i) * 64 + lane], x1 = xs[(3 * i + 1) * 64 + lane], x2 = xs[(3 * i + 2) * 64 + lane];
+      const T x0 = nx0, x1 = nx1, x2 = nx2;
+      if (i + 1 < NQ) {
+        nx0 = xs[(3 * i + 3) * 64 + lane];
+        nx1 = xs[(3 * i + 4) * 64 + lane];
+        nx2 = xs[(3 * i + 5) * 64 + lane];
+      }
       const T PS = dS - (x0 * nS0 + x1 * nS1 + x2 * nS2);  // a times r
       const T cwi = cw * sphw[8 * i + 3];
       // forward: M = sum_j phi_b(y_j) w_j lhs, U = sum_j w_j b eta-free G' factor,
@@ -387,14 +435,23 @@ __device__ __forceinline__ void sym_unit(const RegularArgs<T> a, int tt, int st,
       // sums of rhs'
       T Mfr[3] = {0, 0, 0}, Mfi[3] = {0, 0, 0}, Ufr = 0, Ufi = 0, Gsr = 0, Gsi = 0;
       T Mrr[3] = {0, 0, 0}, Mri[3] = {0, 0, 0}, Frr[3] = {0, 0, 0}, Fri[3] = {0, 0, 0};
-#pragma unroll
-      for (int j = 0; j < NQ; ++j) {
+      // per point pair, first the distance and the phase table entry, one trial
+      // point ahead of the rest (the table read is in flight meanwhile)
+      auto stage = [&](int j, PhaseStage<T>& o) {
         const T rx = y[j][0] - x0, ry = y[j][1] - x1, rz = y[j][2] - x2;
         const T r2 = rx * rx + ry * ry + rz * rz;
-        const T ir = inv_sqrt(r2);
-        const T r = r2 * ir;
+        o.ir = inv_sqrt(r2);
+        phase_begin<WIDE>(r2 * o.ir, krev, a.ksteps, k, trig, o);
+      };
+      PhaseStage<T> cur, nxt;
+      stage(0, cur);
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) {
+        if (j + 1 < NQ) stage(j + 1, nxt);
+        const T ir = cur.ir;
         T s, c;
-        phase_sincos<WIDE>(r, krev, a.ksteps, k, trig, s, c);
+        phase_end<WIDE>(cur, s, c);
+        cur = nxt;
         const T Gr = c * ir, Gi = s * ir;
         const T Dr = -ir * Gr - k * Gi, Di = -ir * Gi + k * Gr;  // G'
         const T at = PS * ir, bt = QW[j] * ir;
@@ -656,13 +713,39 @@ __global__ __launch_bounds__(256, 2) void bm_regular_sym_kernel(const RegularArg
   }
 }
 
-// Touching pairs over the remapped Duffy rules, FP64 arithmetic; one
-// wavefront per pair, lanes stride over the rule points.
-template <typename TO>
+// Sum over the 16 lanes of a DPP row (every lane receives the total): rotations
+// by 8, 4, 2 and 1 within the row.
+template <int N>
+__device__ __forceinline__ double row_ror(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x120 + N, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x120 + N, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double row_sum16(double v) {
+  v += row_ror<8>(v);
+  v += row_ror<4>(v);
+  v += row_ror<2>(v);
+  v += row_ror<1>(v);
+  return v;
+}
+
+// Touching pairs over the remapped Duffy rules, FP64 arithmetic. A group of 16
+// lanes (one DPP row) per pair, four pairs per wavefront; the host orders the
+// pairs by rule size, so a wavefront's pairs mostly share their point count. The
+// lanes of a group stride over the rule points; 1/r is the hardware reciprocal
+// square root with one Newton step and the phase the pi/64 table form (the pi/2
+// form when the host has not bounded |k| r below the table's limit).
+template <typename TO, bool WIDE>
 __global__ __launch_bounds__(256) void bm_singular_kernel(const SingularArgs<TO> a) {
-  const int lane = threadIdx.x & 63;
-  const int64_t pidx = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (pidx >= a.npairs) return;
+  __shared__ double trig[2 * TRIG_N];
+  if (!WIDE) {
+    for (int i = threadIdx.x; i < 2 * TRIG_N; i += blockDim.x) trig[i] = a.trig[i];
+    __syncthreads();
+  }
+  const int sub = threadIdx.x & 15;
+  const int64_t pidx = (int64_t)blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4);
+  if (pidx >= a.npairs) return;  // whole rows: the row sums below stay within live groups
   const SingularPair P = a.pairs[pidx];
   const double* tr = a.rec + (int64_t)P.test * FREC;
   const double* sr = a.rec + (int64_t)P.trial * FREC;
@@ -675,15 +758,25 @@ __global__ __launch_bounds__(256) void bm_singular_kernel(const SingularArgs<TO>
     if (P.xform == x)
 #pragma unroll
       for (int c = 0; c < 6; ++c) sg[c] = a.img[x][c];
-  double tv[3][3], sv[3][3];
+  // vertex 0 and the edge vectors of both faces
+  double t0v[3], t1[3], t2[3], s0v[3], s1[3], s2[3];
   int tvi[3], svi[3];
   for (int v = 0; v < 3; ++v) {
     tvi[v] = a.fverts[(int64_t)P.test * 3 + v];
     svi[v] = a.fverts[(int64_t)P.trial * 3 + v];
-    for (int c = 0; c < 3; ++c) {
-      tv[v][c] = a.verts[3 * (int64_t)tvi[v] + c];
-      sv[v][c] = a.verts[3 * (int64_t)svi[v] + c] * sg[c];
-    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double ta = a.verts[3 * (int64_t)tvi[0] + c], tb = a.verts[3 * (int64_t)tvi[1] + c],
+                 tc = a.verts[3 * (int64_t)tvi[2] + c];
+    const double sa = a.verts[3 * (int64_t)svi[0] + c] * sg[c], sb = a.verts[3 * (int64_t)svi[1] + c] * sg[c],
+                 sc = a.verts[3 * (int64_t)svi[2] + c] * sg[c];
+    t0v[c] = ta;
+    t1[c] = tb - ta;
+    t2[c] = tc - ta;
+    s0v[c] = sa;
+    s1[c] = sb - sa;
+    s2[c] = sc - sa;
   }
   const double nT[3] = {tr[REC_N], tr[REC_N + 1], tr[REC_N + 2]};
   const double nS[3] = {sr[REC_N] * sg[0], sr[REC_N + 1] * sg[1], sr[REC_N + 2] * sg[2]};
@@ -693,21 +786,23 @@ __global__ __launch_bounds__(256) void bm_singular_kernel(const SingularArgs<TO>
   double acc[26];
 #pragma unroll
   for (int i = 0; i < 26; ++i) acc[i] = 0;
-  for (int q = lane; q < cnt; q += 64) {
+  for (int q = sub; q < cnt; q += 16) {
     const double* rd = a.rule_data + 5 * (int64_t)(off + q);
     const double tx = rd[0], ty = rd[1], sx = rd[2], sy = rd[3];
-    const double ta = 1.0 - tx - ty, sa = 1.0 - sx - sy;
     double x[3], y[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-      x[c] = ta * tv[0][c] + tx * tv[1][c] + ty * tv[2][c];
-      y[c] = sa * sv[0][c] + sx * sv[1][c] + sy * sv[2][c];
+      x[c] = fma(tx, t1[c], fma(ty, t2[c], t0v[c]));
+      y[c] = fma(sx, s1[c], fma(sy, s2[c], s0v[c]));
     }
     const double rx = y[0] - x[0], ry = y[1] - x[1], rz = y[2] - x[2];
-    const double radius = sqrt(rx * rx + ry * ry + rz * rz);
-    if (radius == 0.0) continue;
-    const double ir = 1.0 / radius;
+    const double r2 = rx * rx + ry * ry + rz * rz;
+    if (r2 == 0.0) continue;
+    const double ir = inv_sqrt(r2);
+    const double radius = r2 * ir;
     double s, c;
-    sincos_phase(k * radius, &s, &c);
+    if (WIDE) sincos_phase(k * radius, &s, &c);
+    else sincos_steps(radius * a.ksteps, trig, &s, &c);
     const double scale = INV_FOUR_PI * ir;
     const double Gr = c * scale, Gi = s * scale;
     const double gr = Gr * -ir - Gi * k, gi = Gr * k + Gi * -ir;  // G (-1/r + i k)
@@ -716,61 +811,67 @@ __global__ __launch_bounds__(256) void bm_singular_kernel(const SingularArgs<TO>
     const double tdot = (rx * nS[0] + ry * nS[1] + rz * nS[2]) * ir;
     const double sdot = -(rx * nT[0] + ry * nT[1] + rz * nT[2]) * ir;
     // lhs_s = -grad*tdot*w - kn*wg ; rhs_s = -wg - eta*grad*sdot*w
-    const double lr = -gr * tdot * w - (knr * wgr - kni * wgi);
-    const double li = -gi * tdot * w - (knr * wgi + kni * wgr);
+    const double tw = tdot * w, sw = sdot * w;
+    const double lr = -gr * tw - (knr * wgr - kni * wgi);
+    const double li = -gi * tw - (knr * wgi + kni * wgr);
     const double egr = er * gr - ei * gi, egi = er * gi + ei * gr;
-    const double hr = -wgr - egr * sdot * w, hi = -wgi - egi * sdot * w;
-    const double pt[3] = {ta, tx, ty}, ps[3] = {sa, sx, sy};
+    const double hr = -wgr - egr * sw, hi = -wgi - egi * sw;
+    const double pt[3] = {1.0 - tx - ty, tx, ty}, ps[3] = {1.0 - sx - sy, sx, sy};
+    // phi_b(y) lhs, then the test basis
+    double plr[3], pli[3];
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      plr[b] = ps[b] * lr;
+      pli[b] = ps[b] * li;
+    }
+#pragma unroll
     for (int p = 0; p < 3; ++p) {
       acc[18 + 2 * p] += pt[p] * hr;
       acc[19 + 2 * p] += pt[p] * hi;
+#pragma unroll
       for (int b = 0; b < 3; ++b) {
-        const double bp = pt[p] * ps[b];
-        acc[2 * (3 * p + b)] += bp * lr;
-        acc[2 * (3 * p + b) + 1] += bp * li;
+        acc[2 * (3 * p + b)] += pt[p] * plr[b];
+        acc[2 * (3 * p + b) + 1] += pt[p] * pli[b];
       }
     }
     acc[24] += wgr;
     acc[25] += wgi;
   }
 #pragma unroll
-  for (int i = 0; i < 26; ++i) {
-    double v = acc[i];
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    acc[i] = v;
-  }
+  for (int i = 0; i < 26; ++i) acc[i] = row_sum16(acc[i]);
   // eta * sum(G w) times the curl products
   const double ecr = er * acc[24] - ei * acc[25], eci = er * acc[25] + ei * acc[24];
-  if (lane < 9) {
-    const int p = lane / 3, b = lane % 3;
-    const double cc = tr[REC_CURL + 3 * p] * (sr[REC_CURL + 3 * b] * sg[3]) +
-                      tr[REC_CURL + 3 * p + 1] * (sr[REC_CURL + 3 * b + 1] * sg[4]) +
-                      tr[REC_CURL + 3 * p + 2] * (sr[REC_CURL + 3 * b + 2] * sg[5]);
-    double vr = 0, vi = 0;
+  for (int e = sub; e < 9 + 3 * a.nrhs; e += 16) {
+    if (e < 9) {
+      const int p = e / 3, b = e % 3;
+      const double cc = tr[REC_CURL + 3 * p] * (sr[REC_CURL + 3 * b] * sg[3]) +
+                        tr[REC_CURL + 3 * p + 1] * (sr[REC_CURL + 3 * b + 1] * sg[4]) +
+                        tr[REC_CURL + 3 * p + 2] * (sr[REC_CURL + 3 * b + 2] * sg[5]);
+      double vr = 0, vi = 0;
 #pragma unroll
-    for (int i = 0; i < 9; ++i)
-      if (i == lane) {
-        vr = acc[2 * i] + cc * ecr;
-        vi = acc[2 * i + 1] + cc * eci;
-      }
-    TO* dst = a.A + 2 * ((int64_t)tvi[p] * a.lda + svi[b]);
-    gadd(dst, (TO)vr);
-    gadd(dst + 1, (TO)vi);
-  }
-  for (int e = lane; e < 3 * a.nrhs; e += 64) {
-    const int p = e / a.nrhs, d = e - p * a.nrhs;
-    double hr = 0, hi = 0;
+      for (int i = 0; i < 9; ++i)
+        if (i == e) {
+          vr = acc[2 * i] + cc * ecr;
+          vi = acc[2 * i + 1] + cc * eci;
+        }
+      TO* dst = a.A + 2 * ((int64_t)tvi[p] * a.lda + svi[b]);
+      gadd(dst, (TO)vr);
+      gadd(dst + 1, (TO)vi);
+    } else {
+      const int f = e - 9, p = f / a.nrhs, d = f - p * a.nrhs;
+      double hr = 0, hi = 0;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-      if (i == p) {
-        hr = acc[18 + 2 * i];
-        hi = acc[19 + 2 * i];
-      }
-    const double* qq = a.q + 2 * ((int64_t)P.trial * a.nrhs + d);
-    const double vr = hr * qq[0] - hi * qq[1], vi = hr * qq[1] + hi * qq[0];
-    TO* dst = a.B + 2 * ((int64_t)tvi[p] * a.nrhs + d);
-    gadd(dst, (TO)vr);
-    gadd(dst + 1, (TO)vi);
+      for (int i = 0; i < 3; ++i)
+        if (i == p) {
+          hr = acc[18 + 2 * i];
+          hi = acc[19 + 2 * i];
+        }
+      const double* qq = a.q + 2 * ((int64_t)P.trial * a.nrhs + d);
+      const double vr = hr * qq[0] - hi * qq[1], vi = hr * qq[1] + hi * qq[0];
+      TO* dst = a.B + 2 * ((int64_t)tvi[p] * a.nrhs + d);
+      gadd(dst, (TO)vr);
+      gadd(dst + 1, (TO)vi);
+    }
   }
 }
 
@@ -1293,8 +1394,11 @@ hipError_t launch_singular_operators(const SingularOperatorArgs<TO>& a, hipStrea
 template <typename TO>
 hipError_t launch_singular(const SingularArgs<TO>& a, hipStream_t s) {
   if (a.npairs == 0) return hipSuccess;
-  dim3 grid((unsigned)((a.npairs + 3) / 4)), block(256);
-  hipLaunchKernelGGL((bm_singular_kernel<TO>), grid, block, 0, s, a);
+  if (!a.wide && !a.trig) return hipErrorInvalidValue;
+  // 16 pairs per workgroup of 256 (a 16-lane group each)
+  dim3 grid((unsigned)((a.npairs + 15) / 16)), block(256);
+  if (a.wide) hipLaunchKernelGGL((bm_singular_kernel<TO, true>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((bm_singular_kernel<TO, false>), grid, block, 0, s, a);
   return hipGetLastError();
 }
 

@@ -76,6 +76,25 @@ BEAT_HD void sincos_rotate(double y, double S, double C, double* s, double* c) {
   *c = fma(C, cy, -(S * sy));
 }
 
+// The same on the remainder in table steps, u = t - n (|u| <= 1/2, y = u pi/64):
+// the powers of pi/64 sit in the coefficients, sin y = u (h - h^3/3! u^2 + h^5/5!
+// u^4 - h^7/7! u^6), cos y = 1 - h^2/2! u^2 + h^4/4! u^4 - h^6/6! u^6 with h = pi/64
+// (coefficients rounded once from a 50-digit evaluation).
+BEAT_HD void sincos_rotate_steps(double u, double S, double C, double* s, double* c) {
+  const double u2 = u * u;
+  double ps = -1.36256978594427006e-13;
+  ps = fma(ps, u2, 2.37502533931037925e-09);
+  ps = fma(ps, u2, -1.97132598115919886e-05);
+  ps = fma(ps, u2, 4.90873852123405174e-02);
+  const double sy = ps * u;
+  double pc = -1.94306306199663886e-11;
+  pc = fma(pc, u2, 2.41918094540641814e-07);
+  pc = fma(pc, u2, -1.20478569349235327e-03);
+  const double cy = fma(pc, u2, 1.0);
+  *s = fma(S, cy, C * sy);
+  *c = fma(C, cy, -(S * sy));
+}
+
 BEAT_HD void sincos_table(double x, const double* trig, double* s, double* c) {
   const double n = rint(x * 20.371832715762604);  // 64/pi
   double y = fma(-n, 1.5707963276654482 / 32, x);

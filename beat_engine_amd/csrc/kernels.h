@@ -77,6 +77,11 @@ struct SingularArgs {
   double k, eta_re, eta_im;
   // signs by SingularPair::xform (row 0: identity): point signs, then curl signs
   double img[MAX_IMAGES + 1][6];
+  // the phase: the pi/64 table (2 TRIG_N doubles) in steps of k 64/pi, unless wide
+  // (|k| r not bounded below the table's limit: the pi/2 form)
+  const double* trig;
+  double ksteps;
+  int wide;
 };
 
 // Four separate operators (operator_matrices): S, K' (n x ldf, P1 rows x DP0

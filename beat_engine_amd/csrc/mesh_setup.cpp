@@ -526,6 +526,17 @@ std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64
         }
     }
   }
+  // pairs by rule size, largest first, within the ordinary and the image section
+  // (stable; every consumer is order-free): the singular kernel runs several pairs
+  // per wavefront, which then mostly share their point count
+  {
+    auto by_size = [&](const SingularPair& x, const SingularPair& y) {
+      return m.rule_cnt[x.rule] > m.rule_cnt[y.rule];
+    };
+    const auto mid = m.spairs.end() - m.n_image_singular;
+    std::stable_sort(m.spairs.begin(), mid, by_size);
+    std::stable_sort(mid, m.spairs.end(), by_size);
+  }
   // P1 orbit weights: rows on a mirror plane stand for one physical dof shared
   // with its image (symmetry_plane_tolerance / p1_symmetry_orbit_weights,
   // BeatEngineCore.jl:497-517,563-580); the rigid ground has none.
