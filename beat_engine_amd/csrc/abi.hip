@@ -66,6 +66,7 @@ struct beat_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // look-ahead stream of the LU
+  int side_reserve = 0;        // eighths of the CUs the side stream leaves free
   hipEvent_t ev_fact = nullptr, ev_far = nullptr;
   hipStream_t inb = nullptr;  // in-block stream of the LU
   hipEvent_t ev_panel = nullptr, ev_inb = nullptr;
@@ -1905,6 +1906,29 @@ int field_noalloc_impl(beat_ctx* ctx, const beat_mesh* m, int precision, double 
 
 extern "C" {
 
+// The LU's look-ahead stream carries the far trailing updates, the factorization's
+// largest kernels; without a limit they fill every CU and the next leaf's panel
+// kernels on the main stream wait for a CU until they drain. The side stream
+// therefore runs on a CU mask that keeps r/8 of the CUs free (BEAT_LU_SIDE_RESERVE
+// = r, default 1; 0: every CU): bit i is cleared when ((i & 7) - ((i >> 5) & 7)) & 7
+// < r, which keeps 4r CUs of every XCD free whether the mask's bits run XCD by
+// XCD or interleave over the XCDs.
+static hipError_t create_side_stream(beat_ctx* c) {
+  int r = 1;
+  if (const char* v = std::getenv("BEAT_LU_SIDE_RESERVE")) r = std::max(0, std::min(7, std::atoi(v)));
+  int ncu = 0;
+  if (r > 0 && hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) == hipSuccess &&
+      ncu >= 64) {
+    std::vector<uint32_t> mask((size_t)(ncu + 31) / 32, 0u);
+    for (int i = 0; i < ncu; ++i)
+      if ((((i & 7) - ((i >> 5) & 7)) & 7) >= r) mask[(size_t)i / 32] |= 1u << (i % 32);
+    c->side_reserve = r;
+    return hipExtStreamCreateWithCUMask(&c->side, (uint32_t)mask.size(), mask.data());
+  }
+  c->side_reserve = 0;
+  return hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+}
+
 int beat_ctx_create(int device, beat_ctx** out) {
   if (!out) return BEAT_ERR_ARG;
   *out = nullptr;
@@ -1937,7 +1961,7 @@ int beat_ctx_create(int device, beat_ctx** out) {
     delete c;
     return BEAT_ERR_HIP;
   }
-  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+  if (create_side_stream(c) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fact, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_far, hipEventDisableTiming) != hipSuccess ||
       hipStreamCreateWithFlags(&c->inb, hipStreamNonBlocking) != hipSuccess ||
