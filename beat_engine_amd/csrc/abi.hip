@@ -90,6 +90,9 @@ struct beat_ctx {
   // the regular pass over unordered tile pairs (both orders of a face pair from
   // one evaluation of the kernel); false: the ordered-pair kernel
   bool regular_symmetric = true;
+  // FP32 solves assemble in complex128 and round the system once (false: the
+  // complex64 assembly kernels; BEAT_FP32_ASSEMBLY=native at context creation)
+  bool fp32_assembly_fp64 = true;
   // contexts a batch may use at once (BEAT_BATCH_WORKERS, default 4)
   int batch_workers = 4;
   // batches of small systems through the grouped, graph-replayed path
@@ -797,7 +800,7 @@ struct AsmEvents {
 // Stage 1 (stream st): gather q_dev (caller column-major complex128) into internal
 // order and assemble the system of the precision: complex128 for FP64 and the
 // refined path (plus the complex64 copy the refined path factors), complex64 for
-// FP32.
+// FP32 (the complex128 system rounded once, or the complex64 kernels).
 int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, double er, double ei,
                    const double* q_dev, int nrhs, SystemBufs& sb, hipStream_t st, const AsmEvents& ev,
                    bool one_per_cu = false, bool narrow = false) {
@@ -807,7 +810,7 @@ int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, d
   hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, st, q_dev, nf, nrhs,
                      (const int32_t*)m->face_perm, as<double>(*sb.q64));
   HIPCHK(hipGetLastError());
-  if (precision == BEAT_PREC_FP32) {
+  if (precision == BEAT_PREC_FP32 && !ctx->fp32_assembly_fp64) {
     HIPCHK(ensure(*sb.q32, sizeof(float) * 2 * nf * nrhs));
     HIPCHK(launch_downcast(as<double>(*sb.q64), as<float>(*sb.q32), 2 * nf * nrhs, st));
     HIPCHK(ensure(*sb.a32, sizeof(float) * 2 * n * lda));
@@ -815,6 +818,22 @@ int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, d
     if (int rc = assemble<float>(ctx, m, k, er, ei, as<double>(*sb.q64), as<float>(*sb.q32), nrhs, as<float>(*sb.a32),
                                  lda, as<float>(*sb.b32), st, ev.reg0, ev.reg1, one_per_cu, narrow))
       return rc;
+    HIPCHK(ensure(*sb.af, sizeof(float) * 2 * n * lda));
+    HIPCHK(hipMemcpyAsync(sb.af->p, sb.a32->p, sizeof(float) * 2 * n * lda, hipMemcpyDeviceToDevice, st));
+  } else if (precision == BEAT_PREC_FP32) {
+    // the complex64 system as the complex128 assembly rounded once: the FP64
+    // regular kernel (table phase, no transcendental instructions) outruns the
+    // FP32 one, and the rounded matrix is at least as accurate
+    HIPCHK(ensure(*sb.a64, sizeof(double) * 2 * n * lda));
+    HIPCHK(ensure(*sb.b64, sizeof(double) * 2 * n * nrhs));
+    if (int rc = assemble<double>(ctx, m, k, er, ei, as<double>(*sb.q64), as<double>(*sb.q64), nrhs,
+                                  as<double>(*sb.a64), lda, as<double>(*sb.b64), st, ev.reg0, ev.reg1, one_per_cu,
+                                  narrow))
+      return rc;
+    HIPCHK(ensure(*sb.a32, sizeof(float) * 2 * n * lda));
+    HIPCHK(ensure(*sb.b32, sizeof(float) * 2 * n * nrhs));
+    HIPCHK(launch_downcast(as<double>(*sb.a64), as<float>(*sb.a32), 2 * n * lda, st));
+    HIPCHK(launch_downcast(as<double>(*sb.b64), as<float>(*sb.b32), 2 * n * nrhs, st));
     HIPCHK(ensure(*sb.af, sizeof(float) * 2 * n * lda));
     HIPCHK(hipMemcpyAsync(sb.af->p, sb.a32->p, sizeof(float) * 2 * n * lda, hipMemcpyDeviceToDevice, st));
   } else {
@@ -1986,6 +2005,7 @@ int beat_ctx_create(int device, beat_ctx** out) {
   }
   if (const char* v = std::getenv("BEAT_SWEEP_ASM_ONE_PER_CU")) c->sweep_asm_one_per_cu = v[0] != '0';
   if (const char* v = std::getenv("BEAT_REGULAR_SYMMETRIC")) c->regular_symmetric = v[0] != '0';
+  if (const char* v = std::getenv("BEAT_FP32_ASSEMBLY")) c->fp32_assembly_fp64 = std::strcmp(v, "native") != 0;
   if (const char* v = std::getenv("BEAT_BATCH_WORKERS")) c->batch_workers = std::max(1, std::atoi(v));
   if (const char* v = std::getenv("BEAT_BATCH_GROUPED")) c->batch_grouped = v[0] != '0';
   // BEAT_LU_LOOKAHEAD=0 runs the whole factorization in order on one stream
@@ -2673,6 +2693,7 @@ int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, 
     a->debug_abort_waits = ctx->debug_abort_waits;
     a->debug_force_exchange = ctx->debug_force_exchange;
     a->regular_symmetric = ctx->regular_symmetric;
+    a->fp32_assembly_fp64 = ctx->fp32_assembly_fp64;
     a->sweep_asm_one_per_cu = ctx->sweep_asm_one_per_cu;
     std::memcpy(a->lu_layout, ctx->lu_layout, sizeof(ctx->lu_layout));
     std::memcpy(a->asm_occ, ctx->asm_occ, sizeof(ctx->asm_occ));

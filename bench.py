@@ -14,6 +14,10 @@ inputs for the ratio, and the refined pressures are compared with it (relative
 L2 per drive). The CPU baseline solves one frequency of the workload whole on the
 host (the oracle's fused FP64 assembly of every pair, LAPACK getrf/getrs), and the
 GPU's pressures at that frequency are compared with it (rel_err_vs_cpu_oracle).
+Beside the headline the line carries the plain complex64 path (fp32_gpu), the
+precision form (a 32k-face cylinder, 64 drives, 10k field points in FP32 and
+FP64), the differentiation form (an adjoint gradient) and the batching form (32
+small meshes x 32 frequencies, 4 drives, jobs/s).
 
 Prints one JSON line. --gpus above 1 is not supported: this build runs on one GPU.
 """
@@ -33,10 +37,10 @@ METRIC = "freq-solves/s @N=20480,8 RHS: FP32-MFMA+IR vs FP64 GPU at ≤1e-8 rel 
 FP64_VALU_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 FLOP_PER_QPAIR = 72  # fused Burton-Miller flops per quadrature-point pair
-# HBM bytes per launch of the FP64 regular kernel at L5, from two rocprofv3 --pmc
-# runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors make a 1 GiB device copy
-# read as 1 GiB each way); profiles/bm_regular_L5_pmc_traffic.md
-REGULAR_L5_TRAFFIC_BYTES = 19.17e9
+# HBM bytes per launch of the FP64 regular kernel (the unordered-pair pass) at L5,
+# from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0: the factors make
+# a 1 GiB device copy read as 1 GiB each way); profiles/bm_regular_sym_L5_pmc_traffic.md
+REGULAR_L5_TRAFFIC_BYTES = 22.67e9
 
 
 def parse():
@@ -51,13 +55,16 @@ def parse():
     ap.add_argument("--batch-form", action="store_true",
                     help="time the batching form (32 meshes x 128 frequencies, 4 RHS) instead; no JSON line")
     ap.add_argument("--batch-freqs", type=int, default=128)
+    ap.add_argument("--json-batch-freqs", type=int, default=32,
+                    help="frequencies per mesh of the batching form in the JSON line (32 meshes each)")
     ap.add_argument("--counter-run", action="store_true",
                     help="for rocprofv3 --pmc: a 1 GiB device copy (counter calibration), then one complex128 "
                          "freq-solve of the workload; prints no JSON line")
     ap.add_argument("--no-rocsolver", action="store_true",
                     help="skip timing rocSOLVER getrf (the vendor LU) at the workload's size")
     ap.add_argument("--no-extra-forms", action="store_true",
-                    help="skip the plain complex64 sweep, the precision form's field and the differentiation form")
+                    help="skip the plain complex64 sweep, the precision form's field, the differentiation form "
+                         "and the batching form")
     ap.add_argument("--cpu-freq-index", type=int, default=11,
                     help="which of the 20 frequencies the CPU baseline solves (and the GPU result is checked at)")
     return ap.parse_args()
@@ -179,11 +186,11 @@ def rocsolver_getrf_ms(n, reps=2):
     return out
 
 
-def batch_form(args):
+def batch_form(eng, nfreq):
     """32 meshes of uneven size (jittered icospheres and graded capped cylinders, 500-6000
-    triangles, from seeds) x log-spaced frequencies 50 Hz - 8 kHz, 4 RHS each, as one
-    batch per precision; prints jobs/s per precision and the refined-vs-FP64 error."""
-    from beat_engine_amd.exterior import Engine
+    triangles, from seeds) x nfreq log-spaced frequencies 50 Hz - 8 kHz, 4 RHS each, as
+    one batch per precision (grouped batches of equal-size systems, one hipGraph per
+    group shape); jobs/s per precision and the refined-vs-FP64 error."""
     from beat_engine_amd.mesh import capped_cylinder, jittered_icosphere
     from beat_engine_amd.phasor import burton_miller_coupling, neumann_scale
 
@@ -199,8 +206,7 @@ def batch_form(args):
         else:
             m = capped_cylinder(target_faces=target, seed=200 + i)
         meshes.append(m)
-    freqs = np.logspace(np.log10(50.0), np.log10(8000.0), args.batch_freqs)
-    eng = Engine(0)
+    freqs = np.logspace(np.log10(50.0), np.log10(8000.0), nfreq)
     dms = [eng.upload(m) for m in meshes]
     tune = eng.autotune(dms[-1], "float32_refined", 2 * np.pi * 1000 / 343.0, 4)
     log(f"autotune (size class of {meshes[-1].num_vertices} dofs): {tune}")
@@ -224,11 +230,12 @@ def batch_form(args):
         log(f"{prec}: {len(jobs)} jobs in {el:.2f} s = {len(jobs) / el:.1f} jobs/s; not converged: {bad}")
     err = max(float(np.linalg.norm(a - b) / np.linalg.norm(b))
               for a, b in zip(res["float32_refined"][0], res["float64"][0]))
-    print(json.dumps({"batch_form": {"jobs": len(jobs), "meshes": 32, "frequencies": int(args.batch_freqs), "nrhs": 4,
-                                     "refined_jobs_per_s": len(jobs) / res["float32_refined"][1],
-                                     "fp64_jobs_per_s": len(jobs) / res["float64"][1],
-                                     "max_rel_err_refined_vs_fp64": err, "autotune": tune}}), flush=True)
-    eng.close()
+    return {"jobs": len(jobs), "meshes": 32, "frequencies": int(nfreq), "nrhs": 4,
+            "faces_range": [min(faces), max(faces)],
+            "refined_jobs_per_s": len(jobs) / res["float32_refined"][1],
+            "fp64_jobs_per_s": len(jobs) / res["float64"][1],
+            "max_rel_err_refined_vs_fp64": err, "autotune": tune,
+            "timed": "whole batch call per precision: host staging of q and p, grouped assembly and graphs"}
 
 
 FP32_VALU_TFLOPS = 78.6  # FP32 VALU without packed instructions (this build uses none), as FP64
@@ -327,7 +334,9 @@ def main():
     if not torch.cuda.is_available():
         raise RuntimeError("bench.py needs a GPU")
     if args.batch_form:
-        batch_form(args)
+        eng = Engine(0)
+        print(json.dumps({"batch_form": batch_form(eng, args.batch_freqs)}), flush=True)
+        eng.close()
         return
     mesh = jittered_icosphere(args.level, 0.1, 0.02, seed=20480)
     nrhs = args.nrhs
@@ -463,6 +472,9 @@ def main():
         t = time.perf_counter()
         extra["differentiation_form"] = differentiation_form(eng)
         log(f"differentiation form ({time.perf_counter() - t:.1f} s):", extra["differentiation_form"])
+        t = time.perf_counter()
+        extra["batch_form"] = batch_form(eng, args.json_batch_freqs)
+        log(f"batch form ({time.perf_counter() - t:.1f} s):", extra["batch_form"])
 
     reg_s = float(np.mean([r.regular_kernel_s for r in serial]))
     counts = dm.counts()
@@ -511,7 +523,7 @@ def main():
         },
         "element_pair_interactions_per_s": counts["regular_pairs"] / reg_s,
         "roofline": {
-            "kernel": "bm_regular_kernel<double,6>",
+            "kernel": "bm_regular_sym_kernel<double,6>",
             "bound": "valu",
             "achieved": achieved,
             "peak": FP64_VALU_PEAK_TFLOPS,
