@@ -145,6 +145,7 @@ struct beat_mesh {
   uint8_t* flocq = nullptr;
   int32_t* tile_qfirst = nullptr;
   int64_t* sym_unit_start = nullptr;
+  uint8_t* frot = nullptr;
   SingularPair* spairs = nullptr;
   int32_t *rule_off = nullptr, *rule_cnt = nullptr;
   double* rule_data = nullptr;
@@ -617,6 +618,7 @@ void regular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, Regular
   ra.flocq = m->flocq;
   ra.tile_qfirst = m->tile_qfirst;
   ra.unit_start = m->sym_unit_start;
+  ra.frot = m->frot;
   ra.nqtiles = (int)m->h.qtile_start.size();
   ra.nunits = m->h.sym_unit_start.empty() ? 0 : m->h.sym_unit_start.back();
   ra.symmetric = ctx->regular_symmetric ? 1 : 0;
@@ -2212,6 +2214,7 @@ int beat_mesh_upload_symmetric(beat_ctx* ctx, const double* verts, int64_t nv, c
   if (e == hipSuccess) e = upload(&m->flocq, m->h.flocq);
   if (e == hipSuccess) e = upload(&m->tile_qfirst, m->h.tile_qfirst);
   if (e == hipSuccess) e = upload(&m->sym_unit_start, m->h.sym_unit_start);
+  if (e == hipSuccess) e = upload(&m->frot, m->h.frot);
   if (e == hipSuccess) e = upload(&m->spairs, m->h.spairs);
   if (e == hipSuccess) e = upload(&m->rule_off, m->h.rule_off);
   if (e == hipSuccess) e = upload(&m->rule_cnt, m->h.rule_cnt);
@@ -2263,7 +2266,7 @@ void beat_mesh_destroy(beat_mesh* m) {
   void* ptrs[] = {m->rec64, m->rec32, m->verts, m->fverts, m->floc, m->tile_start, m->tile_count, m->tile_nv,
                   m->tile_verts, m->stile_start, m->stile_count, m->stile_nv, m->stile_verts, m->flocs,
                   m->ntile_start, m->ntile_count, m->ntile_nv, m->ntile_verts, m->flocn, m->qtile_start,
-                  m->qtile_count, m->qtile_nv, m->qtile_verts, m->flocq, m->tile_qfirst, m->sym_unit_start, m->spairs,
+                  m->qtile_count, m->qtile_nv, m->qtile_verts, m->flocq, m->tile_qfirst, m->sym_unit_start, m->frot, m->spairs,
                   m->rule_off, m->rule_cnt, m->rule_data, m->rule3, m->rule4, m->face_perm, m->vert_inv, m->wrows,
                   m->wvals};
   if (m->ctx) (void)hipSetDevice(m->ctx->device);

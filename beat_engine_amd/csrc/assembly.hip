@@ -357,8 +357,9 @@ __global__ __launch_bounds__(256, 2) void bm_regular_kernel(const RegularArgs<T>
 // (r' = x - y is the reverse pair's y' - x'). When s lies in t itself, both faces
 // of a pair are test faces of the unit: the pair is taken once, from its lower face.
 // Basis sums: on each side the first basis slot accumulates the sum over the three
-// functions (phi_0 + phi_1 + phi_2 = 1), and phi_0's entry is the difference,
-// formed once per face pair.
+// functions (phi_0 + phi_1 + phi_2 = 1), and the first slot's own entry is the
+// difference, formed once per face pair. The test side's slots run over the face's
+// vertices in a per-face rotated order (the mesh set-up chooses the rotations).
 // LDS pointers keep their address space through the call, so that the atomics on
 // them are known not to touch the face records (which then stay scalar loads)
 template <typename T>
@@ -381,7 +382,13 @@ __device__ __forceinline__ void sym_unit(const RegularArgs<T> a, int tt, int st,
   const T nT0 = tr[REC_N], nT1 = tr[REC_N + 1], nT2 = tr[REC_N + 2];
   const T dT = tr[REC_D], areaT = tr[REC_AREA];
   const int tv0 = fverts[(int64_t)tf * 3], tv1 = fverts[(int64_t)tf * 3 + 1], tv2 = fverts[(int64_t)tf * 3 + 2];
-  const int tl[3] = {a.floc[(int64_t)tf * 3], a.floc[(int64_t)tf * 3 + 1], a.floc[(int64_t)tf * 3 + 2]};
+  // test-side slot k stands for vertex tp[k] = (k + rot) % 3 of the face: the
+  // faces of a tile that share a vertex mostly hold it in different slots, so one
+  // scatter instruction's lanes mostly add to distinct LDS rows
+  const int rot = a.frot[tf];
+  const int tp[3] = {rot, rot == 2 ? 0 : rot + 1, rot == 0 ? 2 : rot - 1};
+  const int tl[3] = {a.floc[(int64_t)tf * 3 + tp[0]], a.floc[(int64_t)tf * 3 + tp[1]],
+                     a.floc[(int64_t)tf * 3 + tp[2]]};
   const T k = a.k, krev = a.krev;
   const T er = a.eta_re, ei = a.eta_im;
   const int wu = __builtin_amdgcn_readfirstlane(wave);
@@ -517,7 +524,7 @@ __device__ __forceinline__ void sym_unit(const RegularArgs<T> a, int tt, int st,
       }
       csr += cwi * Gsr;
       csi += cwi * Gsi;
-      const T ph[3] = {cwi, cwi * sphw[8 * i + 1], cwi * sphw[8 * i + 2]};
+      const T ph[3] = {cwi, cwi * sphw[8 * i + tp[1]], cwi * sphw[8 * i + tp[2]]};
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
         Rfr[p] += ph[p] * Sr;
@@ -569,7 +576,8 @@ __device__ __forceinline__ void sym_unit(const RegularArgs<T> a, int tt, int st,
     const int sl[3] = {flocq[sf * 3], flocq[sf * 3 + 1], flocq[sf * 3 + 2]};
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
-      const T c0 = tr[REC_CURL + 3 * p], c1 = tr[REC_CURL + 3 * p + 1], c2 = tr[REC_CURL + 3 * p + 2];
+      const T* tc = tr + REC_CURL + 3 * tp[p];
+      const T c0 = tc[0], c1 = tc[1], c2 = tc[2];
 #pragma unroll
       for (int b = 0; b < 3; ++b) {
         const T b0 = sr[REC_CURL + 3 * b], b1 = sr[REC_CURL + 3 * b + 1], b2 = sr[REC_CURL + 3 * b + 2];

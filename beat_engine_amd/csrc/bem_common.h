@@ -125,6 +125,11 @@ struct HostMesh {
   std::vector<uint8_t> flocq;        // 3 per internal face: vertex slot in its sub-tile
   std::vector<int32_t> tile_qfirst;
   std::vector<int64_t> sym_unit_start;
+  // per internal face, the rotation r (0..2) of the order in which the symmetric
+  // pass visits its test-side basis (slot k stands for vertex (k + r) % 3): chosen
+  // per test tile so that the faces sharing a vertex hold it in different slots,
+  // which spreads the LDS atomics of one instruction over distinct addresses
+  std::vector<uint8_t> frot;
   std::vector<SingularPair> spairs;
   std::vector<int32_t> rule_off, rule_cnt, rule_kind;
   std::vector<double> rule_data;  // 5 per rule point: test xi, eta, trial xi, eta, weight

@@ -35,6 +35,7 @@ struct RegularArgs {
   const uint8_t* flocq;
   const int32_t* tile_qfirst;
   const int64_t* unit_start;
+  const uint8_t* frot;     // per face: the test-side slot rotation (0..2)
   int nqtiles;
   int64_t nunits;
   int symmetric;           // nonzero: the non-image pass runs the symmetric kernel

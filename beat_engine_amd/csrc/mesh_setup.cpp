@@ -299,6 +299,26 @@ std::string build_host_mesh(const double* V, int64_t nv, const int32_t* F, int64
     m.tile_qfirst[nt] = nq;
     m.sym_unit_start.assign(nt + 1, 0);
     for (size_t t = 0; t < nt; ++t) m.sym_unit_start[t + 1] = m.sym_unit_start[t] + (nq - m.tile_qfirst[t]);
+    // test-side slot rotations: greedily, face by face, the rotation that puts its
+    // vertices into the slots where the tile's faces so far hold them least often
+    m.frot.assign(nf, 0);
+    for (size_t t = 0; t < nt; ++t) {
+      int cnt[TILE_V][3] = {};
+      const int64_t f0 = m.tile_start[t], f1 = f0 + m.tile_count[t];
+      for (int64_t f = f0; f < f1; ++f) {
+        int best = 0, best_cost = 1 << 30;
+        for (int r = 0; r < 3; ++r) {
+          int cost = 0;
+          for (int k = 0; k < 3; ++k) cost += cnt[m.floc[3 * f + (k + r) % 3]][k];
+          if (cost < best_cost) {
+            best_cost = cost;
+            best = r;
+          }
+        }
+        m.frot[f] = (uint8_t)best;
+        for (int k = 0; k < 3; ++k) ++cnt[m.floc[3 * f + (k + best) % 3]][k];
+      }
+    }
   }
 
   // --- face records (internal order)
