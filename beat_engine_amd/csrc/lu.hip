@@ -1621,10 +1621,10 @@ __device__ __forceinline__ void pivot_reciprocal(T ur, T ui, T& ir, T& ii) {
 // with register operands make each step's updates happen in that step (the
 // compiler would otherwise defer them to their first use and keep every step's
 // operands live).
-template <typename T>
+template <typename T, bool LINV = true>
 __device__ __forceinline__ void diag_nopiv_body(T* A, int64_t lda, int64_t j0, int b, T* linv, int* info, int* ctl) {
   __shared__ T prow[2][4][16][2];  // row j of U by column class [parity][c % 4][c / 4][re, im]
-  __shared__ T plow[2][4][16][2];  // row j of L^-1
+  __shared__ T plow[2][4][16][2];  // row j of L^-1 (LINV)
   __shared__ T prcp[2][2];         // 1 / u_jj
   const int t = threadIdx.x, r = t >> 2, q = t & 3;
   T xr[16], xi[16], yr[16], yi[16];
@@ -1657,11 +1657,12 @@ __device__ __forceinline__ void diag_nopiv_body(T* A, int64_t lda, int64_t j0, i
         prow[par][q][s][0] = xr[s];
         prow[par][q][s][1] = xi[s];
       }
+      if constexpr (LINV)
 #pragma unroll
-      for (int s = 0; s <= S; ++s) {
-        plow[par][q][s][0] = yr[s];
-        plow[par][q][s][1] = yi[s];
-      }
+        for (int s = 0; s <= S; ++s) {
+          plow[par][q][s][0] = yr[s];
+          plow[par][q][s][1] = yi[s];
+        }
       if (q == k) pivot_reciprocal(xr[S], xi[S], prcp[par][0], prcp[par][1]);
     }
     __syncthreads();
@@ -1682,18 +1683,20 @@ __device__ __forceinline__ void diag_nopiv_body(T* A, int64_t lda, int64_t j0, i
           xr[s] -= mr * pr - mi * pi;
           xi[s] -= mr * pi + mi * pr;
         }
+      if constexpr (LINV)
 #pragma unroll
-      for (int s = 0; s <= S; ++s)
-        if (s < S || q <= k) {
-          const T pr = plow[par][q][s][0], pi = plow[par][q][s][1];
-          yr[s] -= mr * pr - mi * pi;
-          yi[s] -= mr * pi + mi * pr;
-        }
+        for (int s = 0; s <= S; ++s)
+          if (s < S || q <= k) {
+            const T pr = plow[par][q][s][0], pi = plow[par][q][s][1];
+            yr[s] -= mr * pr - mi * pi;
+            yi[s] -= mr * pi + mi * pr;
+          }
     }
 #pragma unroll
     for (int s = S; s < 16; ++s) __asm__ __volatile__("" : "+v"(xr[s]), "+v"(xi[s]));
+    if constexpr (LINV)
 #pragma unroll
-    for (int s = 0; s <= S; ++s) __asm__ __volatile__("" : "+v"(yr[s]), "+v"(yi[s]));
+      for (int s = 0; s <= S; ++s) __asm__ __volatile__("" : "+v"(yr[s]), "+v"(yi[s]));
     if (ur == T(0) && ui == T(0) && zero == 0 && j < b) zero = j + 1;
   });
 #pragma unroll
@@ -1704,18 +1707,77 @@ __device__ __forceinline__ void diag_nopiv_body(T* A, int64_t lda, int64_t j0, i
       dst[0] = xr[s];
       dst[1] = xi[s];
     }
-    linv[2 * (r * PNB + c)] = yr[s];
-    linv[2 * (r * PNB + c) + 1] = yi[s];
+    if (LINV) {
+      linv[2 * (r * PNB + c)] = yr[s];
+      linv[2 * (r * PNB + c) + 1] = yi[s];
+    }
   }
   if (__ballot(viol) != 0 && (t & 63) == 0) __hip_atomic_fetch_or(ctl + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (t == 0 && zero != 0 && *info == 0) *info = (int)j0 + zero;
 }
 
-template <typename T>
+template <typename T, bool LINV>
 __global__ __launch_bounds__(256) void diag_nopiv_kernel(T* A, int64_t lda, int64_t j0, int b, T* linv, int* info,
                                                          int* ctl) {
   __builtin_amdgcn_s_setprio(3);
-  diag_nopiv_body(A, lda, j0, b, linv, info, ctl);
+  diag_nopiv_body<T, LINV>(A, lda, j0, b, linv, info, ctl);
+}
+
+// The inverse of the leaf's unit-lower factor L11 (strictly below the diagonal of
+// the factored diagonal block), by the row operations of the elimination applied
+// to the identity: step j broadcasts row j of the inverse, final after steps
+// 0..j-1, and every row i > j subtracts L11[i][j] times it. Threads as in
+// diag_nopiv_body; L11[i][j] reaches the row's four lanes by quad broadcast.
+template <typename T>
+__device__ __forceinline__ void linv_body(const T* A, int64_t lda, int64_t j0, int b, T* linv) {
+  __shared__ T plow[2][4][16][2];  // row j of L^-1 by column class [parity][c % 4][c / 4][re, im]
+  const int t = threadIdx.x, r = t >> 2, q = t & 3;
+  T lr[16], li[16], yr[16], yi[16];
+  {
+    const T* src = A + 2 * ((j0 + min(r, b - 1)) * lda + j0 + q);
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      lr[s] = src[8 * s];
+      li[s] = src[8 * s + 1];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int c = 4 * s + q;
+    const bool in = r < b && c < r;  // the strictly lower part
+    lr[s] = in ? lr[s] : T(0);
+    li[s] = in ? li[s] : T(0);
+    yr[s] = r == c ? T(1) : T(0);
+    yi[s] = T(0);
+    __asm__ __volatile__("" : "+v"(lr[s]), "+v"(li[s]), "+v"(yr[s]), "+v"(yi[s]));
+  }
+  static_for<0, PNB>([&](auto J) {
+    constexpr int j = decltype(J)::value, S = j >> 2, k = j & 3, par = j & 1;
+    if (r == j)
+#pragma unroll
+      for (int s = 0; s <= S; ++s) {
+        plow[par][q][s][0] = yr[s];
+        plow[par][q][s][1] = yi[s];
+      }
+    __syncthreads();
+    const T mr = quad_bcast<k>(lr[S]), mi = quad_bcast<k>(li[S]);
+    if (r > j)
+#pragma unroll
+      for (int s = 0; s <= S; ++s)
+        if (s < S || q <= k) {
+          const T pr = plow[par][q][s][0], pi = plow[par][q][s][1];
+          yr[s] -= mr * pr - mi * pi;
+          yi[s] -= mr * pi + mi * pr;
+        }
+#pragma unroll
+    for (int s = 0; s <= S; ++s) __asm__ __volatile__("" : "+v"(yr[s]), "+v"(yi[s]));
+  });
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int c = 4 * s + q;
+    linv[2 * (r * PNB + c)] = yr[s];
+    linv[2 * (r * PNB + c) + 1] = yi[s];
+  }
 }
 
 // Rows below the diagonal block of a leaf: L21 = A21 U11^-1 by substitution
@@ -1791,9 +1853,16 @@ __device__ __forceinline__ void below_nopiv_body(T* A, int64_t n, int64_t lda, i
   if (__ballot(viol) != 0 && (t & 63) == 0) __hip_atomic_fetch_or(ctl + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The rows below the diagonal block, one workgroup per 64 rows, and one more
+// workgroup (the last) that forms the leaf inverse L11^-1 beside them.
 template <typename T>
-__global__ __launch_bounds__(256) void below_nopiv_kernel(T* A, int64_t n, int64_t lda, int64_t j0, int b, int* ctl) {
+__global__ __launch_bounds__(256) void below_nopiv_kernel(T* A, int64_t n, int64_t lda, int64_t j0, int b, int* ctl,
+                                                          T* linv) {
   __builtin_amdgcn_s_setprio(3);
+  if (blockIdx.x == gridDim.x - 1) {
+    linv_body(A, lda, j0, b, linv);
+    return;
+  }
   below_nopiv_body(A, n, lda, j0, b, ctl, (int64_t)blockIdx.x);
 }
 
@@ -1878,13 +1947,18 @@ static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, 
 template <typename T>
 static hipError_t launch_nopiv_leaf(T* A, int64_t n, int64_t lda, int64_t j0, int b, LuWork& w, T* linv,
                                     hipStream_t s) {
-  hipLaunchKernelGGL((diag_nopiv_kernel<T>), dim3(1), dim3(256), 0, s, A, lda, j0, b, linv, w.info, w.ctl);
+  const int64_t below = n - j0 - b;
+  if (below <= 0) {
+    // the last leaf: its inverse in the diagonal kernel
+    hipLaunchKernelGGL((diag_nopiv_kernel<T, true>), dim3(1), dim3(256), 0, s, A, lda, j0, b, linv, w.info, w.ctl);
+    return hipGetLastError();
+  }
+  // the inverse off the chain, in an extra workgroup of the rows-below kernel
+  hipLaunchKernelGGL((diag_nopiv_kernel<T, false>), dim3(1), dim3(256), 0, s, A, lda, j0, b, linv, w.info, w.ctl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const int64_t below = n - j0 - b;
-  if (below <= 0) return hipSuccess;
-  hipLaunchKernelGGL((below_nopiv_kernel<T>), dim3((unsigned)((below + PNB - 1) / PNB)), dim3(256), 0, s, A, n, lda, j0,
-                     b, w.ctl);
+  hipLaunchKernelGGL((below_nopiv_kernel<T>), dim3((unsigned)((below + PNB - 1) / PNB + 1)), dim3(256), 0, s, A, n,
+                     lda, j0, b, w.ctl, linv);
   return hipGetLastError();
 }
 
