@@ -437,11 +437,17 @@ __device__ __forceinline__ void sym_unit(const RegularArgs<T> a, int tt, int st,
       }
       const T PS = dS - (x0 * nS0 + x1 * nS1 + x2 * nS2);  // a times r
       const T cwi = cw * sphw[8 * i + 3];
-      // forward: M = sum_j phi_b(y_j) w_j lhs, U = sum_j w_j b eta-free G' factor,
-      // Gs = sum_j w_j G; reverse: Mr = sum_j phi_p(y_j) w_j lhs', Fr = the same
-      // sums of rhs'
-      T Mfr[3] = {0, 0, 0}, Mfi[3] = {0, 0, 0}, Ufr = 0, Ufi = 0, Gsr = 0, Gsi = 0;
-      T Mrr[3] = {0, 0, 0}, Mri[3] = {0, 0, 0}, Frr[3] = {0, 0, 0}, Fri[3] = {0, 0, 0};
+      // Per trial point three complex quantities, each summed against the three
+      // trial basis weights (slot 0 the rule weight, slots 1 and 2 the weight times
+      // xi, eta): G = e^{ikr}/r, E = G' / r = G (-1/r^2 + i k / r) and b E (b = the
+      // trial face's normal projection d_T - y.n_T). Every block of both orders is
+      // linear in these sums:
+      //   forward lhs   -a E - K        (a = d_S - x.n_S is constant over y)
+      //   reverse lhs   -b E - K        K = eta k^2 (n_T.n_S) G
+      //   reverse rhs   -G - eta a E
+      //   forward rhs   -G - eta b E    (slot 0)
+      T Ar[3] = {0, 0, 0}, Ai[3] = {0, 0, 0}, Gr3[3] = {0, 0, 0}, Gi3[3] = {0, 0, 0};
+      T Br[3] = {0, 0, 0}, Bi[3] = {0, 0, 0};
       // per point pair, first the distance and the phase table entry, one trial
       // point ahead of the rest (the table read is in flight meanwhile)
       auto stage = [&](int j, PhaseStage<T>& o) {
@@ -460,70 +466,55 @@ __device__ __forceinline__ void sym_unit(const RegularArgs<T> a, int tt, int st,
         phase_end<WIDE>(cur, s, c);
         cur = nxt;
         const T Gr = c * ir, Gi = s * ir;
-        const T Dr = -ir * Gr - k * Gi, Di = -ir * Gi + k * Gr;  // G'
-        const T at = PS * ir, bt = QW[j] * ir;
+        const T ir2 = ir * ir, kir = k * ir;
+        const T Er = -ir2 * Gr - kir * Gi, Ei = kir * Gr - ir2 * Gi;
+        const T QEr = QW[j] * Er, QEi = QW[j] * Ei;
         // rule constants: compile-time, so they live in no register across the loop
         const double wjd = TriRule<NQ>::w[j], p1d = wjd * TriRule<NQ>::xi[j], p2d = wjd * TriRule<NQ>::eta[j];
-        const T wj = (T)wjd, p1 = (T)p1d, p2 = (T)p2d;
-        // eta k^2 (n_T.n_S) G
-        T Kr, Ki;
-        if (PURE) {
-          Kr = -kni * Gi;
-          Ki = kni * Gr;
-        } else {
-          Kr = knr * Gr - kni * Gi;
-          Ki = knr * Gi + kni * Gr;
+        const T wb[3] = {(T)wjd, (T)p1d, (T)p2d};
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          Gr3[b] += wb[b] * Gr;
+          Gi3[b] += wb[b] * Gi;
+          Ar[b] += wb[b] * Er;
+          Ai[b] += wb[b] * Ei;
+          Br[b] += wb[b] * QEr;
+          Bi[b] += wb[b] * QEi;
         }
-        const T lfr = -at * Dr - Kr, lfi = -at * Di - Ki;  // forward lhs
-        T hr, hi;                                          // reverse rhs: -G - eta G' a
-        const T ub = wj * bt;
-        if (PURE) {
-          // eta G' = i ei G': the forward sum keeps (Di, Dr); ei and the signs follow
-          Ufr += ub * Di;
-          Ufi += ub * Dr;
-          const T ea = ei * at;
-          hr = ea * Di - Gr;
-          hi = -ea * Dr - Gi;
-        } else {
-          const T EDr = er * Dr - ei * Di, EDi = er * Di + ei * Dr;
-          Ufr += ub * EDr;
-          Ufi += ub * EDi;
-          hr = -Gr - at * EDr;
-          hi = -Gi - at * EDi;
-        }
-        Gsr += wj * Gr;
-        Gsi += wj * Gi;
-        Mfr[0] += wj * lfr;
-        Mfi[0] += wj * lfi;
-        Mfr[1] += p1 * lfr;
-        Mfi[1] += p1 * lfi;
-        Mfr[2] += p2 * lfr;
-        Mfi[2] += p2 * lfi;
-        const T lrr = -bt * Dr - Kr, lri = -bt * Di - Ki;  // reverse lhs
-        Mrr[0] += wj * lrr;
-        Mri[0] += wj * lri;
-        Mrr[1] += p1 * lrr;
-        Mri[1] += p1 * lri;
-        Mrr[2] += p2 * lrr;
-        Mri[2] += p2 * lri;
-        Frr[0] += wj * hr;
-        Fri[0] += wj * hi;
-        Frr[1] += p1 * hr;
-        Fri[1] += p1 * hi;
-        Frr[2] += p2 * hr;
-        Fri[2] += p2 * hi;
       }
-      // forward rhs sum: -Gs - sum_j w_j b eta G'
-      T Sr, Si;
+      // the blocks' trial-side sums from the three basis sums
+      T Mfr[3], Mfi[3], Mrr[3], Mri[3], Frr[3], Fri[3], Sr, Si;
       if (PURE) {
-        Sr = ei * Ufr - Gsr;
-        Si = -ei * Ufi - Gsi;
+        // eta = i ei: K = i kni G
+        const T eps = ei * PS;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          Mfr[b] = kni * Gi3[b] - PS * Ar[b];
+          Mfi[b] = -(kni * Gr3[b]) - PS * Ai[b];
+          Mrr[b] = kni * Gi3[b] - Br[b];
+          Mri[b] = -(kni * Gr3[b]) - Bi[b];
+          Frr[b] = eps * Ai[b] - Gr3[b];
+          Fri[b] = -(eps * Ar[b]) - Gi3[b];
+        }
+        Sr = ei * Bi[0] - Gr3[0];
+        Si = -(ei * Br[0]) - Gi3[0];
       } else {
-        Sr = -Gsr - Ufr;
-        Si = -Gsi - Ufi;
+        const T epr = er * PS, epi = ei * PS;
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const T Kr = knr * Gr3[b] - kni * Gi3[b], Ki = knr * Gi3[b] + kni * Gr3[b];
+          Mfr[b] = -(PS * Ar[b]) - Kr;
+          Mfi[b] = -(PS * Ai[b]) - Ki;
+          Mrr[b] = -Br[b] - Kr;
+          Mri[b] = -Bi[b] - Ki;
+          Frr[b] = epi * Ai[b] - epr * Ar[b] - Gr3[b];
+          Fri[b] = -(epr * Ai[b]) - epi * Ar[b] - Gi3[b];
+        }
+        Sr = ei * Bi[0] - er * Br[0] - Gr3[0];
+        Si = -(er * Bi[0]) - ei * Br[0] - Gi3[0];
       }
-      csr += cwi * Gsr;
-      csi += cwi * Gsi;
+      csr += cwi * Gr3[0];
+      csi += cwi * Gi3[0];
       const T ph[3] = {cwi, cwi * sphw[8 * i + tp[1]], cwi * sphw[8 * i + tp[2]]};
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
