@@ -30,6 +30,7 @@ EXPORTS = (
     "beat_bm_sweep", "beat_ctx_set_cancel_path", "beat_ctx_reclaim", "beat_ctx_memory_info", "beat_capabilities",
     "beat_adjoint_spl_gradient", "beat_bm_batch", "beat_autotune", "beat_ctx_debug_abort_waits",
     "beat_ctx_debug_force_exchange", "beat_ctx_regular_symmetric", "beat_field_eval_device_noalloc",
+    "beat_ctx_set_tuning",
 )
 
 SYMMETRY_CODES = {"off": 0, "x": 1, "xy": 2, "ground": 3}
@@ -126,6 +127,7 @@ def load(path: str = LIB_PATH):
     L.beat_capabilities.argtypes = [P, ctypes.c_char_p, I64]
     L.beat_capabilities.restype = I64
     L.beat_autotune.argtypes = [P, P, I, D, I64, P, P]
+    L.beat_ctx_set_tuning.argtypes = [P, I, I64, P]
     L.beat_bm_batch.argtypes = [P, I64, P, I, P, P, P, P, I64, P, ctypes.POINTER(SolveOptions), P]
     L.beat_adjoint_spl_gradient.argtypes = [P, P, I64, P, P, P, P, P, I64, D, P, P, P]
     L.beat_dense_lu_solve.argtypes = [P, I, P, I64, P, I64, P]

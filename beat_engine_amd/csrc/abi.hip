@@ -62,6 +62,16 @@ T* as(DevBuf& b) {
 
 }  // namespace
 
+// The autotuner's choices for one precision and size class, -1 where not tuned: the
+// factorization's stream layout (0 look-ahead streams, 1 one stream), the batch
+// assembly occupancy (0 every slot, 1 one workgroup per CU), the LU's
+// trailing-update tile shape (LuWork::gemm_tile) and the regular pass's tiles (0
+// unordered pairs over 16-face sub-tiles, 1 ordered pairs over 32-face trial
+// tiles, 2 ordered pairs over 16-face trial tiles).
+struct Tuning {
+  signed char lu_layout = -1, asm_occ = -1, gemm_tile = -1, asm_tile = -1;
+};
+
 struct beat_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -100,16 +110,12 @@ struct beat_ctx {
   bool batch_grouped = true;
   // cancellation marker polled between the frequencies of a sweep (empty: none)
   std::string cancel_path;
-  // autotuned launch strategy per size class (floor(log2 n)): the factorization's
-  // stream layout (-1 not tuned, 0 look-ahead streams, 1 one stream) and the
-  // batch assembly occupancy (-1 not tuned, 0 every slot, 1 one workgroup per CU)
-  signed char lu_layout[40], asm_occ[40];
+  // autotuned launch strategy and tile shapes per precision and size class
+  // (floor(log2 n))
+  Tuning tune[3][40];
   // further contexts on the same device that run batch jobs of small systems
   // concurrently (created on first use, owned by this context)
   std::vector<beat_ctx*> aux;
-  beat_ctx() {
-    for (int i = 0; i < 40; ++i) lu_layout[i] = asm_occ[i] = -1;
-  }
   double* trig = nullptr;  // (sin, cos)(m pi/64) table of the FP64 phase
   // wait control of the last factorization (LuWork::ctl), passed to its solves
   int* lu_ctl = nullptr;
@@ -165,6 +171,11 @@ inline int size_class(int64_t n) {
   int b = 0;
   while (b < 39 && (int64_t(2) << b) <= n) ++b;
   return b;
+}
+
+// the tuned choices of a precision and system size
+inline Tuning& tuned(beat_ctx* ctx, int precision, int64_t n) {
+  return ctx->tune[precision < 0 || precision > 2 ? 0 : precision][size_class(n)];
 }
 
 int fail(beat_ctx* c, int code, const std::string& msg) {
@@ -678,7 +689,7 @@ void singular_geometry(const beat_ctx* ctx, const beat_mesh* m, double k, Singul
 template <typename T>
 int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, const double* q64, const T* qT,
              int nrhs, T* A, int64_t lda, T* B, hipStream_t st, hipEvent_t ev_reg0, hipEvent_t ev_reg1,
-             bool one_per_cu = false, bool narrow = false) {
+             bool one_per_cu = false, bool narrow = false, int tiles = -1) {
   const int64_t n = m->h.nv;
   HIPCHK(launch_fill<T>(A, 2 * n * lda, T(0), st));
   HIPCHK(launch_fill<T>(B, 2 * n * nrhs, T(0), st));
@@ -692,6 +703,11 @@ int assemble(beat_ctx* ctx, const beat_mesh* m, double k, double er, double ei, 
   ra.eta_re = (T)er;
   ra.eta_im = (T)ei;
   ra.one_per_cu = one_per_cu ? 1 : 0;
+  // the tuned tiles, when the autotuner has chosen them
+  if (tiles >= 0) {
+    ra.symmetric = tiles == 0 ? 1 : 0;
+    narrow = tiles == 2;
+  }
   if (ev_reg0) HIPCHK(hipEventRecord(ev_reg0, st));
   const bool wide = !(std::fabs(k) * m->diameter < SINCOS_TABLE_LIMIT);
   set_trial_tiles(m, narrow, ra);
@@ -807,6 +823,7 @@ int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, d
                    const double* q_dev, int nrhs, SystemBufs& sb, hipStream_t st, const AsmEvents& ev,
                    bool one_per_cu = false, bool narrow = false) {
   const int64_t n = m->h.nv, nf = m->h.nf, lda = lda_for(n);
+  const int tiles = tuned(ctx, precision, n).asm_tile;
   HIPCHK(hipEventRecord(ev.start, st));
   HIPCHK(ensure(*sb.q64, sizeof(double) * 2 * nf * nrhs));
   hipLaunchKernelGGL((gather_q_kernel<double>), dim3(blocks_for(nf * nrhs)), dim3(256), 0, st, q_dev, nf, nrhs,
@@ -818,7 +835,7 @@ int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, d
     HIPCHK(ensure(*sb.a32, sizeof(float) * 2 * n * lda));
     HIPCHK(ensure(*sb.b32, sizeof(float) * 2 * n * nrhs));
     if (int rc = assemble<float>(ctx, m, k, er, ei, as<double>(*sb.q64), as<float>(*sb.q32), nrhs, as<float>(*sb.a32),
-                                 lda, as<float>(*sb.b32), st, ev.reg0, ev.reg1, one_per_cu, narrow))
+                                 lda, as<float>(*sb.b32), st, ev.reg0, ev.reg1, one_per_cu, narrow, tiles))
       return rc;
     HIPCHK(ensure(*sb.af, sizeof(float) * 2 * n * lda));
     HIPCHK(hipMemcpyAsync(sb.af->p, sb.a32->p, sizeof(float) * 2 * n * lda, hipMemcpyDeviceToDevice, st));
@@ -830,7 +847,7 @@ int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, d
     HIPCHK(ensure(*sb.b64, sizeof(double) * 2 * n * nrhs));
     if (int rc = assemble<double>(ctx, m, k, er, ei, as<double>(*sb.q64), as<double>(*sb.q64), nrhs,
                                   as<double>(*sb.a64), lda, as<double>(*sb.b64), st, ev.reg0, ev.reg1, one_per_cu,
-                                  narrow))
+                                  narrow, tiles))
       return rc;
     HIPCHK(ensure(*sb.a32, sizeof(float) * 2 * n * lda));
     HIPCHK(ensure(*sb.b32, sizeof(float) * 2 * n * nrhs));
@@ -843,7 +860,7 @@ int stage_assemble(beat_ctx* ctx, const beat_mesh* m, int precision, double k, d
     HIPCHK(ensure(*sb.b64, sizeof(double) * 2 * n * nrhs));
     if (int rc = assemble<double>(ctx, m, k, er, ei, as<double>(*sb.q64), as<double>(*sb.q64), nrhs,
                                   as<double>(*sb.a64), lda, as<double>(*sb.b64), st, ev.reg0, ev.reg1, one_per_cu,
-                                  narrow))
+                                  narrow, tiles))
       return rc;
     if (precision == BEAT_PREC_FP64) {
       HIPCHK(ensure(*sb.af, sizeof(double) * 2 * n * lda));
@@ -875,10 +892,12 @@ int stage_solve(beat_ctx* ctx, const beat_mesh* m, int precision, int nrhs, Syst
   HIPCHK(hipStreamWaitEvent(s, asm_ev.end, 0));
   LuWork lw;
   if (ensure_lu_work<double>(ctx, n, lw) != BEAT_OK) return BEAT_ERR_HIP;
-  if (ctx->lu_layout[size_class(n)] == 1) {  // tuned: one stream for this size class
+  const Tuning tu = tuned(ctx, precision, n);
+  if (tu.lu_layout == 1) {  // tuned: one stream for this precision and size class
     lw.side = nullptr;
     lw.inb = nullptr;
   }
+  lw.gemm_tile = std::max<int>(0, tu.gemm_tile);
   int panels = 0, info = 0, factorizations = 0;
   Refine R;
   // the right-hand sides ride along in the padding columns of the factored
@@ -1072,7 +1091,7 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
     if (i >= 2) HIPCHK(hipStreamWaitEvent(ctx->asm_stream, ctx->sw_free[s2], 0));
     // the first job's assembly has nothing beside it: every slot, wide tiles;
     // later ones run beside a factorization: narrow tiles
-    const signed char occ = ctx->asm_occ[size_class(meshes[i]->h.nv)];
+    const signed char occ = tuned(ctx, precision, meshes[i]->h.nv).asm_occ;
     const bool one_per_cu = i > 0 && (occ < 0 ? ctx->sweep_asm_one_per_cu : occ == 1);
     return stage_assemble(ctx, meshes[i], precision, ks[i], ers[i], eis[i], q_dev + 2 * q_off[i], nrhs, sb[s2],
                           ctx->asm_stream, ev[s2], one_per_cu, i > 0);
@@ -1463,6 +1482,7 @@ int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* mes
       H.hd[J + jj] = BatchVec<float>{as<float>(ctx->g_a32) + 2 * (G.mat * js + n), n, G.lda, nullptr};
       H.hj[jj] = BatchJobDesc{n, m->vert_inv, p_dev + 2 * p_off[g0 + jj]};
     }
+    H.key += "/tile" + std::to_string(std::max<int>(0, tuned(ctx, precision, H.gmax).gemm_tile));
     HIPCHK(hipMemcpyAsync(D.sys64, H.h64.data(), sizeof(BatchSystem<double>) * J, hipMemcpyHostToDevice, sa));
     HIPCHK(hipMemcpyAsync(D.sys32, H.h32.data(), sizeof(BatchSystem<float>) * J, hipMemcpyHostToDevice, sa));
     HIPCHK(hipMemcpyAsync(D.dX, H.hv.data(), sizeof(BatchVec<double>) * 10 * J, hipMemcpyHostToDevice, sa));
@@ -1483,13 +1503,15 @@ int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* mes
         float* A = H.h32[jj].A;
         float* B = as<float>(ctx->g_b32) + 2 * G.vec * js;
         HIPCHK(launch_downcast(q64, q32, 2 * nf * nrhs, sa));
-        if (int rc = assemble<float>(ctx, m, ks[i], ers[i], eis[i], q64, q32, nrhs, A, G.lda, B, sa, nullptr, nullptr))
+        if (int rc = assemble<float>(ctx, m, ks[i], ers[i], eis[i], q64, q32, nrhs, A, G.lda, B, sa, nullptr, nullptr,
+                                     false, false, tuned(ctx, precision, n).asm_tile))
           return rc;
         HIPCHK(lu_append_rhs<float>(A, n, G.lda, B, nrhs, sa));
       } else {
         double* A = H.h64[jj].A;
         double* B = as<double>(ctx->g_b64) + 2 * G.vec * js;
-        if (int rc = assemble<double>(ctx, m, ks[i], ers[i], eis[i], q64, q64, nrhs, A, G.lda, B, sa, nullptr, nullptr))
+        if (int rc = assemble<double>(ctx, m, ks[i], ers[i], eis[i], q64, q64, nrhs, A, G.lda, B, sa, nullptr, nullptr,
+                                      false, false, tuned(ctx, precision, n).asm_tile))
           return rc;
         if (refined) {
           float* A32 = H.h32[jj].A;
@@ -1523,12 +1545,13 @@ int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* mes
         if (ce == hipSuccess) ce = e;
       };
       const int64_t gmax = H.gmax, gncmax = H.gncmax;
+      const int tile = std::max<int>(0, tuned(ctx, precision, gmax).gemm_tile);
       ok(launch_fill<int>(D.lustat, 4 * (int64_t)nj, 0, s));
       const unsigned tb = (unsigned)((nj + 63) / 64);
       const dim3 ge((unsigned)((gmax * nrhs + 255) / 256), (unsigned)nj);
       if (refined) {
         // complex64 speculative LU (three-product updates), first solution
-        ok(lu_factor_batched<float>(D.sys32, nj, gmax, gncmax, true, s));
+        ok(lu_factor_batched<float>(D.sys32, nj, gmax, gncmax, true, tile, s));
         ok(tri_solve_batched<float>(D.sys32, D.dCar32, nj, nrhs, false, true, s));
         hipLaunchKernelGGL((batch_elementwise_kernel<0>), ge, dim3(256), 0, s, D.dX, D.dCar32, nullptr, nullptr,
                            nullptr, nullptr, nullptr, nrhs);
@@ -1555,12 +1578,12 @@ int batch_grouped_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* mes
         hipLaunchKernelGGL((batch_elementwise_kernel<4>), ge, dim3(256), 0, s, D.dOut, nullptr, nullptr, nullptr,
                            nullptr, D.dX, nullptr, nrhs);
       } else if (fp32) {
-        ok(lu_factor_batched<float>(D.sys32, nj, gmax, gncmax, false, s));
+        ok(lu_factor_batched<float>(D.sys32, nj, gmax, gncmax, false, tile, s));
         ok(tri_solve_batched<float>(D.sys32, D.dCar32, nj, nrhs, false, true, s));
         hipLaunchKernelGGL((batch_elementwise_kernel<0>), ge, dim3(256), 0, s, D.dOut, D.dCar32, nullptr, nullptr,
                            nullptr, nullptr, nullptr, nrhs);
       } else {
-        ok(lu_factor_batched<double>(D.sys64, nj, gmax, gncmax, false, s));
+        ok(lu_factor_batched<double>(D.sys64, nj, gmax, gncmax, false, tile, s));
         ok(tri_solve_batched<double>(D.sys64, D.dCar64, nj, nrhs, false, true, s));
         hipLaunchKernelGGL((batch_elementwise_kernel<4>), ge, dim3(256), 0, s, D.dOut, nullptr, nullptr, nullptr,
                            nullptr, D.dCar64, nullptr, nrhs);
@@ -2575,62 +2598,111 @@ int beat_autotune(beat_ctx* ctx, const beat_mesh* m, int precision, double k, in
   std::lock_guard<std::mutex> lock(ctx->mu);
   HIPCHK(hipSetDevice(ctx->device));
   const int64_t n = m->h.nv, nf = m->h.nf;
-  const int b = size_class(n);
+  Tuning& tu = tuned(ctx, precision, n);
+  constexpr int NJ = 8;  // jobs of the trial batches
   // synthetic drive: unit normal velocity on every face and right-hand side
-  std::vector<double> qh(2 * nf * nrhs * 3);
-  for (int64_t i = 0; i < nf * nrhs * 3; ++i) {
+  std::vector<double> qh(2 * nf * nrhs * NJ);
+  for (int64_t i = 0; i < nf * nrhs * NJ; ++i) {
     qh[2 * i] = 0.0;
     qh[2 * i + 1] = 1.0 + 0.01 * (double)(i % 7);
   }
   HIPCHK(ensure(ctx->qin, sizeof(double) * qh.size()));
-  HIPCHK(ensure(ctx->pout, sizeof(double) * 2 * n * nrhs * 3));
+  HIPCHK(ensure(ctx->pout, sizeof(double) * 2 * n * nrhs * NJ));
   HIPCHK(hipMemcpy(ctx->qin.p, qh.data(), sizeof(double) * qh.size(), hipMemcpyHostToDevice));
   const double eta_re = 0.0, eta_im = 1.0 / k;
-  auto run_solve = [&](double* best) -> int {
-    *best = INFINITY;
+  // one solve at a time: the assembly's and the factorization's times from the
+  // solve's own events, best of two after a warm-up
+  auto run_stages = [&](double* asm_best, double* fac_best) -> int {
+    *asm_best = *fac_best = INFINITY;
     for (int r = 0; r < 3; ++r) {
-      const auto t0 = std::chrono::steady_clock::now();
-      int rc = solve_impl(ctx, m, precision, k, eta_re, eta_im, as<double>(ctx->qin), (int)nrhs,
-                          as<double>(ctx->pout), nullptr, nullptr);
-      if (rc) return rc;
-      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      if (r > 0) *best = std::min(*best, dt);  // the first run warms up
+      beat_solve_report rep{};
+      if (int rc = solve_impl(ctx, m, precision, k, eta_re, eta_im, as<double>(ctx->qin), (int)nrhs,
+                              as<double>(ctx->pout), nullptr, &rep))
+        return rc;
+      if (r > 0) {
+        *asm_best = std::min(*asm_best, rep.assembly_s);
+        *fac_best = std::min(*fac_best, rep.factor_s);
+      }
     }
     return BEAT_OK;
   };
-  double t[4];
-  // factorization stream layout
-  for (int layout = 0; layout < 2; ++layout) {
-    ctx->lu_layout[b] = (signed char)layout;
-    if (layout == 0 && (!ctx->side || !ctx->inb)) {
-      t[layout] = INFINITY;
-      continue;
-    }
-    if (int rc = run_solve(&t[layout])) return rc;
+  // a batch of NJ (or 3) jobs on this mesh, wall time, best of two after a warm-up
+  std::vector<const beat_mesh*> ms(NJ, m);
+  std::vector<double> ks(NJ, k), ers(NJ, eta_re), eis(NJ, eta_im);
+  std::vector<int64_t> qo(NJ), po(NJ);
+  for (int i = 0; i < NJ; ++i) {
+    qo[i] = i * nf * nrhs;
+    po[i] = i * n * nrhs;
   }
-  ctx->lu_layout[b] = t[1] < t[0] ? 1 : 0;
-  // assembly occupancy inside a batch of three solves
-  const beat_mesh* ms[3] = {m, m, m};
-  const double ks[3] = {k, k, k}, ers[3] = {eta_re, eta_re, eta_re}, eis[3] = {eta_im, eta_im, eta_im};
-  const int64_t qo[3] = {0, nf * nrhs, 2 * nf * nrhs}, po[3] = {0, n * nrhs, 2 * n * nrhs};
-  for (int occ = 0; occ < 2; ++occ) {
-    ctx->asm_occ[b] = (signed char)occ;
-    double best = INFINITY;
-    for (int r = 0; r < 2; ++r) {
+  auto run_batch = [&](int nj, bool grouped, double* best) -> int {
+    *best = INFINITY;
+    for (int r = 0; r < 3; ++r) {
       const auto t0 = std::chrono::steady_clock::now();
-      int rc = batch_impl(ctx, 3, ms, precision, ks, ers, eis, as<double>(ctx->qin), qo, (int)nrhs,
-                          as<double>(ctx->pout), po, nullptr, nullptr);
+      int rc = grouped ? batch_grouped_impl(ctx, nj, ms.data(), precision, ks.data(), ers.data(), eis.data(),
+                                            as<double>(ctx->qin), qo.data(), (int)nrhs, as<double>(ctx->pout),
+                                            po.data(), nullptr, nullptr)
+                       : batch_impl(ctx, nj, ms.data(), precision, ks.data(), ers.data(), eis.data(),
+                                    as<double>(ctx->qin), qo.data(), (int)nrhs, as<double>(ctx->pout), po.data(),
+                                    nullptr, nullptr);
       if (rc) return rc;
       const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      if (r > 0) best = std::min(best, dt);
+      if (r > 0) *best = std::min(*best, dt);
     }
-    t[2 + occ] = best;
+    return BEAT_OK;
+  };
+  double t[4 + GEMM_TILES + 3];
+  for (double& v : t) v = INFINITY;
+  double unused;
+  // the regular pass's tiles: the assembly alone
+  for (int v = 0; v < 3; ++v) {
+    tu.asm_tile = (signed char)v;
+    if (int rc = run_stages(&t[4 + GEMM_TILES + v], &unused)) return rc;
   }
-  ctx->asm_occ[b] = t[3] < t[2] ? 1 : 0;
-  choice[0] = ctx->lu_layout[b];
-  choice[1] = ctx->asm_occ[b];
+  tu.asm_tile = (signed char)(std::min_element(t + 4 + GEMM_TILES, t + 4 + GEMM_TILES + 3) - (t + 4 + GEMM_TILES));
+  // factorization stream layout: the factorization alone
+  for (int layout = 0; layout < 2; ++layout) {
+    tu.lu_layout = (signed char)layout;
+    if (layout == 0 && (!ctx->side || !ctx->inb)) continue;
+    if (int rc = run_stages(&unused, &t[layout])) return rc;
+  }
+  tu.lu_layout = t[1] < t[0] ? 1 : 0;
+  // the trailing updates' tile shape: the factorization alone, or, for systems the
+  // grouped batches take, a grouped batch of NJ systems (its batched factorization)
+  const bool grouped = ctx->batch_grouped && ctx->cancel_path.empty() && n <= GROUP_NMAX && nrhs <= SOLVE_CHUNK;
+  for (int v = 0; v < GEMM_TILES; ++v) {
+    tu.gemm_tile = (signed char)v;
+    int rc = grouped ? run_batch(NJ, true, &t[4 + v]) : run_stages(&unused, &t[4 + v]);
+    if (rc) return rc;
+  }
+  tu.gemm_tile = (signed char)(std::min_element(t + 4, t + 4 + GEMM_TILES) - (t + 4));
+  // assembly occupancy inside a (per-job pipeline) batch of three solves
+  for (int occ = 0; occ < 2; ++occ) {
+    tu.asm_occ = (signed char)occ;
+    if (int rc = run_batch(3, false, &t[2 + occ])) return rc;
+  }
+  tu.asm_occ = t[3] < t[2] ? 1 : 0;
+  choice[0] = tu.lu_layout;
+  choice[1] = tu.asm_occ;
+  choice[2] = tu.gemm_tile;
+  choice[3] = tu.asm_tile;
   if (timings)
-    for (int i = 0; i < 4; ++i) timings[i] = t[i];
+    for (int i = 0; i < 4 + GEMM_TILES + 3; ++i) timings[i] = t[i];
+  return BEAT_OK;
+}
+
+int beat_ctx_set_tuning(beat_ctx* ctx, int precision, int64_t nv, const int* choice) {
+  if (!ctx) return BEAT_ERR_ARG;
+  if (nv < 1 || !choice) return fail(ctx, BEAT_ERR_ARG, "bad arguments");
+  if (int rc = check_solve_options(ctx, precision, nullptr)) return rc;
+  const int lim[4] = {2, 2, GEMM_TILES, 3};
+  for (int i = 0; i < 4; ++i)
+    if (choice[i] < -1 || choice[i] >= lim[i]) return fail(ctx, BEAT_ERR_ARG, "tuning choice out of range");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  Tuning& tu = tuned(ctx, precision, nv);
+  tu.lu_layout = (signed char)choice[0];
+  tu.asm_occ = (signed char)choice[1];
+  tu.gemm_tile = (signed char)choice[2];
+  tu.asm_tile = (signed char)choice[3];
   return BEAT_OK;
 }
 
@@ -2698,8 +2770,7 @@ int beat_bm_batch(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, 
     a->regular_symmetric = ctx->regular_symmetric;
     a->fp32_assembly_fp64 = ctx->fp32_assembly_fp64;
     a->sweep_asm_one_per_cu = ctx->sweep_asm_one_per_cu;
-    std::memcpy(a->lu_layout, ctx->lu_layout, sizeof(ctx->lu_layout));
-    std::memcpy(a->asm_occ, ctx->asm_occ, sizeof(ctx->asm_occ));
+    std::memcpy(a->tune, ctx->tune, sizeof(ctx->tune));
     cs.push_back(a);
   }
   // worker w takes jobs w, w + workers, ... (interleaved sizes)

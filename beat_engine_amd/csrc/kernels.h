@@ -144,6 +144,9 @@ hipError_t launch_downcast(const double* src, float* dst, int64_t n, hipStream_t
 // applied to that block's columns and every column right of it, never to the
 // columns of earlier blocks (the solves read those through gather tables).
 constexpr int LU_BLOCK_COLS = 256;
+// trailing-update tile shapes of the LU (LuWork::gemm_tile, lu_factor_batched)
+constexpr int GEMM_TILES = 4;
+
 struct LuWork {
   int* ipiv;          // n, global row index chosen at each column
   unsigned long long* slots;  // 2 x maxg slots of 16 words (two tagged halves of the candidate magnitude, padding)
@@ -175,6 +178,9 @@ struct LuWork {
   // exchange would have been made; the factors are then those of an unpivoted LU
   // and the caller factors again with nopiv false)
   bool nopiv = false;
+  // tile shape of the trailing updates (0..GEMM_TILES-1): 64 x 64-complex tiles with
+  // K chunks of 16 (0) or 32 (1), 128 x 64 with 16 (2) or 32 (3)
+  int gemm_tile = 0;
 };
 
 // Columns [n, nc) (n < nc <= lda) may carry right-hand sides: on return they
@@ -232,10 +238,11 @@ struct BatchVec {
 // Speculative LU (no exchanges, every column checked) of nsys systems at once,
 // leaf by leaf: per 64 columns a diagonal-block kernel, a rows-below kernel and
 // the two trailing products, each one launch for every system. nmax / ncmax: the
-// largest n and nc. Zero pivots and needed exchanges land in each status.
+// largest n and nc; tile: the trailing products' tile shape (0..GEMM_TILES-1).
+// Zero pivots and needed exchanges land in each status.
 template <typename T>
 hipError_t lu_factor_batched(const BatchSystem<T>* sys, int nsys, int64_t nmax, int64_t ncmax, bool gauss3,
-                             hipStream_t s);
+                             int tile, hipStream_t s);
 // With the factors of lu_factor_batched: forward (L y = v) and/or backward (U x = y)
 // substitution of each system's vector block in place, one workgroup per system.
 template <typename T>

@@ -767,14 +767,16 @@ __device__ __forceinline__ void zgemm_tile(const T* __restrict__ A, int64_t lda,
 
   constexpr int WR = BM / 2;  // rows per wave (two waves along the rows, two along the columns)
   if constexpr (sizeof(T) == 4 && G3) {
-    // each wave: 32 rows x 32 complex columns of T1, T2, T3; lane (h, l) takes
-    // step kk + h of k, row l of the wave's rows and complex column l of its columns
-    static_assert(BM == 64 && KB % 2 == 0, "G3 tiles: 64 rows, even K chunks");
-    f32x16 t1, t2, t3;
+    // each wave: WR rows (MI blocks of 32) x 32 complex columns of T1, T2, T3;
+    // lane (h, l) takes step kk + h of k, row l of each block and complex column l
+    static_assert(KB % 2 == 0, "G3 tiles: even K chunks");
+    constexpr int MI = WR / 32;
+    f32x16 t1[MI], t2[MI], t3[MI];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) t1[v] = t2[v] = t3[v] = 0.f;
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) t1[mi][v] = t2[mi][v] = t3[mi][v] = 0.f;
     const int h = lane >> 5, l32 = lane & 31;
-    const int arow = (wm * 32 + l32) * AP;
     const int jc = wn * 32 + l32;
     load(0);
     for (int k0 = 0; k0 < kc; k0 += KB) {
@@ -784,29 +786,35 @@ __device__ __forceinline__ void zgemm_tile(const T* __restrict__ A, int64_t lda,
 #pragma unroll 4
       for (int kk = 0; kk < KB; kk += 2) {
         const int kx = kk + h;
-        const float2 av = *reinterpret_cast<const float2*>(&As[arow + 2 * kx]);
         const float2 bv = *reinterpret_cast<const float2*>(&Bs[kx * BP + 2 * jc]);
-        t1 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bv.x, t1, 0, 0, 0);
-        t2 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bv.y, t2, 0, 0, 0);
-        t3 = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x + av.y, bv.x + bv.y, t3, 0, 0, 0);
+        const float bs = bv.x + bv.y;
+#pragma unroll
+        for (int mi = 0; mi < MI; ++mi) {
+          const float2 av = *reinterpret_cast<const float2*>(&As[(wm * WR + mi * 32 + l32) * AP + 2 * kx]);
+          t1[mi] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x, bv.x, t1[mi], 0, 0, 0);
+          t2[mi] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.y, bv.y, t2[mi], 0, 0, 0);
+          t3[mi] = __builtin_amdgcn_mfma_f32_32x32x2f32(av.x + av.y, bs, t3[mi], 0, 0, 0);
+        }
       }
       __syncthreads();
     }
     const int64_t gc = col0 + jc;
 #pragma unroll
-    for (int v = 0; v < 16; ++v) {
-      const int64_t gr = row0 + wm * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
-      if (gr < m && gc < n) {
-        const float cr = t1[v] - t2[v], ci = t3[v] - t1[v] - t2[v];
-        float2* dst = reinterpret_cast<float2*>(C + 2 * (gr * ldc + gc));
-        if (SET) {
-          *dst = make_float2(cr, ci);
-        } else {
-          const float2 o = *dst;
-          *dst = make_float2(o.x - cr, o.y - ci);
+    for (int mi = 0; mi < MI; ++mi)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int64_t gr = row0 + wm * WR + mi * 32 + 8 * (v >> 2) + 4 * h + (v & 3);
+        if (gr < m && gc < n) {
+          const float cr = t1[mi][v] - t2[mi][v], ci = t3[mi][v] - t1[mi][v] - t2[mi][v];
+          float2* dst = reinterpret_cast<float2*>(C + 2 * (gr * ldc + gc));
+          if (SET) {
+            *dst = make_float2(cr, ci);
+          } else {
+            const float2 o = *dst;
+            *dst = make_float2(o.x - cr, o.y - ci);
+          }
         }
       }
-    }
   } else if constexpr (sizeof(T) == 4) {
     constexpr int MI = WR / 32;
     f32x16 acc[MI][2];
@@ -923,22 +931,23 @@ __global__ __launch_bounds__(256) void zgemm_kernel(const T* __restrict__ A, int
 
 // Leaf t's trailing products for every system of a batch: UPPER, U12 = L11^-1 A12
 // in place over the columns [j0 + b, nc); else A22 -= L21 U12 over the same
-// columns and the rows below the leaf. blockIdx.z: the system.
-template <typename T, bool UPPER, bool G3>
+// columns and the rows below the leaf. blockIdx.z: the system; BM x 64-complex
+// tiles with K chunks of KB (the tile shape chosen per size class).
+template <typename T, bool UPPER, bool G3, int BM, int KB>
 __global__ __launch_bounds__(256) void zgemm_batched_kernel(const BatchSystem<T>* sys, int t) {
   const BatchSystem<T> S = sys[blockIdx.z];
   const int64_t j0 = (int64_t)t * PNB;
   if (j0 >= S.n) return;
   const int b = (int)min((int64_t)PNB, S.n - j0);
   const int64_t cols = S.nc - j0 - b, rows = UPPER ? b : S.n - j0 - b;
-  const int64_t row0 = (int64_t)blockIdx.y * 64, col0 = (int64_t)blockIdx.x * 64;
+  const int64_t row0 = (int64_t)blockIdx.y * BM, col0 = (int64_t)blockIdx.x * 64;
   if (cols <= 0 || rows <= 0 || row0 >= rows || col0 >= cols) return;
   T* const U12 = S.A + 2 * (j0 * S.lda + j0 + b);
   if (UPPER)
-    zgemm_tile<T, true, 64, 16, G3>(S.linv + 2 * (int64_t)t * PNB * PNB, PNB, U12, S.lda, U12, S.lda, rows, cols, b,
+    zgemm_tile<T, true, BM, KB, G3>(S.linv + 2 * (int64_t)t * PNB * PNB, PNB, U12, S.lda, U12, S.lda, rows, cols, b,
                                     row0, col0);
   else
-    zgemm_tile<T, false, 64, 16, G3>(S.A + 2 * ((j0 + b) * S.lda + j0), S.lda, U12, S.lda,
+    zgemm_tile<T, false, BM, KB, G3>(S.A + 2 * ((j0 + b) * S.lda + j0), S.lda, U12, S.lda,
                                      S.A + 2 * ((j0 + b) * S.lda + j0 + b), S.lda, rows, cols, b, row0, col0);
 }
 
@@ -1077,41 +1086,60 @@ __global__ __launch_bounds__(1024) void tri_wg_kernel(const BatchSystem<T>* sys,
     }
 }
 
+// The trailing-update tile shapes: BM rows x 64 complex columns, K staged KB at a
+// time. Shape 0 (64 x 16) is the default; the others are candidates for the
+// autotuner (beat_autotune), which keeps the fastest per size class.
+template <typename T, bool SET, bool G3, int BM, int KB>
+static void gemm_go(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n, int kc,
+                    hipStream_t s, bool behind) {
+  dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + BM - 1) / BM));
+  hipLaunchKernelGGL((zgemm_kernel<T, SET, BM, KB, G3>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc,
+                     behind ? 1 : 0);
+}
+
+template <typename T, bool SET, bool G3>
+static void gemm_shape(int tile, const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m,
+                       int64_t n, int kc, hipStream_t s, bool behind) {
+  switch (tile) {
+    case 1: gemm_go<T, SET, G3, 64, 32>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind); break;
+    case 2: gemm_go<T, SET, G3, 128, 16>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind); break;
+    case 3: gemm_go<T, SET, G3, 128, 32>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind); break;
+    default: gemm_go<T, SET, G3, 64, 16>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind); break;
+  }
+}
+
 template <typename T, bool SET>
 hipError_t gemm_launch(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
-                       int kc, hipStream_t s, bool behind, bool g3) {
+                       int kc, hipStream_t s, bool behind, bool g3, int tile) {
   if (m <= 0 || n <= 0 || kc <= 0) return hipSuccess;
+  if (tile < 0 || tile >= GEMM_TILES) return hipErrorInvalidValue;
   // in place (B == C): one row tile only, so no workgroup writes rows another reads
   if (static_cast<const void*>(B) == static_cast<const void*>(C) && (m > 64 || ldb != ldc)) return hipErrorInvalidValue;
-  // 64 x 64-complex tiles, K in chunks of 16. Measured on gfx950 at the LU's
-  // update shapes against 128-row tiles (chunks of 16 or 32), 256-row tiles and
-  // K <= 64 staged at once, 64-row tiles are the fastest for every shape in both
-  // precisions (complex64 ~100 TFLOP/s at m = n = 10^4, K = 256; complex128 ~42),
-  // and chunks of 16 keep the LDS small enough that a panel workgroup still fits
-  // on a CU that runs the update's workgroups at full occupancy.
-  dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64));
+  // Measured on gfx950 at the LU's update shapes of a full factorization, 64-row
+  // tiles with K chunks of 16 are the fastest in both precisions (complex64 ~100
+  // TFLOP/s at m = n = 10^4, K = 256; complex128 ~42), and keep the LDS small
+  // enough that a panel workgroup fits on a CU that runs the update's workgroups
+  // at full occupancy; the autotuner may choose another shape per size class.
   if constexpr (sizeof(T) == 4) {
     if (g3) {
-      hipLaunchKernelGGL((zgemm_kernel<T, SET, 64, 16, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc,
-                         behind ? 1 : 0);
+      gemm_shape<T, SET, true>(tile, A, lda, B, ldb, C, ldc, m, n, kc, s, behind);
       return hipGetLastError();
     }
   }
-  hipLaunchKernelGGL((zgemm_kernel<T, SET, 64, 16>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, m, n, kc,
-                     behind ? 1 : 0);
+  gemm_shape<T, SET, false>(tile, A, lda, B, ldb, C, ldc, m, n, kc, s, behind);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t gemm_sub(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
-                    int kc, hipStream_t s, bool behind = false, bool g3 = false) {
-  return gemm_launch<T, false>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind, g3);
+                    int kc, hipStream_t s, bool behind = false, bool g3 = false, int tile = 0) {
+  return gemm_launch<T, false>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind, g3, tile);
 }
 
 template <typename T>
 hipError_t gemm_set(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int64_t ldc, int64_t m, int64_t n,
-                    int kc, hipStream_t s, bool behind = false, bool g3 = false) {
-  return gemm_launch<T, true>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind, g3);
+                    int kc, hipStream_t s, bool behind = false, bool g3 = false, int tile = 0) {
+  return gemm_launch<T, true>(A, lda, B, ldb, C, ldc, m, n, kc, s, behind, g3, tile);
 }
 
 // ---- solves ----------------------------------------------------------------------
@@ -2002,6 +2030,8 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   if (ncol > lda) return hipErrorInvalidValue;
   const bool g3 = w.gauss3;  // complex64 updates in the three-real-product form
   const bool nopiv = w.nopiv;  // speculative leaves: no exchanges (row swaps are no-ops)
+  const int tile = w.gemm_tile;  // trailing-update tile shape
+  if (tile < 0 || tile >= GEMM_TILES) return hipErrorInvalidValue;
   hipError_t e;
   // zero-pivot report and wait control (info, abort word, give-up record)
   if ((e = launch_fill<int>(w.info, 4, 0, s)) != hipSuccess) return e;
@@ -2031,10 +2061,10 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
     hipError_t r;
     if (!nopiv && (r = launch_laswp<T>(A, lda, c0, c1, w.ipiv, j0, j0 + b, st)) != hipSuccess) return r;
     r = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + c0), lda, A + 2 * (j0 * lda + c0), lda, b, c1 - c0, b,
-                    st, false, g3);
+                    st, false, g3, tile);
     if (r != hipSuccess) return r;
     return gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + c0), lda, A + 2 * ((j0 + b) * lda + c0),
-                       lda, n - j0 - b, c1 - c0, b, st, false, g3);
+                       lda, n - j0 - b, c1 - c0, b, st, false, g3, tile);
   };
   // block [J, Je)'s update of columns [c0, c1): swaps, U12 by leaf inverses, GEMM
   auto update = [&](int64_t J, int64_t Je, int64_t c0, int64_t c1, hipStream_t st) -> hipError_t {
@@ -2045,16 +2075,16 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
     for (int64_t j0 = J; j0 < Je; j0 += nb) {
       const int b = (int)std::min<int64_t>(nb, Je - j0);
       r = gemm_set<T>(leaf_inv(j0, J), PNB, cA + 2 * (j0 * lda + c0), lda, A + 2 * (j0 * lda + c0), lda, b, c1 - c0,
-                      b, st, behind, g3);
+                      b, st, behind, g3, tile);
       if (r != hipSuccess) return r;
       if (j0 + b < Je) {
         r = gemm_sub<T>(cA + 2 * ((j0 + b) * lda + j0), lda, cA + 2 * (j0 * lda + c0), lda,
-                        A + 2 * ((j0 + b) * lda + c0), lda, Je - j0 - b, c1 - c0, b, st, behind, g3);
+                        A + 2 * ((j0 + b) * lda + c0), lda, Je - j0 - b, c1 - c0, b, st, behind, g3, tile);
         if (r != hipSuccess) return r;
       }
     }
     return gemm_sub<T>(cA + 2 * (Je * lda + J), lda, cA + 2 * (J * lda + c0), lda, A + 2 * (Je * lda + c0), lda,
-                       n - Je, c1 - c0, (int)(Je - J), st, behind, g3);
+                       n - Je, c1 - c0, (int)(Je - J), st, behind, g3, tile);
   };
   for (int64_t J = 0; J < n; J += bw) {
     const int64_t Je = std::min<int64_t>(n, J + bw);
@@ -2440,10 +2470,30 @@ template hipError_t lu_solve_transposed<float>(const float*, int64_t, int64_t, f
 template hipError_t lu_solve_transposed<double>(const double*, int64_t, int64_t, double*, double*, const double*,
                                                 hipStream_t);
 
+template <typename T, bool G3, int BM, int KB>
+static void gemm_batched_go(const BatchSystem<T>* sys, int nsys, int t, int64_t cols, int64_t below, hipStream_t s) {
+  const dim3 gu((unsigned)((cols + 63) / 64), 1, (unsigned)nsys);
+  const dim3 gs((unsigned)((cols + 63) / 64), (unsigned)std::max<int64_t>(1, (below + BM - 1) / BM), (unsigned)nsys);
+  hipLaunchKernelGGL((zgemm_batched_kernel<T, true, G3, BM, KB>), gu, dim3(256), 0, s, sys, t);
+  if (below > 0) hipLaunchKernelGGL((zgemm_batched_kernel<T, false, G3, BM, KB>), gs, dim3(256), 0, s, sys, t);
+}
+
+template <typename T, bool G3>
+static void gemm_batched_shape(int tile, const BatchSystem<T>* sys, int nsys, int t, int64_t cols, int64_t below,
+                               hipStream_t s) {
+  switch (tile) {
+    case 1: gemm_batched_go<T, G3, 64, 32>(sys, nsys, t, cols, below, s); break;
+    case 2: gemm_batched_go<T, G3, 128, 16>(sys, nsys, t, cols, below, s); break;
+    case 3: gemm_batched_go<T, G3, 128, 32>(sys, nsys, t, cols, below, s); break;
+    default: gemm_batched_go<T, G3, 64, 16>(sys, nsys, t, cols, below, s); break;
+  }
+}
+
 template <typename T>
 hipError_t lu_factor_batched(const BatchSystem<T>* sys, int nsys, int64_t nmax, int64_t ncmax, bool gauss3,
-                             hipStream_t s) {
+                             int tile, hipStream_t s) {
   if (nsys <= 0 || nmax <= 0) return hipSuccess;
+  if (tile < 0 || tile >= GEMM_TILES) return hipErrorInvalidValue;
   const int leaves = (int)((nmax + PNB - 1) / PNB);
   hipError_t e;
   for (int t = 0; t < leaves; ++t) {
@@ -2458,15 +2508,8 @@ hipError_t lu_factor_batched(const BatchSystem<T>* sys, int nsys, int64_t nmax, 
       if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (cols <= 0) continue;
-    const dim3 gu((unsigned)((cols + 63) / 64), 1, (unsigned)nsys);
-    const dim3 gs((unsigned)((cols + 63) / 64), (unsigned)std::max<int64_t>(1, (below + 63) / 64), (unsigned)nsys);
-    if (sizeof(T) == 4 && gauss3) {
-      hipLaunchKernelGGL((zgemm_batched_kernel<T, true, true>), gu, dim3(256), 0, s, sys, t);
-      if (below > 0) hipLaunchKernelGGL((zgemm_batched_kernel<T, false, true>), gs, dim3(256), 0, s, sys, t);
-    } else {
-      hipLaunchKernelGGL((zgemm_batched_kernel<T, true, false>), gu, dim3(256), 0, s, sys, t);
-      if (below > 0) hipLaunchKernelGGL((zgemm_batched_kernel<T, false, false>), gs, dim3(256), 0, s, sys, t);
-    }
+    if (sizeof(T) == 4 && gauss3) gemm_batched_shape<T, true>(tile, sys, nsys, t, cols, below, s);
+    else gemm_batched_shape<T, false>(tile, sys, nsys, t, cols, below, s);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;
@@ -2493,8 +2536,10 @@ hipError_t residual64_batched(const BatchSystem<double>* sys, const BatchVec<dou
   return hipGetLastError();
 }
 
-template hipError_t lu_factor_batched<float>(const BatchSystem<float>*, int, int64_t, int64_t, bool, hipStream_t);
-template hipError_t lu_factor_batched<double>(const BatchSystem<double>*, int, int64_t, int64_t, bool, hipStream_t);
+template hipError_t lu_factor_batched<float>(const BatchSystem<float>*, int, int64_t, int64_t, bool, int,
+                                            hipStream_t);
+template hipError_t lu_factor_batched<double>(const BatchSystem<double>*, int, int64_t, int64_t, bool, int,
+                                             hipStream_t);
 template hipError_t tri_solve_batched<float>(const BatchSystem<float>*, const BatchVec<float>*, int, int, bool, bool,
                                              hipStream_t);
 template hipError_t tri_solve_batched<double>(const BatchSystem<double>*, const BatchVec<double>*, int, int, bool, bool,

@@ -41,6 +41,12 @@ PRECISIONS = {
     "float32": _native.PREC_FP32,
 }
 
+# the autotuner's choices, in beat_autotune's numbering
+LU_LAYOUTS = ("lookahead", "one_stream")
+ASM_OCCUPANCIES = ("every_slot", "one_per_cu")
+GEMM_TILES = ("64x64_k16", "64x64_k32", "128x64_k16", "128x64_k32")
+REGULAR_TILES = ("unordered_16", "ordered_32", "ordered_16")
+
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
@@ -116,15 +122,27 @@ class Engine:
 
     # -- batching ---------------------------------------------------------------------
     def autotune(self, device_mesh: "DeviceMesh", precision="float32_refined", k=10.0, nrhs=8) -> dict:
-        """Times the launch strategies on this mesh's size class and keeps the faster."""
-        t = np.zeros(4)
-        ch = np.zeros(2, dtype=np.int32)
+        """Times the launch strategies and tile shapes on this mesh's size class in
+        this precision and keeps the fastest of each (beat_autotune)."""
+        t = np.zeros(11)
+        ch = np.zeros(4, dtype=np.int32)
         rc = self._lib.beat_autotune(self._h, device_mesh._h, _precision(precision), float(k), int(nrhs), _ptr(t),
                                      _ptr(ch))
         self._check(rc, "beat_autotune")
-        return dict(lu_layout=("lookahead", "one_stream")[int(ch[0])],
-                    batch_assembly=("every_slot", "one_per_cu")[int(ch[1])],
-                    seconds=dict(lookahead=t[0], one_stream=t[1], every_slot=t[2], one_per_cu=t[3]))
+        return dict(lu_layout=LU_LAYOUTS[int(ch[0])], batch_assembly=ASM_OCCUPANCIES[int(ch[1])],
+                    gemm_tile=GEMM_TILES[int(ch[2])], regular_tiles=REGULAR_TILES[int(ch[3])],
+                    seconds=dict(zip(LU_LAYOUTS + ASM_OCCUPANCIES + GEMM_TILES + REGULAR_TILES, t.tolist())))
+
+    def set_tuning(self, nv: int, precision="float32_refined", lu_layout=None, batch_assembly=None, gemm_tile=None,
+                   regular_tiles=None):
+        """Sets the tuned choices of a precision and the size class of nv (the names
+        autotune() reports; None: not tuned, the built-in default)."""
+        def code(v, names):
+            return -1 if v is None else names.index(v)
+        ch = np.array([code(lu_layout, LU_LAYOUTS), code(batch_assembly, ASM_OCCUPANCIES),
+                       code(gemm_tile, GEMM_TILES), code(regular_tiles, REGULAR_TILES)], dtype=np.int32)
+        self._check(self._lib.beat_ctx_set_tuning(self._h, _precision(precision), int(nv), _ptr(ch)),
+                    "beat_ctx_set_tuning")
 
     def batch(self, meshes, ks, couplings, qs, precision="float32_refined", max_steps=3, rtol=1e-10):
         """Solves job i = (meshes[i], ks[i], couplings[i], qs[i] (F_i x nrhs)) with the

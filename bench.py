@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--counter-run", action="store_true",
                     help="for rocprofv3 --pmc: a 1 GiB device copy (counter calibration), then one complex128 "
                          "freq-solve of the workload; prints no JSON line")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (built-in launch strategies and tile shapes)")
     ap.add_argument("--no-rocsolver", action="store_true",
                     help="skip timing rocSOLVER getrf (the vendor LU) at the workload's size")
     ap.add_argument("--no-extra-forms", action="store_true",
@@ -371,6 +373,17 @@ def main():
     p_mixed = torch.zeros((nrhs, mesh.num_vertices), dtype=torch.complex128, device=dev)
     p_fp64 = torch.zeros_like(p_mixed)
     torch.cuda.synchronize()
+    # the autotuner's launch strategies and tile shapes for this size, per precision
+    # (every path, the complex128 one included, then runs with its own fastest)
+    tuning = {}
+    if not args.no_autotune:
+        k_tune = 2 * math.pi * 1000.0 / 343.0
+        for name in ("float32_refined", "float64", "float32"):
+            t = time.perf_counter()
+            ch = eng.autotune(dm, name, k_tune, nrhs)
+            tuning[name] = {kk: v for kk, v in ch.items() if kk != "seconds"}
+            log(f"autotune {name} ({time.perf_counter() - t:.1f} s): {tuning[name]} "
+                f"{ {kk: round(v * 1e3, 2) for kk, v in ch['seconds'].items()} } ms")
 
     def step(i, prec, out):
         f = freqs[i % len(freqs)]
@@ -544,6 +557,8 @@ def main():
             "complex_product": "three real products (refined path)",
         },
     }
+    if tuning:
+        out["autotune"] = tuning
     out.update(extra)
     if not args.no_rocsolver:
         rs = rocsolver_getrf_ms(n_p1)

@@ -211,17 +211,35 @@ int beat_bm_sweep(beat_ctx* ctx, const beat_mesh* mesh, int precision, int64_t n
                   const double* eta_re, const double* eta_im, const double* q, int64_t nrhs, double* pressure_out,
                   const beat_solve_options* opts, beat_solve_report* reps);
 
-/* Autotuner (no counterpart in the reference): times, on this mesh's size class
- * (floor(log2 nv)) and precision, the factorization with the look-ahead streams
- * against one stream, and a three-solve batch with the next assembly at one
- * workgroup per CU against every slot, and keeps the faster of each in the
- * context for later solves, sweeps and batches of that size class. k: a
- * wavenumber for the trial solves. timings (may be NULL): 4 seconds
- * (look-ahead, one stream, every slot, one per CU; INFINITY when not available);
- * choice[0..1] = stream layout (0 look-ahead, 1 one stream), occupancy (0 every
- * slot, 1 one workgroup per CU). */
+/* Autotuner (no counterpart in the reference): times the launch strategies and
+ * tile shapes on this mesh's size class (floor(log2 nv)) in this precision, keeps
+ * the fastest of each in the context for later solves, sweeps and batches of that
+ * precision and size class, and reports them. In order:
+ *   - the regular pass's tiles, by the assembly's own time: unordered pairs over
+ *     16-face sub-tiles (0), ordered pairs over 32-face trial tiles (1) or over
+ *     16-face trial tiles (2);
+ *   - the factorization's streams, by its own time: look-ahead streams (0) or one
+ *     stream (1);
+ *   - the LU's trailing-update tile shape: 64 x 64-complex tiles with K chunks of
+ *     16 (0) or 32 (1), 128 x 64 with 16 (2) or 32 (3), by the factorization's time,
+ *     or for systems the grouped batches take (nv <= 4096, nrhs <= 8) by the time
+ *     of a grouped batch of 8 solves on this mesh (its batched factorization);
+ *   - the next assembly's occupancy in a three-solve batch: every slot (0) or one
+ *     workgroup per CU (1).
+ * k: a wavenumber for the trial solves. timings (may be NULL): 11 seconds, best of
+ * two timed repeats after a warm-up (INFINITY when not available): [0..1] stream
+ * layouts, [2..3] occupancies, [4..7] trailing-update tile shapes, [8..10]
+ * regular-pass tiles. choice[0..3] = stream layout, occupancy, trailing-update
+ * tile shape, regular-pass tiles. */
 int beat_autotune(beat_ctx* ctx, const beat_mesh* mesh, int precision, double k, int64_t nrhs, double* timings,
                   int* choice);
+
+/* Sets the tuned choices of a precision and the size class of nv, in
+ * beat_autotune's numbering (choice[0..3] = stream layout, occupancy,
+ * trailing-update tile shape, regular-pass tiles; -1: not tuned, the built-in
+ * default), for instance to apply what an earlier process's autotuner chose (no
+ * counterpart in the reference). */
+int beat_ctx_set_tuning(beat_ctx* ctx, int precision, int64_t nv, const int* choice);
 
 /* A batch of solves over several meshes (batching form: many meshes of uneven
  * size, each at its frequencies): job i solves meshes[i] at k[i], eta[i] with the

@@ -63,7 +63,8 @@ def test_autotune_then_batch(jobs):
     m, dm, k, eta, q = J[0]
     choice = eng.autotune(dm, "float32_refined", k, 4)
     print("autotune", choice, file=sys.stderr, flush=True)
-    assert choice["lu_layout"] in ("lookahead", "one_stream")
+    assert choice["lu_layout"] in exterior.LU_LAYOUTS and choice["batch_assembly"] in exterior.ASM_OCCUPANCIES
+    assert choice["gemm_tile"] in exterior.GEMM_TILES and choice["regular_tiles"] in exterior.REGULAR_TILES
     assert all(np.isfinite(v) or v == np.inf for v in choice["seconds"].values())
     P, info = eng.batch([dm, dm], [k, k], [eta, eta], [q, q], "float32_refined")
     single, _ = dm.solve(k, eta, q, "float32_refined")
