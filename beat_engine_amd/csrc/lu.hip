@@ -127,9 +127,19 @@ __device__ __forceinline__ void st_tagged(unsigned long long* q, double re, doub
   st_u64(q + 2, t | (b & 0xffffffffull));
   st_u64(q + 3, t | (b >> 32));
 }
+// The element of TW loaded words (TaggedRow), zero words giving zero.
+__device__ __forceinline__ void untag(const unsigned long long* w, float& re, float& im) {
+  re = __uint_as_float((unsigned)w[0]);
+  im = __uint_as_float((unsigned)w[1]);
+}
+__device__ __forceinline__ void untag(const unsigned long long* w, double& re, double& im) {
+  re = __longlong_as_double((long long)((w[1] << 32) | (w[0] & 0xffffffffull)));
+  im = __longlong_as_double((long long)((w[3] << 32) | (w[2] & 0xffffffffull)));
+}
 // Loads one published element, re-reading until every word carries the tag;
 // false when the wait gave up (the element is then zero).
-__device__ __forceinline__ bool ld_tagged(const unsigned long long* q, float& re, float& im, unsigned tag, int* ctl) {
+__device__ __forceinline__ bool ld_tagged(const unsigned long long* q, float& re, float& im, unsigned tag, int* ctl,
+                                          int site = SPIN_PANEL_ROW) {
   Spin sp;
   for (;;) {
     const unsigned long long a = ld_u64(q), b = ld_u64(q + 1);
@@ -138,14 +148,15 @@ __device__ __forceinline__ bool ld_tagged(const unsigned long long* q, float& re
       im = __uint_as_float((unsigned)b);
       return true;
     }
-    if (spin_expired(ctl, sp, SPIN_PANEL_ROW)) {
+    if (spin_expired(ctl, sp, site)) {
       re = im = 0.f;
       return false;
     }
     __builtin_amdgcn_s_sleep(1);
   }
 }
-__device__ __forceinline__ bool ld_tagged(const unsigned long long* q, double& re, double& im, unsigned tag, int* ctl) {
+__device__ __forceinline__ bool ld_tagged(const unsigned long long* q, double& re, double& im, unsigned tag, int* ctl,
+                                          int site = SPIN_PANEL_ROW) {
   Spin sp;
   for (;;) {
     const unsigned long long a = ld_u64(q), b = ld_u64(q + 1), c = ld_u64(q + 2), d = ld_u64(q + 3);
@@ -155,7 +166,7 @@ __device__ __forceinline__ bool ld_tagged(const unsigned long long* q, double& r
       im = __longlong_as_double((long long)((d << 32) | (c & 0xffffffffull)));
       return true;
     }
-    if (spin_expired(ctl, sp, SPIN_PANEL_ROW)) {
+    if (spin_expired(ctl, sp, site)) {
       re = im = 0.0;
       return false;
     }
@@ -1234,6 +1245,57 @@ __global__ __launch_bounds__(64) void diag_inverse_kernel(const T* LU, int64_t n
     }
 }
 
+// Link matrices of the block triangular solves: forward W_i = Linv_i L_i,i-1 (rows
+// of L read in final order through the gather table of leaf i-1's outer block),
+// backward W_i = Uinv_i U_i,i+1; zero for the chain's first leaf and beyond the
+// blocks' extents. blockIdx.x: the leaf, blockIdx.y: 0 forward, 1 backward. The
+// neighbour block is staged in LDS; thread t forms row t/4, columns 16 (t%4) + u.
+template <typename T>
+__global__ __launch_bounds__(256) void link_matrix_kernel(const T* LU, int64_t n, int64_t lda, const int* G, int bw,
+                                                          int np, const T* Linv, const T* Uinv, T* WL, T* WU) {
+  __shared__ T Ms[2 * PNB * (PNB + 1)];
+  const int i = blockIdx.x;
+  const bool fwd = blockIdx.y == 0;
+  const int t = threadIdx.x, rl = t >> 2, cq = t & 3;
+  const int64_t base = (int64_t)i * PNB;
+  const int nbi = (int)min((int64_t)PNB, n - base);
+  T* W = (fwd ? WL : WU) + 2 * (int64_t)i * PNB * PNB;
+  const bool has = fwd ? i > 0 : i + 1 < np;
+  const int nb = (int)(fwd ? (has ? PNB : 0) : (has ? min((int64_t)PNB, n - base - PNB) : 0));
+  const int64_t c0 = fwd ? base - PNB : base + PNB;  // the neighbour's first column
+  for (int idx = t; idx < PNB * PNB; idx += 256) {
+    const int q = idx / PNB, c = idx % PNB;
+    T vr = 0, vi = 0;
+    if (has && q < nbi && c < nb) {
+      const int64_t row = fwd ? (int64_t)G[(c0 / bw) * n + base + q] : base + q;
+      vr = LU[2 * (row * lda + c0 + c)];
+      vi = LU[2 * (row * lda + c0 + c) + 1];
+    }
+    Ms[2 * (q * (PNB + 1) + c)] = vr;
+    Ms[2 * (q * (PNB + 1) + c) + 1] = vi;
+  }
+  __syncthreads();
+  T wr[16], wi[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) wr[u] = wi[u] = 0;
+  const T* D = (fwd ? Linv : Uinv) + 2 * ((int64_t)i * PNB * PNB + (int64_t)rl * PNB);
+  if (has && rl < nbi)
+    for (int q = 0; q < nbi; ++q) {
+      const T xr = D[2 * q], xi = D[2 * q + 1];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const T mr = Ms[2 * (q * (PNB + 1) + 16 * cq + u)], mi = Ms[2 * (q * (PNB + 1) + 16 * cq + u) + 1];
+        wr[u] += xr * mr - xi * mi;
+        wi[u] += xr * mi + xi * mr;
+      }
+    }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    W[2 * (rl * PNB + 16 * cq + u)] = wr[u];
+    W[2 * (rl * PNB + 16 * cq + u) + 1] = wi[u];
+  }
+}
+
 // Row gather tables of the forward solve. The L columns of outer block J are
 // stored in the row order after block J's own exchanges (later blocks never
 // swap them), while the solve runs in the final order. For final row r,
@@ -1301,16 +1363,57 @@ __global__ void gather_rhs_kernel(const T* src, int64_t n, int nrhs, const int* 
   dst[2 * idx + 1] = im;
 }
 
+// Slot u (0..15) of column quarter kq in a PNB-wide block: pairs of adjacent
+// columns interleaved over the four quarters, c = 8 (u / 2) + 2 kq + u % 2, so that
+// the four threads of a row read 64 (complex64) or 128 (complex128) contiguous
+// bytes per vector load instead of each touching its own cache line.
+__device__ __forceinline__ int qcol(int u, int kq) { return 8 * (u >> 1) + 2 * kq + (u & 1); }
+
+// This thread's 16 entries of a block row in slot order (row: the block's first
+// column, 16-byte aligned); columns at or beyond ncols, and all when !ok, are zero.
+__device__ __forceinline__ void load_row16(const float* row, int ncols, int kq, bool ok, float* vr, float* vi) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = 8 * j + 2 * kq;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ok && c + 1 < ncols) {
+      v = *reinterpret_cast<const float4*>(row + 2 * c);
+    } else if (ok && c < ncols) {
+      v.x = row[2 * c];
+      v.y = row[2 * c + 1];
+    }
+    vr[2 * j] = v.x;
+    vi[2 * j] = v.y;
+    vr[2 * j + 1] = v.z;
+    vi[2 * j + 1] = v.w;
+  }
+}
+__device__ __forceinline__ void load_row16(const double* row, int ncols, int kq, bool ok, double* vr, double* vi) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = 8 * j + 2 * kq;
+    double2 v0 = make_double2(0.0, 0.0), v1 = make_double2(0.0, 0.0);
+    if (ok && c < ncols) v0 = *reinterpret_cast<const double2*>(row + 2 * c);
+    if (ok && c + 1 < ncols) v1 = *reinterpret_cast<const double2*>(row + 2 * c + 2);
+    vr[2 * j] = v0.x;
+    vi[2 * j] = v0.y;
+    vr[2 * j + 1] = v1.x;
+    vi[2 * j + 1] = v1.y;
+  }
+}
+
 template <typename T>
 struct TriArgs {
   const T* LU;
   int64_t n, lda;
   const T* dinv;      // per leaf PNB x PNB: Linv (forward) or Uinv (backward)
+  const T* wlink;     // per leaf PNB x PNB: the link to the previous (forward) / next (backward) leaf
   const int* gather;  // forward: outer-block row gather tables (nJ x n)
   int bw;             // outer block width of the factorization
   const T* src;       // n x 8: forward P b, backward y
   T* pub;             // n x 8: block results, written with coherent stores for the other workgroups
   int* flags;         // per leaf: epoch once the leaf's result is published
+  unsigned long long* tpub;  // n x 8 results as epoch-tagged words (TaggedRow), read by the next leaf on the chain
   int* ctl;           // wait control of the factorization (abort word, give-up record)
   int epoch;
   int np;
@@ -1320,19 +1423,26 @@ struct TriArgs {
 
 // Persistent block triangular solve with 8 right-hand sides (zero-padded): one
 // workgroup per leaf of PNB rows, all resident (cooperative launch). Forward
-// (L y = P b) workgroup i consumes the results of leaves 0..i-1 in order as
-// their flags appear, backward (U x = y) those of leaves np-1..i+1; each adds
-// -M_ik z_k to its rows, then forms its own result with the leaf's diagonal
-// inverse and publishes it. Thread t owns row t/4 and the column quarter t%4 of
-// every PNB-wide block (partial sums reduced over the four threads once per
-// leaf). The matrix segment of the next block is loaded before its flag is
-// awaited, so only the flag latency and the block product sit on the chain.
+// (L y = P b) workgroup i consumes the results of leaves 0..i-1 in order as their
+// flags appear, backward (U x = y) those of leaves np-1..i+1. Every leaf but the
+// neighbour on the chain (i-1 forward, i+1 backward) is folded into
+// z_i = D_i (src_i - sum_k M_ik z_k) while the chain is still further away; when
+// the neighbour's result n arrives, the leaf's own is z_i - W_i n with the link
+// matrix W_i = D_i M_i,nb formed at solve preparation, so one block product sits
+// on the chain per leaf. A result is published twice: as epoch-tagged words,
+// which the next leaf on the chain polls directly (no flag round trip on the
+// chain), and as plain words announced by a flag after the stores have completed,
+// which every later leaf reads. Thread t owns row t/4 and the column quarter t%4
+// (slots qcol) of every PNB-wide block (partial sums reduced over the four
+// threads). The matrix segment of the next block is loaded before its flag is
+// awaited.
 template <typename T, bool FWD>
 __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   // in a sweep the factorization and solves run beside the next system's
   // assembly: their waves issue ahead of the assembly's
   __builtin_amdgcn_s_setprio(2);
   constexpr int YS = PNB * 8 + 6;  // staged block result; 2-complex pad per 16-row group
+  constexpr int TW = TaggedRow<T>::TW;
   __shared__ T ys[2 * YS];
   __shared__ T bs[2 * PNB * 8];
   __shared__ int gave_up;
@@ -1343,41 +1453,36 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   const int nbi = (int)min((int64_t)PNB, a.n - base);
   const int64_t r = base + rl;
   const bool rowok = rl < nbi;
-  // this thread's quarter of its row of the diagonal-block inverse
-  T dr[16], di[16];
-  const T* D = a.dinv + 2 * ((int64_t)i * PNB * PNB + (int64_t)rl * PNB + 16 * kq);
-#pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const bool ok = rowok && 16 * kq + u < nbi;
-    dr[u] = ok ? D[2 * u] : T(0);
-    di[u] = ok ? D[2 * u + 1] : T(0);
-  }
+  const int nk = FWD ? i : a.np - 1 - i;  // leaves before this one on the chain
+  const int site = FWD ? SPIN_TRI_FWD : SPIN_TRI_BWD;
   T accr[8], acci[8];
 #pragma unroll
   for (int d = 0; d < 8; ++d) {
     accr[d] = (kq == 0 && rowok) ? a.src[2 * (r * 8 + d)] : T(0);
     acci[d] = (kq == 0 && rowok) ? a.src[2 * (r * 8 + d) + 1] : T(0);
   }
-  const int nk = FWD ? i : a.np - 1 - i;
-  for (int s = 0; s < nk; ++s) {
-    const int k = FWD ? s : a.np - 1 - s;
-    const int64_t kb = (int64_t)k * PNB;
-    const int nbk = (int)min((int64_t)PNB, a.n - kb);
-    T mr[16], mi[16];
-    {
-      const int64_t srow = FWD ? (rowok ? (int64_t)a.gather[(kb / a.bw) * a.n + r] : 0) : r;
-      const T* M = a.LU + 2 * (srow * a.lda + kb + 16 * kq);
-#pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const bool ok = rowok && 16 * kq + u < nbk;
-        mr[u] = ok ? M[2 * u] : T(0);
-        mi[u] = ok ? M[2 * u + 1] : T(0);
-      }
-    }
+  // this thread's slots of its row of the diagonal-block inverse and of the link:
+  // complex64 at once (the registers are there), complex128 once the earlier
+  // leaves are folded in
+  T dr[16], di[16], wr[16], wi[16];
+  auto load_dw = [&]() {
+    const int64_t o = 2 * ((int64_t)i * PNB * PNB + (int64_t)rl * PNB);
+    const int nbn = nk > 0 ? (int)min((int64_t)PNB, a.n - (int64_t)(FWD ? i - 1 : i + 1) * PNB) : 0;
+    load_row16(a.dinv + o, nbi, kq, rowok, dr, di);
+    load_row16(a.wlink + o, nbn, kq, rowok, wr, wi);
+  };
+  if constexpr (sizeof(T) == 4) load_dw();
+  auto stage = [&](int64_t kb, int kk, int d, T zr, T zi) {
+    const int idx = kk * 8 + d;
+    ys[2 * (idx + (kk >> 4) * 2)] = zr;
+    ys[2 * (idx + (kk >> 4) * 2) + 1] = zi;
+  };
+  // waits for leaf k's flag, then stages its published result in ys
+  auto await_stage = [&](int k) -> bool {
     if (t == 0) {
       Spin sp;
       while (ld_flag(a.flags + k) != a.epoch) {
-        if (spin_expired(a.ctl, sp, FWD ? SPIN_TRI_FWD : SPIN_TRI_BWD)) {
+        if (spin_expired(a.ctl, sp, site)) {
           gave_up = 1;
           break;
         }
@@ -1385,18 +1490,60 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
       }
     }
     __syncthreads();
-    if (gave_up) return;  // uniform after the barrier; the host reports the give-up
+    if (gave_up) return false;  // uniform after the barrier; the host reports the give-up
+    const int64_t kb = (int64_t)k * PNB;
+    const int nbk = (int)min((int64_t)PNB, a.n - kb);
     for (int idx = t; idx < PNB * 8; idx += 256) {
       const int kk = idx >> 3, d = idx & 7;
       T zr = 0, zi = 0;
       if (kk < nbk) ld_cplx(a.pub + 2 * ((kb + kk) * 8 + d), zr, zi);
-      ys[2 * (idx + (kk >> 4) * 2)] = zr;
-      ys[2 * (idx + (kk >> 4) * 2) + 1] = zi;
+      stage(kb, kk, d, zr, zi);
     }
     __syncthreads();
+    return true;
+  };
+  // stages the chain neighbour's tagged result: each thread polls the words of its
+  // two entries (rows t/8 and t/8 + 32, right-hand side t%8) until all carry the epoch
+  auto await_tagged = [&](int k) -> bool {
+    const int64_t kb = (int64_t)k * PNB;
+    const int nbk = (int)min((int64_t)PNB, a.n - kb);
+    const int kk0 = t >> 3, kk1 = kk0 + 32, d = t & 7;
+    const bool v0 = kk0 < nbk, v1 = kk1 < nbk;
+    const unsigned long long* q0 = a.tpub + ((kb + kk0) * 8 + d) * TW;
+    const unsigned long long* q1 = a.tpub + ((kb + kk1) * 8 + d) * TW;
+    const unsigned tag = (unsigned)a.epoch;
+    unsigned long long w0[TW], w1[TW];
+    Spin sp;
+    for (;;) {
+      bool ok = true;
+#pragma unroll
+      for (int e = 0; e < TW; ++e) {
+        w0[e] = v0 ? ld_u64(q0 + e) : 0ull;
+        w1[e] = v1 ? ld_u64(q1 + e) : 0ull;
+      }
+#pragma unroll
+      for (int e = 0; e < TW; ++e) ok = ok && (!v0 || (unsigned)(w0[e] >> 32) == tag) && (!v1 || (unsigned)(w1[e] >> 32) == tag);
+      if (ok) break;
+      if (spin_expired(a.ctl, sp, site)) {
+        gave_up = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    T zr, zi;
+    untag(w0, zr, zi);
+    stage(kb, kk0, d, zr, zi);
+    untag(w1, zr, zi);
+    stage(kb, kk1, d, zr, zi);
+    __syncthreads();
+    return !gave_up;
+  };
+  // acc -= M[rows, quarter] ys[quarter]
+  auto apply = [&](const T* mr, const T* mi) {
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-      const int o = (16 * kq + u) * 8 + kq * 2;
+      const int kk = qcol(u, kq);
+      const int o = kk * 8 + (kk >> 4) * 2;
 #pragma unroll
       for (int d = 0; d < 8; ++d) {
         const T zr = ys[2 * (o + d)], zi = ys[2 * (o + d) + 1];
@@ -1404,15 +1551,32 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
         acci[d] -= mr[u] * zi + mi[u] * zr;
       }
     }
-  }
-  // full row sums over the four threads of the row, then the diagonal block
+  };
+  auto row_reduce = [&]() {
 #pragma unroll
-  for (int d = 0; d < 8; ++d) {
-    accr[d] += __shfl_xor(accr[d], 1, 64);
-    acci[d] += __shfl_xor(acci[d], 1, 64);
-    accr[d] += __shfl_xor(accr[d], 2, 64);
-    acci[d] += __shfl_xor(acci[d], 2, 64);
+    for (int d = 0; d < 8; ++d) {
+      accr[d] += __shfl_xor(accr[d], 1, 64);
+      acci[d] += __shfl_xor(acci[d], 1, 64);
+      accr[d] += __shfl_xor(accr[d], 2, 64);
+      acci[d] += __shfl_xor(acci[d], 2, 64);
+    }
+  };
+  // off the chain: every leaf but the neighbour
+  for (int s = 0; s + 1 < nk; ++s) {
+    const int k = FWD ? s : a.np - 1 - s;
+    const int64_t kb = (int64_t)k * PNB;
+    const int nbk = (int)min((int64_t)PNB, a.n - kb);
+    T mr[16], mi[16];
+    {
+      const int64_t srow = FWD ? (rowok ? (int64_t)a.gather[(kb / a.bw) * a.n + r] : 0) : r;
+      load_row16(a.LU + 2 * (srow * a.lda + kb), nbk, kq, rowok, mr, mi);
+    }
+    if (!await_stage(k)) return;
+    apply(mr, mi);
   }
+  if constexpr (sizeof(T) == 8) load_dw();
+  // z = D acc: full row sums over the four threads, then the diagonal block
+  row_reduce();
   if (kq == 0)
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
@@ -1424,7 +1588,7 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   for (int d = 0; d < 8; ++d) accr[d] = acci[d] = 0;
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
-    const int q = 16 * kq + u;
+    const int q = qcol(u, kq);
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
       const T br = bs[2 * (q * 8 + d)], bi = bs[2 * (q * 8 + d) + 1];
@@ -1432,14 +1596,18 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
       acci[d] += dr[u] * bi + di[u] * br;
     }
   }
-#pragma unroll
-  for (int d = 0; d < 8; ++d) {
-    accr[d] += __shfl_xor(accr[d], 1, 64);
-    acci[d] += __shfl_xor(acci[d], 1, 64);
-    accr[d] += __shfl_xor(accr[d], 2, 64);
-    acci[d] += __shfl_xor(acci[d], 2, 64);
+  // on the chain: each thread holds its quarter's part of z; it subtracts its
+  // quarter's part of W n, and the row sums then give z - W n
+  if (nk > 0) {
+    if (!await_tagged(FWD ? i - 1 : i + 1)) return;
+    apply(wr, wi);
   }
-  // publish: thread kq stores columns 2kq, 2kq+1 of its row
+  row_reduce();
+  // publish: thread kq stores columns 2kq, 2kq+1 of its row, the tagged words first
+  if (rowok)
+#pragma unroll
+    for (int d = 0; d < 8; ++d)
+      if ((d >> 1) == kq) st_tagged(a.tpub + (r * 8 + d) * TW, accr[d], acci[d], (unsigned)a.epoch);
   if (rowok)
 #pragma unroll
     for (int d = 0; d < 8; ++d)
@@ -1454,6 +1622,8 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   __syncthreads();
   if (t == 0) st_flag(a.flags + i, a.epoch);
 }
+
+
 
 // r = b - A x, complex128, 8 right-hand sides (zero-padded). Reading A once is
 // the cost, so lanes run along the row (coalesced 1 KB per wavefront load): each
@@ -2145,7 +2315,7 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
 // from (nJ x n).
 struct SolveLayout {
   int np, nJ;
-  size_t linv, uinv, b0, yf, xb, cend;  // complex offsets
+  size_t linv, uinv, b0, yf, xb, wl, wu, tf, tb, cend;  // complex offsets
 };
 static SolveLayout solve_layout(int64_t n) {
   SolveLayout L;
@@ -2156,7 +2326,12 @@ static SolveLayout solve_layout(int64_t n) {
   L.b0 = L.uinv + (size_t)L.np * PNB * PNB;
   L.yf = L.b0 + (size_t)n * 8;
   L.xb = L.yf + (size_t)n * 8;
-  L.cend = L.xb + (size_t)n * 8;
+  L.wl = L.xb + (size_t)n * 8;
+  L.wu = L.wl + (size_t)L.np * PNB * PNB;
+  // tagged results (TaggedRow: twice a complex entry's size), forward and backward
+  L.tf = L.wu + (size_t)L.np * PNB * PNB;
+  L.tb = L.tf + (size_t)2 * n * 8;
+  L.cend = L.tb + (size_t)2 * n * 8;
   return L;
 }
 
@@ -2194,6 +2369,14 @@ hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv
   hipLaunchKernelGGL(gather_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const int*)R, n, L.nJ, G,
                      perm);
   if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL((link_matrix_kernel<T>), dim3((unsigned)L.np, 2), dim3(256), 0, s, LU, n, lda, (const int*)G,
+                     MAXBW, L.np, (const T*)(work + 2 * L.linv), (const T*)(work + 2 * L.uinv), work + 2 * L.wl,
+                     work + 2 * L.wu);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // tagged results: no word may carry a later solve's epoch by accident
+  if ((e = launch_fill<int>(reinterpret_cast<int*>(work + 2 * L.tf), (int64_t)4 * 2 * n * 8 * sizeof(T) / sizeof(int),
+                            0, s)) != hipSuccess)
+    return e;
   return launch_fill<int>(flags, 2 * (int64_t)L.np, 0, s);
 }
 
@@ -2221,19 +2404,23 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work
   ta.out = B;
   // forward
   ta.dinv = work + 2 * L.linv;
+  ta.wlink = work + 2 * L.wl;
   ta.gather = G;
   ta.src = work + 2 * L.b0;
   ta.pub = work + 2 * L.yf;
   ta.flags = flags;
+  ta.tpub = reinterpret_cast<unsigned long long*>(work + 2 * L.tf);
   e = launch_resident<TriArgs<T>>(reinterpret_cast<const void*>(&tri_solve_kernel<T, true>), tri_solve_kernel<T, true>,
                                   L.np, 256, ta, s);
   if (e != hipSuccess) return e;
   // backward
   ta.dinv = work + 2 * L.uinv;
+  ta.wlink = work + 2 * L.wu;
   ta.gather = nullptr;
   ta.src = work + 2 * L.yf;
   ta.pub = work + 2 * L.xb;
   ta.flags = flags + L.np;
+  ta.tpub = reinterpret_cast<unsigned long long*>(work + 2 * L.tb);
   return launch_resident<TriArgs<T>>(reinterpret_cast<const void*>(&tri_solve_kernel<T, false>),
                                      tri_solve_kernel<T, false>, L.np, 256, ta, s);
 }
@@ -2291,10 +2478,12 @@ hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nr
   ta.nrhs = nrhs;
   ta.out = B;
   ta.dinv = work + 2 * L.uinv;
+  ta.wlink = work + 2 * L.wu;
   ta.gather = nullptr;
   ta.src = work + 2 * L.yf;
   ta.pub = work + 2 * L.xb;
   ta.flags = flags + L.np;
+  ta.tpub = reinterpret_cast<unsigned long long*>(work + 2 * L.tb);
   return launch_resident<TriArgs<T>>(reinterpret_cast<const void*>(&tri_solve_kernel<T, false>),
                                      tri_solve_kernel<T, false>, L.np, 256, ta, s);
 }
