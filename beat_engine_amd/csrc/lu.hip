@@ -77,6 +77,64 @@ __device__ __forceinline__ void ld_cplx(const double* p, double& re, double& im)
 __device__ __forceinline__ void st_flag(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ __forceinline__ int ld_flag(const int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
+// Sum over the four lanes of a quad (DPP quad permutations [1,0,3,2] and
+// [2,3,0,1]; every lane receives the total).
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffull), CTRL, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename T>
+__device__ __forceinline__ T quad_sum(T v) {
+  v += dpp_move<0xB1>(v);
+  return v + dpp_move<0x4E>(v);
+}
+
+// Slot u (0..15) of column quarter kq in a PNB-wide block: pairs of adjacent
+// columns interleaved over the four quarters, c = 8 (u / 2) + 2 kq + u % 2, so that
+// the four threads of a row read 64 (complex64) or 128 (complex128) contiguous
+// bytes per vector load instead of each touching its own cache line.
+__device__ __forceinline__ int qcol(int u, int kq) { return 8 * (u >> 1) + 2 * kq + (u & 1); }
+
+// This thread's 16 entries of a block row in slot order (row: the block's first
+// column, 16-byte aligned); columns at or beyond ncols, and all when !ok, are zero.
+__device__ __forceinline__ void load_row16(const float* row, int ncols, int kq, bool ok, float* vr, float* vi) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = 8 * j + 2 * kq;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ok && c + 1 < ncols) {
+      v = *reinterpret_cast<const float4*>(row + 2 * c);
+    } else if (ok && c < ncols) {
+      v.x = row[2 * c];
+      v.y = row[2 * c + 1];
+    }
+    vr[2 * j] = v.x;
+    vi[2 * j] = v.y;
+    vr[2 * j + 1] = v.z;
+    vi[2 * j + 1] = v.w;
+  }
+}
+__device__ __forceinline__ void load_row16(const double* row, int ncols, int kq, bool ok, double* vr, double* vi) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = 8 * j + 2 * kq;
+    double2 v0 = make_double2(0.0, 0.0), v1 = make_double2(0.0, 0.0);
+    if (ok && c < ncols) v0 = *reinterpret_cast<const double2*>(row + 2 * c);
+    if (ok && c + 1 < ncols) v1 = *reinterpret_cast<const double2*>(row + 2 * c + 2);
+    vr[2 * j] = v0.x;
+    vi[2 * j] = v0.y;
+    vr[2 * j + 1] = v1.x;
+    vi[2 * j + 1] = v1.y;
+  }
+}
+
 // Bounded cross-workgroup waits. Every spin loop of the factorization and of the
 // triangular solves counts its rounds; every 64th round it reads the abort word
 // ctl[0] and the real-time counter (s_memrealtime, 100 MHz). A wait gives up when
@@ -1155,144 +1213,115 @@ hipError_t gemm_set(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int6
 
 // ---- solves ----------------------------------------------------------------------
 
-// Inverses of the unit-lower and upper diagonal blocks of every panel; thread c
-// forms column c of each inverse by substitution, the block and the partial
-// columns kept in LDS (reads of the block are broadcast, of the columns per lane).
+// Per leaf i of the block triangular solves, four triangular solves with the
+// leaf's diagonal block and 64 right-hand-side columns, one workgroup each
+// (blockIdx.y): the inverses Linv_i (L X = I) and Uinv_i (U X = I), and the link
+// matrices W_i = Linv_i L_i,i-1 (L W = L_i,i-1, rows of L in final order through
+// the gather table of leaf i-1's outer block) and W_i = Uinv_i U_i,i+1 (U W =
+// U_i,i+1); zero for the chain's first leaf. The diagonal block is staged in LDS
+// (identity beyond the leaf's extent). All columns are solved at once: four lanes
+// per column, lane kq holding the column's rows q = kq (mod 4) in registers, each
+// row's sum over the solved rows split over the four lanes and combined by two
+// exchanges.
 template <typename T>
-__global__ __launch_bounds__(64) void diag_inverse_kernel(const T* LU, int64_t n, int64_t lda, int nbmax, T* Linv,
-                                                          T* Uinv) {
+__global__ __launch_bounds__(256) void solve_blocks_kernel(const T* LU, int64_t n, int64_t lda, const int* G, int bw,
+                                                           int np, T* Linv, T* Uinv, T* WL, T* WU) {
   constexpr int DL = PNB + 1;
-  __shared__ T D[PNB * DL * 2];  // D[i][q]
-  __shared__ T X[PNB * PNB * 2];  // X[i][c], lane c
-  const int k = blockIdx.x, c = threadIdx.x;
-  const int64_t j0 = (int64_t)k * nbmax;
-  const int nb = (int)min((int64_t)nbmax, n - j0);
-  for (int i = 0; i < nb; ++i)
-    if (c < nb) {
-      D[2 * (i * DL + c)] = LU[2 * ((j0 + i) * lda + j0 + c)];
-      D[2 * (i * DL + c) + 1] = LU[2 * ((j0 + i) * lda + j0 + c) + 1];
-    }
-  __syncthreads();
-  T* Li = Linv + 2 * (int64_t)k * PNB * PNB;
-  T* Ui = Uinv + 2 * (int64_t)k * PNB * PNB;
-  // unit lower: L X = I, column c; X(i,c) = [i == c] - sum_{q<i} D(i,q) X(q,c)
-  for (int i = 0; i < nb; ++i) {
-    T sr = 0, si = 0;
-    int q = 0;
-    for (; q + 8 <= i; q += 8) {
-      T dr[8], di[8], xr[8], xi[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        dr[u] = D[2 * (i * DL + q + u)];
-        di[u] = D[2 * (i * DL + q + u) + 1];
-        xr[u] = X[2 * ((q + u) * PNB + c)];
-        xi[u] = X[2 * ((q + u) * PNB + c) + 1];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        sr += dr[u] * xr[u] - di[u] * xi[u];
-        si += dr[u] * xi[u] + di[u] * xr[u];
-      }
-    }
-    for (; q < i; ++q) {
-      const T dr = D[2 * (i * DL + q)], di = D[2 * (i * DL + q) + 1];
-      const T xr = X[2 * (q * PNB + c)], xi = X[2 * (q * PNB + c) + 1];
-      sr += dr * xr - di * xi;
-      si += dr * xi + di * xr;
-    }
-    X[2 * (i * PNB + c)] = (i == c ? T(1) : T(0)) - sr;
-    X[2 * (i * PNB + c) + 1] = -si;
-  }
-  for (int i = 0; i < nb; ++i)
-    if (c < nb) {
-      Li[2 * (i * PNB + c)] = X[2 * (i * PNB + c)];
-      Li[2 * (i * PNB + c) + 1] = X[2 * (i * PNB + c) + 1];
-    }
-  // upper: U X = I, column c; X(i,c) = ([i == c] - sum_{q>i} D(i,q) X(q,c)) / D(i,i)
-  for (int i = nb - 1; i >= 0; --i) {
-    T sr = (i == c) ? T(1) : T(0), si = 0;
-    int q = i + 1;
-    for (; q + 8 <= nb; q += 8) {
-      T dr[8], di[8], xr[8], xi[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        dr[u] = D[2 * (i * DL + q + u)];
-        di[u] = D[2 * (i * DL + q + u) + 1];
-        xr[u] = X[2 * ((q + u) * PNB + c)];
-        xi[u] = X[2 * ((q + u) * PNB + c) + 1];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        sr -= dr[u] * xr[u] - di[u] * xi[u];
-        si -= dr[u] * xi[u] + di[u] * xr[u];
-      }
-    }
-    for (; q < nb; ++q) {
-      const T dr = D[2 * (i * DL + q)], di = D[2 * (i * DL + q) + 1];
-      const T xr = X[2 * (q * PNB + c)], xi = X[2 * (q * PNB + c) + 1];
-      sr -= dr * xr - di * xi;
-      si -= dr * xi + di * xr;
-    }
-    const T pr = D[2 * (i * DL + i)], pi = D[2 * (i * DL + i) + 1];
-    const T den = pr * pr + pi * pi;
-    X[2 * (i * PNB + c)] = (sr * pr + si * pi) / den;
-    X[2 * (i * PNB + c) + 1] = (si * pr - sr * pi) / den;
-  }
-  for (int i = 0; i < nb; ++i)
-    if (c < nb) {
-      Ui[2 * (i * PNB + c)] = X[2 * (i * PNB + c)];
-      Ui[2 * (i * PNB + c) + 1] = X[2 * (i * PNB + c) + 1];
-    }
-}
-
-// Link matrices of the block triangular solves: forward W_i = Linv_i L_i,i-1 (rows
-// of L read in final order through the gather table of leaf i-1's outer block),
-// backward W_i = Uinv_i U_i,i+1; zero for the chain's first leaf and beyond the
-// blocks' extents. blockIdx.x: the leaf, blockIdx.y: 0 forward, 1 backward. The
-// neighbour block is staged in LDS; thread t forms row t/4, columns 16 (t%4) + u.
-template <typename T>
-__global__ __launch_bounds__(256) void link_matrix_kernel(const T* LU, int64_t n, int64_t lda, const int* G, int bw,
-                                                          int np, const T* Linv, const T* Uinv, T* WL, T* WU) {
-  __shared__ T Ms[2 * PNB * (PNB + 1)];
-  const int i = blockIdx.x;
-  const bool fwd = blockIdx.y == 0;
-  const int t = threadIdx.x, rl = t >> 2, cq = t & 3;
+  __shared__ T As[2 * PNB * DL];  // the diagonal block
+  const int i = blockIdx.x, task = blockIdx.y, t = threadIdx.x, c = t >> 2, kq = t & 3;
+  const bool lower = task == 0 || task == 2;
   const int64_t base = (int64_t)i * PNB;
   const int nbi = (int)min((int64_t)PNB, n - base);
-  T* W = (fwd ? WL : WU) + 2 * (int64_t)i * PNB * PNB;
-  const bool has = fwd ? i > 0 : i + 1 < np;
-  const int nb = (int)(fwd ? (has ? PNB : 0) : (has ? min((int64_t)PNB, n - base - PNB) : 0));
-  const int64_t c0 = fwd ? base - PNB : base + PNB;  // the neighbour's first column
   for (int idx = t; idx < PNB * PNB; idx += 256) {
-    const int q = idx / PNB, c = idx % PNB;
-    T vr = 0, vi = 0;
-    if (has && q < nbi && c < nb) {
-      const int64_t row = fwd ? (int64_t)G[(c0 / bw) * n + base + q] : base + q;
-      vr = LU[2 * (row * lda + c0 + c)];
-      vi = LU[2 * (row * lda + c0 + c) + 1];
+    const int r = idx / PNB, q = idx % PNB;
+    T vr = r == q ? T(1) : T(0), vi = 0;
+    if (r < nbi && q < nbi) {
+      vr = LU[2 * ((base + r) * lda + base + q)];
+      vi = LU[2 * ((base + r) * lda + base + q) + 1];
     }
-    Ms[2 * (q * (PNB + 1) + c)] = vr;
-    Ms[2 * (q * (PNB + 1) + c) + 1] = vi;
+    As[2 * (r * DL + q)] = vr;
+    As[2 * (r * DL + q) + 1] = vi;
+  }
+  // the right-hand side's column c, rows 4 s + kq in slot s (b), and the solution
+  // (x, zero until its row is solved, so that the row sums need no mask)
+  T br[16], bi[16], xr[16], xi[16];
+#pragma unroll
+  for (int sl = 0; sl < 16; ++sl) xr[sl] = xi[sl] = 0;
+  if (task < 2) {
+#pragma unroll
+    for (int sl = 0; sl < 16; ++sl) {
+      br[sl] = 4 * sl + kq == c ? T(1) : T(0);
+      bi[sl] = 0;
+    }
+  } else {
+    const bool fwd = task == 2;
+    const bool has = fwd ? i > 0 : i + 1 < np;
+    const int nb = has ? (int)(fwd ? PNB : min((int64_t)PNB, n - base - PNB)) : 0;
+    const int64_t c0 = fwd ? base - PNB : base + PNB;  // the neighbour's first column
+    const int* Gb = G + (fwd ? (c0 / bw) * n + base : 0);
+    // the rows first (independent loads), then the entries
+    int64_t row[16];
+#pragma unroll
+    for (int sl = 0; sl < 16; ++sl) {
+      const int q = 4 * sl + kq;
+      row[sl] = (q < nbi && c < nb) ? (fwd ? (int64_t)Gb[q] : base + q) : -1;
+    }
+#pragma unroll
+    for (int sl = 0; sl < 16; ++sl) {
+      T vr = 0, vi = 0;
+      if (row[sl] >= 0) {
+        vr = LU[2 * (row[sl] * lda + c0 + c)];
+        vi = LU[2 * (row[sl] * lda + c0 + c) + 1];
+      }
+      br[sl] = vr;
+      bi[sl] = vi;
+    }
   }
   __syncthreads();
-  T wr[16], wi[16];
+  // substitution over the rows (runtime loop, the slots unrolled): the row's sum
+  // over the solved part split over the quad, the solved row's slot written by
+  // selection
+  for (int step = 0; step < PNB; ++step) {
+    const int r = lower ? step : PNB - 1 - step;
+    T sr = 0, si = 0;
 #pragma unroll
-  for (int u = 0; u < 16; ++u) wr[u] = wi[u] = 0;
-  const T* D = (fwd ? Linv : Uinv) + 2 * ((int64_t)i * PNB * PNB + (int64_t)rl * PNB);
-  if (has && rl < nbi)
-    for (int q = 0; q < nbi; ++q) {
-      const T xr = D[2 * q], xi = D[2 * q + 1];
+    for (int sl = 0; sl < 16; ++sl) {
+      const int q = 4 * sl + kq;
+      const T dr = As[2 * (r * DL + q)], di = As[2 * (r * DL + q) + 1];
+      sr += dr * xr[sl] - di * xi[sl];
+      si += dr * xi[sl] + di * xr[sl];
+    }
+    sr = quad_sum(sr);
+    si = quad_sum(si);
+    T rr = 0, ri = 0;
 #pragma unroll
-      for (int u = 0; u < 16; ++u) {
-        const T mr = Ms[2 * (q * (PNB + 1) + 16 * cq + u)], mi = Ms[2 * (q * (PNB + 1) + 16 * cq + u) + 1];
-        wr[u] += xr * mr - xi * mi;
-        wi[u] += xr * mi + xi * mr;
-      }
+    for (int sl = 0; sl < 16; ++sl) {
+      const bool own = 4 * sl + kq == r;
+      rr = own ? br[sl] : rr;
+      ri = own ? bi[sl] : ri;
+    }
+    rr -= sr;
+    ri -= si;
+    if (!lower) {
+      const T pr = As[2 * (r * DL + r)], pi = As[2 * (r * DL + r) + 1];
+      const T den = pr * pr + pi * pi;
+      const T vr = (rr * pr + ri * pi) / den, vi = (ri * pr - rr * pi) / den;
+      rr = vr;
+      ri = vi;
     }
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    W[2 * (rl * PNB + 16 * cq + u)] = wr[u];
-    W[2 * (rl * PNB + 16 * cq + u) + 1] = wi[u];
+    for (int sl = 0; sl < 16; ++sl) {
+      const bool own = 4 * sl + kq == r;
+      xr[sl] = own ? rr : xr[sl];
+      xi[sl] = own ? ri : xi[sl];
+    }
+  }
+  T* out = (task == 0 ? Linv : task == 1 ? Uinv : task == 2 ? WL : WU) + 2 * (int64_t)i * PNB * PNB;
+#pragma unroll
+  for (int sl = 0; sl < 16; ++sl) {
+    const int q = 4 * sl + kq;
+    out[2 * (q * PNB + c)] = xr[sl];
+    out[2 * (q * PNB + c) + 1] = xi[sl];
   }
 }
 
@@ -1361,45 +1390,6 @@ __global__ void gather_rhs_kernel(const T* src, int64_t n, int nrhs, const int* 
   }
   dst[2 * idx] = re;
   dst[2 * idx + 1] = im;
-}
-
-// Slot u (0..15) of column quarter kq in a PNB-wide block: pairs of adjacent
-// columns interleaved over the four quarters, c = 8 (u / 2) + 2 kq + u % 2, so that
-// the four threads of a row read 64 (complex64) or 128 (complex128) contiguous
-// bytes per vector load instead of each touching its own cache line.
-__device__ __forceinline__ int qcol(int u, int kq) { return 8 * (u >> 1) + 2 * kq + (u & 1); }
-
-// This thread's 16 entries of a block row in slot order (row: the block's first
-// column, 16-byte aligned); columns at or beyond ncols, and all when !ok, are zero.
-__device__ __forceinline__ void load_row16(const float* row, int ncols, int kq, bool ok, float* vr, float* vi) {
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int c = 8 * j + 2 * kq;
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (ok && c + 1 < ncols) {
-      v = *reinterpret_cast<const float4*>(row + 2 * c);
-    } else if (ok && c < ncols) {
-      v.x = row[2 * c];
-      v.y = row[2 * c + 1];
-    }
-    vr[2 * j] = v.x;
-    vi[2 * j] = v.y;
-    vr[2 * j + 1] = v.z;
-    vi[2 * j + 1] = v.w;
-  }
-}
-__device__ __forceinline__ void load_row16(const double* row, int ncols, int kq, bool ok, double* vr, double* vi) {
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int c = 8 * j + 2 * kq;
-    double2 v0 = make_double2(0.0, 0.0), v1 = make_double2(0.0, 0.0);
-    if (ok && c < ncols) v0 = *reinterpret_cast<const double2*>(row + 2 * c);
-    if (ok && c + 1 < ncols) v1 = *reinterpret_cast<const double2*>(row + 2 * c + 2);
-    vr[2 * j] = v0.x;
-    vi[2 * j] = v0.y;
-    vr[2 * j + 1] = v1.x;
-    vi[2 * j + 1] = v1.y;
-  }
 }
 
 template <typename T>
@@ -1555,10 +1545,8 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   auto row_reduce = [&]() {
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
-      accr[d] += __shfl_xor(accr[d], 1, 64);
-      acci[d] += __shfl_xor(acci[d], 1, 64);
-      accr[d] += __shfl_xor(accr[d], 2, 64);
-      acci[d] += __shfl_xor(acci[d], 2, 64);
+      accr[d] = quad_sum(accr[d]);
+      acci[d] = quad_sum(acci[d]);
     }
   };
   // off the chain: every leaf but the neighbour
@@ -2358,10 +2346,7 @@ hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv
   int* G = solve_ints(work, L);
   int* perm = G + (size_t)L.nJ * n;
   int* flags = perm + n;
-  hipLaunchKernelGGL((diag_inverse_kernel<T>), dim3(L.np), dim3(64), 0, s, LU, n, lda, PNB, work + 2 * L.linv,
-                     work + 2 * L.uinv);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  hipError_t e;
   int* R = perm + n + 2 * L.np;  // after the flags: nJ x n block maps
   hipLaunchKernelGGL(identity_maps_kernel, dim3((unsigned)std::min<int64_t>(4096, ((int64_t)L.nJ * n + 255) / 256)),
                      dim3(256), 0, s, n, L.nJ, R);
@@ -2369,9 +2354,8 @@ hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv
   hipLaunchKernelGGL(gather_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (const int*)R, n, L.nJ, G,
                      perm);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL((link_matrix_kernel<T>), dim3((unsigned)L.np, 2), dim3(256), 0, s, LU, n, lda, (const int*)G,
-                     MAXBW, L.np, (const T*)(work + 2 * L.linv), (const T*)(work + 2 * L.uinv), work + 2 * L.wl,
-                     work + 2 * L.wu);
+  hipLaunchKernelGGL((solve_blocks_kernel<T>), dim3((unsigned)L.np, 4), dim3(256), 0, s, LU, n, lda, (const int*)G,
+                     MAXBW, L.np, work + 2 * L.linv, work + 2 * L.uinv, work + 2 * L.wl, work + 2 * L.wu);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // tagged results: no word may carry a later solve's epoch by accident
   if ((e = launch_fill<int>(reinterpret_cast<int*>(work + 2 * L.tf), (int64_t)4 * 2 * n * 8 * sizeof(T) / sizeof(int),
