@@ -83,7 +83,7 @@ struct beat_ctx {
   std::string err;
   std::mutex mu;
   DevBuf a64, a32, b64, b32, q64, q32, x64, r64, n64, rn64, xn64, d32, work, chunk, norms, ipiv, lusmall, qin, pout,
-      fld_src, fld_out, fld_pts, grad, adj, af;
+      fld_src, fld_out, fld_pts, grad, adj, af, rpart;
   hipEvent_t ev[7] = {};
   // frequency sweeps: the assembly stream, two systems in flight, their events
   // (assembly start, regular start, regular end, assembly end) and release events
@@ -108,6 +108,8 @@ struct beat_ctx {
   // batches of small systems through the grouped, graph-replayed path
   // (BEAT_BATCH_GROUPED=0: the per-job pipeline)
   bool batch_grouped = true;
+  // set while a batch or sweep runs another system's assembly beside the solves
+  bool shared_device = false;
   // cancellation marker polled between the frequencies of a sweep (empty: none)
   std::string cancel_path;
   // autotuned launch strategy and tile shapes per precision and size class
@@ -416,8 +418,10 @@ int solve_eliminated(beat_ctx* ctx, const T* LU, int64_t n, int64_t lda, const i
 // r = b - A x for all columns; A64 row-major.
 int residual_all(beat_ctx* ctx, const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r,
                  int nrhs) {
+  HIPCHK(ensure(ctx->rpart, residual64_part_bytes(n)));
+  double* part = as<double>(ctx->rpart);
   if (nrhs <= SOLVE_CHUNK) {
-    HIPCHK(residual64(A, n, lda, x, b, r, nrhs, ctx->stream));
+    HIPCHK(residual64(A, n, lda, x, b, r, nrhs, part, !ctx->shared_device, ctx->stream));
     return BEAT_OK;
   }
   HIPCHK(ensure(ctx->chunk, sizeof(double) * 2 * n * SOLVE_CHUNK * 3));
@@ -430,7 +434,7 @@ int residual_all(beat_ctx* ctx, const double* A, int64_t n, int64_t lda, const d
                        c0, nc, cx);
     hipLaunchKernelGGL((extract_cols_kernel<double>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream, b, n, nrhs,
                        c0, nc, cb);
-    HIPCHK(residual64(A, n, lda, cx, cb, cr, nc, ctx->stream));
+    HIPCHK(residual64(A, n, lda, cx, cb, cr, nc, part, !ctx->shared_device, ctx->stream));
     hipLaunchKernelGGL((insert_cols_kernel<double>), dim3(blocks_for(n * nc)), dim3(256), 0, ctx->stream,
                        (const double*)cr, n, nrhs, c0, nc, r);
     HIPCHK(hipGetLastError());
@@ -1075,6 +1079,7 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
     if (precision == BEAT_PREC_FP32_REFINED) {
       HIPCHK(ensure(ctx->x64, sizeof(double) * 2 * nmax * nrhs));
       HIPCHK(ensure(ctx->r64, sizeof(double) * 2 * nmax * nrhs));
+      HIPCHK(ensure(ctx->rpart, residual64_part_bytes(nmax)));
       HIPCHK(ensure(ctx->n64, sizeof(double) * 2 * nmax * nrhs));
       HIPCHK(ensure(ctx->rn64, sizeof(double) * 2 * nmax * nrhs));
       HIPCHK(ensure(ctx->d32, sizeof(float) * 2 * nmax * nrhs + sizeof(int) * 64 + 4096));
@@ -1112,8 +1117,10 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
     }
     if (i + 1 < njobs)
       if (int rc = enqueue(i + 1)) return rc;
+    ctx->shared_device = i + 1 < njobs;
     int rc = stage_solve(ctx, meshes[i], precision, nrhs, sb[s2], p_dev + 2 * p_off[i], opts, ev[s2],
                          reps ? reps + i : nullptr);
+    ctx->shared_device = false;
     if (rc != BEAT_OK && rc != BEAT_ERR_SINGULAR) {
       (void)hipStreamSynchronize(ctx->asm_stream);
       return rc;
@@ -2057,7 +2064,7 @@ void beat_ctx_destroy(beat_ctx* c) {
                     &c->rn64, &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
                     &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af,
                     &c->g_a64, &c->g_a32, &c->g_b64, &c->g_b32, &c->g_q64, &c->g_q32, &c->g_linv, &c->g_vec,
-                    &c->g_state, &c->g_desc};
+                    &c->g_state, &c->g_desc, &c->rpart};
   for (DevBuf* b : bufs)
     if (b->p) (void)hipFree(b->p);
   if (c->asm_stream) (void)hipStreamSynchronize(c->asm_stream);
@@ -2130,7 +2137,7 @@ int beat_ctx_reclaim(beat_ctx* c) {
                     &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin, &c->pout,
                     &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af,
                     &c->g_a64, &c->g_a32, &c->g_b64, &c->g_b32, &c->g_q64, &c->g_q32, &c->g_linv, &c->g_vec,
-                    &c->g_state, &c->g_desc};
+                    &c->g_state, &c->g_desc, &c->rpart};
   for (DevBuf* b : bufs) {
     if (b->p) HIPCHK(hipFree(b->p));
     b->p = nullptr;
@@ -2157,7 +2164,7 @@ int beat_ctx_memory_info(beat_ctx* c, int64_t* info) {
                           &c->xn64, &c->d32, &c->work, &c->chunk, &c->norms, &c->ipiv, &c->lusmall, &c->qin,
                           &c->pout, &c->fld_src, &c->fld_out, &c->fld_pts, &c->grad, &c->adj, &c->af,
                     &c->g_a64, &c->g_a32, &c->g_b64, &c->g_b32, &c->g_q64, &c->g_q32, &c->g_linv, &c->g_vec,
-                    &c->g_state, &c->g_desc};
+                    &c->g_state, &c->g_desc, &c->rpart};
   for (const DevBuf* b : bufs) used += b->bytes;
   for (int s2 = 0; s2 < 2; ++s2) {
     const DevBuf* sw[] = {&c->sw_q64[s2], &c->sw_q32[s2], &c->sw_a64[s2], &c->sw_b64[s2], &c->sw_a32[s2],

@@ -253,9 +253,13 @@ hipError_t tri_solve_batched(const BatchSystem<T>* sys, const BatchVec<T>* vec, 
 hipError_t residual64_batched(const BatchSystem<double>* sys, const BatchVec<double>* x, const BatchVec<double>* b,
                               const BatchVec<double>* r, int nsys, int64_t nmax, int nrhs, hipStream_t s);
 
-// r = b - A x (complex128), A row-major n x n, x and b n x nrhs row-major.
+// r = b - A x (complex128), A row-major n x n, x and b n x nrhs row-major (nrhs
+// <= 8); part: workspace of residual64_part_bytes(n) for the partial products;
+// split: spread each row block's columns over several workgroups to fill the
+// device (false when other streams keep it busy).
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
-                      hipStream_t s);
+                      double* part, bool split, hipStream_t s);
+size_t residual64_part_bytes(int64_t n);
 // per-column 2-norms of an n x nrhs complex128 row-major matrix into out[nrhs]
 // (out holds nrhs norms followed by 64*nrhs partial sums)
 hipError_t column_norms64(const double* X, int64_t n, int nrhs, double* out, hipStream_t s);

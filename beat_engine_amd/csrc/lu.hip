@@ -1620,6 +1620,7 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
 // wavefronts (32 rows). The 64 partial sums per lane are then reduced across the
 // wavefront by recursive halving (63 exchanges), leaving sum number l in lane l.
 constexpr int RES_WAVES = 8, RES_ROWS = 4, RES_KC = 128;
+constexpr int RES_SPLIT_MAX = 8;  // k ranges per row block of the single-system residual
 
 template <int N>
 __device__ __forceinline__ void halve_step(double* v, int lane) {
@@ -1701,14 +1702,6 @@ __device__ __forceinline__ void residual64_body(const double* A, int64_t n, int6
   if (row < n && d < nrhs) r[2 * (row * rld + d) + c] = b[2 * (row * bld + d) + c] - acc[0];
 }
 
-__global__ __launch_bounds__(512) void residual64_kernel(const double* A, int64_t n, int64_t lda, const double* x,
-                                                         const double* b, double* r, int nrhs) {
-  // in a sweep the factorization and solves run beside the next system's
-  // assembly: their waves issue ahead of the assembly's
-  __builtin_amdgcn_s_setprio(2);
-  residual64_body(A, n, lda, x, nrhs, b, nrhs, r, nrhs, nrhs, (int64_t)blockIdx.x);
-}
-
 __global__ __launch_bounds__(512) void residual64_batched_kernel(const BatchSystem<double>* sys,
                                                                  const BatchVec<double>* x, const BatchVec<double>* b,
                                                                  const BatchVec<double>* r, int nrhs) {
@@ -1718,6 +1711,124 @@ __global__ __launch_bounds__(512) void residual64_batched_kernel(const BatchSyst
   if (R.skip && *R.skip) return;
   if ((int64_t)blockIdx.x * RES_WAVES * RES_ROWS >= S.n) return;
   residual64_body(S.A, S.n, S.lda, X.v, X.ld, B.v, B.ld, R.v, R.ld, nrhs, (int64_t)blockIdx.x);
+}
+
+// The single-system residual r = b - A x over a (row block, k range) grid: the
+// workgroups of one row block split its columns into grid.y ranges of 64-column
+// chunks, so that the grid fills whole rounds of the CUs (one workgroup of 8
+// wavefronts per CU), and each writes its partial sums A x to part[range]; a
+// second kernel forms b - sum. Within a workgroup, the next chunk's A values
+// (registers) and x values (the other LDS buffer) are loaded while the current
+// chunk is multiplied; one barrier per chunk. Lanes run along the row as in
+// residual64_body (one 1 KB load per row and chunk per wavefront).
+constexpr int RS_SUB = 1, RS_KC = 64 * RS_SUB;  // columns per chunk: RS_SUB per lane
+__global__ __launch_bounds__(512) void residual64_split_kernel(const double* A, int64_t n, int64_t lda,
+                                                               const double* x, int nrhs, double* part) {
+  __builtin_amdgcn_s_setprio(2);
+  __shared__ double xs[2][8 * RS_KC * 2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t row0 = ((int64_t)blockIdx.x * RES_WAVES + wave) * RES_ROWS;
+  const int64_t nch = (n + RS_KC - 1) / RS_KC;
+  const int64_t per = (nch + gridDim.y - 1) / gridDim.y;
+  const int64_t c0 = (int64_t)blockIdx.y * per, c1 = min(nch, c0 + per);
+  double acc[RES_ROWS * 8 * 2];
+#pragma unroll
+  for (int j = 0; j < RES_ROWS * 8 * 2; ++j) acc[j] = 0;
+  // column lane + 64 s of a chunk; thread tid stages the x entries (k = tid / 8 +
+  // 64 s, right-hand side tid % 8)
+  const int xk = tid >> 3, xd = tid & 7;
+  auto load_a = [&](int64_t c, double (*ar)[RS_SUB], double (*ai)[RS_SUB]) {
+#pragma unroll
+    for (int q = 0; q < RES_ROWS; ++q)
+#pragma unroll
+      for (int sb = 0; sb < RS_SUB; ++sb) {
+        const int64_t row = row0 + q, k = c * RS_KC + lane + 64 * sb;
+        ar[q][sb] = ai[q][sb] = 0;
+        if (row < n && k < n) {
+          const double2 v = *reinterpret_cast<const double2*>(A + 2 * (row * lda + k));
+          ar[q][sb] = v.x;
+          ai[q][sb] = v.y;
+        }
+      }
+  };
+  auto load_x = [&](int64_t c, double2* v) {
+#pragma unroll
+    for (int sb = 0; sb < RS_SUB; ++sb) {
+      const int64_t k = c * RS_KC + xk + 64 * sb;
+      v[sb] = make_double2(0.0, 0.0);
+      if (k < n && xd < nrhs) v[sb] = *reinterpret_cast<const double2*>(x + 2 * (k * nrhs + xd));
+    }
+  };
+  auto put_x = [&](int buf, const double2* v) {
+#pragma unroll
+    for (int sb = 0; sb < RS_SUB; ++sb) {
+      xs[buf][2 * (xd * RS_KC + xk + 64 * sb)] = v[sb].x;
+      xs[buf][2 * (xd * RS_KC + xk + 64 * sb) + 1] = v[sb].y;
+    }
+  };
+  double ar[RES_ROWS][RS_SUB], ai[RES_ROWS][RS_SUB], nr[RES_ROWS][RS_SUB], ni[RES_ROWS][RS_SUB];
+  if (c0 < c1) {
+    double2 v[RS_SUB];
+    load_a(c0, ar, ai);
+    load_x(c0, v);
+    put_x(0, v);
+  }
+  __syncthreads();
+  for (int64_t c = c0; c < c1; ++c) {
+    const int buf = (int)((c - c0) & 1);
+    const bool more = c + 1 < c1;
+    double2 xn[RS_SUB];
+    if (more) {
+      load_a(c + 1, nr, ni);
+      load_x(c + 1, xn);
+    }
+#pragma unroll
+    for (int sb = 0; sb < RS_SUB; ++sb) {
+      double xr[8], xi[8];
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        xr[d] = xs[buf][2 * (d * RS_KC + lane + 64 * sb)];
+        xi[d] = xs[buf][2 * (d * RS_KC + lane + 64 * sb) + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < RES_ROWS; ++q)
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+          acc[(q * 8 + d) * 2] = fma(ar[q][sb], xr[d], fma(-ai[q][sb], xi[d], acc[(q * 8 + d) * 2]));
+          acc[(q * 8 + d) * 2 + 1] = fma(ar[q][sb], xi[d], fma(ai[q][sb], xr[d], acc[(q * 8 + d) * 2 + 1]));
+        }
+    }
+    if (more) put_x(buf ^ 1, xn);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RES_ROWS; ++q)
+#pragma unroll
+      for (int sb = 0; sb < RS_SUB; ++sb) {
+        ar[q][sb] = nr[q][sb];
+        ai[q][sb] = ni[q][sb];
+      }
+  }
+  halve_step<32>(acc, lane);
+  halve_step<16>(acc, lane);
+  halve_step<8>(acc, lane);
+  halve_step<4>(acc, lane);
+  halve_step<2>(acc, lane);
+  halve_step<1>(acc, lane);
+  // lane l holds sum number l = (q * 8 + d) * 2 + (0: re, 1: im)
+  const int q = lane >> 4, d = (lane >> 1) & 7, cpart = lane & 1;
+  const int64_t row = row0 + q;
+  if (row < n) part[((int64_t)blockIdx.y * n + row) * 16 + d * 2 + cpart] = acc[0];
+}
+
+// r = b - sum over the k ranges of the partial products
+__global__ void residual64_finish_kernel(const double* part, int ks, int64_t n, const double* b, double* r, int nrhs) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * nrhs * 2) return;
+  const int64_t row = idx / (nrhs * 2);
+  const int e = (int)(idx % (nrhs * 2));  // d * 2 + (re, im)
+  double s = 0;
+  for (int j = 0; j < ks; ++j) s += part[((int64_t)j * n + row) * 16 + e];
+  r[idx] = b[idx] - s;
 }
 
 __global__ __launch_bounds__(256) void colnorm_partial_kernel(const double* X, int64_t n, int nrhs, double* part) {
@@ -2478,12 +2589,44 @@ template hipError_t lu_solve_eliminated<float>(const float*, int64_t, int64_t, f
 template hipError_t lu_solve_eliminated<double>(const double*, int64_t, int64_t, double*, int, double*, int*,
                                                 hipStream_t);
 
+size_t residual64_part_bytes(int64_t n) { return sizeof(double) * 16 * (size_t)RES_SPLIT_MAX * (size_t)n; }
+
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
-                      hipStream_t s) {
-  if (nrhs > 8) return hipErrorInvalidValue;
+                      double* part, bool split, hipStream_t s) {
+  if (nrhs > 8 || nrhs < 1) return hipErrorInvalidValue;
+  if (n <= 0) return hipSuccess;
   constexpr int rows = RES_WAVES * RES_ROWS;
-  hipLaunchKernelGGL(residual64_kernel, dim3((unsigned)((n + rows - 1) / rows)), dim3(64 * RES_WAVES), 0, s, A, n, lda,
-                     x, b, r, nrhs);
+  const int64_t nblk = (n + rows - 1) / rows;
+  // k ranges per row block (split): the count (up to RES_SPLIT_MAX) whose grid
+  // comes closest to whole rounds of one workgroup per CU, keeping at least 8
+  // chunks per range; the fewest ranges among equals. Without split, one range:
+  // beside other streams' long kernels (a sweep's next assembly) each workgroup
+  // must wait for a CU to drain, and fewer, longer workgroups do better there.
+  static std::atomic<int> cus_cached{0};
+  int cus = cus_cached.load();
+  if (cus <= 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || cus <= 0)
+      cus = 256;
+    cus_cached.store(cus);
+  }
+  const int64_t nch = (n + RS_KC - 1) / RS_KC;
+  int ks = 1;
+  double best = 0;
+  for (int k = 1; split && k <= RES_SPLIT_MAX && nch / k >= 8; ++k) {
+    const double wg = (double)(nblk * k), rounds = std::ceil(wg / cus), eff = wg / (rounds * cus);
+    if (eff > best + 1e-3) {
+      best = eff;
+      ks = k;
+    }
+  }
+  hipLaunchKernelGGL(residual64_split_kernel, dim3((unsigned)nblk, (unsigned)ks), dim3(64 * RES_WAVES), 0, s, A, n,
+                     lda, x, nrhs, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(residual64_finish_kernel, dim3((unsigned)((n * nrhs * 2 + 255) / 256)), dim3(256), 0, s,
+                     (const double*)part, ks, n, b, r, nrhs);
   return hipGetLastError();
 }
 
