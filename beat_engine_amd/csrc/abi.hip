@@ -108,6 +108,10 @@ struct beat_ctx {
   // batches of small systems through the grouped, graph-replayed path
   // (BEAT_BATCH_GROUPED=0: the per-job pipeline)
   bool batch_grouped = true;
+  // trailing rows below which the LU's outer blocks run in order on one stream
+  // (BEAT_LU_TAIL_ROWS at context creation; measured at n = 10242, complex64:
+  // 0 rows 33.0 ms, 768 32.8, 1536 32.6, 2560 32.5, 4096 32.6)
+  int64_t lu_tail_rows = 2560;
   // set while a batch or sweep runs another system's assembly beside the solves
   bool shared_device = false;
   // cancellation marker polled between the frequencies of a sweep (empty: none)
@@ -352,6 +356,7 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   w.ev_panel = ctx->ev_panel;
   w.ev_inb = ctx->ev_inb;
   w.maxg = maxg;
+  w.tail_rows = ctx->lu_tail_rows;
   (void)sizeof(T);
   return BEAT_OK;
 }
@@ -2040,6 +2045,7 @@ int beat_ctx_create(int device, beat_ctx** out) {
   if (const char* v = std::getenv("BEAT_FP32_ASSEMBLY")) c->fp32_assembly_fp64 = std::strcmp(v, "native") != 0;
   if (const char* v = std::getenv("BEAT_BATCH_WORKERS")) c->batch_workers = std::max(1, std::atoi(v));
   if (const char* v = std::getenv("BEAT_BATCH_GROUPED")) c->batch_grouped = v[0] != '0';
+  if (const char* v = std::getenv("BEAT_LU_TAIL_ROWS")) c->lu_tail_rows = std::max(0, std::atoi(v));
   // BEAT_LU_LOOKAHEAD=0 runs the whole factorization in order on one stream
   if (const char* v = std::getenv("BEAT_LU_LOOKAHEAD"))
     if (v[0] == '0') {

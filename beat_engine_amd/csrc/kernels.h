@@ -181,6 +181,9 @@ struct LuWork {
   // tile shape of the trailing updates (0..GEMM_TILES-1): 64 x 64-complex tiles with
   // K chunks of 16 (0) or 32 (1), 128 x 64 with 16 (2) or 32 (3)
   int gemm_tile = 0;
+  // trailing rows below which the outer blocks run in order on one stream (the
+  // look-ahead and in-block streams' events cost more than they overlap there)
+  int64_t tail_rows = 0;
 };
 
 // Columns [n, nc) (n < nc <= lda) may carry right-hand sides: on return they

@@ -2355,8 +2355,15 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
     return gemm_sub<T>(cA + 2 * (Je * lda + J), lda, cA + 2 * (J * lda + c0), lda, A + 2 * (Je * lda + c0), lda,
                        n - Je, c1 - c0, (int)(Je - J), st, behind, g3, tile);
   };
+  // In the factorization's tail (at most w.tail_rows trailing rows after a block)
+  // the look-ahead gains nothing, its updates being small, while every
+  // cross-stream event puts tens of microseconds of latency on the critical path:
+  // the tail's blocks run in order on s.
+  bool far_pending = false;  // a far update on the side stream that s has not yet waited for
   for (int64_t J = 0; J < n; J += bw) {
     const int64_t Je = std::min<int64_t>(n, J + bw);
+    const bool tail = n - Je <= w.tail_rows;
+    const bool split_b = split && !tail;
     for (int64_t j0 = J; j0 < Je; j0 += nb) {
       const int b = (int)std::min<int64_t>(nb, Je - j0);
       const int64_t nx = std::min<int64_t>(Je, j0 + b + nb);  // end of the next leaf's columns
@@ -2364,7 +2371,7 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
                 : launch_panel<T>(A, n, lda, j0, b, w, leaf_inv(j0, J), s);
       if (e != hipSuccess) return e;
       ++panels;
-      if (!split) {
+      if (!split_b) {
         if (!nopiv && (e = launch_laswp<T>(A, lda, J, j0, w.ipiv, j0, j0 + b, s)) != hipSuccess) return e;
         if ((e = leaf_update(J, j0, b, j0 + b, Je, s)) != hipSuccess) return e;
         continue;
@@ -2378,31 +2385,33 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
       if ((e = leaf_update(J, j0, b, nx, Je, w.inb)) != hipSuccess) return e;
       if ((e = hipEventRecord(w.ev_inb, w.inb)) != hipSuccess) return e;
     }
-    if (split && (e = hipStreamWaitEvent(s, w.ev_inb, 0)) != hipSuccess) return e;
+    if (split_b && (e = hipStreamWaitEvent(s, w.ev_inb, 0)) != hipSuccess) return e;
+    // the previous block's far update (of every column beyond this block) comes
+    // before any update of those columns on s
+    if (far_pending) {
+      if ((e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;
+      far_pending = false;
+    }
     if (Je >= n) {
-      // the last block's update of the carried right-hand sides, after the
-      // previous block's far update of them
-      if (ncol > n) {
-        if (look_ahead && (e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;
-        if ((e = update(J, Je, n, ncol, s)) != hipSuccess) return e;
-      }
+      // the last block's update of the carried right-hand sides
+      if (ncol > n && (e = update(J, Je, n, ncol, s)) != hipSuccess) return e;
       continue;
     }
     const int64_t Kn = std::min<int64_t>(n, Je + bw);
-    if (look_ahead) {
-      if ((e = hipEventRecord(w.ev_fact, s)) != hipSuccess) return e;
-      if ((e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;  // previous block's far update
+    if (look_ahead && !tail) {
       if (Kn < ncol) {
+        if ((e = hipEventRecord(w.ev_fact, s)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(w.side, w.ev_fact, 0)) != hipSuccess) return e;
         if ((e = update(J, Je, Kn, ncol, w.side)) != hipSuccess) return e;
         if ((e = hipEventRecord(w.ev_far, w.side)) != hipSuccess) return e;
+        far_pending = true;
       }
       if ((e = update(J, Je, Je, Kn, s)) != hipSuccess) return e;
     } else {
       if ((e = update(J, Je, Je, ncol, s)) != hipSuccess) return e;
     }
   }
-  if (look_ahead && (e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;
+  if (far_pending && (e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;
   if (panels_out) *panels_out = panels;
   return hipSuccess;
 }
