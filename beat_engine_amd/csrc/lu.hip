@@ -1903,7 +1903,17 @@ __device__ __forceinline__ double cabs1(T re, T im) {
 template <typename T>
 __device__ __forceinline__ void pivot_reciprocal(T ur, T ui, T& ir, T& ii) {
   const T den = ur * ur + ui * ui;
-  const T rd = den != T(0) ? T(1) / den : T(0);
+  T rd;
+  if constexpr (sizeof(T) == 4) {
+    // the hardware reciprocal and one Newton step (within an ulp of 1 / den; the
+    // complex64 factors are those of a preconditioner or of the plain complex64
+    // solve), instead of the longer correctly rounded division on each column's
+    // critical path
+    const float y = __builtin_amdgcn_rcpf(den);
+    rd = den != 0.f ? fmaf(fmaf(-den, y, 1.f), y, y) : 0.f;
+  } else {
+    rd = den != T(0) ? T(1) / den : T(0);
+  }
   ir = ur * rd;
   ii = -ui * rd;
 }
