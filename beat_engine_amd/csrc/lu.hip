@@ -2107,13 +2107,22 @@ __device__ __forceinline__ void below_nopiv_body(T* A, int64_t n, int64_t lda, i
       xi[s] = src[8 * s + 1];
     }
   }
-  for (int e = t; e < PNB * PNB; e += 256) {
-    const int i = e >> 6, c = e & 63;
-    T vr = i == c ? T(1) : T(0), vi = T(0);
-    if (i < b && c < b && c >= i) {
-      vr = A[2 * ((j0 + i) * lda + j0 + c)];
-      vi = A[2 * ((j0 + i) * lda + j0 + c) + 1];
-    }
+  // U11 (identity beyond the upper part): every load issued first, from in-bounds
+  // addresses, then selected and staged (a load per iteration followed by its use
+  // would put 16 memory round trips in a row)
+  T gr[PNB * PNB / 256], gi[PNB * PNB / 256];
+#pragma unroll
+  for (int it = 0; it < PNB * PNB / 256; ++it) {
+    const int e = t + 256 * it, i = e >> 6, c = e & 63;
+    const int64_t o = 2 * ((j0 + min(i, b - 1)) * lda + j0 + min(c, b - 1));
+    gr[it] = A[o];
+    gi[it] = A[o + 1];
+  }
+#pragma unroll
+  for (int it = 0; it < PNB * PNB / 256; ++it) {
+    const int e = t + 256 * it, i = e >> 6, c = e & 63;
+    const bool in = i < b && c < b && c >= i;
+    const T vr = in ? gr[it] : (i == c ? T(1) : T(0)), vi = in ? gi[it] : T(0);
     us[i][c & 3][c >> 2][0] = vr;
     us[i][c & 3][c >> 2][1] = vi;
     if (i == c) pivot_reciprocal(vr, vi, rcp[i][0], rcp[i][1]);
