@@ -456,20 +456,39 @@ int col_norms(beat_ctx* ctx, const double* X, int64_t n, int nrhs, std::vector<d
   return BEAT_OK;
 }
 
-// Column norms of X and Y read back with one wait on the stream.
-int col_norms2(beat_ctx* ctx, const double* X, const double* Y, int64_t n, int nrhs, std::vector<double>& ox,
-               std::vector<double>& oy) {
-  HIPCHK(ensure(ctx->norms, sizeof(double) * 2 * (nrhs + 64 * nrhs)));
-  double* nx = as<double>(ctx->norms);
-  double* ny = nx + nrhs + 64 * nrhs;
-  HIPCHK(column_norms64(X, n, nrhs, nx, ctx->stream));
-  HIPCHK(column_norms64(Y, n, nrhs, ny, ctx->stream));
-  ox.resize(nrhs);
-  oy.resize(nrhs);
-  HIPCHK(hipMemcpyAsync(ox.data(), nx, sizeof(double) * nrhs, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipMemcpyAsync(oy.data(), ny, sizeof(double) * nrhs, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  return BEAT_OK;
+// Norm slots of refine_loop's first step in ctx->norms: the column norms of b,
+// of the initial residual and of the corrected one (each nrhs norms followed by
+// 64*nrhs partial sums), then the step's record: bn, err, ne (nrhs doubles each)
+// and the two column masks (nrhs ints each).
+size_t refine_norms_bytes(int nrhs) { return sizeof(double) * (3 * (size_t)(nrhs + 64 * nrhs) + 4 * (size_t)nrhs); }
+
+// The first correction step's decisions, made on the device with the host
+// loop's rules so that the step runs without a wait for the host.
+// phase 0: err = relative errors of the initial residual, m1 = columns above
+// rtol (the columns the step corrects); phase 1: ne = errors after the step,
+// m2 = columns whose error fell to a finite smaller value, all zero when no
+// column was above rtol (the host loop then runs no step).
+__global__ void refine_first_kernel(const double* bn, const double* rn, int nrhs, double rtol, double* rec, int phase) {
+  double* cbn = rec;
+  double* err = rec + nrhs;
+  double* ne = err + nrhs;
+  int* m1 = reinterpret_cast<int*>(ne + nrhs);
+  int* m2 = m1 + nrhs;
+  bool run = false;
+  if (phase == 1)
+    for (int j = 0; j < nrhs; ++j) run |= m1[j] != 0;
+  for (int j = threadIdx.x; j < nrhs; j += blockDim.x) {
+    const double b = phase == 0 ? bn[j] : cbn[j];
+    const double e = b == 0 ? (rn[j] == 0 ? 0.0 : INFINITY) : rn[j] / b;
+    if (phase == 0) {
+      cbn[j] = b;
+      err[j] = e;
+      m1[j] = e <= rtol ? 0 : 1;
+    } else {
+      ne[j] = e;
+      m2[j] = (run && isfinite(e) && e < err[j]) ? 1 : 0;
+    }
+  }
 }
 
 struct Refine {
@@ -496,25 +515,84 @@ int refine_loop(beat_ctx* ctx, const float* LU32, int64_t lda32, const int* ipiv
   std::vector<int> mask(nrhs);
   HIPCHK(ensure(ctx->lusmall, sizeof(int) * nrhs + 256));
   int* dmask = as<int>(ctx->lusmall);
-  std::vector<double> bn, rn;
+  HIPCHK(ensure(ctx->norms, refine_norms_bytes(nrhs)));
+  const int64_t slot = nrhs + 64 * nrhs;
+  double* nb = as<double>(ctx->norms);
+  double* nr0 = nb + slot;
+  double* nr1 = nr0 + slot;
+  double* rec = nr1 + slot;
+  std::vector<double> bn;
   auto errors_of = [&](const std::vector<double>& nrm) {
     std::vector<double> e(nrhs);
     for (int j = 0; j < nrhs; ++j)
       e[j] = bn[j] == 0 ? (nrm[j] == 0 ? 0.0 : INFINITY) : nrm[j] / bn[j];
     return e;
   };
-  if (residual_all(ctx, A64, n, lda64, x64, b64, r, nrhs) != BEAT_OK) return BEAT_ERR_HIP;
-  if (col_norms2(ctx, b64, r, n, nrhs, bn, rn) != BEAT_OK) return BEAT_ERR_HIP;
-  std::vector<double> errors = errors_of(rn);
-  out.history = errors;
-  out.hist_len = 1;
-  int steps = 0, status = BEAT_REFINE_STEP_LIMIT;
   auto all_le = [&](const std::vector<double>& e) {
     for (double v : e)
       if (!(v <= rtol)) return false;
     return true;
   };
-  for (int step = 1; step <= max_steps; ++step) {
+  // Initial residual and the first correction step are queued without a wait:
+  // the step's column mask and acceptance come from refine_first_kernel, and
+  // the host reads the record once, after the step.
+  if (residual_all(ctx, A64, n, lda64, x64, b64, r, nrhs) != BEAT_OK) return BEAT_ERR_HIP;
+  HIPCHK(column_norms64(b64, n, nrhs, nb, ctx->stream));
+  HIPCHK(column_norms64(r, n, nrhs, nr0, ctx->stream));
+  hipLaunchKernelGGL(refine_first_kernel, dim3(1), dim3(64), 0, ctx->stream, (const double*)nb, (const double*)nr0,
+                     nrhs, rtol, rec, 0);
+  HIPCHK(hipGetLastError());
+  const int* m1 = reinterpret_cast<const int*>(rec + 3 * nrhs);
+  const int* m2 = m1 + nrhs;
+  if (max_steps >= 1) {
+    hipLaunchKernelGGL(masked_downcast_kernel, dim3(blocks_for(n * nrhs)), dim3(256), 0, ctx->stream, (const double*)r,
+                       d, n, nrhs, m1);
+    HIPCHK(hipGetLastError());
+    if (solve_all<float>(ctx, LU32, n, lda32, ipiv, d, nrhs, false) != BEAT_OK) return BEAT_ERR_HIP;
+    hipLaunchKernelGGL(add_delta_kernel, dim3(grid_cap(2 * n * nrhs)), dim3(256), 0, ctx->stream, (const double*)x64,
+                       (const float*)d, nx, 2 * n * nrhs);
+    HIPCHK(hipGetLastError());
+    if (residual_all(ctx, A64, n, lda64, nx, b64, nr, nrhs) != BEAT_OK) return BEAT_ERR_HIP;
+    HIPCHK(column_norms64(nr, n, nrhs, nr1, ctx->stream));
+    hipLaunchKernelGGL(refine_first_kernel, dim3(1), dim3(64), 0, ctx->stream, (const double*)nb, (const double*)nr1,
+                       nrhs, rtol, rec, 1);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(accept_kernel, dim3(blocks_for(n * nrhs)), dim3(256), 0, ctx->stream, x64, (const double*)nx, r,
+                       (const double*)nr, n, nrhs, m2);
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<double> hrec(4 * nrhs);
+  HIPCHK(hipMemcpyAsync(hrec.data(), rec, sizeof(double) * 4 * nrhs, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  bn.assign(hrec.begin(), hrec.begin() + nrhs);
+  std::vector<double> errors(hrec.begin() + nrhs, hrec.begin() + 2 * nrhs);
+  out.history = errors;
+  out.hist_len = 1;
+  int steps = 0, status = BEAT_REFINE_STEP_LIMIT;
+  bool done = false;
+  if (max_steps >= 1) {
+    if (all_le(errors)) {
+      status = BEAT_REFINE_CONVERGED;
+      done = true;
+    } else {
+      const double* ne = hrec.data() + 2 * nrhs;
+      const int* acc = reinterpret_cast<const int*>(hrec.data() + 3 * nrhs) + nrhs;
+      bool improved = false;
+      for (int j = 0; j < nrhs; ++j)
+        if (acc[j]) {
+          errors[j] = ne[j];
+          improved = true;
+        }
+      steps = 1;
+      out.history.insert(out.history.end(), errors.begin(), errors.end());
+      out.hist_len += 1;
+      if (!improved) {
+        status = BEAT_REFINE_STAGNATED;
+        done = true;
+      }
+    }
+  }
+  for (int step = 2; step <= max_steps && !done; ++step) {
     if (all_le(errors)) {
       status = BEAT_REFINE_CONVERGED;
       break;
@@ -1088,7 +1166,7 @@ int batch_impl(beat_ctx* ctx, int64_t njobs, const beat_mesh* const* meshes, int
       HIPCHK(ensure(ctx->n64, sizeof(double) * 2 * nmax * nrhs));
       HIPCHK(ensure(ctx->rn64, sizeof(double) * 2 * nmax * nrhs));
       HIPCHK(ensure(ctx->d32, sizeof(float) * 2 * nmax * nrhs + sizeof(int) * 64 + 4096));
-      HIPCHK(ensure(ctx->norms, sizeof(double) * 2 * (nrhs + 64 * nrhs)));
+      HIPCHK(ensure(ctx->norms, refine_norms_bytes(nrhs)));
       HIPCHK(ensure(ctx->lusmall, sizeof(int) * nrhs + 256));
     }
   }

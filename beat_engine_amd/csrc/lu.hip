@@ -1411,6 +1411,66 @@ struct TriArgs {
   int nrhs;
 };
 
+// 16 x 16 matrix-core tiles of the solves (one wavefront, K = 4 per step):
+// A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k; accumulator element v of
+// lane l is row mfma_row<T>(l / 16, v), column l % 16.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+struct Acc4;
+template <>
+struct Acc4<float> {
+  typedef f32x4 type;
+};
+template <>
+struct Acc4<double> {
+  typedef f64x4 type;
+};
+__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <typename T>
+__device__ __forceinline__ int mfma_row(int q, int v) {
+  return sizeof(T) == 4 ? 4 * q + v : q + 4 * v;
+}
+// 16 consecutive complex entries (p 16-byte aligned)
+__device__ __forceinline__ void load_seg16(const float* p, float* vr, float* vi) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const float4 v = *reinterpret_cast<const float4*>(p + 4 * j);
+    vr[2 * j] = v.x;
+    vi[2 * j] = v.y;
+    vr[2 * j + 1] = v.z;
+    vi[2 * j + 1] = v.w;
+  }
+}
+__device__ __forceinline__ void load_seg16(const double* p, double* vr, double* vi) {
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const double2 v = *reinterpret_cast<const double2*>(p + 2 * j);
+    vr[j] = v.x;
+    vi[j] = v.y;
+  }
+}
+// One part (real or imaginary) of a published element: its TW / 2 tagged words
+// (TaggedRow), and its plain word.
+__device__ __forceinline__ void st_tagged_part(unsigned long long* q, float v, unsigned tag) {
+  st_u64(q, ((unsigned long long)tag << 32) | __float_as_uint(v));
+}
+__device__ __forceinline__ void st_tagged_part(unsigned long long* q, double v, unsigned tag) {
+  const unsigned long long t = (unsigned long long)tag << 32, b = (unsigned long long)__double_as_longlong(v);
+  st_u64(q, t | (b & 0xffffffffull));
+  st_u64(q + 1, t | (b >> 32));
+}
+__device__ __forceinline__ void st_part(float* p, float v) {
+  __hip_atomic_store(reinterpret_cast<unsigned*>(p), __float_as_uint(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_part(double* p, double v) {
+  st_u64(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v));
+}
+
 // Persistent block triangular solve with 8 right-hand sides (zero-padded): one
 // workgroup per leaf of PNB rows, all resident (cooperative launch). Forward
 // (L y = P b) workgroup i consumes the results of leaves 0..i-1 in order as their
@@ -1422,10 +1482,16 @@ struct TriArgs {
 // on the chain per leaf. A result is published twice: as epoch-tagged words,
 // which the next leaf on the chain polls directly (no flag round trip on the
 // chain), and as plain words announced by a flag after the stores have completed,
-// which every later leaf reads. Thread t owns row t/4 and the column quarter t%4
-// (slots qcol) of every PNB-wide block (partial sums reduced over the four
-// threads). The matrix segment of the next block is loaded before its flag is
-// awaited.
+// which every later leaf reads.
+// The fold runs on the vector units: thread t owns row t/4 and the column quarter
+// t%4 (slots qcol) of every PNB-wide block, partial sums reduced over the four
+// threads; the matrix segment of the next block is loaded before its flag is
+// awaited. The two block products D_i z and W_i n run on the matrix cores: wave w
+// computes rows 16w..16w+15 of the PNB x 8 complex product as a 16 x 16 real tile
+// (columns: the real parts of the 8 right-hand sides, then their imaginary
+// parts) over K = 2 PNB real terms, the lane holding its row's 16 consecutive
+// columns 16 (lane / 16) .. of D_i and W_i; the result is published from the
+// accumulator, one part per lane.
 template <typename T, bool FWD>
 __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   // in a sweep the factorization and solves run beside the next system's
@@ -1433,10 +1499,12 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   __builtin_amdgcn_s_setprio(2);
   constexpr int YS = PNB * 8 + 6;  // staged block result; 2-complex pad per 16-row group
   constexpr int TW = TaggedRow<T>::TW;
+  typedef typename Acc4<T>::type acc4;
   __shared__ T ys[2 * YS];
   __shared__ T bs[2 * PNB * 8];
   __shared__ int gave_up;
   const int t = threadIdx.x, rl = t >> 2, kq = t & 3;
+  const int wv = t >> 6, l16 = t & 15, mq = (t >> 4) & 3;  // wave, tile column / row, K group
   if (t == 0) gave_up = 0;
   const int i = FWD ? (int)blockIdx.x : a.np - 1 - (int)blockIdx.x;
   const int64_t base = (int64_t)i * PNB;
@@ -1451,18 +1519,17 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
     accr[d] = (kq == 0 && rowok) ? a.src[2 * (r * 8 + d)] : T(0);
     acci[d] = (kq == 0 && rowok) ? a.src[2 * (r * 8 + d) + 1] : T(0);
   }
-  // this thread's slots of its row of the diagonal-block inverse and of the link:
-  // complex64 at once (the registers are there), complex128 once the earlier
-  // leaves are folded in
+  // this lane's columns of its tile row of the diagonal-block inverse and of the
+  // link: complex64 at once (the registers are there), complex128 once the
+  // earlier leaves are folded in
   T dr[16], di[16], wr[16], wi[16];
   auto load_dw = [&]() {
-    const int64_t o = 2 * ((int64_t)i * PNB * PNB + (int64_t)rl * PNB);
-    const int nbn = nk > 0 ? (int)min((int64_t)PNB, a.n - (int64_t)(FWD ? i - 1 : i + 1) * PNB) : 0;
-    load_row16(a.dinv + o, nbi, kq, rowok, dr, di);
-    load_row16(a.wlink + o, nbn, kq, rowok, wr, wi);
+    const int64_t o = 2 * ((int64_t)i * PNB * PNB + (int64_t)(16 * wv + l16) * PNB + 16 * mq);
+    load_seg16(a.dinv + o, dr, di);
+    if (nk > 0) load_seg16(a.wlink + o, wr, wi);
   };
   if constexpr (sizeof(T) == 4) load_dw();
-  auto stage = [&](int64_t kb, int kk, int d, T zr, T zi) {
+  auto stage = [&](int kk, int d, T zr, T zi) {
     const int idx = kk * 8 + d;
     ys[2 * (idx + (kk >> 4) * 2)] = zr;
     ys[2 * (idx + (kk >> 4) * 2) + 1] = zi;
@@ -1487,13 +1554,14 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
       const int kk = idx >> 3, d = idx & 7;
       T zr = 0, zi = 0;
       if (kk < nbk) ld_cplx(a.pub + 2 * ((kb + kk) * 8 + d), zr, zi);
-      stage(kb, kk, d, zr, zi);
+      stage(kk, d, zr, zi);
     }
     __syncthreads();
     return true;
   };
   // stages the chain neighbour's tagged result: each thread polls the words of its
-  // two entries (rows t/8 and t/8 + 32, right-hand side t%8) until all carry the epoch
+  // two entries (rows t/8 and t/8 + 32, right-hand side t%8) until all carry the
+  // epoch, with the next poll's loads issued before the current ones are checked
   auto await_tagged = [&](int k) -> bool {
     const int64_t kb = (int64_t)k * PNB;
     const int nbk = (int)min((int64_t)PNB, a.n - kb);
@@ -1502,29 +1570,50 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
     const unsigned long long* q0 = a.tpub + ((kb + kk0) * 8 + d) * TW;
     const unsigned long long* q1 = a.tpub + ((kb + kk1) * 8 + d) * TW;
     const unsigned tag = (unsigned)a.epoch;
-    unsigned long long w0[TW], w1[TW];
-    Spin sp;
-    for (;;) {
-      bool ok = true;
+    unsigned long long a0[TW], a1[TW], b0[TW], b1[TW];
+    auto issue = [&](unsigned long long* x0, unsigned long long* x1) {
 #pragma unroll
       for (int e = 0; e < TW; ++e) {
-        w0[e] = v0 ? ld_u64(q0 + e) : 0ull;
-        w1[e] = v1 ? ld_u64(q1 + e) : 0ull;
+        x0[e] = v0 ? ld_u64(q0 + e) : 0ull;
+        x1[e] = v1 ? ld_u64(q1 + e) : 0ull;
       }
+    };
+    auto ready = [&](const unsigned long long* x0, const unsigned long long* x1) {
+      bool ok = true;
 #pragma unroll
-      for (int e = 0; e < TW; ++e) ok = ok && (!v0 || (unsigned)(w0[e] >> 32) == tag) && (!v1 || (unsigned)(w1[e] >> 32) == tag);
-      if (ok) break;
+      for (int e = 0; e < TW; ++e)
+        ok = ok && (!v0 || (unsigned)(x0[e] >> 32) == tag) && (!v1 || (unsigned)(x1[e] >> 32) == tag);
+      return ok;
+    };
+    auto take = [&](const unsigned long long* x0, const unsigned long long* x1) {
+      T zr, zi;
+      untag(x0, zr, zi);
+      stage(kk0, d, zr, zi);
+      untag(x1, zr, zi);
+      stage(kk1, d, zr, zi);
+    };
+    Spin sp;
+    issue(a0, a1);
+    for (;;) {
+      issue(b0, b1);
+      if (ready(a0, a1)) {
+        take(a0, a1);
+        break;
+      }
       if (spin_expired(a.ctl, sp, site)) {
         gave_up = 1;
         break;
       }
-      __builtin_amdgcn_s_sleep(1);
+      issue(a0, a1);
+      if (ready(b0, b1)) {
+        take(b0, b1);
+        break;
+      }
+      if (spin_expired(a.ctl, sp, site)) {
+        gave_up = 1;
+        break;
+      }
     }
-    T zr, zi;
-    untag(w0, zr, zi);
-    stage(kb, kk0, d, zr, zi);
-    untag(w1, zr, zi);
-    stage(kb, kk1, d, zr, zi);
     __syncthreads();
     return !gave_up;
   };
@@ -1542,11 +1631,18 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
       }
     }
   };
-  auto row_reduce = [&]() {
+  // c += s M[tile rows, :] Z for a staged PNB x 8 block Z (pad: ys layout) and
+  // sign s: step j takes columns j, 16 + j, 32 + j, 48 + j
+  auto tile_product = [&](acc4& c, const T* mr, const T* mi, const T* zs, bool pad, T sgn) {
+    const int d = l16 & 7;
+    const bool imc = l16 >= 8;
 #pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      accr[d] = quad_sum(accr[d]);
-      acci[d] = quad_sum(acci[d]);
+    for (int j = 0; j < 16; ++j) {
+      const int k = 16 * mq + j;
+      const int o = k * 8 + d + (pad ? mq * 2 : 0);
+      const T zr = sgn * zs[2 * o], zi = sgn * zs[2 * o + 1];
+      c = mfma16(mr[j], imc ? zi : zr, c);
+      c = mfma16(mi[j], imc ? zr : -zi, c);
     }
   };
   // off the chain: every leaf but the neighbour
@@ -1563,8 +1659,12 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
     apply(mr, mi);
   }
   if constexpr (sizeof(T) == 8) load_dw();
-  // z = D acc: full row sums over the four threads, then the diagonal block
-  row_reduce();
+  // z = D acc: the full row sums staged, then the tile product
+#pragma unroll
+  for (int d = 0; d < 8; ++d) {
+    accr[d] = quad_sum(accr[d]);
+    acci[d] = quad_sum(acci[d]);
+  }
   if (kq == 0)
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
@@ -1572,46 +1672,35 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
       bs[2 * (rl * 8 + d) + 1] = acci[d];
     }
   __syncthreads();
+  acc4 c;
 #pragma unroll
-  for (int d = 0; d < 8; ++d) accr[d] = acci[d] = 0;
-#pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int q = qcol(u, kq);
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      const T br = bs[2 * (q * 8 + d)], bi = bs[2 * (q * 8 + d) + 1];
-      accr[d] += dr[u] * br - di[u] * bi;
-      acci[d] += dr[u] * bi + di[u] * br;
-    }
-  }
-  // on the chain: each thread holds its quarter's part of z; it subtracts its
-  // quarter's part of W n, and the row sums then give z - W n
+  for (int v = 0; v < 4; ++v) c[v] = 0;
+  tile_product(c, dr, di, bs, false, T(1));
+  // on the chain: z - W n
   if (nk > 0) {
     if (!await_tagged(FWD ? i - 1 : i + 1)) return;
-    apply(wr, wi);
+    tile_product(c, wr, wi, ys, true, T(-1));
   }
-  row_reduce();
-  // publish: thread kq stores columns 2kq, 2kq+1 of its row, the tagged words first
-  if (rowok)
+  // publish: lane l16 < 8 the real part of right-hand side l16, the others the
+  // imaginary part of l16 - 8; the tagged words first
+  const int d = l16 & 7, part = l16 >> 3;
 #pragma unroll
-    for (int d = 0; d < 8; ++d)
-      if ((d >> 1) == kq) st_tagged(a.tpub + (r * 8 + d) * TW, accr[d], acci[d], (unsigned)a.epoch);
-  if (rowok)
+  for (int v = 0; v < 4; ++v) {
+    const int rr = 16 * wv + mfma_row<T>(mq, v);
+    if (rr < nbi) st_tagged_part(a.tpub + ((base + rr) * 8 + d) * TW + part * (TW / 2), c[v], (unsigned)a.epoch);
+  }
 #pragma unroll
-    for (int d = 0; d < 8; ++d)
-      if ((d >> 1) == kq) {
-        st_cplx(a.pub + 2 * (r * 8 + d), accr[d], acci[d]);
-        if (!FWD && d < a.nrhs) {
-          a.out[2 * (r * a.nrhs + d)] = accr[d];
-          a.out[2 * (r * a.nrhs + d) + 1] = acci[d];
-        }
-      }
+  for (int v = 0; v < 4; ++v) {
+    const int rr = 16 * wv + mfma_row<T>(mq, v);
+    if (rr < nbi) {
+      st_part(a.pub + 2 * ((base + rr) * 8 + d) + part, c[v]);
+      if (!FWD && d < a.nrhs) a.out[2 * ((base + rr) * a.nrhs + d) + part] = c[v];
+    }
+  }
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (t == 0) st_flag(a.flags + i, a.epoch);
 }
-
-
 
 // r = b - A x, complex128, 8 right-hand sides (zero-padded). Reading A once is
 // the cost, so lanes run along the row (coalesced 1 KB per wavefront load): each
