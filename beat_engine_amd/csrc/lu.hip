@@ -1213,23 +1213,25 @@ hipError_t gemm_set(const T* A, int64_t lda, const T* B, int64_t ldb, T* C, int6
 
 // ---- solves ----------------------------------------------------------------------
 
-// Per leaf i of the block triangular solves, four triangular solves with the
-// leaf's diagonal block and 64 right-hand-side columns, one workgroup each
-// (blockIdx.y): the inverses Linv_i (L X = I) and Uinv_i (U X = I), and the link
-// matrices W_i = Linv_i L_i,i-1 (L W = L_i,i-1, rows of L in final order through
-// the gather table of leaf i-1's outer block) and W_i = Uinv_i U_i,i+1 (U W =
-// U_i,i+1); zero for the chain's first leaf. The diagonal block is staged in LDS
-// (identity beyond the leaf's extent). All columns are solved at once: four lanes
-// per column, lane kq holding the column's rows q = kq (mod 4) in registers, each
-// row's sum over the solved rows split over the four lanes and combined by two
-// exchanges.
+// The chain matrices of the block triangular solves (leaf = PNB rows; chain
+// position p of leaf i: i forward, np - 1 - i backward). Per leaf, direction
+// (blockIdx.y / 2: L forward, U backward) and kind (blockIdx.y % 2) one
+// triangular solve with the leaf's diagonal block and 64 right-hand-side
+// columns, one workgroup each: kind 0 the inverse D_i (X = I), kind 1 the link
+// W_i = D_i M_i,k to the leaf k one position earlier on the chain (zero for the
+// first). L's rows are taken in final order through the gather table of leaf
+// k's outer block. Results go to mats + ((dir * 2 + kind) np + i) PNB^2
+// (row-major blocks). The diagonal block is staged in LDS (identity beyond the
+// leaf's extent). All columns are solved at once: four lanes per column, lane kq
+// holding the column's rows q = kq (mod 4) in registers, each row's sum over the
+// solved rows split over the four lanes and combined by two exchanges.
 template <typename T>
 __global__ __launch_bounds__(256) void solve_blocks_kernel(const T* LU, int64_t n, int64_t lda, const int* G, int bw,
-                                                           int np, T* Linv, T* Uinv, T* WL, T* WU) {
+                                                           int np, T* mats) {
   constexpr int DL = PNB + 1;
   __shared__ T As[2 * PNB * DL];  // the diagonal block
-  const int i = blockIdx.x, task = blockIdx.y, t = threadIdx.x, c = t >> 2, kq = t & 3;
-  const bool lower = task == 0 || task == 2;
+  const int i = blockIdx.x, dir = blockIdx.y >> 1, kind = blockIdx.y & 1, t = threadIdx.x, c = t >> 2, kq = t & 3;
+  const bool lower = dir == 0;
   const int64_t base = (int64_t)i * PNB;
   const int nbi = (int)min((int64_t)PNB, n - base);
   for (int idx = t; idx < PNB * PNB; idx += 256) {
@@ -1247,24 +1249,24 @@ __global__ __launch_bounds__(256) void solve_blocks_kernel(const T* LU, int64_t 
   T br[16], bi[16], xr[16], xi[16];
 #pragma unroll
   for (int sl = 0; sl < 16; ++sl) xr[sl] = xi[sl] = 0;
-  if (task < 2) {
+  if (kind == 0) {
 #pragma unroll
     for (int sl = 0; sl < 16; ++sl) {
       br[sl] = 4 * sl + kq == c ? T(1) : T(0);
       bi[sl] = 0;
     }
   } else {
-    const bool fwd = task == 2;
-    const bool has = fwd ? i > 0 : i + 1 < np;
-    const int nb = has ? (int)(fwd ? PNB : min((int64_t)PNB, n - base - PNB)) : 0;
-    const int64_t c0 = fwd ? base - PNB : base + PNB;  // the neighbour's first column
-    const int* Gb = G + (fwd ? (c0 / bw) * n + base : 0);
+    const int k = lower ? i - 1 : i + 1;  // the leaf one position earlier on the chain
+    const bool has = lower ? k >= 0 : k < np;
+    const int64_t c0 = (int64_t)k * PNB;  // its first column
+    const int nb = has ? (int)min((int64_t)PNB, n - c0) : 0;
+    const int* Gb = G + (lower && has ? (c0 / bw) * n + base : 0);
     // the rows first (independent loads), then the entries
     int64_t row[16];
 #pragma unroll
     for (int sl = 0; sl < 16; ++sl) {
       const int q = 4 * sl + kq;
-      row[sl] = (q < nbi && c < nb) ? (fwd ? (int64_t)Gb[q] : base + q) : -1;
+      row[sl] = (q < nbi && c < nb) ? (lower ? (int64_t)Gb[q] : base + q) : -1;
     }
 #pragma unroll
     for (int sl = 0; sl < 16; ++sl) {
@@ -1316,7 +1318,7 @@ __global__ __launch_bounds__(256) void solve_blocks_kernel(const T* LU, int64_t 
       xi[sl] = own ? ri : xi[sl];
     }
   }
-  T* out = (task == 0 ? Linv : task == 1 ? Uinv : task == 2 ? WL : WU) + 2 * (int64_t)i * PNB * PNB;
+  T* out = mats + 2 * (((int64_t)(dir * 2 + kind) * np + i) * PNB * PNB);
 #pragma unroll
   for (int sl = 0; sl < 16; ++sl) {
     const int q = 4 * sl + kq;
@@ -1396,8 +1398,7 @@ template <typename T>
 struct TriArgs {
   const T* LU;
   int64_t n, lda;
-  const T* dinv;      // per leaf PNB x PNB: Linv (forward) or Uinv (backward)
-  const T* wlink;     // per leaf PNB x PNB: the link to the previous (forward) / next (backward) leaf
+  const T* mats;      // this direction's chain matrices: D, W x np blocks of PNB x PNB (solve_blocks_kernel)
   const int* gather;  // forward: outer-block row gather tables (nJ x n)
   int bw;             // outer block width of the factorization
   const T* src;       // n x 8: forward P b, backward y
@@ -1435,21 +1436,29 @@ template <typename T>
 __device__ __forceinline__ int mfma_row(int q, int v) {
   return sizeof(T) == 4 ? 4 * q + v : q + 4 * v;
 }
-// 16 consecutive complex entries (p 16-byte aligned)
-__device__ __forceinline__ void load_seg16(const float* p, float* vr, float* vi) {
+// 16 consecutive complex entries (p 16-byte aligned); entries at or beyond nv,
+// and all when !ok, are zero (and not read)
+__device__ __forceinline__ void load_seg16(const float* p, int nv, bool ok, float* vr, float* vi) {
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
-    const float4 v = *reinterpret_cast<const float4*>(p + 4 * j);
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ok && 2 * j + 1 < nv) {
+      v = *reinterpret_cast<const float4*>(p + 4 * j);
+    } else if (ok && 2 * j < nv) {
+      v.x = p[4 * j];
+      v.y = p[4 * j + 1];
+    }
     vr[2 * j] = v.x;
     vi[2 * j] = v.y;
     vr[2 * j + 1] = v.z;
     vi[2 * j + 1] = v.w;
   }
 }
-__device__ __forceinline__ void load_seg16(const double* p, double* vr, double* vi) {
+__device__ __forceinline__ void load_seg16(const double* p, int nv, bool ok, double* vr, double* vi) {
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    const double2 v = *reinterpret_cast<const double2*>(p + 2 * j);
+    double2 v = make_double2(0.0, 0.0);
+    if (ok && j < nv) v = *reinterpret_cast<const double2*>(p + 2 * j);
     vr[j] = v.x;
     vi[j] = v.y;
   }
@@ -1472,26 +1481,23 @@ __device__ __forceinline__ void st_part(double* p, double v) {
 }
 
 // Persistent block triangular solve with 8 right-hand sides (zero-padded): one
-// workgroup per leaf of PNB rows, all resident (cooperative launch). Forward
-// (L y = P b) workgroup i consumes the results of leaves 0..i-1 in order as their
-// flags appear, backward (U x = y) those of leaves np-1..i+1. Every leaf but the
-// neighbour on the chain (i-1 forward, i+1 backward) is folded into
-// z_i = D_i (src_i - sum_k M_ik z_k) while the chain is still further away; when
-// the neighbour's result n arrives, the leaf's own is z_i - W_i n with the link
-// matrix W_i = D_i M_i,nb formed at solve preparation, so one block product sits
-// on the chain per leaf. A result is published twice: as epoch-tagged words,
-// which the next leaf on the chain polls directly (no flag round trip on the
-// chain), and as plain words announced by a flag after the stores have completed,
-// which every later leaf reads.
-// The fold runs on the vector units: thread t owns row t/4 and the column quarter
-// t%4 (slots qcol) of every PNB-wide block, partial sums reduced over the four
-// threads; the matrix segment of the next block is loaded before its flag is
-// awaited. The two block products D_i z and W_i n run on the matrix cores: wave w
-// computes rows 16w..16w+15 of the PNB x 8 complex product as a 16 x 16 real tile
-// (columns: the real parts of the 8 right-hand sides, then their imaginary
-// parts) over K = 2 PNB real terms, the lane holding its row's 16 consecutive
-// columns 16 (lane / 16) .. of D_i and W_i; the result is published from the
-// accumulator, one part per lane.
+// workgroup per leaf of PNB rows; workgroup p takes chain position p (leaf p
+// forward, L y = P b; leaf np - 1 - p backward, U x = y) and waits only for
+// earlier positions (launch_chain). Every leaf but the neighbour on the chain is
+// folded into a_p = src_p - sum_k M_pk x_k while the chain is still further
+// away, then z_p = D_p a_p; when the neighbour's result n arrives, the leaf's own
+// is z_p - W_p n with the link matrix W_p = D_p M_p,nb of solve_blocks_kernel,
+// so one block product sits on the chain per leaf. A result is published twice:
+// as epoch-tagged words, which the next leaf on the chain polls directly (no
+// flag round trip on the chain), and as plain words announced by a flag after
+// the stores have completed, which the folds of later leaves read.
+// All block products run on the matrix cores: wave w computes rows 16w..16w+15 of
+// a PNB x 8 complex product as a 16 x 16 real tile (columns: the real parts of
+// the 8 right-hand sides, then their imaginary parts) over K = 2 PNB real terms,
+// the lane holding its row's 16 consecutive columns 16 (lane / 16) .. of the
+// left factor; the fold accumulates in the tile, which is published from the
+// accumulator, one part per lane. The next fold block's rows are loaded before
+// its flag is awaited.
 template <typename T, bool FWD>
 __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   // in a sweep the factorization and solves run beside the next system's
@@ -1499,38 +1505,40 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   __builtin_amdgcn_s_setprio(2);
   constexpr int YS = PNB * 8 + 6;  // staged block result; 2-complex pad per 16-row group
   constexpr int TW = TaggedRow<T>::TW;
+  constexpr int64_t BB = (int64_t)PNB * PNB;
   typedef typename Acc4<T>::type acc4;
   __shared__ T ys[2 * YS];
   __shared__ T bs[2 * PNB * 8];
   __shared__ int gave_up;
-  const int t = threadIdx.x, rl = t >> 2, kq = t & 3;
+  const int t = threadIdx.x;
   const int wv = t >> 6, l16 = t & 15, mq = (t >> 4) & 3;  // wave, tile column / row, K group
+  const int d = l16 & 7, part = l16 >> 3;                   // this lane's tile column: right-hand side, part
   if (t == 0) gave_up = 0;
-  const int i = FWD ? (int)blockIdx.x : a.np - 1 - (int)blockIdx.x;
+  const int p = (int)blockIdx.x;
+  auto leaf = [&](int pos) { return FWD ? pos : a.np - 1 - pos; };
+  const int i = leaf(p);
   const int64_t base = (int64_t)i * PNB;
   const int nbi = (int)min((int64_t)PNB, a.n - base);
-  const int64_t r = base + rl;
-  const bool rowok = rl < nbi;
-  const int nk = FWD ? i : a.np - 1 - i;  // leaves before this one on the chain
   const int site = FWD ? SPIN_TRI_FWD : SPIN_TRI_BWD;
-  T accr[8], acci[8];
+  const int ar = 16 * wv + l16;  // this lane's row of a left factor
+  // the accumulator starts from the right-hand side
+  acc4 ca;
 #pragma unroll
-  for (int d = 0; d < 8; ++d) {
-    accr[d] = (kq == 0 && rowok) ? a.src[2 * (r * 8 + d)] : T(0);
-    acci[d] = (kq == 0 && rowok) ? a.src[2 * (r * 8 + d) + 1] : T(0);
+  for (int v = 0; v < 4; ++v) {
+    const int rr = 16 * wv + mfma_row<T>(mq, v);
+    ca[v] = rr < nbi ? a.src[2 * ((base + rr) * 8 + d) + part] : T(0);
   }
-  // this lane's columns of its tile row of the diagonal-block inverse and of the
-  // link: complex64 at once (the registers are there), complex128 once the
-  // earlier leaves are folded in
+  // this lane's columns of its rows of D and W: complex64 at once (the registers
+  // are there), complex128 once the earlier leaves are folded in
   T dr[16], di[16], wr[16], wi[16];
   auto load_dw = [&]() {
-    const int64_t o = 2 * ((int64_t)i * PNB * PNB + (int64_t)(16 * wv + l16) * PNB + 16 * mq);
-    load_seg16(a.dinv + o, dr, di);
-    if (nk > 0) load_seg16(a.wlink + o, wr, wi);
+    const int64_t o = 2 * ((int64_t)i * BB + (int64_t)ar * PNB + 16 * mq);
+    load_seg16(a.mats + o, 16, true, dr, di);
+    if (p > 0) load_seg16(a.mats + 2 * a.np * BB + o, 16, true, wr, wi);
   };
   if constexpr (sizeof(T) == 4) load_dw();
-  auto stage = [&](int kk, int d, T zr, T zi) {
-    const int idx = kk * 8 + d;
+  auto stage = [&](int kk, int dd, T zr, T zi) {
+    const int idx = kk * 8 + dd;
     ys[2 * (idx + (kk >> 4) * 2)] = zr;
     ys[2 * (idx + (kk >> 4) * 2) + 1] = zi;
   };
@@ -1551,10 +1559,10 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
     const int64_t kb = (int64_t)k * PNB;
     const int nbk = (int)min((int64_t)PNB, a.n - kb);
     for (int idx = t; idx < PNB * 8; idx += 256) {
-      const int kk = idx >> 3, d = idx & 7;
+      const int kk = idx >> 3, dd = idx & 7;
       T zr = 0, zi = 0;
-      if (kk < nbk) ld_cplx(a.pub + 2 * ((kb + kk) * 8 + d), zr, zi);
-      stage(kk, d, zr, zi);
+      if (kk < nbk) ld_cplx(a.pub + 2 * ((kb + kk) * 8 + dd), zr, zi);
+      stage(kk, dd, zr, zi);
     }
     __syncthreads();
     return true;
@@ -1565,10 +1573,10 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
   auto await_tagged = [&](int k) -> bool {
     const int64_t kb = (int64_t)k * PNB;
     const int nbk = (int)min((int64_t)PNB, a.n - kb);
-    const int kk0 = t >> 3, kk1 = kk0 + 32, d = t & 7;
+    const int kk0 = t >> 3, kk1 = kk0 + 32, dd = t & 7;
     const bool v0 = kk0 < nbk, v1 = kk1 < nbk;
-    const unsigned long long* q0 = a.tpub + ((kb + kk0) * 8 + d) * TW;
-    const unsigned long long* q1 = a.tpub + ((kb + kk1) * 8 + d) * TW;
+    const unsigned long long* q0 = a.tpub + ((kb + kk0) * 8 + dd) * TW;
+    const unsigned long long* q1 = a.tpub + ((kb + kk1) * 8 + dd) * TW;
     const unsigned tag = (unsigned)a.epoch;
     unsigned long long a0[TW], a1[TW], b0[TW], b1[TW];
     auto issue = [&](unsigned long long* x0, unsigned long long* x1) {
@@ -1588,9 +1596,9 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
     auto take = [&](const unsigned long long* x0, const unsigned long long* x1) {
       T zr, zi;
       untag(x0, zr, zi);
-      stage(kk0, d, zr, zi);
+      stage(kk0, dd, zr, zi);
       untag(x1, zr, zi);
-      stage(kk1, d, zr, zi);
+      stage(kk1, dd, zr, zi);
     };
     Spin sp;
     issue(a0, a1);
@@ -1617,25 +1625,10 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
     __syncthreads();
     return !gave_up;
   };
-  // acc -= M[rows, quarter] ys[quarter]
-  auto apply = [&](const T* mr, const T* mi) {
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const int kk = qcol(u, kq);
-      const int o = kk * 8 + (kk >> 4) * 2;
-#pragma unroll
-      for (int d = 0; d < 8; ++d) {
-        const T zr = ys[2 * (o + d)], zi = ys[2 * (o + d) + 1];
-        accr[d] -= mr[u] * zr - mi[u] * zi;
-        acci[d] -= mr[u] * zi + mi[u] * zr;
-      }
-    }
-  };
-  // c += s M[tile rows, :] Z for a staged PNB x 8 block Z (pad: ys layout) and
-  // sign s: step j takes columns j, 16 + j, 32 + j, 48 + j
+  // c += s M[tile rows, :] Z for a staged PNB x 8 block Z (pad: the ys layout)
+  // and sign s: step j takes columns j, 16 + j, 32 + j, 48 + j
   auto tile_product = [&](acc4& c, const T* mr, const T* mi, const T* zs, bool pad, T sgn) {
-    const int d = l16 & 7;
-    const bool imc = l16 >= 8;
+    const bool imc = part != 0;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int k = 16 * mq + j;
@@ -1646,44 +1639,44 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
     }
   };
   // off the chain: every leaf but the neighbour
-  for (int s = 0; s + 1 < nk; ++s) {
-    const int k = FWD ? s : a.np - 1 - s;
+  for (int s = 0; s + 1 < p; ++s) {
+    const int k = leaf(s);
     const int64_t kb = (int64_t)k * PNB;
-    const int nbk = (int)min((int64_t)PNB, a.n - kb);
+    const int nv = (int)min((int64_t)PNB, a.n - kb) - 16 * mq;  // valid columns of this lane's segment
     T mr[16], mi[16];
     {
-      const int64_t srow = FWD ? (rowok ? (int64_t)a.gather[(kb / a.bw) * a.n + r] : 0) : r;
-      load_row16(a.LU + 2 * (srow * a.lda + kb), nbk, kq, rowok, mr, mi);
+      const bool ok = ar < nbi;
+      const int64_t srow = FWD ? (ok ? (int64_t)a.gather[(kb / a.bw) * a.n + base + ar] : 0) : base + ar;
+      load_seg16(a.LU + 2 * (srow * a.lda + kb + 16 * mq), nv, ok, mr, mi);
     }
     if (!await_stage(k)) return;
-    apply(mr, mi);
+    // each block's product summed on its own, then added: rounding grows with the
+    // 2 PNB terms of a block and the number of blocks, not with their product
+    acc4 tb;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) tb[v] = 0;
+    tile_product(tb, mr, mi, ys, true, T(-1));
+    ca += tb;
   }
   if constexpr (sizeof(T) == 8) load_dw();
-  // z = D acc: the full row sums staged, then the tile product
+  // z = D a: the accumulator staged as the right operand
 #pragma unroll
-  for (int d = 0; d < 8; ++d) {
-    accr[d] = quad_sum(accr[d]);
-    acci[d] = quad_sum(acci[d]);
+  for (int v = 0; v < 4; ++v) {
+    const int rr = 16 * wv + mfma_row<T>(mq, v);
+    bs[2 * (rr * 8 + d) + part] = ca[v];
   }
-  if (kq == 0)
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      bs[2 * (rl * 8 + d)] = accr[d];
-      bs[2 * (rl * 8 + d) + 1] = acci[d];
-    }
   __syncthreads();
   acc4 c;
 #pragma unroll
   for (int v = 0; v < 4; ++v) c[v] = 0;
   tile_product(c, dr, di, bs, false, T(1));
   // on the chain: z - W n
-  if (nk > 0) {
-    if (!await_tagged(FWD ? i - 1 : i + 1)) return;
+  if (p > 0) {
+    if (!await_tagged(leaf(p - 1))) return;
     tile_product(c, wr, wi, ys, true, T(-1));
   }
   // publish: lane l16 < 8 the real part of right-hand side l16, the others the
   // imaginary part of l16 - 8; the tagged words first
-  const int d = l16 & 7, part = l16 >> 3;
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int rr = 16 * wv + mfma_row<T>(mq, v);
@@ -2348,6 +2341,15 @@ static hipError_t launch_panel(T* A, int64_t n, int64_t lda, int64_t j0, int b, 
   return launch_resident<PanelArgs<T>>(reinterpret_cast<const void*>(&panel_kernel<T>), panel_kernel<T>, G, 256, pa, s);
 }
 
+// The triangular solves wait only for workgroups of lower index, which the
+// dispatcher places first: the lowest unfinished one is always resident, so the
+// grid need not be.
+template <typename Args>
+static hipError_t launch_chain(void (*fn)(Args), int grid, Args& args, hipStream_t s) {
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, s, args);
+  return hipGetLastError();
+}
+
 // The speculative leaf: diagonal block, then the rows below it.
 template <typename T>
 static hipError_t launch_nopiv_leaf(T* A, int64_t n, int64_t lda, int64_t j0, int b, LuWork& w, T* linv,
@@ -2531,21 +2533,19 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
 // from (nJ x n).
 struct SolveLayout {
   int np, nJ;
-  size_t linv, uinv, b0, yf, xb, wl, wu, tf, tb, cend;  // complex offsets
+  size_t mat, b0, yf, xb, tf, tb, cend;  // complex offsets
 };
 static SolveLayout solve_layout(int64_t n) {
   SolveLayout L;
   L.np = (int)((n + PNB - 1) / PNB);
   L.nJ = (int)((n + MAXBW - 1) / MAXBW);
-  L.linv = 0;
-  L.uinv = L.linv + (size_t)L.np * PNB * PNB;
-  L.b0 = L.uinv + (size_t)L.np * PNB * PNB;
+  // chain matrices: 2 directions x 2 kinds (solve_blocks_kernel) x np blocks
+  L.mat = 0;
+  L.b0 = L.mat + (size_t)4 * L.np * PNB * PNB;
   L.yf = L.b0 + (size_t)n * 8;
   L.xb = L.yf + (size_t)n * 8;
-  L.wl = L.xb + (size_t)n * 8;
-  L.wu = L.wl + (size_t)L.np * PNB * PNB;
   // tagged results (TaggedRow: twice a complex entry's size), forward and backward
-  L.tf = L.wu + (size_t)L.np * PNB * PNB;
+  L.tf = L.xb + (size_t)n * 8;
   L.tb = L.tf + (size_t)2 * n * 8;
   L.cend = L.tb + (size_t)2 * n * 8;
   return L;
@@ -2583,7 +2583,7 @@ hipError_t lu_solve_prepare(const T* LU, int64_t n, int64_t lda, const int* ipiv
                      perm);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL((solve_blocks_kernel<T>), dim3((unsigned)L.np, 4), dim3(256), 0, s, LU, n, lda, (const int*)G,
-                     MAXBW, L.np, work + 2 * L.linv, work + 2 * L.uinv, work + 2 * L.wl, work + 2 * L.wu);
+                     MAXBW, L.np, work + 2 * L.mat);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   // tagged results: no word may carry a later solve's epoch by accident
   if ((e = launch_fill<int>(reinterpret_cast<int*>(work + 2 * L.tf), (int64_t)4 * 2 * n * 8 * sizeof(T) / sizeof(int),
@@ -2615,26 +2615,22 @@ hipError_t lu_solve(const T* LU, int64_t n, int64_t lda, T* B, int nrhs, T* work
   ta.nrhs = nrhs;
   ta.out = B;
   // forward
-  ta.dinv = work + 2 * L.linv;
-  ta.wlink = work + 2 * L.wl;
+  ta.mats = work + 2 * L.mat;
   ta.gather = G;
   ta.src = work + 2 * L.b0;
   ta.pub = work + 2 * L.yf;
   ta.flags = flags;
   ta.tpub = reinterpret_cast<unsigned long long*>(work + 2 * L.tf);
-  e = launch_resident<TriArgs<T>>(reinterpret_cast<const void*>(&tri_solve_kernel<T, true>), tri_solve_kernel<T, true>,
-                                  L.np, 256, ta, s);
+  e = launch_chain(tri_solve_kernel<T, true>, L.np, ta, s);
   if (e != hipSuccess) return e;
   // backward
-  ta.dinv = work + 2 * L.uinv;
-  ta.wlink = work + 2 * L.wu;
+  ta.mats = work + 2 * (L.mat + (size_t)2 * L.np * PNB * PNB);
   ta.gather = nullptr;
   ta.src = work + 2 * L.yf;
   ta.pub = work + 2 * L.xb;
   ta.flags = flags + L.np;
   ta.tpub = reinterpret_cast<unsigned long long*>(work + 2 * L.tb);
-  return launch_resident<TriArgs<T>>(reinterpret_cast<const void*>(&tri_solve_kernel<T, false>),
-                                     tri_solve_kernel<T, false>, L.np, 256, ta, s);
+  return launch_chain(tri_solve_kernel<T, false>, L.np, ta, s);
 }
 
 // Right-hand sides carried by the factorization: in (n x nrhs) to the LU's
@@ -2689,15 +2685,13 @@ hipError_t lu_solve_eliminated(const T* LU, int64_t n, int64_t lda, T* B, int nr
   ta.np = L.np;
   ta.nrhs = nrhs;
   ta.out = B;
-  ta.dinv = work + 2 * L.uinv;
-  ta.wlink = work + 2 * L.wu;
+  ta.mats = work + 2 * (L.mat + (size_t)2 * L.np * PNB * PNB);
   ta.gather = nullptr;
   ta.src = work + 2 * L.yf;
   ta.pub = work + 2 * L.xb;
   ta.flags = flags + L.np;
   ta.tpub = reinterpret_cast<unsigned long long*>(work + 2 * L.tb);
-  return launch_resident<TriArgs<T>>(reinterpret_cast<const void*>(&tri_solve_kernel<T, false>),
-                                     tri_solve_kernel<T, false>, L.np, 256, ta, s);
+  return launch_chain(tri_solve_kernel<T, false>, L.np, ta, s);
 }
 
 template hipError_t lu_append_rhs<float>(float*, int64_t, int64_t, const float*, int, hipStream_t);
