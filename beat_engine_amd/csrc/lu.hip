@@ -1902,6 +1902,121 @@ __global__ __launch_bounds__(512) void residual64_split_kernel(const double* A, 
   if (row < n) part[((int64_t)blockIdx.y * n + row) * 16 + d * 2 + cpart] = acc[0];
 }
 
+// The same residual in small workgroups, for a GPU shared with other streams'
+// long kernels (a sweep's next assembly): 4 wavefronts of 2 rows x 8 right-hand
+// sides each (164 VGPRs), so that a workgroup fits the slot one retired
+// workgroup of the other kernel leaves on a CU, where residual64_split_kernel's
+// 8 wavefronts at 220 VGPRs wait for a whole CU to drain (in the L5 sweep the
+// solve and refinement take 4.5 against 8.4 ms; alone it is slower, 615 against
+// 419 us). Each wavefront keeps the next two chunks' A values in flight (three
+// register buffers, rotated by unrolling) while the current chunk is
+// multiplied; x in two LDS buffers, one barrier per chunk; one k range.
+constexpr int RS_ROWS = 2, RS_WAVES = 4;  // rows per wavefront, wavefronts per workgroup
+__global__ __launch_bounds__(64 * RS_WAVES) void residual64_narrow_kernel(const double* A, int64_t n, int64_t lda,
+                                                                         const double* x, int nrhs, double* part) {
+  __builtin_amdgcn_s_setprio(2);
+  __shared__ double xs[2][8 * RS_KC * 2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t row0 = ((int64_t)blockIdx.x * RS_WAVES + wave) * RS_ROWS;
+  const int64_t nch = (n + RS_KC - 1) / RS_KC;
+  const int64_t per = (nch + gridDim.y - 1) / gridDim.y;
+  const int64_t c0 = (int64_t)blockIdx.y * per, c1 = min(nch, c0 + per);
+  double acc[RS_ROWS * 8 * 2];
+#pragma unroll
+  for (int j = 0; j < RS_ROWS * 8 * 2; ++j) acc[j] = 0;
+  // column lane + 64 s of a chunk; thread tid stages the x entries (k = tid / 8 +
+  // 32 e, right-hand side tid % 8)
+  constexpr int XE = RS_KC * 8 / (64 * RS_WAVES);
+  const int xk = tid >> 3, xd = tid & 7;
+  auto load_a = [&](int64_t c, double (*ar)[RS_SUB], double (*ai)[RS_SUB]) {
+#pragma unroll
+    for (int q = 0; q < RS_ROWS; ++q)
+#pragma unroll
+      for (int sb = 0; sb < RS_SUB; ++sb) {
+        const int64_t row = row0 + q, k = c * RS_KC + lane + 64 * sb;
+        ar[q][sb] = ai[q][sb] = 0;
+        if (row < n && k < n) {
+          const double2 v = *reinterpret_cast<const double2*>(A + 2 * (row * lda + k));
+          ar[q][sb] = v.x;
+          ai[q][sb] = v.y;
+        }
+      }
+  };
+  auto load_x = [&](int64_t c, double2* v) {
+#pragma unroll
+    for (int e = 0; e < XE; ++e) {
+      const int64_t k = c * RS_KC + xk + 8 * RS_WAVES * e;
+      v[e] = make_double2(0.0, 0.0);
+      if (k < n && xd < nrhs) v[e] = *reinterpret_cast<const double2*>(x + 2 * (k * nrhs + xd));
+    }
+  };
+  auto put_x = [&](int buf, const double2* v) {
+#pragma unroll
+    for (int e = 0; e < XE; ++e) {
+      xs[buf][2 * (xd * RS_KC + xk + 8 * RS_WAVES * e)] = v[e].x;
+      xs[buf][2 * (xd * RS_KC + xk + 8 * RS_WAVES * e) + 1] = v[e].y;
+    }
+  };
+  // A in three register buffers, rotated by unrolling: while chunk c is
+  // multiplied, chunks c + 1 and c + 2 are in flight; x in two LDS buffers
+  double pr[RS_ROWS][RS_SUB], pi[RS_ROWS][RS_SUB], qr[RS_ROWS][RS_SUB], qi[RS_ROWS][RS_SUB];
+  double sr[RS_ROWS][RS_SUB], si[RS_ROWS][RS_SUB];
+  if (c0 < c1) {
+    double2 v[XE];
+    load_a(c0, pr, pi);
+    load_x(c0, v);
+    put_x(0, v);
+  }
+  if (c0 + 1 < c1) load_a(c0 + 1, qr, qi);
+  if (c0 + 2 < c1) load_a(c0 + 2, sr, si);
+  __syncthreads();
+  // chunk c from buffer (ar, ai): the next chunk's x staged during its products
+  auto step = [&](int64_t c, double (*ar)[RS_SUB], double (*ai)[RS_SUB]) {
+    const int buf = (int)((c - c0) & 1);
+    const bool more = c + 1 < c1;
+    double2 xn[XE];
+    if (more) load_x(c + 1, xn);
+#pragma unroll
+    for (int sb = 0; sb < RS_SUB; ++sb)
+#pragma unroll
+      for (int d = 0; d < 8; ++d) {
+        const double xr = xs[buf][2 * (d * RS_KC + lane + 64 * sb)];
+        const double xi = xs[buf][2 * (d * RS_KC + lane + 64 * sb) + 1];
+#pragma unroll
+        for (int q = 0; q < RS_ROWS; ++q) {
+          acc[(q * 8 + d) * 2] = fma(ar[q][sb], xr, fma(-ai[q][sb], xi, acc[(q * 8 + d) * 2]));
+          acc[(q * 8 + d) * 2 + 1] = fma(ar[q][sb], xi, fma(ai[q][sb], xr, acc[(q * 8 + d) * 2 + 1]));
+        }
+      }
+    if (more) put_x(buf ^ 1, xn);
+    __syncthreads();
+  };
+  for (int64_t c = c0; c < c1; c += 3) {
+    step(c, pr, pi);
+    if (c + 3 < c1) load_a(c + 3, pr, pi);
+    if (c + 1 < c1) {
+      step(c + 1, qr, qi);
+      if (c + 4 < c1) load_a(c + 4, qr, qi);
+    }
+    if (c + 2 < c1) {
+      step(c + 2, sr, si);
+      if (c + 5 < c1) load_a(c + 5, sr, si);
+    }
+  }
+  // the two half-wavefronts' sums first, then recursive halving over the 32
+  // sums: lanes l and l + 32 end with sum number l % 32 = (q * 8 + d) * 2 + part
+#pragma unroll
+  for (int j = 0; j < RS_ROWS * 16; ++j) acc[j] += __shfl_xor(acc[j], 32, 64);
+  halve_step<16>(acc, lane);
+  halve_step<8>(acc, lane);
+  halve_step<4>(acc, lane);
+  halve_step<2>(acc, lane);
+  halve_step<1>(acc, lane);
+  const int q = (lane >> 4) & 1, d = (lane >> 1) & 7, cpart = lane & 1;
+  const int64_t row = row0 + q;
+  if (lane < 32 && row < n) part[((int64_t)blockIdx.y * n + row) * 16 + d * 2 + cpart] = acc[0];
+}
+
 // r = b - sum over the k ranges of the partial products
 __global__ void residual64_finish_kernel(const double* part, int ks, int64_t n, const double* b, double* r, int nrhs) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2706,34 +2821,42 @@ hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, 
                       double* part, bool split, hipStream_t s) {
   if (nrhs > 8 || nrhs < 1) return hipErrorInvalidValue;
   if (n <= 0) return hipSuccess;
-  constexpr int rows = RES_WAVES * RES_ROWS;
-  const int64_t nblk = (n + rows - 1) / rows;
-  // k ranges per row block (split): the count (up to RES_SPLIT_MAX) whose grid
-  // comes closest to whole rounds of one workgroup per CU, keeping at least 8
-  // chunks per range; the fewest ranges among equals. Without split, one range:
-  // beside other streams' long kernels (a sweep's next assembly) each workgroup
-  // must wait for a CU to drain, and fewer, longer workgroups do better there.
-  static std::atomic<int> cus_cached{0};
-  int cus = cus_cached.load();
-  if (cus <= 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
-                                                 hipSuccess || cus <= 0)
-      cus = 256;
-    cus_cached.store(cus);
-  }
-  const int64_t nch = (n + RS_KC - 1) / RS_KC;
   int ks = 1;
-  double best = 0;
-  for (int k = 1; split && k <= RES_SPLIT_MAX && nch / k >= 8; ++k) {
-    const double wg = (double)(nblk * k), rounds = std::ceil(wg / cus), eff = wg / (rounds * cus);
-    if (eff > best + 1e-3) {
-      best = eff;
-      ks = k;
+  if (split) {
+    // alone on the GPU: residual64_split_kernel with k ranges per row block, the
+    // count (up to RES_SPLIT_MAX) whose grid comes closest to whole rounds of one
+    // workgroup per CU, keeping at least 8 chunks per range; the fewest ranges
+    // among equals
+    constexpr int rows = RES_WAVES * RES_ROWS;
+    const int64_t nblk = (n + rows - 1) / rows;
+    static std::atomic<int> cus_cached{0};
+    int cus = cus_cached.load();
+    if (cus <= 0) {
+      int dev = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+      cus_cached.store(cus);
     }
+    const int64_t nch = (n + RS_KC - 1) / RS_KC;
+    double best = 0;
+    for (int k = 1; k <= RES_SPLIT_MAX && nch / k >= 8; ++k) {
+      const double wg = (double)(nblk * k), rounds = std::ceil(wg / cus), eff = wg / (rounds * cus);
+      if (eff > best + 1e-3) {
+        best = eff;
+        ks = k;
+      }
+    }
+    hipLaunchKernelGGL(residual64_split_kernel, dim3((unsigned)nblk, (unsigned)ks), dim3(64 * RES_WAVES), 0, s, A, n,
+                       lda, x, nrhs, part);
+  } else {
+    // beside other streams' long kernels (a sweep's next assembly): the small
+    // workgroups of residual64_narrow_kernel, one k range
+    constexpr int rows = RS_WAVES * RS_ROWS;
+    const int64_t nblk = (n + rows - 1) / rows;
+    hipLaunchKernelGGL(residual64_narrow_kernel, dim3((unsigned)nblk, 1), dim3(64 * RS_WAVES), 0, s, A, n, lda, x,
+                       nrhs, part);
   }
-  hipLaunchKernelGGL(residual64_split_kernel, dim3((unsigned)nblk, (unsigned)ks), dim3(64 * RES_WAVES), 0, s, A, n,
-                     lda, x, nrhs, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(residual64_finish_kernel, dim3((unsigned)((n * nrhs * 2 + 255) / 256)), dim3(256), 0, s,
