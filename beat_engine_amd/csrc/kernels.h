@@ -193,7 +193,7 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
                      int64_t nc = 0);
 
 // Forward/backward solves with the factors of lu_factor. lu_solve_prepare runs
-// once per factorization (diagonal-block inverses, row gather tables) into work
+// once per factorization (leaf inverses and link matrices, row gather tables) into work
 // (lu_solve_work_bytes); lu_solve then overwrites B (n x nrhs row-major complex
 // T, nrhs <= 8) with the solution; ctl is the factorization's wait control
 // (LuWork::ctl). Solves sharing one work area run in stream order.
@@ -259,7 +259,8 @@ hipError_t residual64_batched(const BatchSystem<double>* sys, const BatchVec<dou
 // r = b - A x (complex128), A row-major n x n, x and b n x nrhs row-major (nrhs
 // <= 8); part: workspace of residual64_part_bytes(n) for the partial products;
 // split: spread each row block's columns over several workgroups to fill the
-// device (false when other streams keep it busy).
+// device; false when other streams keep it busy (small workgroups that fit beside
+// the other streams' kernels, one k range).
 hipError_t residual64(const double* A, int64_t n, int64_t lda, const double* x, const double* b, double* r, int nrhs,
                       double* part, bool split, hipStream_t s);
 size_t residual64_part_bytes(int64_t n);
