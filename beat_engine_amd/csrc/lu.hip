@@ -1020,14 +1020,65 @@ __global__ __launch_bounds__(256) void zgemm_batched_kernel(const BatchSystem<T>
                                      S.A + 2 * ((j0 + b) * S.lda + j0 + b), S.lda, rows, cols, b, row0, col0);
 }
 
-// Triangular substitutions of one system's vector block by one workgroup (1024
+// 16 x 16 matrix-core tiles of the solves (one wavefront, K = 4 per step):
+// A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k; accumulator element v of
+// lane l is row mfma_row<T>(l / 16, v), column l % 16.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+struct Acc4;
+template <>
+struct Acc4<float> {
+  typedef f32x4 type;
+};
+template <>
+struct Acc4<double> {
+  typedef f64x4 type;
+};
+__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <typename T>
+__device__ __forceinline__ int mfma_row(int q, int v) {
+  return sizeof(T) == 4 ? 4 * q + v : q + 4 * v;
+}
+// 16 consecutive complex entries (p 16-byte aligned); entries at or beyond nv,
+// and all when !ok, are zero (and not read)
+__device__ __forceinline__ void load_seg16(const float* p, int nv, bool ok, float* vr, float* vi) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ok && 2 * j + 1 < nv) {
+      v = *reinterpret_cast<const float4*>(p + 4 * j);
+    } else if (ok && 2 * j < nv) {
+      v.x = p[4 * j];
+      v.y = p[4 * j + 1];
+    }
+    vr[2 * j] = v.x;
+    vi[2 * j] = v.y;
+    vr[2 * j + 1] = v.z;
+    vi[2 * j + 1] = v.w;
+  }
+}
+__device__ __forceinline__ void load_seg16(const double* p, int nv, bool ok, double* vr, double* vi) {
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    double2 v = make_double2(0.0, 0.0);
+    if (ok && j < nv) v = *reinterpret_cast<const double2*>(p + 2 * j);
+    vr[j] = v.x;
+    vi[j] = v.y;
+  }
+}
+// Triangular substitutions of one system's vector block by one workgroup (512
 // threads): forward L y = v (unit lower, below the diagonal of A), then backward
 // U x = y, block of 64 rows by block. Per block the diagonal part is solved by
 // one wavefront per right-hand side (lane = row, the solved entry broadcast by
 // shuffle) from the block staged in LDS; the other rows then subtract the block's
-// contribution, each thread owning rows and reading its row of the block.
+// contribution on the matrix cores (update_rows).
 template <typename T, bool FWD, bool BWD>
-__global__ __launch_bounds__(1024) void tri_wg_kernel(const BatchSystem<T>* sys, const BatchVec<T>* vec, int nrhs) {
+__global__ __launch_bounds__(512) void tri_wg_kernel(const BatchSystem<T>* sys, const BatchVec<T>* vec, int nrhs) {
   const BatchSystem<T> S = sys[blockIdx.x];
   const BatchVec<T> V = vec[blockIdx.x];
   if (V.skip && *V.skip) return;
@@ -1038,11 +1089,41 @@ __global__ __launch_bounds__(1024) void tri_wg_kernel(const BatchSystem<T>* sys,
   T* v = V.v;
   const T* A = S.A;
   const int nblk = (int)((n + PNB - 1) / PNB);
+  typedef typename Acc4<T>::type acc4;
+  // rows [r0, r1) subtract the block's contribution: A[rows, jb..jb+nb) ys, by
+  // 16-row tiles on the matrix cores, tile u on wavefront u % 8 (lanes as in
+  // tri_solve_kernel: row lane % 16, columns 16 (lane / 16) .. + 15 of the block;
+  // accumulator columns: the right-hand sides' real parts, then their imaginary
+  // parts; columns of ys beyond nrhs only reach accumulator columns not stored)
+  auto update_rows = [&](int64_t r0, int64_t r1, int64_t jb, int nb) {
+    const int l16 = lane & 15, mq = lane >> 4, d = l16 & 7, part = l16 >> 3;
+    const int nv = nb - 16 * mq;
+    for (int64_t tr = r0 + 16 * (int64_t)wave; tr < r1; tr += 16 * 8) {
+      const int64_t ar = tr + l16;
+      T mr[16], mi[16];
+      load_seg16(A + 2 * ((ar < r1 ? ar : r0) * lda + jb + 16 * mq), nv, ar < r1, mr, mi);
+      acc4 c;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) c[e] = 0;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int k = 16 * mq + j;
+        const T zr = ys[k][d][0], zi = ys[k][d][1];
+        c = mfma16(mr[j], part ? zi : zr, c);
+        c = mfma16(mi[j], part ? zr : -zi, c);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t row = tr + mfma_row<T>(mq, e);
+        if (row < r1 && d < nrhs) v[2 * (row * ld + d) + part] -= c[e];
+      }
+    }
+  };
   if (FWD)
     for (int kb = 0; kb < nblk; ++kb) {
       const int64_t jb = (int64_t)kb * PNB;
       const int nb = (int)min((int64_t)PNB, n - jb);
-      for (int e = t; e < PNB * PNB; e += 1024) {
+      for (int e = t; e < PNB * PNB; e += 512) {
         const int i = e / PNB, j = e - i * PNB;
         const bool in = i < nb && j < i;
         blk[i][j][0] = in ? A[2 * ((jb + i) * lda + jb + j)] : T(0);
@@ -1068,35 +1149,14 @@ __global__ __launch_bounds__(1024) void tri_wg_kernel(const BatchSystem<T>* sys,
         ys[lane][d][1] = lane < nb ? yi : T(0);
       }
       __syncthreads();
-      for (int64_t i = jb + nb + t; i < n; i += 1024) {
-        T ar[8], ai[8];
-#pragma unroll
-        for (int d = 0; d < 8; ++d) ar[d] = ai[d] = T(0);
-        const T* Li = A + 2 * (i * lda + jb);
-        for (int j = 0; j < nb; ++j) {
-          const T lr = Li[2 * j], li = Li[2 * j + 1];
-#pragma unroll
-          for (int d = 0; d < 8; ++d)
-            if (d < nrhs) {
-              const T zr = ys[j][d][0], zi = ys[j][d][1];
-              ar[d] += lr * zr - li * zi;
-              ai[d] += lr * zi + li * zr;
-            }
-        }
-#pragma unroll
-        for (int d = 0; d < 8; ++d)
-          if (d < nrhs) {
-            v[2 * (i * ld + d)] -= ar[d];
-            v[2 * (i * ld + d) + 1] -= ai[d];
-          }
-      }
+      update_rows(jb + nb, n, jb, nb);
       __syncthreads();
     }
   if (BWD)
     for (int kb = nblk - 1; kb >= 0; --kb) {
       const int64_t jb = (int64_t)kb * PNB;
       const int nb = (int)min((int64_t)PNB, n - jb);
-      for (int e = t; e < PNB * PNB; e += 1024) {
+      for (int e = t; e < PNB * PNB; e += 512) {
         const int i = e / PNB, j = e - i * PNB;
         const bool in = i < nb && j >= i && j < nb;
         blk[i][j][0] = in ? A[2 * ((jb + i) * lda + jb + j)] : T(0);
@@ -1129,28 +1189,7 @@ __global__ __launch_bounds__(1024) void tri_wg_kernel(const BatchSystem<T>* sys,
         ys[lane][d][1] = lane < nb ? xi : T(0);
       }
       __syncthreads();
-      for (int64_t i = t; i < jb; i += 1024) {
-        T ar[8], ai[8];
-#pragma unroll
-        for (int d = 0; d < 8; ++d) ar[d] = ai[d] = T(0);
-        const T* Ui = A + 2 * (i * lda + jb);
-        for (int j = 0; j < nb; ++j) {
-          const T ur = Ui[2 * j], ui = Ui[2 * j + 1];
-#pragma unroll
-          for (int d = 0; d < 8; ++d)
-            if (d < nrhs) {
-              const T zr = ys[j][d][0], zi = ys[j][d][1];
-              ar[d] += ur * zr - ui * zi;
-              ai[d] += ur * zi + ui * zr;
-            }
-        }
-#pragma unroll
-        for (int d = 0; d < 8; ++d)
-          if (d < nrhs) {
-            v[2 * (i * ld + d)] -= ar[d];
-            v[2 * (i * ld + d) + 1] -= ai[d];
-          }
-      }
+      update_rows(0, jb, jb, nb);
       __syncthreads();
     }
 }
@@ -1412,57 +1451,6 @@ struct TriArgs {
   int nrhs;
 };
 
-// 16 x 16 matrix-core tiles of the solves (one wavefront, K = 4 per step):
-// A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k; accumulator element v of
-// lane l is row mfma_row<T>(l / 16, v), column l % 16.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-template <typename T>
-struct Acc4;
-template <>
-struct Acc4<float> {
-  typedef f32x4 type;
-};
-template <>
-struct Acc4<double> {
-  typedef f64x4 type;
-};
-__device__ __forceinline__ f32x4 mfma16(float a, float b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ f64x4 mfma16(double a, double b, f64x4 c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-template <typename T>
-__device__ __forceinline__ int mfma_row(int q, int v) {
-  return sizeof(T) == 4 ? 4 * q + v : q + 4 * v;
-}
-// 16 consecutive complex entries (p 16-byte aligned); entries at or beyond nv,
-// and all when !ok, are zero (and not read)
-__device__ __forceinline__ void load_seg16(const float* p, int nv, bool ok, float* vr, float* vi) {
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (ok && 2 * j + 1 < nv) {
-      v = *reinterpret_cast<const float4*>(p + 4 * j);
-    } else if (ok && 2 * j < nv) {
-      v.x = p[4 * j];
-      v.y = p[4 * j + 1];
-    }
-    vr[2 * j] = v.x;
-    vi[2 * j] = v.y;
-    vr[2 * j + 1] = v.z;
-    vi[2 * j + 1] = v.w;
-  }
-}
-__device__ __forceinline__ void load_seg16(const double* p, int nv, bool ok, double* vr, double* vi) {
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    double2 v = make_double2(0.0, 0.0);
-    if (ok && j < nv) v = *reinterpret_cast<const double2*>(p + 2 * j);
-    vr[j] = v.x;
-    vi[j] = v.y;
-  }
-}
 // One part (real or imaginary) of a published element: its TW / 2 tagged words
 // (TaggedRow), and its plain word.
 __device__ __forceinline__ void st_tagged_part(unsigned long long* q, float v, unsigned tag) {
@@ -3070,9 +3058,9 @@ hipError_t tri_solve_batched(const BatchSystem<T>* sys, const BatchVec<T>* vec, 
                              hipStream_t s) {
   if (nsys <= 0) return hipSuccess;
   if (nrhs < 1 || nrhs > 8) return hipErrorInvalidValue;
-  if (fwd && bwd) hipLaunchKernelGGL((tri_wg_kernel<T, true, true>), dim3((unsigned)nsys), dim3(1024), 0, s, sys, vec, nrhs);
-  else if (fwd) hipLaunchKernelGGL((tri_wg_kernel<T, true, false>), dim3((unsigned)nsys), dim3(1024), 0, s, sys, vec, nrhs);
-  else if (bwd) hipLaunchKernelGGL((tri_wg_kernel<T, false, true>), dim3((unsigned)nsys), dim3(1024), 0, s, sys, vec, nrhs);
+  if (fwd && bwd) hipLaunchKernelGGL((tri_wg_kernel<T, true, true>), dim3((unsigned)nsys), dim3(512), 0, s, sys, vec, nrhs);
+  else if (fwd) hipLaunchKernelGGL((tri_wg_kernel<T, true, false>), dim3((unsigned)nsys), dim3(512), 0, s, sys, vec, nrhs);
+  else if (bwd) hipLaunchKernelGGL((tri_wg_kernel<T, false, true>), dim3((unsigned)nsys), dim3(512), 0, s, sys, vec, nrhs);
   return hipGetLastError();
 }
 
