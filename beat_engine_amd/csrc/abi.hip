@@ -77,7 +77,7 @@ struct beat_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // look-ahead stream of the LU
   int side_reserve = 0;        // eighths of the CUs the side stream leaves free
-  hipEvent_t ev_fact = nullptr, ev_far = nullptr;
+  hipEvent_t ev_fact = nullptr, ev_far = nullptr, ev_far2 = nullptr;
   hipStream_t inb = nullptr;  // in-block stream of the LU
   hipEvent_t ev_panel = nullptr, ev_inb = nullptr;
   std::string err;
@@ -352,6 +352,7 @@ int ensure_lu_work(beat_ctx* ctx, int64_t n, LuWork& w) {
   w.side = ctx->side;
   w.ev_fact = ctx->ev_fact;
   w.ev_far = ctx->ev_far;
+  w.ev_far2 = ctx->ev_far2;
   w.inb = ctx->inb;
   w.ev_panel = ctx->ev_panel;
   w.ev_inb = ctx->ev_inb;
@@ -2098,6 +2099,7 @@ int beat_ctx_create(int device, beat_ctx** out) {
   if (create_side_stream(c) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_fact, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_far, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_far2, hipEventDisableTiming) != hipSuccess ||
       hipStreamCreateWithFlags(&c->inb, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_panel, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_inb, hipEventDisableTiming) != hipSuccess) {
@@ -2166,6 +2168,7 @@ void beat_ctx_destroy(beat_ctx* c) {
   if (c->side) (void)hipStreamDestroy(c->side);
   if (c->ev_fact) (void)hipEventDestroy(c->ev_fact);
   if (c->ev_far) (void)hipEventDestroy(c->ev_far);
+  if (c->ev_far2) (void)hipEventDestroy(c->ev_far2);
   if (c->inb) (void)hipStreamDestroy(c->inb);
   if (c->ev_panel) (void)hipEventDestroy(c->ev_panel);
   if (c->ev_inb) (void)hipEventDestroy(c->ev_inb);

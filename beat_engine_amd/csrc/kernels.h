@@ -167,7 +167,7 @@ struct LuWork {
   void* linv;         // 2 x 4 x 64 x 64 complex (double-sized): leaf inverses of two outer blocks
   int maxg;
   hipStream_t side;   // look-ahead stream (nullptr: no look-ahead)
-  hipEvent_t ev_fact, ev_far;
+  hipEvent_t ev_fact, ev_far, ev_far2;  // far updates of alternate blocks record ev_far / ev_far2
   hipStream_t inb;    // in-block stream (nullptr: in-block work in order on the main stream)
   hipEvent_t ev_panel, ev_inb;
   // complex64 trailing updates as three real products instead of four (normwise

@@ -2572,7 +2572,14 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   // the look-ahead gains nothing, its updates being small, while every
   // cross-stream event puts tens of microseconds of latency on the critical path:
   // the tail's blocks run in order on s.
+  // The side stream's far update of a block waits only for the block's factors on
+  // s (ev_fact, recorded right after them), not for s to have seen the previous
+  // far update end: the far updates follow each other on the side stream without
+  // a cross-stream hop between them. s waits for each far update (alternate
+  // blocks record ev_far and ev_far2, so that the wait names that update and not
+  // the next one) before its own update of the same columns.
   bool far_pending = false;  // a far update on the side stream that s has not yet waited for
+  hipEvent_t far_ev = w.ev_far, far_next = w.ev_far2;
   for (int64_t J = 0; J < n; J += bw) {
     const int64_t Je = std::min<int64_t>(n, J + bw);
     const bool tail = n - Je <= w.tail_rows;
@@ -2599,32 +2606,38 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
       if ((e = hipEventRecord(w.ev_inb, w.inb)) != hipSuccess) return e;
     }
     if (split_b && (e = hipStreamWaitEvent(s, w.ev_inb, 0)) != hipSuccess) return e;
+    const int64_t Kn = std::min<int64_t>(n, Je + bw);
+    const bool far = Je < n && look_ahead && !tail && Kn < ncol;
+    if (far) {
+      // this block's far update (columns beyond the next block) on the side stream,
+      // behind the previous one there
+      if ((e = hipEventRecord(w.ev_fact, s)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(w.side, w.ev_fact, 0)) != hipSuccess) return e;
+      if ((e = update(J, Je, Kn, ncol, w.side)) != hipSuccess) return e;
+      if ((e = hipEventRecord(far_next, w.side)) != hipSuccess) return e;
+    }
     // the previous block's far update (of every column beyond this block) comes
     // before any update of those columns on s
     if (far_pending) {
-      if ((e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(s, far_ev, 0)) != hipSuccess) return e;
       far_pending = false;
+    }
+    if (far) {
+      std::swap(far_ev, far_next);
+      far_pending = true;
     }
     if (Je >= n) {
       // the last block's update of the carried right-hand sides
       if (ncol > n && (e = update(J, Je, n, ncol, s)) != hipSuccess) return e;
       continue;
     }
-    const int64_t Kn = std::min<int64_t>(n, Je + bw);
     if (look_ahead && !tail) {
-      if (Kn < ncol) {
-        if ((e = hipEventRecord(w.ev_fact, s)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(w.side, w.ev_fact, 0)) != hipSuccess) return e;
-        if ((e = update(J, Je, Kn, ncol, w.side)) != hipSuccess) return e;
-        if ((e = hipEventRecord(w.ev_far, w.side)) != hipSuccess) return e;
-        far_pending = true;
-      }
       if ((e = update(J, Je, Je, Kn, s)) != hipSuccess) return e;
     } else {
       if ((e = update(J, Je, Je, ncol, s)) != hipSuccess) return e;
     }
   }
-  if (far_pending && (e = hipStreamWaitEvent(s, w.ev_far, 0)) != hipSuccess) return e;
+  if (far_pending && (e = hipStreamWaitEvent(s, far_ev, 0)) != hipSuccess) return e;
   if (panels_out) *panels_out = panels;
   return hipSuccess;
 }
