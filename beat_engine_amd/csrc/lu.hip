@@ -852,7 +852,7 @@ __device__ __forceinline__ void zgemm_tile(const T* __restrict__ A, int64_t lda,
       stash();
       __syncthreads();
       if (k0 + KB < kc) load(k0 + KB);
-#pragma unroll 4
+#pragma unroll
       for (int kk = 0; kk < KB; kk += 2) {
         const int kx = kk + h;
         const float2 bv = *reinterpret_cast<const float2*>(&Bs[kx * BP + 2 * jc]);
@@ -899,7 +899,7 @@ __device__ __forceinline__ void zgemm_tile(const T* __restrict__ A, int64_t lda,
       stash();
       __syncthreads();
       if (k0 + KB < kc) load(k0 + KB);
-#pragma unroll 4
+#pragma unroll
       for (int kk = 0; kk < KB; ++kk) {
         float av[MI], bv[2];
 #pragma unroll
