@@ -785,7 +785,26 @@ __device__ __forceinline__ void zgemm_tile(const T* __restrict__ A, int64_t lda,
   const int wm = wave >> 1, wn = wave & 1;
   vec ra[AV], rb[BV];
 
+  // whole tiles of 16-byte aligned rows (the factorization's case): vector loads
+  // with no per-vector checks
+  const bool whole_rows = row0 + BM <= m && col0 + BN <= n && ((uintptr_t)A & 15) == 0 &&
+                          ((uintptr_t)B & 15) == 0 && (lda & (VEC / 2 - 1)) == 0 && (ldb & (VEC / 2 - 1)) == 0;
   auto load = [&](int k0) {
+    if (whole_rows && k0 + KB <= kc) {
+#pragma unroll
+      for (int it = 0; it < AV; ++it) {
+        const int idx = tid + 256 * it;
+        const int r = idx / (2 * KB / VEC), cv = idx % (2 * KB / VEC);
+        ra[it] = *reinterpret_cast<const vec*>(A + 2 * ((row0 + r) * lda + k0 + cv * VEC / 2));
+      }
+#pragma unroll
+      for (int it = 0; it < BV; ++it) {
+        const int idx = tid + 256 * it;
+        const int r = idx / (2 * BN / VEC), cv = idx % (2 * BN / VEC);
+        rb[it] = *reinterpret_cast<const vec*>(B + 2 * ((int64_t)(k0 + r) * ldb + col0 + cv * VEC / 2));
+      }
+      return;
+    }
 #pragma unroll
     for (int it = 0; it < AV; ++it) {
       const int idx = tid + 256 * it;
