@@ -1066,6 +1066,17 @@ __device__ __forceinline__ int mfma_row(int q, int v) {
 // 16 consecutive complex entries (p 16-byte aligned); entries at or beyond nv,
 // and all when !ok, are zero (and not read)
 __device__ __forceinline__ void load_seg16(const float* p, int nv, bool ok, float* vr, float* vi) {
+  if (ok && nv >= 16) {  // a whole segment: no per-vector checks
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float4 v = *reinterpret_cast<const float4*>(p + 4 * j);
+      vr[2 * j] = v.x;
+      vi[2 * j] = v.y;
+      vr[2 * j + 1] = v.z;
+      vi[2 * j + 1] = v.w;
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1082,6 +1093,15 @@ __device__ __forceinline__ void load_seg16(const float* p, int nv, bool ok, floa
   }
 }
 __device__ __forceinline__ void load_seg16(const double* p, int nv, bool ok, double* vr, double* vi) {
+  if (ok && nv >= 16) {  // a whole segment: no per-vector checks
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const double2 v = *reinterpret_cast<const double2*>(p + 2 * j);
+      vr[j] = v.x;
+      vi[j] = v.y;
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     double2 v = make_double2(0.0, 0.0);
