@@ -2681,11 +2681,12 @@ hipError_t lu_factor(T* A, int64_t n, int64_t lda, LuWork& w, int nb, hipStream_
   return hipSuccess;
 }
 
-// Solve workspace (complex T units, then ints): Linv and Uinv per leaf (PNB x PNB
-// each), three n x 8 blocks (gathered rhs, forward results, backward results),
-// the row gather tables (nJ x n), the total row permutation (n), the forward
-// and backward flags (np each) and the per-block row maps the tables are built
-// from (nJ x n).
+// Solve workspace (complex T units, then ints): the chain matrices (per direction
+// and leaf the inverse D and the link W, PNB x PNB each), three n x 8 blocks
+// (gathered rhs, forward results, backward results), the forward and backward
+// results as tagged words (twice a complex entry's size), then the row gather
+// tables (nJ x n), the total row permutation (n), the forward and backward flags
+// (np each) and the per-block row maps the tables are built from (nJ x n).
 struct SolveLayout {
   int np, nJ;
   size_t mat, b0, yf, xb, tf, tb, cend;  // complex offsets
