@@ -1585,11 +1585,17 @@ __global__ __launch_bounds__(256) void tri_solve_kernel(const TriArgs<T> a) {
     if (gave_up) return false;  // uniform after the barrier; the host reports the give-up
     const int64_t kb = (int64_t)k * PNB;
     const int nbk = (int)min((int64_t)PNB, a.n - kb);
-    for (int idx = t; idx < PNB * 8; idx += 256) {
-      const int kk = idx >> 3, dd = idx & 7;
-      T zr = 0, zi = 0;
-      if (kk < nbk) ld_cplx(a.pub + 2 * ((kb + kk) * 8 + dd), zr, zi);
-      stage(kk, dd, zr, zi);
+    T zr[2], zi[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {  // both entries' loads first
+      const int idx = t + 256 * e, kk = idx >> 3, dd = idx & 7;
+      zr[e] = zi[e] = 0;
+      if (nbk == PNB || kk < nbk) ld_cplx(a.pub + 2 * ((kb + kk) * 8 + dd), zr[e], zi[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int idx = t + 256 * e;
+      stage(idx >> 3, idx & 7, zr[e], zi[e]);
     }
     __syncthreads();
     return true;
